@@ -1,0 +1,234 @@
+/*
+ * tgmm_em.h -- C-ABI of the MI355X (gfx950) variational-inference EM for
+ * supervoxel-based Gaussian mixtures (the per-frame fit of the TGMM tracker).
+ *
+ * Two layers are exported from libtgmm_em.so:
+ *
+ *  1. Drop-in entry points with the argument lists of the reference host API
+ *     (nmeth.3036-S2/src/UtilsCUDA/GMEMupdateCUDA.h). They are exported with a
+ *     "tgmm_" prefix and return a status code instead of calling exit(); the
+ *     C++ header tgmm_em_compat.hpp wraps them in inline functions carrying the
+ *     reference's exact names, parameter lists and exit-on-error behaviour.
+ *
+ *  2. A widened, handle-based API for batches of frames of uneven size
+ *     (64-bit voxel/supervoxel counts, runtime K <= 16, more than 65535
+ *     Gaussians), hipGraph-replayed iterations, and a continuous-batching
+ *     server for concurrent time-lapse streams.
+ *
+ * All pointers are plain host or device pointers; no torch types appear here.
+ * Every function returns TGMM_OK (0) or a negative TGMM_ERR_* code; the last
+ * error message is available from tgmm_last_error().
+ */
+#ifndef TGMM_EM_H_
+#define TGMM_EM_H_
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TGMM_DIMS 3
+#define TGMM_MAX_SUPERVOXELS_PER_GAUSSIAN 16 /* GMEMupdateCUDA.h:20 */
+#define TGMM_KMAX 16                         /* widened API: K in [1, 16] */
+#define TGMM_LEGACY_K 5                      /* constants.h:15 maxGaussiansPerVoxel */
+
+/* status codes */
+#define TGMM_OK 0
+#define TGMM_ERR_ARG (-1)     /* invalid argument (NULL, negative size, bad K) */
+#define TGMM_ERR_LIMIT (-2)   /* size beyond a compiled or configured limit */
+#define TGMM_ERR_OOM (-3)     /* device or host allocation failed / budget */
+#define TGMM_ERR_DEVICE (-4)  /* HIP runtime error */
+#define TGMM_ERR_NOT_SET (-5) /* regularisation constants were never set */
+#define TGMM_ERR_IO (-6)      /* debug dump could not be written */
+#define TGMM_ERR_BUSY (-7)    /* server: no free slot / capacity right now */
+
+/*
+ * Per-Gaussian record, 288 bytes, byte-for-byte the layout of
+ * GaussianMixtureModelCUDA (GMEMupdateCUDA.h:22-59): 27 doubles, the `fixed`
+ * flag (one byte, C++ bool), 16 supervoxel indices and their count.
+ * Offsets: beta_k 0, nu_k 8, alpha_k 16, m_k 24, W_k 48, E[ln det] 96,
+ * E[ln pi] 104, splitScore 112, beta_o 120, nu_o 128, alpha_o 136, m_o 144,
+ * W_o 168, fixed 216, supervoxelIdx 220, supervoxelNum 284.
+ * W_k and W_o are packed symmetric [xx, xy, xz, yy, yz, zz].
+ * A Gaussian is dead when m_o[0] < -1e31.
+ */
+typedef struct GaussianMixtureModelCUDA {
+  double beta_k, nu_k, alpha_k;
+  double m_k[TGMM_DIMS];
+  double W_k[6];
+  double expectedLogDetCovarianceCUDA;
+  double expectedLogResponsivityCUDA;
+  double splitScore;
+  double beta_o, nu_o, alpha_o;
+  double m_o[TGMM_DIMS];
+  double W_o[6];
+  unsigned char fixed;
+  int supervoxelIdx[TGMM_MAX_SUPERVOXELS_PER_GAUSSIAN];
+  int supervoxelNum;
+} GaussianMixtureModelCUDA;
+
+/* Last error message of the calling thread (never NULL). */
+const char* tgmm_last_error(void);
+/* Number of HIP devices visible (0 when none); never initialises streams. */
+int tgmm_device_count(int* count);
+
+/* ---------------------------------------------------------------------------
+ * 1. Drop-in entry points (K = 5, unsigned short supervoxel count, G <= 65535)
+ * ------------------------------------------------------------------------ */
+
+/* Replaces GMEMinitializeMemoryWithSupervoxels (GMEMupdateCUDA.h:132-137,
+ * GMEMupdateCUDA.cu:4580-4636). Allocates query[3N] f32 SoA, img[N],
+ * rnk[5S], ind[5S], indTr[5S], centroids[3S] SoA, ptr[S+1] on the current
+ * device, copies the host arrays, and records `scale` as the kNN anisotropy
+ * (uploadScaleCUDA, knnCuda.cu:827-830). */
+int tgmm_GMEMinitializeMemoryWithSupervoxels(
+    float** queryCUDA, const float* query, float** imgDataCUDA,
+    const float* imgData, const float* scale, long long query_nb,
+    float** rnkCUDA, int** indCUDA, int** indCUDAtr, unsigned short numLabels,
+    float** centroidLabelPositionCUDA, const float* centroidLabelPosition,
+    long long** labelListPtrCUDA, const long long* labelListPtrHOST);
+
+/* Replaces copyScaleToDvice (GMEMupdateCUDA.h:82, GMEMupdateCUDA.cu:4760):
+ * anisotropy used by the precision-matrix regularisation. */
+int tgmm_copyScaleToDvice(const float* scale);
+
+/* Replaces the host statics regularizePrecisionMatrixConstants::setConstants
+ * (constants.h:38-44) that GMEMvariationalInferenceCUDAWithSupervoxels reads
+ * (GMEMupdateCUDA.cu:4217-4226). Until called, the EM returns
+ * TGMM_ERR_NOT_SET (the reference exits with status 3). */
+int tgmm_setRegularizePrecisionMatrixConstants(double lambdaMin,
+                                               double lambdaMax,
+                                               double maxExcentricity);
+
+/* Replaces GMEMvariationalInferenceCUDAWithSupervoxels (GMEMupdateCUDA.h:92-98,
+ * GMEMupdateCUDA.cu:3985-4421). Device inputs from the init call; vecGMtemp is
+ * host in/out [ref_nb]. On return: posterior and expectation fields updated,
+ * supervoxelIdx/supervoxelNum rebuilt from r > 0.5 (<= 16, slot-major scan),
+ * rnkCUDA/indCUDA (device, slot-major s + j*S) hold the last iteration with
+ * candidates sorted by (distance, index) and unused slots ind = -1, r = 0.
+ * rnkCUDAtr may be NULL (unused, as in the reference); indCUDAtr is scratch.
+ * debugPath NULL or "" disables the {int S; int K; float rnk[]; int ind[]}
+ * dump. This entry point is the plain per-frame form: one iteration per host
+ * round trip with the convergence test on the host, as the reference does. */
+int tgmm_GMEMvariationalInferenceCUDAWithSupervoxels(
+    float* queryCUDA, float* imgDataCUDA, float* rnkCUDA, float* rnkCUDAtr,
+    int* indCUDA, int* indCUDAtr, float* centroidLabelPositionCUDA,
+    long long* labelListPtrCUDA, GaussianMixtureModelCUDA* vecGMtemp,
+    long long query_nb, int ref_nb, unsigned short numLabels, int maxIterEM,
+    double tolLikelihood, int devCUDA, int frame, int W4DOF,
+    const char* debugPath);
+
+/* Replaces GMEMreleaseMemoryWithSupervoxels (GMEMupdateCUDA.h:140-144,
+ * GMEMupdateCUDA.cu:4654-4676): frees and nulls every pointer. */
+int tgmm_GMEMreleaseMemoryWithSupervoxels(float** queryCUDA,
+                                          float** imgDataCUDA, float** rnkCUDA,
+                                          int** indCUDA, int** indCUDAtr,
+                                          float** centroidLabelPositionCUDA,
+                                          long long** labelListPtrCUDA);
+
+/* ---------------------------------------------------------------------------
+ * 2. Widened API: batches of frames, graphs, serving
+ * ------------------------------------------------------------------------ */
+
+typedef struct tgmm_em_opts {
+  int K;                   /* candidates per supervoxel, 1..TGMM_KMAX */
+  int max_iter;            /* maxIterEM */
+  double tol;              /* tolLikelihood; < 0 runs exactly max_iter */
+  double lambda_min;       /* regularisation constants (constants.h:19-45) */
+  double lambda_max;
+  double max_excentricity;
+  int w4dof;               /* zero xz/yz of W in regularisation */
+  float scale[TGMM_DIMS];  /* anisotropy for kNN and regularisation */
+  int max_sv_per_gaussian; /* cap of supervoxelIdx, 0..16 (16 = reference) */
+} tgmm_em_opts;
+
+/* One frame. Input arrays are HOST memory with the reference's boundary
+ * layout (query SoA x..,y..,z.., voxels of supervoxel s in
+ * [sv_ptr[s], sv_ptr[s+1]), centroids SoA). gm is host in/out; rnk/ind are
+ * optional host outputs of K*n_supervoxels entries, slot-major (s + j*S). */
+typedef struct tgmm_frame_desc {
+  long long n_voxels;
+  long long n_supervoxels;
+  int n_gaussians;
+  const float* query;
+  const float* img;
+  const float* centroids;
+  const long long* sv_ptr;
+  GaussianMixtureModelCUDA* gm;
+  float* rnk;
+  int* ind;
+  double ll;      /* out: final log-likelihood (GMEM.cu:4254-4262) */
+  int iterations; /* out: EM iterations run */
+  int status;     /* out: TGMM_OK or error code for this frame */
+} tgmm_frame_desc;
+
+typedef struct tgmm_batch tgmm_batch;
+
+/* Solve modes */
+#define TGMM_SOLVE_EAGER 0     /* kernels launched per iteration, no host sync */
+#define TGMM_SOLVE_GRAPH 1     /* one iteration captured in a hipGraph, replayed */
+#define TGMM_SOLVE_HOSTSYNC 2  /* plain form: host reads ll/convergence each iteration */
+
+/* Uploads all frames into one device workspace (one allocation set), builds
+ * per-supervoxel moments and per-frame kNN grids. Host arrays may be freed
+ * after this returns. */
+int tgmm_batch_create(int device, const tgmm_frame_desc* frames, int nframes,
+                      const tgmm_em_opts* opts, tgmm_batch** out);
+/* Re-loads every frame's initial Gaussian records (as given at create) into
+ * the device state, device-to-device. */
+int tgmm_batch_reset(tgmm_batch* b);
+/* Runs EM on all frames; stream-ordered, returns after the last iteration has
+ * been enqueued and completed (synchronises the batch stream). */
+int tgmm_batch_solve(tgmm_batch* b, int mode);
+/* Copies results into frames[i].gm/rnk/ind/ll/iterations (host). */
+int tgmm_batch_fetch(tgmm_batch* b, tgmm_frame_desc* frames, int nframes);
+/* Average device time (ms) of one launch of the named kernel over `reps`
+ * eager iterations, measured with hipEvents on the batch stream. Names:
+ * "loglik", "estep", "knn", "mstep", "moments". Leaves the state advanced. */
+int tgmm_batch_time_kernel(tgmm_batch* b, const char* name, int reps,
+                           double* avg_ms, double* bytes_per_launch);
+int tgmm_batch_destroy(tgmm_batch* b);
+
+/* Continuous-batching server for concurrent time-lapse streams. Frames are
+ * admitted into a fixed-capacity device workspace sized by an HBM budget;
+ * every hipGraph replay advances every in-flight frame by one EM iteration;
+ * frames join as capacity frees up and leave when done; each stream's
+ * posterior Gaussians stay on the device and become the next frame's priors
+ * (GaussianMixtureModel::updatePriors, GaussianMixtureModel.cpp:892-910,
+ * plus the driver's nu_k*2, W_k/2 step, main.cpp:961-969). */
+typedef struct tgmm_server tgmm_server;
+
+typedef struct tgmm_server_opts {
+  tgmm_em_opts em;
+  long long hbm_budget_bytes; /* device workspace budget */
+  int max_frames_in_flight;
+  double beta_percentage_of_nk;  /* updatePriors arguments */
+  double nu_percentage_of_nk;
+  double alpha_percentage;
+} tgmm_server_opts;
+
+int tgmm_server_create(int device, const tgmm_server_opts* opts,
+                       tgmm_server** out);
+/* Queue a frame. stream_id >= 0 names a time-lapse; when the stream already
+ * has device-resident Gaussians and frame->gm is NULL, the priors come from
+ * the stream's previous posterior. Returns a ticket >= 0 or an error. Input
+ * host arrays are copied before return. */
+int tgmm_server_submit(tgmm_server* s, int stream_id,
+                       const tgmm_frame_desc* frame, long long* ticket);
+/* Runs scheduling steps until the ticket completes (or, with ticket < 0,
+ * until every queued frame completes). */
+int tgmm_server_wait(tgmm_server* s, long long ticket);
+/* Advances the server by one scheduling step (admit, one iteration, retire);
+ * returns the number of frames still in flight or queued via *pending. */
+int tgmm_server_step(tgmm_server* s, int* pending);
+/* Fetch a completed ticket's outputs into frame (gm must have room for the
+ * frame's Gaussians; rnk/ind optional). Also returns admission and
+ * completion wall times (seconds, monotonic clock). */
+int tgmm_server_fetch(tgmm_server* s, long long ticket, tgmm_frame_desc* frame,
+                      double* t_submit, double* t_admit, double* t_done);
+int tgmm_server_destroy(tgmm_server* s);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TGMM_EM_H_ */

@@ -1,0 +1,581 @@
+/*
+ * oracle.c -- FP64 CPU restatement of the reference's supervoxel-mode VI-EM.
+ *
+ * TEST INFRASTRUCTURE ONLY (the checker): used by tests/, by
+ * __graft_entry__.smoke() and by bench.py's cpu_baseline leg; never by the
+ * product library. Written from the reference's equations, independently of
+ * the HIP code:
+ *   kNN            knnCuda.cu:61-144 (distance order), GMEM.cu:527-539
+ *   Sigma alpha    GMEM.cu:579-604, 2765-2799
+ *   expectations   GMEM.cu:553-628 (K7, K8)
+ *   E-step         GMEM.cu:2244-2336 (K9), per voxel, FP64
+ *   M-step         GMEM.cu:1286-1455 (K13), per voxel, FP64, r > 1e-3 rule
+ *   posterior      GMEM.cu:1601-1677 (K14), FP64 throughout
+ *   regularise     GMEM.cu:1693-1809 (K15), eig3 eigenvalue order
+ *   death          GMEM.cu:1813-1843 (K16)
+ *   log-likelihood GMEM.cu:1913-1978 (K17), per voxel, FP64
+ *   convergence    GMEM.cu:4143-4148; supervoxel assignment GMEM.cu:4328-4350
+ *
+ * Declared deviations from the reference (identical in the HIP code):
+ *   - kNN candidates are the K smallest (distance, index) pairs, output sorted
+ *     ascending; fewer than K finite candidates are padded with index -1 and
+ *     r = 0 (the reference leaves such slots uninitialised, knnCuda.cu:74).
+ *   - E-step normalisation is max-subtracted log-sum-exp (same value as
+ *     1/sum_j exp(L_j - L_k), GMEM.cu:2327-2335, without overflow).
+ *   - M-step statistics, N_k and the posterior are FP64 (the reference
+ *     accumulates in FP32 shared memory and narrows N_k, beta_o, x_k, m_o to
+ *     float, GMEM.cu:1381-1394, 1614-1629).
+ *   - log-likelihood sums every voxel of a supervoxel (the reference's
+ *     `cache[tid] =` keeps only the last term per thread for supervoxels of
+ *     more than 1024 voxels, GMEM.cu:1961-1962).
+ *   - death NaN test reads W_k[0..5] and E[ln det Lambda] (the reference's
+ *     7-element loop over W_k reaches that field, GMEM.cu:1823-1824).
+ *
+ * Eigen-decomposition uses cyclic Jacobi rotations (not the reference's
+ * closed form), then orders the eigenvalues as eig3 does (GMEM.cu:198-262):
+ * non-singular -> (largest, smallest, middle); |det| < 1e-24 -> (0, larger,
+ * smaller). Digamma uses upward recurrence to x >= 10 and the asymptotic
+ * Bernoulli series (Abramowitz & Stegun 6.3.6 / 6.3.18), with the reference's
+ * special value -1e300 at non-positive integers (GMEM.cu:515-518).
+ */
+#include "oracle.h"
+
+#include <float.h>
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+static int g_threads = 0;
+
+void orc_set_threads(int n) {
+  g_threads = n;
+#ifdef _OPENMP
+  if (n > 0) omp_set_num_threads(n);
+#endif
+}
+
+int orc_get_threads(void) {
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
+}
+
+static int is_dead(const GaussianMixtureModelCUDA* g) { return g->m_o[0] < -1e31; }
+
+/* ------------------------------------------------------------------------ */
+double orc_digamma(double x) {
+  static const double PI = 3.14159265358979323846;
+  if (x <= 0.0 && floor(x) == x) return -1e300;
+  if (x < 0.0) return orc_digamma(1.0 - x) + PI / tan(-PI * x);
+  double acc = 0.0;
+  while (x < 10.0) {
+    acc -= 1.0 / x;
+    x += 1.0;
+  }
+  /* psi(x) ~ ln x - 1/(2x) - sum_n B_2n / (2n x^2n) */
+  const double x2 = 1.0 / (x * x);
+  double series =
+      x2 * (1.0 / 12 -
+            x2 * (1.0 / 120 -
+                  x2 * (1.0 / 252 -
+                        x2 * (1.0 / 240 -
+                              x2 * (1.0 / 132 -
+                                    x2 * (691.0 / 32760 - x2 * (1.0 / 12)))))));
+  return acc + log(x) - 0.5 / x - series;
+}
+
+/* packed symmetric [xx, xy, xz, yy, yz, zz] */
+double orc_det3(const double w[6]) {
+  const double a = w[0], b = w[1], c = w[2], d = w[3], e = w[4], f = w[5];
+  return a * (d * f - e * e) - b * (b * f - c * e) + c * (b * e - c * d);
+}
+
+/* adjugate / det with the reference's singular sentinel (GMEM.cu:170-192) */
+int orc_inv3(const double w[6], double o[6]) {
+  const double det = orc_det3(w);
+  if (fabs(det) < 1e-16) {
+    for (int i = 0; i < 6; i++) o[i] = 1e300;
+    return 1;
+  }
+  const double a = w[0], b = w[1], c = w[2], d = w[3], e = w[4], f = w[5];
+  o[0] = (d * f - e * e) / det;
+  o[1] = (c * e - b * f) / det;
+  o[2] = (b * e - c * d) / det;
+  o[3] = (a * f - c * c) / det;
+  o[4] = (b * c - a * e) / det;
+  o[5] = (a * d - b * b) / det;
+  return 0;
+}
+
+/* cyclic Jacobi; A full 3x3 row-major in, eigenvalues in ev[], vectors as
+ * columns of V (row-major) */
+static void jacobi3(double A[3][3], double ev[3], double V[3][3]) {
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) V[i][j] = (i == j) ? 1.0 : 0.0;
+  for (int sweep = 0; sweep < 64; sweep++) {
+    double off = fabs(A[0][1]) + fabs(A[0][2]) + fabs(A[1][2]);
+    double scale = fabs(A[0][0]) + fabs(A[1][1]) + fabs(A[2][2]);
+    if (off == 0.0 || off <= 1e-18 * scale) break;
+    for (int p = 0; p < 2; p++)
+      for (int q = p + 1; q < 3; q++) {
+        if (A[p][q] == 0.0) continue;
+        double theta = (A[q][q] - A[p][p]) / (2.0 * A[p][q]);
+        double t = (theta >= 0 ? 1.0 : -1.0) /
+                   (fabs(theta) + sqrt(theta * theta + 1.0));
+        double c = 1.0 / sqrt(t * t + 1.0), s = t * c;
+        for (int k = 0; k < 3; k++) {
+          double akp = A[k][p], akq = A[k][q];
+          A[k][p] = c * akp - s * akq;
+          A[k][q] = s * akp + c * akq;
+        }
+        for (int k = 0; k < 3; k++) {
+          double apk = A[p][k], aqk = A[q][k];
+          A[p][k] = c * apk - s * aqk;
+          A[q][k] = s * apk + c * aqk;
+        }
+        for (int k = 0; k < 3; k++) {
+          double vkp = V[k][p], vkq = V[k][q];
+          V[k][p] = c * vkp - s * vkq;
+          V[k][q] = s * vkp + c * vkq;
+        }
+      }
+  }
+  for (int i = 0; i < 3; i++) ev[i] = A[i][i];
+}
+
+/* d[i] with eigenvector v[3i..3i+2], ordered like eig3 (GMEM.cu:198-262) */
+void orc_eig3_ordered(const double w[6], double d[3], double v[9]) {
+  double A[3][3] = {{w[0], w[1], w[2]}, {w[1], w[3], w[4]}, {w[2], w[4], w[5]}};
+  double ev[3], V[3][3];
+  jacobi3(A, ev, V);
+  int order[3];
+  const int singular = fabs(orc_det3(w)) < 1e-24;
+  if (!singular) {
+    int imax = 0, imin = 0;
+    for (int i = 1; i < 3; i++) {
+      if (ev[i] > ev[imax]) imax = i;
+      if (ev[i] < ev[imin]) imin = i;
+    }
+    if (imax == imin) imin = (imax + 1) % 3;
+    order[0] = imax;
+    order[1] = imin;
+    order[2] = 3 - imax - imin;
+  } else {
+    int iz = 0;
+    for (int i = 1; i < 3; i++)
+      if (fabs(ev[i]) < fabs(ev[iz])) iz = i;
+    int a = (iz + 1) % 3, b = (iz + 2) % 3;
+    order[0] = iz;
+    order[1] = ev[a] >= ev[b] ? a : b;
+    order[2] = ev[a] >= ev[b] ? b : a;
+  }
+  for (int i = 0; i < 3; i++) {
+    d[i] = ev[order[i]];
+    for (int k = 0; k < 3; k++) v[3 * i + k] = V[k][order[i]];
+  }
+  if (singular) d[0] = 0.0;
+}
+
+/* GMEM.cu:1693-1809 */
+void orc_regularize_W(double W[6], double nu, int w4dof, const float scale[3],
+                      double lambda_min, double lambda_max, double max_exc) {
+  const double auxMax = lambda_max / nu, auxMin = lambda_min / nu;
+  double w[6];
+  double ss[6];
+  int c = 0;
+  for (int i = 0; i < 3; i++)
+    for (int j = i; j < 3; j++) ss[c++] = (double)(scale[i] * scale[j]);
+  for (int i = 0; i < 6; i++) w[i] = W[i] / ss[i];
+  if (w4dof) {
+    w[2] = 0.0;
+    w[4] = 0.0;
+  }
+  double d[3], v[9];
+  orc_eig3_ordered(w, d, v);
+  for (int i = 0; i < 3; i++)
+    if (d[i] > auxMax) d[i] = auxMax;
+  for (int i = 0; i < 3; i++)
+    if (d[i] < auxMin) d[i] = auxMin;
+  static const int pa[3] = {0, 0, 1}, pb[3] = {1, 2, 2};
+  for (int p = 0; p < 3; p++) {
+    const int a = pa[p], b = pb[p];
+    const double r = d[a] / d[b];
+    if (r > max_exc)
+      d[a] = max_exc * d[b];
+    else if (1.0 / r > max_exc)
+      d[b] = max_exc * d[a];
+  }
+  /* reconstruct V diag(d) V^T */
+  double r6[6];
+  c = 0;
+  for (int a = 0; a < 3; a++)
+    for (int b = a; b < 3; b++) {
+      double s = 0.0;
+      for (int i = 0; i < 3; i++) s += d[i] * v[3 * i + a] * v[3 * i + b];
+      r6[c++] = s;
+    }
+  w[0] = r6[0];
+  w[1] = r6[1];
+  w[3] = r6[3];
+  w[5] = r6[5];
+  if (!w4dof) {
+    w[2] = r6[2];
+    w[4] = r6[4];
+  }
+  for (int i = 0; i < 6; i++) W[i] = w[i] * ss[i];
+}
+
+/* GMEM.cu:1601-1677 (FP64 throughout) */
+void orc_posterior(GaussianMixtureModelCUDA* g, double Nk, const double xk[3],
+                   const double Ck[6]) {
+  const double beta_o = g->beta_o;
+  g->beta_k = beta_o + Nk;
+  g->nu_k = g->nu_o + Nk;
+  g->alpha_k = fmax(g->alpha_o + Nk, 0.0);
+  for (int i = 0; i < 3; i++) g->m_k[i] = (beta_o * g->m_o[i] + Nk * xk[i]) / g->beta_k;
+  if (g->W_o[5] < 1e-8) g->W_o[5] = 0.5 * (g->W_o[0] + g->W_o[3]);
+  double A[6];
+  orc_inv3(g->W_o, A);
+  const double dx[3] = {xk[0] - g->m_o[0], xk[1] - g->m_o[1], xk[2] - g->m_o[2]};
+  const double f = beta_o / (beta_o + Nk);
+  int c = 0;
+  for (int a = 0; a < 3; a++)
+    for (int b = a; b < 3; b++) {
+      A[c] += Ck[c] + Nk * f * dx[a] * dx[b];
+      c++;
+    }
+  orc_inv3(A, g->W_k);
+}
+
+/* ------------------------------------------------------------------------ */
+static int lex_less(float da, int ia, float db, int ib) {
+  return da < db || (da == db && ia < ib);
+}
+
+/* knnCuda.cu:104-130 distance order: ((c-m)*s)^2 summed x, y, z in FP32 */
+void orc_knn(long long S, const float* cent, int G,
+             const GaussianMixtureModelCUDA* gm, const float scale[3], int K,
+             int* ind) {
+  float* mf = (float*)malloc(sizeof(float) * 3 * (size_t)(G > 0 ? G : 1));
+  for (int k = 0; k < G; k++)
+    for (int i = 0; i < 3; i++) mf[3 * k + i] = (float)gm[k].m_k[i];
+#pragma omp parallel for schedule(dynamic, 64)
+  for (long long s = 0; s < S; s++) {
+    float bd[TGMM_KMAX];
+    int bi[TGMM_KMAX];
+    int n = 0;
+    const float q0 = cent[s], q1 = cent[s + S], q2 = cent[s + 2 * S];
+    for (int k = 0; k < G; k++) {
+      const float a = (q0 - mf[3 * k]) * scale[0];
+      float dist = a * a;
+      const float b = (q1 - mf[3 * k + 1]) * scale[1];
+      dist += b * b;
+      const float e = (q2 - mf[3 * k + 2]) * scale[2];
+      dist += e * e;
+      if (!(dist <= 1e32f)) continue; /* dead Gaussians give inf */
+      if (n == K && !lex_less(dist, k, bd[K - 1], bi[K - 1])) continue;
+      int p = (n < K) ? n++ : K - 1;
+      while (p > 0 && lex_less(dist, k, bd[p - 1], bi[p - 1])) {
+        bd[p] = bd[p - 1];
+        bi[p] = bi[p - 1];
+        p--;
+      }
+      bd[p] = dist;
+      bi[p] = k;
+    }
+    for (int j = 0; j < K; j++) ind[s + (long long)j * S] = (j < n) ? bi[j] : -1;
+  }
+  free(mf);
+}
+
+double orc_sum_alpha(int G, const GaussianMixtureModelCUDA* gm) {
+  double s = 0.0;
+  for (int k = 0; k < G; k++) s += gm[k].alpha_k;
+  return s;
+}
+
+/* GMEM.cu:553-628 */
+void orc_expectations(int G, GaussianMixtureModelCUDA* gm, double sum_alpha) {
+  const double psiA = orc_digamma(sum_alpha);
+#pragma omp parallel for
+  for (int k = 0; k < G; k++) {
+    GaussianMixtureModelCUDA* g = &gm[k];
+    if (is_dead(g)) {
+      g->expectedLogDetCovarianceCUDA = -1e32;
+      g->expectedLogResponsivityCUDA = -1e32;
+    } else {
+      double out = 3.0 * log(2.0) + log(orc_det3(g->W_k));
+      for (int i = 0; i < 3; i++) out += orc_digamma(0.5 * (g->nu_k + 1.0 - i));
+      g->expectedLogDetCovarianceCUDA = out;
+      g->expectedLogResponsivityCUDA = orc_digamma(g->alpha_k) - psiA;
+    }
+  }
+}
+
+static double quad(const double W[6], double d0, double d1, double d2) {
+  return W[0] * d0 * d0 + W[3] * d1 * d1 + W[5] * d2 * d2 + 2.0 * W[1] * d0 * d1 +
+         2.0 * W[2] * d0 * d2 + 2.0 * W[4] * d1 * d2;
+}
+
+/* GMEM.cu:2244-2336, per voxel in FP64 */
+void orc_estep(long long N, long long S, const float* query,
+               const long long* ptr, int G, const GaussianMixtureModelCUDA* gm,
+               int K, const int* ind, float* rnk) {
+  (void)G;
+#pragma omp parallel for schedule(dynamic, 16)
+  for (long long s = 0; s < S; s++) {
+    double L[TGMM_KMAX];
+    int valid[TGMM_KMAX];
+    for (int j = 0; j < K; j++) {
+      const int k = ind[s + (long long)j * S];
+      valid[j] = k >= 0;
+      L[j] = 0.0;
+      if (!valid[j]) continue;
+      const GaussianMixtureModelCUDA* g = &gm[k];
+      double acc = 0.0;
+      for (long long v = ptr[s]; v < ptr[s + 1]; v++) {
+        const double d0 = (double)query[v] - g->m_k[0];
+        const double d1 = (double)query[v + N] - g->m_k[1];
+        const double d2 = (double)query[v + 2 * N] - g->m_k[2];
+        const double maha = 3.0 / g->beta_k + g->nu_k * quad(g->W_k, d0, d1, d2);
+        acc += -0.5 * maha + g->expectedLogResponsivityCUDA +
+               0.5 * g->expectedLogDetCovarianceCUDA;
+      }
+      L[j] = acc;
+    }
+    double mx = -INFINITY;
+    for (int j = 0; j < K; j++)
+      if (valid[j] && L[j] > mx) mx = L[j];
+    double den = 0.0;
+    if (mx > -INFINITY)
+      for (int j = 0; j < K; j++)
+        if (valid[j]) den += exp(L[j] - mx);
+    for (int j = 0; j < K; j++) {
+      double r = 0.0;
+      if (valid[j] && mx > -INFINITY) r = exp(L[j] - mx) / den;
+      rnk[s + (long long)j * S] = (float)r;
+    }
+  }
+}
+
+/* GMEM.cu:1286-1455: weights r_sk * img_v over pairs with r > 1e-3; centred
+ * second pass in FP64 (Ck = sum w (x - xk)(x - xk)^T, unnormalised). */
+void orc_mstep(long long N, long long S, const float* query, const float* img,
+               const long long* ptr, int G, int K, const int* ind,
+               const float* rnk, double* Nk, double* xk, double* Ck) {
+  /* pair lists per Gaussian, slot-major scan order */
+  int* cnt = (int*)calloc((size_t)G + 1, sizeof(int));
+  for (int j = 0; j < K; j++)
+    for (long long s = 0; s < S; s++) {
+      const long long p = s + (long long)j * S;
+      if (ind[p] >= 0 && rnk[p] > 1e-3f) cnt[ind[p] + 1]++;
+    }
+  for (int k = 0; k < G; k++) cnt[k + 1] += cnt[k];
+  long long* list = (long long*)malloc(sizeof(long long) * (size_t)(cnt[G] > 0 ? cnt[G] : 1));
+  int* fill = (int*)calloc((size_t)G + 1, sizeof(int));
+  for (int j = 0; j < K; j++)
+    for (long long s = 0; s < S; s++) {
+      const long long p = s + (long long)j * S;
+      if (ind[p] >= 0 && rnk[p] > 1e-3f) list[cnt[ind[p]] + fill[ind[p]]++] = p;
+    }
+#pragma omp parallel for schedule(dynamic, 16)
+  for (int k = 0; k < G; k++) {
+    double n = 0.0, sx[3] = {0, 0, 0};
+    for (int e = cnt[k]; e < cnt[k + 1]; e++) {
+      const long long p = list[e];
+      const long long s = p % S;
+      const double r = (double)rnk[p];
+      for (long long v = ptr[s]; v < ptr[s + 1]; v++) {
+        const double w = r * (double)img[v];
+        n += w;
+        sx[0] += w * query[v];
+        sx[1] += w * query[v + N];
+        sx[2] += w * query[v + 2 * N];
+      }
+    }
+    double m[3] = {0, 0, 0}, C[6] = {0, 0, 0, 0, 0, 0};
+    if (n > 0.0) {
+      for (int i = 0; i < 3; i++) m[i] = sx[i] / n;
+      for (int e = cnt[k]; e < cnt[k + 1]; e++) {
+        const long long p = list[e];
+        const long long s = p % S;
+        const double r = (double)rnk[p];
+        for (long long v = ptr[s]; v < ptr[s + 1]; v++) {
+          const double w = r * (double)img[v];
+          const double d[3] = {query[v] - m[0], query[v + N] - m[1], query[v + 2 * N] - m[2]};
+          int c = 0;
+          for (int a = 0; a < 3; a++)
+            for (int b = a; b < 3; b++) C[c++] += w * d[a] * d[b];
+        }
+      }
+    }
+    Nk[k] = n;
+    for (int i = 0; i < 3; i++) xk[3 * k + i] = m[i];
+    for (int i = 0; i < 6; i++) Ck[6 * k + i] = C[i];
+  }
+  free(cnt);
+  free(fill);
+  free(list);
+}
+
+void orc_update(int G, GaussianMixtureModelCUDA* gm, const double* Nk,
+                const double* xk, const double* Ck, const tgmm_em_opts* o) {
+#pragma omp parallel for
+  for (int k = 0; k < G; k++) {
+    GaussianMixtureModelCUDA* g = &gm[k];
+    if (is_dead(g) || g->fixed) continue;
+    orc_posterior(g, Nk[k], xk + 3 * k, Ck + 6 * k);
+    orc_regularize_W(g->W_k, g->nu_k, o->w4dof, o->scale, o->lambda_min,
+                     o->lambda_max, o->max_excentricity);
+  }
+}
+
+/* GMEM.cu:1813-1843 */
+void orc_death(int G, GaussianMixtureModelCUDA* gm, double sum_alpha) {
+  for (int k = 0; k < G; k++) {
+    GaussianMixtureModelCUDA* g = &gm[k];
+    int nanW = isnan(g->expectedLogDetCovarianceCUDA);
+    for (int i = 0; i < 6; i++) nanW |= isnan(g->W_k[i]);
+    if (nanW || (g->alpha_k / sum_alpha < 1e-5 && !is_dead(g) && !g->fixed)) {
+      g->alpha_k = 0.0;
+      for (int i = 0; i < 3; i++) {
+        g->m_o[i] = -1e32;
+        g->m_k[i] = -1e32;
+      }
+      g->W_k[0] = 1.0;
+      g->W_k[1] = 0.0;
+      g->W_k[2] = 0.0;
+      g->W_k[3] = 1.0;
+      g->W_k[4] = 0.0;
+      g->W_k[5] = 1.0;
+    }
+  }
+}
+
+/* GMEM.cu:1913-1978 + 2834-2873 */
+double orc_loglik(long long N, long long S, const float* query,
+                  const float* img, const long long* ptr, int G,
+                  const GaussianMixtureModelCUDA* gm, int K, const int* ind,
+                  double sum_alpha) {
+  (void)G;
+  static const double two_pi_3_2 = 15.749609945722419; /* (2 pi)^1.5 */
+  double* part = (double*)calloc((size_t)(S > 0 ? S : 1), sizeof(double));
+#pragma omp parallel for schedule(dynamic, 16)
+  for (long long s = 0; s < S; s++) {
+    double coef[TGMM_KMAX];
+    int kk[TGMM_KMAX];
+    for (int j = 0; j < K; j++) {
+      kk[j] = ind[s + (long long)j * S];
+      if (kk[j] < 0) continue;
+      const GaussianMixtureModelCUDA* g = &gm[kk[j]];
+      coef[j] = sqrt(orc_det3(g->W_k) * g->nu_k * g->nu_k * g->nu_k) / two_pi_3_2 *
+                (g->alpha_k / sum_alpha);
+    }
+    double acc = 0.0;
+    for (long long v = ptr[s]; v < ptr[s + 1]; v++) {
+      double like = 0.0;
+      for (int j = 0; j < K; j++) {
+        if (kk[j] < 0) continue;
+        const GaussianMixtureModelCUDA* g = &gm[kk[j]];
+        const double d0 = (double)query[v] - g->m_k[0];
+        const double d1 = (double)query[v + N] - g->m_k[1];
+        const double d2 = (double)query[v + 2 * N] - g->m_k[2];
+        like += coef[j] * exp(-0.5 * g->nu_k * quad(g->W_k, d0, d1, d2));
+      }
+      if (like > 0.0) acc += (double)img[v] * log(like);
+    }
+    part[s] = acc;
+  }
+  double ll = 0.0;
+  for (long long s = 0; s < S; s++) ll += part[s];
+  free(part);
+  return ll;
+}
+
+/* loop condition GMEM.cu:4145-4148: 1 = keep iterating */
+int orc_converged(double ll, double ll1, double ll2, double ll3, double tol) {
+  return !(fabs(ll - ll1) / fabs(ll1) > tol && fabs(ll - ll2) / fabs(ll2) > tol &&
+           fabs(ll - ll3) / fabs(ll3) > tol);
+}
+
+/* GMEM.cu:4328-4350 */
+void orc_assign_supervoxels(long long S, int G, GaussianMixtureModelCUDA* gm,
+                            int K, const float* rnk, const int* ind, int cap) {
+  for (int k = 0; k < G; k++) gm[k].supervoxelNum = 0;
+  for (int j = 0; j < K; j++)
+    for (long long s = 0; s < S; s++) {
+      const long long p = s + (long long)j * S;
+      if (rnk[p] > 0.5f && ind[p] >= 0) {
+        GaussianMixtureModelCUDA* g = &gm[ind[p]];
+        if (g->supervoxelNum < cap) g->supervoxelIdx[g->supervoxelNum++] = (int)s;
+      }
+    }
+}
+
+static int em_loop(long long N, long long S, int G, const float* query,
+                   const float* img, const float* cent, const long long* ptr,
+                   GaussianMixtureModelCUDA* gm, const tgmm_em_opts* o,
+                   int stop_after, float* rnk, int* ind, double* Nk_out,
+                   double* xk_out, double* Ck_out, double* ll_hist,
+                   int* iters, double* ll_final) {
+  const int K = o->K;
+  if (K < 1 || K > TGMM_KMAX || G < 0 || S < 0 || N < 0) return TGMM_ERR_ARG;
+  double* Nk = (double*)malloc(sizeof(double) * (size_t)(G > 0 ? G : 1));
+  double* xk = (double*)malloc(sizeof(double) * 3 * (size_t)(G > 0 ? G : 1));
+  double* Ck = (double*)malloc(sizeof(double) * 6 * (size_t)(G > 0 ? G : 1));
+  double ll = -1.0, ll1 = -10.0, ll2 = -100.0, ll3 = -1000.0;
+  int it = 1;
+  while (!orc_converged(ll, ll1, ll2, ll3, o->tol) && it <= o->max_iter) {
+    orc_knn(S, cent, G, gm, o->scale, K, ind);
+    orc_expectations(G, gm, orc_sum_alpha(G, gm));
+    orc_estep(N, S, query, ptr, G, gm, K, ind, rnk);
+    orc_mstep(N, S, query, img, ptr, G, K, ind, rnk, Nk, xk, Ck);
+    if (stop_after > 0 && it == stop_after) {
+      memcpy(Nk_out, Nk, sizeof(double) * G);
+      memcpy(xk_out, xk, sizeof(double) * 3 * G);
+      memcpy(Ck_out, Ck, sizeof(double) * 6 * G);
+      break;
+    }
+    orc_update(G, gm, Nk, xk, Ck, o);
+    orc_death(G, gm, orc_sum_alpha(G, gm));
+    ll3 = ll2;
+    ll2 = ll1;
+    ll1 = ll;
+    ll = orc_loglik(N, S, query, img, ptr, G, gm, K, ind, orc_sum_alpha(G, gm));
+    if (ll_hist) ll_hist[it - 1] = ll;
+    it++;
+  }
+  if (iters) *iters = it - 1;
+  if (ll_final) *ll_final = ll;
+  free(Nk);
+  free(xk);
+  free(Ck);
+  return TGMM_OK;
+}
+
+int orc_em_run(long long N, long long S, int G, const float* query,
+               const float* img, const float* cent, const long long* ptr,
+               GaussianMixtureModelCUDA* gm, const tgmm_em_opts* o, float* rnk,
+               int* ind, double* ll_hist, int* iters, double* ll_final) {
+  int rc = em_loop(N, S, G, query, img, cent, ptr, gm, o, 0, rnk, ind, NULL,
+                   NULL, NULL, ll_hist, iters, ll_final);
+  if (rc == TGMM_OK) {
+    const int cap = o->max_sv_per_gaussian > 0 ? o->max_sv_per_gaussian : 0;
+    orc_assign_supervoxels(S, G, gm, o->K, rnk, ind, cap);
+  }
+  return rc;
+}
+
+int orc_em_partial(long long N, long long S, int G, const float* query,
+                   const float* img, const float* cent, const long long* ptr,
+                   GaussianMixtureModelCUDA* gm, const tgmm_em_opts* o,
+                   int iters, float* rnk, int* ind, double* Nk, double* xk,
+                   double* Ck) {
+  return em_loop(N, S, G, query, img, cent, ptr, gm, o, iters, rnk, ind, Nk,
+                 xk, Ck, NULL, NULL, NULL);
+}

@@ -1,0 +1,47 @@
+"""Shared helpers for the parity tests."""
+import numpy as np
+
+from tgmm_amd.records import GMM_DTYPE, is_dead
+
+
+def rel_err(a, b, floor=0.0):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    den = np.maximum(np.abs(b), floor)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        r = np.where(den > 0, np.abs(a - b) / den, np.abs(a - b))
+    return float(np.max(r)) if r.size else 0.0
+
+
+def record_errors(gm, ref):
+    """Max relative errors over live Gaussians: means (per component, relative
+    to |m|), precision (Frobenius), nu, beta, alpha."""
+    live = ~is_dead(ref)
+    out = {}
+    if not live.any():
+        return dict(m=0.0, W=0.0, nu=0.0, beta=0.0, alpha=0.0)
+    a, b = gm[live], ref[live]
+    out["m"] = float(np.max(np.linalg.norm(a["m_k"] - b["m_k"], axis=1) /
+                            np.linalg.norm(b["m_k"], axis=1)))
+    dw = np.linalg.norm(a["W_k"] - b["W_k"], axis=1)
+    out["W"] = float(np.max(dw / np.linalg.norm(b["W_k"], axis=1)))
+    for f in ("nu_k", "beta_k", "alpha_k"):
+        out[f[:-2]] = rel_err(a[f], b[f], floor=1e-300)
+    return out
+
+
+def dup_gaussians(frame, offset=1.5, seed=7):
+    """Two Gaussians per cell (the second displaced by `offset` voxels), so
+    that supervoxels are shared and responsibilities are soft."""
+    rng = np.random.default_rng(seed)
+    g = frame.gm
+    g2 = g.copy()
+    d = rng.normal(size=(g.shape[0], 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    g2["m_k"] += offset * d
+    g2["m_o"] = g2["m_k"]
+    out = np.concatenate([g, g2]).astype(GMM_DTYPE)
+    import copy
+    f2 = copy.copy(frame)
+    f2.gm = out
+    return f2

@@ -1,0 +1,163 @@
+"""GPU parity: the HIP library against the FP64 oracle on seeded frames."""
+import copy
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+from helpers import dup_gaussians, record_errors, rel_err
+from tgmm_amd import em, synth
+from tgmm_amd.records import GMM_DTYPE
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-5  # relative, means / precisions / nu / beta / alpha / log-likelihood
+
+
+@pytest.fixture(scope="module")
+def c1():
+    return synth.config1()
+
+
+def _opts(K=5, iters=10, tol=-1.0):
+    return em.make_opts(K=K, max_iter=iters, tol=tol)
+
+
+def _oopts(K=5, iters=10, tol=-1.0):
+    return O.make_opts(K=K, max_iter=iters, tol=tol)
+
+
+def _run_batch(frames, opts, mode=em.SOLVE_GRAPH):
+    with em.Batch(frames, opts) as b:
+        b.solve(mode)
+        return b.fetch()
+
+
+def test_knn_bit_exact_first_iteration(c1):
+    """After one iteration, ind holds the kNN of the initial means."""
+    out = _run_batch([c1], _opts(iters=1))[0]
+    ind_ref = np.zeros(5 * c1.S, np.int32)
+    O.oracle().orc_knn(c1.S, O.p(c1.centroids, O._fp), c1.G, c1.gm.ctypes.data,
+                       O.p(np.array([1, 1, 5], np.float32), O._fp), 5, O.p(ind_ref, O._ip))
+    assert np.array_equal(out["ind"], ind_ref)
+
+
+@pytest.mark.parametrize("K", [5, 8])
+def test_knn_bit_exact_dense(c1, K):
+    """Many Gaussians per cell (ties and near ties): 4 copies of each mean
+    with sub-voxel offsets, kNN must equal brute force exactly."""
+    f = copy.copy(c1)
+    rng = np.random.default_rng(3)
+    gms = [c1.gm]
+    for r in range(3):
+        g = c1.gm.copy()
+        g["m_k"] += np.round(rng.normal(scale=2.0, size=(c1.G, 3)) * 4) / 4
+        g["m_o"] = g["m_k"]
+        gms.append(g)
+    f.gm = np.concatenate(gms).astype(GMM_DTYPE)
+    out = _run_batch([f], _opts(K=K, iters=1))[0]
+    ind_ref = np.zeros(K * f.S, np.int32)
+    O.oracle().orc_knn(f.S, O.p(f.centroids, O._fp), f.G, f.gm.ctypes.data,
+                       O.p(np.array([1, 1, 5], np.float32), O._fp), K, O.p(ind_ref, O._ip))
+    assert np.array_equal(out["ind"], ind_ref)
+
+
+def test_first_iteration_matches_oracle(c1):
+    out = _run_batch([c1], _opts(iters=1))[0]
+    gm_o, rnk_o, ind_o, hist, it, ll = O.em_run(c1, _oopts(iters=1))
+    assert it == out["iterations"] == 1
+    assert np.array_equal(out["ind"], ind_o)
+    assert np.max(np.abs(out["rnk"] - rnk_o)) < 1e-6
+    e = record_errors(out["gm"], gm_o)
+    assert max(e.values()) < TOL, e
+    assert rel_err(out["ll"], ll) < TOL
+
+
+def test_ten_iterations_match_oracle(c1):
+    out = _run_batch([c1], _opts(iters=10))[0]
+    gm_o, rnk_o, ind_o, hist, it, ll = O.em_run(c1, _oopts(iters=10))
+    assert out["iterations"] == it == 10
+    e = record_errors(out["gm"], gm_o)
+    assert max(e.values()) < TOL, e
+    assert rel_err(out["ll"], ll) < TOL
+    assert np.array_equal(out["gm"]["supervoxelNum"], gm_o["supervoxelNum"])
+
+
+def soft_frame(max_sv=2, offset=1.0):
+    """Tiny supervoxels and two Gaussians per cell: thousands of (s, k) pairs
+    with responsibilities strictly between 1e-3 and 0.999."""
+    cells = synth.sample_cells(99, 120, (256.0, 256.0, 60.0), 5.0, 3.0, 6.0)
+    f = synth.rasterize(100, cells, (256.0, 256.0, 60.0), 5.0, 20000, max_sv_size=max_sv)
+    return dup_gaussians(f, offset=offset)
+
+
+@pytest.mark.parametrize("max_sv", [2, 4])
+def test_soft_assignments_match_oracle(max_sv):
+    f = soft_frame(max_sv)
+    out = _run_batch([f], _opts(iters=10))[0]
+    gm_o, rnk_o, ind_o, hist, it, ll = O.em_run(f, _oopts(iters=10))
+    soft = ((rnk_o > 1e-3) & (rnk_o < 0.999)).sum()
+    assert soft > 1000  # the case really exercises soft responsibilities
+    e = record_errors(out["gm"], gm_o)
+    assert max(e.values()) < TOL, e
+    assert rel_err(out["ll"], ll) < TOL
+    # responsibilities: FP32 exponentials after FP64 max-subtraction
+    both = (rnk_o > 1e-6) & (out["ind"] == ind_o)
+    assert rel_err(out["rnk"][both], rnk_o[both]) < 1e-5
+
+
+def test_modes_bitwise_identical(c1):
+    a = _run_batch([c1], _opts(iters=5), em.SOLVE_GRAPH)[0]
+    b = _run_batch([c1], _opts(iters=5), em.SOLVE_EAGER)[0]
+    c = _run_batch([c1], _opts(iters=5), em.SOLVE_HOSTSYNC)[0]
+    for x in (b, c):
+        assert a["gm"].tobytes() == x["gm"].tobytes()
+        assert np.array_equal(a["rnk"], x["rnk"]) and np.array_equal(a["ind"], x["ind"])
+        assert a["ll"] == x["ll"]
+
+
+def test_alone_equals_in_the_mix(c1):
+    """Each frame's result in a 3-frame batch is bitwise its result alone."""
+    f2 = dup_gaussians(c1)
+    f3 = synth.config1(seed=12345, n_cells=50, target_voxels=9000)
+    mix = _run_batch([c1, f2, f3], _opts(iters=6))
+    for f, m in zip([c1, f2, f3], mix):
+        alone = _run_batch([f], _opts(iters=6))[0]
+        assert alone["gm"].tobytes() == m["gm"].tobytes()
+        assert np.array_equal(alone["rnk"], m["rnk"])
+        assert alone["ll"] == m["ll"]
+
+
+def test_legacy_entry_points(c1, tmp_path):
+    em.regularizePrecisionMatrixConstants.setConstants(0.02, 0.2222, 9.0)
+    scale = np.array([1, 1, 5], np.float32)
+    d = em.GMEMinitializeMemoryWithSupervoxels(c1.query, c1.img, scale, c1.N, c1.S,
+                                               c1.centroids, c1.sv_ptr)
+    em.copyScaleToDvice(scale)
+    gm = c1.gm.copy()
+    dbg = str(tmp_path / "rnk.bin")
+    em.GMEMvariationalInferenceCUDAWithSupervoxels(d, gm, c1.N, c1.G, c1.S, 10, -1.0,
+                                                   debugPath=dbg)
+    rnk = em.copy_device_to_host(d.rnkCUDA, 5 * c1.S * 4, np.float32)
+    ind = em.copy_device_to_host(d.indCUDA, 5 * c1.S * 4, np.int32)
+    em.GMEMreleaseMemoryWithSupervoxels(d)
+    assert d.queryCUDA.value is None
+    gm_o, rnk_o, ind_o, hist, it, ll = O.em_run(c1, _oopts(iters=10))
+    e = record_errors(gm, gm_o)
+    assert max(e.values()) < TOL, e
+    assert np.array_equal(ind, ind_o)
+    assert np.max(np.abs(rnk - rnk_o)) < 1e-5
+    assert np.array_equal(gm["supervoxelIdx"], gm_o["supervoxelIdx"])
+    raw = open(dbg, "rb").read()
+    hdr = np.frombuffer(raw[:8], np.int32)
+    assert hdr[0] == c1.S and hdr[1] == 5
+    assert np.array_equal(np.frombuffer(raw[8 + 20 * c1.S:], np.int32), ind)
+
+
+def test_legacy_requires_constants_and_limits(c1):
+    L = em.load()
+    # ref_nb above the reference limit -> TGMM_ERR_LIMIT, no exit()
+    rc = L.tgmm_GMEMvariationalInferenceCUDAWithSupervoxels(
+        None, None, None, None, None, None, None, None, None, 10, 70000, 0, 1, -1.0, 0, 0,
+        1, None)
+    assert rc == -2

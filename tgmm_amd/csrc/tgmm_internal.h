@@ -1,0 +1,123 @@
+// tgmm_internal.h -- device workspace layout shared by the kernels
+// (tgmm_kernels.hip) and the host runtime (tgmm_runtime.hip).
+//
+// One workspace holds any number of frames. Every per-element array is a
+// flat, capacity-sized structure of arrays:
+//   voxels       [Vcap]   x, y, z, img (f32)
+//   supervoxels  [Scap]   frame id, voxel range, kNN centroid, moments,
+//                          K candidate slots (slot-major: j*Scap + s)
+//   Gaussians    [Gcap]   27-double state (record field order), E-step and
+//                          log-likelihood parameters, pair segments
+//   grid cells   [Ccap]   per-frame uniform kNN grids
+// A frame owns a contiguous range in each space; supervoxel and Gaussian
+// ranges start on multiples of 256 so that every 256-wide block (and every
+// 64-wide log-likelihood block) belongs to exactly one frame, which makes the
+// per-frame reductions deterministic without atomics on floating point.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace tgmm {
+
+constexpr int kAlign = 256;      // supervoxel / Gaussian range alignment
+constexpr int kGBlock = 256;     // threads per block, Gaussian-parallel kernels
+constexpr int kSBlock = 256;     // threads per block, supervoxel-lane kernels
+constexpr int kLLsv = 64;        // supervoxels per log-likelihood block
+constexpr int kScanTile = 4096;  // elements per scan block
+constexpr int kPairShift = 4;    // pair code = (s << 4) | j, j < 16
+
+// state field indices (record order, GMEMupdateCUDA.h:22-34)
+enum {
+  GS_BETA = 0, GS_NU = 1, GS_ALPHA = 2, GS_M = 3, GS_W = 6, GS_ELD = 12, GS_ELP = 13,
+  GS_SPLIT = 14, GS_BETA_O = 15, GS_NU_O = 16, GS_ALPHA_O = 17, GS_M_O = 18, GS_W_O = 21,
+  GS_NFIELDS = 27
+};
+
+struct FrameDev {
+  long long v0, nv;   // voxel range
+  long long s0, ns;   // supervoxel range (s0 % 256 == 0)
+  long long g0, ng;   // Gaussian range (g0 % 256 == 0)
+  long long c0, ncap; // grid-cell range
+  float org[3];       // grid origin in scaled coordinates
+  float h;            // cell edge
+  int dims[3];
+  int nbg;            // 256-blocks over the Gaussian range
+  int nbl;            // 64-supervoxel blocks over the supervoxel range
+  int max_iter;
+  double tol;
+  double sum_alpha, psi_sum_alpha, log_sum_alpha, sum_alpha2;
+  double ll, ll1, ll2, ll3;
+  int iter;           // reference numIterEM (starts at 1)
+  int active;         // keeps iterating
+  int status;
+  int pad;
+};
+
+struct KOpts {
+  int K;
+  int w4dof;
+  float scale[3];      // kNN anisotropy (uploadScaleCUDA, knnCuda.cu:827-830)
+  float scale_reg[3];  // regularisation anisotropy (copyScaleToDvice)
+  double lmin, lmax, emax;
+};
+
+struct DevPtrs {
+  // voxels
+  float *vx, *vy, *vz, *vimg;
+  // supervoxels
+  int* sv_frame;
+  long long* sv_beg;
+  int* sv_len;
+  float* sv_cent;   // 3*Scap SoA
+  double* mom;      // 10*Scap: n, c[3], Q[6]  (unweighted, FP64)
+  double* wmom;     // 10*Scap: W, mu[3], C[6] (intensity-weighted, FP64)
+  float* anchor;    // 3*Scap: first voxel of the supervoxel
+  int* ind;         // K*Scap
+  float* rnk;       // K*Scap
+  int* sblk_frame;  // Scap/256
+  // Gaussians
+  int* g_frame;
+  double* gs;       // 27*Gcap
+  unsigned char* fixed;
+  double* kappa;    // E-step per-voxel constant
+  float* nuW;       // 6*Gcap: nu_k * W_k (f32) for the log-likelihood
+  double* lg;       // log alpha + 0.5 log det(nu W) - 1.5 log 2pi
+  float* mf;        // 3*Gcap f32 means (kNN references)
+  int* gcell;
+  int* gblk_frame;  // Gcap/256
+  // M-step segments
+  int* seg_cnt;     // Gcap+1
+  int* seg_start;   // Gcap+1
+  int* pairs;       // K*Scap
+  int* pairs_sorted;
+  // grid
+  int* cell_cnt;    // Ccap+1
+  int* cell_start;  // Ccap+1
+  float* cg;        // 3*Gcap sorted reference positions
+  int* cg_idx;      // Gcap
+  // reductions
+  double* part_g;   // Gcap/256
+  double* part_s;   // Scap/64
+  unsigned* done;   // 4*Fcap
+  int* scan_blk;    // scan partials
+  FrameDev* fr;
+  long long Scap, Gcap, Ccap, Vcap;
+  int Fcap;
+  int nframes;
+};
+
+// kernel launchers (tgmm_kernels.hip); all enqueue on `st`
+// per-supervoxel moments (once per frame)
+hipError_t launch_prep(const DevPtrs& P, hipStream_t st);
+// kNN grid shape, iteration state and initial sum of alpha per frame
+hipError_t launch_reset(const DevPtrs& P, const KOpts& O, hipStream_t st);
+// one EM iteration of every active frame; optional events around one kernel
+hipError_t launch_iteration(const DevPtrs& P, const KOpts& O, hipStream_t st,
+                            hipEvent_t ev_before = nullptr, hipEvent_t ev_after = nullptr,
+                            int timed_kernel = -1);
+int kmax_for(int K);
+
+// timed_kernel ids for launch_iteration
+enum { TK_KNN = 0, TK_ESTEP = 1, TK_MSTEP = 2, TK_LOGLIK = 3 };
+
+}  // namespace tgmm

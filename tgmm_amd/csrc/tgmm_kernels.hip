@@ -1,0 +1,915 @@
+// tgmm_kernels.hip -- gfx950 kernels of the supervoxel variational-inference
+// EM (one EM iteration = launch_iteration). Wave64 throughout; no MFMA (the
+// 3x3 forms are not dense contractions); no floating-point atomics, so every
+// result is independent of scheduling and of which other frames share the
+// launch.
+//
+// Per iteration, over all frames of a workspace:
+//   gauss_prep     E[ln det Lambda], E[ln pi] (GMEM.cu:553-628), E-step
+//                  constant, f32 means, kNN grid binning
+//   scan/scatter   counting sort of the means into per-frame uniform grids
+//   knn            exact K nearest means per supervoxel centroid by ring
+//                  search (knnCuda.cu:61-144 distance, (dist, idx) order)
+//   estep          responsibilities from per-supervoxel moments
+//                  (GMEM.cu:2244-2336), significant-pair counts
+//   scan/scatter   pairs (s, j) with r > 1e-3 grouped per Gaussian
+//   seg_sort       each Gaussian's pair list ordered by supervoxel
+//   mstep_update   FP64 segmented sufficient statistics (GMEM.cu:1286-1455),
+//                  posterior (1601-1677), regularisation (1693-1809),
+//                  per-frame sum of alpha
+//   death          dead-cell rule (1813-1843), log-likelihood parameters,
+//                  per-frame sum of alpha
+//   loglik         voxel stream: img * log sum_k pi_k N(x; m_k, (nu W)^-1)
+//                  (1913-1978), per-frame total, convergence (4145-4148)
+#include <hip/hip_runtime.h>
+
+#include "tgmm_device.h"
+#include "tgmm_internal.h"
+
+namespace tgmm {
+
+// ---------------------------------------------------------------- helpers --
+__device__ inline double wave_sum(double v) {
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
+  return v;
+}
+
+// 256-thread block sum in a fixed order; result valid in thread 0.
+__device__ inline double block_sum256(double v, double* red4) {
+  v = wave_sum(v);
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  __syncthreads();
+  if (l == 0) red4[w] = v;
+  __syncthreads();
+  return (threadIdx.x == 0) ? ((red4[0] + red4[1]) + red4[2]) + red4[3] : 0.0;
+}
+
+__device__ inline double load_coherent(const double* p) {
+  unsigned long long u =
+      __hip_atomic_load((const unsigned long long*)p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  return __longlong_as_double((long long)u);
+}
+
+// Per-frame "last block" reduction: every block of the frame stores its
+// partial, the block that arrives last sums all partials in index order and
+// resets the arrival counter. Returns the frame total in thread 0 of the
+// last block; *is_last tells every thread of the block whether it was last.
+__device__ inline double frame_reduce(double v, double* part, long long pidx, long long pbase,
+                                      int nblk, unsigned* done, double* red4, int* is_last) {
+  __shared__ int s_last;
+  const double t = block_sum256(v, red4);
+  if (threadIdx.x == 0) {
+    part[pidx] = t;
+    __threadfence();
+    const unsigned prev = atomicAdd(done, 1u);
+    s_last = (prev == (unsigned)(nblk - 1));
+  }
+  __syncthreads();
+  *is_last = s_last;
+  if (!s_last) return 0.0;
+  __threadfence();
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < nblk; i += 256) acc += load_coherent(part + pbase + i);
+  const double tot = block_sum256(acc, red4);
+  if (threadIdx.x == 0) *done = 0u;
+  return tot;
+}
+
+__device__ inline double gsv(const DevPtrs& P, int f, long long k) { return P.gs[f * P.Gcap + k]; }
+
+// reference loop condition (GMEM.cu:4145-4148)
+__device__ inline int keep_going(double ll, double l1, double l2, double l3, double tol) {
+  return (fabs(ll - l1) / fabs(l1) > tol) && (fabs(ll - l2) / fabs(l2) > tol) &&
+         (fabs(ll - l3) / fabs(l3) > tol);
+}
+
+// FP32 kNN distance in the reference's order (knnCuda.cu:116-124), never
+// contracted into FMAs so that host and device agree bit for bit.
+__device__ inline float knn_dist(float q0, float q1, float q2, float m0, float m1, float m2,
+                                 float s0, float s1, float s2) {
+#pragma clang fp contract(off)
+  const float a = (q0 - m0) * s0;
+  float d = a * a;
+  const float b = (q1 - m1) * s1;
+  d = d + b * b;
+  const float e = (q2 - m2) * s2;
+  d = d + e * e;
+  return d;
+}
+
+__device__ inline bool lex_less(float da, int ia, float db, int ib) {
+  return da < db || (da == db && ia < ib);
+}
+
+// ------------------------------------------------------------ prep kernels --
+// One wave per supervoxel: unweighted moments (E-step) and intensity-weighted
+// moments (M-step), FP64, centred about the supervoxel's own means.
+__global__ __launch_bounds__(256) void k_moments(DevPtrs P) {
+  const long long s = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (s >= P.Scap) return;
+  const int f = P.sv_frame[s];
+  if (f < 0) return;
+  const long long beg = P.sv_beg[s];
+  const int n = P.sv_len[s];
+  const long long S = P.Scap;
+  if (n <= 0) {
+    if (lane < 10) {
+      P.mom[lane * S + s] = 0.0;
+      P.wmom[lane * S + s] = 0.0;
+    }
+    if (lane < 3) P.anchor[lane * S + s] = 0.0f;
+    return;
+  }
+  const float a0 = P.vx[beg], a1 = P.vy[beg], a2 = P.vz[beg];
+  double sx = 0, sy = 0, sz = 0, sw = 0, swx = 0, swy = 0, swz = 0;
+  for (int i = lane; i < n; i += 64) {
+    const double x = (double)P.vx[beg + i] - a0, y = (double)P.vy[beg + i] - a1,
+                 z = (double)P.vz[beg + i] - a2, w = (double)P.vimg[beg + i];
+    sx += x;
+    sy += y;
+    sz += z;
+    sw += w;
+    swx += w * x;
+    swy += w * y;
+    swz += w * z;
+  }
+  sx = wave_sum(sx);
+  sy = wave_sum(sy);
+  sz = wave_sum(sz);
+  sw = wave_sum(sw);
+  swx = wave_sum(swx);
+  swy = wave_sum(swy);
+  swz = wave_sum(swz);
+  const double cx = sx / n, cy = sy / n, cz = sz / n;
+  const double ux = sw > 0 ? swx / sw : cx, uy = sw > 0 ? swy / sw : cy,
+               uz = sw > 0 ? swz / sw : cz;
+  double q[6] = {0, 0, 0, 0, 0, 0}, c[6] = {0, 0, 0, 0, 0, 0};
+  for (int i = lane; i < n; i += 64) {
+    const double x = (double)P.vx[beg + i] - a0, y = (double)P.vy[beg + i] - a1,
+                 z = (double)P.vz[beg + i] - a2, w = (double)P.vimg[beg + i];
+    const double d0 = x - cx, d1 = y - cy, d2 = z - cz;
+    q[0] += d0 * d0;
+    q[1] += d0 * d1;
+    q[2] += d0 * d2;
+    q[3] += d1 * d1;
+    q[4] += d1 * d2;
+    q[5] += d2 * d2;
+    const double e0 = x - ux, e1 = y - uy, e2 = z - uz;
+    c[0] += w * e0 * e0;
+    c[1] += w * e0 * e1;
+    c[2] += w * e0 * e2;
+    c[3] += w * e1 * e1;
+    c[4] += w * e1 * e2;
+    c[5] += w * e2 * e2;
+  }
+#pragma unroll
+  for (int i = 0; i < 6; i++) {
+    q[i] = wave_sum(q[i]);
+    c[i] = wave_sum(c[i]);
+  }
+  if (lane == 0) {
+    P.mom[0 * S + s] = (double)n;
+    P.mom[1 * S + s] = (double)a0 + cx;
+    P.mom[2 * S + s] = (double)a1 + cy;
+    P.mom[3 * S + s] = (double)a2 + cz;
+    for (int i = 0; i < 6; i++) P.mom[(4 + i) * S + s] = q[i];
+    P.wmom[0 * S + s] = sw;
+    P.wmom[1 * S + s] = (double)a0 + ux;
+    P.wmom[2 * S + s] = (double)a1 + uy;
+    P.wmom[3 * S + s] = (double)a2 + uz;
+    for (int i = 0; i < 6; i++) P.wmom[(4 + i) * S + s] = c[i];
+    P.anchor[0 * S + s] = a0;
+    P.anchor[1 * S + s] = a1;
+    P.anchor[2 * S + s] = a2;
+  }
+}
+
+// One block per frame: bounding box of the scaled kNN centroids, grid shape
+// (about two means per cell, at most ncap cells), iteration state init.
+__global__ __launch_bounds__(256) void k_grid_setup(DevPtrs P, KOpts O) {
+  const int f = blockIdx.x;
+  if (f >= P.nframes) return;
+  FrameDev* F = P.fr + f;
+  const long long s0 = F->s0, ns = F->ns;
+  const long long S = P.Scap;
+  __shared__ double red[6][256];
+  double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+  for (long long i = threadIdx.x; i < ns; i += 256) {
+    for (int a = 0; a < 3; a++) {
+      const double u = (double)P.sv_cent[a * S + s0 + i] * (double)O.scale[a];
+      lo[a] = fmin(lo[a], u);
+      hi[a] = fmax(hi[a], u);
+    }
+  }
+  for (int a = 0; a < 3; a++) {
+    red[a][threadIdx.x] = lo[a];
+    red[3 + a][threadIdx.x] = hi[a];
+  }
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if ((int)threadIdx.x < st)
+      for (int a = 0; a < 3; a++) {
+        red[a][threadIdx.x] = fmin(red[a][threadIdx.x], red[a][threadIdx.x + st]);
+        red[3 + a][threadIdx.x] = fmax(red[3 + a][threadIdx.x], red[3 + a][threadIdx.x + st]);
+      }
+    __syncthreads();
+  }
+  if (threadIdx.x != 0) return;
+  double ext[3], org[3];
+  for (int a = 0; a < 3; a++) {
+    org[a] = ns > 0 ? red[a][0] : 0.0;
+    ext[a] = ns > 0 ? fmax(red[3 + a][0] - red[a][0], 0.0) : 0.0;
+  }
+  const double target = fmax(1.0, ceil(0.5 * (double)F->ng));
+  const double vol = fmax(ext[0], 1e-3) * fmax(ext[1], 1e-3) * fmax(ext[2], 1e-3);
+  double h = cbrt(vol / target);
+  if (!(h > 0.0) || !(h < 1e30)) h = 1.0;
+  // the grid stores f32 origin and edge; cell of u is floor((u - org) / h)
+  float orgf[3];
+  for (int a = 0; a < 3; a++) {
+    orgf[a] = (float)org[a];
+    if ((double)orgf[a] > org[a]) orgf[a] = nextafterf(orgf[a], -INFINITY);
+  }
+  long long dims[3] = {1, 1, 1};
+  float hf = (float)h;
+  for (int guard = 0; guard < 400; guard++) {
+    hf = (float)h;
+    for (int a = 0; a < 3; a++) {
+      const double top = org[a] + ext[a];
+      long long d = (long long)floor((top - (double)orgf[a]) / (double)hf) + 1;
+      if (d < 1) d = 1;
+      // the last cell must reach past the largest coordinate
+      while ((double)orgf[a] + (double)d * (double)hf <= top) d++;
+      dims[a] = d;
+    }
+    if (dims[0] * dims[1] * dims[2] <= F->ncap) break;
+    h *= 1.25;
+  }
+  for (int a = 0; a < 3; a++) {
+    F->org[a] = orgf[a];
+    F->dims[a] = (int)dims[a];
+  }
+  F->h = hf;
+  if (dims[0] * dims[1] * dims[2] > F->ncap) {  // cannot happen for finite input
+    F->dims[0] = F->dims[1] = F->dims[2] = 1;
+    F->h = 3.0e38f;
+  }
+  F->ll = -1.0;
+  F->ll1 = -10.0;
+  F->ll2 = -100.0;
+  F->ll3 = -1000.0;
+  F->iter = 1;
+  F->active = (ns > 0 && F->ng > 0 && F->max_iter >= 1 &&
+               keep_going(F->ll, F->ll1, F->ll2, F->ll3, F->tol))
+                  ? 1
+                  : 0;
+  F->status = 0;
+}
+
+// Sum of alpha per frame before the first iteration (GMEM.cu:4167).
+__global__ __launch_bounds__(256) void k_alpha_init(DevPtrs P) {
+  __shared__ double red[4];
+  const int f = P.gblk_frame[blockIdx.x];
+  if (f < 0) return;
+  FrameDev* F = P.fr + f;
+  if (!F->active) return;
+  const long long k = (long long)blockIdx.x * 256 + threadIdx.x;
+  const double a = (P.g_frame[k] == f) ? gsv(P, GS_ALPHA, k) : 0.0;
+  int last;
+  const double tot = frame_reduce(a, P.part_g, blockIdx.x, F->g0 / 256, F->nbg,
+                                  P.done + 4 * f + 0, red, &last);
+  if (last && threadIdx.x == 0) {
+    F->sum_alpha = tot;
+    F->psi_sum_alpha = digamma(tot);
+    F->log_sum_alpha = log(tot);
+  }
+}
+
+// ---------------------------------------------------------------- scans --
+__global__ __launch_bounds__(256) void k_scan_partial(const int* in, long long n, int* blk) {
+  const long long base = (long long)blockIdx.x * kScanTile;
+  int acc = 0;
+  for (int i = threadIdx.x; i < kScanTile; i += 256) {
+    const long long j = base + i;
+    if (j < n) acc += in[j];
+  }
+  __shared__ int red[256];
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if ((int)threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) blk[blockIdx.x] = red[0];
+}
+
+// exclusive scan; each thread owns 16 consecutive elements of the tile
+__global__ __launch_bounds__(256) void k_scan_final(const int* in, long long n, const int* blk,
+                                                    int* out) {
+  __shared__ int red[256];
+  int pre = 0;
+  for (int i = threadIdx.x; i < (int)blockIdx.x; i += 256) pre += blk[i];
+  red[threadIdx.x] = pre;
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if ((int)threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
+    __syncthreads();
+  }
+  const int block_pre = red[0];
+  __syncthreads();
+  const long long base = (long long)blockIdx.x * kScanTile + threadIdx.x * 16;
+  int v[16];
+  int tsum = 0;
+#pragma unroll
+  for (int i = 0; i < 16; i++) {
+    v[i] = (base + i < n) ? in[base + i] : 0;
+    tsum += v[i];
+  }
+  red[threadIdx.x] = tsum;
+  __syncthreads();
+  // Hillis-Steele inclusive scan of thread sums
+  for (int st = 1; st < 256; st <<= 1) {
+    const int t = ((int)threadIdx.x >= st) ? red[threadIdx.x - st] : 0;
+    __syncthreads();
+    red[threadIdx.x] += t;
+    __syncthreads();
+  }
+  int run = block_pre + red[threadIdx.x] - tsum;
+#pragma unroll
+  for (int i = 0; i < 16; i++) {
+    if (base + i < n) out[base + i] = run;
+    run += v[i];
+  }
+}
+
+// ------------------------------------------------------- iteration kernels --
+// E-step constants and kNN grid binning (GMEM.cu:553-628, 527-539).
+__global__ __launch_bounds__(256) void k_gauss_prep(DevPtrs P, KOpts O) {
+  const int f = P.gblk_frame[blockIdx.x];
+  if (f < 0) return;
+  const FrameDev* F = P.fr + f;
+  if (!F->active) return;
+  const long long k = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (P.g_frame[k] != f) return;
+  const long long G = P.Gcap;
+  if (gsv(P, GS_M_O, k) < -1e31) {
+    P.gs[GS_ELD * G + k] = -1e32;
+    P.gs[GS_ELP * G + k] = -1e32;
+    P.gcell[k] = -1;
+    return;
+  }
+  double W[6];
+  for (int i = 0; i < 6; i++) W[i] = gsv(P, GS_W + i, k);
+  const double nu = gsv(P, GS_NU, k), beta = gsv(P, GS_BETA, k), alpha = gsv(P, GS_ALPHA, k);
+  double eld = 3.0 * 0.69314718055994530942 + log(det3(W));
+  for (int i = 0; i < 3; i++) eld += digamma(0.5 * (nu + 1.0 - i));
+  const double elp = digamma(alpha) - F->psi_sum_alpha;
+  P.gs[GS_ELD * G + k] = eld;
+  P.gs[GS_ELP * G + k] = elp;
+  P.kappa[k] = elp + 0.5 * eld - 1.5 / beta;
+  int c[3];
+  for (int a = 0; a < 3; a++) {
+    const float m = (float)gsv(P, GS_M + a, k);
+    P.mf[a * G + k] = m;
+    const double u = ((double)m * (double)O.scale[a] - (double)F->org[a]) / (double)F->h;
+    int ci = 0;
+    if (u >= (double)F->dims[a]) ci = F->dims[a] - 1;
+    else if (u > 0.0) ci = (int)floor(u);  // NaN and negatives go to 0
+    c[a] = ci;
+  }
+  const long long cell = F->c0 + ((long long)c[2] * F->dims[1] + c[1]) * F->dims[0] + c[0];
+  P.gcell[k] = (int)cell;
+  atomicAdd(P.cell_cnt + cell, 1);
+}
+
+__global__ __launch_bounds__(256) void k_grid_scatter(DevPtrs P) {
+  const int f = P.gblk_frame[blockIdx.x];
+  if (f < 0 || !P.fr[f].active) return;
+  const long long k = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (P.g_frame[k] != f) return;
+  const int c = P.gcell[k];
+  if (c < 0) return;
+  const int pos = P.cell_start[c] + atomicSub(P.cell_cnt + c, 1) - 1;
+  const long long G = P.Gcap;
+  P.cg[pos] = P.mf[k];
+  P.cg[G + pos] = P.mf[G + k];
+  P.cg[2 * G + pos] = P.mf[2 * G + k];
+  P.cg_idx[pos] = (int)k;
+}
+
+// Exact K nearest means by expanding shells of grid cells. After shell r,
+// every unvisited mean lies beyond an open face of the visited block, so its
+// scaled distance is at least LB; the search stops once the K-th best
+// distance is below LB^2 with a relative margin far above f32 rounding, so
+// the result equals brute force (ties broken by the lower index).
+template <int KM>
+__global__ __launch_bounds__(256) void k_knn(DevPtrs P, KOpts O) {
+  const int f = P.sblk_frame[blockIdx.x];
+  if (f < 0 || !P.fr[f].active) return;
+  const long long s = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (P.sv_frame[s] != f) return;
+  const FrameDev* F = P.fr + f;
+  const long long S = P.Scap, G = P.Gcap;
+  const int K = O.K;
+  const float s0 = O.scale[0], s1 = O.scale[1], s2 = O.scale[2];
+  const float q0 = P.sv_cent[s], q1 = P.sv_cent[S + s], q2 = P.sv_cent[2 * S + s];
+  const int nx = F->dims[0], ny = F->dims[1], nz = F->dims[2];
+  const double h = F->h;
+  const double u[3] = {(double)q0 * s0, (double)q1 * s1, (double)q2 * s2};
+  const double org[3] = {F->org[0], F->org[1], F->org[2]};
+  const int dims[3] = {nx, ny, nz};
+  int c[3];
+  for (int a = 0; a < 3; a++) {
+    const double t = (u[a] - org[a]) / h;
+    int ci = 0;
+    if (t >= (double)dims[a]) ci = dims[a] - 1;
+    else if (t > 0.0) ci = (int)floor(t);
+    c[a] = ci;
+  }
+  float bd[KM];
+  int bi[KM];
+#pragma unroll
+  for (int j = 0; j < KM; j++) {
+    bd[j] = INFINITY;
+    bi[j] = 0x7fffffff;
+  }
+  int found = 0;
+  const long long c0 = F->c0;
+  const int rmax = max(nx, max(ny, nz));
+  for (int r = 0; r <= rmax; r++) {
+    const int zlo = max(c[2] - r, 0), zhi = min(c[2] + r, nz - 1);
+    const int ylo = max(c[1] - r, 0), yhi = min(c[1] + r, ny - 1);
+    for (int z = zlo; z <= zhi; z++) {
+      const bool zface = (z == c[2] - r) || (z == c[2] + r);
+      for (int y = ylo; y <= yhi; y++) {
+        const bool yface = (y == c[1] - r) || (y == c[1] + r);
+        const long long row = c0 + ((long long)z * ny + y) * nx;
+        // x segments: full row on z/y faces, else the two end cells
+        int segs = 0;
+        int xa[2], xb[2];
+        if (zface || yface) {
+          xa[0] = max(c[0] - r, 0);
+          xb[0] = min(c[0] + r, nx - 1);
+          segs = 1;
+        } else {
+          if (c[0] - r >= 0) {
+            xa[segs] = xb[segs] = c[0] - r;
+            segs++;
+          }
+          if (r > 0 && c[0] + r < nx) {
+            xa[segs] = xb[segs] = c[0] + r;
+            segs++;
+          }
+        }
+        for (int sg = 0; sg < segs; sg++) {
+          const int pb = P.cell_start[row + xa[sg]], pe = P.cell_start[row + xb[sg] + 1];
+          for (int p = pb; p < pe; p++) {
+            const float d = knn_dist(q0, q1, q2, P.cg[p], P.cg[G + p], P.cg[2 * G + p], s0, s1, s2);
+            if (!(d <= 1e32f)) continue;
+            const int idx = P.cg_idx[p];
+            if (!lex_less(d, idx, bd[KM - 1], bi[KM - 1])) continue;
+            found++;
+            // insertion into the sorted list
+            float cd = d;
+            int ci2 = idx;
+#pragma unroll
+            for (int j = 0; j < KM; j++) {
+              if (lex_less(cd, ci2, bd[j], bi[j])) {
+                const float td = bd[j];
+                const int ti = bi[j];
+                bd[j] = cd;
+                bi[j] = ci2;
+                cd = td;
+                ci2 = ti;
+              }
+            }
+          }
+        }
+      }
+    }
+    // lower bound on the distance to any unvisited mean
+    double lb = INFINITY;
+    for (int a = 0; a < 3; a++) {
+      if (c[a] - r > 0) lb = fmin(lb, u[a] - (org[a] + (double)(c[a] - r) * h));
+      if (c[a] + r < dims[a] - 1) lb = fmin(lb, (org[a] + (double)(c[a] + r + 1) * h) - u[a]);
+    }
+    if (lb == INFINITY) break;
+    lb = fmax(lb, 0.0) * (1.0 - 1e-5);
+    float kth = INFINITY;
+#pragma unroll
+    for (int t = 0; t < KM; t++)
+      if (t == K - 1) kth = bd[t];
+    if (found >= K && (double)kth < lb * lb) break;
+  }
+  for (int j = 0; j < K; j++) {
+    int out = -1;
+#pragma unroll
+    for (int t = 0; t < KM; t++)
+      if (t == j) out = (bd[t] <= 1e32f) ? bi[t] : -1;
+    P.ind[(long long)j * S + s] = out;
+  }
+}
+
+// Responsibilities of the K candidates of each supervoxel. The reference's
+// per-voxel sum of -0.5(3/beta + nu d^T W d) + E[ln pi] + 0.5 E[ln det]
+// (GMEM.cu:2281-2298) equals, exactly,
+//   n kappa - 0.5 nu (tr(W Q) + n (c - m)^T W (c - m))
+// with n, c, Q the voxel count, mean and scatter of the supervoxel, so each
+// (s, k) score is one FP64 evaluation; normalisation is max-subtracted
+// log-sum-exp with FP32 exponentials.
+template <int KM>
+__global__ __launch_bounds__(256) void k_estep(DevPtrs P, KOpts O) {
+  const int f = P.sblk_frame[blockIdx.x];
+  if (f < 0 || !P.fr[f].active) return;
+  const long long s = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (P.sv_frame[s] != f) return;
+  const long long S = P.Scap, G = P.Gcap;
+  const int K = O.K;
+  const double n = P.mom[s];
+  const double c0 = P.mom[S + s], c1 = P.mom[2 * S + s], c2 = P.mom[3 * S + s];
+  double Q[6];
+#pragma unroll
+  for (int i = 0; i < 6; i++) Q[i] = P.mom[(4 + i) * S + s];
+  double L[KM];
+  int kk[KM];
+  double mx = -INFINITY;
+#pragma unroll
+  for (int j = 0; j < KM; j++) {
+    L[j] = -INFINITY;
+    kk[j] = -1;
+    if (j < K) {
+      const int k = P.ind[(long long)j * S + s];
+      kk[j] = k;
+      if (k >= 0) {
+        const double d0 = c0 - P.gs[GS_M * G + k], d1 = c1 - P.gs[(GS_M + 1) * G + k],
+                     d2 = c2 - P.gs[(GS_M + 2) * G + k];
+        double W[6];
+#pragma unroll
+        for (int i = 0; i < 6; i++) W[i] = P.gs[(GS_W + i) * G + k];
+        const double nu = P.gs[GS_NU * G + k];
+        const double qf = W[0] * d0 * d0 + W[3] * d1 * d1 + W[5] * d2 * d2 +
+                          2.0 * (W[1] * d0 * d1 + W[2] * d0 * d2 + W[4] * d1 * d2);
+        const double tq = W[0] * Q[0] + W[3] * Q[3] + W[5] * Q[5] +
+                          2.0 * (W[1] * Q[1] + W[2] * Q[2] + W[4] * Q[4]);
+        L[j] = n * P.kappa[k] - 0.5 * nu * (tq + n * qf);
+        mx = fmax(mx, L[j]);
+      }
+    }
+  }
+  float e[KM];
+  float den = 0.0f;
+#pragma unroll
+  for (int j = 0; j < KM; j++) {
+    e[j] = 0.0f;
+    if (j < K && kk[j] >= 0 && mx > -INFINITY) {
+      e[j] = expf((float)(L[j] - mx));
+      den += e[j];
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < KM; j++) {
+    if (j < K) {
+      const float r = (den > 0.0f || den != den) ? e[j] / den : 0.0f;
+      P.rnk[(long long)j * S + s] = r;
+      if (kk[j] >= 0 && r > 1e-3f) atomicAdd(P.seg_cnt + kk[j], 1);
+    }
+  }
+}
+
+__global__ __launch_bounds__(256) void k_pair_scatter(DevPtrs P, KOpts O) {
+  const int f = P.sblk_frame[blockIdx.x];
+  if (f < 0 || !P.fr[f].active) return;
+  const long long s = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (P.sv_frame[s] != f) return;
+  const long long S = P.Scap;
+  for (int j = 0; j < O.K; j++) {
+    const int k = P.ind[(long long)j * S + s];
+    if (k < 0) continue;
+    if (P.rnk[(long long)j * S + s] > 1e-3f) {
+      const int pos = P.seg_start[k] + atomicSub(P.seg_cnt + k, 1) - 1;
+      P.pairs[pos] = (int)((s << kPairShift) | j);
+    }
+  }
+}
+
+// 16 lanes per Gaussian: order its pair list by code (= by supervoxel) with a
+// rank sort, so that the FP64 accumulation order is fixed.
+__global__ __launch_bounds__(256) void k_seg_sort(DevPtrs P) {
+  const long long k = (long long)blockIdx.x * 16 + (threadIdx.x >> 4);
+  const int lane = threadIdx.x & 15;
+  const int f = P.gblk_frame[k >> 8];
+  if (f < 0 || !P.fr[f].active) return;
+  if (P.g_frame[k] != f) return;
+  const int st = P.seg_start[k], L = P.seg_start[k + 1] - st;
+  for (int i = lane; i < L; i += 16) {
+    const int e = P.pairs[st + i];
+    int rank = 0;
+    for (int j = 0; j < L; j++) rank += (P.pairs[st + j] < e) ? 1 : 0;
+    P.pairs_sorted[st + rank] = e;
+  }
+}
+
+// Segmented FP64 sufficient statistics over each Gaussian's significant
+// pairs, accumulated about the current mean m_k:
+//   N   = sum r W_s
+//   S1  = sum r W_s (mu_s - m_k)
+//   S2  = sum r [C_s + W_s (mu_s - m_k)(mu_s - m_k)^T]
+// so xbar = m_k + S1/N and sum w (x - xbar)(x - xbar)^T = S2 - S1 S1^T / N,
+// identical to the reference's per-voxel sums over r_sk * img_v
+// (GMEM.cu:1338-1453). Then the posterior (GMEM.cu:1601-1677) and the
+// regularisation (GMEM.cu:1693-1809) for alive, non-fixed Gaussians, and the
+// per-frame sum of alpha for the death test.
+__global__ __launch_bounds__(256) void k_mstep_update(DevPtrs P, KOpts O) {
+  __shared__ double red[4];
+  const int f = P.gblk_frame[blockIdx.x];
+  if (f < 0) return;
+  FrameDev* F = P.fr + f;
+  if (!F->active) return;
+  const long long k = (long long)blockIdx.x * 256 + threadIdx.x;
+  const long long G = P.Gcap, S = P.Scap;
+  double alpha_out = 0.0;
+  if (P.g_frame[k] == f) {
+    const bool alive = gsv(P, GS_M_O, k) > -1e31;
+    const bool fixed = P.fixed[k] != 0;
+    if (alive && !fixed) {
+      const double r0 = gsv(P, GS_M, k), r1 = gsv(P, GS_M + 1, k), r2 = gsv(P, GS_M + 2, k);
+      double N = 0.0, S1[3] = {0, 0, 0}, S2[6] = {0, 0, 0, 0, 0, 0};
+      const int st = P.seg_start[k], en = P.seg_start[k + 1];
+      for (int e = st; e < en; e++) {
+        const int code = P.pairs_sorted[e];
+        const long long s = code >> kPairShift;
+        const int j = code & ((1 << kPairShift) - 1);
+        const double r = (double)P.rnk[(long long)j * S + s];
+        const double Ws = P.wmom[s];
+        if (!(Ws != 0.0)) continue;
+        const double w = r * Ws;
+        const double d0 = P.wmom[S + s] - r0, d1 = P.wmom[2 * S + s] - r1,
+                     d2 = P.wmom[3 * S + s] - r2;
+        N += w;
+        S1[0] += w * d0;
+        S1[1] += w * d1;
+        S1[2] += w * d2;
+        S2[0] += r * P.wmom[4 * S + s] + w * d0 * d0;
+        S2[1] += r * P.wmom[5 * S + s] + w * d0 * d1;
+        S2[2] += r * P.wmom[6 * S + s] + w * d0 * d2;
+        S2[3] += r * P.wmom[7 * S + s] + w * d1 * d1;
+        S2[4] += r * P.wmom[8 * S + s] + w * d1 * d2;
+        S2[5] += r * P.wmom[9 * S + s] + w * d2 * d2;
+      }
+      double xk[3] = {0, 0, 0}, Ck[6] = {0, 0, 0, 0, 0, 0};
+      if (N > 0.0) {
+        const double m0 = S1[0] / N, m1 = S1[1] / N, m2 = S1[2] / N;
+        xk[0] = r0 + m0;
+        xk[1] = r1 + m1;
+        xk[2] = r2 + m2;
+        Ck[0] = S2[0] - S1[0] * m0;
+        Ck[1] = S2[1] - S1[0] * m1;
+        Ck[2] = S2[2] - S1[0] * m2;
+        Ck[3] = S2[3] - S1[1] * m1;
+        Ck[4] = S2[4] - S1[1] * m2;
+        Ck[5] = S2[5] - S1[2] * m2;
+      }
+      // posterior (FP64 throughout)
+      const double beta_o = gsv(P, GS_BETA_O, k), nu_o = gsv(P, GS_NU_O, k),
+                   alpha_o = gsv(P, GS_ALPHA_O, k);
+      double mo[3], Wo[6];
+      for (int i = 0; i < 3; i++) mo[i] = gsv(P, GS_M_O + i, k);
+      for (int i = 0; i < 6; i++) Wo[i] = gsv(P, GS_W_O + i, k);
+      const double beta = beta_o + N, nu = nu_o + N, alpha = fmax(alpha_o + N, 0.0);
+      double m[3];
+      for (int i = 0; i < 3; i++) m[i] = (beta_o * mo[i] + N * xk[i]) / beta;
+      if (Wo[5] < 1e-8) {
+        Wo[5] = 0.5 * (Wo[0] + Wo[3]);
+        P.gs[(GS_W_O + 5) * G + k] = Wo[5];
+      }
+      double A[6], W[6];
+      inv3(Wo, A);
+      const double dx[3] = {xk[0] - mo[0], xk[1] - mo[1], xk[2] - mo[2]};
+      const double fct = beta_o / (beta_o + N);
+      {
+        int c = 0;
+        for (int a = 0; a < 3; a++)
+          for (int b = a; b < 3; b++) {
+            A[c] += Ck[c] + N * fct * dx[a] * dx[b];
+            c++;
+          }
+      }
+      inv3(A, W);
+      regularize_W(W, nu, O.w4dof, O.scale_reg, O.lmin, O.lmax, O.emax);
+      P.gs[GS_BETA * G + k] = beta;
+      P.gs[GS_NU * G + k] = nu;
+      P.gs[GS_ALPHA * G + k] = alpha;
+      for (int i = 0; i < 3; i++) P.gs[(GS_M + i) * G + k] = m[i];
+      for (int i = 0; i < 6; i++) P.gs[(GS_W + i) * G + k] = W[i];
+      alpha_out = alpha;
+    } else {
+      alpha_out = gsv(P, GS_ALPHA, k);
+    }
+  }
+  int last;
+  const double tot = frame_reduce(alpha_out, P.part_g, blockIdx.x, F->g0 / 256, F->nbg,
+                                  P.done + 4 * f + 1, red, &last);
+  if (last && threadIdx.x == 0) F->sum_alpha2 = tot;
+}
+
+// Dead-cell rule (GMEM.cu:1813-1843), then the log-likelihood parameters and
+// the per-frame sum of alpha used by the log-likelihood and by the next
+// iteration's E[ln pi].
+__global__ __launch_bounds__(256) void k_death(DevPtrs P) {
+  __shared__ double red[4];
+  const int f = P.gblk_frame[blockIdx.x];
+  if (f < 0) return;
+  FrameDev* F = P.fr + f;
+  if (!F->active) return;
+  const long long k = (long long)blockIdx.x * 256 + threadIdx.x;
+  const long long G = P.Gcap;
+  double alpha_out = 0.0;
+  if (P.g_frame[k] == f) {
+    double W[6];
+    bool nanW = isnan(gsv(P, GS_ELD, k));
+    for (int i = 0; i < 6; i++) {
+      W[i] = gsv(P, GS_W + i, k);
+      nanW |= isnan(W[i]);
+    }
+    double alpha = gsv(P, GS_ALPHA, k);
+    bool alive = gsv(P, GS_M_O, k) > -1e31;
+    const bool fixed = P.fixed[k] != 0;
+    if (nanW || (alpha / F->sum_alpha2 < 1e-5 && alive && !fixed)) {
+      alpha = 0.0;
+      P.gs[GS_ALPHA * G + k] = 0.0;
+      for (int i = 0; i < 3; i++) {
+        P.gs[(GS_M_O + i) * G + k] = -1e32;
+        P.gs[(GS_M + i) * G + k] = -1e32;
+      }
+      const double I6[6] = {1, 0, 0, 1, 0, 1};
+      for (int i = 0; i < 6; i++) {
+        W[i] = I6[i];
+        P.gs[(GS_W + i) * G + k] = I6[i];
+      }
+      alive = false;
+    }
+    const double nu = gsv(P, GS_NU, k);
+    for (int i = 0; i < 6; i++) P.nuW[i * G + k] = (float)(nu * W[i]);
+    const double dt = det3(W) * nu * nu * nu;
+    P.lg[k] = (alive && alpha > 0.0 && dt > 0.0)
+                  ? log(alpha) + 0.5 * log(dt) - 1.5 * 1.8378770664093454836
+                  : -INFINITY;
+    alpha_out = alpha;
+  }
+  int last;
+  const double tot = frame_reduce(alpha_out, P.part_g, blockIdx.x, F->g0 / 256, F->nbg,
+                                  P.done + 4 * f + 2, red, &last);
+  if (last && threadIdx.x == 0) {
+    F->sum_alpha = tot;
+    F->psi_sum_alpha = digamma(tot);
+    F->log_sum_alpha = log(tot);
+  }
+}
+
+// Log-likelihood over the voxel stream (GMEM.cu:1913-1978): one wave per
+// supervoxel, its K candidates' parameters broadcast from lanes 0..K-1.
+// Per voxel, d = (x - a_s) + (a_s - m_k) with the anchor a_s a voxel of the
+// supervoxel (exact in f32) and (a_s - m_k) rounded once from FP64, so the
+// f32 quadratic form keeps full relative precision far from the origin;
+// log sum exp in f32, img-weighted accumulation in FP64. A voxel whose
+// largest term is below the FP64 underflow of the reference's direct sum
+// (e^-744.44) contributes nothing, as in the reference.
+template <int KM>
+__global__ __launch_bounds__(256) void k_loglik(DevPtrs P, KOpts O) {
+  __shared__ double red[4];
+  const long long sb = (long long)blockIdx.x * kLLsv;
+  const int f = P.sblk_frame[sb >> 8];
+  if (f < 0) return;
+  FrameDev* F = P.fr + f;
+  if (!F->active) return;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const long long S = P.Scap, G = P.Gcap;
+  const int K = O.K;
+  const double log_sa = F->log_sum_alpha;
+  double acc = 0.0;
+  for (int t = wv; t < kLLsv; t += 4) {
+    const long long s = sb + t;
+    if (P.sv_frame[s] != f) continue;
+    // lane j < K loads candidate j
+    float pd0 = 0.f, pd1 = 0.f, pd2 = 0.f, pw[6] = {0, 0, 0, 0, 0, 0}, pg = -INFINITY;
+    const float a0 = P.anchor[s], a1 = P.anchor[S + s], a2 = P.anchor[2 * S + s];
+    if (lane < K) {
+      const int k = P.ind[(long long)lane * S + s];
+      if (k >= 0) {
+        pd0 = (float)((double)a0 - P.gs[GS_M * G + k]);
+        pd1 = (float)((double)a1 - P.gs[(GS_M + 1) * G + k]);
+        pd2 = (float)((double)a2 - P.gs[(GS_M + 2) * G + k]);
+        for (int i = 0; i < 6; i++) pw[i] = P.nuW[i * G + k];
+        pg = (float)(P.lg[k] - log_sa);
+      }
+    }
+    float cd0[KM], cd1[KM], cd2[KM], cw[KM][6], cg[KM];
+#pragma unroll
+    for (int j = 0; j < KM; j++) {
+      cd0[j] = __shfl(pd0, j, 64);
+      cd1[j] = __shfl(pd1, j, 64);
+      cd2[j] = __shfl(pd2, j, 64);
+#pragma unroll
+      for (int i = 0; i < 6; i++) cw[j][i] = __shfl(pw[i], j, 64);
+      cg[j] = (j < K) ? __shfl(pg, j, 64) : -INFINITY;
+    }
+    const long long beg = P.sv_beg[s];
+    const int n = P.sv_len[s];
+    for (int i = lane; i < n; i += 64) {
+      const float x0 = P.vx[beg + i] - a0, x1 = P.vy[beg + i] - a1, x2 = P.vz[beg + i] - a2;
+      const float w = P.vimg[beg + i];
+      float tj[KM];
+      float mx = -INFINITY;
+#pragma unroll
+      for (int j = 0; j < KM; j++) {
+        const float d0 = x0 + cd0[j], d1 = x1 + cd1[j], d2 = x2 + cd2[j];
+        const float q = cw[j][0] * d0 * d0 + cw[j][3] * d1 * d1 + cw[j][5] * d2 * d2 +
+                        2.0f * (cw[j][1] * d0 * d1 + cw[j][2] * d0 * d2 + cw[j][4] * d1 * d2);
+        tj[j] = cg[j] - 0.5f * q;
+        mx = fmaxf(mx, tj[j]);
+      }
+      if (mx > -744.44f) {
+        float sum = 0.0f;
+#pragma unroll
+        for (int j = 0; j < KM; j++) sum += expf(tj[j] - mx);
+        acc += (double)w * (double)(mx + logf(sum));
+      }
+    }
+  }
+  int last;
+  const double tot = frame_reduce(acc, P.part_s, blockIdx.x, F->s0 / kLLsv, F->nbl,
+                                  P.done + 4 * f + 3, red, &last);
+  if (last && threadIdx.x == 0) {
+    F->ll3 = F->ll2;
+    F->ll2 = F->ll1;
+    F->ll1 = F->ll;
+    F->ll = tot;
+    F->iter += 1;
+    F->active = (keep_going(F->ll, F->ll1, F->ll2, F->ll3, F->tol) && F->iter <= F->max_iter) ? 1 : 0;
+  }
+}
+
+// ---------------------------------------------------------------- launchers --
+int kmax_for(int K) { return K <= 5 ? 5 : (K <= 8 ? 8 : 16); }
+
+static inline unsigned nb(long long n, int per) { return (unsigned)((n + per - 1) / per); }
+
+static hipError_t scan(const int* in, long long n1, int* out, int* blk, hipStream_t st) {
+  const unsigned b = nb(n1, kScanTile);
+  k_scan_partial<<<b, 256, 0, st>>>(in, n1, blk);
+  k_scan_final<<<b, 256, 0, st>>>(in, n1, blk, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_prep(const DevPtrs& P, hipStream_t st) {
+  k_moments<<<nb(P.Scap, 4), 256, 0, st>>>(P);
+  return hipGetLastError();
+}
+
+hipError_t launch_reset(const DevPtrs& P, const KOpts& O, hipStream_t st) {
+  k_grid_setup<<<P.nframes, 256, 0, st>>>(P, O);
+  k_alpha_init<<<nb(P.Gcap, 256), 256, 0, st>>>(P);
+  return hipGetLastError();
+}
+
+template <int KM>
+static hipError_t iteration_t(const DevPtrs& P, const KOpts& O, hipStream_t st, hipEvent_t e0,
+                              hipEvent_t e1, int tk) {
+  const unsigned gb = nb(P.Gcap, 256), sb = nb(P.Scap, 256);
+  k_gauss_prep<<<gb, 256, 0, st>>>(P, O);
+  (void)scan(P.cell_cnt, P.Ccap + 1, P.cell_start, P.scan_blk, st);
+  k_grid_scatter<<<gb, 256, 0, st>>>(P);
+  if (tk == TK_KNN && e0) (void)hipEventRecord(e0, st);
+  k_knn<KM><<<sb, 256, 0, st>>>(P, O);
+  if (tk == TK_KNN && e1) (void)hipEventRecord(e1, st);
+  if (tk == TK_ESTEP && e0) (void)hipEventRecord(e0, st);
+  k_estep<KM><<<sb, 256, 0, st>>>(P, O);
+  if (tk == TK_ESTEP && e1) (void)hipEventRecord(e1, st);
+  (void)scan(P.seg_cnt, P.Gcap + 1, P.seg_start, P.scan_blk, st);
+  k_pair_scatter<<<sb, 256, 0, st>>>(P, O);
+  k_seg_sort<<<nb(P.Gcap, 16), 256, 0, st>>>(P);
+  if (tk == TK_MSTEP && e0) (void)hipEventRecord(e0, st);
+  k_mstep_update<<<gb, 256, 0, st>>>(P, O);
+  if (tk == TK_MSTEP && e1) (void)hipEventRecord(e1, st);
+  k_death<<<gb, 256, 0, st>>>(P);
+  if (tk == TK_LOGLIK && e0) (void)hipEventRecord(e0, st);
+  k_loglik<KM><<<nb(P.Scap, kLLsv), 256, 0, st>>>(P, O);
+  if (tk == TK_LOGLIK && e1) (void)hipEventRecord(e1, st);
+  return hipGetLastError();
+}
+
+hipError_t launch_iteration(const DevPtrs& P, const KOpts& O, hipStream_t st, hipEvent_t e0,
+                            hipEvent_t e1, int tk) {
+  switch (kmax_for(O.K)) {
+    case 5:
+      return iteration_t<5>(P, O, st, e0, e1, tk);
+    case 8:
+      return iteration_t<8>(P, O, st, e0, e1, tk);
+    default:
+      return iteration_t<16>(P, O, st, e0, e1, tk);
+  }
+}
+
+}  // namespace tgmm

@@ -1,0 +1,231 @@
+// tgmm_legacy.hip -- the reference host API (GMEMupdateCUDA.h) as C-ABI
+// entry points with status codes. This is the plain per-frame form: each VI
+// call builds a one-frame workspace, runs one EM iteration per host round
+// trip with the convergence test read back on the host (as the reference's
+// loop does, GMEM.cu:4145-4264), and releases the workspace.
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <vector>
+
+#include "../../include/tgmm_em.h"
+#include "tgmm_runtime_internal.h"
+
+using namespace tgmm;
+
+#define HIPCHK(x)                                                        \
+  do {                                                                   \
+    hipError_t e_ = (x);                                                 \
+    if (e_ != hipSuccess) {                                              \
+      tgmm_set_error("%s failed: %s", #x, hipGetErrorString(e_));        \
+      return e_ == hipErrorOutOfMemory ? TGMM_ERR_OOM : TGMM_ERR_DEVICE; \
+    }                                                                    \
+  } while (0)
+
+namespace {
+// Host statics of the reference: kNN scale (uploadScaleCUDA), regularisation
+// scale (copyScaleToDvice) and regularizePrecisionMatrixConstants.
+std::mutex g_mu;
+float g_scale_knn[3] = {1.f, 1.f, 1.f};
+float g_scale_reg[3] = {1.f, 1.f, 1.f};
+double g_lmin = -1.0, g_lmax = -1.0, g_emax = -1.0;
+}  // namespace
+
+extern "C" int tgmm_setRegularizePrecisionMatrixConstants(double lambdaMin, double lambdaMax,
+                                                          double maxExcentricity) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  g_lmin = lambdaMin;
+  g_lmax = lambdaMax;
+  g_emax = maxExcentricity;
+  return TGMM_OK;
+}
+
+extern "C" int tgmm_copyScaleToDvice(const float* scale) {
+  if (!scale) {
+    tgmm_set_error("copyScaleToDvice: scale is NULL");
+    return TGMM_ERR_ARG;
+  }
+  std::lock_guard<std::mutex> lk(g_mu);
+  for (int a = 0; a < 3; a++) g_scale_reg[a] = scale[a];
+  return TGMM_OK;
+}
+
+extern "C" int tgmm_GMEMinitializeMemoryWithSupervoxels(
+    float** queryCUDA, const float* query, float** imgDataCUDA, const float* imgData,
+    const float* scale, long long query_nb, float** rnkCUDA, int** indCUDA, int** indCUDAtr,
+    unsigned short numLabels, float** centroidLabelPositionCUDA,
+    const float* centroidLabelPosition, long long** labelListPtrCUDA,
+    const long long* labelListPtrHOST) {
+  if (!queryCUDA || !imgDataCUDA || !rnkCUDA || !indCUDA || !indCUDAtr ||
+      !centroidLabelPositionCUDA || !labelListPtrCUDA || !scale || query_nb < 0 ||
+      (query_nb > 0 && (!query || !imgData)) || !labelListPtrHOST ||
+      (numLabels > 0 && !centroidLabelPosition)) {
+    tgmm_set_error("GMEMinitializeMemoryWithSupervoxels: NULL argument or negative size");
+    return TGMM_ERR_ARG;
+  }
+  const long long N = query_nb, S = numLabels, K = TGMM_LEGACY_K;
+  auto M = [](void** p, size_t n) { return hipMalloc(p, n > 0 ? n : 4); };
+  HIPCHK(M((void**)queryCUDA, 3 * N * sizeof(float)));
+  HIPCHK(M((void**)imgDataCUDA, N * sizeof(float)));
+  HIPCHK(M((void**)rnkCUDA, K * S * sizeof(float)));
+  HIPCHK(M((void**)indCUDA, K * S * sizeof(int)));
+  HIPCHK(M((void**)indCUDAtr, K * S * sizeof(int)));
+  HIPCHK(M((void**)centroidLabelPositionCUDA, 3 * S * sizeof(float)));
+  HIPCHK(M((void**)labelListPtrCUDA, (S + 1) * sizeof(long long)));
+  if (N > 0) {
+    HIPCHK(hipMemcpy(*queryCUDA, query, 3 * N * sizeof(float), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(*imgDataCUDA, imgData, N * sizeof(float), hipMemcpyHostToDevice));
+  }
+  if (S > 0)
+    HIPCHK(hipMemcpy(*centroidLabelPositionCUDA, centroidLabelPosition, 3 * S * sizeof(float),
+                     hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(*labelListPtrCUDA, labelListPtrHOST, (S + 1) * sizeof(long long),
+                   hipMemcpyHostToDevice));
+  std::lock_guard<std::mutex> lk(g_mu);
+  for (int a = 0; a < 3; a++) g_scale_knn[a] = scale[a];
+  return TGMM_OK;
+}
+
+extern "C" int tgmm_GMEMreleaseMemoryWithSupervoxels(float** queryCUDA, float** imgDataCUDA,
+                                                     float** rnkCUDA, int** indCUDA,
+                                                     int** indCUDAtr,
+                                                     float** centroidLabelPositionCUDA,
+                                                     long long** labelListPtrCUDA) {
+  void** ps[7] = {(void**)queryCUDA, (void**)imgDataCUDA, (void**)rnkCUDA, (void**)indCUDA,
+                  (void**)indCUDAtr, (void**)centroidLabelPositionCUDA, (void**)labelListPtrCUDA};
+  int rc = TGMM_OK;
+  for (void** p : ps) {
+    if (!p) {
+      rc = TGMM_ERR_ARG;
+      continue;
+    }
+    if (*p) {
+      hipError_t e = hipFree(*p);
+      if (e != hipSuccess) {
+        tgmm_set_error("hipFree failed: %s", hipGetErrorString(e));
+        rc = TGMM_ERR_DEVICE;
+      }
+    }
+    *p = nullptr;
+  }
+  return rc;
+}
+
+extern "C" int tgmm_GMEMvariationalInferenceCUDAWithSupervoxels(
+    float* queryCUDA, float* imgDataCUDA, float* rnkCUDA, float* rnkCUDAtr, int* indCUDA,
+    int* indCUDAtr, float* centroidLabelPositionCUDA, long long* labelListPtrCUDA,
+    GaussianMixtureModelCUDA* vecGMtemp, long long query_nb, int ref_nb,
+    unsigned short numLabels, int maxIterEM, double tolLikelihood, int devCUDA, int frame,
+    int W4DOF, const char* debugPath) {
+  (void)rnkCUDAtr;
+  // limits of the reference (GMEM.cu:3992-4007)
+  if (query_nb > 1024LL * 65535LL) {
+    tgmm_set_error("Too many query points for the GPU resources (%lld)", query_nb);
+    return TGMM_ERR_LIMIT;
+  }
+  if (ref_nb > 65535) {
+    tgmm_set_error("Too many reference points for the GPU resources (%d)", ref_nb);
+    return TGMM_ERR_LIMIT;
+  }
+  if (query_nb < 0 || ref_nb < 0 || !labelListPtrCUDA || (ref_nb > 0 && !vecGMtemp) ||
+      (numLabels > 0 && (!rnkCUDA || !indCUDA || !centroidLabelPositionCUDA)) ||
+      (query_nb > 0 && (!queryCUDA || !imgDataCUDA))) {
+    tgmm_set_error("GMEMvariationalInferenceCUDAWithSupervoxels: NULL argument");
+    return TGMM_ERR_ARG;
+  }
+  tgmm_em_opts o;
+  float sreg[3];
+  {
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (g_lmin < 0) {
+      tgmm_set_error(
+          "before GMEMregularizeWkKernel: regularizePrecisionMatrixConstants were not set");
+      return TGMM_ERR_NOT_SET;
+    }
+    std::memset(&o, 0, sizeof(o));
+    o.K = TGMM_LEGACY_K;
+    o.max_iter = maxIterEM;
+    o.tol = tolLikelihood;
+    o.lambda_min = g_lmin;
+    o.lambda_max = g_lmax;
+    o.max_excentricity = g_emax;
+    o.w4dof = W4DOF ? 1 : 0;
+    for (int a = 0; a < 3; a++) {
+      o.scale[a] = g_scale_knn[a];
+      sreg[a] = g_scale_reg[a];
+    }
+    o.max_sv_per_gaussian = TGMM_MAX_SUPERVOXELS_PER_GAUSSIAN;
+  }
+  HIPCHK(hipSetDevice(devCUDA));
+  const long long S = numLabels;
+  std::vector<long long> ptr(S + 1);
+  HIPCHK(hipMemcpy(ptr.data(), labelListPtrCUDA, (S + 1) * sizeof(long long),
+                   hipMemcpyDeviceToHost));
+  FrameSrc f;
+  f.N = query_nb;
+  f.S = S;
+  f.G = ref_nb;
+  f.query = queryCUDA;
+  f.img = imgDataCUDA;
+  f.cent = centroidLabelPositionCUDA;
+  f.ptr = ptr.data();
+  f.gm = vecGMtemp;
+  tgmm_batch* b = nullptr;
+  int rc = batch_build(devCUDA, std::vector<FrameSrc>{f}, o, sreg, nullptr, &b);
+  if (rc) return rc;
+  rc = tgmm_batch_solve(b, TGMM_SOLVE_HOSTSYNC);
+  std::vector<double> gs;
+  std::vector<float> rnk;
+  std::vector<int> ind;
+  if (!rc) rc = batch_fetch_host(b, gs, rnk, ind);
+  double ll = 0.0;
+  int iters = 0;
+  if (!rc) {
+    frame_results(b, 0, gs, rnk, ind, vecGMtemp, nullptr, nullptr, &ll, &iters);
+    const long long Sc = b->ws.sh.Scap;
+    const size_t w = (size_t)S * sizeof(float);
+    if (S > 0) {
+      hipError_t e = hipMemcpy2D(rnkCUDA, w, b->ws.P.rnk, Sc * sizeof(float), w, TGMM_LEGACY_K,
+                                 hipMemcpyDeviceToDevice);
+      if (e == hipSuccess)
+        e = hipMemcpy2D(indCUDA, w, b->ws.P.ind, Sc * sizeof(int), w, TGMM_LEGACY_K,
+                        hipMemcpyDeviceToDevice);
+      if (e == hipSuccess && indCUDAtr)
+        e = hipMemcpy(indCUDAtr, indCUDA, TGMM_LEGACY_K * w, hipMemcpyDeviceToDevice);
+      if (e != hipSuccess) {
+        tgmm_set_error("copy of rnk/ind to caller buffers failed: %s", hipGetErrorString(e));
+        rc = TGMM_ERR_DEVICE;
+      }
+    }
+  }
+  if (!rc && getenv("TGMM_VERBOSE"))
+    printf("Frame=%d;iter=%d;Log-likelihood =%16.12f\n", frame, iters + 1, ll);
+  if (!rc && debugPath && debugPath[0]) {
+    // {int S; int K; float rnk[S*K]; int ind[S*K]} (GMEM.cu:4364-4384)
+    FILE* fid = fopen(debugPath, "wb");
+    if (!fid) {
+      tgmm_set_error("opening file %s to save rnk values from variational inference", debugPath);
+      rc = TGMM_ERR_IO;
+    } else {
+      const int Si = (int)S, Ki = TGMM_LEGACY_K;
+      const long long Sc = b->ws.sh.Scap;
+      std::vector<float> r2((size_t)S * Ki);
+      std::vector<int> i2((size_t)S * Ki);
+      for (int j = 0; j < Ki; j++)
+        for (long long s = 0; s < S; s++) {
+          r2[j * S + s] = rnk[j * Sc + s];
+          i2[j * S + s] = ind[j * Sc + s];
+        }
+      fwrite(&Si, sizeof(int), 1, fid);
+      fwrite(&Ki, sizeof(int), 1, fid);
+      fwrite(r2.data(), sizeof(float), r2.size(), fid);
+      fwrite(i2.data(), sizeof(int), i2.size(), fid);
+      fclose(fid);
+    }
+  }
+  tgmm_batch_destroy(b);
+  return rc;
+}

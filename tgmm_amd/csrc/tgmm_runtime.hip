@@ -1,0 +1,589 @@
+// tgmm_runtime.hip -- host runtime of libtgmm_em.so: device workspaces for
+// batches of frames, hipGraph-replayed EM iterations, the drop-in entry
+// points of the reference host API, and the C-ABI error convention.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/tgmm_em.h"
+#include "tgmm_internal.h"
+#include "tgmm_runtime_internal.h"
+
+using namespace tgmm;
+
+// ------------------------------------------------------------ error state --
+static thread_local std::string g_err = "";
+
+void tgmm_set_error(const char* fmt, ...) {
+  char buf[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  if (getenv("TGMM_VERBOSE")) fprintf(stderr, "tgmm: %s\n", buf);
+}
+
+extern "C" const char* tgmm_last_error(void) { return g_err.c_str(); }
+
+#define HIPCHK(x)                                                            \
+  do {                                                                       \
+    hipError_t e_ = (x);                                                     \
+    if (e_ != hipSuccess) {                                                  \
+      tgmm_set_error("%s failed: %s", #x, hipGetErrorString(e_));            \
+      return e_ == hipErrorOutOfMemory ? TGMM_ERR_OOM : TGMM_ERR_DEVICE;     \
+    }                                                                        \
+  } while (0)
+
+extern "C" int tgmm_device_count(int* count) {
+  if (!count) return TGMM_ERR_ARG;
+  int n = 0;
+  hipError_t e = hipGetDeviceCount(&n);
+  *count = (e == hipSuccess) ? n : 0;
+  return TGMM_OK;
+}
+
+// -------------------------------------------------------------- workspace --
+static long long round_up(long long x, long long a) { return ((x + a - 1) / a) * a; }
+
+int ws_alloc(Workspace* w, const WsShape& sh, int K) {
+  w->sh = sh;
+  w->K = K;
+  DevPtrs& P = w->P;
+  std::memset(&P, 0, sizeof(P));
+  P.Scap = sh.Scap;
+  P.Gcap = sh.Gcap;
+  P.Ccap = sh.Ccap;
+  P.Vcap = sh.Vcap;
+  P.Fcap = sh.Fcap;
+  auto A = [&](void** p, size_t bytes) -> int {
+    bytes = std::max<size_t>(bytes, 256);
+    hipError_t e = hipMalloc(p, bytes);
+    if (e != hipSuccess) {
+      tgmm_set_error("hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
+      return TGMM_ERR_OOM;
+    }
+    w->allocs.push_back(*p);
+    w->bytes += bytes;
+    return TGMM_OK;
+  };
+#define AL(ptr, n) \
+  if (int rc_ = A((void**)&(ptr), (size_t)(n) * sizeof(*(ptr)))) return rc_
+  const long long S = sh.Scap, G = sh.Gcap, C = sh.Ccap, V = sh.Vcap, F = sh.Fcap;
+  AL(P.vx, V);
+  AL(P.vy, V);
+  AL(P.vz, V);
+  AL(P.vimg, V);
+  AL(P.sv_frame, S);
+  AL(P.sv_beg, S);
+  AL(P.sv_len, S);
+  AL(P.sv_cent, 3 * S);
+  AL(P.mom, 10 * S);
+  AL(P.wmom, 10 * S);
+  AL(P.anchor, 3 * S);
+  AL(P.ind, (long long)K * S);
+  AL(P.rnk, (long long)K * S);
+  AL(P.sblk_frame, S / kAlign);
+  AL(P.g_frame, G);
+  AL(P.gs, GS_NFIELDS * G);
+  AL(P.fixed, G);
+  AL(P.kappa, G);
+  AL(P.nuW, 6 * G);
+  AL(P.lg, G);
+  AL(P.mf, 3 * G);
+  AL(P.gcell, G);
+  AL(P.gblk_frame, G / kAlign);
+  AL(P.seg_cnt, G + 1);
+  AL(P.seg_start, G + 1);
+  AL(P.pairs, (long long)K * S);
+  AL(P.pairs_sorted, (long long)K * S);
+  AL(P.cell_cnt, C + 1);
+  AL(P.cell_start, C + 1);
+  AL(P.cg, 3 * G);
+  AL(P.cg_idx, G);
+  AL(P.part_g, G / kAlign);
+  AL(P.part_s, S / kLLsv);
+  AL(P.done, 4 * F);
+  AL(P.scan_blk, (std::max(C, G) + 1 + kScanTile - 1) / kScanTile + 1);
+  AL(P.fr, F);
+  AL(w->gs_init, GS_NFIELDS * G);
+#undef AL
+  return TGMM_OK;
+}
+
+void ws_free(Workspace* w) {
+  for (void* p : w->allocs) (void)hipFree(p);
+  w->allocs.clear();
+  w->bytes = 0;
+}
+
+long long ws_bytes_estimate(const WsShape& sh, int K) {
+  const long long S = sh.Scap, G = sh.Gcap, C = sh.Ccap, V = sh.Vcap;
+  return 16 * V + S * (4 + 8 + 4 + 12 + 80 + 80 + 12 + 16LL * K + 1) +
+         G * (4 + 8 * GS_NFIELDS * 2 + 1 + 8 + 24 + 8 + 12 + 4 + 8 + 12 + 4 + 1) + 8 * C + 64 * sh.Fcap;
+}
+
+// ------------------------------------------------------------------ batch --
+static int validate_opts(const tgmm_em_opts* o) {
+  if (!o) {
+    tgmm_set_error("opts is NULL");
+    return TGMM_ERR_ARG;
+  }
+  if (o->K < 1 || o->K > TGMM_KMAX) {
+    tgmm_set_error("K=%d outside [1, %d]", o->K, TGMM_KMAX);
+    return TGMM_ERR_ARG;
+  }
+  if (o->max_iter < 0) {
+    tgmm_set_error("max_iter < 0");
+    return TGMM_ERR_ARG;
+  }
+  if (o->lambda_min < 0 || o->lambda_max < 0 || o->max_excentricity < 0) {
+    tgmm_set_error("regularisation constants were not set (negative)");
+    return TGMM_ERR_NOT_SET;
+  }
+  for (int a = 0; a < 3; a++)
+    if (!(o->scale[a] > 1e-3f)) {
+      tgmm_set_error("scale[%d]=%g must be > 1e-3", a, (double)o->scale[a]);
+      return TGMM_ERR_ARG;
+    }
+  if (o->max_sv_per_gaussian < 0 || o->max_sv_per_gaussian > TGMM_MAX_SUPERVOXELS_PER_GAUSSIAN) {
+    tgmm_set_error("max_sv_per_gaussian outside [0, 16]");
+    return TGMM_ERR_ARG;
+  }
+  return TGMM_OK;
+}
+
+KOpts make_kopts(const tgmm_em_opts& o, const float* scale_reg) {
+  KOpts k;
+  std::memset(&k, 0, sizeof(k));
+  k.K = o.K;
+  k.w4dof = o.w4dof ? 1 : 0;
+  for (int a = 0; a < 3; a++) {
+    k.scale[a] = o.scale[a];
+    k.scale_reg[a] = scale_reg ? scale_reg[a] : o.scale[a];
+  }
+  k.lmin = o.lambda_min;
+  k.lmax = o.lambda_max;
+  k.emax = o.max_excentricity;
+  return k;
+}
+
+// records (AoS, host) -> 27 x G SoA columns at [g0, g0+G) of a Gcap table
+void records_to_soa(const GaussianMixtureModelCUDA* gm, long long G, long long g0, long long Gcap,
+                    double* gs, unsigned char* fixed) {
+  for (long long k = 0; k < G; k++) {
+    const double* d = (const double*)&gm[k];
+    for (int fidx = 0; fidx < GS_NFIELDS; fidx++) gs[fidx * Gcap + g0 + k] = d[fidx];
+    fixed[g0 + k] = gm[k].fixed ? 1 : 0;
+  }
+}
+
+void soa_to_records(const double* gs, long long G, long long g0, long long Gcap,
+                    GaussianMixtureModelCUDA* gm) {
+  for (long long k = 0; k < G; k++) {
+    double* d = (double*)&gm[k];
+    for (int fidx = 0; fidx < GS_NFIELDS; fidx++) d[fidx] = gs[fidx * Gcap + g0 + k];
+  }
+}
+
+int batch_build(int device, const std::vector<FrameSrc>& fs, const tgmm_em_opts& o,
+                const float* scale_reg, hipStream_t stream, tgmm_batch** out) {
+  if (int rc = validate_opts(&o)) return rc;
+  HIPCHK(hipSetDevice(device));
+  tgmm_batch* b = new tgmm_batch();
+  b->device = device;
+  b->opts = o;
+  b->ko = make_kopts(o, scale_reg);
+  b->nframes = (int)fs.size();
+  // placement
+  WsShape sh;
+  long long v = 0, s = 0, g = 0, c = 0;
+  b->fh.resize(fs.size());
+  for (size_t i = 0; i < fs.size(); i++) {
+    const FrameSrc& f = fs[i];
+    if (f.N < 0 || f.S < 0 || f.G < 0) {
+      delete b;
+      tgmm_set_error("frame %zu: negative size", i);
+      return TGMM_ERR_ARG;
+    }
+    FrameDev& F = b->fh[i];
+    std::memset(&F, 0, sizeof(F));
+    F.v0 = v;
+    F.nv = f.N;
+    F.s0 = s;
+    F.ns = f.S;
+    F.g0 = g;
+    F.ng = f.G;
+    F.c0 = c;
+    F.ncap = std::max<long long>(8, 2LL * f.G);
+    F.nbg = (int)((f.G + 255) / 256);
+    F.nbl = (int)((f.S + kLLsv - 1) / kLLsv);
+    F.max_iter = o.max_iter;
+    F.tol = o.tol;
+    v += f.N;
+    s += round_up(std::max<long long>(f.S, 1), kAlign);
+    g += round_up(std::max<long long>(f.G, 1), kAlign);
+    c += F.ncap;
+  }
+  sh.Vcap = std::max<long long>(v, 1);
+  sh.Scap = std::max<long long>(s, kAlign);
+  sh.Gcap = std::max<long long>(g, kAlign);
+  sh.Ccap = std::max<long long>(c, 8);
+  sh.Fcap = std::max<int>((int)fs.size(), 1);
+  if ((sh.Scap << kPairShift) >= (1LL << 31) || (long long)o.K * sh.Scap >= (1LL << 31) ||
+      sh.Gcap >= (1LL << 31) || sh.Ccap >= (1LL << 31)) {
+    delete b;
+    tgmm_set_error("workspace exceeds 32-bit index limits (S=%lld, G=%lld)", sh.Scap, sh.Gcap);
+    return TGMM_ERR_LIMIT;
+  }
+  if (int rc = ws_alloc(&b->ws, sh, o.K)) {
+    ws_free(&b->ws);
+    delete b;
+    return rc;
+  }
+  DevPtrs& P = b->ws.P;
+  P.nframes = (int)fs.size();
+  if (stream) {
+    b->st = stream;
+    b->own_stream = false;
+  } else {
+    if (hipStreamCreateWithFlags(&b->st, hipStreamNonBlocking) != hipSuccess) {
+      ws_free(&b->ws);
+      delete b;
+      tgmm_set_error("hipStreamCreate failed");
+      return TGMM_ERR_DEVICE;
+    }
+    b->own_stream = true;
+  }
+  int rc = batch_upload(b, fs);
+  if (rc) {
+    tgmm_batch_destroy(b);
+    return rc;
+  }
+  *out = b;
+  return TGMM_OK;
+}
+
+int batch_upload(tgmm_batch* b, const std::vector<FrameSrc>& fs) {
+  DevPtrs& P = b->ws.P;
+  const WsShape& sh = b->ws.sh;
+  hipStream_t st = b->st;
+  const long long S = sh.Scap, G = sh.Gcap;
+  std::vector<int> sv_frame(S, -1), sv_len(S, 0), g_frame(G, -1);
+  std::vector<long long> sv_beg(S, 0);
+  std::vector<int> sblk(S / kAlign, -1), gblk(G / kAlign, -1);
+  std::vector<double> gs((size_t)GS_NFIELDS * G, 0.0);
+  std::vector<unsigned char> fixed(G, 0);
+  for (size_t i = 0; i < fs.size(); i++) {
+    const FrameSrc& f = fs[i];
+    const FrameDev& F = b->fh[i];
+    for (long long j = 0; j < f.S; j++) {
+      const long long a = f.ptr[j], e = f.ptr[j + 1];
+      if (a < 0 || e < a || e > f.N) {
+        tgmm_set_error("frame %zu: supervoxel pointer array is not monotone in [0, N]", i);
+        return TGMM_ERR_ARG;
+      }
+      if (e - a > 0x7fffffffLL) {
+        tgmm_set_error("frame %zu: supervoxel with more than 2^31 voxels", i);
+        return TGMM_ERR_LIMIT;
+      }
+      sv_frame[F.s0 + j] = (int)i;
+      sv_beg[F.s0 + j] = F.v0 + a;
+      sv_len[F.s0 + j] = (int)(e - a);
+    }
+    for (long long j = 0; j < round_up(std::max<long long>(f.S, 1), kAlign) / kAlign; j++)
+      sblk[F.s0 / kAlign + j] = (int)i;
+    for (long long j = 0; j < round_up(std::max<long long>(f.G, 1), kAlign) / kAlign; j++)
+      gblk[F.g0 / kAlign + j] = (int)i;
+    for (long long k = 0; k < f.G; k++) g_frame[F.g0 + k] = (int)i;
+    if (f.G > 0) records_to_soa(f.gm, f.G, F.g0, G, gs.data(), fixed.data());
+  }
+  const hipMemcpyKind kv = hipMemcpyDefault;
+  for (size_t i = 0; i < fs.size(); i++) {
+    const FrameSrc& f = fs[i];
+    const FrameDev& F = b->fh[i];
+    if (f.N > 0) {
+      HIPCHK(hipMemcpyAsync(P.vx + F.v0, f.query, f.N * 4, kv, st));
+      HIPCHK(hipMemcpyAsync(P.vy + F.v0, f.query + f.N, f.N * 4, kv, st));
+      HIPCHK(hipMemcpyAsync(P.vz + F.v0, f.query + 2 * f.N, f.N * 4, kv, st));
+      HIPCHK(hipMemcpyAsync(P.vimg + F.v0, f.img, f.N * 4, kv, st));
+    }
+    if (f.S > 0)
+      for (int a = 0; a < 3; a++)
+        HIPCHK(hipMemcpyAsync(P.sv_cent + a * S + F.s0, f.cent + a * f.S, f.S * 4, kv, st));
+  }
+  HIPCHK(hipMemcpyAsync(P.sv_frame, sv_frame.data(), S * 4, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(P.sv_beg, sv_beg.data(), S * 8, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(P.sv_len, sv_len.data(), S * 4, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(P.sblk_frame, sblk.data(), sblk.size() * 4, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(P.g_frame, g_frame.data(), G * 4, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(P.gblk_frame, gblk.data(), gblk.size() * 4, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(P.gs, gs.data(), gs.size() * 8, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(b->ws.gs_init, gs.data(), gs.size() * 8, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(P.fixed, fixed.data(), G, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(P.fr, b->fh.data(), b->fh.size() * sizeof(FrameDev), hipMemcpyHostToDevice,
+                        st));
+  HIPCHK(hipMemsetAsync(P.ind, 0xFF, (size_t)b->ws.K * S * 4, st));
+  HIPCHK(hipMemsetAsync(P.rnk, 0, (size_t)b->ws.K * S * 4, st));
+  HIPCHK(hipMemsetAsync(P.seg_cnt, 0, (G + 1) * 4, st));
+  HIPCHK(hipMemsetAsync(P.cell_cnt, 0, (sh.Ccap + 1) * 4, st));
+  HIPCHK(hipMemsetAsync(P.done, 0, 4 * sh.Fcap * 4, st));
+  HIPCHK(launch_prep(P, st));
+  HIPCHK(launch_reset(P, b->ko, st));
+  HIPCHK(hipStreamSynchronize(st));
+  // keep the host copies the fetch needs
+  b->fixed_host = fixed;
+  b->sizes.clear();
+  for (const FrameSrc& f : fs) b->sizes.push_back({f.N, f.S, (long long)f.G});
+  return TGMM_OK;
+}
+
+extern "C" int tgmm_batch_create(int device, const tgmm_frame_desc* frames, int nframes,
+                                 const tgmm_em_opts* opts, tgmm_batch** out) {
+  if (!frames || nframes < 1 || !out) {
+    tgmm_set_error("tgmm_batch_create: bad arguments");
+    return TGMM_ERR_ARG;
+  }
+  if (int rc = validate_opts(opts)) return rc;
+  std::vector<FrameSrc> fs(nframes);
+  for (int i = 0; i < nframes; i++) {
+    const tgmm_frame_desc& d = frames[i];
+    if (d.n_voxels < 0 || d.n_supervoxels < 0 || d.n_gaussians < 0 ||
+        (d.n_voxels > 0 && (!d.query || !d.img)) ||
+        (d.n_supervoxels > 0 && (!d.centroids)) || !d.sv_ptr || (d.n_gaussians > 0 && !d.gm)) {
+      tgmm_set_error("frame %d: NULL array or negative size", i);
+      return TGMM_ERR_ARG;
+    }
+    FrameSrc& f = fs[i];
+    f.N = d.n_voxels;
+    f.S = d.n_supervoxels;
+    f.G = d.n_gaussians;
+    f.query = d.query;
+    f.img = d.img;
+    f.cent = d.centroids;
+    f.ptr = d.sv_ptr;
+    f.gm = d.gm;
+  }
+  return batch_build(device, fs, *opts, nullptr, nullptr, out);
+}
+
+extern "C" int tgmm_batch_reset(tgmm_batch* b) {
+  if (!b) return TGMM_ERR_ARG;
+  HIPCHK(hipSetDevice(b->device));
+  DevPtrs& P = b->ws.P;
+  HIPCHK(hipMemcpyAsync(P.gs, b->ws.gs_init, (size_t)GS_NFIELDS * b->ws.sh.Gcap * 8,
+                        hipMemcpyDeviceToDevice, b->st));
+  HIPCHK(launch_reset(P, b->ko, b->st));
+  HIPCHK(hipStreamSynchronize(b->st));
+  return TGMM_OK;
+}
+
+static int any_active(tgmm_batch* b, bool* act) {
+  DevPtrs& P = b->ws.P;
+  HIPCHK(hipMemcpyAsync(b->fh.data(), P.fr, b->fh.size() * sizeof(FrameDev),
+                        hipMemcpyDeviceToHost, b->st));
+  HIPCHK(hipStreamSynchronize(b->st));
+  *act = false;
+  for (const FrameDev& F : b->fh) *act |= F.active != 0;
+  return TGMM_OK;
+}
+
+int batch_capture(tgmm_batch* b) {
+  if (b->gexec) return TGMM_OK;
+  HIPCHK(hipStreamBeginCapture(b->st, hipStreamCaptureModeThreadLocal));
+  hipError_t le = launch_iteration(b->ws.P, b->ko, b->st);
+  hipGraph_t g = nullptr;
+  hipError_t ee = hipStreamEndCapture(b->st, &g);
+  if (le != hipSuccess || ee != hipSuccess) {
+    if (g) (void)hipGraphDestroy(g);
+    tgmm_set_error("graph capture failed: %s / %s", hipGetErrorString(le), hipGetErrorString(ee));
+    return TGMM_ERR_DEVICE;
+  }
+  b->graph = g;
+  HIPCHK(hipGraphInstantiate(&b->gexec, g, nullptr, nullptr, 0));
+  return TGMM_OK;
+}
+
+extern "C" int tgmm_batch_solve(tgmm_batch* b, int mode) {
+  if (!b) return TGMM_ERR_ARG;
+  HIPCHK(hipSetDevice(b->device));
+  const int iters = b->opts.max_iter;
+  const bool poll = b->opts.tol >= 0.0;
+  if (mode == TGMM_SOLVE_GRAPH) {
+    if (int rc = batch_capture(b)) return rc;
+  }
+  for (int it = 0; it < iters; it++) {
+    if (mode == TGMM_SOLVE_GRAPH) {
+      HIPCHK(hipGraphLaunch(b->gexec, b->st));
+    } else {
+      HIPCHK(launch_iteration(b->ws.P, b->ko, b->st));
+    }
+    const bool check = (mode == TGMM_SOLVE_HOSTSYNC) || (poll && (it % 4 == 3));
+    if (check && it + 1 < iters) {
+      bool act = true;
+      if (int rc = any_active(b, &act)) return rc;
+      if (!act) break;
+    }
+  }
+  HIPCHK(hipStreamSynchronize(b->st));
+  return TGMM_OK;
+}
+
+// supervoxel assignment r > 0.5 in slot-major order (GMEM.cu:4328-4350)
+static void assign_supervoxels(GaussianMixtureModelCUDA* gm, long long G, long long S, int K,
+                               const float* rnk, long long rpitch, const int* ind, long long g0,
+                               int cap) {
+  for (long long k = 0; k < G; k++) gm[k].supervoxelNum = 0;
+  for (int j = 0; j < K; j++)
+    for (long long s = 0; s < S; s++) {
+      const float r = rnk[j * rpitch + s];
+      const int k = ind[j * rpitch + s];
+      if (r > 0.5f && k >= 0) {
+        GaussianMixtureModelCUDA& g = gm[k - g0];
+        if (g.supervoxelNum < cap) g.supervoxelIdx[g.supervoxelNum++] = (int)s;
+      }
+    }
+}
+
+int batch_fetch_host(tgmm_batch* b, std::vector<double>& gs, std::vector<float>& rnk,
+                     std::vector<int>& ind) {
+  DevPtrs& P = b->ws.P;
+  const long long S = b->ws.sh.Scap, G = b->ws.sh.Gcap;
+  const int K = b->ws.K;
+  gs.resize((size_t)GS_NFIELDS * G);
+  rnk.resize((size_t)K * S);
+  ind.resize((size_t)K * S);
+  HIPCHK(hipMemcpyAsync(gs.data(), P.gs, gs.size() * 8, hipMemcpyDeviceToHost, b->st));
+  HIPCHK(hipMemcpyAsync(rnk.data(), P.rnk, rnk.size() * 4, hipMemcpyDeviceToHost, b->st));
+  HIPCHK(hipMemcpyAsync(ind.data(), P.ind, ind.size() * 4, hipMemcpyDeviceToHost, b->st));
+  HIPCHK(hipMemcpyAsync(b->fh.data(), P.fr, b->fh.size() * sizeof(FrameDev),
+                        hipMemcpyDeviceToHost, b->st));
+  HIPCHK(hipStreamSynchronize(b->st));
+  return TGMM_OK;
+}
+
+void frame_results(tgmm_batch* b, int i, const std::vector<double>& gs,
+                   const std::vector<float>& rnk, const std::vector<int>& ind,
+                   GaussianMixtureModelCUDA* gm, float* rnk_out, int* ind_out, double* ll,
+                   int* iters) {
+  const FrameDev& F = b->fh[i];
+  const long long S = b->ws.sh.Scap, G = b->ws.sh.Gcap;
+  const int K = b->ws.K;
+  if (gm && F.ng > 0) {
+    soa_to_records(gs.data(), F.ng, F.g0, G, gm);
+    const int cap = b->opts.max_sv_per_gaussian;
+    assign_supervoxels(gm, F.ng, F.ns, K, rnk.data() + F.s0, S, ind.data() + F.s0, F.g0, cap);
+  }
+  for (int j = 0; j < K; j++)
+    for (long long s = 0; s < F.ns; s++) {
+      if (rnk_out) rnk_out[j * F.ns + s] = rnk[j * S + F.s0 + s];
+      if (ind_out) {
+        const int k = ind[j * S + F.s0 + s];
+        ind_out[j * F.ns + s] = k >= 0 ? (int)(k - F.g0) : -1;
+      }
+    }
+  if (ll) *ll = F.ll;
+  if (iters) *iters = F.iter - 1;
+}
+
+extern "C" int tgmm_batch_fetch(tgmm_batch* b, tgmm_frame_desc* frames, int nframes) {
+  if (!b || !frames || nframes != b->nframes) {
+    tgmm_set_error("tgmm_batch_fetch: bad arguments");
+    return TGMM_ERR_ARG;
+  }
+  HIPCHK(hipSetDevice(b->device));
+  std::vector<double> gs;
+  std::vector<float> rnk;
+  std::vector<int> ind;
+  if (int rc = batch_fetch_host(b, gs, rnk, ind)) return rc;
+  for (int i = 0; i < nframes; i++) {
+    tgmm_frame_desc& d = frames[i];
+    frame_results(b, i, gs, rnk, ind, d.gm, d.rnk, d.ind, &d.ll, &d.iterations);
+    d.status = TGMM_OK;
+  }
+  return TGMM_OK;
+}
+
+// Algorithmic bytes of one launch of a kernel over all frames of the batch
+// (compulsory traffic; Gaussian-side gathers counted once per Gaussian).
+static double kernel_bytes(tgmm_batch* b, int tk) {
+  const int K = b->ws.K;
+  double t = 0.0;
+  for (const auto& z : b->sizes) {
+    const double N = (double)z.N, S = (double)z.S, G = (double)z.G;
+    switch (tk) {
+      case TK_LOGLIK:
+        t += 16.0 * N + S * (4 + 8 + 4 + 12 + 4.0 * K) + G * (24 + 24 + 8);
+        break;
+      case TK_ESTEP:
+        t += S * (4 + 80 + 4.0 * K + 4.0 * K) + G * (8 * 11);
+        break;
+      case TK_KNN:
+        t += S * (4 + 12 + 4.0 * K) + G * 16;
+        break;
+      case TK_MSTEP:
+        t += G * (27 * 8 * 2 + 8) + S * (4 + 4 + 80);
+        break;
+    }
+  }
+  return t;
+}
+
+extern "C" int tgmm_batch_time_kernel(tgmm_batch* b, const char* name, int reps, double* avg_ms,
+                                      double* bytes_per_launch) {
+  if (!b || !name || reps < 1 || !avg_ms) return TGMM_ERR_ARG;
+  int tk = -1;
+  if (!strcmp(name, "loglik")) tk = TK_LOGLIK;
+  else if (!strcmp(name, "estep")) tk = TK_ESTEP;
+  else if (!strcmp(name, "knn")) tk = TK_KNN;
+  else if (!strcmp(name, "mstep")) tk = TK_MSTEP;
+  if (tk < 0) {
+    tgmm_set_error("unknown kernel name '%s'", name);
+    return TGMM_ERR_ARG;
+  }
+  if (b->opts.max_iter < 1) return TGMM_ERR_ARG;
+  HIPCHK(hipSetDevice(b->device));
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  // eager iterations from the initial state; `reps` launches of the kernel
+  double tot = 0.0;
+  int it = b->opts.max_iter;
+  for (int r = 0; r < reps; r++) {
+    if (it == b->opts.max_iter) {
+      if (int rc = tgmm_batch_reset(b)) return rc;
+      it = 0;
+    }
+    HIPCHK(launch_iteration(b->ws.P, b->ko, b->st, e0, e1, tk));
+    HIPCHK(hipEventSynchronize(e1));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+    tot += ms;
+    it++;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  *avg_ms = tot / reps;
+  if (bytes_per_launch) *bytes_per_launch = kernel_bytes(b, tk);
+  return TGMM_OK;
+}
+
+extern "C" int tgmm_batch_destroy(tgmm_batch* b) {
+  if (!b) return TGMM_ERR_ARG;
+  (void)hipSetDevice(b->device);
+  if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  if (b->st) (void)hipStreamSynchronize(b->st);
+  ws_free(&b->ws);
+  if (b->own_stream && b->st) (void)hipStreamDestroy(b->st);
+  delete b;
+  return TGMM_OK;
+}

@@ -1,0 +1,317 @@
+"""Python host mirror of the EM library's C-ABI (include/tgmm_em.h).
+
+Two layers, as in the C header:
+
+* the reference's host API, same names and argument meaning
+  (nmeth.3036-S2/src/UtilsCUDA/GMEMupdateCUDA.h:82, 92-98, 132-144):
+  ``GMEMinitializeMemoryWithSupervoxels``, ``copyScaleToDvice``,
+  ``GMEMvariationalInferenceCUDAWithSupervoxels``,
+  ``GMEMreleaseMemoryWithSupervoxels`` plus
+  ``regularizePrecisionMatrixConstants.setConstants`` (constants.h:38-44).
+  Errors raise ``TgmmError`` where the reference calls ``exit()``.
+* ``Batch``: many frames in one device workspace, graph-replayed iterations.
+
+The library is loaded from the package directory; importing this module on
+a machine without the built library raises immediately (there is no CPU
+fallback).
+"""
+import ctypes
+import os
+
+import numpy as np
+
+from .records import GMM_DTYPE
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libtgmm_em.so")
+
+TGMM_OK = 0
+ERR_NAMES = {-1: "TGMM_ERR_ARG", -2: "TGMM_ERR_LIMIT", -3: "TGMM_ERR_OOM",
+             -4: "TGMM_ERR_DEVICE", -5: "TGMM_ERR_NOT_SET", -6: "TGMM_ERR_IO",
+             -7: "TGMM_ERR_BUSY"}
+SOLVE_EAGER, SOLVE_GRAPH, SOLVE_HOSTSYNC = 0, 1, 2
+LEGACY_K = 5
+
+
+class TgmmError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("%s (%d): %s" % (ERR_NAMES.get(code, "?"), code, msg))
+        self.code = code
+
+
+class EmOpts(ctypes.Structure):
+    _fields_ = [
+        ("K", ctypes.c_int),
+        ("max_iter", ctypes.c_int),
+        ("tol", ctypes.c_double),
+        ("lambda_min", ctypes.c_double),
+        ("lambda_max", ctypes.c_double),
+        ("max_excentricity", ctypes.c_double),
+        ("w4dof", ctypes.c_int),
+        ("scale", ctypes.c_float * 3),
+        ("max_sv_per_gaussian", ctypes.c_int),
+    ]
+
+
+def make_opts(K=5, max_iter=10, tol=-1.0, lambda_min=0.02, lambda_max=0.2222,
+              max_excentricity=9.0, w4dof=True, scale=(1.0, 1.0, 5.0), cap=16):
+    """Defaults: parseConfigFile.h:193-195 constants, W4DOF as the tracker
+    (main.cpp:124), anisotropy (1, 1, 5)."""
+    o = EmOpts()
+    o.K, o.max_iter, o.tol = int(K), int(max_iter), float(tol)
+    o.lambda_min, o.lambda_max = float(lambda_min), float(lambda_max)
+    o.max_excentricity = float(max_excentricity)
+    o.w4dof = 1 if w4dof else 0
+    o.scale[:] = [float(x) for x in scale]
+    o.max_sv_per_gaussian = int(cap)
+    return o
+
+
+class FrameDesc(ctypes.Structure):
+    _fields_ = [
+        ("n_voxels", ctypes.c_longlong),
+        ("n_supervoxels", ctypes.c_longlong),
+        ("n_gaussians", ctypes.c_int),
+        ("query", ctypes.c_void_p),
+        ("img", ctypes.c_void_p),
+        ("centroids", ctypes.c_void_p),
+        ("sv_ptr", ctypes.c_void_p),
+        ("gm", ctypes.c_void_p),
+        ("rnk", ctypes.c_void_p),
+        ("ind", ctypes.c_void_p),
+        ("ll", ctypes.c_double),
+        ("iterations", ctypes.c_int),
+        ("status", ctypes.c_int),
+    ]
+
+
+_lib = None
+_vp = ctypes.c_void_p
+_pp = ctypes.POINTER(ctypes.c_void_p)
+
+
+def load():
+    """Load libtgmm_em.so (raises if it was not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError("libtgmm_em.so is not built at %s; run "
+                          "python -m tgmm_amd.build" % LIB_PATH)
+    L = ctypes.CDLL(LIB_PATH)
+    L.tgmm_last_error.restype = ctypes.c_char_p
+    L.tgmm_device_count.argtypes = [ctypes.POINTER(ctypes.c_int)]
+    L.tgmm_GMEMinitializeMemoryWithSupervoxels.argtypes = [
+        _pp, _vp, _pp, _vp, _vp, ctypes.c_longlong, _pp, _pp, _pp, ctypes.c_ushort,
+        _pp, _vp, _pp, _vp]
+    L.tgmm_copyScaleToDvice.argtypes = [_vp]
+    L.tgmm_setRegularizePrecisionMatrixConstants.argtypes = [ctypes.c_double] * 3
+    L.tgmm_GMEMvariationalInferenceCUDAWithSupervoxels.argtypes = [
+        _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, ctypes.c_longlong, ctypes.c_int,
+        ctypes.c_ushort, ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int,
+        ctypes.c_int, ctypes.c_char_p]
+    L.tgmm_GMEMreleaseMemoryWithSupervoxels.argtypes = [_pp] * 7
+    L.tgmm_batch_create.argtypes = [ctypes.c_int, ctypes.POINTER(FrameDesc), ctypes.c_int,
+                                    ctypes.POINTER(EmOpts), _pp]
+    L.tgmm_batch_reset.argtypes = [_vp]
+    L.tgmm_batch_solve.argtypes = [_vp, ctypes.c_int]
+    L.tgmm_batch_fetch.argtypes = [_vp, ctypes.POINTER(FrameDesc), ctypes.c_int]
+    L.tgmm_batch_time_kernel.argtypes = [_vp, ctypes.c_char_p, ctypes.c_int,
+                                         ctypes.POINTER(ctypes.c_double),
+                                         ctypes.POINTER(ctypes.c_double)]
+    L.tgmm_batch_destroy.argtypes = [_vp]
+    _lib = L
+    return L
+
+
+def _check(rc):
+    if rc != TGMM_OK:
+        raise TgmmError(rc, load().tgmm_last_error().decode())
+    return rc
+
+
+def device_count():
+    n = ctypes.c_int(0)
+    _check(load().tgmm_device_count(ctypes.byref(n)))
+    return n.value
+
+
+def as_records(a):
+    """C-contiguous array with exactly the 288-byte record layout (numpy
+    operations such as concatenate may repack a padded structured dtype)."""
+    a = np.asarray(a)
+    if a.dtype != GMM_DTYPE:
+        a = a.astype(GMM_DTYPE)
+    return np.ascontiguousarray(a)
+
+
+def _ptr(a):
+    return a.ctypes.data if a is not None else None
+
+
+# ---------------------------------------------------------------- drop-in --
+class DevicePointers:
+    """The seven device arrays GMEMinitializeMemoryWithSupervoxels returns."""
+
+    def __init__(self):
+        self.queryCUDA = ctypes.c_void_p()
+        self.imgDataCUDA = ctypes.c_void_p()
+        self.rnkCUDA = ctypes.c_void_p()
+        self.indCUDA = ctypes.c_void_p()
+        self.indCUDAtr = ctypes.c_void_p()
+        self.centroidLabelPositionCUDA = ctypes.c_void_p()
+        self.labelListPtrCUDA = ctypes.c_void_p()
+
+
+def GMEMinitializeMemoryWithSupervoxels(query, imgData, scale, query_nb, numLabels,
+                                        centroidLabelPosition, labelListPtrHOST):
+    """Allocate and upload one frame (GMEMupdateCUDA.h:132-137). Host arrays:
+    query f32 SoA [3N], imgData f32 [N], scale f32 [3], centroids f32 SoA [3S],
+    labelListPtr i64 [S+1]. Returns a DevicePointers."""
+    L = load()
+    q = np.ascontiguousarray(query, np.float32)
+    im = np.ascontiguousarray(imgData, np.float32)
+    sc = np.ascontiguousarray(scale, np.float32)
+    ce = np.ascontiguousarray(centroidLabelPosition, np.float32)
+    pt = np.ascontiguousarray(labelListPtrHOST, np.int64)
+    d = DevicePointers()
+    _check(L.tgmm_GMEMinitializeMemoryWithSupervoxels(
+        ctypes.byref(d.queryCUDA), _ptr(q), ctypes.byref(d.imgDataCUDA), _ptr(im), _ptr(sc),
+        int(query_nb), ctypes.byref(d.rnkCUDA), ctypes.byref(d.indCUDA),
+        ctypes.byref(d.indCUDAtr), int(numLabels), ctypes.byref(d.centroidLabelPositionCUDA),
+        _ptr(ce), ctypes.byref(d.labelListPtrCUDA), _ptr(pt)))
+    return d
+
+
+def copyScaleToDvice(scale):
+    """Anisotropy for the regularisation (GMEMupdateCUDA.h:82)."""
+    sc = np.ascontiguousarray(scale, np.float32)
+    _check(load().tgmm_copyScaleToDvice(_ptr(sc)))
+
+
+class regularizePrecisionMatrixConstants:
+    """Mirror of the reference's host statics (constants.h:19-45)."""
+
+    @staticmethod
+    def setConstants(lambdaMin, lambdaMax, maxExcentricity):
+        _check(load().tgmm_setRegularizePrecisionMatrixConstants(
+            float(lambdaMin), float(lambdaMax), float(maxExcentricity)))
+
+
+def GMEMvariationalInferenceCUDAWithSupervoxels(d, vecGMtemp, query_nb, ref_nb, numLabels,
+                                                maxIterEM, tolLikelihood, devCUDA=0,
+                                                frame=0, W4DOF=True, debugPath=""):
+    """One frame's EM (GMEMupdateCUDA.h:92-98). vecGMtemp: GMM_DTYPE array,
+    updated in place. Device rnk/ind in `d` hold the last iteration."""
+    if vecGMtemp.dtype != GMM_DTYPE or not vecGMtemp.flags["C_CONTIGUOUS"]:
+        raise TypeError("vecGMtemp must be a C-contiguous GMM_DTYPE array (288-byte records)")
+    _check(load().tgmm_GMEMvariationalInferenceCUDAWithSupervoxels(
+        d.queryCUDA, d.imgDataCUDA, d.rnkCUDA, None, d.indCUDA, d.indCUDAtr,
+        d.centroidLabelPositionCUDA, d.labelListPtrCUDA, vecGMtemp.ctypes.data,
+        int(query_nb), int(ref_nb), int(numLabels), int(maxIterEM), float(tolLikelihood),
+        int(devCUDA), int(frame), 1 if W4DOF else 0,
+        debugPath.encode() if debugPath else None))
+
+
+def GMEMreleaseMemoryWithSupervoxels(d):
+    """Free the seven arrays and null them (GMEMupdateCUDA.h:140-144)."""
+    _check(load().tgmm_GMEMreleaseMemoryWithSupervoxels(
+        ctypes.byref(d.queryCUDA), ctypes.byref(d.imgDataCUDA), ctypes.byref(d.rnkCUDA),
+        ctypes.byref(d.indCUDA), ctypes.byref(d.indCUDAtr),
+        ctypes.byref(d.centroidLabelPositionCUDA), ctypes.byref(d.labelListPtrCUDA)))
+
+
+_hip = None
+
+
+def copy_device_to_host(dev_ptr, nbytes, dtype):
+    """hipMemcpy of a device buffer (e.g. rnkCUDA) into a new numpy array."""
+    global _hip
+    if _hip is None:
+        _hip = ctypes.CDLL("libamdhip64.so")
+        _hip.hipMemcpy.argtypes = [_vp, _vp, ctypes.c_size_t, ctypes.c_int]
+    out = np.empty(nbytes // np.dtype(dtype).itemsize, dtype)
+    rc = _hip.hipMemcpy(out.ctypes.data, dev_ptr, nbytes, 2)  # DeviceToHost
+    if rc != 0:
+        raise TgmmError(-4, "hipMemcpy D2H failed (%d)" % rc)
+    return out
+
+
+# ------------------------------------------------------------------- batch --
+class Batch:
+    """Frames of uneven size in one device workspace.
+
+    frames: objects with ``query, img, centroids, sv_ptr, gm`` numpy arrays
+    (the boundary layout; see tgmm_amd.synth.Frame)."""
+
+    def __init__(self, frames, opts, device=0):
+        self.L = load()
+        self.opts = opts
+        self.frames = list(frames)
+        self._keep = []
+        n = len(self.frames)
+        self.desc = (FrameDesc * n)()
+        for i, f in enumerate(self.frames):
+            arrs = [np.ascontiguousarray(f.query, np.float32),
+                    np.ascontiguousarray(f.img, np.float32),
+                    np.ascontiguousarray(f.centroids, np.float32),
+                    np.ascontiguousarray(f.sv_ptr, np.int64),
+                    as_records(f.gm)]
+            self._keep.append(arrs)
+            d = self.desc[i]
+            d.n_voxels = arrs[1].shape[0]
+            d.n_supervoxels = arrs[3].shape[0] - 1
+            d.n_gaussians = arrs[4].shape[0]
+            d.query, d.img, d.centroids, d.sv_ptr, d.gm = [a.ctypes.data for a in arrs]
+        self.h = ctypes.c_void_p()
+        _check(self.L.tgmm_batch_create(device, self.desc, n, ctypes.byref(opts),
+                                        ctypes.byref(self.h)))
+        self._keep = None  # host copies are no longer needed after create
+
+    def reset(self):
+        _check(self.L.tgmm_batch_reset(self.h))
+
+    def solve(self, mode=SOLVE_GRAPH):
+        _check(self.L.tgmm_batch_solve(self.h, mode))
+
+    def fetch(self):
+        """List of dicts: gm (records), rnk/ind (slot-major [K, S]), ll, iterations."""
+        K = self.opts.K
+        outs = []
+        for i, f in enumerate(self.frames):
+            S = int(np.asarray(f.sv_ptr).shape[0] - 1)
+            gm = np.zeros(int(np.asarray(f.gm).shape[0]), GMM_DTYPE)
+            rnk = np.zeros(K * S, np.float32)
+            ind = np.zeros(K * S, np.int32)
+            d = self.desc[i]
+            d.gm, d.rnk, d.ind = gm.ctypes.data, rnk.ctypes.data, ind.ctypes.data
+            outs.append(dict(gm=gm, rnk=rnk, ind=ind))
+        _check(self.L.tgmm_batch_fetch(self.h, self.desc, len(self.frames)))
+        for i, o in enumerate(outs):
+            o["ll"] = self.desc[i].ll
+            o["iterations"] = self.desc[i].iterations
+        return outs
+
+    def time_kernel(self, name, reps):
+        ms = ctypes.c_double()
+        by = ctypes.c_double()
+        _check(self.L.tgmm_batch_time_kernel(self.h, name.encode(), int(reps),
+                                             ctypes.byref(ms), ctypes.byref(by)))
+        return ms.value, by.value
+
+    def close(self):
+        if self.h:
+            self.L.tgmm_batch_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
