@@ -1,0 +1,80 @@
+"""C-ABI checks that need no GPU: the library loads, exports every function
+include/tgmm_em.h declares, keeps the 288-byte record layout, and reports
+argument errors as status codes (never exit())."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+from tgmm_amd import em, synth
+from tgmm_amd.records import GMM_DTYPE
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "tgmm_em.h")
+
+
+def declared_functions():
+    txt = open(HEADER).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    names = re.findall(r"\b(tgmm_\w+)\s*\(", txt)
+    return sorted(set(names))
+
+
+def test_library_exports_every_declared_function():
+    L = em.load()
+    names = declared_functions()
+    assert len(names) >= 19
+    missing = [n for n in names if not hasattr(L, n)]
+    assert not missing, missing
+
+
+def test_record_layout():
+    offs = {"beta_k": 0, "nu_k": 8, "alpha_k": 16, "m_k": 24, "W_k": 48,
+            "expectedLogDetCovarianceCUDA": 96, "expectedLogResponsivityCUDA": 104,
+            "splitScore": 112, "beta_o": 120, "nu_o": 128, "alpha_o": 136, "m_o": 144,
+            "W_o": 168, "fixed": 216, "supervoxelIdx": 220, "supervoxelNum": 284}
+    for k, v in offs.items():
+        assert GMM_DTYPE.fields[k][1] == v, k
+    assert GMM_DTYPE.itemsize == 288
+    assert synth._load().tgmm_synth_record_size() == 288
+
+
+def test_device_count_without_gpu_is_not_an_error():
+    assert em.device_count() >= 0
+
+
+def test_bad_options_are_status_codes():
+    L = em.load()
+    f = synth.config1(n_cells=10, target_voxels=2000)
+    for K, code in ((0, -1), (17, -1)):
+        with pytest.raises(em.TgmmError) as e:
+            em.Batch([f], em.make_opts(K=K))
+        assert e.value.code == code
+    with pytest.raises(em.TgmmError) as e:
+        em.Batch([f], em.make_opts(lambda_min=-1.0))
+    assert e.value.code == -5
+
+
+def test_legacy_limits_and_missing_constants_are_status_codes():
+    L = em.load()
+    rc = L.tgmm_GMEMvariationalInferenceCUDAWithSupervoxels(
+        None, None, None, None, None, None, None, None, None, 10, 70000, 0, 1, -1.0, 0, 0, 1,
+        None)
+    assert rc == -2  # ref_nb > 65535
+    rc = L.tgmm_GMEMvariationalInferenceCUDAWithSupervoxels(
+        None, None, None, None, None, None, None, None, None, 65535 * 1024 + 1, 10, 0, 1, -1.0,
+        0, 0, 1, None)
+    assert rc == -2  # query_nb above 1024 * 65535
+    assert b"Too many" in L.tgmm_last_error()
+
+
+def test_generator_is_deterministic():
+    a = synth.config1()
+    b = synth.config1()
+    assert np.array_equal(a.query, b.query) and np.array_equal(a.img, b.img)
+    assert a.gm.tobytes() == b.gm.tobytes()
+    assert a.sv_ptr[0] == 0 and a.sv_ptr[-1] == a.N
+    assert np.all(np.diff(a.sv_ptr) >= 1) and np.all(np.diff(a.sv_ptr) <= 64)
+    assert 0.0 <= a.img.min() and a.img.max() == 1.0

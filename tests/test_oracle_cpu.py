@@ -1,0 +1,356 @@
+"""CPU tests that pin the FP64 oracle.
+
+Pins, per oracle function:
+  orc_digamma         GSL psi (the reference's CPU twin) golden vectors and live
+                      twin; known values; recurrence
+  orc_inv3/orc_det3   W W^-1 = I; rotated diagonals; singular sentinel
+  orc_eig3_ordered    known eigenvalues in the reference's order; V D V^T = W
+  orc_regularize_W    reference CPU twin (GaussianMixtureModel.h:283-368)
+  orc_posterior       reference CPU twin (GaussianMixtureModel.h:252-272)
+  orc_expectations    reference CPU twin (GaussianMixtureModel.h:229-243)
+  orc_knn             numpy brute force with the same FP32 operation order
+  orc_estep           invariants (rows sum to 1, symmetric ties, single
+                      candidate), dead-free padding
+  orc_mstep           numpy weighted mean / scatter
+  orc_loglik          scipy multivariate normal log-density
+  orc_death, orc_converged, orc_assign_supervoxels  rule-by-rule cases
+"""
+import math
+import os
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+from helpers import rel_err
+from tgmm_amd import synth
+from tgmm_amd.records import GMM_DTYPE, w_full
+
+GOLD = np.load(os.path.join(os.path.dirname(__file__), "golden", "reftwin_vectors.npz"))
+L = O.oracle()
+dp, fp, ip = O._dp, O._fp, O._ip
+
+
+def packed(M):
+    return np.array([M[0, 0], M[0, 1], M[0, 2], M[1, 1], M[1, 2], M[2, 2]])
+
+
+def rot(rng):
+    Q, _ = np.linalg.qr(rng.normal(size=(3, 3)))
+    return Q
+
+
+# --------------------------------------------------------------- digamma --
+def test_digamma_vs_gsl_golden():
+    x, ref = GOLD["psi_x"], GOLD["psi"]
+    got = np.array([L.orc_digamma(float(v)) for v in x])
+    pos = x > 0
+    assert rel_err(got[pos], ref[pos]) < 1e-13
+    # reflection for negative non-integers: cancellation near the poles
+    assert rel_err(got[~pos], ref[~pos], floor=1.0) < 1e-9
+
+
+def test_digamma_live_twin():
+    T = O.reftwin()
+    if T is None:
+        pytest.skip("reference CPU twin not built here (no /root/reference)")
+    rng = np.random.default_rng(5)
+    for v in np.concatenate([rng.uniform(1e-6, 20, 200), rng.uniform(20, 1e5, 50)]):
+        assert abs(L.orc_digamma(v) - T.ref_psi(v)) <= 1e-13 * max(1.0, abs(T.ref_psi(v)))
+
+
+def test_digamma_known_values():
+    g = 0.57721566490153286061
+    assert abs(L.orc_digamma(1.0) + g) < 1e-15
+    assert abs(L.orc_digamma(0.5) + g + 2 * math.log(2)) < 1e-15
+    rng = np.random.default_rng(1)
+    for v in rng.uniform(0.01, 50, 100):
+        assert abs(L.orc_digamma(v + 1) - L.orc_digamma(v) - 1 / v) < 1e-12 * max(1, 1 / v)
+    # the reference's value at non-positive integers (GMEM.cu:515-518)
+    for v in (0.0, -1.0, -7.0):
+        assert L.orc_digamma(v) == -1e300
+
+
+# ----------------------------------------------------------- 3x3 algebra --
+def test_det_inv():
+    rng = np.random.default_rng(2)
+    for _ in range(50):
+        Q = rot(rng)
+        d = rng.uniform(0.01, 3, 3)
+        M = Q @ np.diag(d) @ Q.T
+        w = packed(M)
+        assert abs(L.orc_det3(O.p(w, dp)) - np.prod(d)) < 1e-12 * np.prod(d)
+        inv = np.zeros(6)
+        assert L.orc_inv3(O.p(w, dp), O.p(inv, dp)) == 0
+        assert np.allclose(w_full(inv) @ M, np.eye(3), atol=1e-10)
+    sing = packed(np.diag([1.0, 2.0, 0.0]))
+    inv = np.zeros(6)
+    assert L.orc_inv3(O.p(sing, dp), O.p(inv, dp)) == 1
+    assert np.all(inv == 1e300)
+
+
+def test_eig3_order_and_reconstruction():
+    rng = np.random.default_rng(3)
+    for _ in range(50):
+        Q = rot(rng)
+        ev = np.sort(rng.uniform(0.01, 2.0, 3))[::-1]  # a > b > c
+        M = Q @ np.diag(ev) @ Q.T
+        d, v = np.zeros(3), np.zeros(9)
+        L.orc_eig3_ordered(O.p(packed(M), dp), O.p(d, dp), O.p(v, dp))
+        # eig3 order: (largest, smallest, middle)
+        assert np.allclose(d, [ev[0], ev[2], ev[1]], rtol=1e-12)
+        V = v.reshape(3, 3).T
+        assert np.allclose(V.T @ V, np.eye(3), atol=1e-12)
+        assert np.allclose(V @ np.diag(d) @ V.T, M, atol=1e-12)
+    # singular branch: (0, larger, smaller)
+    M = np.array([[1.0, 1.0, 0.0], [1.0, 1.0, 0.0], [0.0, 0.0, 3.0]])  # det exactly 0
+    d, v = np.zeros(3), np.zeros(9)
+    L.orc_eig3_ordered(O.p(packed(M), dp), O.p(d, dp), O.p(v, dp))
+    assert d[0] == 0.0 and abs(d[1] - 3) < 1e-12 and abs(d[2] - 2) < 1e-12
+    V = v.reshape(3, 3).T
+    assert np.allclose(V @ np.diag(d) @ V.T, M, atol=1e-12)
+
+
+def _regularize(W6, nu, w4, scale=(1, 1, 5)):
+    w = np.ascontiguousarray(W6, dtype=np.float64).copy()
+    sc = np.array(scale, np.float32)
+    L.orc_regularize_W(O.p(w, dp), float(nu), int(w4), O.p(sc, fp), 0.02, 0.2222, 9.0)
+    return w
+
+
+@pytest.mark.parametrize("w4", [0, 1])
+def test_regularize_vs_reference_twin_golden(w4):
+    Wr, nur, ref = GOLD["Wr"], GOLD["nur"], GOLD["reg_w4%d" % w4]
+    for i in range(Wr.shape[0]):
+        got = _regularize(Wr[i], nur[i], w4)
+        err = np.linalg.norm(got - ref[i]) / np.linalg.norm(ref[i])
+        assert err < 1e-9, (i, got, ref[i])
+
+
+def test_regularize_clamps_known_answer():
+    # diagonal W' (scale (1,1,5)): eigenvalues clamped into [lmin, lmax]/nu,
+    # then the eccentricity ratio 9 enforced pairwise in eig3 order
+    nu = 10.0
+    W = packed(np.diag([1.0, 1e-6, 0.005 * 25]))   # W' = diag(1, 1e-6, 0.005)
+    got = _regularize(W, nu, 1)
+    hi, lo = 0.2222 / nu, 0.02 / nu
+    # d = (1 -> hi, 1e-6 -> lo, 0.005 stays); ratio hi/lo = 11.1 > 9 -> d0 = 9 lo
+    expect = np.diag([9 * lo, lo, 0.005 * 25])
+    assert np.allclose(w_full(got), expect, rtol=1e-12)
+
+
+@pytest.mark.parametrize("variant", ["upd_reg0_w40", "upd_reg1_w40", "upd_reg0_w41",
+                                     "upd_reg1_w41"])
+def test_posterior_vs_reference_twin_golden(variant):
+    reg = variant[7] == "1"
+    w4 = int(variant[-1])
+    prior, Nk, xk, Sk, ref = GOLD["prior"], GOLD["Nk"], GOLD["xk"], GOLD["Sk"], GOLD[variant]
+    for i in range(prior.shape[0]):
+        g = np.zeros(1, GMM_DTYPE)
+        g["beta_o"], g["nu_o"], g["alpha_o"] = prior[i, 0], prior[i, 1], prior[i, 2]
+        g["m_o"] = prior[i, 3:6]
+        g["W_o"] = prior[i, 6:12]
+        Ck = np.ascontiguousarray(Nk[i] * Sk[i])
+        x = np.ascontiguousarray(xk[i])
+        L.orc_posterior(g.ctypes.data, float(Nk[i]), O.p(x, dp), O.p(Ck, dp))
+        if reg:
+            sc = np.array([1, 1, 5], np.float32)
+            W = np.ascontiguousarray(g["W_k"][0]).copy()
+            L.orc_regularize_W(O.p(W, dp), float(g["nu_k"][0]), w4, O.p(sc, fp), 0.02,
+                               0.2222, 9.0)
+            g["W_k"] = W
+        r = ref[i]
+        assert rel_err(g["beta_k"], r[0]) < 1e-13
+        assert rel_err(g["nu_k"], r[1]) < 1e-13
+        assert rel_err(g["alpha_k"], r[2]) < 1e-13
+        assert np.linalg.norm(g["m_k"][0] - r[3:6]) < 1e-12 * np.linalg.norm(r[3:6])
+        assert np.linalg.norm(g["W_k"][0] - r[6:12]) < 1e-9 * np.linalg.norm(r[6:12])
+        assert np.array_equal(g["W_o"][0], r[12:18])
+
+
+def test_expectations_vs_reference_twin_golden():
+    W, nu, eld = GOLD["W"], GOLD["nu"], GOLD["eld"]
+    al, tot, elp = GOLD["alpha"], GOLD["total"], GOLD["elp"]
+    for i in range(W.shape[0]):
+        g = np.zeros(1, GMM_DTYPE)
+        g["W_k"], g["nu_k"], g["alpha_k"] = W[i], nu[i], al[i]
+        L.orc_expectations(1, g.ctypes.data, float(tot[i]))
+        assert abs(g["expectedLogDetCovarianceCUDA"][0] - eld[i]) < 1e-12 * abs(eld[i])
+        assert abs(g["expectedLogResponsivityCUDA"][0] - elp[i]) < 1e-12 * max(1, abs(elp[i]))
+    dead = np.zeros(1, GMM_DTYPE)
+    dead["m_o"] = -1e32
+    L.orc_expectations(1, dead.ctypes.data, 10.0)
+    assert dead["expectedLogDetCovarianceCUDA"][0] == -1e32
+    assert dead["expectedLogResponsivityCUDA"][0] == -1e32
+
+
+# -------------------------------------------------------------------- kNN --
+@np.errstate(over="ignore", invalid="ignore")
+def numpy_knn(cent, S, means_f32, alive, scale, K):
+    c = cent.reshape(3, S).T.astype(np.float32)
+    m = means_f32.astype(np.float32)
+    a = (c[:, None, 0] - m[None, :, 0]) * np.float32(scale[0])
+    d = a * a
+    b = (c[:, None, 1] - m[None, :, 1]) * np.float32(scale[1])
+    d = d + b * b
+    e = (c[:, None, 2] - m[None, :, 2]) * np.float32(scale[2])
+    d = d + e * e
+    d = np.where(alive[None, :] & (d <= np.float32(1e32)), d, np.inf).astype(np.float32)
+    out = np.full((K, S), -1, np.int32)
+    idx = np.arange(m.shape[0])
+    for s in range(S):
+        order = np.lexsort((idx, d[s]))
+        take = [k for k in order[:K] if np.isfinite(d[s, k])]
+        out[: len(take), s] = take
+    return out.reshape(-1)
+
+
+@pytest.mark.parametrize("K,G", [(5, 200), (8, 60), (5, 3)])
+def test_knn_vs_numpy_bruteforce_with_ties(K, G):
+    rng = np.random.default_rng(K * 100 + G)
+    S = 300
+    cent = rng.integers(0, 40, size=3 * S).astype(np.float32)
+    gm = np.zeros(G, GMM_DTYPE)
+    gm["m_k"] = rng.integers(0, 40, size=(G, 3)).astype(np.float64)  # many exact ties
+    gm["m_o"] = gm["m_k"]
+    dead = rng.random(G) < 0.1
+    gm["m_o"][dead] = -1e32
+    gm["m_k"][dead] = -1e32
+    scale = np.array([1, 1, 5], np.float32)
+    ind = np.zeros(K * S, np.int32)
+    L.orc_knn(S, O.p(cent, fp), G, gm.ctypes.data, O.p(scale, fp), K, O.p(ind, ip))
+    ref = numpy_knn(cent, S, gm["m_k"], ~dead, scale, K)
+    assert np.array_equal(ind, ref)
+
+
+# ------------------------------------------------------------- E / M / LL --
+def tiny_frame(rng, n_sv=20, per=7, G=4):
+    S = n_sv
+    ptr = np.arange(S + 1, dtype=np.int64) * per
+    N = int(ptr[-1])
+    q = rng.normal(size=(3, N)) * 3 + 50
+    query = q.reshape(-1).astype(np.float32)
+    img = rng.uniform(0.1, 1.0, N).astype(np.float32)
+    cent = np.concatenate([query[a * N:(a + 1) * N].reshape(S, per).mean(1)
+                           for a in range(3)]).astype(np.float32)
+    gm = np.zeros(G, GMM_DTYPE)
+    gm["m_k"] = 50 + rng.normal(size=(G, 3))
+    gm["m_o"] = gm["m_k"]
+    gm["W_k"] = [0.1, 0.01, 0, 0.12, 0, 0.3]
+    gm["W_o"] = gm["W_k"]
+    gm["nu_k"] = gm["nu_o"] = 6.0
+    gm["beta_k"] = gm["beta_o"] = 2.0
+    gm["alpha_k"] = gm["alpha_o"] = 3.0
+    return N, S, query, img, cent, ptr, gm
+
+
+def test_estep_invariants():
+    rng = np.random.default_rng(11)
+    N, S, query, img, cent, ptr, gm = tiny_frame(rng)
+    K = 3
+    ind = np.zeros(K * S, np.int32)
+    sc = np.array([1, 1, 1], np.float32)
+    L.orc_knn(S, O.p(cent, fp), gm.shape[0], gm.ctypes.data, O.p(sc, fp), K, O.p(ind, ip))
+    L.orc_expectations(gm.shape[0], gm.ctypes.data, float(gm["alpha_k"].sum()))
+    rnk = np.zeros(K * S, np.float32)
+    L.orc_estep(N, S, O.p(query, fp), O.p(ptr, O._lp), gm.shape[0], gm.ctypes.data, K,
+                O.p(ind, ip), O.p(rnk, fp))
+    r = rnk.reshape(K, S)
+    assert np.all(r >= 0) and np.all(r <= 1)
+    assert np.allclose(r.sum(0), 1.0, atol=1e-6)
+    # two identical Gaussians share every supervoxel equally
+    g2 = np.concatenate([gm[:1], gm[:1]]).astype(GMM_DTYPE)
+    ind2 = np.tile(np.array([0, 1], np.int32)[:, None], (1, S)).reshape(-1)
+    L.orc_expectations(2, g2.ctypes.data, float(g2["alpha_k"].sum()))
+    r2 = np.zeros(2 * S, np.float32)
+    L.orc_estep(N, S, O.p(query, fp), O.p(ptr, O._lp), 2, g2.ctypes.data, 2, O.p(ind2, ip),
+                O.p(r2, fp))
+    assert np.allclose(r2, 0.5, atol=1e-7)
+    # one candidate and padding: r = 1 and 0
+    ind3 = np.concatenate([np.zeros(S, np.int32), -np.ones(S, np.int32)])
+    r3 = np.zeros(2 * S, np.float32)
+    L.orc_estep(N, S, O.p(query, fp), O.p(ptr, O._lp), 2, g2.ctypes.data, 2, O.p(ind3, ip),
+                O.p(r3, fp))
+    assert np.all(r3[:S] == 1.0) and np.all(r3[S:] == 0.0)
+
+
+def test_mstep_matches_numpy_weighted_moments():
+    rng = np.random.default_rng(12)
+    N, S, query, img, cent, ptr, gm = tiny_frame(rng, G=2)
+    K = 2
+    ind = np.concatenate([np.zeros(S, np.int32), np.ones(S, np.int32)])
+    r = rng.uniform(0.0, 1.0, S)
+    r[:3] = 1e-4  # below the 1e-3 cut: excluded (GMEM.cu:1350)
+    rnk = np.concatenate([r, 1 - r]).astype(np.float32)
+    Nk, xk, Ck = np.zeros(2), np.zeros(6), np.zeros(12)
+    L.orc_mstep(N, S, O.p(query, fp), O.p(img, fp), O.p(ptr, O._lp), 2, K, O.p(ind, ip),
+                O.p(rnk, fp), O.p(Nk, dp), O.p(xk, dp), O.p(Ck, dp))
+    X = query.reshape(3, N).T.astype(np.float64)
+    svid = np.repeat(np.arange(S), 7)
+    for k in range(2):
+        rk = rnk[k * S:(k + 1) * S].astype(np.float64)
+        rk = np.where(rk > np.float32(1e-3), rk, 0.0)
+        w = rk[svid] * img.astype(np.float64)
+        mean = np.average(X, axis=0, weights=w)
+        cov = np.cov(X.T, aweights=w, bias=True) * w.sum()
+        assert abs(Nk[k] - w.sum()) < 1e-12 * w.sum()
+        assert np.allclose(xk[3 * k:3 * k + 3], mean, rtol=1e-13)
+        assert np.allclose(w_full(Ck[6 * k:6 * k + 6]), cov, rtol=1e-10)
+
+
+def test_loglik_single_gaussian_closed_form():
+    from scipy.stats import multivariate_normal
+    rng = np.random.default_rng(13)
+    N, S, query, img, cent, ptr, gm = tiny_frame(rng, G=1)
+    ind = np.zeros(S, np.int32)
+    ll = L.orc_loglik(N, S, O.p(query, fp), O.p(img, fp), O.p(ptr, O._lp), 1, gm.ctypes.data, 1,
+                      O.p(ind, ip), float(gm["alpha_k"][0]))
+    X = query.reshape(3, N).T.astype(np.float64)
+    prec = gm["nu_k"][0] * w_full(gm["W_k"][0])
+    lp = multivariate_normal(mean=gm["m_k"][0], cov=np.linalg.inv(prec)).logpdf(X)
+    assert abs(ll - np.sum(img * lp)) < 1e-10 * abs(ll)
+
+
+def test_death_rule():
+    g = np.zeros(4, GMM_DTYPE)
+    g["alpha_k"] = [1.0, 1e-7, 1e-7, 5.0]
+    g["W_k"] = [1, 0, 0, 1, 0, 1]
+    g["fixed"][2] = 1
+    g["W_k"][3, 1] = np.nan
+    L.orc_death(4, g.ctypes.data, float(g["alpha_k"][:3].sum() + 5.0))
+    assert g["m_o"][0, 0] == 0.0                # survives
+    assert g["m_o"][1, 0] == -1e32 and g["alpha_k"][1] == 0  # pi < 1e-5
+    assert g["m_o"][2, 0] == 0.0                # fixed survives the alpha rule
+    assert g["m_o"][3, 0] == -1e32              # NaN W always dies
+    assert np.array_equal(g["W_k"][3], [1, 0, 0, 1, 0, 1])
+
+
+def test_convergence_rule():
+    c = L.orc_converged
+    assert c(-1.0, -10.0, -100.0, -1000.0, 1e-6) == 0
+    assert c(-100.0, -100.0 * (1 + 1e-7), -90.0, -80.0, 1e-6) == 1
+    assert c(-100.0, -90.0, -100.0, -80.0, 1e-6) == 1  # oscillation caught by ll-2
+    assert c(-100.0, -90.0, -95.0, -80.0, -1.0) == 0   # tol < 0 never converges
+    assert c(float("nan"), -1.0, -2.0, -3.0, -1.0) == 1  # NaN stops (comparison false)
+
+
+def test_assign_supervoxels_rule():
+    S, K, G = 40, 2, 2
+    rnk = np.zeros(K * S, np.float32)
+    ind = np.zeros(K * S, np.int32)
+    rnk[:S] = 0.9          # slot 0: all supervoxels to Gaussian 0 (cap truncates)
+    ind[S:] = 1
+    rnk[S + 5] = 0.51      # slot 1: one supervoxel to Gaussian 1
+    g = np.zeros(G, GMM_DTYPE)
+    L.orc_assign_supervoxels(S, G, g.ctypes.data, K, O.p(rnk, fp), O.p(ind, ip), 16)
+    assert g["supervoxelNum"][0] == 16 and list(g["supervoxelIdx"][0]) == list(range(16))
+    assert g["supervoxelNum"][1] == 1 and g["supervoxelIdx"][1][0] == 5
+
+
+def test_oracle_em_config1_recovers_cells():
+    f = synth.config1()
+    gm, rnk, ind, hist, it, ll = O.em_run(f, O.make_opts(K=5, max_iter=10))
+    assert it == 10 and np.isfinite(ll)
+    err = np.abs(gm["m_k"] - f.true_centres)
+    assert np.median(err) < 0.3  # means move from ~1.6 voxel noise to the cells
+    assert (gm["m_o"][:, 0] > -1e31).all()
