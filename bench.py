@@ -74,6 +74,10 @@ def main():
                     help="skip the FP64 accuracy check and the CPU baseline")
     ap.add_argument("--cpu-iters", type=int, default=2,
                     help="EM iterations of the oracle timed for the CPU baseline")
+    ap.add_argument("--check-iters", type=int, default=0,
+                    help="iterations of the accuracy solve (0 = the workload's count)")
+    ap.add_argument("--plain-frames", type=int, default=64,
+                    help="config 3: frames timed in the plain per-frame form")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: the EM of a frame runs on one "
@@ -96,19 +100,26 @@ def main():
     N_tot = sum(f.N for f in frames)
 
     # accuracy: one solve against the FP64 oracle (all frames when cheap)
-    batch.reset()
-    batch.solve(em.SOLVE_GRAPH)
-    outs = batch.fetch()
+    chk_iters = args.check_iters or iters
     acc = None
     if not args.no_oracle:
         import oracle_lib as O
         from helpers import record_errors, rel_err
+        if chk_iters != iters:
+            with em.Batch(frames, em.make_opts(K=K, max_iter=chk_iters, tol=-1.0)) as cb:
+                cb.solve(em.SOLVE_GRAPH)
+                outs = cb.fetch()
+        else:
+            batch.reset()
+            batch.solve(em.SOLVE_GRAPH)
+            outs = batch.fetch()
         errs = []
         knn_same = True
-        sample = frames if args.config != 3 else frames[::8]
-        for f in sample:
-            o = outs[frames.index(f)]
-            gm_o, rnk_o, ind_o, hist, it, ll = O.em_run(f, O.make_opts(K=K, max_iter=iters))
+        idx = list(range(len(frames))) if args.config != 3 else list(range(0, len(frames), 8))
+        sample = [frames[i] for i in idx]
+        for i, f in zip(idx, sample):
+            o = outs[i]
+            gm_o, rnk_o, ind_o, hist, it, ll = O.em_run(f, O.make_opts(K=K, max_iter=chk_iters))
             e = record_errors(o["gm"], gm_o)
             e["ll"] = rel_err(o["ll"], ll)
             errs.append(e)
@@ -116,6 +127,7 @@ def main():
         acc = {k: max(e[k] for e in errs) for k in errs[0]}
         acc["knn_sets_identical"] = knn_same
         acc["frames_checked"] = len(sample)
+        acc["iterations"] = chk_iters
         print("# accuracy vs FP64 oracle: %s" % json.dumps(acc), file=sys.stderr, flush=True)
 
     # warmup + timed steps
@@ -155,6 +167,24 @@ def main():
                 "avg_ms": kern[dom]["avg_ms"]}
     print("# kernels: %s" % json.dumps(kern), file=sys.stderr, flush=True)
     batch.close()
+
+    # config 3: the plain per-frame GPU form over the same frames (one
+    # workspace allocated per frame, host-synchronised iterations, no graph,
+    # no batching), the denominator of the batched-over-plain ratio
+    plain = None
+    if args.config == 3 and args.plain_frames > 0:
+        sub = frames[: args.plain_frames]
+        t1 = time.perf_counter()
+        for f in sub:
+            with em.Batch([f], opts) as pb:
+                pb.solve(em.SOLVE_HOSTSYNC)
+                pb.fetch()
+        dt = time.perf_counter() - t1
+        vox_sub = sum(f.N for f in sub)
+        plain = {"frames": len(sub), "seconds": dt, "frames_per_s": len(sub) / dt,
+                 "voxels_per_s": vox_sub / dt,
+                 "batched_over_plain_frames_per_s": frames_per_s / (len(sub) / dt)}
+        print("# plain per-frame form: %s" % json.dumps(plain), file=sys.stderr, flush=True)
 
     cpu = None
     if not args.no_oracle:
@@ -197,6 +227,8 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
+    if plain is not None:
+        line["plain_per_frame_form"] = plain
     print(json.dumps(line), flush=True)
 
 

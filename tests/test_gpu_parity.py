@@ -101,9 +101,14 @@ def test_soft_assignments_match_oracle(max_sv):
     e = record_errors(out["gm"], gm_o)
     assert max(e.values()) < TOL, e
     assert rel_err(out["ll"], ll) < TOL
-    # responsibilities: FP32 exponentials after FP64 max-subtraction
-    both = (rnk_o > 1e-6) & (out["ind"] == ind_o)
-    assert rel_err(out["rnk"][both], rnk_o[both]) < 1e-5
+    # responsibilities after 10 iterations: the FP32 r of each side feeds its
+    # M-step, so the Gaussians drift apart by ~1e-7 relative and a score of a
+    # few hundred nats moves by ~1e-5: r agrees to 1e-4 relative where it
+    # enters the M-step (r > 1e-3) and to 1e-6 absolute everywhere
+    same = out["ind"] == ind_o
+    big = same & (rnk_o > 1e-3)
+    assert rel_err(out["rnk"][big], rnk_o[big]) < 1e-4
+    assert np.max(np.abs(out["rnk"][same] - rnk_o[same])) < 1e-6
 
 
 def test_modes_bitwise_identical(c1):
@@ -161,3 +166,21 @@ def test_legacy_requires_constants_and_limits(c1):
         None, None, None, None, None, None, None, None, None, 10, 70000, 0, 1, -1.0, 0, 0,
         1, None)
     assert rc == -2
+
+
+def test_repeatable_and_order_independent(c1):
+    """Three runs of a 3-frame batch are bitwise identical, and permuting the
+    frames inside the batch changes nothing (per-frame reductions use every
+    block of the frame's aligned range, in a fixed order)."""
+    f3 = synth.config1(seed=777, n_cells=37, target_voxels=7000)
+    f2 = dup_gaussians(f3)
+    frames = [c1, f2, f3]
+    runs = [_run_batch(frames, _opts(iters=4)) for _ in range(3)]
+    perm = _run_batch([f3, c1, f2], _opts(iters=4))
+    for i in range(3):
+        for r in runs[1:]:
+            assert runs[0][i]["gm"].tobytes() == r[i]["gm"].tobytes()
+            assert runs[0][i]["ll"] == r[i]["ll"]
+    for a, b in zip([runs[0][2], runs[0][0], runs[0][1]], perm):
+        assert a["gm"].tobytes() == b["gm"].tobytes()
+        assert a["ll"] == b["ll"]

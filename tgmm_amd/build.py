@@ -74,9 +74,22 @@ def build_oracle():
         _run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "ref"])
 
 
+def build_devmath_host():
+    """Host build of the kernels' per-Gaussian math, for CPU tests only."""
+    src = os.path.join(ROOT, "tests", "native", "devmath_host.cpp")
+    if not os.path.exists(src):
+        return None
+    so = os.path.join(ROOT, "tests", "native", "libdevmath_host.so")
+    tmp = so + ".tmp"
+    _run([HIPCC, "-O2", "-fPIC", "-shared", "-std=c++17", "-o", tmp, src])
+    os.replace(tmp, so)
+    return so
+
+
 def build_all():
     build_synth()
     build_oracle()
+    build_devmath_host()
     build_em()
 
 

@@ -156,14 +156,15 @@ __host__ __device__ inline void eig3sym(const double* w, double* d, double* v) {
   int i0 = 0;
   if (g1 > g0) i0 = 1;
   if (g2 > (i0 == 0 ? g0 : g1)) i0 = 2;
-  const int i1 = (i0 + 1) % 3, i2 = (i0 + 2) % 3;
+  const double l0 = (i0 == 0) ? d[0] : (i0 == 1 ? d[1] : d[2]);
+  const double l1 = (i0 == 0) ? d[1] : (i0 == 1 ? d[2] : d[0]);
   double a0[3], a1[3], a2[3];
-  if (!(eigvec_cross(w, d[i0], a0) > 0.0)) {
+  if (!(eigvec_cross(w, l0, a0) > 0.0)) {
     // all three equal (w is a multiple of I): identity basis
     for (int i = 0; i < 9; i++) v[i] = (i % 4 == 0) ? 1.0 : 0.0;
     return;
   }
-  double n1 = eigvec_cross(w, d[i1], a1);
+  double n1 = eigvec_cross(w, l1, a1);
   if (n1 > 0.0) {
     const double pr = dot3(a1, a0);
     for (int k = 0; k < 3; k++) a1[k] -= pr * a0[k];
@@ -177,10 +178,12 @@ __host__ __device__ inline void eig3sym(const double* w, double* d, double* v) {
   }
   if (!(n1 > 0.0)) any_orthogonal(a0, a1);
   cross3(a0, a1, a2);
+  // vectors of eigenvalues i0, i0+1, i0+2 (mod 3); constant indices only
   for (int k = 0; k < 3; k++) {
-    v[3 * i0 + k] = a0[k];
-    v[3 * i1 + k] = a1[k];
-    v[3 * i2 + k] = a2[k];
+    const double x0 = a0[k], x1 = a1[k], x2 = a2[k];
+    v[k] = (i0 == 0) ? x0 : (i0 == 1 ? x2 : x1);
+    v[3 + k] = (i0 == 0) ? x1 : (i0 == 1 ? x0 : x2);
+    v[6 + k] = (i0 == 0) ? x2 : (i0 == 1 ? x1 : x0);
   }
 }
 

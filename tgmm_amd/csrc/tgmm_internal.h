@@ -20,9 +20,9 @@
 namespace tgmm {
 
 constexpr int kAlign = 256;      // supervoxel / Gaussian range alignment
-constexpr int kGBlock = 256;     // threads per block, Gaussian-parallel kernels
-constexpr int kSBlock = 256;     // threads per block, supervoxel-lane kernels
-constexpr int kLLsv = 64;        // supervoxels per log-likelihood block
+constexpr int kGB = 64;          // threads per block, Gaussian-lane kernels
+constexpr int kSB = 256;         // threads per block, supervoxel-lane kernels
+constexpr int kLLsv = 16;        // supervoxels per log-likelihood block (one per 16 lanes)
 constexpr int kScanTile = 4096;  // elements per scan block
 constexpr int kPairShift = 4;    // pair code = (s << 4) | j, j < 16
 
@@ -41,8 +41,8 @@ struct FrameDev {
   float org[3];       // grid origin in scaled coordinates
   float h;            // cell edge
   int dims[3];
-  int nbg;            // 256-blocks over the Gaussian range
-  int nbl;            // 64-supervoxel blocks over the supervoxel range
+  int nbg;            // kGB-blocks over the 256-aligned Gaussian range
+  int nbl;            // kLLsv-supervoxel blocks over the 256-aligned range
   int max_iter;
   double tol;
   double sum_alpha, psi_sum_alpha, log_sum_alpha, sum_alpha2;
@@ -51,6 +51,23 @@ struct FrameDev {
   int active;         // keeps iterating
   int status;
   int pad;
+};
+
+// Per-Gaussian parameter packs gathered by the supervoxel kernels: one
+// 64-byte (log-likelihood) or 96-byte (E-step) record per Gaussian instead of
+// one cache line per field.
+struct alignas(64) LLPack {
+  double m[3];  // mean
+  double lg2;   // (log alpha + log det(nu W)/2 - 1.5 log 2pi) * log2(e)
+  float M[6];   // nu W * log2(e) / 2, packed symmetric
+  float pad[2];
+};
+struct alignas(32) EPack {
+  double m[3];
+  double W[6];
+  double nu;
+  double kappa;  // E[ln pi] + E[ln det]/2 - 1.5/beta
+  double pad;
 };
 
 struct KOpts {
@@ -75,29 +92,30 @@ struct DevPtrs {
   int* ind;         // K*Scap
   float* rnk;       // K*Scap
   int* sblk_frame;  // Scap/256
+  int* qperm;       // Scap: supervoxels of each frame ordered by kNN grid cell
+  int* qcell;       // Scap: grid cell of each supervoxel centroid
   // Gaussians
   int* g_frame;
   double* gs;       // 27*Gcap
   unsigned char* fixed;
-  double* kappa;    // E-step per-voxel constant
-  float* nuW;       // 6*Gcap: nu_k * W_k (f32) for the log-likelihood
-  double* lg;       // log alpha + 0.5 log det(nu W) - 1.5 log 2pi
+  EPack* epack;     // Gcap: E-step parameters (written by gauss_prep)
+  LLPack* llpack;   // Gcap: log-likelihood parameters (written by death)
   float* mf;        // 3*Gcap f32 means (kNN references)
   int* gcell;
-  int* gblk_frame;  // Gcap/256
+  int* gblk_frame;  // Gcap/kGB
   // M-step segments
   int* seg_cnt;     // Gcap+1
   int* seg_start;   // Gcap+1
   int* pairs;       // K*Scap
   int* pairs_sorted;
+  double* mstat;    // 10*Gcap: N, S1[3], S2[6] of the current M-step
   // grid
   int* cell_cnt;    // Ccap+1
   int* cell_start;  // Ccap+1
-  float* cg;        // 3*Gcap sorted reference positions
-  int* cg_idx;      // Gcap
+  float4* cg4;      // Gcap: cell-sorted references (x, y, z, index bits)
   // reductions
-  double* part_g;   // Gcap/256
-  double* part_s;   // Scap/64
+  double* part_g;   // Gcap/kGB
+  double* part_s;   // Scap/kLLsv
   unsigned* done;   // 4*Fcap
   int* scan_blk;    // scan partials
   FrameDev* fr;
@@ -107,10 +125,10 @@ struct DevPtrs {
 };
 
 // kernel launchers (tgmm_kernels.hip); all enqueue on `st`
-// per-supervoxel moments (once per frame)
-hipError_t launch_prep(const DevPtrs& P, hipStream_t st);
-// kNN grid shape, iteration state and initial sum of alpha per frame
-hipError_t launch_reset(const DevPtrs& P, const KOpts& O, hipStream_t st);
+// once per frame: supervoxel moments, kNN grid shape, queries sorted by cell
+hipError_t launch_prep(const DevPtrs& P, const KOpts& O, hipStream_t st);
+// iteration state and initial sum of alpha per frame
+hipError_t launch_reset(const DevPtrs& P, hipStream_t st);
 // one EM iteration of every active frame; optional events around one kernel
 hipError_t launch_iteration(const DevPtrs& P, const KOpts& O, hipStream_t st,
                             hipEvent_t ev_before = nullptr, hipEvent_t ev_after = nullptr,

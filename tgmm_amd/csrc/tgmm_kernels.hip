@@ -4,10 +4,12 @@
 // result is independent of scheduling and of which other frames share the
 // launch.
 //
+// Once per frame (launch_prep): per-supervoxel FP64 moments, the frame's
+// uniform kNN grid, and the supervoxel order sorted by grid cell.
 // Per iteration, over all frames of a workspace:
 //   gauss_prep     E[ln det Lambda], E[ln pi] (GMEM.cu:553-628), E-step
 //                  constant, f32 means, kNN grid binning
-//   scan/scatter   counting sort of the means into per-frame uniform grids
+//   scan/scatter   counting sort of the means into per-frame grid cells
 //   knn            exact K nearest means per supervoxel centroid by ring
 //                  search (knnCuda.cu:61-144 distance, (dist, idx) order)
 //   estep          responsibilities from per-supervoxel moments
@@ -35,14 +37,19 @@ __device__ inline double wave_sum(double v) {
   return v;
 }
 
-// 256-thread block sum in a fixed order; result valid in thread 0.
-__device__ inline double block_sum256(double v, double* red4) {
+// Sum over a block of NW waves in a fixed order; result valid in thread 0.
+template <int NW>
+__device__ inline double block_sum(double v, double* red) {
   v = wave_sum(v);
+  if (NW == 1) return v;
   const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
   __syncthreads();
-  if (l == 0) red4[w] = v;
+  if (l == 0) red[w] = v;
   __syncthreads();
-  return (threadIdx.x == 0) ? ((red4[0] + red4[1]) + red4[2]) + red4[3] : 0.0;
+  double t = 0.0;
+  if (threadIdx.x == 0)
+    for (int i = 0; i < NW; i++) t += red[i];
+  return t;
 }
 
 __device__ inline double load_coherent(const double* p) {
@@ -55,10 +62,11 @@ __device__ inline double load_coherent(const double* p) {
 // partial, the block that arrives last sums all partials in index order and
 // resets the arrival counter. Returns the frame total in thread 0 of the
 // last block; *is_last tells every thread of the block whether it was last.
+template <int NW>
 __device__ inline double frame_reduce(double v, double* part, long long pidx, long long pbase,
-                                      int nblk, unsigned* done, double* red4, int* is_last) {
+                                      int nblk, unsigned* done, double* red, int* is_last) {
   __shared__ int s_last;
-  const double t = block_sum256(v, red4);
+  const double t = block_sum<NW>(v, red);
   if (threadIdx.x == 0) {
     part[pidx] = t;
     __threadfence();
@@ -69,9 +77,18 @@ __device__ inline double frame_reduce(double v, double* part, long long pidx, lo
   *is_last = s_last;
   if (!s_last) return 0.0;
   __threadfence();
-  double acc = 0.0;
-  for (int i = threadIdx.x; i < nblk; i += 256) acc += load_coherent(part + pbase + i);
-  const double tot = block_sum256(acc, red4);
+  // four independent partial sums per thread keep several loads in flight;
+  // the assignment of partials to (thread, slot) is fixed, so is the order
+  double a4[4] = {0.0, 0.0, 0.0, 0.0};
+  const int stride = NW * 64;
+  int i = threadIdx.x;
+  for (; i + 3 * stride < nblk; i += 4 * stride) {
+#pragma unroll
+    for (int q = 0; q < 4; q++) a4[q] += load_coherent(part + pbase + i + q * stride);
+  }
+  for (int q = 0; i < nblk; i += stride, q++) a4[q & 3] += load_coherent(part + pbase + i);
+  const double acc = (a4[0] + a4[1]) + (a4[2] + a4[3]);
+  const double tot = block_sum<NW>(acc, red);
   if (threadIdx.x == 0) *done = 0u;
   return tot;
 }
@@ -100,6 +117,15 @@ __device__ inline float knn_dist(float q0, float q1, float q2, float m0, float m
 
 __device__ inline bool lex_less(float da, int ia, float db, int ib) {
   return da < db || (da == db && ia < ib);
+}
+
+// Grid cell (per axis) of scaled coordinate u; NaN and values below the
+// origin go to 0, values beyond the last cell to the last cell.
+__device__ inline int axis_cell(double u, float org, float h, int dim) {
+  const double t = (u - (double)org) / (double)h;
+  if (t >= (double)dim) return dim - 1;
+  if (t > 0.0) return (int)floor(t);
+  return 0;
 }
 
 // ------------------------------------------------------------ prep kernels --
@@ -186,17 +212,17 @@ __global__ __launch_bounds__(256) void k_moments(DevPtrs P) {
   }
 }
 
-// One block per frame: bounding box of the scaled kNN centroids, grid shape
-// (about two means per cell, at most ncap cells), iteration state init.
-__global__ __launch_bounds__(256) void k_grid_setup(DevPtrs P, KOpts O) {
+// One block per frame: bounding box of the scaled kNN centroids and the grid
+// shape (about two means per cell, at most ncap cells).
+__global__ __launch_bounds__(1024) void k_grid_setup(DevPtrs P, KOpts O) {
   const int f = blockIdx.x;
   if (f >= P.nframes) return;
   FrameDev* F = P.fr + f;
   const long long s0 = F->s0, ns = F->ns;
   const long long S = P.Scap;
-  __shared__ double red[6][256];
+  __shared__ double red[6][1024];
   double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
-  for (long long i = threadIdx.x; i < ns; i += 256) {
+  for (long long i = threadIdx.x; i < ns; i += 1024) {
     for (int a = 0; a < 3; a++) {
       const double u = (double)P.sv_cent[a * S + s0 + i] * (double)O.scale[a];
       lo[a] = fmin(lo[a], u);
@@ -208,7 +234,7 @@ __global__ __launch_bounds__(256) void k_grid_setup(DevPtrs P, KOpts O) {
     red[3 + a][threadIdx.x] = hi[a];
   }
   __syncthreads();
-  for (int st = 128; st > 0; st >>= 1) {
+  for (int st = 512; st > 0; st >>= 1) {
     if ((int)threadIdx.x < st)
       for (int a = 0; a < 3; a++) {
         red[a][threadIdx.x] = fmin(red[a][threadIdx.x], red[a][threadIdx.x + st]);
@@ -219,8 +245,9 @@ __global__ __launch_bounds__(256) void k_grid_setup(DevPtrs P, KOpts O) {
   if (threadIdx.x != 0) return;
   double ext[3], org[3];
   for (int a = 0; a < 3; a++) {
-    org[a] = ns > 0 ? red[a][0] : 0.0;
-    ext[a] = ns > 0 ? fmax(red[3 + a][0] - red[a][0], 0.0) : 0.0;
+    const bool ok = ns > 0 && isfinite(red[a][0]) && isfinite(red[3 + a][0]);
+    org[a] = ok ? red[a][0] : 0.0;
+    ext[a] = ok ? fmax(red[3 + a][0] - red[a][0], 0.0) : 0.0;
   }
   const double target = fmax(1.0, ceil(0.5 * (double)F->ng));
   const double vol = fmax(ext[0], 1e-3) * fmax(ext[1], 1e-3) * fmax(ext[2], 1e-3);
@@ -256,12 +283,50 @@ __global__ __launch_bounds__(256) void k_grid_setup(DevPtrs P, KOpts O) {
     F->dims[0] = F->dims[1] = F->dims[2] = 1;
     F->h = 3.0e38f;
   }
+}
+
+// Supervoxels sorted by the grid cell of their centroid (once per frame), so
+// that neighbouring lanes of the kNN kernel walk the same cells.
+__global__ __launch_bounds__(256) void k_qcell_count(DevPtrs P, KOpts O) {
+  const int f = P.sblk_frame[blockIdx.x];
+  if (f < 0) return;
+  const long long s = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (P.sv_frame[s] != f) return;
+  const FrameDev* F = P.fr + f;
+  const long long S = P.Scap;
+  int c[3];
+  for (int a = 0; a < 3; a++)
+    c[a] = axis_cell((double)P.sv_cent[a * S + s] * (double)O.scale[a], F->org[a], F->h,
+                     F->dims[a]);
+  const long long cell = F->c0 + ((long long)c[2] * F->dims[1] + c[1]) * F->dims[0] + c[0];
+  P.qcell[s] = (int)cell;
+  atomicAdd(P.cell_cnt + cell, 1);
+}
+
+__global__ __launch_bounds__(256) void k_qcell_scatter(DevPtrs P) {
+  const int f = P.sblk_frame[blockIdx.x];
+  if (f < 0) return;
+  const long long s = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (P.sv_frame[s] != f) return;
+  const FrameDev* F = P.fr + f;
+  const int c = P.qcell[s];
+  const long long pos = F->s0 + (P.cell_start[c] - P.cell_start[F->c0]) +
+                        atomicSub(P.cell_cnt + c, 1) - 1;
+  P.qperm[pos] = (int)s;
+}
+
+// Iteration state of every frame (reference: ll = -1, -10, -100, -1000 and
+// numIterEM = 1 before the loop, GMEM.cu:4143-4144).
+__global__ void k_iter_init(DevPtrs P) {
+  const int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= P.nframes) return;
+  FrameDev* F = P.fr + f;
   F->ll = -1.0;
   F->ll1 = -10.0;
   F->ll2 = -100.0;
   F->ll3 = -1000.0;
   F->iter = 1;
-  F->active = (ns > 0 && F->ng > 0 && F->max_iter >= 1 &&
+  F->active = (F->ns > 0 && F->ng > 0 && F->max_iter >= 1 &&
                keep_going(F->ll, F->ll1, F->ll2, F->ll3, F->tol))
                   ? 1
                   : 0;
@@ -269,17 +334,17 @@ __global__ __launch_bounds__(256) void k_grid_setup(DevPtrs P, KOpts O) {
 }
 
 // Sum of alpha per frame before the first iteration (GMEM.cu:4167).
-__global__ __launch_bounds__(256) void k_alpha_init(DevPtrs P) {
-  __shared__ double red[4];
+__global__ __launch_bounds__(kGB) void k_alpha_init(DevPtrs P) {
+  __shared__ double red[1];
   const int f = P.gblk_frame[blockIdx.x];
   if (f < 0) return;
   FrameDev* F = P.fr + f;
   if (!F->active) return;
-  const long long k = (long long)blockIdx.x * 256 + threadIdx.x;
+  const long long k = (long long)blockIdx.x * kGB + threadIdx.x;
   const double a = (P.g_frame[k] == f) ? gsv(P, GS_ALPHA, k) : 0.0;
   int last;
-  const double tot = frame_reduce(a, P.part_g, blockIdx.x, F->g0 / 256, F->nbg,
-                                  P.done + 4 * f + 0, red, &last);
+  const double tot = frame_reduce<1>(a, P.part_g, blockIdx.x, F->g0 / kGB, F->nbg,
+                                     P.done + 4 * f + 0, red, &last);
   if (last && threadIdx.x == 0) {
     F->sum_alpha = tot;
     F->psi_sum_alpha = digamma(tot);
@@ -288,6 +353,70 @@ __global__ __launch_bounds__(256) void k_alpha_init(DevPtrs P) {
 }
 
 // ---------------------------------------------------------------- scans --
+// Exclusive scan of 1024 per-thread values; returns this thread's prefix.
+__device__ inline int block_exclusive_scan1024(int v, int* lds) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  int x = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int y = __shfl_up(x, o, 64);
+    if (lane >= o) x += y;
+  }
+  if (lane == 63) lds[w] = x;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int run = 0;
+    for (int i = 0; i < 16; i++) {
+      const int t = lds[i];
+      lds[16 + i] = run;
+      run += t;
+    }
+  }
+  __syncthreads();
+  return lds[16 + w] + x - v;
+}
+
+// single-block exclusive scan for small arrays: tiles of 16384, each thread
+// 16 consecutive elements loaded as four int4
+__global__ __launch_bounds__(1024) void k_scan_single(const int* in, long long n, int* out) {
+  __shared__ int lds[32];
+  __shared__ int carry;
+  if (threadIdx.x == 0) carry = 0;
+  __syncthreads();
+  for (long long base = 0; base < n; base += 16384) {
+    const int c = carry;
+    const long long b = base + (long long)threadIdx.x * 16;
+    int v[16];
+    if (b + 16 <= n) {
+      const int4* p4 = reinterpret_cast<const int4*>(in + b);
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        const int4 t = p4[q];
+        v[4 * q] = t.x;
+        v[4 * q + 1] = t.y;
+        v[4 * q + 2] = t.z;
+        v[4 * q + 3] = t.w;
+      }
+    } else {
+#pragma unroll
+      for (int i = 0; i < 16; i++) v[i] = (b + i < n) ? in[b + i] : 0;
+    }
+    int tsum = 0;
+#pragma unroll
+    for (int i = 0; i < 16; i++) tsum += v[i];
+    int run = c + block_exclusive_scan1024(tsum, lds);
+    const int incl_last = run + tsum;
+#pragma unroll
+    for (int i = 0; i < 16; i++) {
+      if (b + i < n) out[b + i] = run;
+      run += v[i];
+    }
+    __syncthreads();
+    if (threadIdx.x == 1023) carry = incl_last;
+    __syncthreads();
+  }
+}
+
 __global__ __launch_bounds__(256) void k_scan_partial(const int* in, long long n, int* blk) {
   const long long base = (long long)blockIdx.x * kScanTile;
   int acc = 0;
@@ -329,7 +458,6 @@ __global__ __launch_bounds__(256) void k_scan_final(const int* in, long long n, 
   }
   red[threadIdx.x] = tsum;
   __syncthreads();
-  // Hillis-Steele inclusive scan of thread sums
   for (int st = 1; st < 256; st <<= 1) {
     const int t = ((int)threadIdx.x >= st) ? red[threadIdx.x - st] : 0;
     __syncthreads();
@@ -346,12 +474,12 @@ __global__ __launch_bounds__(256) void k_scan_final(const int* in, long long n, 
 
 // ------------------------------------------------------- iteration kernels --
 // E-step constants and kNN grid binning (GMEM.cu:553-628, 527-539).
-__global__ __launch_bounds__(256) void k_gauss_prep(DevPtrs P, KOpts O) {
+__global__ __launch_bounds__(kGB) void k_gauss_prep(DevPtrs P, KOpts O) {
   const int f = P.gblk_frame[blockIdx.x];
   if (f < 0) return;
   const FrameDev* F = P.fr + f;
   if (!F->active) return;
-  const long long k = (long long)blockIdx.x * 256 + threadIdx.x;
+  const long long k = (long long)blockIdx.x * kGB + threadIdx.x;
   if (P.g_frame[k] != f) return;
   const long long G = P.Gcap;
   if (gsv(P, GS_M_O, k) < -1e31) {
@@ -368,50 +496,51 @@ __global__ __launch_bounds__(256) void k_gauss_prep(DevPtrs P, KOpts O) {
   const double elp = digamma(alpha) - F->psi_sum_alpha;
   P.gs[GS_ELD * G + k] = eld;
   P.gs[GS_ELP * G + k] = elp;
-  P.kappa[k] = elp + 0.5 * eld - 1.5 / beta;
+  EPack ep;
+  for (int i = 0; i < 3; i++) ep.m[i] = gsv(P, GS_M + i, k);
+  for (int i = 0; i < 6; i++) ep.W[i] = W[i];
+  ep.nu = nu;
+  ep.kappa = elp + 0.5 * eld - 1.5 / beta;
+  ep.pad = 0.0;
+  P.epack[k] = ep;
   int c[3];
   for (int a = 0; a < 3; a++) {
     const float m = (float)gsv(P, GS_M + a, k);
     P.mf[a * G + k] = m;
-    const double u = ((double)m * (double)O.scale[a] - (double)F->org[a]) / (double)F->h;
-    int ci = 0;
-    if (u >= (double)F->dims[a]) ci = F->dims[a] - 1;
-    else if (u > 0.0) ci = (int)floor(u);  // NaN and negatives go to 0
-    c[a] = ci;
+    c[a] = axis_cell((double)m * (double)O.scale[a], F->org[a], F->h, F->dims[a]);
   }
   const long long cell = F->c0 + ((long long)c[2] * F->dims[1] + c[1]) * F->dims[0] + c[0];
   P.gcell[k] = (int)cell;
   atomicAdd(P.cell_cnt + cell, 1);
 }
 
-__global__ __launch_bounds__(256) void k_grid_scatter(DevPtrs P) {
+__global__ __launch_bounds__(kGB) void k_grid_scatter(DevPtrs P) {
   const int f = P.gblk_frame[blockIdx.x];
   if (f < 0 || !P.fr[f].active) return;
-  const long long k = (long long)blockIdx.x * 256 + threadIdx.x;
+  const long long k = (long long)blockIdx.x * kGB + threadIdx.x;
   if (P.g_frame[k] != f) return;
   const int c = P.gcell[k];
   if (c < 0) return;
   const int pos = P.cell_start[c] + atomicSub(P.cell_cnt + c, 1) - 1;
   const long long G = P.Gcap;
-  P.cg[pos] = P.mf[k];
-  P.cg[G + pos] = P.mf[G + k];
-  P.cg[2 * G + pos] = P.mf[2 * G + k];
-  P.cg_idx[pos] = (int)k;
+  P.cg4[pos] = make_float4(P.mf[k], P.mf[G + k], P.mf[2 * G + k], __int_as_float((int)k));
 }
 
 // Exact K nearest means by expanding shells of grid cells. After shell r,
 // every unvisited mean lies beyond an open face of the visited block, so its
 // scaled distance is at least LB; the search stops once the K-th best
 // distance is below LB^2 with a relative margin far above f32 rounding, so
-// the result equals brute force (ties broken by the lower index).
+// the result equals brute force (ties broken by the lower index). Lanes take
+// the supervoxels in grid-cell order (qperm).
 template <int KM>
 __global__ __launch_bounds__(256) void k_knn(DevPtrs P, KOpts O) {
   const int f = P.sblk_frame[blockIdx.x];
   if (f < 0 || !P.fr[f].active) return;
-  const long long s = (long long)blockIdx.x * 256 + threadIdx.x;
-  if (P.sv_frame[s] != f) return;
   const FrameDev* F = P.fr + f;
-  const long long S = P.Scap, G = P.Gcap;
+  const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (i >= F->s0 + F->ns) return;
+  const long long s = P.qperm[i];
+  const long long S = P.Scap;
   const int K = O.K;
   const float s0 = O.scale[0], s1 = O.scale[1], s2 = O.scale[2];
   const float q0 = P.sv_cent[s], q1 = P.sv_cent[S + s], q2 = P.sv_cent[2 * S + s];
@@ -421,13 +550,7 @@ __global__ __launch_bounds__(256) void k_knn(DevPtrs P, KOpts O) {
   const double org[3] = {F->org[0], F->org[1], F->org[2]};
   const int dims[3] = {nx, ny, nz};
   int c[3];
-  for (int a = 0; a < 3; a++) {
-    const double t = (u[a] - org[a]) / h;
-    int ci = 0;
-    if (t >= (double)dims[a]) ci = dims[a] - 1;
-    else if (t > 0.0) ci = (int)floor(t);
-    c[a] = ci;
-  }
+  for (int a = 0; a < 3; a++) c[a] = axis_cell(u[a], F->org[a], F->h, dims[a]);
   float bd[KM];
   int bi[KM];
 #pragma unroll
@@ -437,79 +560,95 @@ __global__ __launch_bounds__(256) void k_knn(DevPtrs P, KOpts O) {
   }
   int found = 0;
   const long long c0 = F->c0;
-  const int rmax = max(nx, max(ny, nz));
-  for (int r = 0; r <= rmax; r++) {
-    const int zlo = max(c[2] - r, 0), zhi = min(c[2] + r, nz - 1);
-    const int ylo = max(c[1] - r, 0), yhi = min(c[1] + r, ny - 1);
-    for (int z = zlo; z <= zhi; z++) {
-      const bool zface = (z == c[2] - r) || (z == c[2] + r);
-      for (int y = ylo; y <= yhi; y++) {
-        const bool yface = (y == c[1] - r) || (y == c[1] + r);
-        const long long row = c0 + ((long long)z * ny + y) * nx;
-        // x segments: full row on z/y faces, else the two end cells
-        int segs = 0;
-        int xa[2], xb[2];
-        if (zface || yface) {
-          xa[0] = max(c[0] - r, 0);
-          xb[0] = min(c[0] + r, nx - 1);
-          segs = 1;
-        } else {
-          if (c[0] - r >= 0) {
-            xa[segs] = xb[segs] = c[0] - r;
-            segs++;
-          }
-          if (r > 0 && c[0] + r < nx) {
-            xa[segs] = xb[segs] = c[0] + r;
-            segs++;
-          }
-        }
-        for (int sg = 0; sg < segs; sg++) {
-          const int pb = P.cell_start[row + xa[sg]], pe = P.cell_start[row + xb[sg] + 1];
-          for (int p = pb; p < pe; p++) {
-            const float d = knn_dist(q0, q1, q2, P.cg[p], P.cg[G + p], P.cg[2 * G + p], s0, s1, s2);
-            if (!(d <= 1e32f)) continue;
-            const int idx = P.cg_idx[p];
-            if (!lex_less(d, idx, bd[KM - 1], bi[KM - 1])) continue;
-            found++;
-            // insertion into the sorted list
-            float cd = d;
-            int ci2 = idx;
+  // candidate update with a one-ahead prefetch of the packed references
+  auto scan_range = [&](int pb, int pe) {
+    if (pb >= pe) return;
+    float4 cur = P.cg4[pb];
+    for (int p = pb; p < pe; p++) {
+      const float4 nxt = (p + 1 < pe) ? P.cg4[p + 1] : cur;
+      const float d = knn_dist(q0, q1, q2, cur.x, cur.y, cur.z, s0, s1, s2);
+      const int idx = __float_as_int(cur.w);
+      if ((d <= 1e32f) && lex_less(d, idx, bd[KM - 1], bi[KM - 1])) {
+        found++;
+        float cd = d;
+        int ci2 = idx;
 #pragma unroll
-            for (int j = 0; j < KM; j++) {
-              if (lex_less(cd, ci2, bd[j], bi[j])) {
-                const float td = bd[j];
-                const int ti = bi[j];
-                bd[j] = cd;
-                bi[j] = ci2;
-                cd = td;
-                ci2 = ti;
-              }
-            }
+        for (int j = 0; j < KM; j++) {
+          if (lex_less(cd, ci2, bd[j], bi[j])) {
+            const float td = bd[j];
+            const int ti = bi[j];
+            bd[j] = cd;
+            bi[j] = ci2;
+            cd = td;
+            ci2 = ti;
           }
         }
       }
+      cur = nxt;
     }
-    // lower bound on the distance to any unvisited mean
+  };
+  auto lower_bound_sq = [&](int r) -> double {
     double lb = INFINITY;
     for (int a = 0; a < 3; a++) {
       if (c[a] - r > 0) lb = fmin(lb, u[a] - (org[a] + (double)(c[a] - r) * h));
       if (c[a] + r < dims[a] - 1) lb = fmin(lb, (org[a] + (double)(c[a] + r + 1) * h) - u[a]);
     }
-    if (lb == INFINITY) break;
+    if (lb == INFINITY) return INFINITY;
     lb = fmax(lb, 0.0) * (1.0 - 1e-5);
+    return lb * lb;
+  };
+  auto done_at = [&](int r) -> bool {
+    const double lb2 = lower_bound_sq(r);
+    if (lb2 == INFINITY) return true;  // every cell visited
     float kth = INFINITY;
 #pragma unroll
     for (int t = 0; t < KM; t++)
       if (t == K - 1) kth = bd[t];
-    if (found >= K && (double)kth < lb * lb) break;
-  }
-  for (int j = 0; j < K; j++) {
-    int out = -1;
+    return found >= K && (double)kth < lb2;
+  };
+  // shells 0 and 1 together: the 3x3 rows of the 3x3x3 block, all 18 row
+  // bounds loaded before any candidate is read
+  {
+    const int xlo = max(c[0] - 1, 0), xhi = min(c[0] + 1, nx - 1);
+    int pb[9], pe[9];
 #pragma unroll
-    for (int t = 0; t < KM; t++)
-      if (t == j) out = (bd[t] <= 1e32f) ? bi[t] : -1;
-    P.ind[(long long)j * S + s] = out;
+    for (int t = 0; t < 9; t++) {
+      const int z = c[2] + t / 3 - 1, y = c[1] + t % 3 - 1;
+      const bool ok = z >= 0 && z < nz && y >= 0 && y < ny;
+      const long long row = ok ? c0 + ((long long)z * ny + y) * nx : c0;
+      pb[t] = ok ? P.cell_start[row + xlo] : 0;
+      pe[t] = ok ? P.cell_start[row + xhi + 1] : 0;
+    }
+#pragma unroll
+    for (int t = 0; t < 9; t++) scan_range(pb[t], pe[t]);
   }
+  const int rmax = max(nx, max(ny, nz));
+  if (!done_at(1)) {
+    for (int r = 2; r <= rmax; r++) {
+      const int zlo = max(c[2] - r, 0), zhi = min(c[2] + r, nz - 1);
+      const int ylo = max(c[1] - r, 0), yhi = min(c[1] + r, ny - 1);
+      for (int z = zlo; z <= zhi; z++) {
+        const bool zface = (z == c[2] - r) || (z == c[2] + r);
+        for (int y = ylo; y <= yhi; y++) {
+          const bool yface = (y == c[1] - r) || (y == c[1] + r);
+          const long long row = c0 + ((long long)z * ny + y) * nx;
+          if (zface || yface) {
+            const int xa = max(c[0] - r, 0), xb = min(c[0] + r, nx - 1);
+            scan_range(P.cell_start[row + xa], P.cell_start[row + xb + 1]);
+          } else {
+            if (c[0] - r >= 0)
+              scan_range(P.cell_start[row + c[0] - r], P.cell_start[row + c[0] - r + 1]);
+            if (c[0] + r < nx)
+              scan_range(P.cell_start[row + c[0] + r], P.cell_start[row + c[0] + r + 1]);
+          }
+        }
+      }
+      if (done_at(r)) break;
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < KM; j++)
+    if (j < K) P.ind[(long long)j * S + s] = (bd[j] <= 1e32f) ? bi[j] : -1;
 }
 
 // Responsibilities of the K candidates of each supervoxel. The reference's
@@ -525,7 +664,7 @@ __global__ __launch_bounds__(256) void k_estep(DevPtrs P, KOpts O) {
   if (f < 0 || !P.fr[f].active) return;
   const long long s = (long long)blockIdx.x * 256 + threadIdx.x;
   if (P.sv_frame[s] != f) return;
-  const long long S = P.Scap, G = P.Gcap;
+  const long long S = P.Scap;
   const int K = O.K;
   const double n = P.mom[s];
   const double c0 = P.mom[S + s], c1 = P.mom[2 * S + s], c2 = P.mom[3 * S + s];
@@ -543,17 +682,14 @@ __global__ __launch_bounds__(256) void k_estep(DevPtrs P, KOpts O) {
       const int k = P.ind[(long long)j * S + s];
       kk[j] = k;
       if (k >= 0) {
-        const double d0 = c0 - P.gs[GS_M * G + k], d1 = c1 - P.gs[(GS_M + 1) * G + k],
-                     d2 = c2 - P.gs[(GS_M + 2) * G + k];
-        double W[6];
-#pragma unroll
-        for (int i = 0; i < 6; i++) W[i] = P.gs[(GS_W + i) * G + k];
-        const double nu = P.gs[GS_NU * G + k];
+        const EPack ep = P.epack[k];
+        const double d0 = c0 - ep.m[0], d1 = c1 - ep.m[1], d2 = c2 - ep.m[2];
+        const double* W = ep.W;
         const double qf = W[0] * d0 * d0 + W[3] * d1 * d1 + W[5] * d2 * d2 +
                           2.0 * (W[1] * d0 * d1 + W[2] * d0 * d2 + W[4] * d1 * d2);
         const double tq = W[0] * Q[0] + W[3] * Q[3] + W[5] * Q[5] +
                           2.0 * (W[1] * Q[1] + W[2] * Q[2] + W[4] * Q[4]);
-        L[j] = n * P.kappa[k] - 0.5 * nu * (tq + n * qf);
+        L[j] = n * ep.kappa - 0.5 * ep.nu * (tq + n * qf);
         mx = fmax(mx, L[j]);
       }
     }
@@ -599,7 +735,7 @@ __global__ __launch_bounds__(256) void k_pair_scatter(DevPtrs P, KOpts O) {
 __global__ __launch_bounds__(256) void k_seg_sort(DevPtrs P) {
   const long long k = (long long)blockIdx.x * 16 + (threadIdx.x >> 4);
   const int lane = threadIdx.x & 15;
-  const int f = P.gblk_frame[k >> 8];
+  const int f = P.gblk_frame[k / kGB];
   if (f < 0 || !P.fr[f].active) return;
   if (P.g_frame[k] != f) return;
   const int st = P.seg_start[k], L = P.seg_start[k + 1] - st;
@@ -612,52 +748,80 @@ __global__ __launch_bounds__(256) void k_seg_sort(DevPtrs P) {
 }
 
 // Segmented FP64 sufficient statistics over each Gaussian's significant
-// pairs, accumulated about the current mean m_k:
+// pairs (16 lanes per Gaussian), accumulated about the current mean m_k:
 //   N   = sum r W_s
 //   S1  = sum r W_s (mu_s - m_k)
 //   S2  = sum r [C_s + W_s (mu_s - m_k)(mu_s - m_k)^T]
 // so xbar = m_k + S1/N and sum w (x - xbar)(x - xbar)^T = S2 - S1 S1^T / N,
 // identical to the reference's per-voxel sums over r_sk * img_v
-// (GMEM.cu:1338-1453). Then the posterior (GMEM.cu:1601-1677) and the
-// regularisation (GMEM.cu:1693-1809) for alive, non-fixed Gaussians, and the
-// per-frame sum of alpha for the death test.
-__global__ __launch_bounds__(256) void k_mstep_update(DevPtrs P, KOpts O) {
-  __shared__ double red[4];
+// (GMEM.cu:1338-1453). Lane l takes the sorted pairs l, l+16, ...; the 16
+// partial sums are combined in a fixed butterfly order.
+__global__ __launch_bounds__(256) void k_mstep_stats(DevPtrs P) {
+  const long long k = (long long)blockIdx.x * 16 + (threadIdx.x >> 4);
+  const int lane = threadIdx.x & 15;
+  const int f = P.gblk_frame[k / kGB];
+  if (f < 0 || !P.fr[f].active) return;
+  if (P.g_frame[k] != f) return;
+  const long long G = P.Gcap, S = P.Scap;
+  const double r0 = gsv(P, GS_M, k), r1 = gsv(P, GS_M + 1, k), r2 = gsv(P, GS_M + 2, k);
+  double a[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+  const int st = P.seg_start[k], en = P.seg_start[k + 1];
+  for (int e = st + lane; e < en; e += 16) {
+    const int code = P.pairs_sorted[e];
+    const long long s = code >> kPairShift;
+    const int j = code & ((1 << kPairShift) - 1);
+    const double r = (double)P.rnk[(long long)j * S + s];
+    const double Ws = P.wmom[s];
+    if (!(Ws != 0.0)) continue;
+    const double w = r * Ws;
+    const double d0 = P.wmom[S + s] - r0, d1 = P.wmom[2 * S + s] - r1,
+                 d2 = P.wmom[3 * S + s] - r2;
+    a[0] += w;
+    a[1] += w * d0;
+    a[2] += w * d1;
+    a[3] += w * d2;
+    a[4] += r * P.wmom[4 * S + s] + w * d0 * d0;
+    a[5] += r * P.wmom[5 * S + s] + w * d0 * d1;
+    a[6] += r * P.wmom[6 * S + s] + w * d0 * d2;
+    a[7] += r * P.wmom[7 * S + s] + w * d1 * d1;
+    a[8] += r * P.wmom[8 * S + s] + w * d1 * d2;
+    a[9] += r * P.wmom[9 * S + s] + w * d2 * d2;
+  }
+#pragma unroll
+  for (int i = 0; i < 10; i++) {
+#pragma unroll
+    for (int m = 8; m >= 1; m >>= 1) a[i] += __shfl_xor(a[i], m, 16);
+  }
+  if (lane < 10) {
+    double v = a[0];
+#pragma unroll
+    for (int i = 1; i < 10; i++)
+      if (lane == i) v = a[i];
+    P.mstat[(long long)lane * G + k] = v;
+  }
+}
+
+// Posterior (GMEM.cu:1601-1677) and regularisation (GMEM.cu:1693-1809) for
+// alive, non-fixed Gaussians from the M-step statistics, and the per-frame sum
+// of alpha for the death test.
+__global__ __launch_bounds__(kGB) void k_mstep_update(DevPtrs P, KOpts O) {
+  __shared__ double red[1];
   const int f = P.gblk_frame[blockIdx.x];
   if (f < 0) return;
   FrameDev* F = P.fr + f;
   if (!F->active) return;
-  const long long k = (long long)blockIdx.x * 256 + threadIdx.x;
-  const long long G = P.Gcap, S = P.Scap;
+  const long long k = (long long)blockIdx.x * kGB + threadIdx.x;
+  const long long G = P.Gcap;
   double alpha_out = 0.0;
   if (P.g_frame[k] == f) {
     const bool alive = gsv(P, GS_M_O, k) > -1e31;
     const bool fixed = P.fixed[k] != 0;
     if (alive && !fixed) {
       const double r0 = gsv(P, GS_M, k), r1 = gsv(P, GS_M + 1, k), r2 = gsv(P, GS_M + 2, k);
-      double N = 0.0, S1[3] = {0, 0, 0}, S2[6] = {0, 0, 0, 0, 0, 0};
-      const int st = P.seg_start[k], en = P.seg_start[k + 1];
-      for (int e = st; e < en; e++) {
-        const int code = P.pairs_sorted[e];
-        const long long s = code >> kPairShift;
-        const int j = code & ((1 << kPairShift) - 1);
-        const double r = (double)P.rnk[(long long)j * S + s];
-        const double Ws = P.wmom[s];
-        if (!(Ws != 0.0)) continue;
-        const double w = r * Ws;
-        const double d0 = P.wmom[S + s] - r0, d1 = P.wmom[2 * S + s] - r1,
-                     d2 = P.wmom[3 * S + s] - r2;
-        N += w;
-        S1[0] += w * d0;
-        S1[1] += w * d1;
-        S1[2] += w * d2;
-        S2[0] += r * P.wmom[4 * S + s] + w * d0 * d0;
-        S2[1] += r * P.wmom[5 * S + s] + w * d0 * d1;
-        S2[2] += r * P.wmom[6 * S + s] + w * d0 * d2;
-        S2[3] += r * P.wmom[7 * S + s] + w * d1 * d1;
-        S2[4] += r * P.wmom[8 * S + s] + w * d1 * d2;
-        S2[5] += r * P.wmom[9 * S + s] + w * d2 * d2;
-      }
+      const double N = P.mstat[k];
+      double S1[3], S2[6];
+      for (int i = 0; i < 3; i++) S1[i] = P.mstat[(1 + i) * G + k];
+      for (int i = 0; i < 6; i++) S2[i] = P.mstat[(4 + i) * G + k];
       double xk[3] = {0, 0, 0}, Ck[6] = {0, 0, 0, 0, 0, 0};
       if (N > 0.0) {
         const double m0 = S1[0] / N, m1 = S1[1] / N, m2 = S1[2] / N;
@@ -688,14 +852,12 @@ __global__ __launch_bounds__(256) void k_mstep_update(DevPtrs P, KOpts O) {
       inv3(Wo, A);
       const double dx[3] = {xk[0] - mo[0], xk[1] - mo[1], xk[2] - mo[2]};
       const double fct = beta_o / (beta_o + N);
-      {
-        int c = 0;
-        for (int a = 0; a < 3; a++)
-          for (int b = a; b < 3; b++) {
-            A[c] += Ck[c] + N * fct * dx[a] * dx[b];
-            c++;
-          }
-      }
+      A[0] += Ck[0] + N * fct * dx[0] * dx[0];
+      A[1] += Ck[1] + N * fct * dx[0] * dx[1];
+      A[2] += Ck[2] + N * fct * dx[0] * dx[2];
+      A[3] += Ck[3] + N * fct * dx[1] * dx[1];
+      A[4] += Ck[4] + N * fct * dx[1] * dx[2];
+      A[5] += Ck[5] + N * fct * dx[2] * dx[2];
       inv3(A, W);
       regularize_W(W, nu, O.w4dof, O.scale_reg, O.lmin, O.lmax, O.emax);
       P.gs[GS_BETA * G + k] = beta;
@@ -709,21 +871,22 @@ __global__ __launch_bounds__(256) void k_mstep_update(DevPtrs P, KOpts O) {
     }
   }
   int last;
-  const double tot = frame_reduce(alpha_out, P.part_g, blockIdx.x, F->g0 / 256, F->nbg,
-                                  P.done + 4 * f + 1, red, &last);
+  const double tot = frame_reduce<1>(alpha_out, P.part_g, blockIdx.x, F->g0 / kGB, F->nbg,
+                                     P.done + 4 * f + 1, red, &last);
   if (last && threadIdx.x == 0) F->sum_alpha2 = tot;
 }
 
 // Dead-cell rule (GMEM.cu:1813-1843), then the log-likelihood parameters and
 // the per-frame sum of alpha used by the log-likelihood and by the next
-// iteration's E[ln pi].
-__global__ __launch_bounds__(256) void k_death(DevPtrs P) {
-  __shared__ double red[4];
+// iteration's E[ln pi]. The log-likelihood parameters are stored in base 2:
+// M = nu W log2(e)/2 and lg2 = (log alpha + log det(nu W)/2 - 1.5 log 2pi) log2(e).
+__global__ __launch_bounds__(kGB) void k_death(DevPtrs P) {
+  __shared__ double red[1];
   const int f = P.gblk_frame[blockIdx.x];
   if (f < 0) return;
   FrameDev* F = P.fr + f;
   if (!F->active) return;
-  const long long k = (long long)blockIdx.x * 256 + threadIdx.x;
+  const long long k = (long long)blockIdx.x * kGB + threadIdx.x;
   const long long G = P.Gcap;
   double alpha_out = 0.0;
   if (P.g_frame[k] == f) {
@@ -750,17 +913,22 @@ __global__ __launch_bounds__(256) void k_death(DevPtrs P) {
       }
       alive = false;
     }
+    const double LOG2E = 1.4426950408889634074;
     const double nu = gsv(P, GS_NU, k);
-    for (int i = 0; i < 6; i++) P.nuW[i * G + k] = (float)(nu * W[i]);
+    LLPack lp;
+    for (int i = 0; i < 3; i++) lp.m[i] = gsv(P, GS_M + i, k);
+    for (int i = 0; i < 6; i++) lp.M[i] = (float)(nu * W[i] * (0.5 * LOG2E));
+    lp.pad[0] = lp.pad[1] = 0.0f;
     const double dt = det3(W) * nu * nu * nu;
-    P.lg[k] = (alive && alpha > 0.0 && dt > 0.0)
-                  ? log(alpha) + 0.5 * log(dt) - 1.5 * 1.8378770664093454836
-                  : -INFINITY;
+    lp.lg2 = (alive && alpha > 0.0 && dt > 0.0)
+                 ? (log(alpha) + 0.5 * log(dt) - 1.5 * 1.8378770664093454836) * LOG2E
+                 : -INFINITY;
+    P.llpack[k] = lp;
     alpha_out = alpha;
   }
   int last;
-  const double tot = frame_reduce(alpha_out, P.part_g, blockIdx.x, F->g0 / 256, F->nbg,
-                                  P.done + 4 * f + 2, red, &last);
+  const double tot = frame_reduce<1>(alpha_out, P.part_g, blockIdx.x, F->g0 / kGB, F->nbg,
+                                     P.done + 4 * f + 2, red, &last);
   if (last && threadIdx.x == 0) {
     F->sum_alpha = tot;
     F->psi_sum_alpha = digamma(tot);
@@ -768,14 +936,16 @@ __global__ __launch_bounds__(256) void k_death(DevPtrs P) {
   }
 }
 
-// Log-likelihood over the voxel stream (GMEM.cu:1913-1978): one wave per
-// supervoxel, its K candidates' parameters broadcast from lanes 0..K-1.
+// Log-likelihood over the voxel stream (GMEM.cu:1913-1978). A block covers
+// 16 supervoxels, one per 16-lane group; the K candidates' parameter packs
+// are loaded by lanes 0..K-1 of the group and broadcast within it.
 // Per voxel, d = (x - a_s) + (a_s - m_k) with the anchor a_s a voxel of the
 // supervoxel (exact in f32) and (a_s - m_k) rounded once from FP64, so the
-// f32 quadratic form keeps full relative precision far from the origin;
-// log sum exp in f32, img-weighted accumulation in FP64. A voxel whose
+// f32 quadratic form keeps full relative precision far from the origin.
+// Terms are evaluated in base 2 (hardware exp2/log2), log-sum-exp with the
+// largest term factored out, img-weighted accumulation in FP64. A voxel whose
 // largest term is below the FP64 underflow of the reference's direct sum
-// (e^-744.44) contributes nothing, as in the reference.
+// (2^-1074) contributes nothing, as in the reference.
 template <int KM>
 __global__ __launch_bounds__(256) void k_loglik(DevPtrs P, KOpts O) {
   __shared__ double red[4];
@@ -784,63 +954,69 @@ __global__ __launch_bounds__(256) void k_loglik(DevPtrs P, KOpts O) {
   if (f < 0) return;
   FrameDev* F = P.fr + f;
   if (!F->active) return;
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const long long S = P.Scap, G = P.Gcap;
+  const int l16 = threadIdx.x & 15, grp = threadIdx.x >> 4;
+  const long long S = P.Scap;
   const int K = O.K;
-  const double log_sa = F->log_sum_alpha;
+  const long long s = sb + grp;
+  const bool valid = P.sv_frame[s] == f;
   double acc = 0.0;
-  for (int t = wv; t < kLLsv; t += 4) {
-    const long long s = sb + t;
-    if (P.sv_frame[s] != f) continue;
-    // lane j < K loads candidate j
-    float pd0 = 0.f, pd1 = 0.f, pd2 = 0.f, pw[6] = {0, 0, 0, 0, 0, 0}, pg = -INFINITY;
-    const float a0 = P.anchor[s], a1 = P.anchor[S + s], a2 = P.anchor[2 * S + s];
-    if (lane < K) {
-      const int k = P.ind[(long long)lane * S + s];
+  float pd0 = 0.f, pd1 = 0.f, pd2 = 0.f, pw[6] = {0, 0, 0, 0, 0, 0}, pg = -INFINITY;
+  float a0 = 0.f, a1 = 0.f, a2 = 0.f;
+  long long beg = 0;
+  int n = 0;
+  if (valid) {
+    a0 = P.anchor[s];
+    a1 = P.anchor[S + s];
+    a2 = P.anchor[2 * S + s];
+    beg = P.sv_beg[s];
+    n = P.sv_len[s];
+    if (l16 < K) {
+      const int k = P.ind[(long long)l16 * S + s];
       if (k >= 0) {
-        pd0 = (float)((double)a0 - P.gs[GS_M * G + k]);
-        pd1 = (float)((double)a1 - P.gs[(GS_M + 1) * G + k]);
-        pd2 = (float)((double)a2 - P.gs[(GS_M + 2) * G + k]);
-        for (int i = 0; i < 6; i++) pw[i] = P.nuW[i * G + k];
-        pg = (float)(P.lg[k] - log_sa);
-      }
-    }
-    float cd0[KM], cd1[KM], cd2[KM], cw[KM][6], cg[KM];
+        const LLPack lp = P.llpack[k];
+        pd0 = (float)((double)a0 - lp.m[0]);
+        pd1 = (float)((double)a1 - lp.m[1]);
+        pd2 = (float)((double)a2 - lp.m[2]);
 #pragma unroll
-    for (int j = 0; j < KM; j++) {
-      cd0[j] = __shfl(pd0, j, 64);
-      cd1[j] = __shfl(pd1, j, 64);
-      cd2[j] = __shfl(pd2, j, 64);
-#pragma unroll
-      for (int i = 0; i < 6; i++) cw[j][i] = __shfl(pw[i], j, 64);
-      cg[j] = (j < K) ? __shfl(pg, j, 64) : -INFINITY;
-    }
-    const long long beg = P.sv_beg[s];
-    const int n = P.sv_len[s];
-    for (int i = lane; i < n; i += 64) {
-      const float x0 = P.vx[beg + i] - a0, x1 = P.vy[beg + i] - a1, x2 = P.vz[beg + i] - a2;
-      const float w = P.vimg[beg + i];
-      float tj[KM];
-      float mx = -INFINITY;
-#pragma unroll
-      for (int j = 0; j < KM; j++) {
-        const float d0 = x0 + cd0[j], d1 = x1 + cd1[j], d2 = x2 + cd2[j];
-        const float q = cw[j][0] * d0 * d0 + cw[j][3] * d1 * d1 + cw[j][5] * d2 * d2 +
-                        2.0f * (cw[j][1] * d0 * d1 + cw[j][2] * d0 * d2 + cw[j][4] * d1 * d2);
-        tj[j] = cg[j] - 0.5f * q;
-        mx = fmaxf(mx, tj[j]);
-      }
-      if (mx > -744.44f) {
-        float sum = 0.0f;
-#pragma unroll
-        for (int j = 0; j < KM; j++) sum += expf(tj[j] - mx);
-        acc += (double)w * (double)(mx + logf(sum));
+        for (int i = 0; i < 6; i++) pw[i] = lp.M[i];
+        pg = (float)(lp.lg2 - F->log_sum_alpha * 1.4426950408889634074);
       }
     }
   }
+  float cd0[KM], cd1[KM], cd2[KM], cw[KM][6], cg[KM];
+#pragma unroll
+  for (int j = 0; j < KM; j++) {
+    cd0[j] = __shfl(pd0, j, 16);
+    cd1[j] = __shfl(pd1, j, 16);
+    cd2[j] = __shfl(pd2, j, 16);
+#pragma unroll
+    for (int i = 0; i < 6; i++) cw[j][i] = __shfl(pw[i], j, 16);
+    cg[j] = (j < K) ? __shfl(pg, j, 16) : -INFINITY;
+  }
+  for (int i = l16; i < n; i += 16) {
+    const float x0 = P.vx[beg + i] - a0, x1 = P.vy[beg + i] - a1, x2 = P.vz[beg + i] - a2;
+    const float w = P.vimg[beg + i];
+    float tj[KM];
+    float mx = -INFINITY;
+#pragma unroll
+    for (int j = 0; j < KM; j++) {
+      const float d0 = x0 + cd0[j], d1 = x1 + cd1[j], d2 = x2 + cd2[j];
+      const float q = cw[j][0] * d0 * d0 + cw[j][3] * d1 * d1 + cw[j][5] * d2 * d2 +
+                      2.0f * (cw[j][1] * d0 * d1 + cw[j][2] * d0 * d2 + cw[j][4] * d1 * d2);
+      tj[j] = cg[j] - q;
+      mx = fmaxf(mx, tj[j]);
+    }
+    if (mx > -1074.0f) {
+      float sum = 0.0f;
+#pragma unroll
+      for (int j = 0; j < KM; j++) sum += __builtin_amdgcn_exp2f(tj[j] - mx);
+      acc += (double)w * (double)(mx + __builtin_amdgcn_logf(sum));
+    }
+  }
+  acc *= 0.69314718055994530942;
   int last;
-  const double tot = frame_reduce(acc, P.part_s, blockIdx.x, F->s0 / kLLsv, F->nbl,
-                                  P.done + 4 * f + 3, red, &last);
+  const double tot = frame_reduce<4>(acc, P.part_s, blockIdx.x, F->s0 / kLLsv, F->nbl,
+                                     P.done + 4 * f + 3, red, &last);
   if (last && threadIdx.x == 0) {
     F->ll3 = F->ll2;
     F->ll2 = F->ll1;
@@ -857,30 +1033,38 @@ int kmax_for(int K) { return K <= 5 ? 5 : (K <= 8 ? 8 : 16); }
 static inline unsigned nb(long long n, int per) { return (unsigned)((n + per - 1) / per); }
 
 static hipError_t scan(const int* in, long long n1, int* out, int* blk, hipStream_t st) {
-  const unsigned b = nb(n1, kScanTile);
-  k_scan_partial<<<b, 256, 0, st>>>(in, n1, blk);
-  k_scan_final<<<b, 256, 0, st>>>(in, n1, blk, out);
+  if (n1 <= 65536) {
+    k_scan_single<<<1, 1024, 0, st>>>(in, n1, out);
+  } else {
+    const unsigned b = nb(n1, kScanTile);
+    k_scan_partial<<<b, 256, 0, st>>>(in, n1, blk);
+    k_scan_final<<<b, 256, 0, st>>>(in, n1, blk, out);
+  }
   return hipGetLastError();
 }
 
-hipError_t launch_prep(const DevPtrs& P, hipStream_t st) {
+hipError_t launch_prep(const DevPtrs& P, const KOpts& O, hipStream_t st) {
   k_moments<<<nb(P.Scap, 4), 256, 0, st>>>(P);
+  k_grid_setup<<<P.nframes, 1024, 0, st>>>(P, O);
+  k_qcell_count<<<nb(P.Scap, 256), 256, 0, st>>>(P, O);
+  (void)scan(P.cell_cnt, P.Ccap + 1, P.cell_start, P.scan_blk, st);
+  k_qcell_scatter<<<nb(P.Scap, 256), 256, 0, st>>>(P);
   return hipGetLastError();
 }
 
-hipError_t launch_reset(const DevPtrs& P, const KOpts& O, hipStream_t st) {
-  k_grid_setup<<<P.nframes, 256, 0, st>>>(P, O);
-  k_alpha_init<<<nb(P.Gcap, 256), 256, 0, st>>>(P);
+hipError_t launch_reset(const DevPtrs& P, hipStream_t st) {
+  k_iter_init<<<nb(P.nframes, 64), 64, 0, st>>>(P);
+  k_alpha_init<<<nb(P.Gcap, kGB), kGB, 0, st>>>(P);
   return hipGetLastError();
 }
 
 template <int KM>
 static hipError_t iteration_t(const DevPtrs& P, const KOpts& O, hipStream_t st, hipEvent_t e0,
                               hipEvent_t e1, int tk) {
-  const unsigned gb = nb(P.Gcap, 256), sb = nb(P.Scap, 256);
-  k_gauss_prep<<<gb, 256, 0, st>>>(P, O);
+  const unsigned gb = nb(P.Gcap, kGB), sb = nb(P.Scap, 256);
+  k_gauss_prep<<<gb, kGB, 0, st>>>(P, O);
   (void)scan(P.cell_cnt, P.Ccap + 1, P.cell_start, P.scan_blk, st);
-  k_grid_scatter<<<gb, 256, 0, st>>>(P);
+  k_grid_scatter<<<gb, kGB, 0, st>>>(P);
   if (tk == TK_KNN && e0) (void)hipEventRecord(e0, st);
   k_knn<KM><<<sb, 256, 0, st>>>(P, O);
   if (tk == TK_KNN && e1) (void)hipEventRecord(e1, st);
@@ -891,9 +1075,10 @@ static hipError_t iteration_t(const DevPtrs& P, const KOpts& O, hipStream_t st, 
   k_pair_scatter<<<sb, 256, 0, st>>>(P, O);
   k_seg_sort<<<nb(P.Gcap, 16), 256, 0, st>>>(P);
   if (tk == TK_MSTEP && e0) (void)hipEventRecord(e0, st);
-  k_mstep_update<<<gb, 256, 0, st>>>(P, O);
+  k_mstep_stats<<<nb(P.Gcap, 16), 256, 0, st>>>(P);
+  k_mstep_update<<<gb, kGB, 0, st>>>(P, O);
   if (tk == TK_MSTEP && e1) (void)hipEventRecord(e1, st);
-  k_death<<<gb, 256, 0, st>>>(P);
+  k_death<<<gb, kGB, 0, st>>>(P);
   if (tk == TK_LOGLIK && e0) (void)hipEventRecord(e0, st);
   k_loglik<KM><<<nb(P.Scap, kLLsv), 256, 0, st>>>(P, O);
   if (tk == TK_LOGLIK && e1) (void)hipEventRecord(e1, st);

@@ -92,24 +92,25 @@ int ws_alloc(Workspace* w, const WsShape& sh, int K) {
   AL(P.ind, (long long)K * S);
   AL(P.rnk, (long long)K * S);
   AL(P.sblk_frame, S / kAlign);
+  AL(P.qperm, S);
+  AL(P.qcell, S);
   AL(P.g_frame, G);
   AL(P.gs, GS_NFIELDS * G);
   AL(P.fixed, G);
-  AL(P.kappa, G);
-  AL(P.nuW, 6 * G);
-  AL(P.lg, G);
+  AL(P.epack, G);
+  AL(P.llpack, G);
   AL(P.mf, 3 * G);
   AL(P.gcell, G);
-  AL(P.gblk_frame, G / kAlign);
+  AL(P.gblk_frame, G / kGB);
   AL(P.seg_cnt, G + 1);
   AL(P.seg_start, G + 1);
   AL(P.pairs, (long long)K * S);
   AL(P.pairs_sorted, (long long)K * S);
+  AL(P.mstat, 10 * G);
   AL(P.cell_cnt, C + 1);
   AL(P.cell_start, C + 1);
-  AL(P.cg, 3 * G);
-  AL(P.cg_idx, G);
-  AL(P.part_g, G / kAlign);
+  AL(P.cg4, G);
+  AL(P.part_g, G / kGB);
   AL(P.part_s, S / kLLsv);
   AL(P.done, 4 * F);
   AL(P.scan_blk, (std::max(C, G) + 1 + kScanTile - 1) / kScanTile + 1);
@@ -224,8 +225,10 @@ int batch_build(int device, const std::vector<FrameSrc>& fs, const tgmm_em_opts&
     F.ng = f.G;
     F.c0 = c;
     F.ncap = std::max<long long>(8, 2LL * f.G);
-    F.nbg = (int)((f.G + 255) / 256);
-    F.nbl = (int)((f.S + kLLsv - 1) / kLLsv);
+    // every block mapped to the frame's 256-aligned ranges takes part in the
+    // per-frame reductions, including trailing blocks with no valid element
+    F.nbg = (int)(round_up(std::max<long long>(f.G, 1), kAlign) / kGB);
+    F.nbl = (int)(round_up(std::max<long long>(f.S, 1), kAlign) / kLLsv);
     F.max_iter = o.max_iter;
     F.tol = o.tol;
     v += f.N;
@@ -279,7 +282,7 @@ int batch_upload(tgmm_batch* b, const std::vector<FrameSrc>& fs) {
   const long long S = sh.Scap, G = sh.Gcap;
   std::vector<int> sv_frame(S, -1), sv_len(S, 0), g_frame(G, -1);
   std::vector<long long> sv_beg(S, 0);
-  std::vector<int> sblk(S / kAlign, -1), gblk(G / kAlign, -1);
+  std::vector<int> sblk(S / kAlign, -1), gblk(G / kGB, -1);
   std::vector<double> gs((size_t)GS_NFIELDS * G, 0.0);
   std::vector<unsigned char> fixed(G, 0);
   for (size_t i = 0; i < fs.size(); i++) {
@@ -301,8 +304,8 @@ int batch_upload(tgmm_batch* b, const std::vector<FrameSrc>& fs) {
     }
     for (long long j = 0; j < round_up(std::max<long long>(f.S, 1), kAlign) / kAlign; j++)
       sblk[F.s0 / kAlign + j] = (int)i;
-    for (long long j = 0; j < round_up(std::max<long long>(f.G, 1), kAlign) / kAlign; j++)
-      gblk[F.g0 / kAlign + j] = (int)i;
+    for (long long j = 0; j < round_up(std::max<long long>(f.G, 1), kAlign) / kGB; j++)
+      gblk[F.g0 / kGB + j] = (int)i;
     for (long long k = 0; k < f.G; k++) g_frame[F.g0 + k] = (int)i;
     if (f.G > 0) records_to_soa(f.gm, f.G, F.g0, G, gs.data(), fixed.data());
   }
@@ -336,8 +339,8 @@ int batch_upload(tgmm_batch* b, const std::vector<FrameSrc>& fs) {
   HIPCHK(hipMemsetAsync(P.seg_cnt, 0, (G + 1) * 4, st));
   HIPCHK(hipMemsetAsync(P.cell_cnt, 0, (sh.Ccap + 1) * 4, st));
   HIPCHK(hipMemsetAsync(P.done, 0, 4 * sh.Fcap * 4, st));
-  HIPCHK(launch_prep(P, st));
-  HIPCHK(launch_reset(P, b->ko, st));
+  HIPCHK(launch_prep(P, b->ko, st));
+  HIPCHK(launch_reset(P, st));
   HIPCHK(hipStreamSynchronize(st));
   // keep the host copies the fetch needs
   b->fixed_host = fixed;
@@ -381,7 +384,7 @@ extern "C" int tgmm_batch_reset(tgmm_batch* b) {
   DevPtrs& P = b->ws.P;
   HIPCHK(hipMemcpyAsync(P.gs, b->ws.gs_init, (size_t)GS_NFIELDS * b->ws.sh.Gcap * 8,
                         hipMemcpyDeviceToDevice, b->st));
-  HIPCHK(launch_reset(P, b->ko, b->st));
+  HIPCHK(launch_reset(P, b->st));
   HIPCHK(hipStreamSynchronize(b->st));
   return TGMM_OK;
 }
