@@ -45,3 +45,21 @@ def dup_gaussians(frame, offset=1.5, seed=7):
     f2 = copy.copy(frame)
     f2.gm = out
     return f2
+
+
+def next_priors(gm, beta_pct, nu_pct, alpha_pct, alpha_total):
+    """A stream's next-frame Gaussians from the previous posterior:
+    GaussianMixtureModel::updatePriors (GaussianMixtureModel.cpp:892-910) with
+    N_k = nu_k - nu_o, then nu_k *= 2 and W_k /= 2 (main.cpp:961-969)."""
+    g = gm.copy()
+    nk = g["nu_k"] - g["nu_o"]
+    g["m_o"] = g["m_k"]
+    g["beta_o"] = np.maximum(beta_pct * nk, 1e-5)
+    nu_o = np.maximum(nu_pct * nk, 3.0)
+    g["W_o"] = g["W_k"] * g["nu_k"][:, None] / nu_o[:, None]
+    g["nu_o"] = nu_o
+    if alpha_total >= 0:
+        g["alpha_o"] = alpha_total * 1e-5 * alpha_pct
+    g["nu_k"] = g["nu_k"] * 2.0
+    g["W_k"] = g["W_k"] / 2.0
+    return g

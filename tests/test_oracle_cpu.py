@@ -354,3 +354,24 @@ def test_oracle_em_config1_recovers_cells():
     err = np.abs(gm["m_k"] - f.true_centres)
     assert np.median(err) < 0.3  # means move from ~1.6 voxel noise to the cells
     assert (gm["m_o"][:, 0] > -1e31).all()
+
+
+def test_next_priors_helper_vs_reference_twin_golden():
+    """The host restatement of updatePriors used to check the server's
+    device-side prior propagation, against the reference's CPU code."""
+    from helpers import next_priors
+    st, pri, pct = GOLD["st"], GOLD["pri"], GOLD["pct"]
+    for i in range(st.shape[0]):
+        g = np.zeros(1, GMM_DTYPE)
+        g["beta_k"], g["nu_k"], g["alpha_k"] = st[i, 0], st[i, 1], st[i, 2]
+        g["m_k"] = st[i, 3:6]
+        g["W_k"] = st[i, 6:12]
+        g["nu_o"] = st[i, 1] - st[i, 12]  # so that nu_k - nu_o = N_k
+        g["alpha_o"] = st[i, 13]
+        out = next_priors(g, pct[0], pct[1], pct[2], 1000.0 + i)
+        r = pri[i]
+        assert abs(out["beta_o"][0] - r[0]) <= 1e-12 * abs(r[0])
+        assert abs(out["nu_o"][0] - r[1]) <= 1e-12 * abs(r[1])
+        assert abs(out["alpha_o"][0] - r[2]) <= 1e-12 * abs(r[2])
+        assert np.array_equal(out["m_o"][0], r[3:6])
+        assert np.allclose(out["W_o"][0], r[6:12], rtol=1e-12, atol=0)

@@ -122,6 +122,7 @@ struct DevPtrs {
   long long Scap, Gcap, Ccap, Vcap;
   int Fcap;
   int nframes;
+  int only_frame;   // prep/reset kernels: -1 = every frame, else this frame
 };
 
 // kernel launchers (tgmm_kernels.hip); all enqueue on `st`
@@ -134,6 +135,14 @@ hipError_t launch_iteration(const DevPtrs& P, const KOpts& O, hipStream_t st,
                             hipEvent_t ev_before = nullptr, hipEvent_t ev_after = nullptr,
                             int timed_kernel = -1);
 int kmax_for(int K);
+
+// Priors of a stream's next frame from its previous posterior, in place on
+// the Gaussian range [g0, g0+ng) (GaussianMixtureModel::updatePriors,
+// GaussianMixtureModel.cpp:892-910, then nu_k *= 2, W_k /= 2 as the tracker
+// does, main.cpp:961-969).
+hipError_t launch_next_priors(const DevPtrs& P, long long g0, long long ng, double beta_pct,
+                              double nu_pct, double alpha_pct, double alpha_total,
+                              hipStream_t st);
 
 // timed_kernel ids for launch_iteration
 enum { TK_KNN = 0, TK_ESTEP = 1, TK_MSTEP = 2, TK_LOGLIK = 3 };

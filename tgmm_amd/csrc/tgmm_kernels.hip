@@ -136,7 +136,7 @@ __global__ __launch_bounds__(256) void k_moments(DevPtrs P) {
   const int lane = threadIdx.x & 63;
   if (s >= P.Scap) return;
   const int f = P.sv_frame[s];
-  if (f < 0) return;
+  if (f < 0 || (P.only_frame >= 0 && f != P.only_frame)) return;
   const long long beg = P.sv_beg[s];
   const int n = P.sv_len[s];
   const long long S = P.Scap;
@@ -215,8 +215,8 @@ __global__ __launch_bounds__(256) void k_moments(DevPtrs P) {
 // One block per frame: bounding box of the scaled kNN centroids and the grid
 // shape (about two means per cell, at most ncap cells).
 __global__ __launch_bounds__(1024) void k_grid_setup(DevPtrs P, KOpts O) {
-  const int f = blockIdx.x;
-  if (f >= P.nframes) return;
+  const int f = (P.only_frame >= 0) ? P.only_frame : (int)blockIdx.x;
+  if (f >= P.Fcap || (P.only_frame < 0 && f >= P.nframes)) return;
   FrameDev* F = P.fr + f;
   const long long s0 = F->s0, ns = F->ns;
   const long long S = P.Scap;
@@ -289,7 +289,7 @@ __global__ __launch_bounds__(1024) void k_grid_setup(DevPtrs P, KOpts O) {
 // that neighbouring lanes of the kNN kernel walk the same cells.
 __global__ __launch_bounds__(256) void k_qcell_count(DevPtrs P, KOpts O) {
   const int f = P.sblk_frame[blockIdx.x];
-  if (f < 0) return;
+  if (f < 0 || (P.only_frame >= 0 && f != P.only_frame)) return;
   const long long s = (long long)blockIdx.x * 256 + threadIdx.x;
   if (P.sv_frame[s] != f) return;
   const FrameDev* F = P.fr + f;
@@ -305,7 +305,7 @@ __global__ __launch_bounds__(256) void k_qcell_count(DevPtrs P, KOpts O) {
 
 __global__ __launch_bounds__(256) void k_qcell_scatter(DevPtrs P) {
   const int f = P.sblk_frame[blockIdx.x];
-  if (f < 0) return;
+  if (f < 0 || (P.only_frame >= 0 && f != P.only_frame)) return;
   const long long s = (long long)blockIdx.x * 256 + threadIdx.x;
   if (P.sv_frame[s] != f) return;
   const FrameDev* F = P.fr + f;
@@ -318,8 +318,9 @@ __global__ __launch_bounds__(256) void k_qcell_scatter(DevPtrs P) {
 // Iteration state of every frame (reference: ll = -1, -10, -100, -1000 and
 // numIterEM = 1 before the loop, GMEM.cu:4143-4144).
 __global__ void k_iter_init(DevPtrs P) {
-  const int f = blockIdx.x * blockDim.x + threadIdx.x;
-  if (f >= P.nframes) return;
+  const int f = (P.only_frame >= 0) ? (threadIdx.x == 0 && blockIdx.x == 0 ? P.only_frame : -1)
+                                    : (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (f < 0 || f >= P.Fcap || (P.only_frame < 0 && f >= P.nframes)) return;
   FrameDev* F = P.fr + f;
   F->ll = -1.0;
   F->ll1 = -10.0;
@@ -337,7 +338,7 @@ __global__ void k_iter_init(DevPtrs P) {
 __global__ __launch_bounds__(kGB) void k_alpha_init(DevPtrs P) {
   __shared__ double red[1];
   const int f = P.gblk_frame[blockIdx.x];
-  if (f < 0) return;
+  if (f < 0 || (P.only_frame >= 0 && f != P.only_frame)) return;
   FrameDev* F = P.fr + f;
   if (!F->active) return;
   const long long k = (long long)blockIdx.x * kGB + threadIdx.x;
@@ -1045,7 +1046,7 @@ static hipError_t scan(const int* in, long long n1, int* out, int* blk, hipStrea
 
 hipError_t launch_prep(const DevPtrs& P, const KOpts& O, hipStream_t st) {
   k_moments<<<nb(P.Scap, 4), 256, 0, st>>>(P);
-  k_grid_setup<<<P.nframes, 1024, 0, st>>>(P, O);
+  k_grid_setup<<<P.only_frame >= 0 ? 1 : P.nframes, 1024, 0, st>>>(P, O);
   k_qcell_count<<<nb(P.Scap, 256), 256, 0, st>>>(P, O);
   (void)scan(P.cell_cnt, P.Ccap + 1, P.cell_start, P.scan_blk, st);
   k_qcell_scatter<<<nb(P.Scap, 256), 256, 0, st>>>(P);
@@ -1055,6 +1056,37 @@ hipError_t launch_prep(const DevPtrs& P, const KOpts& O, hipStream_t st) {
 hipError_t launch_reset(const DevPtrs& P, hipStream_t st) {
   k_iter_init<<<nb(P.nframes, 64), 64, 0, st>>>(P);
   k_alpha_init<<<nb(P.Gcap, kGB), kGB, 0, st>>>(P);
+  return hipGetLastError();
+}
+
+// GaussianMixtureModel::updatePriors (GaussianMixtureModel.cpp:892-910) with
+// N_k = nu_k - nu_o (GaussianMixtureModel.h:169), then the tracker's
+// nu_k *= 2, W_k /= 2 (main.cpp:961-969).
+__global__ void k_next_priors(DevPtrs P, long long g0, long long ng, double bpct, double npct,
+                              double apct, double alpha_total) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= ng) return;
+  const long long k = g0 + i;
+  const long long G = P.Gcap;
+  double* gs = P.gs;
+  const double nu_k = gs[GS_NU * G + k], nu_o = gs[GS_NU_O * G + k];
+  const double Nk = nu_k - nu_o;
+  for (int a = 0; a < 3; a++) gs[(GS_M_O + a) * G + k] = gs[(GS_M + a) * G + k];
+  gs[GS_BETA_O * G + k] = fmax(bpct * Nk, 1e-5);
+  const double nu_o_new = fmax(npct * Nk, 3.0);
+  gs[GS_NU_O * G + k] = nu_o_new;
+  for (int a = 0; a < 6; a++) gs[(GS_W_O + a) * G + k] = gs[(GS_W + a) * G + k] * nu_k / nu_o_new;
+  if (alpha_total >= 0.0) gs[GS_ALPHA_O * G + k] = alpha_total * 1e-5 * apct;
+  gs[GS_NU * G + k] = nu_k * 2.0;
+  for (int a = 0; a < 6; a++) gs[(GS_W + a) * G + k] /= 2.0;
+}
+
+hipError_t launch_next_priors(const DevPtrs& P, long long g0, long long ng, double beta_pct,
+                              double nu_pct, double alpha_pct, double alpha_total,
+                              hipStream_t st) {
+  if (ng <= 0) return hipSuccess;
+  k_next_priors<<<nb(ng, 256), 256, 0, st>>>(P, g0, ng, beta_pct, nu_pct, alpha_pct,
+                                             alpha_total);
   return hipGetLastError();
 }
 

@@ -254,6 +254,7 @@ int batch_build(int device, const std::vector<FrameSrc>& fs, const tgmm_em_opts&
   }
   DevPtrs& P = b->ws.P;
   P.nframes = (int)fs.size();
+  P.only_frame = -1;
   if (stream) {
     b->st = stream;
     b->own_stream = false;

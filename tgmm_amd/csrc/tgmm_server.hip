@@ -1,18 +1,587 @@
-// tgmm_server.hip -- continuous-batching server (placeholder until built).
+// tgmm_server.hip -- continuous-batching server for concurrent time-lapse
+// streams (no reference counterpart: the reference solves one frame at a
+// time, main.cpp:387, 1100).
+//
+// One device workspace of fixed capacity, sized from an HBM budget, holds
+// every in-flight frame. Each scheduling step
+//   1. retires frames whose EM has stopped (results copied to the host),
+//   2. admits queued frames whose voxel, supervoxel, Gaussian and grid-cell
+//      ranges fit (first fit), in arrival order, one frame per stream at a time,
+//   3. replays one captured EM iteration over every in-flight frame.
+// Frames join and leave at iteration boundaries, so requests that arrive
+// while others are on the GPU start as soon as capacity frees up and their
+// iterations share launches with the frames already running. A stream's
+// posterior Gaussians stay resident in the workspace and become, in place,
+// the priors of the stream's next frame.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <vector>
+
 #include "../../include/tgmm_em.h"
 #include "tgmm_runtime_internal.h"
 
-extern "C" int tgmm_server_create(int, const tgmm_server_opts*, tgmm_server**) {
-  tgmm_set_error("server not built yet");
-  return TGMM_ERR_ARG;
+using namespace tgmm;
+
+#define HIPCHK(x)                                                        \
+  do {                                                                   \
+    hipError_t e_ = (x);                                                 \
+    if (e_ != hipSuccess) {                                              \
+      tgmm_set_error("%s failed: %s", #x, hipGetErrorString(e_));        \
+      return e_ == hipErrorOutOfMemory ? TGMM_ERR_OOM : TGMM_ERR_DEVICE; \
+    }                                                                    \
+  } while (0)
+
+namespace {
+
+double now_s() {
+  using namespace std::chrono;
+  return duration<double>(steady_clock::now().time_since_epoch()).count();
 }
-extern "C" int tgmm_server_submit(tgmm_server*, int, const tgmm_frame_desc*, long long*) {
-  return TGMM_ERR_ARG;
+
+long long rup(long long x, long long a) { return ((x + a - 1) / a) * a; }
+
+// first-fit allocator of aligned ranges in [0, cap)
+class RangeAlloc {
+ public:
+  void init(long long cap) {
+    free_.clear();
+    if (cap > 0) free_[0] = cap;
+  }
+  bool alloc(long long n, long long align, long long* out) {
+    if (n <= 0) {
+      *out = 0;
+      return true;
+    }
+    for (auto it = free_.begin(); it != free_.end(); ++it) {
+      const long long st = rup(it->first, align);
+      const long long end = it->first + it->second;
+      if (st + n <= end) {
+        const long long a = it->first, b = end;
+        free_.erase(it);
+        if (st > a) free_[a] = st - a;
+        if (st + n < b) free_[st + n] = b - (st + n);
+        *out = st;
+        return true;
+      }
+    }
+    return false;
+  }
+  void release(long long st, long long n) {
+    if (n <= 0) return;
+    auto it = free_.emplace(st, n).first;
+    auto nx = std::next(it);
+    if (nx != free_.end() && it->first + it->second == nx->first) {
+      it->second += nx->second;
+      free_.erase(nx);
+    }
+    if (it != free_.begin()) {
+      auto pv = std::prev(it);
+      if (pv->first + pv->second == it->first) {
+        pv->second += it->second;
+        free_.erase(it);
+      }
+    }
+  }
+
+ private:
+  std::map<long long, long long> free_;
+};
+
+struct Ticket {
+  long long id = 0;
+  int stream = -1;
+  long long N = 0, S = 0, G = 0;
+  std::vector<float> query, img, cent;
+  std::vector<long long> ptr;
+  std::vector<GaussianMixtureModelCUDA> gm_in;  // empty: priors from the stream
+  int state = 0;                                // 0 queued, 1 running, 2 done
+  double t_submit = 0, t_admit = 0, t_done = 0;
+  int slot = -1;
+  long long v0 = 0, s0 = 0, sa = 0, g0 = 0, ga = 0, c0 = 0, ncap = 0;
+  bool g_resident = false;
+  std::vector<GaussianMixtureModelCUDA> gm_out;
+  std::vector<float> rnk;
+  std::vector<int> ind;
+  double ll = 0;
+  int iters = 0;
+  int status = TGMM_OK;
+};
+
+struct Stream {
+  long long g0 = -1, ga = 0, G = 0;  // resident Gaussian range
+  long long G_next = -1;             // Gaussians of the last submitted frame
+  double alpha_total = -1.0;
+  bool busy = false;                 // a frame of this stream is in flight
+};
+
+}  // namespace
+
+struct tgmm_server {
+  int device = 0;
+  hipStream_t st = nullptr;
+  tgmm_server_opts o;
+  KOpts ko;
+  Workspace ws;
+  RangeAlloc av, as, ag, ac;
+  std::vector<int> slot_ticket;
+  std::vector<FrameDev> fh;
+  std::map<long long, Ticket> tickets;
+  std::deque<long long> queue;
+  std::map<int, Stream> streams;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  long long next_id = 0;
+  int in_flight = 0;
+};
+
+static int server_capture(tgmm_server* s) {
+  if (s->gexec) return TGMM_OK;
+  HIPCHK(hipStreamBeginCapture(s->st, hipStreamCaptureModeThreadLocal));
+  hipError_t le = launch_iteration(s->ws.P, s->ko, s->st);
+  hipGraph_t g = nullptr;
+  hipError_t ee = hipStreamEndCapture(s->st, &g);
+  if (le != hipSuccess || ee != hipSuccess) {
+    if (g) (void)hipGraphDestroy(g);
+    tgmm_set_error("server graph capture failed");
+    return TGMM_ERR_DEVICE;
+  }
+  s->graph = g;
+  HIPCHK(hipGraphInstantiate(&s->gexec, g, nullptr, nullptr, 0));
+  return TGMM_OK;
 }
-extern "C" int tgmm_server_wait(tgmm_server*, long long) { return TGMM_ERR_ARG; }
-extern "C" int tgmm_server_step(tgmm_server*, int*) { return TGMM_ERR_ARG; }
-extern "C" int tgmm_server_fetch(tgmm_server*, long long, tgmm_frame_desc*, double*, double*,
-                                 double*) {
-  return TGMM_ERR_ARG;
+
+extern "C" int tgmm_server_create(int device, const tgmm_server_opts* opts, tgmm_server** out) {
+  if (!opts || !out || opts->max_frames_in_flight < 1 || opts->hbm_budget_bytes <= 0) {
+    tgmm_set_error("tgmm_server_create: bad arguments");
+    return TGMM_ERR_ARG;
+  }
+  const tgmm_em_opts& e = opts->em;
+  if (e.K < 1 || e.K > TGMM_KMAX || e.max_iter < 0 || e.lambda_min < 0) {
+    tgmm_set_error("tgmm_server_create: bad EM options");
+    return e.lambda_min < 0 ? TGMM_ERR_NOT_SET : TGMM_ERR_ARG;
+  }
+  HIPCHK(hipSetDevice(device));
+  tgmm_server* s = new tgmm_server();
+  s->device = device;
+  s->o = *opts;
+  s->ko = make_kopts(e, nullptr);
+  // capacity from the budget: supervoxels >= 8 voxels and Gaussians >= 32
+  // voxels on average, two grid cells per Gaussian plus slack per slot
+  WsShape sh;
+  sh.Fcap = opts->max_frames_in_flight;
+  long long V = 1 << 20;
+  for (;;) {
+    WsShape t;
+    t.Fcap = sh.Fcap;
+    const long long V2 = V * 2;
+    t.Vcap = V2;
+    t.Scap = rup(V2 / 8 + 256LL * t.Fcap, kAlign);
+    t.Gcap = rup(V2 / 32 + 256LL * t.Fcap, kAlign);
+    t.Ccap = 2 * t.Gcap + 8LL * t.Fcap;
+    if (ws_bytes_estimate(t, e.K) > opts->hbm_budget_bytes || V2 > (1LL << 31)) break;
+    V = V2;
+  }
+  sh.Vcap = V;
+  sh.Scap = rup(V / 8 + 256LL * sh.Fcap, kAlign);
+  sh.Gcap = rup(V / 32 + 256LL * sh.Fcap, kAlign);
+  sh.Ccap = 2 * sh.Gcap + 8LL * sh.Fcap;
+  if (int rc = ws_alloc(&s->ws, sh, e.K)) {
+    ws_free(&s->ws);
+    delete s;
+    return rc;
+  }
+  DevPtrs& P = s->ws.P;
+  P.nframes = sh.Fcap;
+  P.only_frame = -1;
+  if (hipStreamCreateWithFlags(&s->st, hipStreamNonBlocking) != hipSuccess) {
+    ws_free(&s->ws);
+    delete s;
+    tgmm_set_error("hipStreamCreate failed");
+    return TGMM_ERR_DEVICE;
+  }
+  s->av.init(sh.Vcap);
+  s->as.init(sh.Scap);
+  s->ag.init(sh.Gcap);
+  s->ac.init(sh.Ccap);
+  s->slot_ticket.assign(sh.Fcap, -1);
+  s->fh.assign(sh.Fcap, FrameDev());
+  std::memset(s->fh.data(), 0, sizeof(FrameDev) * sh.Fcap);
+  const long long S = sh.Scap, G = sh.Gcap;
+  auto fail = [&](hipError_t e2) {
+    tgmm_set_error("server init: %s", hipGetErrorString(e2));
+    tgmm_server_destroy(s);
+    return TGMM_ERR_DEVICE;
+  };
+  hipError_t er;
+  if ((er = hipMemsetAsync(P.sv_frame, 0xFF, S * 4, s->st)) ||
+      (er = hipMemsetAsync(P.sblk_frame, 0xFF, (S / kAlign) * 4, s->st)) ||
+      (er = hipMemsetAsync(P.g_frame, 0xFF, G * 4, s->st)) ||
+      (er = hipMemsetAsync(P.gblk_frame, 0xFF, (G / kGB) * 4, s->st)) ||
+      (er = hipMemsetAsync(P.ind, 0xFF, (size_t)e.K * S * 4, s->st)) ||
+      (er = hipMemsetAsync(P.rnk, 0, (size_t)e.K * S * 4, s->st)) ||
+      (er = hipMemsetAsync(P.seg_cnt, 0, (G + 1) * 4, s->st)) ||
+      (er = hipMemsetAsync(P.cell_cnt, 0, (sh.Ccap + 1) * 4, s->st)) ||
+      (er = hipMemsetAsync(P.done, 0, 4 * sh.Fcap * 4, s->st)) ||
+      (er = hipMemcpyAsync(P.fr, s->fh.data(), sh.Fcap * sizeof(FrameDev),
+                           hipMemcpyHostToDevice, s->st)) ||
+      (er = hipStreamSynchronize(s->st)))
+    return fail(er);
+  *out = s;
+  return TGMM_OK;
 }
-extern "C" int tgmm_server_destroy(tgmm_server*) { return TGMM_ERR_ARG; }
+
+extern "C" int tgmm_server_submit(tgmm_server* s, int stream_id, const tgmm_frame_desc* d,
+                                  long long* ticket) {
+  if (!s || !d || !ticket || d->n_voxels < 0 || d->n_supervoxels < 0 || d->n_gaussians < 0 ||
+      !d->sv_ptr || (d->n_voxels > 0 && (!d->query || !d->img)) ||
+      (d->n_supervoxels > 0 && !d->centroids)) {
+    tgmm_set_error("tgmm_server_submit: bad arguments");
+    return TGMM_ERR_ARG;
+  }
+  Ticket t;
+  t.id = s->next_id++;
+  t.stream = stream_id;
+  t.N = d->n_voxels;
+  t.S = d->n_supervoxels;
+  t.query.assign(d->query, d->query + 3 * t.N);
+  t.img.assign(d->img, d->img + t.N);
+  t.cent.assign(d->centroids, d->centroids + 3 * t.S);
+  t.ptr.assign(d->sv_ptr, d->sv_ptr + t.S + 1);
+  for (long long j = 0; j < t.S; j++)
+    if (t.ptr[j] < 0 || t.ptr[j + 1] < t.ptr[j] || t.ptr[j + 1] > t.N) {
+      tgmm_set_error("tgmm_server_submit: supervoxel pointer array not monotone in [0, N]");
+      return TGMM_ERR_ARG;
+    }
+  if (d->gm) {
+    t.G = d->n_gaussians;
+    t.gm_in.assign(d->gm, d->gm + t.G);
+  } else {
+    auto it = s->streams.find(stream_id);
+    if (stream_id < 0 || it == s->streams.end() || it->second.G_next < 0) {
+      tgmm_set_error("tgmm_server_submit: no Gaussians given and no earlier frame of the stream");
+      return TGMM_ERR_ARG;
+    }
+    t.G = it->second.G_next;  // priors come from the stream's previous posterior
+  }
+  if (stream_id >= 0) s->streams[stream_id].G_next = t.G;
+  const WsShape& sh = s->ws.sh;
+  if (t.N > sh.Vcap || rup(std::max<long long>(t.S, 1), kAlign) > sh.Scap ||
+      rup(std::max<long long>(t.G, 1), kAlign) > sh.Gcap) {
+    tgmm_set_error("frame exceeds the server's capacity (N=%lld S=%lld G=%lld)", t.N, t.S, t.G);
+    return TGMM_ERR_LIMIT;
+  }
+  t.t_submit = now_s();
+  s->queue.push_back(t.id);
+  s->tickets.emplace(t.id, std::move(t));
+  *ticket = t.id;
+  return TGMM_OK;
+}
+
+// Map the frame's ranges to slot f (or -1) in the per-element frame tables.
+static int map_ranges(tgmm_server* s, const Ticket& t, int f) {
+  DevPtrs& P = s->ws.P;
+  hipStream_t st = s->st;
+  std::vector<int> sv(t.sa, f), sb(t.sa / kAlign, f), gv(t.ga, f), gb(t.ga / kGB, f);
+  if (f >= 0) {
+    for (long long j = t.S; j < t.sa; j++) sv[j] = -1;
+    for (long long j = t.G; j < t.ga; j++) gv[j] = -1;
+  }
+  HIPCHK(hipMemcpyAsync(P.sv_frame + t.s0, sv.data(), t.sa * 4, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(P.sblk_frame + t.s0 / kAlign, sb.data(), sb.size() * 4,
+                        hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(P.g_frame + t.g0, gv.data(), t.ga * 4, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(P.gblk_frame + t.g0 / kGB, gb.data(), gb.size() * 4,
+                        hipMemcpyHostToDevice, st));
+  HIPCHK(hipStreamSynchronize(st));  // host vectors go out of scope
+  return TGMM_OK;
+}
+
+static int admit(tgmm_server* s, Ticket& t, int f) {
+  DevPtrs& P = s->ws.P;
+  hipStream_t st = s->st;
+  const long long S = s->ws.sh.Scap, G = s->ws.sh.Gcap;
+  // voxels and supervoxels
+  if (t.N > 0) {
+    HIPCHK(hipMemcpyAsync(P.vx + t.v0, t.query.data(), t.N * 4, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(P.vy + t.v0, t.query.data() + t.N, t.N * 4, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(P.vz + t.v0, t.query.data() + 2 * t.N, t.N * 4, hipMemcpyHostToDevice,
+                          st));
+    HIPCHK(hipMemcpyAsync(P.vimg + t.v0, t.img.data(), t.N * 4, hipMemcpyHostToDevice, st));
+  }
+  std::vector<long long> beg(std::max<long long>(t.S, 1));
+  std::vector<int> len(std::max<long long>(t.S, 1));
+  for (long long j = 0; j < t.S; j++) {
+    beg[j] = t.v0 + t.ptr[j];
+    len[j] = (int)(t.ptr[j + 1] - t.ptr[j]);
+  }
+  if (t.S > 0) {
+    for (int a = 0; a < 3; a++)
+      HIPCHK(hipMemcpyAsync(P.sv_cent + a * S + t.s0, t.cent.data() + a * t.S, t.S * 4,
+                            hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(P.sv_beg + t.s0, beg.data(), t.S * 8, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(P.sv_len + t.s0, len.data(), t.S * 4, hipMemcpyHostToDevice, st));
+  }
+  // Gaussians: given records, or the stream's posterior turned into priors
+  Stream& sr = s->streams[t.stream];
+  if (!t.gm_in.empty()) {
+    std::vector<double> gs((size_t)GS_NFIELDS * t.G);
+    std::vector<unsigned char> fx(t.G);
+    records_to_soa(t.gm_in.data(), t.G, 0, t.G, gs.data(), fx.data());
+    for (int fi = 0; fi < GS_NFIELDS; fi++)
+      HIPCHK(hipMemcpyAsync(P.gs + fi * G + t.g0, gs.data() + fi * t.G, t.G * 8,
+                            hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(P.fixed + t.g0, fx.data(), t.G, hipMemcpyHostToDevice, st));
+    HIPCHK(hipStreamSynchronize(st));
+  } else {
+    HIPCHK(launch_next_priors(P, t.g0, t.G, s->o.beta_percentage_of_nk,
+                              s->o.nu_percentage_of_nk, s->o.alpha_percentage, sr.alpha_total,
+                              st));
+  }
+  // frame descriptor
+  FrameDev& F = s->fh[f];
+  std::memset(&F, 0, sizeof(F));
+  F.v0 = t.v0;
+  F.nv = t.N;
+  F.s0 = t.s0;
+  F.ns = t.S;
+  F.g0 = t.g0;
+  F.ng = t.G;
+  F.c0 = t.c0;
+  F.ncap = t.ncap;
+  F.nbg = (int)(t.ga / kGB);
+  F.nbl = (int)(t.sa / kLLsv);
+  F.max_iter = s->o.em.max_iter;
+  F.tol = s->o.em.tol;
+  HIPCHK(hipMemcpyAsync(P.fr + f, &F, sizeof(FrameDev), hipMemcpyHostToDevice, st));
+  if (int rc = map_ranges(s, t, f)) return rc;
+  DevPtrs Q = P;
+  Q.only_frame = f;
+  HIPCHK(launch_prep(Q, s->ko, st));
+  HIPCHK(launch_reset(Q, st));
+  t.slot = f;
+  t.state = 1;
+  t.t_admit = now_s();
+  s->slot_ticket[f] = (int)t.id;
+  s->in_flight++;
+  sr.busy = true;
+  return TGMM_OK;
+}
+
+static int try_admit(tgmm_server* s) {
+  for (auto it = s->queue.begin(); it != s->queue.end();) {
+    if (s->in_flight >= s->ws.sh.Fcap) break;
+    Ticket& t = s->tickets[*it];
+    Stream& sr = s->streams[t.stream];
+    // frames of one stream run one after another (each needs the previous
+    // posterior); frames from other streams may pass it
+    bool ahead = false;
+    for (auto jt = s->queue.begin(); jt != it; ++jt)
+      if (s->tickets[*jt].stream == t.stream && t.stream >= 0) ahead = true;
+    if ((t.stream >= 0 && sr.busy) || ahead) {
+      ++it;
+      continue;
+    }
+    int f = -1;
+    for (int i = 0; i < (int)s->slot_ticket.size(); i++)
+      if (s->slot_ticket[i] < 0) {
+        f = i;
+        break;
+      }
+    if (f < 0) break;
+    t.sa = rup(std::max<long long>(t.S, 1), kAlign);
+    t.ga = rup(std::max<long long>(t.G, 1), kAlign);
+    t.ncap = std::max<long long>(8, 2 * t.G);
+    const bool reuse_g = t.gm_in.empty();  // priors from the resident range
+    long long v0, s0, g0 = 0, c0;
+    if (!s->av.alloc(t.N, 1, &v0)) {
+      ++it;
+      continue;
+    }
+    if (!s->as.alloc(t.sa, kAlign, &s0)) {
+      s->av.release(v0, t.N);
+      ++it;
+      continue;
+    }
+    if (!s->ac.alloc(t.ncap, 1, &c0)) {
+      s->av.release(v0, t.N);
+      s->as.release(s0, t.sa);
+      ++it;
+      continue;
+    }
+    if (reuse_g) {
+      g0 = sr.g0;
+    } else if (!s->ag.alloc(t.ga, kAlign, &g0)) {
+      s->av.release(v0, t.N);
+      s->as.release(s0, t.sa);
+      s->ac.release(c0, t.ncap);
+      ++it;
+      continue;
+    }
+    if (!reuse_g && t.stream >= 0 && sr.g0 >= 0) {  // new records replace the resident state
+      s->ag.release(sr.g0, sr.ga);
+      sr.g0 = -1;
+    }
+    t.v0 = v0;
+    t.s0 = s0;
+    t.g0 = g0;
+    t.c0 = c0;
+    if (int rc = admit(s, t, f)) return rc;
+    it = s->queue.erase(it);
+  }
+  return TGMM_OK;
+}
+
+static int retire(tgmm_server* s, int f) {
+  const long long id = s->slot_ticket[f];
+  Ticket& t = s->tickets[id];
+  DevPtrs& P = s->ws.P;
+  hipStream_t st = s->st;
+  const long long S = s->ws.sh.Scap, G = s->ws.sh.Gcap;
+  const int K = s->ws.K;
+  std::vector<double> gs((size_t)GS_NFIELDS * std::max<long long>(t.G, 1));
+  for (int fi = 0; fi < GS_NFIELDS; fi++)
+    HIPCHK(hipMemcpyAsync(gs.data() + fi * t.G, P.gs + fi * G + t.g0, t.G * 8,
+                          hipMemcpyDeviceToHost, st));
+  std::vector<unsigned char> fx(std::max<long long>(t.G, 1));
+  HIPCHK(hipMemcpyAsync(fx.data(), P.fixed + t.g0, t.G, hipMemcpyDeviceToHost, st));
+  t.rnk.resize((size_t)K * t.S);
+  t.ind.resize((size_t)K * t.S);
+  if (t.S > 0) {
+    HIPCHK(hipMemcpy2DAsync(t.rnk.data(), t.S * 4, P.rnk + t.s0, S * 4, t.S * 4, K,
+                            hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpy2DAsync(t.ind.data(), t.S * 4, P.ind + t.s0, S * 4, t.S * 4, K,
+                            hipMemcpyDeviceToHost, st));
+  }
+  HIPCHK(hipStreamSynchronize(st));
+  t.gm_out.resize(t.G);
+  soa_to_records(gs.data(), t.G, 0, t.G, t.gm_out.data());
+  for (long long k = 0; k < t.G; k++) {
+    t.gm_out[k].fixed = fx[k];
+    t.gm_out[k].supervoxelNum = 0;
+  }
+  const int cap = s->o.em.max_sv_per_gaussian;
+  for (int j = 0; j < K; j++)
+    for (long long sv = 0; sv < t.S; sv++) {
+      int& k = t.ind[j * t.S + sv];
+      if (k >= 0) k -= (int)t.g0;
+      if (t.rnk[j * t.S + sv] > 0.5f && k >= 0) {
+        GaussianMixtureModelCUDA& g = t.gm_out[k];
+        if (g.supervoxelNum < cap) g.supervoxelIdx[g.supervoxelNum++] = (int)sv;
+      }
+    }
+  const FrameDev& F = s->fh[f];
+  t.ll = F.ll;
+  t.iters = F.iter - 1;
+  t.state = 2;
+  t.t_done = now_s();
+  // unmap and release; the stream keeps its Gaussian range for the next frame
+  if (int rc = map_ranges(s, t, -1)) return rc;
+  FrameDev z;
+  std::memset(&z, 0, sizeof(z));
+  s->fh[f] = z;
+  HIPCHK(hipMemcpyAsync(P.fr + f, &z, sizeof(FrameDev), hipMemcpyHostToDevice, st));
+  s->av.release(t.v0, t.N);
+  s->as.release(t.s0, t.sa);
+  s->ac.release(t.c0, t.ncap);
+  Stream& sr = s->streams[t.stream];
+  if (t.stream >= 0) {
+    sr.g0 = t.g0;
+    sr.ga = t.ga;
+    sr.G = t.G;
+    sr.alpha_total = F.sum_alpha;
+  } else {
+    s->ag.release(t.g0, t.ga);
+  }
+  sr.busy = false;
+  s->slot_ticket[f] = -1;
+  s->in_flight--;
+  t.query.clear();
+  t.query.shrink_to_fit();
+  t.img.clear();
+  t.img.shrink_to_fit();
+  return TGMM_OK;
+}
+
+extern "C" int tgmm_server_step(tgmm_server* s, int* pending) {
+  if (!s) return TGMM_ERR_ARG;
+  HIPCHK(hipSetDevice(s->device));
+  // retire frames that stopped in the previous iteration
+  for (int f = 0; f < (int)s->slot_ticket.size(); f++)
+    if (s->slot_ticket[f] >= 0 && !s->fh[f].active)
+      if (int rc = retire(s, f)) return rc;
+  if (int rc = try_admit(s)) return rc;
+  if (s->in_flight > 0) {
+    if (int rc = server_capture(s)) return rc;
+    HIPCHK(hipGraphLaunch(s->gexec, s->st));
+    HIPCHK(hipMemcpyAsync(s->fh.data(), s->ws.P.fr, s->fh.size() * sizeof(FrameDev),
+                          hipMemcpyDeviceToHost, s->st));
+    HIPCHK(hipStreamSynchronize(s->st));
+  }
+  if (pending) *pending = s->in_flight + (int)s->queue.size();
+  return TGMM_OK;
+}
+
+extern "C" int tgmm_server_wait(tgmm_server* s, long long ticket) {
+  if (!s) return TGMM_ERR_ARG;
+  if (ticket >= 0 && !s->tickets.count(ticket)) {
+    tgmm_set_error("unknown ticket %lld", ticket);
+    return TGMM_ERR_ARG;
+  }
+  for (;;) {
+    int pending = 0;
+    if (ticket >= 0 && s->tickets[ticket].state == 2) return TGMM_OK;
+    if (int rc = tgmm_server_step(s, &pending)) return rc;
+    if (ticket < 0 && pending == 0) return TGMM_OK;
+    if (ticket >= 0 && s->tickets[ticket].state == 0 && s->in_flight == 0) {
+      // nothing in flight and the ticket cannot be admitted: it never fits
+      int again = 0;
+      if (int rc = tgmm_server_step(s, &again)) return rc;
+      if (s->tickets[ticket].state == 0 && s->in_flight == 0) {
+        tgmm_set_error("ticket %lld cannot be admitted (capacity)", ticket);
+        return TGMM_ERR_BUSY;
+      }
+    }
+  }
+}
+
+extern "C" int tgmm_server_fetch(tgmm_server* s, long long ticket, tgmm_frame_desc* d,
+                                 double* t_submit, double* t_admit, double* t_done) {
+  if (!s || !d || !s->tickets.count(ticket)) {
+    tgmm_set_error("tgmm_server_fetch: bad arguments");
+    return TGMM_ERR_ARG;
+  }
+  Ticket& t = s->tickets[ticket];
+  if (t.state != 2) {
+    tgmm_set_error("ticket %lld not finished", ticket);
+    return TGMM_ERR_BUSY;
+  }
+  if (d->gm) std::memcpy(d->gm, t.gm_out.data(), t.gm_out.size() * sizeof(GaussianMixtureModelCUDA));
+  if (d->rnk) std::memcpy(d->rnk, t.rnk.data(), t.rnk.size() * 4);
+  if (d->ind) std::memcpy(d->ind, t.ind.data(), t.ind.size() * 4);
+  d->ll = t.ll;
+  d->iterations = t.iters;
+  d->status = t.status;
+  d->n_gaussians = (int)t.G;
+  if (t_submit) *t_submit = t.t_submit;
+  if (t_admit) *t_admit = t.t_admit;
+  if (t_done) *t_done = t.t_done;
+  s->tickets.erase(ticket);
+  return TGMM_OK;
+}
+
+extern "C" int tgmm_server_destroy(tgmm_server* s) {
+  if (!s) return TGMM_ERR_ARG;
+  (void)hipSetDevice(s->device);
+  if (s->st) (void)hipStreamSynchronize(s->st);
+  if (s->gexec) (void)hipGraphExecDestroy(s->gexec);
+  if (s->graph) (void)hipGraphDestroy(s->graph);
+  ws_free(&s->ws);
+  if (s->st) (void)hipStreamDestroy(s->st);
+  delete s;
+  return TGMM_OK;
+}

@@ -85,6 +85,17 @@ class FrameDesc(ctypes.Structure):
     ]
 
 
+class ServerOpts(ctypes.Structure):
+    _fields_ = [
+        ("em", EmOpts),
+        ("hbm_budget_bytes", ctypes.c_longlong),
+        ("max_frames_in_flight", ctypes.c_int),
+        ("beta_percentage_of_nk", ctypes.c_double),
+        ("nu_percentage_of_nk", ctypes.c_double),
+        ("alpha_percentage", ctypes.c_double),
+    ]
+
+
 _lib = None
 _vp = ctypes.c_void_p
 _pp = ctypes.POINTER(ctypes.c_void_p)
@@ -120,6 +131,16 @@ def load():
                                          ctypes.POINTER(ctypes.c_double),
                                          ctypes.POINTER(ctypes.c_double)]
     L.tgmm_batch_destroy.argtypes = [_vp]
+    L.tgmm_server_create.argtypes = [ctypes.c_int, ctypes.POINTER(ServerOpts), _pp]
+    L.tgmm_server_submit.argtypes = [_vp, ctypes.c_int, ctypes.POINTER(FrameDesc),
+                                     ctypes.POINTER(ctypes.c_longlong)]
+    L.tgmm_server_wait.argtypes = [_vp, ctypes.c_longlong]
+    L.tgmm_server_step.argtypes = [_vp, ctypes.POINTER(ctypes.c_int)]
+    L.tgmm_server_fetch.argtypes = [_vp, ctypes.c_longlong, ctypes.POINTER(FrameDesc),
+                                    ctypes.POINTER(ctypes.c_double),
+                                    ctypes.POINTER(ctypes.c_double),
+                                    ctypes.POINTER(ctypes.c_double)]
+    L.tgmm_server_destroy.argtypes = [_vp]
     _lib = L
     return L
 
@@ -302,6 +323,89 @@ class Batch:
     def close(self):
         if self.h:
             self.L.tgmm_batch_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# ------------------------------------------------------------------ server --
+class Server:
+    """Continuous-batching server for concurrent time-lapse streams
+    (tgmm_server_* in include/tgmm_em.h)."""
+
+    def __init__(self, opts, hbm_budget_bytes=8 << 30, max_frames_in_flight=16,
+                 beta_pct=0.5, nu_pct=0.5, alpha_pct=0.8, device=0):
+        self.L = load()
+        self.opts = opts
+        so = ServerOpts()
+        so.em = opts
+        so.hbm_budget_bytes = int(hbm_budget_bytes)
+        so.max_frames_in_flight = int(max_frames_in_flight)
+        so.beta_percentage_of_nk = beta_pct
+        so.nu_percentage_of_nk = nu_pct
+        so.alpha_percentage = alpha_pct
+        self.h = ctypes.c_void_p()
+        _check(self.L.tgmm_server_create(device, ctypes.byref(so), ctypes.byref(self.h)))
+        self.shapes = {}
+
+    def submit(self, frame, stream=-1, with_gaussians=True):
+        """Queue a frame (host arrays are copied); returns a ticket. With
+        with_gaussians=False the priors come from the stream's previous
+        posterior on the device."""
+        d = FrameDesc()
+        q = np.ascontiguousarray(frame.query, np.float32)
+        im = np.ascontiguousarray(frame.img, np.float32)
+        ce = np.ascontiguousarray(frame.centroids, np.float32)
+        pt = np.ascontiguousarray(frame.sv_ptr, np.int64)
+        d.n_voxels = im.shape[0]
+        d.n_supervoxels = pt.shape[0] - 1
+        d.query, d.img, d.centroids, d.sv_ptr = q.ctypes.data, im.ctypes.data, ce.ctypes.data, \
+            pt.ctypes.data
+        gm = None
+        if with_gaussians:
+            gm = as_records(frame.gm)
+            d.n_gaussians = gm.shape[0]
+            d.gm = gm.ctypes.data
+        t = ctypes.c_longlong()
+        _check(self.L.tgmm_server_submit(self.h, int(stream), ctypes.byref(d), ctypes.byref(t)))
+        self.shapes[t.value] = int(d.n_supervoxels)
+        return t.value
+
+    def step(self):
+        p = ctypes.c_int()
+        _check(self.L.tgmm_server_step(self.h, ctypes.byref(p)))
+        return p.value
+
+    def wait(self, ticket=-1):
+        _check(self.L.tgmm_server_wait(self.h, int(ticket)))
+
+    def fetch(self, ticket, n_gaussians):
+        S = self.shapes.pop(ticket)
+        K = self.opts.K
+        gm = np.zeros(int(n_gaussians), GMM_DTYPE)
+        rnk = np.zeros(K * S, np.float32)
+        ind = np.zeros(K * S, np.int32)
+        d = FrameDesc()
+        d.gm, d.rnk, d.ind = gm.ctypes.data, rnk.ctypes.data, ind.ctypes.data
+        ts, ta, td = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+        _check(self.L.tgmm_server_fetch(self.h, int(ticket), ctypes.byref(d), ctypes.byref(ts),
+                                        ctypes.byref(ta), ctypes.byref(td)))
+        return dict(gm=gm, rnk=rnk, ind=ind, ll=d.ll, iterations=d.iterations,
+                    t_submit=ts.value, t_admit=ta.value, t_done=td.value)
+
+    def close(self):
+        if self.h:
+            self.L.tgmm_server_destroy(self.h)
             self.h = ctypes.c_void_p()
 
     def __enter__(self):

@@ -216,3 +216,44 @@ def config5(seed=CONFIG_SEEDS[5], n_cells=60000, target_voxels=30_000_000):
     f = rasterize(seed + 1, cells, vol, 5.0, target_voxels)
     f.meta = dict(config=5, K=8, iters=25, vol=vol, aniso_z=5.0)
     return f
+
+
+def timelapse(seed, n_cells, n_frames, vol, aniso_z=5.0, rmin=3.0, rmax=6.0,
+              voxels_per_cell=200.0, motion_sd=1.5, p_divide=0.005):
+    """Frames of a synthetic time-lapse: cells move by N(0, motion_sd^2)
+    voxels per frame and divide with probability p_divide (daughters 3
+    voxels either side of the mother along a random direction). Frame 0
+    carries initial Gaussians (one per cell); later frames carry the same
+    Gaussian count (a served stream keeps its Gaussians)."""
+    rng = np.random.default_rng(seed)
+    cells = sample_cells(seed, n_cells, vol, aniso_z, rmin, rmax)
+    frames = []
+    for t in range(n_frames):
+        f = rasterize(seed * 1000 + t, cells, vol, aniso_z, voxels_per_cell * cells.n,
+                      init_seed=seed + 17)
+        if t == 0:
+            f.gm = f.gm[:n_cells].copy()
+        else:
+            f.gm = None
+        f.meta = dict(stream_seed=seed, t=t, cells=cells.n)
+        frames.append(f)
+        # motion and divisions for the next frame
+        c = cells.centre + rng.normal(scale=motion_sd, size=cells.centre.shape)
+        c[:, 2] = cells.centre[:, 2] + rng.normal(scale=motion_sd / aniso_z, size=cells.n)
+        div = rng.random(cells.n) < p_divide
+        if div.any():
+            d = rng.normal(size=(int(div.sum()), 3))
+            d /= np.linalg.norm(d, axis=1, keepdims=True)
+            d *= 3.0
+            d[:, 2] /= aniso_z
+            kids = c[div] + d
+            c[div] -= d
+            c = np.concatenate([c, kids])
+            cells = Cells(c, np.concatenate([cells.radius, cells.radius[div]]),
+                          np.concatenate([cells.quat, cells.quat[div]]),
+                          np.concatenate([cells.bright, cells.bright[div]]))
+        else:
+            cells = Cells(c, cells.radius, cells.quat, cells.bright)
+        for a in range(3):
+            np.clip(cells.centre[:, a], 0, vol[a] - 1, out=cells.centre[:, a])
+    return frames
