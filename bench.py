@@ -17,6 +17,15 @@ roofline of the dominant kernel (hipEvent timing on the library's stream)
 and the oracle timed on the host cores as the CPU baseline.
 
     python bench.py [--steps K] [--warmup W] [--config 1|2|3|5]
+
+With --config 4 it measures the continuous-batching server instead: four
+synthetic time-lapse streams of 200 frames (2,000-15,000 cells, ~200 voxels
+per cell, K = 5, 10 iterations), each stream's posterior kept on the device as
+its next frame's priors. Two phases on fresh servers: a closed loop that keeps
+every stream's next frame queued (throughput, `value`), then Poisson arrivals
+at --qps (default half the closed-loop rate) for p50/p99 latency from a
+frame's arrival to its results on the host. Input upload (host to device) is
+inside both timed regions.
 """
 import argparse
 import json
@@ -78,6 +87,13 @@ def main():
                     help="iterations of the accuracy solve (0 = the workload's count)")
     ap.add_argument("--plain-frames", type=int, default=64,
                     help="config 3: frames timed in the plain per-frame form")
+    ap.add_argument("--stream-frames", type=int, default=200,
+                    help="config 4: frames per time-lapse stream")
+    ap.add_argument("--qps", type=float, default=0.0,
+                    help="config 4: Poisson arrival rate, all streams (0 = half the "
+                         "closed-loop throughput)")
+    ap.add_argument("--hbm-budget-gb", type=float, default=24.0,
+                    help="config 4: server workspace budget")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: the EM of a frame runs on one "
@@ -87,6 +103,9 @@ def main():
 
     import numpy as np
     from tgmm_amd import em
+
+    if args.config == 4:
+        return serve_bench(args)
 
     t0 = time.time()
     frames, K, iters, workload = _frames_for(args.config)
@@ -229,6 +248,161 @@ def main():
     }
     if plain is not None:
         line["plain_per_frame_form"] = plain
+    print(json.dumps(line), flush=True)
+
+
+def _serve(streams, opts, budget, order, arrivals=None, pct=(0.5, 0.5, 0.8)):
+    """Feed the frames of `streams` to a fresh server. Closed loop when
+    `arrivals` is None (every stream keeps its next frame queued); otherwise
+    arrivals[i] is the arrival time (s after the start) of each frame of
+    stream i. Returns (elapsed seconds, per-frame records, outputs kept for
+    `order`)."""
+    from tgmm_amd import em
+    ns = len(streams)
+    G = [fr[0].G for fr in streams]
+    srv = em.Server(opts, hbm_budget_bytes=budget, max_frames_in_flight=2 * ns,
+                    beta_pct=pct[0], nu_pct=pct[1], alpha_pct=pct[2])
+    nxt = [0] * ns
+    open_t = {}
+    recs, keep = [], {}
+    total = sum(len(fr) for fr in streams)
+    depth = [0] * ns
+    if arrivals is not None:
+        events = sorted((arrivals[i][t], i, t) for i in range(ns) for t in range(len(streams[i])))
+    ai = 0
+    t0 = time.monotonic()
+    while len(recs) < total:
+        now = time.monotonic() - t0
+        if arrivals is None:
+            for i in range(ns):
+                while depth[i] < 2 and nxt[i] < len(streams[i]):
+                    t = nxt[i]
+                    tk = srv.submit(streams[i][t], stream=i, with_gaussians=(t == 0))
+                    open_t[tk] = (i, t, time.monotonic() - t0)
+                    nxt[i] += 1
+                    depth[i] += 1
+        else:
+            while ai < len(events) and events[ai][0] <= now:
+                ta, i, t = events[ai]
+                tk = srv.submit(streams[i][t], stream=i, with_gaussians=(t == 0))
+                open_t[tk] = (i, t, ta)
+                ai += 1
+        if open_t:
+            srv.step()
+            for tk in list(open_t):
+                if srv.query(tk) == 2:
+                    i, t, ta = open_t.pop(tk)
+                    o = srv.fetch(tk, G[i])
+                    recs.append(dict(stream=i, t=t, arrival=ta, done=o["t_done"] - t0,
+                                     admit=o["t_admit"] - t0))
+                    depth[i] -= 1
+                    if (i, t) in order:
+                        keep[(i, t)] = o
+        elif ai < len(events):
+            time.sleep(max(0.0, min(0.002, events[ai][0] - now)))
+    elapsed = time.monotonic() - t0
+    srv.close()
+    return elapsed, recs, keep
+
+
+def serve_bench(args):
+    import numpy as np
+    from tgmm_amd import em, synth
+    K, iters = 5, 10
+    t0 = time.time()
+    streams = synth.config4_streams(n_frames=args.stream_frames)
+    n_frames = sum(len(fr) for fr in streams)
+    n_vox = sum(f.N for fr in streams for f in fr)
+    print("# generated %d streams x %d frames in %.1fs: %d voxels, cells %d-%d per frame" % (
+        len(streams), args.stream_frames, time.time() - t0, n_vox,
+        min(f.meta["cells"] for fr in streams for f in fr),
+        max(f.meta["cells"] for fr in streams for f in fr)), file=sys.stderr, flush=True)
+    opts = em.make_opts(K=K, max_iter=iters, tol=-1.0)
+    budget = int(args.hbm_budget_gb * (1 << 30))
+    pct = (0.5, 0.5, 0.8)
+
+    # warm-up: two frames per stream on a throw-away server
+    _serve([fr[:2] for fr in streams], opts, budget, set())
+
+    # closed loop: throughput
+    el, recs, keep = _serve(streams, opts, budget, {(0, 0), (0, 1)}, pct=pct)
+    thr = n_frames / el
+    print("# closed loop: %d frames in %.3fs = %.1f frames/s" % (n_frames, el, thr),
+          file=sys.stderr, flush=True)
+
+    # Poisson arrivals: latency
+    qps = args.qps if args.qps > 0 else 0.5 * thr
+    rng = np.random.default_rng(0xC0FFEE04)
+    ns = len(streams)
+    arrivals = [np.cumsum(rng.exponential(ns / qps, len(fr))) for fr in streams]
+    el2, recs2, _ = _serve(streams, opts, budget, set(), arrivals=arrivals, pct=pct)
+    lat = [r["done"] - r["arrival"] for r in recs2]
+    qwait = [r["admit"] - r["arrival"] for r in recs2]
+    lat_ms = {"p50": 1000 * _percentile(lat, 50), "p99": 1000 * _percentile(lat, 99),
+              "max": 1000 * max(lat)}
+    print("# Poisson at %.1f frames/s: served %.1f frames/s, latency %s" % (
+        qps, n_frames / el2, json.dumps(lat_ms)), file=sys.stderr, flush=True)
+
+    # accuracy: stream 0, frames 0 and 1 (device-propagated priors) against
+    # the FP64 oracle chained through the host restatement of the prior update
+    acc, cpu = None, None
+    if not args.no_oracle:
+        import oracle_lib as O
+        from helpers import next_priors, record_errors, rel_err
+        f0, f1 = streams[0][0], streams[0][1]
+        oo = O.make_opts(K=K, max_iter=iters)
+        t1 = time.perf_counter()
+        g0, _, ind0, _, _, ll0 = O.em_run(f0, oo)
+        dt = time.perf_counter() - t1
+        pri = next_priors(g0, pct[0], pct[1], pct[2], float(g0["alpha_k"].sum()))
+        f1g = synth.Frame(f1.query, f1.img, f1.centroids, f1.sv_ptr, pri)
+        g1, _, ind1, _, _, ll1 = O.em_run(f1g, oo)
+        errs = []
+        for o, g, ll, ind in ((keep[(0, 0)], g0, ll0, ind0), (keep[(0, 1)], g1, ll1, ind1)):
+            e = record_errors(o["gm"], g)
+            e["ll"] = rel_err(o["ll"], ll)
+            e["knn_sets_identical"] = bool(np.array_equal(o["ind"], ind))
+            errs.append(e)
+        acc = {k: (max(e[k] for e in errs) if k != "knn_sets_identical"
+                   else all(e[k] for e in errs)) for k in errs[0]}
+        acc["frames_checked"] = "stream 0, frames 0 and 1"
+        print("# accuracy vs FP64 oracle: %s" % json.dumps(acc), file=sys.stderr, flush=True)
+        threads = O.oracle().orc_get_threads()
+        cpu = {"value": 1.0 / dt, "unit": "frames/s", "cores": threads, "kind": "port",
+               "sample": "one full %d-iteration solve of stream 0 frame 0 (N=%d, S=%d, G=%d) "
+                         "by the FP64 oracle on %d host threads" % (iters, f0.N, f0.S, f0.G,
+                                                                     threads)}
+
+    line = {
+        "metric": METRIC,
+        "value": thr,
+        "unit": "frames/s",
+        "n_gpus": 1,
+        "steps": n_frames,
+        "warmup": 2 * ns,
+        "ms_per_step": 1000.0 * el / n_frames,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64",
+        "data": "synthetic (seeded generator, tgmm_amd/csrc/synth.cpp)",
+        "config": {"workload": "config4: %d time-lapse streams x %d frames, %d-%d cells per "
+                               "frame, ~200 voxels per cell, K=5, 10 iterations, continuous "
+                               "batching, priors from each stream's posterior on the device"
+                               % (ns, args.stream_frames,
+                                  min(f.meta["cells"] for fr in streams for f in fr),
+                                  max(f.meta["cells"] for fr in streams for f in fr)),
+                   "voxels": n_vox, "K": K, "iterations": iters, "parallelism": "1 GPU"},
+        "closed_loop": {"frames": n_frames, "seconds": el, "frames_per_s": thr},
+        "poisson": {"qps": qps, "seconds": el2, "frames_per_s": n_frames / el2,
+                    "latency_ms": lat_ms,
+                    "queue_wait_ms": {"p50": 1000 * _percentile(qwait, 50),
+                                      "p99": 1000 * _percentile(qwait, 99)}},
+        "latency_ms": lat_ms,
+        "accuracy_vs_fp64": acc,
+        "roofline": None,
+        "cpu_baseline": cpu,
+    }
     print(json.dumps(line), flush=True)
 
 

@@ -220,6 +220,9 @@ int tgmm_server_wait(tgmm_server* s, long long ticket);
 /* Advances the server by one scheduling step (admit, one iteration, retire);
  * returns the number of frames still in flight or queued via *pending. */
 int tgmm_server_step(tgmm_server* s, int* pending);
+/* State of a ticket that was not fetched yet: 0 queued, 1 in flight,
+ * 2 done (results on the host, ready for tgmm_server_fetch). */
+int tgmm_server_query(tgmm_server* s, long long ticket, int* state);
 /* Fetch a completed ticket's outputs into frame (gm must have room for the
  * frame's Gaussians; rnk/ind optional). Also returns admission and
  * completion wall times (seconds, monotonic clock). */

@@ -20,6 +20,7 @@
 #include <cstring>
 #include <deque>
 #include <map>
+#include <set>
 #include <vector>
 
 #include "../../include/tgmm_em.h"
@@ -137,6 +138,7 @@ struct tgmm_server {
   hipGraphExec_t gexec = nullptr;
   long long next_id = 0;
   int in_flight = 0;
+  long long anon_queued = 0;  // queued frames with no stream
 };
 
 static int server_capture(tgmm_server* s) {
@@ -276,6 +278,7 @@ extern "C" int tgmm_server_submit(tgmm_server* s, int stream_id, const tgmm_fram
     return TGMM_ERR_LIMIT;
   }
   t.t_submit = now_s();
+  if (stream_id < 0) s->anon_queued++;
   s->queue.push_back(t.id);
   s->tickets.emplace(t.id, std::move(t));
   *ticket = t.id;
@@ -373,19 +376,22 @@ static int admit(tgmm_server* s, Ticket& t, int f) {
 }
 
 static int try_admit(tgmm_server* s) {
+  // streams with a frame in flight or an earlier frame still queued; frames of
+  // one stream run one after another (each needs the previous posterior),
+  // frames of other streams may pass them
+  std::set<int> blocked;
+  for (const auto& kv : s->streams)
+    if (kv.second.busy) blocked.insert(kv.first);
   for (auto it = s->queue.begin(); it != s->queue.end();) {
     if (s->in_flight >= s->ws.sh.Fcap) break;
+    if (s->anon_queued == 0 && blocked.size() >= s->streams.size()) break;
     Ticket& t = s->tickets[*it];
     Stream& sr = s->streams[t.stream];
-    // frames of one stream run one after another (each needs the previous
-    // posterior); frames from other streams may pass it
-    bool ahead = false;
-    for (auto jt = s->queue.begin(); jt != it; ++jt)
-      if (s->tickets[*jt].stream == t.stream && t.stream >= 0) ahead = true;
-    if ((t.stream >= 0 && sr.busy) || ahead) {
+    if (t.stream >= 0 && blocked.count(t.stream)) {
       ++it;
       continue;
     }
+    if (t.stream >= 0) blocked.insert(t.stream);  // later frames of it wait
     int f = -1;
     for (int i = 0; i < (int)s->slot_ticket.size(); i++)
       if (s->slot_ticket[i] < 0) {
@@ -431,6 +437,7 @@ static int try_admit(tgmm_server* s) {
     t.g0 = g0;
     t.c0 = c0;
     if (int rc = admit(s, t, f)) return rc;
+    if (t.stream < 0) s->anon_queued--;
     it = s->queue.erase(it);
   }
   return TGMM_OK;
@@ -547,6 +554,15 @@ extern "C" int tgmm_server_wait(tgmm_server* s, long long ticket) {
       }
     }
   }
+}
+
+extern "C" int tgmm_server_query(tgmm_server* s, long long ticket, int* state) {
+  if (!s || !state || !s->tickets.count(ticket)) {
+    tgmm_set_error("tgmm_server_query: bad arguments");
+    return TGMM_ERR_ARG;
+  }
+  *state = s->tickets[ticket].state;
+  return TGMM_OK;
 }
 
 extern "C" int tgmm_server_fetch(tgmm_server* s, long long ticket, tgmm_frame_desc* d,

@@ -140,6 +140,7 @@ def load():
                                     ctypes.POINTER(ctypes.c_double),
                                     ctypes.POINTER(ctypes.c_double),
                                     ctypes.POINTER(ctypes.c_double)]
+    L.tgmm_server_query.argtypes = [_vp, ctypes.c_longlong, ctypes.POINTER(ctypes.c_int)]
     L.tgmm_server_destroy.argtypes = [_vp]
     _lib = L
     return L
@@ -388,6 +389,12 @@ class Server:
 
     def wait(self, ticket=-1):
         _check(self.L.tgmm_server_wait(self.h, int(ticket)))
+
+    def query(self, ticket):
+        """0 queued, 1 in flight, 2 done (ready to fetch)."""
+        st = ctypes.c_int()
+        _check(self.L.tgmm_server_query(self.h, int(ticket), ctypes.byref(st)))
+        return st.value
 
     def fetch(self, ticket, n_gaussians):
         S = self.shapes.pop(ticket)

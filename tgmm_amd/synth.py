@@ -208,6 +208,30 @@ def config3_frames(seed=CONFIG_SEEDS[3], n_frames=64, total_voxels=25_000_000,
     return frames
 
 
+def config4_streams(seed=CONFIG_SEEDS[4], n_frames=200, cells=(2000, 3000, 4000, 5500),
+                    voxels_per_cell=200.0, workers=4):
+    """Time-lapse streams (one per entry of `cells`, seed + stream): cells
+    move by N(0, 1.5^2) voxels per frame and divide with p = 0.005, so a
+    stream of 200 frames grows by about e (5,500 -> ~15,000 cells);
+    ~200 voxels per cell, volume at config-1 density for the first frame.
+    Streams are generated in parallel threads (the generator releases the
+    GIL); returns a list of frame lists."""
+    from concurrent.futures import ThreadPoolExecutor
+    dens = 200.0 / (512.0 * 512.0 * 100.0)
+
+    def one(i):
+        x = (cells[i] / dens * 512.0 / 100.0) ** (1.0 / 3.0)
+        vol = (x, x, x * 100.0 / 512.0)
+        fr = timelapse(seed + i, cells[i], n_frames, vol, 5.0, 3.0, 6.0,
+                       voxels_per_cell=voxels_per_cell)
+        for f in fr:
+            f.meta.update(config=4, stream=i, K=5, iters=10, vol=vol, aniso_z=5.0)
+        return fr
+
+    with ThreadPoolExecutor(max(1, min(workers, len(cells)))) as ex:
+        return list(ex.map(one, range(len(cells))))
+
+
 def config5(seed=CONFIG_SEEDS[5], n_cells=60000, target_voxels=30_000_000):
     """60k cells in 4096x4096x500, radii U[3,8], lognormal brightness 0.8,
     ~30M voxels; K=8, 25 iterations."""
