@@ -181,6 +181,30 @@ int tgmm_batch_reset(tgmm_batch* b);
 int tgmm_batch_solve(tgmm_batch* b, int mode);
 /* Copies results into frames[i].gm/rnk/ind/ll/iterations (host). */
 int tgmm_batch_fetch(tgmm_batch* b, tgmm_frame_desc* frames, int nframes);
+/* Launch shapes of the tunable kernels. Every valid choice gives bitwise the
+ * same results (the kernels fix their summation order independently of the
+ * shape); only the speed changes. Defaults: {16, 4, 16, 256, 256}. */
+typedef struct tgmm_tiles {
+  int ll_group;    /* lanes per supervoxel, log-likelihood: 4, 8 or 16 */
+  int ll_unroll;   /* voxels per lane per pass: 4, 8 or 16, with
+                      ll_group * ll_unroll = 32 or 64 */
+  int mstep_lanes; /* lanes per Gaussian, M-step statistics: 4, 8 or 16 */
+  int sv_block;    /* threads per block, E-step and pair scatter: 64, 128, 256 */
+  int knn_block;   /* threads per block, kNN search: 64, 128 or 256 */
+} tgmm_tiles;
+
+int tgmm_batch_set_tiles(tgmm_batch* b, const tgmm_tiles* tiles);
+int tgmm_batch_get_tiles(tgmm_batch* b, tgmm_tiles* tiles);
+/* Autotuner. For each knob in turn (log-likelihood shape, M-step lanes,
+ * E-step block, kNN block) it times every candidate, the other knobs held at
+ * the best so far, over `reps` eager iterations of the batch's own frames
+ * from their initial state (hipEvents around the kernel on the batch
+ * stream), and keeps the fastest. Leaves the batch in its initial state with
+ * the chosen tiles. `chosen` and `report` (one line per candidate: knob,
+ * shape, microseconds per launch) are optional. */
+int tgmm_batch_autotune(tgmm_batch* b, int reps, tgmm_tiles* chosen, char* report,
+                        int report_cap);
+
 /* Average device time (ms) of one launch of the named kernel over `reps`
  * eager iterations, measured with hipEvents on the batch stream. Names:
  * "loglik", "estep", "knn", "mstep", "moments". Leaves the state advanced. */
@@ -228,6 +252,9 @@ int tgmm_server_query(tgmm_server* s, long long ticket, int* state);
  * completion wall times (seconds, monotonic clock). */
 int tgmm_server_fetch(tgmm_server* s, long long ticket, tgmm_frame_desc* frame,
                       double* t_submit, double* t_admit, double* t_done);
+/* Launch shapes for the server's iterations (for example those the
+ * autotuner chose on a representative batch). */
+int tgmm_server_set_tiles(tgmm_server* s, const tgmm_tiles* tiles);
 int tgmm_server_destroy(tgmm_server* s);
 
 #ifdef __cplusplus

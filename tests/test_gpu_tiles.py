@@ -1,0 +1,78 @@
+"""GPU tests of the tunable launch shapes: every valid tile shape gives
+bitwise the same results (the kernels fix their summation order
+independently of the shape), and the autotuner returns a valid choice that
+changes nothing but speed."""
+import numpy as np
+import pytest
+
+from helpers import dup_gaussians
+from tgmm_amd import em, synth
+
+pytestmark = pytest.mark.gpu
+
+SHAPES = [
+    dict(ll_group=16, ll_unroll=4, mstep_lanes=16, sv_block=256, knn_block=256),
+    dict(ll_group=8, ll_unroll=4, mstep_lanes=8, sv_block=128, knn_block=64),
+    dict(ll_group=8, ll_unroll=8, mstep_lanes=4, sv_block=64, knn_block=128),
+    dict(ll_group=4, ll_unroll=8, mstep_lanes=16, sv_block=128, knn_block=256),
+    dict(ll_group=4, ll_unroll=16, mstep_lanes=4, sv_block=256, knn_block=64),
+]
+
+
+@pytest.fixture(scope="module")
+def frames():
+    c1 = synth.config1()
+    cells = synth.sample_cells(99, 120, (256.0, 256.0, 60.0), 5.0, 3.0, 6.0)
+    soft = dup_gaussians(synth.rasterize(100, cells, (256.0, 256.0, 60.0), 5.0, 20000,
+                                         max_sv_size=4))
+    # supervoxels of up to 200 voxels: several passes of every unroll
+    cells = synth.sample_cells(41, 60, (256.0, 256.0, 60.0), 5.0, 4.0, 7.0)
+    big = synth.rasterize(42, cells, (256.0, 256.0, 60.0), 5.0, 30000, max_sv_size=200)
+    assert np.diff(big.sv_ptr).max() > 64
+    return [c1, soft, big]
+
+
+def _run(frames, K, tiles, iters=4):
+    with em.Batch(frames, em.make_opts(K=K, max_iter=iters, tol=-1.0)) as b:
+        if tiles is not None:
+            b.set_tiles(tiles)
+            assert b.get_tiles() == tiles
+        b.solve(em.SOLVE_GRAPH)
+        return b.fetch()
+
+
+def _same(a, b):
+    for x, y in zip(a, b):
+        assert x["gm"].tobytes() == y["gm"].tobytes()
+        assert np.array_equal(x["rnk"], y["rnk"]) and np.array_equal(x["ind"], y["ind"])
+        assert x["ll"] == y["ll"]
+
+
+@pytest.mark.parametrize("K", [5, 8])
+def test_every_tile_shape_gives_identical_results(frames, K):
+    ref = _run(frames, K, None)
+    for t in SHAPES:
+        print("tiles", t, flush=True)
+        _same(ref, _run(frames, K, t))
+
+
+def test_invalid_tiles_are_rejected(frames):
+    with em.Batch(frames[:1], em.make_opts(K=5, max_iter=2)) as b:
+        bad = dict(em.DEFAULT_TILES, ll_group=2)
+        with pytest.raises(em.TgmmError):
+            b.set_tiles(bad)
+        bad = dict(em.DEFAULT_TILES, ll_group=16, ll_unroll=16)  # 256 voxels per pass
+        with pytest.raises(em.TgmmError):
+            b.set_tiles(bad)
+
+
+def test_autotune_choice_is_valid_and_changes_nothing(frames):
+    ref = _run(frames, 5, None, iters=5)
+    with em.Batch(frames, em.make_opts(K=5, max_iter=5, tol=-1.0)) as b:
+        tiles, report = b.autotune(reps=2, cache=False)
+        print(tiles, "\n", report, flush=True)
+        assert b.get_tiles() == tiles
+        assert len(report.strip().splitlines()) == 14
+        b.solve(em.SOLVE_GRAPH)
+        out = b.fetch()
+    _same(ref, out)

@@ -70,12 +70,27 @@ struct alignas(32) EPack {
   double pad;
 };
 
+// Launch shapes of the tunable kernels (chosen per workload by the
+// autotuner, or the defaults below). Every choice gives bitwise the same
+// results: the kernels fix their floating-point summation order
+// independently of the shape.
+struct Tiles {
+  int ll_gw;      // lanes per supervoxel, log-likelihood: 4, 8, 16
+  int ll_u;       // voxels per lane per pass, log-likelihood (ll_gw*ll_u = 32 or 64)
+  int ms_lanes;   // lanes per Gaussian, M-step statistics: 4, 8, 16
+  int sv_block;   // threads per block, E-step and pair scatter: 64, 128, 256
+  int knn_block;  // threads per block, kNN search: 64, 128, 256
+};
+constexpr Tiles kDefaultTiles = {16, 4, 16, 256, 256};
+bool tiles_valid(const Tiles& t);
+
 struct KOpts {
   int K;
   int w4dof;
   float scale[3];      // kNN anisotropy (uploadScaleCUDA, knnCuda.cu:827-830)
   float scale_reg[3];  // regularisation anisotropy (copyScaleToDvice)
   double lmin, lmax, emax;
+  Tiles tiles;
 };
 
 struct DevPtrs {

@@ -535,10 +535,10 @@ __global__ __launch_bounds__(kGB) void k_grid_scatter(DevPtrs P) {
 // the supervoxels in grid-cell order (qperm).
 template <int KM>
 __global__ __launch_bounds__(256) void k_knn(DevPtrs P, KOpts O) {
-  const int f = P.sblk_frame[blockIdx.x];
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int f = P.sblk_frame[((long long)blockIdx.x * blockDim.x) >> 8];
   if (f < 0 || !P.fr[f].active) return;
   const FrameDev* F = P.fr + f;
-  const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
   if (i >= F->s0 + F->ns) return;
   const long long s = P.qperm[i];
   const long long S = P.Scap;
@@ -661,9 +661,9 @@ __global__ __launch_bounds__(256) void k_knn(DevPtrs P, KOpts O) {
 // log-sum-exp with FP32 exponentials.
 template <int KM>
 __global__ __launch_bounds__(256) void k_estep(DevPtrs P, KOpts O) {
-  const int f = P.sblk_frame[blockIdx.x];
+  const long long s = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int f = P.sblk_frame[s >> 8];
   if (f < 0 || !P.fr[f].active) return;
-  const long long s = (long long)blockIdx.x * 256 + threadIdx.x;
   if (P.sv_frame[s] != f) return;
   const long long S = P.Scap;
   const int K = O.K;
@@ -716,9 +716,9 @@ __global__ __launch_bounds__(256) void k_estep(DevPtrs P, KOpts O) {
 }
 
 __global__ __launch_bounds__(256) void k_pair_scatter(DevPtrs P, KOpts O) {
-  const int f = P.sblk_frame[blockIdx.x];
+  const long long s = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int f = P.sblk_frame[s >> 8];
   if (f < 0 || !P.fr[f].active) return;
-  const long long s = (long long)blockIdx.x * 256 + threadIdx.x;
   if (P.sv_frame[s] != f) return;
   const long long S = P.Scap;
   for (int j = 0; j < O.K; j++) {
@@ -749,56 +749,79 @@ __global__ __launch_bounds__(256) void k_seg_sort(DevPtrs P) {
 }
 
 // Segmented FP64 sufficient statistics over each Gaussian's significant
-// pairs (16 lanes per Gaussian), accumulated about the current mean m_k:
+// pairs, accumulated about the current mean m_k:
 //   N   = sum r W_s
 //   S1  = sum r W_s (mu_s - m_k)
 //   S2  = sum r [C_s + W_s (mu_s - m_k)(mu_s - m_k)^T]
 // so xbar = m_k + S1/N and sum w (x - xbar)(x - xbar)^T = S2 - S1 S1^T / N,
 // identical to the reference's per-voxel sums over r_sk * img_v
-// (GMEM.cu:1338-1453). Lane l takes the sorted pairs l, l+16, ...; the 16
-// partial sums are combined in a fixed butterfly order.
-__global__ __launch_bounds__(256) void k_mstep_stats(DevPtrs P) {
-  const long long k = (long long)blockIdx.x * 16 + (threadIdx.x >> 4);
-  const int lane = threadIdx.x & 15;
+// (GMEM.cu:1338-1453). A block covers 16 Gaussians with ML lanes each.
+// Summation order, identical for every ML: sorted pair e of a Gaussian is
+// added, in increasing e, to class sum e mod 16, and the class sums are
+// combined as ((c + c^8) + ...^4) ...^1.
+template <int ML>
+__global__ __launch_bounds__(16 * ML) void k_mstep_stats(DevPtrs P) {
+  constexpr int NC = 16 / ML;
+  const long long k = (long long)blockIdx.x * 16 + threadIdx.x / ML;
+  const int lane = threadIdx.x % ML;
   const int f = P.gblk_frame[k / kGB];
   if (f < 0 || !P.fr[f].active) return;
   if (P.g_frame[k] != f) return;
   const long long G = P.Gcap, S = P.Scap;
   const double r0 = gsv(P, GS_M, k), r1 = gsv(P, GS_M + 1, k), r2 = gsv(P, GS_M + 2, k);
-  double a[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+  double a[NC][10];
+#pragma unroll
+  for (int q = 0; q < NC; q++)
+#pragma unroll
+    for (int i = 0; i < 10; i++) a[q][i] = 0.0;
   const int st = P.seg_start[k], en = P.seg_start[k + 1];
-  for (int e = st + lane; e < en; e += 16) {
-    const int code = P.pairs_sorted[e];
-    const long long s = code >> kPairShift;
-    const int j = code & ((1 << kPairShift) - 1);
-    const double r = (double)P.rnk[(long long)j * S + s];
-    const double Ws = P.wmom[s];
-    if (!(Ws != 0.0)) continue;
-    const double w = r * Ws;
-    const double d0 = P.wmom[S + s] - r0, d1 = P.wmom[2 * S + s] - r1,
-                 d2 = P.wmom[3 * S + s] - r2;
-    a[0] += w;
-    a[1] += w * d0;
-    a[2] += w * d1;
-    a[3] += w * d2;
-    a[4] += r * P.wmom[4 * S + s] + w * d0 * d0;
-    a[5] += r * P.wmom[5 * S + s] + w * d0 * d1;
-    a[6] += r * P.wmom[6 * S + s] + w * d0 * d2;
-    a[7] += r * P.wmom[7 * S + s] + w * d1 * d1;
-    a[8] += r * P.wmom[8 * S + s] + w * d1 * d2;
-    a[9] += r * P.wmom[9 * S + s] + w * d2 * d2;
+  for (int e0 = st; e0 < en; e0 += 16) {
+#pragma unroll
+    for (int q = 0; q < NC; q++) {
+      const int e = e0 + lane + ML * q;
+      if (e >= en) continue;
+      const int code = P.pairs_sorted[e];
+      const long long s = code >> kPairShift;
+      const int j = code & ((1 << kPairShift) - 1);
+      const double r = (double)P.rnk[(long long)j * S + s];
+      const double Ws = P.wmom[s];
+      if (!(Ws != 0.0)) continue;
+      const double w = r * Ws;
+      const double d0 = P.wmom[S + s] - r0, d1 = P.wmom[2 * S + s] - r1,
+                   d2 = P.wmom[3 * S + s] - r2;
+      double* c = a[q];
+      c[0] += w;
+      c[1] += w * d0;
+      c[2] += w * d1;
+      c[3] += w * d2;
+      c[4] += r * P.wmom[4 * S + s] + w * d0 * d0;
+      c[5] += r * P.wmom[5 * S + s] + w * d0 * d1;
+      c[6] += r * P.wmom[6 * S + s] + w * d0 * d2;
+      c[7] += r * P.wmom[7 * S + s] + w * d1 * d1;
+      c[8] += r * P.wmom[8 * S + s] + w * d1 * d2;
+      c[9] += r * P.wmom[9 * S + s] + w * d2 * d2;
+    }
+  }
+#pragma unroll
+  for (int m = 8; m >= ML; m >>= 1) {
+    const int qm = m / ML;
+#pragma unroll
+    for (int q = 0; q < qm; q++)
+#pragma unroll
+      for (int i = 0; i < 10; i++) a[q][i] += a[q + qm][i];
   }
 #pragma unroll
   for (int i = 0; i < 10; i++) {
 #pragma unroll
-    for (int m = 8; m >= 1; m >>= 1) a[i] += __shfl_xor(a[i], m, 16);
+    for (int m = ML / 2; m >= 1; m >>= 1) a[0][i] += __shfl_xor(a[0][i], m, ML);
   }
-  if (lane < 10) {
-    double v = a[0];
 #pragma unroll
-    for (int i = 1; i < 10; i++)
-      if (lane == i) v = a[i];
-    P.mstat[(long long)lane * G + k] = v;
+  for (int i = lane; i < 10; i += ML) {
+    double v = a[0][0];
+#pragma unroll
+    for (int t = 1; t < 10; t++)
+      if (i == t) v = a[0][t];
+    P.mstat[(long long)i * G + k] = v;
   }
 }
 
@@ -937,9 +960,26 @@ __global__ __launch_bounds__(kGB) void k_death(DevPtrs P) {
   }
 }
 
-// Log-likelihood over the voxel stream (GMEM.cu:1913-1978). A block covers
-// 16 supervoxels, one per 16-lane group; the K candidates' parameter packs
-// are loaded by lanes 0..K-1 of the group and broadcast within it.
+// Sum of a frame's per-block partials by wave 0 of the frame's last block, in
+// an order fixed by the partial index alone (lane l takes partials l, l+64,
+// ... into four interleaved sums, then a fixed butterfly), so the result does
+// not depend on the block shape of the kernel that produced the partials.
+// Every lane of wave 0 returns the total.
+__device__ inline double frame_sum_w64(const double* part, int nblk) {
+  double a4[4] = {0.0, 0.0, 0.0, 0.0};
+  const int l = threadIdx.x;
+  int i = l;
+  for (; i + 192 < nblk; i += 256) {
+#pragma unroll
+    for (int q = 0; q < 4; q++) a4[q] += load_coherent(part + i + 64 * q);
+  }
+  for (int q = 0; i < nblk; i += 64, q++) a4[q & 3] += load_coherent(part + i);
+  return wave_sum((a4[0] + a4[1]) + (a4[2] + a4[3]));
+}
+
+// Log-likelihood over the voxel stream (GMEM.cu:1913-1978). A block covers 16
+// supervoxels with GW lanes each; every lane reads the K candidates'
+// parameter packs itself (the same cache lines across its group).
 // Per voxel, d = (x - a_s) + (a_s - m_k) with the anchor a_s a voxel of the
 // supervoxel (exact in f32) and (a_s - m_k) rounded once from FP64, so the
 // f32 quadratic form keeps full relative precision far from the origin.
@@ -947,63 +987,88 @@ __global__ __launch_bounds__(kGB) void k_death(DevPtrs P) {
 // largest term factored out, img-weighted accumulation in FP64. A voxel whose
 // largest term is below the FP64 underflow of the reference's direct sum
 // (2^-1074) contributes nothing, as in the reference.
-template <int KM>
-__global__ __launch_bounds__(256) void k_loglik(DevPtrs P, KOpts O) {
-  __shared__ double red[4];
+//
+// Summation order, identical for every (GW, U): voxel i of a supervoxel is
+// added, in increasing i, to class sum i mod 16; the 16 class sums are added
+// as ((c + c^8) + ...^4) ...^1; a block's 16 supervoxel sums by a fixed
+// pairwise tree; the frame's block partials by frame_sum_w64.
+// All loads that depend only on the supervoxel index are issued before the
+// frame-state check, and the first pass of voxels (GW*U of them) before the
+// candidate parameters are formed, to shorten the chain of memory latencies.
+template <int KM, int GW, int U>
+__global__ __launch_bounds__(16 * GW) void k_loglik(DevPtrs P, KOpts O) {
+  constexpr int NC = 16 / GW;  // class sums per lane
+  constexpr int PASS = GW * U;
+  static_assert(PASS % 16 == 0, "a pass must cover whole classes");
+  __shared__ double svsum[16];
   const long long sb = (long long)blockIdx.x * kLLsv;
+  const int lane = threadIdx.x % GW, grp = threadIdx.x / GW;
+  const long long S = P.Scap;
+  const long long s = sb + grp;
+  const int K = O.K;
   const int f = P.sblk_frame[sb >> 8];
+  const int sf = P.sv_frame[s];
+  const long long beg = P.sv_beg[s];
+  const int len = P.sv_len[s];
+  const float a0 = P.anchor[s], a1 = P.anchor[S + s], a2 = P.anchor[2 * S + s];
+  int kk[KM];
+#pragma unroll
+  for (int j = 0; j < KM; j++) kk[j] = (j < K) ? P.ind[(long long)j * S + s] : -1;
   if (f < 0) return;
   FrameDev* F = P.fr + f;
   if (!F->active) return;
-  const int l16 = threadIdx.x & 15, grp = threadIdx.x >> 4;
-  const long long S = P.Scap;
-  const int K = O.K;
-  const long long s = sb + grp;
-  const bool valid = P.sv_frame[s] == f;
-  double acc = 0.0;
-  float pd0 = 0.f, pd1 = 0.f, pd2 = 0.f, pw[6] = {0, 0, 0, 0, 0, 0}, pg = -INFINITY;
-  float a0 = 0.f, a1 = 0.f, a2 = 0.f;
-  long long beg = 0;
-  int n = 0;
-  if (valid) {
-    a0 = P.anchor[s];
-    a1 = P.anchor[S + s];
-    a2 = P.anchor[2 * S + s];
-    beg = P.sv_beg[s];
-    n = P.sv_len[s];
-    if (l16 < K) {
-      const int k = P.ind[(long long)l16 * S + s];
-      if (k >= 0) {
-        const LLPack lp = P.llpack[k];
-        pd0 = (float)((double)a0 - lp.m[0]);
-        pd1 = (float)((double)a1 - lp.m[1]);
-        pd2 = (float)((double)a2 - lp.m[2]);
+  const int n = (sf == f) ? len : 0;
+  float vx[U], vy[U], vz[U], vw[U];
 #pragma unroll
-        for (int i = 0; i < 6; i++) pw[i] = lp.M[i];
-        pg = (float)(lp.lg2 - F->log_sum_alpha * 1.4426950408889634074);
-      }
+  for (int u = 0; u < U; u++) {
+    const int i = lane + GW * u;
+    vx[u] = vy[u] = vz[u] = vw[u] = 0.0f;
+    if (i < n) {
+      vx[u] = P.vx[beg + i];
+      vy[u] = P.vy[beg + i];
+      vz[u] = P.vz[beg + i];
+      vw[u] = P.vimg[beg + i];
     }
   }
+  const double lsa2 = F->log_sum_alpha * 1.4426950408889634074;
   float cd0[KM], cd1[KM], cd2[KM], cw[KM][6], cg[KM];
 #pragma unroll
   for (int j = 0; j < KM; j++) {
-    cd0[j] = __shfl(pd0, j, 16);
-    cd1[j] = __shfl(pd1, j, 16);
-    cd2[j] = __shfl(pd2, j, 16);
+    cd0[j] = cd1[j] = cd2[j] = 0.0f;
 #pragma unroll
-    for (int i = 0; i < 6; i++) cw[j][i] = __shfl(pw[i], j, 16);
-    cg[j] = (j < K) ? __shfl(pg, j, 16) : -INFINITY;
+    for (int i = 0; i < 6; i++) cw[j][i] = 0.0f;
+    cg[j] = -INFINITY;
+    if (n > 0 && kk[j] >= 0) {
+      const LLPack lp = P.llpack[kk[j]];
+      cd0[j] = (float)((double)a0 - lp.m[0]);
+      cd1[j] = (float)((double)a1 - lp.m[1]);
+      cd2[j] = (float)((double)a2 - lp.m[2]);
+#pragma unroll
+      for (int i = 0; i < 6; i++) cw[j][i] = lp.M[i];
+      cg[j] = (float)(lp.lg2 - lsa2);
+    }
   }
-  for (int i = l16; i < n; i += 16) {
-    const float x0 = P.vx[beg + i] - a0, x1 = P.vy[beg + i] - a1, x2 = P.vz[beg + i] - a2;
-    const float w = P.vimg[beg + i];
+  double cls[NC];
+#pragma unroll
+  for (int q = 0; q < NC; q++) cls[q] = 0.0;
+  // one voxel's term log2(sum_j 2^t_j), added to its class sum
+  auto term = [&](float x0, float x1, float x2, float w, double& acc) {
+#pragma clang fp contract(off)
+    x0 -= a0;
+    x1 -= a1;
+    x2 -= a2;
     float tj[KM];
     float mx = -INFINITY;
 #pragma unroll
     for (int j = 0; j < KM; j++) {
       const float d0 = x0 + cd0[j], d1 = x1 + cd1[j], d2 = x2 + cd2[j];
-      const float q = cw[j][0] * d0 * d0 + cw[j][3] * d1 * d1 + cw[j][5] * d2 * d2 +
-                      2.0f * (cw[j][1] * d0 * d1 + cw[j][2] * d0 * d2 + cw[j][4] * d1 * d2);
+      float q = __builtin_fmaf(cw[j][0] * d0, d0, 0.0f);
+      q = __builtin_fmaf(cw[j][3] * d1, d1, q);
+      q = __builtin_fmaf(cw[j][5] * d2, d2, q);
+      float o = cw[j][1] * d0 * d1;
+      o = __builtin_fmaf(cw[j][2] * d0, d2, o);
+      o = __builtin_fmaf(cw[j][4] * d1, d2, o);
+      q = __builtin_fmaf(2.0f, o, q);
       tj[j] = cg[j] - q;
       mx = fmaxf(mx, tj[j]);
     }
@@ -1013,16 +1078,52 @@ __global__ __launch_bounds__(256) void k_loglik(DevPtrs P, KOpts O) {
       for (int j = 0; j < KM; j++) sum += __builtin_amdgcn_exp2f(tj[j] - mx);
       acc += (double)w * (double)(mx + __builtin_amdgcn_logf(sum));
     }
+  };
+#pragma unroll
+  for (int u = 0; u < U; u++)
+    if (lane + GW * u < n) term(vx[u], vy[u], vz[u], vw[u], cls[u % NC]);
+  for (int base = PASS; base < n; base += PASS) {
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const int i = base + lane + GW * u;
+      if (i < n) term(P.vx[beg + i], P.vy[beg + i], P.vz[beg + i], P.vimg[beg + i], cls[u % NC]);
+    }
   }
-  acc *= 0.69314718055994530942;
-  int last;
-  const double tot = frame_reduce<4>(acc, P.part_s, blockIdx.x, F->s0 / kLLsv, F->nbl,
-                                     P.done + 4 * f + 3, red, &last);
-  if (last && threadIdx.x == 0) {
+  // class sums: in-lane stages (strides >= GW), then lane butterflies
+#pragma unroll
+  for (int m = 8; m >= GW; m >>= 1) {
+    const int qm = m / GW;
+#pragma unroll
+    for (int q = 0; q < qm; q++) cls[q] += cls[q + qm];
+  }
+  double tot = cls[0];
+#pragma unroll
+  for (int m = GW / 2; m >= 1; m >>= 1) tot += __shfl_xor(tot, m, GW);
+  if (lane == 0) svsum[grp] = tot * 0.69314718055994530942;
+  __syncthreads();
+  __shared__ int s_last;
+  if (threadIdx.x == 0) {
+    double t8[8];
+#pragma unroll
+    for (int i = 0; i < 8; i++) t8[i] = svsum[2 * i] + svsum[2 * i + 1];
+#pragma unroll
+    for (int i = 0; i < 4; i++) t8[i] = t8[2 * i] + t8[2 * i + 1];
+    const double t = (t8[0] + t8[1]) + (t8[2] + t8[3]);
+    P.part_s[blockIdx.x] = t;
+    __threadfence();
+    const unsigned prev = atomicAdd(P.done + 4 * f + 3, 1u);
+    s_last = (prev == (unsigned)(F->nbl - 1));
+  }
+  __syncthreads();
+  if (!s_last || threadIdx.x >= 64) return;
+  __threadfence();
+  const double ll = frame_sum_w64(P.part_s + F->s0 / kLLsv, F->nbl);
+  if (threadIdx.x == 0) {
+    P.done[4 * f + 3] = 0u;
     F->ll3 = F->ll2;
     F->ll2 = F->ll1;
     F->ll1 = F->ll;
-    F->ll = tot;
+    F->ll = ll;
     F->iter += 1;
     F->active = (keep_going(F->ll, F->ll1, F->ll2, F->ll3, F->tol) && F->iter <= F->max_iter) ? 1 : 0;
   }
@@ -1091,34 +1192,62 @@ hipError_t launch_next_priors(const DevPtrs& P, long long g0, long long ng, doub
 }
 
 template <int KM>
+static void launch_loglik(const DevPtrs& P, const KOpts& O, hipStream_t st) {
+  const unsigned g = nb(P.Scap, kLLsv);
+  const int gw = O.tiles.ll_gw, u = O.tiles.ll_u;
+  if (gw == 16)
+    k_loglik<KM, 16, 4><<<g, 256, 0, st>>>(P, O);
+  else if (gw == 8 && u == 4)
+    k_loglik<KM, 8, 4><<<g, 128, 0, st>>>(P, O);
+  else if (gw == 8)
+    k_loglik<KM, 8, 8><<<g, 128, 0, st>>>(P, O);
+  else if (u == 8)
+    k_loglik<KM, 4, 8><<<g, 64, 0, st>>>(P, O);
+  else
+    k_loglik<KM, 4, 16><<<g, 64, 0, st>>>(P, O);
+}
+
+static void launch_mstep_stats(const DevPtrs& P, const KOpts& O, hipStream_t st) {
+  const unsigned g = nb(P.Gcap, 16);
+  if (O.tiles.ms_lanes == 4)
+    k_mstep_stats<4><<<g, 64, 0, st>>>(P);
+  else if (O.tiles.ms_lanes == 8)
+    k_mstep_stats<8><<<g, 128, 0, st>>>(P);
+  else
+    k_mstep_stats<16><<<g, 256, 0, st>>>(P);
+}
+
+template <int KM>
 static hipError_t iteration_t(const DevPtrs& P, const KOpts& O, hipStream_t st, hipEvent_t e0,
                               hipEvent_t e1, int tk) {
-  const unsigned gb = nb(P.Gcap, kGB), sb = nb(P.Scap, 256);
+  const unsigned gb = nb(P.Gcap, kGB);
+  const int bs = O.tiles.sv_block, bk = O.tiles.knn_block;
   k_gauss_prep<<<gb, kGB, 0, st>>>(P, O);
   (void)scan(P.cell_cnt, P.Ccap + 1, P.cell_start, P.scan_blk, st);
   k_grid_scatter<<<gb, kGB, 0, st>>>(P);
   if (tk == TK_KNN && e0) (void)hipEventRecord(e0, st);
-  k_knn<KM><<<sb, 256, 0, st>>>(P, O);
+  k_knn<KM><<<nb(P.Scap, bk), bk, 0, st>>>(P, O);
   if (tk == TK_KNN && e1) (void)hipEventRecord(e1, st);
   if (tk == TK_ESTEP && e0) (void)hipEventRecord(e0, st);
-  k_estep<KM><<<sb, 256, 0, st>>>(P, O);
+  k_estep<KM><<<nb(P.Scap, bs), bs, 0, st>>>(P, O);
   if (tk == TK_ESTEP && e1) (void)hipEventRecord(e1, st);
   (void)scan(P.seg_cnt, P.Gcap + 1, P.seg_start, P.scan_blk, st);
-  k_pair_scatter<<<sb, 256, 0, st>>>(P, O);
+  k_pair_scatter<<<nb(P.Scap, bs), bs, 0, st>>>(P, O);
   k_seg_sort<<<nb(P.Gcap, 16), 256, 0, st>>>(P);
   if (tk == TK_MSTEP && e0) (void)hipEventRecord(e0, st);
-  k_mstep_stats<<<nb(P.Gcap, 16), 256, 0, st>>>(P);
+  launch_mstep_stats(P, O, st);
   k_mstep_update<<<gb, kGB, 0, st>>>(P, O);
   if (tk == TK_MSTEP && e1) (void)hipEventRecord(e1, st);
   k_death<<<gb, kGB, 0, st>>>(P);
   if (tk == TK_LOGLIK && e0) (void)hipEventRecord(e0, st);
-  k_loglik<KM><<<nb(P.Scap, kLLsv), 256, 0, st>>>(P, O);
+  launch_loglik<KM>(P, O, st);
   if (tk == TK_LOGLIK && e1) (void)hipEventRecord(e1, st);
   return hipGetLastError();
 }
 
 hipError_t launch_iteration(const DevPtrs& P, const KOpts& O, hipStream_t st, hipEvent_t e0,
                             hipEvent_t e1, int tk) {
+  if (!tiles_valid(O.tiles)) return hipErrorInvalidValue;
   switch (kmax_for(O.K)) {
     case 5:
       return iteration_t<5>(P, O, st, e0, e1, tk);

@@ -174,7 +174,15 @@ KOpts make_kopts(const tgmm_em_opts& o, const float* scale_reg) {
   k.lmin = o.lambda_min;
   k.lmax = o.lambda_max;
   k.emax = o.max_excentricity;
+  k.tiles = kDefaultTiles;
   return k;
+}
+
+bool tgmm::tiles_valid(const Tiles& t) {
+  auto in = [](int v, int a, int b, int c) { return v == a || v == b || v == c; };
+  return in(t.ll_gw, 4, 8, 16) && in(t.ll_u, 4, 8, 16) &&
+         (t.ll_gw * t.ll_u == 32 || t.ll_gw * t.ll_u == 64) && in(t.ms_lanes, 4, 8, 16) &&
+         in(t.sv_block, 64, 128, 256) && in(t.knn_block, 64, 128, 256);
 }
 
 // records (AoS, host) -> 27 x G SoA columns at [g0, g0+G) of a Gcap table
@@ -577,6 +585,130 @@ extern "C" int tgmm_batch_time_kernel(tgmm_batch* b, const char* name, int reps,
   (void)hipEventDestroy(e1);
   *avg_ms = tot / reps;
   if (bytes_per_launch) *bytes_per_launch = kernel_bytes(b, tk);
+  return TGMM_OK;
+}
+
+static Tiles from_public(const tgmm_tiles& t) {
+  return Tiles{t.ll_group, t.ll_unroll, t.mstep_lanes, t.sv_block, t.knn_block};
+}
+static tgmm_tiles to_public(const Tiles& t) {
+  return tgmm_tiles{t.ll_gw, t.ll_u, t.ms_lanes, t.sv_block, t.knn_block};
+}
+
+static void drop_graph(tgmm_batch* b) {
+  if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  b->gexec = nullptr;
+  b->graph = nullptr;
+}
+
+extern "C" int tgmm_batch_set_tiles(tgmm_batch* b, const tgmm_tiles* t) {
+  if (!b || !t) return TGMM_ERR_ARG;
+  const Tiles x = from_public(*t);
+  if (!tiles_valid(x)) {
+    tgmm_set_error("tgmm_batch_set_tiles: invalid tile shape {%d, %d, %d, %d, %d}", x.ll_gw,
+                   x.ll_u, x.ms_lanes, x.sv_block, x.knn_block);
+    return TGMM_ERR_ARG;
+  }
+  HIPCHK(hipSetDevice(b->device));
+  HIPCHK(hipStreamSynchronize(b->st));
+  b->ko.tiles = x;
+  drop_graph(b);  // recaptured by the next graph solve
+  return TGMM_OK;
+}
+
+extern "C" int tgmm_batch_get_tiles(tgmm_batch* b, tgmm_tiles* t) {
+  if (!b || !t) return TGMM_ERR_ARG;
+  *t = to_public(b->ko.tiles);
+  return TGMM_OK;
+}
+
+// Device time of kernel `tk` summed over `iters` eager iterations from the
+// initial state, with launch shapes `t`.
+static int time_shape(tgmm_batch* b, const Tiles& t, int tk, int iters, hipEvent_t e0,
+                      hipEvent_t e1, double* ms_out) {
+  if (int rc = tgmm_batch_reset(b)) return rc;
+  KOpts ko = b->ko;
+  ko.tiles = t;
+  double tot = 0.0;
+  for (int it = 0; it < iters; it++) {
+    HIPCHK(launch_iteration(b->ws.P, ko, b->st, e0, e1, tk));
+    HIPCHK(hipEventSynchronize(e1));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+    tot += ms;
+  }
+  *ms_out = tot / iters;
+  return TGMM_OK;
+}
+
+extern "C" int tgmm_batch_autotune(tgmm_batch* b, int reps, tgmm_tiles* chosen, char* report,
+                                   int report_cap) {
+  if (!b || reps < 1) return TGMM_ERR_ARG;
+  if (b->opts.max_iter < 1) {
+    tgmm_set_error("tgmm_batch_autotune: the batch runs no iterations");
+    return TGMM_ERR_ARG;
+  }
+  HIPCHK(hipSetDevice(b->device));
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  const int iters = std::min(reps, b->opts.max_iter);
+  Tiles best = b->ko.tiles;
+  std::string rep;
+  char line[160];
+  struct Knob {
+    const char* name;
+    int tk;
+    int n;
+    int v[5][2];
+  };
+  const Knob knobs[4] = {
+      {"loglik", TK_LOGLIK, 5, {{16, 4}, {8, 4}, {8, 8}, {4, 8}, {4, 16}}},
+      {"mstep", TK_MSTEP, 3, {{4, 0}, {8, 0}, {16, 0}}},
+      {"estep", TK_ESTEP, 3, {{64, 0}, {128, 0}, {256, 0}}},
+      {"knn", TK_KNN, 3, {{64, 0}, {128, 0}, {256, 0}}},
+  };
+  int rc = TGMM_OK;
+  for (int q = 0; q < 4 && rc == TGMM_OK; q++) {
+    const Knob& kb = knobs[q];
+    double tbest = 1e300;
+    Tiles pick = best;
+    for (int c = 0; c < kb.n; c++) {
+      Tiles t = best;
+      switch (q) {
+        case 0: t.ll_gw = kb.v[c][0]; t.ll_u = kb.v[c][1]; break;
+        case 1: t.ms_lanes = kb.v[c][0]; break;
+        case 2: t.sv_block = kb.v[c][0]; break;
+        default: t.knn_block = kb.v[c][0]; break;
+      }
+      double ms = 0.0;
+      if ((rc = time_shape(b, t, kb.tk, iters, e0, e1, &ms)) != TGMM_OK) break;
+      if (kb.v[c][1])
+        snprintf(line, sizeof(line), "%s %dx%d %.2f\n", kb.name, kb.v[c][0], kb.v[c][1],
+                 1000.0 * ms);
+      else
+        snprintf(line, sizeof(line), "%s %d %.2f\n", kb.name, kb.v[c][0], 1000.0 * ms);
+      rep += line;
+      if (ms < tbest) {
+        tbest = ms;
+        pick = t;
+      }
+    }
+    best = pick;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (rc != TGMM_OK) return rc;
+  b->ko.tiles = best;
+  drop_graph(b);
+  if (int r2 = tgmm_batch_reset(b)) return r2;
+  if (chosen) *chosen = to_public(best);
+  if (report && report_cap > 0) {
+    const size_t n = std::min((size_t)report_cap - 1, rep.size());
+    std::memcpy(report, rep.data(), n);
+    report[n] = 0;
+  }
   return TGMM_OK;
 }
 

@@ -590,6 +590,23 @@ extern "C" int tgmm_server_fetch(tgmm_server* s, long long ticket, tgmm_frame_de
   return TGMM_OK;
 }
 
+extern "C" int tgmm_server_set_tiles(tgmm_server* s, const tgmm_tiles* t) {
+  if (!s || !t) return TGMM_ERR_ARG;
+  const Tiles x{t->ll_group, t->ll_unroll, t->mstep_lanes, t->sv_block, t->knn_block};
+  if (!tiles_valid(x)) {
+    tgmm_set_error("tgmm_server_set_tiles: invalid tile shape");
+    return TGMM_ERR_ARG;
+  }
+  HIPCHK(hipSetDevice(s->device));
+  HIPCHK(hipStreamSynchronize(s->st));
+  s->ko.tiles = x;
+  if (s->gexec) (void)hipGraphExecDestroy(s->gexec);
+  if (s->graph) (void)hipGraphDestroy(s->graph);
+  s->gexec = nullptr;
+  s->graph = nullptr;
+  return TGMM_OK;
+}
+
 extern "C" int tgmm_server_destroy(tgmm_server* s) {
   if (!s) return TGMM_ERR_ARG;
   (void)hipSetDevice(s->device);

@@ -16,6 +16,7 @@ a machine without the built library raises immediately (there is no CPU
 fallback).
 """
 import ctypes
+import json
 import os
 
 import numpy as np
@@ -96,6 +97,28 @@ class ServerOpts(ctypes.Structure):
     ]
 
 
+class Tiles(ctypes.Structure):
+    """Launch shapes of the tunable kernels (tgmm_tiles). Results are bitwise
+    the same for every valid choice."""
+    _fields_ = [("ll_group", ctypes.c_int), ("ll_unroll", ctypes.c_int),
+                ("mstep_lanes", ctypes.c_int), ("sv_block", ctypes.c_int),
+                ("knn_block", ctypes.c_int)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+    @classmethod
+    def from_dict(cls, d):
+        t = cls()
+        for k, _ in cls._fields_:
+            setattr(t, k, int(d[k]))
+        return t
+
+
+DEFAULT_TILES = dict(ll_group=16, ll_unroll=4, mstep_lanes=16, sv_block=256, knn_block=256)
+
+
+
 _lib = None
 _vp = ctypes.c_void_p
 _pp = ctypes.POINTER(ctypes.c_void_p)
@@ -131,6 +154,11 @@ def load():
                                          ctypes.POINTER(ctypes.c_double),
                                          ctypes.POINTER(ctypes.c_double)]
     L.tgmm_batch_destroy.argtypes = [_vp]
+    L.tgmm_batch_set_tiles.argtypes = [_vp, ctypes.POINTER(Tiles)]
+    L.tgmm_batch_get_tiles.argtypes = [_vp, ctypes.POINTER(Tiles)]
+    L.tgmm_batch_autotune.argtypes = [_vp, ctypes.c_int, ctypes.POINTER(Tiles),
+                                      ctypes.c_char_p, ctypes.c_int]
+    L.tgmm_server_set_tiles.argtypes = [_vp, ctypes.POINTER(Tiles)]
     L.tgmm_server_create.argtypes = [ctypes.c_int, ctypes.POINTER(ServerOpts), _pp]
     L.tgmm_server_submit.argtypes = [_vp, ctypes.c_int, ctypes.POINTER(FrameDesc),
                                      ctypes.POINTER(ctypes.c_longlong)]
@@ -144,6 +172,12 @@ def load():
     L.tgmm_server_destroy.argtypes = [_vp]
     _lib = L
     return L
+
+
+def _tiles_cache_path():
+    """Autotuner cache: $TGMM_TILES_CACHE, else ~/.cache/tgmm_amd/tiles.json."""
+    return os.environ.get("TGMM_TILES_CACHE") or os.path.join(
+        os.path.expanduser("~"), ".cache", "tgmm_amd", "tiles.json")
 
 
 def _check(rc):
@@ -314,6 +348,59 @@ class Batch:
             o["iterations"] = self.desc[i].iterations
         return outs
 
+    def set_tiles(self, tiles):
+        t = tiles if isinstance(tiles, Tiles) else Tiles.from_dict(tiles)
+        _check(self.L.tgmm_batch_set_tiles(self.h, ctypes.byref(t)))
+
+    def get_tiles(self):
+        t = Tiles()
+        _check(self.L.tgmm_batch_get_tiles(self.h, ctypes.byref(t)))
+        return t.as_dict()
+
+    def signature(self):
+        """Workload key for the tile cache: K and log2 buckets of the total
+        supervoxels, Gaussians and voxels per supervoxel."""
+        S = sum(int(np.asarray(f.sv_ptr).shape[0] - 1) for f in self.frames)
+        G = sum(int(np.asarray(f.gm).shape[0]) for f in self.frames)
+        N = sum(int(np.asarray(f.img).shape[0]) for f in self.frames)
+        b = lambda x: int(round(np.log2(max(x, 1))))
+        return "K%d_S%d_G%d_v%d_f%d" % (self.opts.K, b(S), b(G), b(N / max(S, 1)),
+                                        b(len(self.frames)))
+
+    def autotune(self, reps=3, cache=True):
+        """Pick the tile shapes by timing the candidates on this batch
+        (tgmm_batch_autotune), or reuse the choice cached for the same
+        workload signature. Leaves the batch in its initial state. Returns
+        (tiles dict, report text or None when cached)."""
+        key = self.signature()
+        path = _tiles_cache_path()
+        table = {}
+        if cache and os.path.exists(path):
+            try:
+                with open(path) as fh:
+                    table = json.load(fh)
+            except (OSError, ValueError):
+                table = {}
+        if cache and key in table:
+            self.set_tiles(table[key])
+            self.reset()
+            return table[key], None
+        t = Tiles()
+        buf = ctypes.create_string_buffer(4096)
+        _check(self.L.tgmm_batch_autotune(self.h, int(reps), ctypes.byref(t), buf, 4096))
+        d = t.as_dict()
+        if cache:
+            table[key] = d
+            try:
+                os.makedirs(os.path.dirname(path), exist_ok=True)
+                tmp = path + ".%d" % os.getpid()
+                with open(tmp, "w") as fh:
+                    json.dump(table, fh, indent=1, sort_keys=True)
+                os.replace(tmp, path)
+            except OSError:
+                pass
+        return d, buf.value.decode()
+
     def time_kernel(self, name, reps):
         ms = ctypes.c_double()
         by = ctypes.c_double()
@@ -381,6 +468,10 @@ class Server:
         _check(self.L.tgmm_server_submit(self.h, int(stream), ctypes.byref(d), ctypes.byref(t)))
         self.shapes[t.value] = int(d.n_supervoxels)
         return t.value
+
+    def set_tiles(self, tiles):
+        t = tiles if isinstance(tiles, Tiles) else Tiles.from_dict(tiles)
+        _check(self.L.tgmm_server_set_tiles(self.h, ctypes.byref(t)))
 
     def step(self):
         p = ctypes.c_int()
