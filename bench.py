@@ -149,6 +149,12 @@ def main():
         acc["iterations"] = chk_iters
         print("# accuracy vs FP64 oracle: %s" % json.dumps(acc), file=sys.stderr, flush=True)
 
+    # launch shapes: the autotuner times the candidates on this batch (not
+    # part of the timed steps; results do not depend on the choice)
+    tiles, tune_report = batch.autotune(reps=3, cache=False)
+    print("# autotuned tiles %s\n%s" % (json.dumps(tiles), tune_report or ""),
+          file=sys.stderr, flush=True)
+
     # warmup + timed steps
     for _ in range(args.warmup):
         batch.reset()
@@ -243,6 +249,7 @@ def main():
         "latency_ms": {"p50": 1000 * _percentile(lat, 50), "p99": 1000 * _percentile(lat, 99)},
         "accuracy_vs_fp64": acc,
         "kernels": kern,
+        "tiles": tiles,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
@@ -251,7 +258,7 @@ def main():
     print(json.dumps(line), flush=True)
 
 
-def _serve(streams, opts, budget, order, arrivals=None, pct=(0.5, 0.5, 0.8)):
+def _serve(streams, opts, budget, order, arrivals=None, pct=(0.5, 0.5, 0.8), tiles=None):
     """Feed the frames of `streams` to a fresh server. Closed loop when
     `arrivals` is None (every stream keeps its next frame queued); otherwise
     arrivals[i] is the arrival time (s after the start) of each frame of
@@ -262,6 +269,8 @@ def _serve(streams, opts, budget, order, arrivals=None, pct=(0.5, 0.5, 0.8)):
     G = [fr[0].G for fr in streams]
     srv = em.Server(opts, hbm_budget_bytes=budget, max_frames_in_flight=2 * ns,
                     beta_pct=pct[0], nu_pct=pct[1], alpha_pct=pct[2])
+    if tiles is not None:
+        srv.set_tiles(tiles)
     nxt = [0] * ns
     open_t = {}
     recs, keep = [], {}
@@ -321,11 +330,16 @@ def serve_bench(args):
     budget = int(args.hbm_budget_gb * (1 << 30))
     pct = (0.5, 0.5, 0.8)
 
+    # launch shapes: autotuned on a batch of the streams' first frames
+    with em.Batch([fr[0] for fr in streams], opts) as tb:
+        tiles, _ = tb.autotune(reps=3, cache=False)
+    print("# autotuned tiles %s" % json.dumps(tiles), file=sys.stderr, flush=True)
+
     # warm-up: two frames per stream on a throw-away server
-    _serve([fr[:2] for fr in streams], opts, budget, set())
+    _serve([fr[:2] for fr in streams], opts, budget, set(), tiles=tiles)
 
     # closed loop: throughput
-    el, recs, keep = _serve(streams, opts, budget, {(0, 0), (0, 1)}, pct=pct)
+    el, recs, keep = _serve(streams, opts, budget, {(0, 0), (0, 1)}, pct=pct, tiles=tiles)
     thr = n_frames / el
     print("# closed loop: %d frames in %.3fs = %.1f frames/s" % (n_frames, el, thr),
           file=sys.stderr, flush=True)
@@ -335,7 +349,8 @@ def serve_bench(args):
     rng = np.random.default_rng(0xC0FFEE04)
     ns = len(streams)
     arrivals = [np.cumsum(rng.exponential(ns / qps, len(fr))) for fr in streams]
-    el2, recs2, _ = _serve(streams, opts, budget, set(), arrivals=arrivals, pct=pct)
+    el2, recs2, _ = _serve(streams, opts, budget, set(), arrivals=arrivals, pct=pct,
+                           tiles=tiles)
     lat = [r["done"] - r["arrival"] for r in recs2]
     qwait = [r["admit"] - r["arrival"] for r in recs2]
     lat_ms = {"p50": 1000 * _percentile(lat, 50), "p99": 1000 * _percentile(lat, 99),
@@ -400,6 +415,7 @@ def serve_bench(args):
                                       "p99": 1000 * _percentile(qwait, 99)}},
         "latency_ms": lat_ms,
         "accuracy_vs_fp64": acc,
+        "tiles": tiles,
         "roofline": None,
         "cpu_baseline": cpu,
     }

@@ -107,7 +107,8 @@ struct DevPtrs {
   int* ind;         // K*Scap
   float* rnk;       // K*Scap
   int* sblk_frame;  // Scap/256
-  int* qperm;       // Scap: supervoxels of each frame ordered by kNN grid cell
+  float4* q4;       // Scap: kNN queries of each frame in grid-cell order:
+                    // centroid x, y, z and the supervoxel index (int bits)
   int* qcell;       // Scap: grid cell of each supervoxel centroid
   // Gaussians
   int* g_frame;

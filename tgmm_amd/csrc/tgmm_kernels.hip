@@ -312,7 +312,9 @@ __global__ __launch_bounds__(256) void k_qcell_scatter(DevPtrs P) {
   const int c = P.qcell[s];
   const long long pos = F->s0 + (P.cell_start[c] - P.cell_start[F->c0]) +
                         atomicSub(P.cell_cnt + c, 1) - 1;
-  P.qperm[pos] = (int)s;
+  const long long S = P.Scap;
+  P.q4[pos] = make_float4(P.sv_cent[s], P.sv_cent[S + s], P.sv_cent[2 * S + s],
+                          __int_as_float((int)s));
 }
 
 // Iteration state of every frame (reference: ll = -1, -10, -100, -1000 and
@@ -532,7 +534,7 @@ __global__ __launch_bounds__(kGB) void k_grid_scatter(DevPtrs P) {
 // scaled distance is at least LB; the search stops once the K-th best
 // distance is below LB^2 with a relative margin far above f32 rounding, so
 // the result equals brute force (ties broken by the lower index). Lanes take
-// the supervoxels in grid-cell order (qperm).
+// the supervoxels in grid-cell order (q4).
 template <int KM>
 __global__ __launch_bounds__(256) void k_knn(DevPtrs P, KOpts O) {
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -540,11 +542,12 @@ __global__ __launch_bounds__(256) void k_knn(DevPtrs P, KOpts O) {
   if (f < 0 || !P.fr[f].active) return;
   const FrameDev* F = P.fr + f;
   if (i >= F->s0 + F->ns) return;
-  const long long s = P.qperm[i];
+  const float4 qq = P.q4[i];
+  const long long s = __float_as_int(qq.w);
   const long long S = P.Scap;
   const int K = O.K;
   const float s0 = O.scale[0], s1 = O.scale[1], s2 = O.scale[2];
-  const float q0 = P.sv_cent[s], q1 = P.sv_cent[S + s], q2 = P.sv_cent[2 * S + s];
+  const float q0 = qq.x, q1 = qq.y, q2 = qq.z;
   const int nx = F->dims[0], ny = F->dims[1], nz = F->dims[2];
   const double h = F->h;
   const double u[3] = {(double)q0 * s0, (double)q1 * s1, (double)q2 * s2};
@@ -561,31 +564,38 @@ __global__ __launch_bounds__(256) void k_knn(DevPtrs P, KOpts O) {
   }
   int found = 0;
   const long long c0 = F->c0;
-  // candidate update with a one-ahead prefetch of the packed references
-  auto scan_range = [&](int pb, int pe) {
-    if (pb >= pe) return;
-    float4 cur = P.cg4[pb];
-    for (int p = pb; p < pe; p++) {
-      const float4 nxt = (p + 1 < pe) ? P.cg4[p + 1] : cur;
-      const float d = knn_dist(q0, q1, q2, cur.x, cur.y, cur.z, s0, s1, s2);
-      const int idx = __float_as_int(cur.w);
-      if ((d <= 1e32f) && lex_less(d, idx, bd[KM - 1], bi[KM - 1])) {
-        found++;
-        float cd = d;
-        int ci2 = idx;
+  // candidate update; the packed references of a row segment are loaded 8 at
+  // a time, all 8 requests in flight before the first is used (a segment
+  // spans about 6 references at 2 Gaussians per cell)
+  auto consider = [&](const float4 cur) {
+    const float d = knn_dist(q0, q1, q2, cur.x, cur.y, cur.z, s0, s1, s2);
+    const int idx = __float_as_int(cur.w);
+    if ((d <= 1e32f) && lex_less(d, idx, bd[KM - 1], bi[KM - 1])) {
+      found++;
+      float cd = d;
+      int ci2 = idx;
 #pragma unroll
-        for (int j = 0; j < KM; j++) {
-          if (lex_less(cd, ci2, bd[j], bi[j])) {
-            const float td = bd[j];
-            const int ti = bi[j];
-            bd[j] = cd;
-            bi[j] = ci2;
-            cd = td;
-            ci2 = ti;
-          }
+      for (int j = 0; j < KM; j++) {
+        if (lex_less(cd, ci2, bd[j], bi[j])) {
+          const float td = bd[j];
+          const int ti = bi[j];
+          bd[j] = cd;
+          bi[j] = ci2;
+          cd = td;
+          ci2 = ti;
         }
       }
-      cur = nxt;
+    }
+  };
+  auto scan_range = [&](int pb, int pe) {
+    for (int p = pb; p < pe; p += 8) {
+      float4 v[8];
+#pragma unroll
+      for (int t = 0; t < 8; t++)
+        v[t] = (p + t < pe) ? P.cg4[p + t] : make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+      for (int t = 0; t < 8; t++)
+        if (p + t < pe) consider(v[t]);
     }
   };
   auto lower_bound_sq = [&](int r) -> double {

@@ -92,7 +92,7 @@ int ws_alloc(Workspace* w, const WsShape& sh, int K) {
   AL(P.ind, (long long)K * S);
   AL(P.rnk, (long long)K * S);
   AL(P.sblk_frame, S / kAlign);
-  AL(P.qperm, S);
+  AL(P.q4, S);
   AL(P.qcell, S);
   AL(P.g_frame, G);
   AL(P.gs, GS_NFIELDS * G);
@@ -128,7 +128,7 @@ void ws_free(Workspace* w) {
 
 long long ws_bytes_estimate(const WsShape& sh, int K) {
   const long long S = sh.Scap, G = sh.Gcap, C = sh.Ccap, V = sh.Vcap;
-  return 16 * V + S * (4 + 8 + 4 + 12 + 80 + 80 + 12 + 16LL * K + 1) +
+  return 16 * V + S * (16 + 8 + 4 + 12 + 80 + 80 + 12 + 16LL * K + 1) +
          G * (4 + 8 * GS_NFIELDS * 2 + 1 + 8 + 24 + 8 + 12 + 4 + 8 + 12 + 4 + 1) + 8 * C + 64 * sh.Fcap;
 }
 
