@@ -183,7 +183,7 @@ int tgmm_batch_solve(tgmm_batch* b, int mode);
 int tgmm_batch_fetch(tgmm_batch* b, tgmm_frame_desc* frames, int nframes);
 /* Launch shapes of the tunable kernels. Every valid choice gives bitwise the
  * same results (the kernels fix their summation order independently of the
- * shape); only the speed changes. Defaults: {16, 4, 16, 256, 256}. */
+ * shape); only the speed changes. Defaults: {16, 4, 16, 256, 256, 8}. */
 typedef struct tgmm_tiles {
   int ll_group;    /* lanes per supervoxel, log-likelihood: 4, 8 or 16 */
   int ll_unroll;   /* voxels per lane per pass: 4, 8 or 16, with
@@ -191,6 +191,7 @@ typedef struct tgmm_tiles {
   int mstep_lanes; /* lanes per Gaussian, M-step statistics: 4, 8 or 16 */
   int sv_block;    /* threads per block, E-step and pair scatter: 64, 128, 256 */
   int knn_block;   /* threads per block, kNN search: 64, 128 or 256 */
+  int knn_chunk;   /* candidate references loaded together, kNN search: 4 or 8 */
 } tgmm_tiles;
 
 int tgmm_batch_set_tiles(tgmm_batch* b, const tgmm_tiles* tiles);

@@ -104,11 +104,12 @@ def test_soft_assignments_match_oracle(max_sv):
     # responsibilities after 10 iterations: the FP32 r of each side feeds its
     # M-step, so the Gaussians drift apart by ~1e-7 relative and a score of a
     # few hundred nats moves by ~1e-5: r agrees to 1e-4 relative where it
-    # enters the M-step (r > 1e-3) and to 1e-6 absolute everywhere
+    # enters the M-step (r > 1e-3) and to 1e-6 absolute below that
     same = out["ind"] == ind_o
     big = same & (rnk_o > 1e-3)
     assert rel_err(out["rnk"][big], rnk_o[big]) < 1e-4
-    assert np.max(np.abs(out["rnk"][same] - rnk_o[same])) < 1e-6
+    small = same & ~big
+    assert np.max(np.abs(out["rnk"][small] - rnk_o[small])) < 1e-6
 
 
 def test_modes_bitwise_identical(c1):

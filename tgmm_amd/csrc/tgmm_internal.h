@@ -80,8 +80,9 @@ struct Tiles {
   int ms_lanes;   // lanes per Gaussian, M-step statistics: 4, 8, 16
   int sv_block;   // threads per block, E-step and pair scatter: 64, 128, 256
   int knn_block;  // threads per block, kNN search: 64, 128, 256
+  int knn_chunk;  // candidate references loaded together, kNN search: 4, 8
 };
-constexpr Tiles kDefaultTiles = {16, 4, 16, 256, 256};
+constexpr Tiles kDefaultTiles = {16, 4, 16, 256, 256, 8};
 bool tiles_valid(const Tiles& t);
 
 struct KOpts {

@@ -535,7 +535,7 @@ __global__ __launch_bounds__(kGB) void k_grid_scatter(DevPtrs P) {
 // distance is below LB^2 with a relative margin far above f32 rounding, so
 // the result equals brute force (ties broken by the lower index). Lanes take
 // the supervoxels in grid-cell order (q4).
-template <int KM>
+template <int KM, int KC>
 __global__ __launch_bounds__(256) void k_knn(DevPtrs P, KOpts O) {
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   const int f = P.sblk_frame[((long long)blockIdx.x * blockDim.x) >> 8];
@@ -564,9 +564,6 @@ __global__ __launch_bounds__(256) void k_knn(DevPtrs P, KOpts O) {
   }
   int found = 0;
   const long long c0 = F->c0;
-  // candidate update; the packed references of a row segment are loaded 8 at
-  // a time, all 8 requests in flight before the first is used (a segment
-  // spans about 6 references at 2 Gaussians per cell)
   auto consider = [&](const float4 cur) {
     const float d = knn_dist(q0, q1, q2, cur.x, cur.y, cur.z, s0, s1, s2);
     const int idx = __float_as_int(cur.w);
@@ -587,17 +584,40 @@ __global__ __launch_bounds__(256) void k_knn(DevPtrs P, KOpts O) {
       }
     }
   };
-  auto scan_range = [&](int pb, int pe) {
-    for (int p = pb; p < pe; p += 8) {
-      float4 v[8];
+  // shells 0 and 1 together: the 9 rows of the 3x3x3 block. All 18 row
+  // bounds are loaded first; the rows' references are then walked as one
+  // flat list, KC loads in flight at a time.
+  {
+    const int xlo = max(c[0] - 1, 0), xhi = min(c[0] + 1, nx - 1);
+    int pb[9], cum[10];
+    cum[0] = 0;
 #pragma unroll
-      for (int t = 0; t < 8; t++)
-        v[t] = (p + t < pe) ? P.cg4[p + t] : make_float4(0.f, 0.f, 0.f, 0.f);
-#pragma unroll
-      for (int t = 0; t < 8; t++)
-        if (p + t < pe) consider(v[t]);
+    for (int t = 0; t < 9; t++) {
+      const int z = c[2] + t / 3 - 1, y = c[1] + t % 3 - 1;
+      const bool ok = z >= 0 && z < nz && y >= 0 && y < ny;
+      const long long row = ok ? c0 + ((long long)z * ny + y) * nx : c0;
+      const int b0 = ok ? P.cell_start[row + xlo] : 0;
+      const int e0 = ok ? P.cell_start[row + xhi + 1] : 0;
+      pb[t] = b0;
+      cum[t + 1] = cum[t] + (e0 - b0);
     }
-  };
+    const int T = cum[9];
+    for (int q = 0; q < T; q += KC) {
+      float4 v[KC];
+#pragma unroll
+      for (int u = 0; u < KC; u++) {
+        const int x = q + u;
+        int addr = pb[0] + x;
+#pragma unroll
+        for (int t = 1; t < 9; t++)
+          if (x >= cum[t]) addr = pb[t] + (x - cum[t]);
+        v[u] = (x < T) ? P.cg4[addr] : make_float4(0.f, 0.f, 0.f, 0.f);
+      }
+#pragma unroll
+      for (int u = 0; u < KC; u++)
+        if (q + u < T) consider(v[u]);
+    }
+  }
   auto lower_bound_sq = [&](int r) -> double {
     double lb = INFINITY;
     for (int a = 0; a < 3; a++) {
@@ -617,22 +637,8 @@ __global__ __launch_bounds__(256) void k_knn(DevPtrs P, KOpts O) {
       if (t == K - 1) kth = bd[t];
     return found >= K && (double)kth < lb2;
   };
-  // shells 0 and 1 together: the 3x3 rows of the 3x3x3 block, all 18 row
-  // bounds loaded before any candidate is read
-  {
-    const int xlo = max(c[0] - 1, 0), xhi = min(c[0] + 1, nx - 1);
-    int pb[9], pe[9];
-#pragma unroll
-    for (int t = 0; t < 9; t++) {
-      const int z = c[2] + t / 3 - 1, y = c[1] + t % 3 - 1;
-      const bool ok = z >= 0 && z < nz && y >= 0 && y < ny;
-      const long long row = ok ? c0 + ((long long)z * ny + y) * nx : c0;
-      pb[t] = ok ? P.cell_start[row + xlo] : 0;
-      pe[t] = ok ? P.cell_start[row + xhi + 1] : 0;
-    }
-#pragma unroll
-    for (int t = 0; t < 9; t++) scan_range(pb[t], pe[t]);
-  }
+  // further shells: the rows of shell r; a row on a z or y face of the
+  // shell contributes its whole x range, any other row its two end cells
   const int rmax = max(nx, max(ny, nz));
   if (!done_at(1)) {
     for (int r = 2; r <= rmax; r++) {
@@ -641,16 +647,29 @@ __global__ __launch_bounds__(256) void k_knn(DevPtrs P, KOpts O) {
       for (int z = zlo; z <= zhi; z++) {
         const bool zface = (z == c[2] - r) || (z == c[2] + r);
         for (int y = ylo; y <= yhi; y++) {
-          const bool yface = (y == c[1] - r) || (y == c[1] + r);
+          const bool face = zface || (y == c[1] - r) || (y == c[1] + r);
           const long long row = c0 + ((long long)z * ny + y) * nx;
-          if (zface || yface) {
-            const int xa = max(c[0] - r, 0), xb = min(c[0] + r, nx - 1);
-            scan_range(P.cell_start[row + xa], P.cell_start[row + xb + 1]);
-          } else {
-            if (c[0] - r >= 0)
-              scan_range(P.cell_start[row + c[0] - r], P.cell_start[row + c[0] - r + 1]);
-            if (c[0] + r < nx)
-              scan_range(P.cell_start[row + c[0] + r], P.cell_start[row + c[0] + r + 1]);
+#pragma unroll 1
+          for (int g = 0; g < 2; g++) {
+            int xa, xb;
+            if (face) {
+              xa = g == 0 ? max(c[0] - r, 0) : 1;
+              xb = g == 0 ? min(c[0] + r, nx - 1) : 0;
+            } else {
+              xa = xb = g == 0 ? c[0] - r : c[0] + r;
+              if (xa < 0 || xa >= nx) xb = xa - 1;
+            }
+            if (xa > xb) continue;
+            const int pb = P.cell_start[row + xa], pe = P.cell_start[row + xb + 1];
+            for (int p = pb; p < pe; p += 4) {
+              float4 v[4];
+#pragma unroll
+              for (int u = 0; u < 4; u++)
+                v[u] = (p + u < pe) ? P.cg4[p + u] : make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+              for (int u = 0; u < 4; u++)
+                if (p + u < pe) consider(v[u]);
+            }
           }
         }
       }
@@ -988,8 +1007,8 @@ __device__ inline double frame_sum_w64(const double* part, int nblk) {
 }
 
 // Log-likelihood over the voxel stream (GMEM.cu:1913-1978). A block covers 16
-// supervoxels with GW lanes each; every lane reads the K candidates'
-// parameter packs itself (the same cache lines across its group).
+// supervoxels with GW lanes each; each candidate's parameter pack is read by
+// one lane of the group and broadcast to the others.
 // Per voxel, d = (x - a_s) + (a_s - m_k) with the anchor a_s a voxel of the
 // supervoxel (exact in f32) and (a_s - m_k) rounded once from FP64, so the
 // f32 quadratic form keeps full relative precision far from the origin.
@@ -1021,9 +1040,14 @@ __global__ __launch_bounds__(16 * GW) void k_loglik(DevPtrs P, KOpts O) {
   const long long beg = P.sv_beg[s];
   const int len = P.sv_len[s];
   const float a0 = P.anchor[s], a1 = P.anchor[S + s], a2 = P.anchor[2 * S + s];
-  int kk[KM];
+  // lane l of the group owns candidates l, l+GW, ... (R rounds)
+  constexpr int R = (KM + GW - 1) / GW;
+  int kk[R];
 #pragma unroll
-  for (int j = 0; j < KM; j++) kk[j] = (j < K) ? P.ind[(long long)j * S + s] : -1;
+  for (int r = 0; r < R; r++) {
+    const int j = lane + GW * r;
+    kk[r] = (j < K) ? P.ind[(long long)j * S + s] : -1;
+  }
   if (f < 0) return;
   FrameDev* F = P.fr + f;
   if (!F->active) return;
@@ -1041,22 +1065,35 @@ __global__ __launch_bounds__(16 * GW) void k_loglik(DevPtrs P, KOpts O) {
     }
   }
   const double lsa2 = F->log_sum_alpha * 1.4426950408889634074;
+  // each candidate's pack is read once per group, by its owner lane, and
+  // broadcast to the group
+  float pd0[R], pd1[R], pd2[R], pw[R][6], pg[R];
+#pragma unroll
+  for (int r = 0; r < R; r++) {
+    pd0[r] = pd1[r] = pd2[r] = 0.0f;
+#pragma unroll
+    for (int i = 0; i < 6; i++) pw[r][i] = 0.0f;
+    pg[r] = -INFINITY;
+    if (n > 0 && kk[r] >= 0) {
+      const LLPack lp = P.llpack[kk[r]];
+      pd0[r] = (float)((double)a0 - lp.m[0]);
+      pd1[r] = (float)((double)a1 - lp.m[1]);
+      pd2[r] = (float)((double)a2 - lp.m[2]);
+#pragma unroll
+      for (int i = 0; i < 6; i++) pw[r][i] = lp.M[i];
+      pg[r] = (float)(lp.lg2 - lsa2);
+    }
+  }
   float cd0[KM], cd1[KM], cd2[KM], cw[KM][6], cg[KM];
 #pragma unroll
   for (int j = 0; j < KM; j++) {
-    cd0[j] = cd1[j] = cd2[j] = 0.0f;
+    const int r = j / GW, src = j % GW;
+    cd0[j] = __shfl(pd0[r], src, GW);
+    cd1[j] = __shfl(pd1[r], src, GW);
+    cd2[j] = __shfl(pd2[r], src, GW);
 #pragma unroll
-    for (int i = 0; i < 6; i++) cw[j][i] = 0.0f;
-    cg[j] = -INFINITY;
-    if (n > 0 && kk[j] >= 0) {
-      const LLPack lp = P.llpack[kk[j]];
-      cd0[j] = (float)((double)a0 - lp.m[0]);
-      cd1[j] = (float)((double)a1 - lp.m[1]);
-      cd2[j] = (float)((double)a2 - lp.m[2]);
-#pragma unroll
-      for (int i = 0; i < 6; i++) cw[j][i] = lp.M[i];
-      cg[j] = (float)(lp.lg2 - lsa2);
-    }
+    for (int i = 0; i < 6; i++) cw[j][i] = __shfl(pw[r][i], src, GW);
+    cg[j] = (j < K) ? __shfl(pg[r], src, GW) : -INFINITY;
   }
   double cls[NC];
 #pragma unroll
@@ -1236,7 +1273,10 @@ static hipError_t iteration_t(const DevPtrs& P, const KOpts& O, hipStream_t st, 
   (void)scan(P.cell_cnt, P.Ccap + 1, P.cell_start, P.scan_blk, st);
   k_grid_scatter<<<gb, kGB, 0, st>>>(P);
   if (tk == TK_KNN && e0) (void)hipEventRecord(e0, st);
-  k_knn<KM><<<nb(P.Scap, bk), bk, 0, st>>>(P, O);
+  if (O.tiles.knn_chunk == 4)
+    k_knn<KM, 4><<<nb(P.Scap, bk), bk, 0, st>>>(P, O);
+  else
+    k_knn<KM, 8><<<nb(P.Scap, bk), bk, 0, st>>>(P, O);
   if (tk == TK_KNN && e1) (void)hipEventRecord(e1, st);
   if (tk == TK_ESTEP && e0) (void)hipEventRecord(e0, st);
   k_estep<KM><<<nb(P.Scap, bs), bs, 0, st>>>(P, O);

@@ -182,7 +182,8 @@ bool tgmm::tiles_valid(const Tiles& t) {
   auto in = [](int v, int a, int b, int c) { return v == a || v == b || v == c; };
   return in(t.ll_gw, 4, 8, 16) && in(t.ll_u, 4, 8, 16) &&
          (t.ll_gw * t.ll_u == 32 || t.ll_gw * t.ll_u == 64) && in(t.ms_lanes, 4, 8, 16) &&
-         in(t.sv_block, 64, 128, 256) && in(t.knn_block, 64, 128, 256);
+         in(t.sv_block, 64, 128, 256) && in(t.knn_block, 64, 128, 256) &&
+         (t.knn_chunk == 4 || t.knn_chunk == 8);
 }
 
 // records (AoS, host) -> 27 x G SoA columns at [g0, g0+G) of a Gcap table
@@ -589,10 +590,10 @@ extern "C" int tgmm_batch_time_kernel(tgmm_batch* b, const char* name, int reps,
 }
 
 static Tiles from_public(const tgmm_tiles& t) {
-  return Tiles{t.ll_group, t.ll_unroll, t.mstep_lanes, t.sv_block, t.knn_block};
+  return Tiles{t.ll_group, t.ll_unroll, t.mstep_lanes, t.sv_block, t.knn_block, t.knn_chunk};
 }
 static tgmm_tiles to_public(const Tiles& t) {
-  return tgmm_tiles{t.ll_gw, t.ll_u, t.ms_lanes, t.sv_block, t.knn_block};
+  return tgmm_tiles{t.ll_gw, t.ll_u, t.ms_lanes, t.sv_block, t.knn_block, t.knn_chunk};
 }
 
 static void drop_graph(tgmm_batch* b) {
@@ -661,13 +662,13 @@ extern "C" int tgmm_batch_autotune(tgmm_batch* b, int reps, tgmm_tiles* chosen, 
     const char* name;
     int tk;
     int n;
-    int v[5][2];
+    int v[6][2];
   };
   const Knob knobs[4] = {
       {"loglik", TK_LOGLIK, 5, {{16, 4}, {8, 4}, {8, 8}, {4, 8}, {4, 16}}},
       {"mstep", TK_MSTEP, 3, {{4, 0}, {8, 0}, {16, 0}}},
       {"estep", TK_ESTEP, 3, {{64, 0}, {128, 0}, {256, 0}}},
-      {"knn", TK_KNN, 3, {{64, 0}, {128, 0}, {256, 0}}},
+      {"knn", TK_KNN, 6, {{64, 4}, {128, 4}, {256, 4}, {64, 8}, {128, 8}, {256, 8}}},
   };
   int rc = TGMM_OK;
   for (int q = 0; q < 4 && rc == TGMM_OK; q++) {
@@ -680,7 +681,7 @@ extern "C" int tgmm_batch_autotune(tgmm_batch* b, int reps, tgmm_tiles* chosen, 
         case 0: t.ll_gw = kb.v[c][0]; t.ll_u = kb.v[c][1]; break;
         case 1: t.ms_lanes = kb.v[c][0]; break;
         case 2: t.sv_block = kb.v[c][0]; break;
-        default: t.knn_block = kb.v[c][0]; break;
+        default: t.knn_block = kb.v[c][0]; t.knn_chunk = kb.v[c][1]; break;
       }
       double ms = 0.0;
       if ((rc = time_shape(b, t, kb.tk, iters, e0, e1, &ms)) != TGMM_OK) break;

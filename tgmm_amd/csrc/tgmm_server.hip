@@ -481,7 +481,7 @@ static int retire(tgmm_server* s, int f) {
         if (g.supervoxelNum < cap) g.supervoxelIdx[g.supervoxelNum++] = (int)sv;
       }
     }
-  const FrameDev& F = s->fh[f];
+  const FrameDev F = s->fh[f];  // a copy: the slot's entry is cleared below
   t.ll = F.ll;
   t.iters = F.iter - 1;
   t.state = 2;
@@ -592,7 +592,8 @@ extern "C" int tgmm_server_fetch(tgmm_server* s, long long ticket, tgmm_frame_de
 
 extern "C" int tgmm_server_set_tiles(tgmm_server* s, const tgmm_tiles* t) {
   if (!s || !t) return TGMM_ERR_ARG;
-  const Tiles x{t->ll_group, t->ll_unroll, t->mstep_lanes, t->sv_block, t->knn_block};
+  const Tiles x{t->ll_group, t->ll_unroll, t->mstep_lanes, t->sv_block, t->knn_block,
+                t->knn_chunk};
   if (!tiles_valid(x)) {
     tgmm_set_error("tgmm_server_set_tiles: invalid tile shape");
     return TGMM_ERR_ARG;
