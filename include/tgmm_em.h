@@ -183,7 +183,7 @@ int tgmm_batch_solve(tgmm_batch* b, int mode);
 int tgmm_batch_fetch(tgmm_batch* b, tgmm_frame_desc* frames, int nframes);
 /* Launch shapes of the tunable kernels. Every valid choice gives bitwise the
  * same results (the kernels fix their summation order independently of the
- * shape); only the speed changes. Defaults: {16, 4, 16, 256, 256, 8}. */
+ * shape); only the speed changes. Defaults: {16, 4, 16, 256, 256, 8, 4, 1}. */
 typedef struct tgmm_tiles {
   int ll_group;    /* lanes per supervoxel, log-likelihood: 4, 8 or 16 */
   int ll_unroll;   /* voxels per lane per pass: 4, 8 or 16, with
@@ -192,6 +192,8 @@ typedef struct tgmm_tiles {
   int sv_block;    /* threads per block, E-step and pair scatter: 64, 128, 256 */
   int knn_block;   /* threads per block, kNN search: 64, 128 or 256 */
   int knn_chunk;   /* candidate references loaded together, kNN search: 4 or 8 */
+  int knn_cell;    /* mean Gaussians per kNN grid cell: 2, 4 or 6 */
+  int knn_lanes;   /* lanes per query, kNN search: 1, 2 or 4 */
 } tgmm_tiles;
 
 int tgmm_batch_set_tiles(tgmm_batch* b, const tgmm_tiles* tiles);

@@ -63,3 +63,37 @@ def next_priors(gm, beta_pct, nu_pct, alpha_pct, alpha_total):
     g["nu_k"] = g["nu_k"] * 2.0
     g["W_k"] = g["W_k"] / 2.0
     return g
+
+
+def _butterfly64(v):
+    """Sum over 64 lanes by xor-butterfly (strides 32 .. 1); lane 0's value."""
+    a = list(v)
+    for m in (32, 16, 8, 4, 2, 1):
+        a = [a[i] + a[i ^ m] for i in range(64)]
+    return a[0]
+
+
+def device_alpha_sum(alpha):
+    """The device's per-frame sum of alpha_k, in its summation order: blocks of
+    64 Gaussians over the 256-aligned range, each summed by a 64-lane
+    butterfly; the block partials summed by one 64-lane wave, lane l taking
+    partials l, l+64, ... into four interleaved sums, then a butterfly."""
+    a = [float(x) for x in np.asarray(alpha, np.float64)]
+    nb = ((max(len(a), 1) + 255) // 256) * 4
+    a += [0.0] * (nb * 64 - len(a))
+    parts = [_butterfly64(a[64 * b:64 * (b + 1)]) for b in range(nb)]
+    acc = []
+    for t in range(64):
+        a4 = [0.0, 0.0, 0.0, 0.0]
+        i = t
+        while i + 192 < nb:
+            for q in range(4):
+                a4[q] += parts[i + 64 * q]
+            i += 256
+        q = 0
+        while i < nb:
+            a4[q & 3] += parts[i]
+            i += 64
+            q += 1
+        acc.append((a4[0] + a4[1]) + (a4[2] + a4[3]))
+    return _butterfly64(acc)

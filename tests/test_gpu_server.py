@@ -4,7 +4,7 @@ updatePriors, and every frame's result is the same alone and in the mix."""
 import numpy as np
 import pytest
 
-from helpers import next_priors
+from helpers import device_alpha_sum, next_priors
 from tgmm_amd import em, synth
 
 pytestmark = pytest.mark.gpu
@@ -57,7 +57,10 @@ def test_stream_priors_on_device_match_host_restatement():
     same(o[0], ref0)
     prev = ref0
     for t in (1, 2):
-        g = next_priors(prev["gm"], PCT[0], PCT[1], PCT[2], float(prev["gm"]["alpha_k"].sum()))
+        # the stream's alpha total is the device's frame sum (its own order)
+        tot = device_alpha_sum(prev["gm"]["alpha_k"])
+        assert abs(tot - prev["gm"]["alpha_k"].sum()) <= 1e-13 * tot
+        g = next_priors(prev["gm"], PCT[0], PCT[1], PCT[2], tot)
         ref = batch_one(fr[t], g)
         same(o[t], ref)
         prev = ref

@@ -22,7 +22,7 @@ namespace tgmm {
 constexpr int kAlign = 256;      // supervoxel / Gaussian range alignment
 constexpr int kGB = 64;          // threads per block, Gaussian-lane kernels
 constexpr int kSB = 256;         // threads per block, supervoxel-lane kernels
-constexpr int kLLsv = 16;        // supervoxels per log-likelihood block (one per 16 lanes)
+constexpr int kLLsv = 64;        // supervoxels per log-likelihood block
 constexpr int kScanTile = 4096;  // elements per scan block
 constexpr int kPairShift = 4;    // pair code = (s << 4) | j, j < 16
 
@@ -81,8 +81,11 @@ struct Tiles {
   int sv_block;   // threads per block, E-step and pair scatter: 64, 128, 256
   int knn_block;  // threads per block, kNN search: 64, 128, 256
   int knn_chunk;  // candidate references loaded together, kNN search: 4, 8
+  int knn_cell;   // mean Gaussians per kNN grid cell: 2, 4, 6 (set per frame
+                  // when its grid is built)
+  int knn_lanes;  // lanes per query, kNN search: 1, 2, 4
 };
-constexpr Tiles kDefaultTiles = {16, 4, 16, 256, 256, 8};
+constexpr Tiles kDefaultTiles = {16, 4, 16, 256, 256, 8, 4, 1};
 bool tiles_valid(const Tiles& t);
 
 struct KOpts {

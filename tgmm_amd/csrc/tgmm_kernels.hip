@@ -213,7 +213,7 @@ __global__ __launch_bounds__(256) void k_moments(DevPtrs P) {
 }
 
 // One block per frame: bounding box of the scaled kNN centroids and the grid
-// shape (about two means per cell, at most ncap cells).
+// shape (about knn_cell means per cell, at most ncap cells).
 __global__ __launch_bounds__(1024) void k_grid_setup(DevPtrs P, KOpts O) {
   const int f = (P.only_frame >= 0) ? P.only_frame : (int)blockIdx.x;
   if (f >= P.Fcap || (P.only_frame < 0 && f >= P.nframes)) return;
@@ -249,7 +249,7 @@ __global__ __launch_bounds__(1024) void k_grid_setup(DevPtrs P, KOpts O) {
     org[a] = ok ? red[a][0] : 0.0;
     ext[a] = ok ? fmax(red[3 + a][0] - red[a][0], 0.0) : 0.0;
   }
-  const double target = fmax(1.0, ceil(0.5 * (double)F->ng));
+  const double target = fmax(1.0, ceil((double)F->ng / (double)O.tiles.knn_cell));
   const double vol = fmax(ext[0], 1e-3) * fmax(ext[1], 1e-3) * fmax(ext[2], 1e-3);
   double h = cbrt(vol / target);
   if (!(h > 0.0) || !(h < 1e30)) h = 1.0;
@@ -533,15 +533,55 @@ __global__ __launch_bounds__(kGB) void k_grid_scatter(DevPtrs P) {
 // every unvisited mean lies beyond an open face of the visited block, so its
 // scaled distance is at least LB; the search stops once the K-th best
 // distance is below LB^2 with a relative margin far above f32 rounding, so
-// the result equals brute force (ties broken by the lower index). Lanes take
-// the supervoxels in grid-cell order (q4).
-template <int KM, int KC>
+// the result equals brute force (ties broken by the lower index). Queries are
+// taken in grid-cell order (q4). QL lanes share a query: each scans every
+// QL-th reference of the visited cells into its own sorted list, and the
+// lists are merged by a butterfly after each shell (a reference seen by two
+// lanes of a merged list is kept once), so every lane of the group holds the
+// K nearest of all references seen.
+template <int KM, int QL>
+__device__ inline void knn_merge(float (&bd)[KM], int (&bi)[KM]) {
+#pragma unroll
+  for (int m = QL / 2; m >= 1; m >>= 1) {
+    float od[KM];
+    int oi[KM];
+#pragma unroll
+    for (int t = 0; t < KM; t++) {
+      od[t] = __shfl_xor(bd[t], m, QL);
+      oi[t] = __shfl_xor(bi[t], m, QL);
+    }
+#pragma unroll
+    for (int t = 0; t < KM; t++) {
+      bool dup = false;
+#pragma unroll
+      for (int j = 0; j < KM; j++) dup |= (bi[j] == oi[t]);
+      float cd = od[t];
+      int ci = oi[t];
+      if (dup || !lex_less(cd, ci, bd[KM - 1], bi[KM - 1])) continue;
+#pragma unroll
+      for (int j = 0; j < KM; j++) {
+        if (lex_less(cd, ci, bd[j], bi[j])) {
+          const float td = bd[j];
+          const int ti = bi[j];
+          bd[j] = cd;
+          bi[j] = ci;
+          cd = td;
+          ci = ti;
+        }
+      }
+    }
+  }
+}
+
+template <int KM, int KC, int QL>
 __global__ __launch_bounds__(256) void k_knn(DevPtrs P, KOpts O) {
-  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  const int f = P.sblk_frame[((long long)blockIdx.x * blockDim.x) >> 8];
+  const int sub = threadIdx.x % QL;
+  const long long qpb = blockDim.x / QL;  // queries per block
+  const long long i = (long long)blockIdx.x * qpb + threadIdx.x / QL;
+  const int f = P.sblk_frame[((long long)blockIdx.x * qpb) >> 8];
   if (f < 0 || !P.fr[f].active) return;
   const FrameDev* F = P.fr + f;
-  if (i >= F->s0 + F->ns) return;
+  if (i >= F->s0 + F->ns) return;  // the whole group of the query leaves
   const float4 qq = P.q4[i];
   const long long s = __float_as_int(qq.w);
   const long long S = P.Scap;
@@ -562,13 +602,11 @@ __global__ __launch_bounds__(256) void k_knn(DevPtrs P, KOpts O) {
     bd[j] = INFINITY;
     bi[j] = 0x7fffffff;
   }
-  int found = 0;
   const long long c0 = F->c0;
   auto consider = [&](const float4 cur) {
     const float d = knn_dist(q0, q1, q2, cur.x, cur.y, cur.z, s0, s1, s2);
     const int idx = __float_as_int(cur.w);
     if ((d <= 1e32f) && lex_less(d, idx, bd[KM - 1], bi[KM - 1])) {
-      found++;
       float cd = d;
       int ci2 = idx;
 #pragma unroll
@@ -586,7 +624,7 @@ __global__ __launch_bounds__(256) void k_knn(DevPtrs P, KOpts O) {
   };
   // shells 0 and 1 together: the 9 rows of the 3x3x3 block. All 18 row
   // bounds are loaded first; the rows' references are then walked as one
-  // flat list, KC loads in flight at a time.
+  // flat list, KC loads per lane in flight at a time.
   {
     const int xlo = max(c[0] - 1, 0), xhi = min(c[0] + 1, nx - 1);
     int pb[9], cum[10];
@@ -602,22 +640,23 @@ __global__ __launch_bounds__(256) void k_knn(DevPtrs P, KOpts O) {
       cum[t + 1] = cum[t] + (e0 - b0);
     }
     const int T = cum[9];
-    for (int q = 0; q < T; q += KC) {
+    for (int q = 0; q < T; q += KC * QL) {
       float4 v[KC];
 #pragma unroll
-      for (int u = 0; u < KC; u++) {
-        const int x = q + u;
+      for (int w = 0; w < KC; w++) {
+        const int x = q + w * QL + sub;
         int addr = pb[0] + x;
 #pragma unroll
         for (int t = 1; t < 9; t++)
           if (x >= cum[t]) addr = pb[t] + (x - cum[t]);
-        v[u] = (x < T) ? P.cg4[addr] : make_float4(0.f, 0.f, 0.f, 0.f);
+        v[w] = (x < T) ? P.cg4[addr] : make_float4(0.f, 0.f, 0.f, 0.f);
       }
 #pragma unroll
-      for (int u = 0; u < KC; u++)
-        if (q + u < T) consider(v[u]);
+      for (int w = 0; w < KC; w++)
+        if (q + w * QL + sub < T) consider(v[w]);
     }
   }
+  if (QL > 1) knn_merge<KM, QL>(bd, bi);
   auto lower_bound_sq = [&](int r) -> double {
     double lb = INFINITY;
     for (int a = 0; a < 3; a++) {
@@ -628,6 +667,8 @@ __global__ __launch_bounds__(256) void k_knn(DevPtrs P, KOpts O) {
     lb = fmax(lb, 0.0) * (1.0 - 1e-5);
     return lb * lb;
   };
+  // done once the K-th best is below the bound (a finite K-th best means K
+  // references were found)
   auto done_at = [&](int r) -> bool {
     const double lb2 = lower_bound_sq(r);
     if (lb2 == INFINITY) return true;  // every cell visited
@@ -635,7 +676,7 @@ __global__ __launch_bounds__(256) void k_knn(DevPtrs P, KOpts O) {
 #pragma unroll
     for (int t = 0; t < KM; t++)
       if (t == K - 1) kth = bd[t];
-    return found >= K && (double)kth < lb2;
+    return (double)kth < lb2;
   };
   // further shells: the rows of shell r; a row on a z or y face of the
   // shell contributes its whole x range, any other row its two end cells
@@ -661,24 +702,27 @@ __global__ __launch_bounds__(256) void k_knn(DevPtrs P, KOpts O) {
             }
             if (xa > xb) continue;
             const int pb = P.cell_start[row + xa], pe = P.cell_start[row + xb + 1];
-            for (int p = pb; p < pe; p += 4) {
+            for (int p = pb + sub; p < pe; p += 4 * QL) {
               float4 v[4];
 #pragma unroll
-              for (int u = 0; u < 4; u++)
-                v[u] = (p + u < pe) ? P.cg4[p + u] : make_float4(0.f, 0.f, 0.f, 0.f);
+              for (int w = 0; w < 4; w++)
+                v[w] = (p + w * QL < pe) ? P.cg4[p + w * QL] : make_float4(0.f, 0.f, 0.f, 0.f);
 #pragma unroll
-              for (int u = 0; u < 4; u++)
-                if (p + u < pe) consider(v[u]);
+              for (int w = 0; w < 4; w++)
+                if (p + w * QL < pe) consider(v[w]);
             }
           }
         }
       }
+      if (QL > 1) knn_merge<KM, QL>(bd, bi);
       if (done_at(r)) break;
     }
   }
+  if (sub == 0) {
 #pragma unroll
-  for (int j = 0; j < KM; j++)
-    if (j < K) P.ind[(long long)j * S + s] = (bd[j] <= 1e32f) ? bi[j] : -1;
+    for (int j = 0; j < KM; j++)
+      if (j < K) P.ind[(long long)j * S + s] = (bd[j] <= 1e32f) ? bi[j] : -1;
+  }
 }
 
 // Responsibilities of the K candidates of each supervoxel. The reference's
@@ -1006,9 +1050,12 @@ __device__ inline double frame_sum_w64(const double* part, int nblk) {
   return wave_sum((a4[0] + a4[1]) + (a4[2] + a4[3]));
 }
 
-// Log-likelihood over the voxel stream (GMEM.cu:1913-1978). A block covers 16
-// supervoxels with GW lanes each; each candidate's parameter pack is read by
-// one lane of the group and broadcast to the others.
+// Log-likelihood over the voxel stream (GMEM.cu:1913-1978). A block covers
+// kLLsv = 64 supervoxels with 16 groups of GW lanes; group g takes supervoxels
+// g, g+16, g+32, g+48 of the block in turn, with the next one's header
+// (frame id, voxel range, anchor, candidate indices) loaded while the current
+// one is evaluated. Each candidate's parameter pack is read by one lane of
+// the group and broadcast to the others.
 // Per voxel, d = (x - a_s) + (a_s - m_k) with the anchor a_s a voxel of the
 // supervoxel (exact in f32) and (a_s - m_k) rounded once from FP64, so the
 // f32 quadratic form keeps full relative precision far from the origin.
@@ -1019,144 +1066,162 @@ __device__ inline double frame_sum_w64(const double* part, int nblk) {
 //
 // Summation order, identical for every (GW, U): voxel i of a supervoxel is
 // added, in increasing i, to class sum i mod 16; the 16 class sums are added
-// as ((c + c^8) + ...^4) ...^1; a block's 16 supervoxel sums by a fixed
+// as ((c + c^8) + ...^4) ...^1; a block's 64 supervoxel sums by a fixed
 // pairwise tree; the frame's block partials by frame_sum_w64.
-// All loads that depend only on the supervoxel index are issued before the
-// frame-state check, and the first pass of voxels (GW*U of them) before the
-// candidate parameters are formed, to shorten the chain of memory latencies.
+template <int KM, int R>
+struct SvHead {
+  long long beg;
+  int n;
+  float a0, a1, a2;
+  int kk[R];
+};
+
+template <int KM, int GW, int R>
+__device__ inline SvHead<KM, R> load_head(const DevPtrs& P, long long s, int f, int lane, int K) {
+  const long long S = P.Scap;
+  SvHead<KM, R> h;
+  const int sf = P.sv_frame[s];
+  h.beg = P.sv_beg[s];
+  const int len = P.sv_len[s];
+  h.a0 = P.anchor[s];
+  h.a1 = P.anchor[S + s];
+  h.a2 = P.anchor[2 * S + s];
+#pragma unroll
+  for (int r = 0; r < R; r++) {
+    const int j = lane + GW * r;
+    h.kk[r] = (j < K) ? P.ind[(long long)j * S + s] : -1;
+  }
+  h.n = (sf == f) ? len : 0;
+  return h;
+}
+
 template <int KM, int GW, int U>
 __global__ __launch_bounds__(16 * GW) void k_loglik(DevPtrs P, KOpts O) {
   constexpr int NC = 16 / GW;  // class sums per lane
   constexpr int PASS = GW * U;
+  constexpr int R = (KM + GW - 1) / GW;  // candidates owned per lane
+  constexpr int SPG = kLLsv / 16;        // supervoxels per group
   static_assert(PASS % 16 == 0, "a pass must cover whole classes");
-  __shared__ double svsum[16];
+  __shared__ double svsum[kLLsv];
+  __shared__ int s_last;
   const long long sb = (long long)blockIdx.x * kLLsv;
   const int lane = threadIdx.x % GW, grp = threadIdx.x / GW;
-  const long long S = P.Scap;
-  const long long s = sb + grp;
   const int K = O.K;
   const int f = P.sblk_frame[sb >> 8];
-  const int sf = P.sv_frame[s];
-  const long long beg = P.sv_beg[s];
-  const int len = P.sv_len[s];
-  const float a0 = P.anchor[s], a1 = P.anchor[S + s], a2 = P.anchor[2 * S + s];
-  // lane l of the group owns candidates l, l+GW, ... (R rounds)
-  constexpr int R = (KM + GW - 1) / GW;
-  int kk[R];
-#pragma unroll
-  for (int r = 0; r < R; r++) {
-    const int j = lane + GW * r;
-    kk[r] = (j < K) ? P.ind[(long long)j * S + s] : -1;
-  }
   if (f < 0) return;
   FrameDev* F = P.fr + f;
   if (!F->active) return;
-  const int n = (sf == f) ? len : 0;
-  float vx[U], vy[U], vz[U], vw[U];
-#pragma unroll
-  for (int u = 0; u < U; u++) {
-    const int i = lane + GW * u;
-    vx[u] = vy[u] = vz[u] = vw[u] = 0.0f;
-    if (i < n) {
-      vx[u] = P.vx[beg + i];
-      vy[u] = P.vy[beg + i];
-      vz[u] = P.vz[beg + i];
-      vw[u] = P.vimg[beg + i];
-    }
-  }
   const double lsa2 = F->log_sum_alpha * 1.4426950408889634074;
-  // each candidate's pack is read once per group, by its owner lane, and
-  // broadcast to the group
-  float pd0[R], pd1[R], pd2[R], pw[R][6], pg[R];
-#pragma unroll
-  for (int r = 0; r < R; r++) {
-    pd0[r] = pd1[r] = pd2[r] = 0.0f;
-#pragma unroll
-    for (int i = 0; i < 6; i++) pw[r][i] = 0.0f;
-    pg[r] = -INFINITY;
-    if (n > 0 && kk[r] >= 0) {
-      const LLPack lp = P.llpack[kk[r]];
-      pd0[r] = (float)((double)a0 - lp.m[0]);
-      pd1[r] = (float)((double)a1 - lp.m[1]);
-      pd2[r] = (float)((double)a2 - lp.m[2]);
-#pragma unroll
-      for (int i = 0; i < 6; i++) pw[r][i] = lp.M[i];
-      pg[r] = (float)(lp.lg2 - lsa2);
-    }
-  }
-  float cd0[KM], cd1[KM], cd2[KM], cw[KM][6], cg[KM];
-#pragma unroll
-  for (int j = 0; j < KM; j++) {
-    const int r = j / GW, src = j % GW;
-    cd0[j] = __shfl(pd0[r], src, GW);
-    cd1[j] = __shfl(pd1[r], src, GW);
-    cd2[j] = __shfl(pd2[r], src, GW);
-#pragma unroll
-    for (int i = 0; i < 6; i++) cw[j][i] = __shfl(pw[r][i], src, GW);
-    cg[j] = (j < K) ? __shfl(pg[r], src, GW) : -INFINITY;
-  }
-  double cls[NC];
-#pragma unroll
-  for (int q = 0; q < NC; q++) cls[q] = 0.0;
-  // one voxel's term log2(sum_j 2^t_j), added to its class sum
-  auto term = [&](float x0, float x1, float x2, float w, double& acc) {
-#pragma clang fp contract(off)
-    x0 -= a0;
-    x1 -= a1;
-    x2 -= a2;
-    float tj[KM];
-    float mx = -INFINITY;
-#pragma unroll
-    for (int j = 0; j < KM; j++) {
-      const float d0 = x0 + cd0[j], d1 = x1 + cd1[j], d2 = x2 + cd2[j];
-      float q = __builtin_fmaf(cw[j][0] * d0, d0, 0.0f);
-      q = __builtin_fmaf(cw[j][3] * d1, d1, q);
-      q = __builtin_fmaf(cw[j][5] * d2, d2, q);
-      float o = cw[j][1] * d0 * d1;
-      o = __builtin_fmaf(cw[j][2] * d0, d2, o);
-      o = __builtin_fmaf(cw[j][4] * d1, d2, o);
-      q = __builtin_fmaf(2.0f, o, q);
-      tj[j] = cg[j] - q;
-      mx = fmaxf(mx, tj[j]);
-    }
-    if (mx > -1074.0f) {
-      float sum = 0.0f;
-#pragma unroll
-      for (int j = 0; j < KM; j++) sum += __builtin_amdgcn_exp2f(tj[j] - mx);
-      acc += (double)w * (double)(mx + __builtin_amdgcn_logf(sum));
-    }
-  };
-#pragma unroll
-  for (int u = 0; u < U; u++)
-    if (lane + GW * u < n) term(vx[u], vy[u], vz[u], vw[u], cls[u % NC]);
-  for (int base = PASS; base < n; base += PASS) {
+  SvHead<KM, R> h = load_head<KM, GW, R>(P, sb + grp, f, lane, K);
+  for (int it = 0; it < SPG; it++) {
+    const SvHead<KM, R> c = h;
+    if (it + 1 < SPG) h = load_head<KM, GW, R>(P, sb + grp + 16 * (it + 1), f, lane, K);
+    const int n = c.n;
+    float vx[U], vy[U], vz[U], vw[U];
 #pragma unroll
     for (int u = 0; u < U; u++) {
-      const int i = base + lane + GW * u;
-      if (i < n) term(P.vx[beg + i], P.vy[beg + i], P.vz[beg + i], P.vimg[beg + i], cls[u % NC]);
+      const int i = lane + GW * u;
+      vx[u] = vy[u] = vz[u] = vw[u] = 0.0f;
+      if (i < n) {
+        vx[u] = P.vx[c.beg + i];
+        vy[u] = P.vy[c.beg + i];
+        vz[u] = P.vz[c.beg + i];
+        vw[u] = P.vimg[c.beg + i];
+      }
     }
+    float pd0[R], pd1[R], pd2[R], pw[R][6], pg[R];
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+      pd0[r] = pd1[r] = pd2[r] = 0.0f;
+#pragma unroll
+      for (int i = 0; i < 6; i++) pw[r][i] = 0.0f;
+      pg[r] = -INFINITY;
+      if (n > 0 && c.kk[r] >= 0) {
+        const LLPack lp = P.llpack[c.kk[r]];
+        pd0[r] = (float)((double)c.a0 - lp.m[0]);
+        pd1[r] = (float)((double)c.a1 - lp.m[1]);
+        pd2[r] = (float)((double)c.a2 - lp.m[2]);
+#pragma unroll
+        for (int i = 0; i < 6; i++) pw[r][i] = lp.M[i];
+        pg[r] = (float)(lp.lg2 - lsa2);
+      }
+    }
+    float cd0[KM], cd1[KM], cd2[KM], cw[KM][6], cg[KM];
+#pragma unroll
+    for (int j = 0; j < KM; j++) {
+      const int r = j / GW, src = j % GW;
+      cd0[j] = __shfl(pd0[r], src, GW);
+      cd1[j] = __shfl(pd1[r], src, GW);
+      cd2[j] = __shfl(pd2[r], src, GW);
+#pragma unroll
+      for (int i = 0; i < 6; i++) cw[j][i] = __shfl(pw[r][i], src, GW);
+      cg[j] = (j < K) ? __shfl(pg[r], src, GW) : -INFINITY;
+    }
+    double cls[NC];
+#pragma unroll
+    for (int q = 0; q < NC; q++) cls[q] = 0.0;
+    // one voxel's term log2(sum_j 2^t_j), added to its class sum
+    auto term = [&](float x0, float x1, float x2, float w, double& acc) {
+#pragma clang fp contract(off)
+      x0 -= c.a0;
+      x1 -= c.a1;
+      x2 -= c.a2;
+      float tj[KM];
+      float mx = -INFINITY;
+#pragma unroll
+      for (int j = 0; j < KM; j++) {
+        const float d0 = x0 + cd0[j], d1 = x1 + cd1[j], d2 = x2 + cd2[j];
+        float q = __builtin_fmaf(cw[j][0] * d0, d0, 0.0f);
+        q = __builtin_fmaf(cw[j][3] * d1, d1, q);
+        q = __builtin_fmaf(cw[j][5] * d2, d2, q);
+        float o = cw[j][1] * d0 * d1;
+        o = __builtin_fmaf(cw[j][2] * d0, d2, o);
+        o = __builtin_fmaf(cw[j][4] * d1, d2, o);
+        q = __builtin_fmaf(2.0f, o, q);
+        tj[j] = cg[j] - q;
+        mx = fmaxf(mx, tj[j]);
+      }
+      if (mx > -1074.0f) {
+        float sum = 0.0f;
+#pragma unroll
+        for (int j = 0; j < KM; j++) sum += __builtin_amdgcn_exp2f(tj[j] - mx);
+        acc += (double)w * (double)(mx + __builtin_amdgcn_logf(sum));
+      }
+    };
+#pragma unroll
+    for (int u = 0; u < U; u++)
+      if (lane + GW * u < n) term(vx[u], vy[u], vz[u], vw[u], cls[u % NC]);
+    for (int base = PASS; base < n; base += PASS) {
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        const int i = base + lane + GW * u;
+        if (i < n)
+          term(P.vx[c.beg + i], P.vy[c.beg + i], P.vz[c.beg + i], P.vimg[c.beg + i],
+               cls[u % NC]);
+      }
+    }
+    // class sums: in-lane stages (strides >= GW), then lane butterflies
+#pragma unroll
+    for (int m = 8; m >= GW; m >>= 1) {
+      const int qm = m / GW;
+#pragma unroll
+      for (int q = 0; q < qm; q++) cls[q] += cls[q + qm];
+    }
+    double tot = cls[0];
+#pragma unroll
+    for (int m = GW / 2; m >= 1; m >>= 1) tot += __shfl_xor(tot, m, GW);
+    if (lane == 0) svsum[grp + 16 * it] = tot * 0.69314718055994530942;
   }
-  // class sums: in-lane stages (strides >= GW), then lane butterflies
-#pragma unroll
-  for (int m = 8; m >= GW; m >>= 1) {
-    const int qm = m / GW;
-#pragma unroll
-    for (int q = 0; q < qm; q++) cls[q] += cls[q + qm];
-  }
-  double tot = cls[0];
-#pragma unroll
-  for (int m = GW / 2; m >= 1; m >>= 1) tot += __shfl_xor(tot, m, GW);
-  if (lane == 0) svsum[grp] = tot * 0.69314718055994530942;
   __syncthreads();
-  __shared__ int s_last;
   if (threadIdx.x == 0) {
-    double t8[8];
+    double t[kLLsv / 2];
 #pragma unroll
-    for (int i = 0; i < 8; i++) t8[i] = svsum[2 * i] + svsum[2 * i + 1];
+    for (int i = 0; i < kLLsv / 2; i++) t[i] = svsum[2 * i] + svsum[2 * i + 1];
 #pragma unroll
-    for (int i = 0; i < 4; i++) t8[i] = t8[2 * i] + t8[2 * i + 1];
-    const double t = (t8[0] + t8[1]) + (t8[2] + t8[3]);
-    P.part_s[blockIdx.x] = t;
+    for (int w = kLLsv / 4; w >= 1; w >>= 1)
+#pragma unroll
+      for (int i = 0; i < w; i++) t[i] = t[2 * i] + t[2 * i + 1];
+    P.part_s[blockIdx.x] = t[0];
     __threadfence();
     const unsigned prev = atomicAdd(P.done + 4 * f + 3, 1u);
     s_last = (prev == (unsigned)(F->nbl - 1));
@@ -1265,18 +1330,30 @@ static void launch_mstep_stats(const DevPtrs& P, const KOpts& O, hipStream_t st)
 }
 
 template <int KM>
+static void launch_knn(const DevPtrs& P, const KOpts& O, hipStream_t st) {
+  const int bk = O.tiles.knn_block, ql = O.tiles.knn_lanes;
+  const unsigned g = nb(P.Scap, bk / ql);
+  if (O.tiles.knn_chunk == 4) {
+    if (ql == 1) k_knn<KM, 4, 1><<<g, bk, 0, st>>>(P, O);
+    else if (ql == 2) k_knn<KM, 4, 2><<<g, bk, 0, st>>>(P, O);
+    else k_knn<KM, 4, 4><<<g, bk, 0, st>>>(P, O);
+  } else {
+    if (ql == 1) k_knn<KM, 8, 1><<<g, bk, 0, st>>>(P, O);
+    else if (ql == 2) k_knn<KM, 8, 2><<<g, bk, 0, st>>>(P, O);
+    else k_knn<KM, 8, 4><<<g, bk, 0, st>>>(P, O);
+  }
+}
+
+template <int KM>
 static hipError_t iteration_t(const DevPtrs& P, const KOpts& O, hipStream_t st, hipEvent_t e0,
                               hipEvent_t e1, int tk) {
   const unsigned gb = nb(P.Gcap, kGB);
-  const int bs = O.tiles.sv_block, bk = O.tiles.knn_block;
+  const int bs = O.tiles.sv_block;
   k_gauss_prep<<<gb, kGB, 0, st>>>(P, O);
   (void)scan(P.cell_cnt, P.Ccap + 1, P.cell_start, P.scan_blk, st);
   k_grid_scatter<<<gb, kGB, 0, st>>>(P);
   if (tk == TK_KNN && e0) (void)hipEventRecord(e0, st);
-  if (O.tiles.knn_chunk == 4)
-    k_knn<KM, 4><<<nb(P.Scap, bk), bk, 0, st>>>(P, O);
-  else
-    k_knn<KM, 8><<<nb(P.Scap, bk), bk, 0, st>>>(P, O);
+  launch_knn<KM>(P, O, st);
   if (tk == TK_KNN && e1) (void)hipEventRecord(e1, st);
   if (tk == TK_ESTEP && e0) (void)hipEventRecord(e0, st);
   k_estep<KM><<<nb(P.Scap, bs), bs, 0, st>>>(P, O);

@@ -183,7 +183,8 @@ bool tgmm::tiles_valid(const Tiles& t) {
   return in(t.ll_gw, 4, 8, 16) && in(t.ll_u, 4, 8, 16) &&
          (t.ll_gw * t.ll_u == 32 || t.ll_gw * t.ll_u == 64) && in(t.ms_lanes, 4, 8, 16) &&
          in(t.sv_block, 64, 128, 256) && in(t.knn_block, 64, 128, 256) &&
-         (t.knn_chunk == 4 || t.knn_chunk == 8);
+         (t.knn_chunk == 4 || t.knn_chunk == 8) && in(t.knn_cell, 2, 4, 6) &&
+         in(t.knn_lanes, 1, 2, 4);
 }
 
 // records (AoS, host) -> 27 x G SoA columns at [g0, g0+G) of a Gcap table
@@ -350,6 +351,7 @@ int batch_upload(tgmm_batch* b, const std::vector<FrameSrc>& fs) {
   HIPCHK(hipMemsetAsync(P.cell_cnt, 0, (sh.Ccap + 1) * 4, st));
   HIPCHK(hipMemsetAsync(P.done, 0, 4 * sh.Fcap * 4, st));
   HIPCHK(launch_prep(P, b->ko, st));
+  b->grid_cell = b->ko.tiles.knn_cell;
   HIPCHK(launch_reset(P, st));
   HIPCHK(hipStreamSynchronize(st));
   // keep the host copies the fetch needs
@@ -590,10 +592,12 @@ extern "C" int tgmm_batch_time_kernel(tgmm_batch* b, const char* name, int reps,
 }
 
 static Tiles from_public(const tgmm_tiles& t) {
-  return Tiles{t.ll_group, t.ll_unroll, t.mstep_lanes, t.sv_block, t.knn_block, t.knn_chunk};
+  return Tiles{t.ll_group,  t.ll_unroll, t.mstep_lanes, t.sv_block,
+               t.knn_block, t.knn_chunk, t.knn_cell,    t.knn_lanes};
 }
 static tgmm_tiles to_public(const Tiles& t) {
-  return tgmm_tiles{t.ll_gw, t.ll_u, t.ms_lanes, t.sv_block, t.knn_block, t.knn_chunk};
+  return tgmm_tiles{t.ll_gw,     t.ll_u,      t.ms_lanes, t.sv_block,
+                    t.knn_block, t.knn_chunk, t.knn_cell, t.knn_lanes};
 }
 
 static void drop_graph(tgmm_batch* b) {
@@ -615,6 +619,11 @@ extern "C" int tgmm_batch_set_tiles(tgmm_batch* b, const tgmm_tiles* t) {
   HIPCHK(hipStreamSynchronize(b->st));
   b->ko.tiles = x;
   drop_graph(b);  // recaptured by the next graph solve
+  if (x.knn_cell != b->grid_cell) {  // the kNN grids are sized by knn_cell
+    HIPCHK(launch_prep(b->ws.P, b->ko, b->st));
+    HIPCHK(hipStreamSynchronize(b->st));
+    b->grid_cell = x.knn_cell;
+  }
   return TGMM_OK;
 }
 
@@ -628,9 +637,13 @@ extern "C" int tgmm_batch_get_tiles(tgmm_batch* b, tgmm_tiles* t) {
 // initial state, with launch shapes `t`.
 static int time_shape(tgmm_batch* b, const Tiles& t, int tk, int iters, hipEvent_t e0,
                       hipEvent_t e1, double* ms_out) {
-  if (int rc = tgmm_batch_reset(b)) return rc;
   KOpts ko = b->ko;
   ko.tiles = t;
+  if (t.knn_cell != b->grid_cell) {  // rebuild the kNN grids for this density
+    HIPCHK(launch_prep(b->ws.P, ko, b->st));
+    b->grid_cell = t.knn_cell;
+  }
+  if (int rc = tgmm_batch_reset(b)) return rc;
   double tot = 0.0;
   for (int it = 0; it < iters; it++) {
     HIPCHK(launch_iteration(b->ws.P, ko, b->st, e0, e1, tk));
@@ -662,16 +675,20 @@ extern "C" int tgmm_batch_autotune(tgmm_batch* b, int reps, tgmm_tiles* chosen, 
     const char* name;
     int tk;
     int n;
-    int v[6][2];
+    int v[18][3];
   };
-  const Knob knobs[4] = {
+  const Knob knobs[5] = {
       {"loglik", TK_LOGLIK, 5, {{16, 4}, {8, 4}, {8, 8}, {4, 8}, {4, 16}}},
       {"mstep", TK_MSTEP, 3, {{4, 0}, {8, 0}, {16, 0}}},
       {"estep", TK_ESTEP, 3, {{64, 0}, {128, 0}, {256, 0}}},
-      {"knn", TK_KNN, 6, {{64, 4}, {128, 4}, {256, 4}, {64, 8}, {128, 8}, {256, 8}}},
+      {"knn_cell", TK_KNN, 3, {{2, 0}, {4, 0}, {6, 0}}},
+      {"knn", TK_KNN, 18, {{64, 4, 1},  {128, 4, 1}, {256, 4, 1}, {64, 8, 1},  {128, 8, 1},
+                           {256, 8, 1}, {64, 4, 2},  {128, 4, 2}, {256, 4, 2}, {64, 8, 2},
+                           {128, 8, 2}, {256, 8, 2}, {64, 4, 4},  {128, 4, 4}, {256, 4, 4},
+                           {64, 8, 4},  {128, 8, 4}, {256, 8, 4}}},
   };
   int rc = TGMM_OK;
-  for (int q = 0; q < 4 && rc == TGMM_OK; q++) {
+  for (int q = 0; q < 5 && rc == TGMM_OK; q++) {
     const Knob& kb = knobs[q];
     double tbest = 1e300;
     Tiles pick = best;
@@ -681,11 +698,19 @@ extern "C" int tgmm_batch_autotune(tgmm_batch* b, int reps, tgmm_tiles* chosen, 
         case 0: t.ll_gw = kb.v[c][0]; t.ll_u = kb.v[c][1]; break;
         case 1: t.ms_lanes = kb.v[c][0]; break;
         case 2: t.sv_block = kb.v[c][0]; break;
-        default: t.knn_block = kb.v[c][0]; t.knn_chunk = kb.v[c][1]; break;
+        case 3: t.knn_cell = kb.v[c][0]; break;
+        default:
+          t.knn_block = kb.v[c][0];
+          t.knn_chunk = kb.v[c][1];
+          t.knn_lanes = kb.v[c][2];
+          break;
       }
       double ms = 0.0;
       if ((rc = time_shape(b, t, kb.tk, iters, e0, e1, &ms)) != TGMM_OK) break;
-      if (kb.v[c][1])
+      if (kb.v[c][2])
+        snprintf(line, sizeof(line), "%s %dx%dx%d %.2f\n", kb.name, kb.v[c][0], kb.v[c][1],
+                 kb.v[c][2], 1000.0 * ms);
+      else if (kb.v[c][1])
         snprintf(line, sizeof(line), "%s %dx%d %.2f\n", kb.name, kb.v[c][0], kb.v[c][1],
                  1000.0 * ms);
       else
@@ -703,6 +728,10 @@ extern "C" int tgmm_batch_autotune(tgmm_batch* b, int reps, tgmm_tiles* chosen, 
   if (rc != TGMM_OK) return rc;
   b->ko.tiles = best;
   drop_graph(b);
+  if (best.knn_cell != b->grid_cell) {
+    HIPCHK(launch_prep(b->ws.P, b->ko, b->st));
+    b->grid_cell = best.knn_cell;
+  }
   if (int r2 = tgmm_batch_reset(b)) return r2;
   if (chosen) *chosen = to_public(best);
   if (report && report_cap > 0) {

@@ -51,6 +51,7 @@ struct tgmm_batch {
   Workspace ws;
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
+  int grid_cell = 0;  // knn_cell the kNN grids were last built with
 };
 
 void tgmm_set_error(const char* fmt, ...);

@@ -592,15 +592,20 @@ extern "C" int tgmm_server_fetch(tgmm_server* s, long long ticket, tgmm_frame_de
 
 extern "C" int tgmm_server_set_tiles(tgmm_server* s, const tgmm_tiles* t) {
   if (!s || !t) return TGMM_ERR_ARG;
-  const Tiles x{t->ll_group, t->ll_unroll, t->mstep_lanes, t->sv_block, t->knn_block,
-                t->knn_chunk};
+  const Tiles x{t->ll_group,  t->ll_unroll, t->mstep_lanes, t->sv_block,
+                t->knn_block, t->knn_chunk, t->knn_cell,    t->knn_lanes};
   if (!tiles_valid(x)) {
     tgmm_set_error("tgmm_server_set_tiles: invalid tile shape");
     return TGMM_ERR_ARG;
   }
   HIPCHK(hipSetDevice(s->device));
   HIPCHK(hipStreamSynchronize(s->st));
+  const bool regrid = x.knn_cell != s->ko.tiles.knn_cell;
   s->ko.tiles = x;
+  if (regrid && s->in_flight > 0) {  // rebuild the in-flight frames' kNN grids
+    HIPCHK(launch_prep(s->ws.P, s->ko, s->st));
+    HIPCHK(hipStreamSynchronize(s->st));
+  }
   if (s->gexec) (void)hipGraphExecDestroy(s->gexec);
   if (s->graph) (void)hipGraphDestroy(s->graph);
   s->gexec = nullptr;

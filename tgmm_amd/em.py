@@ -102,7 +102,8 @@ class Tiles(ctypes.Structure):
     the same for every valid choice."""
     _fields_ = [("ll_group", ctypes.c_int), ("ll_unroll", ctypes.c_int),
                 ("mstep_lanes", ctypes.c_int), ("sv_block", ctypes.c_int),
-                ("knn_block", ctypes.c_int), ("knn_chunk", ctypes.c_int)]
+                ("knn_block", ctypes.c_int), ("knn_chunk", ctypes.c_int),
+                ("knn_cell", ctypes.c_int), ("knn_lanes", ctypes.c_int)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -116,7 +117,7 @@ class Tiles(ctypes.Structure):
 
 
 DEFAULT_TILES = dict(ll_group=16, ll_unroll=4, mstep_lanes=16, sv_block=256, knn_block=256,
-                     knn_chunk=8)
+                     knn_chunk=8, knn_cell=4, knn_lanes=1)
 
 
 
