@@ -185,9 +185,8 @@ int tgmm_batch_fetch(tgmm_batch* b, tgmm_frame_desc* frames, int nframes);
  * same results (the kernels fix their summation order independently of the
  * shape); only the speed changes. Defaults: {16, 4, 16, 256, 256, 8, 4, 1}. */
 typedef struct tgmm_tiles {
-  int ll_group;    /* lanes per supervoxel, log-likelihood: 4, 8 or 16 */
-  int ll_unroll;   /* voxels per lane per pass: 4, 8 or 16, with
-                      ll_group * ll_unroll = 32 or 64 */
+  int ll_group;    /* lanes per supervoxel, log-likelihood: 8 or 16 */
+  int ll_unroll;   /* voxels per lane per pass: 4, or 8 with ll_group 8 */
   int mstep_lanes; /* lanes per Gaussian, M-step statistics: 4, 8 or 16 */
   int sv_block;    /* threads per block, E-step and pair scatter: 64, 128, 256 */
   int knn_block;   /* threads per block, kNN search: 64, 128 or 256 */

@@ -22,7 +22,9 @@ namespace tgmm {
 constexpr int kAlign = 256;      // supervoxel / Gaussian range alignment
 constexpr int kGB = 64;          // threads per block, Gaussian-lane kernels
 constexpr int kSB = 256;         // threads per block, supervoxel-lane kernels
-constexpr int kLLsv = 64;        // supervoxels per log-likelihood block
+constexpr int kLLsv = 32;        // supervoxels per log-likelihood block
+constexpr int kLLgrp = 8;        // log-likelihood blocks per partial group (256 supervoxels)
+constexpr int kLLcap = 2048;     // voxels of a log-likelihood block staged in LDS
 constexpr int kScanTile = 4096;  // elements per scan block
 constexpr int kPairShift = 4;    // pair code = (s << 4) | j, j < 16
 
@@ -75,8 +77,8 @@ struct alignas(32) EPack {
 // results: the kernels fix their floating-point summation order
 // independently of the shape.
 struct Tiles {
-  int ll_gw;      // lanes per supervoxel, log-likelihood: 4, 8, 16
-  int ll_u;       // voxels per lane per pass, log-likelihood (ll_gw*ll_u = 32 or 64)
+  int ll_gw;      // lanes per supervoxel, log-likelihood: 8, 16
+  int ll_u;       // voxels per lane per pass, log-likelihood: 4 (ll_gw 8, 16), 8 (ll_gw 8)
   int ms_lanes;   // lanes per Gaussian, M-step statistics: 4, 8, 16
   int sv_block;   // threads per block, E-step and pair scatter: 64, 128, 256
   int knn_block;  // threads per block, kNN search: 64, 128, 256
@@ -135,7 +137,9 @@ struct DevPtrs {
   float4* cg4;      // Gcap: cell-sorted references (x, y, z, index bits)
   // reductions
   double* part_g;   // Gcap/kGB
-  double* part_s;   // Scap/kLLsv
+  double* part_s;   // Scap/kLLsv: log-likelihood block partials
+  double* part_sg;  // Scap/256: partials of groups of kLLgrp blocks
+  unsigned* grp_cnt;  // Scap/256: arrivals per group (zero between launches)
   unsigned* done;   // 4*Fcap
   int* scan_blk;    // scan partials
   FrameDev* fr;

@@ -1050,122 +1050,150 @@ __device__ inline double frame_sum_w64(const double* part, int nblk) {
   return wave_sum((a4[0] + a4[1]) + (a4[2] + a4[3]));
 }
 
-// Log-likelihood over the voxel stream (GMEM.cu:1913-1978). A block covers
-// kLLsv = 64 supervoxels with 16 groups of GW lanes; group g takes supervoxels
-// g, g+16, g+32, g+48 of the block in turn, with the next one's header
-// (frame id, voxel range, anchor, candidate indices) loaded while the current
-// one is evaluated. Each candidate's parameter pack is read by one lane of
-// the group and broadcast to the others.
+// Log-likelihood over the voxel stream (GMEM.cu:1913-1978). A block of 256
+// threads covers kLLsv = 32 supervoxels of one frame.
+//  1. Threads 0..31 read the supervoxel headers (frame id, voxel range,
+//     anchor, candidate indices) into LDS.
+//  2. The block's voxels, one contiguous range of the stream, are copied into
+//     LDS as (x, y, z, img) with every load issued before any is used (ranges
+//     above kLLcap voxels are read from HBM in step 3 instead), and the
+//     candidates' parameter packs are read and turned into f32 constants:
+//     (a_s - m_k) rounded once from FP64, the scaled precision, the log-weight.
+//  3. Groups of GW lanes evaluate one supervoxel at a time from LDS.
 // Per voxel, d = (x - a_s) + (a_s - m_k) with the anchor a_s a voxel of the
-// supervoxel (exact in f32) and (a_s - m_k) rounded once from FP64, so the
-// f32 quadratic form keeps full relative precision far from the origin.
-// Terms are evaluated in base 2 (hardware exp2/log2), log-sum-exp with the
-// largest term factored out, img-weighted accumulation in FP64. A voxel whose
-// largest term is below the FP64 underflow of the reference's direct sum
-// (2^-1074) contributes nothing, as in the reference.
+// supervoxel (exact in f32), so the f32 quadratic form keeps full relative
+// precision far from the origin. Terms are evaluated in base 2 (hardware
+// exp2/log2), log-sum-exp with the largest term factored out, img-weighted
+// accumulation in FP64. A voxel whose largest term is below the FP64
+// underflow of the reference's direct sum (2^-1074) contributes nothing, as
+// in the reference.
 //
 // Summation order, identical for every (GW, U): voxel i of a supervoxel is
 // added, in increasing i, to class sum i mod 16; the 16 class sums are added
-// as ((c + c^8) + ...^4) ...^1; a block's 64 supervoxel sums by a fixed
-// pairwise tree; the frame's block partials by frame_sum_w64.
-template <int KM, int R>
-struct SvHead {
-  long long beg;
-  int n;
-  float a0, a1, a2;
-  int kk[R];
-};
-
-template <int KM, int GW, int R>
-__device__ inline SvHead<KM, R> load_head(const DevPtrs& P, long long s, int f, int lane, int K) {
-  const long long S = P.Scap;
-  SvHead<KM, R> h;
-  const int sf = P.sv_frame[s];
-  h.beg = P.sv_beg[s];
-  const int len = P.sv_len[s];
-  h.a0 = P.anchor[s];
-  h.a1 = P.anchor[S + s];
-  h.a2 = P.anchor[2 * S + s];
-#pragma unroll
-  for (int r = 0; r < R; r++) {
-    const int j = lane + GW * r;
-    h.kk[r] = (j < K) ? P.ind[(long long)j * S + s] : -1;
-  }
-  h.n = (sf == f) ? len : 0;
-  return h;
-}
-
+// as ((c + c^8) + ...^4) ...^1; a block's 32 supervoxel sums by a fixed
+// pairwise tree; the partials of 8 consecutive blocks (256 supervoxels) by a
+// fixed tree in the last of them to finish; the frame's group partials by
+// frame_sum_w64 in the last group to finish.
 template <int KM, int GW, int U>
-__global__ __launch_bounds__(16 * GW) void k_loglik(DevPtrs P, KOpts O) {
+__global__ __launch_bounds__(256) void k_loglik(DevPtrs P, KOpts O) {
   constexpr int NC = 16 / GW;  // class sums per lane
   constexpr int PASS = GW * U;
-  constexpr int R = (KM + GW - 1) / GW;  // candidates owned per lane
-  constexpr int SPG = kLLsv / 16;        // supervoxels per group
+  constexpr int NG = 256 / GW;  // groups per block
+  constexpr int NP = 12;        // floats per candidate constant set (3 x float4)
   static_assert(PASS % 16 == 0, "a pass must cover whole classes");
+  static_assert(NG <= kLLsv || NG % kLLsv == 0, "groups per block");
+  __shared__ long long h_beg[kLLsv];
+  __shared__ int h_n[kLLsv];
+  __shared__ float h_a[kLLsv][3];
+  __shared__ int h_k[kLLsv][KM];
+  __shared__ float4 cp[kLLsv][KM][NP / 4];
+  __shared__ float4 vox[kLLcap];
   __shared__ double svsum[kLLsv];
-  __shared__ int s_last;
+  __shared__ long long s_vbase;
+  __shared__ int s_nv, s_last;
   const long long sb = (long long)blockIdx.x * kLLsv;
-  const int lane = threadIdx.x % GW, grp = threadIdx.x / GW;
+  const int t = threadIdx.x;
   const int K = O.K;
+  const long long S = P.Scap;
   const int f = P.sblk_frame[sb >> 8];
   if (f < 0) return;
   FrameDev* F = P.fr + f;
   if (!F->active) return;
+  // 1. headers
+  if (t < kLLsv) {
+    const long long s = sb + t;
+    const int sf = P.sv_frame[s];
+    const long long beg = P.sv_beg[s];
+    const int len = P.sv_len[s];
+    h_a[t][0] = P.anchor[s];
+    h_a[t][1] = P.anchor[S + s];
+    h_a[t][2] = P.anchor[2 * S + s];
+#pragma unroll
+    for (int j = 0; j < KM; j++) h_k[t][j] = (j < K) ? P.ind[(long long)j * S + s] : -1;
+    h_beg[t] = beg;
+    h_n[t] = (sf == f) ? len : 0;
+  }
+  __syncthreads();
+  if (t == 0) {
+    // the frame's supervoxels are consecutive in the voxel stream; padding
+    // entries (n = 0) come after the last one
+    int last = -1;
+    for (int j = 0; j < kLLsv; j++)
+      if (h_n[j] > 0) last = j;
+    const long long vb = last >= 0 ? h_beg[0] : 0;
+    s_vbase = vb;
+    s_nv = last >= 0 ? (int)min(h_beg[last] + h_n[last] - vb, (long long)kLLcap + 1) : 0;
+  }
+  __syncthreads();
+  // 2. voxels into LDS, candidate constants: all of the thread's loads are
+  // issued before the first LDS store
+  const long long vbase = s_vbase;
+  const int nv = s_nv;
+  const bool staged = nv <= kLLcap;
+  constexpr int VR = kLLcap / 256;
+  float4 tv[VR];
+#pragma unroll
+  for (int r = 0; r < VR; r++) {
+    const int i = t + 256 * r;
+    tv[r] = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (staged && i < nv)
+      tv[r] = make_float4(P.vx[vbase + i], P.vy[vbase + i], P.vz[vbase + i], P.vimg[vbase + i]);
+  }
   const double lsa2 = F->log_sum_alpha * 1.4426950408889634074;
-  SvHead<KM, R> h = load_head<KM, GW, R>(P, sb + grp, f, lane, K);
-  for (int it = 0; it < SPG; it++) {
-    const SvHead<KM, R> c = h;
-    if (it + 1 < SPG) h = load_head<KM, GW, R>(P, sb + grp + 16 * (it + 1), f, lane, K);
-    const int n = c.n;
-    float vx[U], vy[U], vz[U], vw[U];
+  for (int e = t; e < kLLsv * KM; e += 256) {
+    const int sv = e / KM, j = e % KM;
+    const int k = h_k[sv][j];
+    float c[NP];
 #pragma unroll
-    for (int u = 0; u < U; u++) {
-      const int i = lane + GW * u;
-      vx[u] = vy[u] = vz[u] = vw[u] = 0.0f;
-      if (i < n) {
-        vx[u] = P.vx[c.beg + i];
-        vy[u] = P.vy[c.beg + i];
-        vz[u] = P.vz[c.beg + i];
-        vw[u] = P.vimg[c.beg + i];
-      }
+    for (int i = 0; i < NP; i++) c[i] = 0.0f;
+    c[9] = -INFINITY;
+    if (h_n[sv] > 0 && k >= 0) {
+      const LLPack lp = P.llpack[k];
+      c[0] = (float)((double)h_a[sv][0] - lp.m[0]);
+      c[1] = (float)((double)h_a[sv][1] - lp.m[1]);
+      c[2] = (float)((double)h_a[sv][2] - lp.m[2]);
+#pragma unroll
+      for (int i = 0; i < 6; i++) c[3 + i] = lp.M[i];
+      c[9] = (float)(lp.lg2 - lsa2);
     }
-    float pd0[R], pd1[R], pd2[R], pw[R][6], pg[R];
+    cp[sv][j][0] = make_float4(c[0], c[1], c[2], c[3]);
+    cp[sv][j][1] = make_float4(c[4], c[5], c[6], c[7]);
+    cp[sv][j][2] = make_float4(c[8], c[9], c[10], c[11]);
+  }
 #pragma unroll
-    for (int r = 0; r < R; r++) {
-      pd0[r] = pd1[r] = pd2[r] = 0.0f;
-#pragma unroll
-      for (int i = 0; i < 6; i++) pw[r][i] = 0.0f;
-      pg[r] = -INFINITY;
-      if (n > 0 && c.kk[r] >= 0) {
-        const LLPack lp = P.llpack[c.kk[r]];
-        pd0[r] = (float)((double)c.a0 - lp.m[0]);
-        pd1[r] = (float)((double)c.a1 - lp.m[1]);
-        pd2[r] = (float)((double)c.a2 - lp.m[2]);
-#pragma unroll
-        for (int i = 0; i < 6; i++) pw[r][i] = lp.M[i];
-        pg[r] = (float)(lp.lg2 - lsa2);
-      }
-    }
+  for (int r = 0; r < VR; r++) {
+    const int i = t + 256 * r;
+    if (staged && i < nv) vox[i] = tv[r];
+  }
+  __syncthreads();
+  // 3. one supervoxel per group at a time
+  const int lane = t % GW, grp = t / GW;
+  for (int sv = grp; sv < kLLsv; sv += NG) {
+    const int n = h_n[sv];
+    const float a0 = h_a[sv][0], a1 = h_a[sv][1], a2 = h_a[sv][2];
+    const int off = (int)(h_beg[sv] - vbase);
     float cd0[KM], cd1[KM], cd2[KM], cw[KM][6], cg[KM];
 #pragma unroll
     for (int j = 0; j < KM; j++) {
-      const int r = j / GW, src = j % GW;
-      cd0[j] = __shfl(pd0[r], src, GW);
-      cd1[j] = __shfl(pd1[r], src, GW);
-      cd2[j] = __shfl(pd2[r], src, GW);
-#pragma unroll
-      for (int i = 0; i < 6; i++) cw[j][i] = __shfl(pw[r][i], src, GW);
-      cg[j] = (j < K) ? __shfl(pg[r], src, GW) : -INFINITY;
+      const float4 p0 = cp[sv][j][0], p1 = cp[sv][j][1], p2 = cp[sv][j][2];
+      cd0[j] = p0.x;
+      cd1[j] = p0.y;
+      cd2[j] = p0.z;
+      cw[j][0] = p0.w;
+      cw[j][1] = p1.x;
+      cw[j][2] = p1.y;
+      cw[j][3] = p1.z;
+      cw[j][4] = p1.w;
+      cw[j][5] = p2.x;
+      cg[j] = (j < K) ? p2.y : -INFINITY;
     }
     double cls[NC];
 #pragma unroll
     for (int q = 0; q < NC; q++) cls[q] = 0.0;
     // one voxel's term log2(sum_j 2^t_j), added to its class sum
-    auto term = [&](float x0, float x1, float x2, float w, double& acc) {
+    auto term = [&](const float4 v, double& acc) {
 #pragma clang fp contract(off)
-      x0 -= c.a0;
-      x1 -= c.a1;
-      x2 -= c.a2;
+      const float x0 = v.x - a0, x1 = v.y - a1, x2 = v.z - a2;
       float tj[KM];
       float mx = -INFINITY;
 #pragma unroll
@@ -1185,20 +1213,27 @@ __global__ __launch_bounds__(16 * GW) void k_loglik(DevPtrs P, KOpts O) {
         float sum = 0.0f;
 #pragma unroll
         for (int j = 0; j < KM; j++) sum += __builtin_amdgcn_exp2f(tj[j] - mx);
-        acc += (double)w * (double)(mx + __builtin_amdgcn_logf(sum));
+        acc += (double)v.w * (double)(mx + __builtin_amdgcn_logf(sum));
       }
     };
-#pragma unroll
-    for (int u = 0; u < U; u++)
-      if (lane + GW * u < n) term(vx[u], vy[u], vz[u], vw[u], cls[u % NC]);
-    for (int base = PASS; base < n; base += PASS) {
+    for (int base = 0; base < n; base += PASS) {
+      float4 v[U];
 #pragma unroll
       for (int u = 0; u < U; u++) {
         const int i = base + lane + GW * u;
-        if (i < n)
-          term(P.vx[c.beg + i], P.vy[c.beg + i], P.vz[c.beg + i], P.vimg[c.beg + i],
-               cls[u % NC]);
+        v[u] = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (i < n) {
+          if (staged) {
+            v[u] = vox[off + i];
+          } else {
+            const long long g = h_beg[sv] + i;
+            v[u] = make_float4(P.vx[g], P.vy[g], P.vz[g], P.vimg[g]);
+          }
+        }
       }
+#pragma unroll
+      for (int u = 0; u < U; u++)
+        if (base + lane + GW * u < n) term(v[u], cls[u % NC]);
     }
     // class sums: in-lane stages (strides >= GW), then lane butterflies
 #pragma unroll
@@ -1210,27 +1245,45 @@ __global__ __launch_bounds__(16 * GW) void k_loglik(DevPtrs P, KOpts O) {
     double tot = cls[0];
 #pragma unroll
     for (int m = GW / 2; m >= 1; m >>= 1) tot += __shfl_xor(tot, m, GW);
-    if (lane == 0) svsum[grp + 16 * it] = tot * 0.69314718055994530942;
+    if (lane == 0) svsum[sv] = tot * 0.69314718055994530942;
   }
   __syncthreads();
-  if (threadIdx.x == 0) {
-    double t[kLLsv / 2];
+  // block partial, then the 8-block group, then the frame
+  if (t == 0) {
+    double tr[kLLsv / 2];
 #pragma unroll
-    for (int i = 0; i < kLLsv / 2; i++) t[i] = svsum[2 * i] + svsum[2 * i + 1];
+    for (int i = 0; i < kLLsv / 2; i++) tr[i] = svsum[2 * i] + svsum[2 * i + 1];
 #pragma unroll
     for (int w = kLLsv / 4; w >= 1; w >>= 1)
 #pragma unroll
-      for (int i = 0; i < w; i++) t[i] = t[2 * i] + t[2 * i + 1];
-    P.part_s[blockIdx.x] = t[0];
+      for (int i = 0; i < w; i++) tr[i] = tr[2 * i] + tr[2 * i + 1];
+    P.part_s[blockIdx.x] = tr[0];
     __threadfence();
-    const unsigned prev = atomicAdd(P.done + 4 * f + 3, 1u);
-    s_last = (prev == (unsigned)(F->nbl - 1));
+    const long long g = blockIdx.x / kLLgrp;
+    const unsigned prev = atomicAdd(P.grp_cnt + g, 1u);
+    int last = 0;
+    if (prev == (unsigned)(kLLgrp - 1)) {
+      __threadfence();
+      P.grp_cnt[g] = 0u;
+      double q8[kLLgrp];
+#pragma unroll
+      for (int i = 0; i < kLLgrp; i++) q8[i] = load_coherent(P.part_s + g * kLLgrp + i);
+#pragma unroll
+      for (int w = kLLgrp / 2; w >= 1; w >>= 1)
+#pragma unroll
+        for (int i = 0; i < w; i++) q8[i] = q8[2 * i] + q8[2 * i + 1];
+      P.part_sg[g] = q8[0];
+      __threadfence();
+      const unsigned pf = atomicAdd(P.done + 4 * f + 3, 1u);
+      last = (pf == (unsigned)(F->nbl / kLLgrp - 1));
+    }
+    s_last = last;
   }
   __syncthreads();
-  if (!s_last || threadIdx.x >= 64) return;
+  if (!s_last || t >= 64) return;
   __threadfence();
-  const double ll = frame_sum_w64(P.part_s + F->s0 / kLLsv, F->nbl);
-  if (threadIdx.x == 0) {
+  const double ll = frame_sum_w64(P.part_sg + F->s0 / (kLLsv * kLLgrp), F->nbl / kLLgrp);
+  if (t == 0) {
     P.done[4 * f + 3] = 0u;
     F->ll3 = F->ll2;
     F->ll2 = F->ll1;
@@ -1306,17 +1359,12 @@ hipError_t launch_next_priors(const DevPtrs& P, long long g0, long long ng, doub
 template <int KM>
 static void launch_loglik(const DevPtrs& P, const KOpts& O, hipStream_t st) {
   const unsigned g = nb(P.Scap, kLLsv);
-  const int gw = O.tiles.ll_gw, u = O.tiles.ll_u;
-  if (gw == 16)
+  if (O.tiles.ll_gw == 16)
     k_loglik<KM, 16, 4><<<g, 256, 0, st>>>(P, O);
-  else if (gw == 8 && u == 4)
-    k_loglik<KM, 8, 4><<<g, 128, 0, st>>>(P, O);
-  else if (gw == 8)
-    k_loglik<KM, 8, 8><<<g, 128, 0, st>>>(P, O);
-  else if (u == 8)
-    k_loglik<KM, 4, 8><<<g, 64, 0, st>>>(P, O);
+  else if (O.tiles.ll_u == 4)
+    k_loglik<KM, 8, 4><<<g, 256, 0, st>>>(P, O);
   else
-    k_loglik<KM, 4, 16><<<g, 64, 0, st>>>(P, O);
+    k_loglik<KM, 8, 8><<<g, 256, 0, st>>>(P, O);
 }
 
 static void launch_mstep_stats(const DevPtrs& P, const KOpts& O, hipStream_t st) {

@@ -112,6 +112,8 @@ int ws_alloc(Workspace* w, const WsShape& sh, int K) {
   AL(P.cg4, G);
   AL(P.part_g, G / kGB);
   AL(P.part_s, S / kLLsv);
+  AL(P.part_sg, S / 256);
+  AL(P.grp_cnt, S / 256);
   AL(P.done, 4 * F);
   AL(P.scan_blk, (std::max(C, G) + 1 + kScanTile - 1) / kScanTile + 1);
   AL(P.fr, F);
@@ -180,8 +182,8 @@ KOpts make_kopts(const tgmm_em_opts& o, const float* scale_reg) {
 
 bool tgmm::tiles_valid(const Tiles& t) {
   auto in = [](int v, int a, int b, int c) { return v == a || v == b || v == c; };
-  return in(t.ll_gw, 4, 8, 16) && in(t.ll_u, 4, 8, 16) &&
-         (t.ll_gw * t.ll_u == 32 || t.ll_gw * t.ll_u == 64) && in(t.ms_lanes, 4, 8, 16) &&
+  return ((t.ll_gw == 16 && t.ll_u == 4) || (t.ll_gw == 8 && (t.ll_u == 4 || t.ll_u == 8))) &&
+         in(t.ms_lanes, 4, 8, 16) &&
          in(t.sv_block, 64, 128, 256) && in(t.knn_block, 64, 128, 256) &&
          (t.knn_chunk == 4 || t.knn_chunk == 8) && in(t.knn_cell, 2, 4, 6) &&
          in(t.knn_lanes, 1, 2, 4);
@@ -350,6 +352,7 @@ int batch_upload(tgmm_batch* b, const std::vector<FrameSrc>& fs) {
   HIPCHK(hipMemsetAsync(P.seg_cnt, 0, (G + 1) * 4, st));
   HIPCHK(hipMemsetAsync(P.cell_cnt, 0, (sh.Ccap + 1) * 4, st));
   HIPCHK(hipMemsetAsync(P.done, 0, 4 * sh.Fcap * 4, st));
+  HIPCHK(hipMemsetAsync(P.grp_cnt, 0, (sh.Scap / 256) * 4, st));
   HIPCHK(launch_prep(P, b->ko, st));
   b->grid_cell = b->ko.tiles.knn_cell;
   HIPCHK(launch_reset(P, st));
@@ -678,7 +681,7 @@ extern "C" int tgmm_batch_autotune(tgmm_batch* b, int reps, tgmm_tiles* chosen, 
     int v[18][3];
   };
   const Knob knobs[5] = {
-      {"loglik", TK_LOGLIK, 5, {{16, 4}, {8, 4}, {8, 8}, {4, 8}, {4, 16}}},
+      {"loglik", TK_LOGLIK, 3, {{16, 4}, {8, 4}, {8, 8}}},
       {"mstep", TK_MSTEP, 3, {{4, 0}, {8, 0}, {16, 0}}},
       {"estep", TK_ESTEP, 3, {{64, 0}, {128, 0}, {256, 0}}},
       {"knn_cell", TK_KNN, 3, {{2, 0}, {4, 0}, {6, 0}}},
