@@ -25,7 +25,6 @@ constexpr int kSB = 256;         // threads per block, supervoxel-lane kernels
 constexpr int kLLsv = 32;        // supervoxels per log-likelihood block
 constexpr int kLLgrp = 8;        // log-likelihood blocks per partial group (256 supervoxels)
 constexpr int kLLcap = 2048;     // voxels of a log-likelihood block staged in LDS
-constexpr int kScanTile = 4096;  // elements per scan block
 constexpr int kPairShift = 4;    // pair code = (s << 4) | j, j < 16
 
 // state field indices (record order, GMEMupdateCUDA.h:22-34)
@@ -141,7 +140,6 @@ struct DevPtrs {
   double* part_sg;  // Scap/256: partials of groups of kLLgrp blocks
   unsigned* grp_cnt;  // Scap/256: arrivals per group (zero between launches)
   unsigned* done;   // 4*Fcap
-  int* scan_blk;    // scan partials
   FrameDev* fr;
   long long Scap, Gcap, Ccap, Vcap;
   int Fcap;

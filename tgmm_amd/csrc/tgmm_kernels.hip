@@ -271,7 +271,7 @@ __global__ __launch_bounds__(1024) void k_grid_setup(DevPtrs P, KOpts O) {
       while ((double)orgf[a] + (double)d * (double)hf <= top) d++;
       dims[a] = d;
     }
-    if (dims[0] * dims[1] * dims[2] <= F->ncap) break;
+    if (dims[0] * dims[1] * dims[2] < F->ncap) break;
     h *= 1.25;
   }
   for (int a = 0; a < 3; a++) {
@@ -279,7 +279,7 @@ __global__ __launch_bounds__(1024) void k_grid_setup(DevPtrs P, KOpts O) {
     F->dims[a] = (int)dims[a];
   }
   F->h = hf;
-  if (dims[0] * dims[1] * dims[2] > F->ncap) {  // cannot happen for finite input
+  if (dims[0] * dims[1] * dims[2] >= F->ncap) {  // cannot happen for finite input
     F->dims[0] = F->dims[1] = F->dims[2] = 1;
     F->h = 3.0e38f;
   }
@@ -379,16 +379,45 @@ __device__ inline int block_exclusive_scan1024(int v, int* lds) {
   return lds[16 + w] + x - v;
 }
 
-// single-block exclusive scan for small arrays: tiles of 16384, each thread
-// 16 consecutive elements loaded as four int4
-__global__ __launch_bounds__(1024) void k_scan_single(const int* in, long long n, int* out) {
+// Exclusive scan of each frame's counts, one 1024-thread block per frame:
+// out[off + i] = base + sum(in[off .. off + i)) for i = 0..n, with
+//   which 0: the frame's kNN grid cells (off c0, n cells, base g0: the
+//            references of a frame's grid go to its Gaussian range),
+//   which 1: the frame's Gaussians (off g0, n ng, base K*s0: a frame has at
+//            most K*ns significant pairs).
+// Both ranges hold n+1 entries (the grid has fewer cells than its range, the
+// Gaussian range at least one padding entry), so frames never share an entry.
+// Tiles of 16384 entries, 16 consecutive per thread as four int4 loads (the
+// ranges start on multiples of 4).
+__global__ __launch_bounds__(1024) void k_scan_frames(DevPtrs P, int which, int need_active,
+                                                      int K) {
   __shared__ int lds[32];
   __shared__ int carry;
-  if (threadIdx.x == 0) carry = 0;
+  const int f = (P.only_frame >= 0) ? P.only_frame : (int)blockIdx.x;
+  if (f >= P.Fcap || (P.only_frame < 0 && f >= P.nframes)) return;
+  const FrameDev* F = P.fr + f;
+  if (F->ncap == 0 || (need_active && !F->active)) return;  // empty slot / finished
+  const int* in;
+  int* out;
+  long long n;
+  int base;
+  if (which == 0) {
+    in = P.cell_cnt + F->c0;
+    out = P.cell_start + F->c0;
+    n = (long long)F->dims[0] * F->dims[1] * F->dims[2];
+    base = (int)F->g0;
+  } else {
+    in = P.seg_cnt + F->g0;
+    out = P.seg_start + F->g0;
+    n = F->ng;
+    base = (int)(K * F->s0);
+  }
+  if (threadIdx.x == 0) carry = base;
   __syncthreads();
-  for (long long base = 0; base < n; base += 16384) {
+  const long long n1 = n + 1;  // entry n carries the total
+  for (long long b0 = 0; b0 < n1; b0 += 16384) {
     const int c = carry;
-    const long long b = base + (long long)threadIdx.x * 16;
+    const long long b = b0 + (long long)threadIdx.x * 16;
     int v[16];
     if (b + 16 <= n) {
       const int4* p4 = reinterpret_cast<const int4*>(in + b);
@@ -411,7 +440,7 @@ __global__ __launch_bounds__(1024) void k_scan_single(const int* in, long long n
     const int incl_last = run + tsum;
 #pragma unroll
     for (int i = 0; i < 16; i++) {
-      if (b + i < n) out[b + i] = run;
+      if (b + i < n1) out[b + i] = run;
       run += v[i];
     }
     __syncthreads();
@@ -420,60 +449,8 @@ __global__ __launch_bounds__(1024) void k_scan_single(const int* in, long long n
   }
 }
 
-__global__ __launch_bounds__(256) void k_scan_partial(const int* in, long long n, int* blk) {
-  const long long base = (long long)blockIdx.x * kScanTile;
-  int acc = 0;
-  for (int i = threadIdx.x; i < kScanTile; i += 256) {
-    const long long j = base + i;
-    if (j < n) acc += in[j];
-  }
-  __shared__ int red[256];
-  red[threadIdx.x] = acc;
-  __syncthreads();
-  for (int st = 128; st > 0; st >>= 1) {
-    if ((int)threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) blk[blockIdx.x] = red[0];
-}
 
 // exclusive scan; each thread owns 16 consecutive elements of the tile
-__global__ __launch_bounds__(256) void k_scan_final(const int* in, long long n, const int* blk,
-                                                    int* out) {
-  __shared__ int red[256];
-  int pre = 0;
-  for (int i = threadIdx.x; i < (int)blockIdx.x; i += 256) pre += blk[i];
-  red[threadIdx.x] = pre;
-  __syncthreads();
-  for (int st = 128; st > 0; st >>= 1) {
-    if ((int)threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
-    __syncthreads();
-  }
-  const int block_pre = red[0];
-  __syncthreads();
-  const long long base = (long long)blockIdx.x * kScanTile + threadIdx.x * 16;
-  int v[16];
-  int tsum = 0;
-#pragma unroll
-  for (int i = 0; i < 16; i++) {
-    v[i] = (base + i < n) ? in[base + i] : 0;
-    tsum += v[i];
-  }
-  red[threadIdx.x] = tsum;
-  __syncthreads();
-  for (int st = 1; st < 256; st <<= 1) {
-    const int t = ((int)threadIdx.x >= st) ? red[threadIdx.x - st] : 0;
-    __syncthreads();
-    red[threadIdx.x] += t;
-    __syncthreads();
-  }
-  int run = block_pre + red[threadIdx.x] - tsum;
-#pragma unroll
-  for (int i = 0; i < 16; i++) {
-    if (base + i < n) out[base + i] = run;
-    run += v[i];
-  }
-}
 
 // ------------------------------------------------------- iteration kernels --
 // E-step constants and kNN grid binning (GMEM.cu:553-628, 527-539).
@@ -1299,22 +1276,12 @@ int kmax_for(int K) { return K <= 5 ? 5 : (K <= 8 ? 8 : 16); }
 
 static inline unsigned nb(long long n, int per) { return (unsigned)((n + per - 1) / per); }
 
-static hipError_t scan(const int* in, long long n1, int* out, int* blk, hipStream_t st) {
-  if (n1 <= 65536) {
-    k_scan_single<<<1, 1024, 0, st>>>(in, n1, out);
-  } else {
-    const unsigned b = nb(n1, kScanTile);
-    k_scan_partial<<<b, 256, 0, st>>>(in, n1, blk);
-    k_scan_final<<<b, 256, 0, st>>>(in, n1, blk, out);
-  }
-  return hipGetLastError();
-}
 
 hipError_t launch_prep(const DevPtrs& P, const KOpts& O, hipStream_t st) {
   k_moments<<<nb(P.Scap, 4), 256, 0, st>>>(P);
   k_grid_setup<<<P.only_frame >= 0 ? 1 : P.nframes, 1024, 0, st>>>(P, O);
   k_qcell_count<<<nb(P.Scap, 256), 256, 0, st>>>(P, O);
-  (void)scan(P.cell_cnt, P.Ccap + 1, P.cell_start, P.scan_blk, st);
+  k_scan_frames<<<P.only_frame >= 0 ? 1 : P.nframes, 1024, 0, st>>>(P, 0, 0, O.K);
   k_qcell_scatter<<<nb(P.Scap, 256), 256, 0, st>>>(P);
   return hipGetLastError();
 }
@@ -1398,7 +1365,7 @@ static hipError_t iteration_t(const DevPtrs& P, const KOpts& O, hipStream_t st, 
   const unsigned gb = nb(P.Gcap, kGB);
   const int bs = O.tiles.sv_block;
   k_gauss_prep<<<gb, kGB, 0, st>>>(P, O);
-  (void)scan(P.cell_cnt, P.Ccap + 1, P.cell_start, P.scan_blk, st);
+  k_scan_frames<<<P.nframes, 1024, 0, st>>>(P, 0, 1, O.K);
   k_grid_scatter<<<gb, kGB, 0, st>>>(P);
   if (tk == TK_KNN && e0) (void)hipEventRecord(e0, st);
   launch_knn<KM>(P, O, st);
@@ -1406,7 +1373,7 @@ static hipError_t iteration_t(const DevPtrs& P, const KOpts& O, hipStream_t st, 
   if (tk == TK_ESTEP && e0) (void)hipEventRecord(e0, st);
   k_estep<KM><<<nb(P.Scap, bs), bs, 0, st>>>(P, O);
   if (tk == TK_ESTEP && e1) (void)hipEventRecord(e1, st);
-  (void)scan(P.seg_cnt, P.Gcap + 1, P.seg_start, P.scan_blk, st);
+  k_scan_frames<<<P.nframes, 1024, 0, st>>>(P, 1, 1, O.K);
   k_pair_scatter<<<nb(P.Scap, bs), bs, 0, st>>>(P, O);
   k_seg_sort<<<nb(P.Gcap, 16), 256, 0, st>>>(P);
   if (tk == TK_MSTEP && e0) (void)hipEventRecord(e0, st);

@@ -115,7 +115,6 @@ int ws_alloc(Workspace* w, const WsShape& sh, int K) {
   AL(P.part_sg, S / 256);
   AL(P.grp_cnt, S / 256);
   AL(P.done, 4 * F);
-  AL(P.scan_blk, (std::max(C, G) + 1 + kScanTile - 1) / kScanTile + 1);
   AL(P.fr, F);
   AL(w->gs_init, GS_NFIELDS * G);
 #undef AL
@@ -236,16 +235,19 @@ int batch_build(int device, const std::vector<FrameSrc>& fs, const tgmm_em_opts&
     F.g0 = g;
     F.ng = f.G;
     F.c0 = c;
-    F.ncap = std::max<long long>(8, 2LL * f.G);
+    // grid-cell range: more entries than cells (the per-frame scan writes one
+    // past the last cell), a multiple of 4 (vector loads)
+    F.ncap = round_up(std::max<long long>(8, 2LL * f.G), 4);
     // every block mapped to the frame's 256-aligned ranges takes part in the
-    // per-frame reductions, including trailing blocks with no valid element
-    F.nbg = (int)(round_up(std::max<long long>(f.G, 1), kAlign) / kGB);
+    // per-frame reductions, including trailing blocks with no valid element;
+    // the Gaussian range has at least one padding entry (segment end)
+    F.nbg = (int)(round_up(f.G + 1, kAlign) / kGB);
     F.nbl = (int)(round_up(std::max<long long>(f.S, 1), kAlign) / kLLsv);
     F.max_iter = o.max_iter;
     F.tol = o.tol;
     v += f.N;
     s += round_up(std::max<long long>(f.S, 1), kAlign);
-    g += round_up(std::max<long long>(f.G, 1), kAlign);
+    g += round_up(f.G + 1, kAlign);
     c += F.ncap;
   }
   sh.Vcap = std::max<long long>(v, 1);
@@ -317,8 +319,7 @@ int batch_upload(tgmm_batch* b, const std::vector<FrameSrc>& fs) {
     }
     for (long long j = 0; j < round_up(std::max<long long>(f.S, 1), kAlign) / kAlign; j++)
       sblk[F.s0 / kAlign + j] = (int)i;
-    for (long long j = 0; j < round_up(std::max<long long>(f.G, 1), kAlign) / kGB; j++)
-      gblk[F.g0 / kGB + j] = (int)i;
+    for (long long j = 0; j < F.nbg; j++) gblk[F.g0 / kGB + j] = (int)i;
     for (long long k = 0; k < f.G; k++) g_frame[F.g0 + k] = (int)i;
     if (f.G > 0) records_to_soa(f.gm, f.G, F.g0, G, gs.data(), fixed.data());
   }

@@ -274,7 +274,7 @@ extern "C" int tgmm_server_submit(tgmm_server* s, int stream_id, const tgmm_fram
   if (stream_id >= 0) s->streams[stream_id].G_next = t.G;
   const WsShape& sh = s->ws.sh;
   if (t.N > sh.Vcap || rup(std::max<long long>(t.S, 1), kAlign) > sh.Scap ||
-      rup(std::max<long long>(t.G, 1), kAlign) > sh.Gcap) {
+      rup(t.G + 1, kAlign) > sh.Gcap) {
     tgmm_set_error("frame exceeds the server's capacity (N=%lld S=%lld G=%lld)", t.N, t.S, t.G);
     return TGMM_ERR_LIMIT;
   }
@@ -401,8 +401,8 @@ static int try_admit(tgmm_server* s) {
       }
     if (f < 0) break;
     t.sa = rup(std::max<long long>(t.S, 1), kAlign);
-    t.ga = rup(std::max<long long>(t.G, 1), kAlign);
-    t.ncap = std::max<long long>(8, 2 * t.G);
+    t.ga = rup(t.G + 1, kAlign);  // at least one padding entry (segment end)
+    t.ncap = rup(std::max<long long>(8, 2 * t.G), 4);
     const bool reuse_g = t.gm_in.empty();  // priors from the resident range
     long long v0, s0, g0 = 0, c0;
     if (!s->av.alloc(t.N, 1, &v0)) {
@@ -414,7 +414,7 @@ static int try_admit(tgmm_server* s) {
       ++it;
       continue;
     }
-    if (!s->ac.alloc(t.ncap, 1, &c0)) {
+    if (!s->ac.alloc(t.ncap, 4, &c0)) {
       s->av.release(v0, t.N);
       s->as.release(s0, t.sa);
       ++it;
