@@ -195,6 +195,8 @@ typedef struct tgmm_tiles {
   int knn_lanes;   /* lanes per query, kNN search: 1, 2 or 4 */
 } tgmm_tiles;
 
+/* A change of knn_cell rebuilds the kNN grids and resets the batch to its
+ * initial state (as tgmm_batch_reset). */
 int tgmm_batch_set_tiles(tgmm_batch* b, const tgmm_tiles* tiles);
 int tgmm_batch_get_tiles(tgmm_batch* b, tgmm_tiles* tiles);
 /* Autotuner. For each knob in turn (log-likelihood shape, M-step lanes,
@@ -255,7 +257,8 @@ int tgmm_server_query(tgmm_server* s, long long ticket, int* state);
 int tgmm_server_fetch(tgmm_server* s, long long ticket, tgmm_frame_desc* frame,
                       double* t_submit, double* t_admit, double* t_done);
 /* Launch shapes for the server's iterations (for example those the
- * autotuner chose on a representative batch). */
+ * autotuner chose on a representative batch). knn_cell can change only while
+ * no frame is in flight (TGMM_ERR_BUSY otherwise). */
 int tgmm_server_set_tiles(tgmm_server* s, const tgmm_tiles* tiles);
 int tgmm_server_destroy(tgmm_server* s);
 

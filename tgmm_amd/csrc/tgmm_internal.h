@@ -67,8 +67,8 @@ struct alignas(32) EPack {
   double m[3];
   double W[6];
   double nu;
-  double kappa;  // E[ln pi] + E[ln det]/2 - 1.5/beta
-  double pad;
+  double kappa0;  // E[ln det]/2 - 1.5/beta
+  double dpsi;    // digamma(alpha); E[ln pi] = dpsi - digamma(sum of alpha)
 };
 
 // Launch shapes of the tunable kernels (chosen per workload by the
@@ -119,7 +119,9 @@ struct DevPtrs {
   int* g_frame;
   double* gs;       // 27*Gcap
   unsigned char* fixed;
-  EPack* epack;     // Gcap: E-step parameters (written by gauss_prep)
+  EPack* epack;     // Gcap: E-step parameters of the next iteration
+  double* eldn;     // Gcap: E[ln det] of the next iteration (committed to the state
+                    // when that iteration's M-step runs)
   LLPack* llpack;   // Gcap: log-likelihood parameters (written by death)
   float* mf;        // 3*Gcap f32 means (kNN references)
   int* gcell;
@@ -150,8 +152,9 @@ struct DevPtrs {
 // kernel launchers (tgmm_kernels.hip); all enqueue on `st`
 // once per frame: supervoxel moments, kNN grid shape, queries sorted by cell
 hipError_t launch_prep(const DevPtrs& P, const KOpts& O, hipStream_t st);
-// iteration state and initial sum of alpha per frame
-hipError_t launch_reset(const DevPtrs& P, hipStream_t st);
+// iteration state, initial sum of alpha and first E-step constants per frame;
+// the frames' kNN grid counts must be zero
+hipError_t launch_reset(const DevPtrs& P, const KOpts& O, hipStream_t st);
 // one EM iteration of every active frame; optional events around one kernel
 hipError_t launch_iteration(const DevPtrs& P, const KOpts& O, hipStream_t st,
                             hipEvent_t ev_before = nullptr, hipEvent_t ev_after = nullptr,

@@ -449,49 +449,57 @@ __global__ __launch_bounds__(1024) void k_scan_frames(DevPtrs P, int which, int 
   }
 }
 
-
-// exclusive scan; each thread owns 16 consecutive elements of the tile
-
 // ------------------------------------------------------- iteration kernels --
-// E-step constants and kNN grid binning (GMEM.cu:553-628, 527-539).
-__global__ __launch_bounds__(kGB) void k_gauss_prep(DevPtrs P, KOpts O) {
-  const int f = P.gblk_frame[blockIdx.x];
-  if (f < 0) return;
-  const FrameDev* F = P.fr + f;
-  if (!F->active) return;
-  const long long k = (long long)blockIdx.x * kGB + threadIdx.x;
-  if (P.g_frame[k] != f) return;
+// E-step constants and kNN grid binning of Gaussian k for the next iteration
+// (GMEM.cu:553-628, 527-539), from its state (m, W, nu, beta, alpha). E[ln pi]
+// needs the frame's sum of alpha after the death test, so the pack keeps
+// digamma(alpha) and the E-step subtracts digamma(sum); E[ln det] and
+// E[ln pi] reach the Gaussian's record when that iteration's M-step runs,
+// as the reference computes them at the start of each iteration.
+__device__ inline void gauss_prep_one(const DevPtrs& P, const KOpts& O, const FrameDev* F,
+                                      long long k, bool alive, const double* m,
+                                      const double* W, double nu, double beta, double alpha) {
   const long long G = P.Gcap;
-  if (gsv(P, GS_M_O, k) < -1e31) {
-    P.gs[GS_ELD * G + k] = -1e32;
-    P.gs[GS_ELP * G + k] = -1e32;
+  if (!alive) {
+    P.eldn[k] = -1e32;
     P.gcell[k] = -1;
     return;
   }
-  double W[6];
-  for (int i = 0; i < 6; i++) W[i] = gsv(P, GS_W + i, k);
-  const double nu = gsv(P, GS_NU, k), beta = gsv(P, GS_BETA, k), alpha = gsv(P, GS_ALPHA, k);
   double eld = 3.0 * 0.69314718055994530942 + log(det3(W));
   for (int i = 0; i < 3; i++) eld += digamma(0.5 * (nu + 1.0 - i));
-  const double elp = digamma(alpha) - F->psi_sum_alpha;
-  P.gs[GS_ELD * G + k] = eld;
-  P.gs[GS_ELP * G + k] = elp;
+  P.eldn[k] = eld;
   EPack ep;
-  for (int i = 0; i < 3; i++) ep.m[i] = gsv(P, GS_M + i, k);
+  for (int i = 0; i < 3; i++) ep.m[i] = m[i];
   for (int i = 0; i < 6; i++) ep.W[i] = W[i];
   ep.nu = nu;
-  ep.kappa = elp + 0.5 * eld - 1.5 / beta;
-  ep.pad = 0.0;
+  ep.kappa0 = 0.5 * eld - 1.5 / beta;
+  ep.dpsi = digamma(alpha);
   P.epack[k] = ep;
   int c[3];
   for (int a = 0; a < 3; a++) {
-    const float m = (float)gsv(P, GS_M + a, k);
-    P.mf[a * G + k] = m;
-    c[a] = axis_cell((double)m * (double)O.scale[a], F->org[a], F->h, F->dims[a]);
+    const float mf = (float)m[a];
+    P.mf[a * G + k] = mf;
+    c[a] = axis_cell((double)mf * (double)O.scale[a], F->org[a], F->h, F->dims[a]);
   }
   const long long cell = F->c0 + ((long long)c[2] * F->dims[1] + c[1]) * F->dims[0] + c[0];
   P.gcell[k] = (int)cell;
   atomicAdd(P.cell_cnt + cell, 1);
+}
+
+// The first iteration's E-step constants and grid binning, from the initial
+// state (later iterations get theirs from k_death).
+__global__ __launch_bounds__(kGB) void k_gauss_prep(DevPtrs P, KOpts O) {
+  const int f = P.gblk_frame[blockIdx.x];
+  if (f < 0 || (P.only_frame >= 0 && f != P.only_frame)) return;
+  const FrameDev* F = P.fr + f;
+  if (!F->active) return;
+  const long long k = (long long)blockIdx.x * kGB + threadIdx.x;
+  if (P.g_frame[k] != f) return;
+  double m[3], W[6];
+  for (int i = 0; i < 3; i++) m[i] = gsv(P, GS_M + i, k);
+  for (int i = 0; i < 6; i++) W[i] = gsv(P, GS_W + i, k);
+  gauss_prep_one(P, O, F, k, gsv(P, GS_M_O, k) > -1e31, m, W, gsv(P, GS_NU, k),
+                 gsv(P, GS_BETA, k), gsv(P, GS_ALPHA, k));
 }
 
 __global__ __launch_bounds__(kGB) void k_grid_scatter(DevPtrs P) {
@@ -717,6 +725,7 @@ __global__ __launch_bounds__(256) void k_estep(DevPtrs P, KOpts O) {
   if (P.sv_frame[s] != f) return;
   const long long S = P.Scap;
   const int K = O.K;
+  const double psi_sum = P.fr[f].psi_sum_alpha;
   const double n = P.mom[s];
   const double c0 = P.mom[S + s], c1 = P.mom[2 * S + s], c2 = P.mom[3 * S + s];
   double Q[6];
@@ -740,7 +749,8 @@ __global__ __launch_bounds__(256) void k_estep(DevPtrs P, KOpts O) {
                           2.0 * (W[1] * d0 * d1 + W[2] * d0 * d2 + W[4] * d1 * d2);
         const double tq = W[0] * Q[0] + W[3] * Q[3] + W[5] * Q[5] +
                           2.0 * (W[1] * Q[1] + W[2] * Q[2] + W[4] * Q[4]);
-        L[j] = n * ep.kappa - 0.5 * ep.nu * (tq + n * qf);
+        const double kappa = (ep.dpsi - psi_sum) + ep.kappa0;
+        L[j] = n * kappa - 0.5 * ep.nu * (tq + n * qf);
         mx = fmax(mx, L[j]);
       }
     }
@@ -890,6 +900,9 @@ __global__ __launch_bounds__(kGB) void k_mstep_update(DevPtrs P, KOpts O) {
   if (P.g_frame[k] == f) {
     const bool alive = gsv(P, GS_M_O, k) > -1e31;
     const bool fixed = P.fixed[k] != 0;
+    // this iteration's expectations, as used by its E-step
+    P.gs[GS_ELD * G + k] = P.eldn[k];
+    P.gs[GS_ELP * G + k] = alive ? P.epack[k].dpsi - F->psi_sum_alpha : -1e32;
     if (alive && !fixed) {
       const double r0 = gsv(P, GS_M, k), r1 = gsv(P, GS_M + 1, k), r2 = gsv(P, GS_M + 2, k);
       const double N = P.mstat[k];
@@ -954,7 +967,7 @@ __global__ __launch_bounds__(kGB) void k_mstep_update(DevPtrs P, KOpts O) {
 // the per-frame sum of alpha used by the log-likelihood and by the next
 // iteration's E[ln pi]. The log-likelihood parameters are stored in base 2:
 // M = nu W log2(e)/2 and lg2 = (log alpha + log det(nu W)/2 - 1.5 log 2pi) log2(e).
-__global__ __launch_bounds__(kGB) void k_death(DevPtrs P) {
+__global__ __launch_bounds__(kGB) void k_death(DevPtrs P, KOpts O) {
   __shared__ double red[1];
   const int f = P.gblk_frame[blockIdx.x];
   if (f < 0) return;
@@ -989,8 +1002,11 @@ __global__ __launch_bounds__(kGB) void k_death(DevPtrs P) {
     }
     const double LOG2E = 1.4426950408889634074;
     const double nu = gsv(P, GS_NU, k);
+    double m[3];
+    for (int i = 0; i < 3; i++) m[i] = gsv(P, GS_M + i, k);
+    gauss_prep_one(P, O, F, k, alive, m, W, nu, gsv(P, GS_BETA, k), alpha);
     LLPack lp;
-    for (int i = 0; i < 3; i++) lp.m[i] = gsv(P, GS_M + i, k);
+    for (int i = 0; i < 3; i++) lp.m[i] = m[i];
     for (int i = 0; i < 6; i++) lp.M[i] = (float)(nu * W[i] * (0.5 * LOG2E));
     lp.pad[0] = lp.pad[1] = 0.0f;
     const double dt = det3(W) * nu * nu * nu;
@@ -1286,9 +1302,10 @@ hipError_t launch_prep(const DevPtrs& P, const KOpts& O, hipStream_t st) {
   return hipGetLastError();
 }
 
-hipError_t launch_reset(const DevPtrs& P, hipStream_t st) {
+hipError_t launch_reset(const DevPtrs& P, const KOpts& O, hipStream_t st) {
   k_iter_init<<<nb(P.nframes, 64), 64, 0, st>>>(P);
   k_alpha_init<<<nb(P.Gcap, kGB), kGB, 0, st>>>(P);
+  k_gauss_prep<<<nb(P.Gcap, kGB), kGB, 0, st>>>(P, O);
   return hipGetLastError();
 }
 
@@ -1364,7 +1381,6 @@ static hipError_t iteration_t(const DevPtrs& P, const KOpts& O, hipStream_t st, 
                               hipEvent_t e1, int tk) {
   const unsigned gb = nb(P.Gcap, kGB);
   const int bs = O.tiles.sv_block;
-  k_gauss_prep<<<gb, kGB, 0, st>>>(P, O);
   k_scan_frames<<<P.nframes, 1024, 0, st>>>(P, 0, 1, O.K);
   k_grid_scatter<<<gb, kGB, 0, st>>>(P);
   if (tk == TK_KNN && e0) (void)hipEventRecord(e0, st);
@@ -1380,7 +1396,7 @@ static hipError_t iteration_t(const DevPtrs& P, const KOpts& O, hipStream_t st, 
   launch_mstep_stats(P, O, st);
   k_mstep_update<<<gb, kGB, 0, st>>>(P, O);
   if (tk == TK_MSTEP && e1) (void)hipEventRecord(e1, st);
-  k_death<<<gb, kGB, 0, st>>>(P);
+  k_death<<<gb, kGB, 0, st>>>(P, O);
   if (tk == TK_LOGLIK && e0) (void)hipEventRecord(e0, st);
   launch_loglik<KM>(P, O, st);
   if (tk == TK_LOGLIK && e1) (void)hipEventRecord(e1, st);

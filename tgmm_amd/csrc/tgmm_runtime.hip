@@ -98,6 +98,7 @@ int ws_alloc(Workspace* w, const WsShape& sh, int K) {
   AL(P.gs, GS_NFIELDS * G);
   AL(P.fixed, G);
   AL(P.epack, G);
+  AL(P.eldn, G);
   AL(P.llpack, G);
   AL(P.mf, 3 * G);
   AL(P.gcell, G);
@@ -130,7 +131,7 @@ void ws_free(Workspace* w) {
 long long ws_bytes_estimate(const WsShape& sh, int K) {
   const long long S = sh.Scap, G = sh.Gcap, C = sh.Ccap, V = sh.Vcap;
   return 16 * V + S * (16 + 8 + 4 + 12 + 80 + 80 + 12 + 16LL * K + 1) +
-         G * (4 + 8 * GS_NFIELDS * 2 + 1 + 8 + 24 + 8 + 12 + 4 + 8 + 12 + 4 + 1) + 8 * C + 64 * sh.Fcap;
+         G * (4 + 8 * GS_NFIELDS * 2 + 1 + 96 + 8 + 64 + 12 + 4 + 8 + 12 + 4 + 1) + 8 * C + 64 * sh.Fcap;
 }
 
 // ------------------------------------------------------------------ batch --
@@ -356,7 +357,7 @@ int batch_upload(tgmm_batch* b, const std::vector<FrameSrc>& fs) {
   HIPCHK(hipMemsetAsync(P.grp_cnt, 0, (sh.Scap / 256) * 4, st));
   HIPCHK(launch_prep(P, b->ko, st));
   b->grid_cell = b->ko.tiles.knn_cell;
-  HIPCHK(launch_reset(P, st));
+  HIPCHK(launch_reset(P, b->ko, st));
   HIPCHK(hipStreamSynchronize(st));
   // keep the host copies the fetch needs
   b->fixed_host = fixed;
@@ -400,7 +401,9 @@ extern "C" int tgmm_batch_reset(tgmm_batch* b) {
   DevPtrs& P = b->ws.P;
   HIPCHK(hipMemcpyAsync(P.gs, b->ws.gs_init, (size_t)GS_NFIELDS * b->ws.sh.Gcap * 8,
                         hipMemcpyDeviceToDevice, b->st));
-  HIPCHK(launch_reset(P, b->st));
+  // grid counts left by frames that stopped after their last binning
+  HIPCHK(hipMemsetAsync(P.cell_cnt, 0, (b->ws.sh.Ccap + 1) * 4, b->st));
+  HIPCHK(launch_reset(P, b->ko, b->st));
   HIPCHK(hipStreamSynchronize(b->st));
   return TGMM_OK;
 }
@@ -595,6 +598,18 @@ extern "C" int tgmm_batch_time_kernel(tgmm_batch* b, const char* name, int reps,
   return TGMM_OK;
 }
 
+// Rebuild every frame's kNN grid for ko.tiles.knn_cell. The grid counts
+// are cleared first (a frame's last binning may have left counts); the
+// iteration state must be reset afterwards.
+static int regrid(tgmm_batch* b, const KOpts& ko) {
+  DevPtrs& P = b->ws.P;
+  HIPCHK(hipMemsetAsync(P.cell_cnt, 0, (b->ws.sh.Ccap + 1) * 4, b->st));
+  HIPCHK(launch_prep(P, ko, b->st));
+  HIPCHK(hipStreamSynchronize(b->st));
+  b->grid_cell = ko.tiles.knn_cell;
+  return TGMM_OK;
+}
+
 static Tiles from_public(const tgmm_tiles& t) {
   return Tiles{t.ll_group,  t.ll_unroll, t.mstep_lanes, t.sv_block,
                t.knn_block, t.knn_chunk, t.knn_cell,    t.knn_lanes};
@@ -624,9 +639,8 @@ extern "C" int tgmm_batch_set_tiles(tgmm_batch* b, const tgmm_tiles* t) {
   b->ko.tiles = x;
   drop_graph(b);  // recaptured by the next graph solve
   if (x.knn_cell != b->grid_cell) {  // the kNN grids are sized by knn_cell
-    HIPCHK(launch_prep(b->ws.P, b->ko, b->st));
-    HIPCHK(hipStreamSynchronize(b->st));
-    b->grid_cell = x.knn_cell;
+    if (int rc = regrid(b, b->ko)) return rc;
+    return tgmm_batch_reset(b);
   }
   return TGMM_OK;
 }
@@ -643,10 +657,8 @@ static int time_shape(tgmm_batch* b, const Tiles& t, int tk, int iters, hipEvent
                       hipEvent_t e1, double* ms_out) {
   KOpts ko = b->ko;
   ko.tiles = t;
-  if (t.knn_cell != b->grid_cell) {  // rebuild the kNN grids for this density
-    HIPCHK(launch_prep(b->ws.P, ko, b->st));
-    b->grid_cell = t.knn_cell;
-  }
+  if (t.knn_cell != b->grid_cell)  // rebuild the kNN grids for this density
+    if (int rc = regrid(b, ko)) return rc;
   if (int rc = tgmm_batch_reset(b)) return rc;
   double tot = 0.0;
   for (int it = 0; it < iters; it++) {
@@ -732,10 +744,8 @@ extern "C" int tgmm_batch_autotune(tgmm_batch* b, int reps, tgmm_tiles* chosen, 
   if (rc != TGMM_OK) return rc;
   b->ko.tiles = best;
   drop_graph(b);
-  if (best.knn_cell != b->grid_cell) {
-    HIPCHK(launch_prep(b->ws.P, b->ko, b->st));
-    b->grid_cell = best.knn_cell;
-  }
+  if (best.knn_cell != b->grid_cell)
+    if (int r2 = regrid(b, b->ko)) return r2;
   if (int r2 = tgmm_batch_reset(b)) return r2;
   if (chosen) *chosen = to_public(best);
   if (report && report_cap > 0) {

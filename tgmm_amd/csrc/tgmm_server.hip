@@ -365,8 +365,10 @@ static int admit(tgmm_server* s, Ticket& t, int f) {
   if (int rc = map_ranges(s, t, f)) return rc;
   DevPtrs Q = P;
   Q.only_frame = f;
+  // the cell range may hold counts left by an earlier frame's last binning
+  HIPCHK(hipMemsetAsync(P.cell_cnt + t.c0, 0, t.ncap * 4, st));
   HIPCHK(launch_prep(Q, s->ko, st));
-  HIPCHK(launch_reset(Q, st));
+  HIPCHK(launch_reset(Q, s->ko, st));
   t.slot = f;
   t.state = 1;
   t.t_admit = now_s();
@@ -599,14 +601,14 @@ extern "C" int tgmm_server_set_tiles(tgmm_server* s, const tgmm_tiles* t) {
     tgmm_set_error("tgmm_server_set_tiles: invalid tile shape");
     return TGMM_ERR_ARG;
   }
+  if (x.knn_cell != s->ko.tiles.knn_cell && s->in_flight > 0) {
+    // grids are built when a frame is admitted
+    tgmm_set_error("tgmm_server_set_tiles: knn_cell cannot change while frames are in flight");
+    return TGMM_ERR_BUSY;
+  }
   HIPCHK(hipSetDevice(s->device));
   HIPCHK(hipStreamSynchronize(s->st));
-  const bool regrid = x.knn_cell != s->ko.tiles.knn_cell;
   s->ko.tiles = x;
-  if (regrid && s->in_flight > 0) {  // rebuild the in-flight frames' kNN grids
-    HIPCHK(launch_prep(s->ws.P, s->ko, s->st));
-    HIPCHK(hipStreamSynchronize(s->st));
-  }
   if (s->gexec) (void)hipGraphExecDestroy(s->gexec);
   if (s->graph) (void)hipGraphDestroy(s->graph);
   s->gexec = nullptr;
