@@ -310,8 +310,9 @@ def _serve(streams, opts, budget, order, arrivals=None, pct=(0.5, 0.5, 0.8), til
         elif ai < len(events):
             time.sleep(max(0.0, min(0.002, events[ai][0] - now)))
     elapsed = time.monotonic() - t0
+    stats = srv.stats()
     srv.close()
-    return elapsed, recs, keep
+    return elapsed, recs, keep, stats
 
 
 def serve_bench(args):
@@ -339,18 +340,18 @@ def serve_bench(args):
     _serve([fr[:2] for fr in streams], opts, budget, set(), tiles=tiles)
 
     # closed loop: throughput
-    el, recs, keep = _serve(streams, opts, budget, {(0, 0), (0, 1)}, pct=pct, tiles=tiles)
+    el, recs, keep, st1 = _serve(streams, opts, budget, {(0, 0), (0, 1)}, pct=pct, tiles=tiles)
     thr = n_frames / el
-    print("# closed loop: %d frames in %.3fs = %.1f frames/s" % (n_frames, el, thr),
-          file=sys.stderr, flush=True)
+    print("# closed loop: %d frames in %.3fs = %.1f frames/s; host phases %s" % (
+        n_frames, el, thr, json.dumps(st1)), file=sys.stderr, flush=True)
 
     # Poisson arrivals: latency
     qps = args.qps if args.qps > 0 else 0.5 * thr
     rng = np.random.default_rng(0xC0FFEE04)
     ns = len(streams)
     arrivals = [np.cumsum(rng.exponential(ns / qps, len(fr))) for fr in streams]
-    el2, recs2, _ = _serve(streams, opts, budget, set(), arrivals=arrivals, pct=pct,
-                           tiles=tiles)
+    el2, recs2, _, st2 = _serve(streams, opts, budget, set(), arrivals=arrivals, pct=pct,
+                                tiles=tiles)
     lat = [r["done"] - r["arrival"] for r in recs2]
     qwait = [r["admit"] - r["arrival"] for r in recs2]
     lat_ms = {"p50": 1000 * _percentile(lat, 50), "p99": 1000 * _percentile(lat, 99),
@@ -408,7 +409,8 @@ def serve_bench(args):
                                   min(f.meta["cells"] for fr in streams for f in fr),
                                   max(f.meta["cells"] for fr in streams for f in fr)),
                    "voxels": n_vox, "K": K, "iterations": iters, "parallelism": "1 GPU"},
-        "closed_loop": {"frames": n_frames, "seconds": el, "frames_per_s": thr},
+        "closed_loop": {"frames": n_frames, "seconds": el, "frames_per_s": thr,
+                        "host_phases": st1},
         "poisson": {"qps": qps, "seconds": el2, "frames_per_s": n_frames / el2,
                     "latency_ms": lat_ms,
                     "queue_wait_ms": {"p50": 1000 * _percentile(qwait, 50),

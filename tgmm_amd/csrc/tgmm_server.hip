@@ -139,6 +139,9 @@ struct tgmm_server {
   long long next_id = 0;
   int in_flight = 0;
   long long anon_queued = 0;  // queued frames with no stream
+  // cumulative host seconds: submit, admit, iterate (launch + wait), retire
+  double t_phase[4] = {0, 0, 0, 0};
+  long long n_steps = 0, n_iter_frames = 0;
 };
 
 static int server_capture(tgmm_server* s) {
@@ -246,6 +249,7 @@ extern "C" int tgmm_server_submit(tgmm_server* s, int stream_id, const tgmm_fram
     tgmm_set_error("tgmm_server_submit: bad arguments");
     return TGMM_ERR_ARG;
   }
+  const double t_enter = now_s();
   Ticket t;
   t.id = s->next_id++;
   t.stream = stream_id;
@@ -279,6 +283,7 @@ extern "C" int tgmm_server_submit(tgmm_server* s, int stream_id, const tgmm_fram
     return TGMM_ERR_LIMIT;
   }
   t.t_submit = now_s();
+  s->t_phase[0] += t.t_submit - t_enter;
   if (stream_id < 0) s->anon_queued++;
   s->queue.push_back(t.id);
   s->tickets.emplace(t.id, std::move(t));
@@ -521,17 +526,26 @@ extern "C" int tgmm_server_step(tgmm_server* s, int* pending) {
   if (!s) return TGMM_ERR_ARG;
   HIPCHK(hipSetDevice(s->device));
   // retire frames that stopped in the previous iteration
+  const double t0 = now_s();
   for (int f = 0; f < (int)s->slot_ticket.size(); f++)
     if (s->slot_ticket[f] >= 0 && !s->fh[f].active)
       if (int rc = retire(s, f)) return rc;
+  const double t1 = now_s();
   if (int rc = try_admit(s)) return rc;
+  const double t2 = now_s();
   if (s->in_flight > 0) {
     if (int rc = server_capture(s)) return rc;
     HIPCHK(hipGraphLaunch(s->gexec, s->st));
     HIPCHK(hipMemcpyAsync(s->fh.data(), s->ws.P.fr, s->fh.size() * sizeof(FrameDev),
                           hipMemcpyDeviceToHost, s->st));
     HIPCHK(hipStreamSynchronize(s->st));
+    s->n_steps++;
+    s->n_iter_frames += s->in_flight;
   }
+  const double t3 = now_s();
+  s->t_phase[3] += t1 - t0;
+  s->t_phase[1] += t2 - t1;
+  s->t_phase[2] += t3 - t2;
   if (pending) *pending = s->in_flight + (int)s->queue.size();
   return TGMM_OK;
 }
@@ -590,6 +604,19 @@ extern "C" int tgmm_server_fetch(tgmm_server* s, long long ticket, tgmm_frame_de
   if (t_admit) *t_admit = t.t_admit;
   if (t_done) *t_done = t.t_done;
   s->tickets.erase(ticket);
+  return TGMM_OK;
+}
+
+extern "C" int tgmm_server_stats(tgmm_server* s, tgmm_server_stats_t* st) {
+  if (!s || !st) return TGMM_ERR_ARG;
+  st->submit_s = s->t_phase[0];
+  st->admit_s = s->t_phase[1];
+  st->iterate_s = s->t_phase[2];
+  st->retire_s = s->t_phase[3];
+  st->steps = s->n_steps;
+  st->frame_iterations = s->n_iter_frames;
+  st->in_flight = s->in_flight;
+  st->queued = (long long)s->queue.size();
   return TGMM_OK;
 }
 

@@ -97,6 +97,13 @@ class ServerOpts(ctypes.Structure):
     ]
 
 
+class ServerStats(ctypes.Structure):
+    _fields_ = [("submit_s", ctypes.c_double), ("admit_s", ctypes.c_double),
+                ("iterate_s", ctypes.c_double), ("retire_s", ctypes.c_double),
+                ("steps", ctypes.c_longlong), ("frame_iterations", ctypes.c_longlong),
+                ("in_flight", ctypes.c_longlong), ("queued", ctypes.c_longlong)]
+
+
 class Tiles(ctypes.Structure):
     """Launch shapes of the tunable kernels (tgmm_tiles). Results are bitwise
     the same for every valid choice."""
@@ -161,6 +168,7 @@ def load():
     L.tgmm_batch_autotune.argtypes = [_vp, ctypes.c_int, ctypes.POINTER(Tiles),
                                       ctypes.c_char_p, ctypes.c_int]
     L.tgmm_server_set_tiles.argtypes = [_vp, ctypes.POINTER(Tiles)]
+    L.tgmm_server_stats.argtypes = [_vp, ctypes.POINTER(ServerStats)]
     L.tgmm_server_create.argtypes = [ctypes.c_int, ctypes.POINTER(ServerOpts), _pp]
     L.tgmm_server_submit.argtypes = [_vp, ctypes.c_int, ctypes.POINTER(FrameDesc),
                                      ctypes.POINTER(ctypes.c_longlong)]
@@ -474,6 +482,12 @@ class Server:
     def set_tiles(self, tiles):
         t = tiles if isinstance(tiles, Tiles) else Tiles.from_dict(tiles)
         _check(self.L.tgmm_server_set_tiles(self.h, ctypes.byref(t)))
+
+    def stats(self):
+        """Cumulative host seconds per phase and step counts (tgmm_server_stats)."""
+        st = ServerStats()
+        _check(self.L.tgmm_server_stats(self.h, ctypes.byref(st)))
+        return {k: getattr(st, k) for k, _ in ServerStats._fields_}
 
     def step(self):
         p = ctypes.c_int()
