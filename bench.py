@@ -65,6 +65,26 @@ def _frames_for(cfg):
     raise SystemExit("unknown config %r" % cfg)
 
 
+def _copy_calibration(nbytes):
+    """One device-to-device hipMemcpy of nbytes (reads and writes nbytes of
+    HBM): a dispatch of known traffic beside the profiled kernels."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    if hip.hipMalloc(ctypes.byref(a), ctypes.c_size_t(nbytes)) or \
+            hip.hipMalloc(ctypes.byref(b), ctypes.c_size_t(nbytes)):
+        raise RuntimeError("hipMalloc failed")
+    hip.hipMemset(a, 1, ctypes.c_size_t(nbytes))
+    hip.hipDeviceSynchronize()
+    rc = hip.hipMemcpy(b, a, ctypes.c_size_t(nbytes), 3)  # hipMemcpyDeviceToDevice
+    hip.hipDeviceSynchronize()
+    hip.hipFree(a)
+    hip.hipFree(b)
+    if rc:
+        raise RuntimeError("hipMemcpy failed: %d" % rc)
+    return nbytes
+
+
 def _percentile(xs, q):
     xs = sorted(xs)
     if not xs:
@@ -94,6 +114,9 @@ def main():
                          "closed-loop throughput)")
     ap.add_argument("--hbm-budget-gb", type=float, default=24.0,
                     help="config 4: server workspace budget")
+    ap.add_argument("--copy-calibration-mb", type=int, default=0,
+                    help="also run one device-to-device copy of this many MiB (hipMemcpy), "
+                         "the known-size reference for HBM traffic counters")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: the EM of a frame runs on one "
@@ -255,6 +278,8 @@ def main():
     }
     if plain is not None:
         line["plain_per_frame_form"] = plain
+    if args.copy_calibration_mb > 0:
+        line["copy_calibration_bytes"] = _copy_calibration(args.copy_calibration_mb << 20)
     print(json.dumps(line), flush=True)
 
 

@@ -11,15 +11,15 @@ from tgmm_amd import em, synth
 pytestmark = pytest.mark.gpu
 
 SHAPES = [
-    dict(ll_group=16, ll_unroll=4, mstep_lanes=16, sv_block=256, knn_block=256, knn_chunk=4,
+    dict(ll_group=64, ll_unroll=1, mstep_lanes=16, sv_block=256, knn_block=256, knn_chunk=4,
          knn_cell=2, knn_lanes=1),
-    dict(ll_group=8, ll_unroll=4, mstep_lanes=8, sv_block=128, knn_block=64, knn_chunk=8,
+    dict(ll_group=32, ll_unroll=1, mstep_lanes=8, sv_block=128, knn_block=64, knn_chunk=8,
          knn_cell=6, knn_lanes=2),
-    dict(ll_group=8, ll_unroll=8, mstep_lanes=4, sv_block=64, knn_block=128, knn_chunk=4,
+    dict(ll_group=16, ll_unroll=1, mstep_lanes=4, sv_block=64, knn_block=128, knn_chunk=4,
          knn_cell=4, knn_lanes=4),
-    dict(ll_group=8, ll_unroll=4, mstep_lanes=16, sv_block=128, knn_block=256, knn_chunk=8,
+    dict(ll_group=64, ll_unroll=1, mstep_lanes=16, sv_block=128, knn_block=256, knn_chunk=8,
          knn_cell=2, knn_lanes=2),
-    dict(ll_group=16, ll_unroll=4, mstep_lanes=4, sv_block=256, knn_block=64, knn_chunk=4,
+    dict(ll_group=32, ll_unroll=1, mstep_lanes=4, sv_block=256, knn_block=64, knn_chunk=4,
          knn_cell=6, knn_lanes=4),
 ]
 
@@ -63,10 +63,10 @@ def test_every_tile_shape_gives_identical_results(frames, K):
 
 def test_invalid_tiles_are_rejected(frames):
     with em.Batch(frames[:1], em.make_opts(K=5, max_iter=2)) as b:
-        bad = dict(em.DEFAULT_TILES, ll_group=2)
+        bad = dict(em.DEFAULT_TILES, ll_group=8)
         with pytest.raises(em.TgmmError):
             b.set_tiles(bad)
-        bad = dict(em.DEFAULT_TILES, ll_group=16, ll_unroll=8)
+        bad = dict(em.DEFAULT_TILES, ll_group=16, ll_unroll=4)
         with pytest.raises(em.TgmmError):
             b.set_tiles(bad)
 

@@ -24,7 +24,6 @@ constexpr int kGB = 64;          // threads per block, Gaussian-lane kernels
 constexpr int kSB = 256;         // threads per block, supervoxel-lane kernels
 constexpr int kLLsv = 32;        // supervoxels per log-likelihood block
 constexpr int kLLgrp = 8;        // log-likelihood blocks per partial group (256 supervoxels)
-constexpr int kLLcap = 2048;     // voxels of a log-likelihood block staged in LDS
 constexpr int kPairShift = 4;    // pair code = (s << 4) | j, j < 16
 
 // state field indices (record order, GMEMupdateCUDA.h:22-34)
@@ -76,8 +75,8 @@ struct alignas(32) EPack {
 // results: the kernels fix their floating-point summation order
 // independently of the shape.
 struct Tiles {
-  int ll_gw;      // lanes per supervoxel, log-likelihood: 8, 16
-  int ll_u;       // voxels per lane per pass, log-likelihood: 4 (ll_gw 8, 16), 8 (ll_gw 8)
+  int ll_gw;      // lanes per supervoxel, log-likelihood: 16, 32, 64
+  int ll_u;       // voxels per lane per pass, log-likelihood: 1
   int ms_lanes;   // lanes per Gaussian, M-step statistics: 4, 8, 16
   int sv_block;   // threads per block, E-step and pair scatter: 64, 128, 256
   int knn_block;  // threads per block, kNN search: 64, 128, 256
@@ -86,7 +85,7 @@ struct Tiles {
                   // when its grid is built)
   int knn_lanes;  // lanes per query, kNN search: 1, 2, 4
 };
-constexpr Tiles kDefaultTiles = {16, 4, 16, 256, 256, 8, 4, 1};
+constexpr Tiles kDefaultTiles = {64, 1, 16, 256, 256, 8, 4, 1};
 bool tiles_valid(const Tiles& t);
 
 struct KOpts {

@@ -1043,206 +1043,138 @@ __device__ inline double frame_sum_w64(const double* part, int nblk) {
   return wave_sum((a4[0] + a4[1]) + (a4[2] + a4[3]));
 }
 
-// Log-likelihood over the voxel stream (GMEM.cu:1913-1978). A block of 256
-// threads covers kLLsv = 32 supervoxels of one frame.
-//  1. Threads 0..31 read the supervoxel headers (frame id, voxel range,
-//     anchor, candidate indices) into LDS.
-//  2. The block's voxels, one contiguous range of the stream, are copied into
-//     LDS as (x, y, z, img) with every load issued before any is used (ranges
-//     above kLLcap voxels are read from HBM in step 3 instead), and the
-//     candidates' parameter packs are read and turned into f32 constants:
-//     (a_s - m_k) rounded once from FP64, the scaled precision, the log-weight.
-//  3. Groups of GW lanes evaluate one supervoxel at a time from LDS.
+// Log-likelihood over the voxel stream (GMEM.cu:1913-1978). A block is one
+// wave and covers kLLsv = 32 consecutive supervoxels of one frame; groups of
+// GW lanes (GW = 64: the whole wave, so the supervoxel's header and its
+// candidates' parameters are wave-uniform) take one supervoxel at a time,
+// one voxel per lane per pass, with the next supervoxel's header loaded while
+// the current one is evaluated.
 // Per voxel, d = (x - a_s) + (a_s - m_k) with the anchor a_s a voxel of the
-// supervoxel (exact in f32), so the f32 quadratic form keeps full relative
-// precision far from the origin. Terms are evaluated in base 2 (hardware
-// exp2/log2), log-sum-exp with the largest term factored out, img-weighted
-// accumulation in FP64. A voxel whose largest term is below the FP64
-// underflow of the reference's direct sum (2^-1074) contributes nothing, as
-// in the reference.
+// supervoxel (exact in f32) and (a_s - m_k) rounded once from FP64, so the
+// f32 quadratic form keeps full relative precision far from the origin.
+// Terms are evaluated in base 2 (hardware exp2/log2), log-sum-exp with the
+// largest term factored out; a voxel's contribution img * term is rounded
+// to FP64 and accumulated in FP64. A voxel whose largest term is below the
+// FP64 underflow of the reference's direct sum (2^-1074) contributes nothing,
+// as in the reference.
 //
-// Summation order, identical for every (GW, U): voxel i of a supervoxel is
-// added, in increasing i, to class sum i mod 16; the 16 class sums are added
-// as ((c + c^8) + ...^4) ...^1; a block's 32 supervoxel sums by a fixed
+// Summation order, identical for every GW: voxel i of a supervoxel is added,
+// in increasing i, to class sum i mod 16; the 16 class sums are added as
+// ((c + c^8) + ...^4) ...^1; a block's 32 supervoxel sums by a fixed
 // pairwise tree; the partials of 8 consecutive blocks (256 supervoxels) by a
 // fixed tree in the last of them to finish; the frame's group partials by
 // frame_sum_w64 in the last group to finish.
-template <int KM, int GW, int U>
-__global__ __launch_bounds__(256) void k_loglik(DevPtrs P, KOpts O) {
-  constexpr int NC = 16 / GW;  // class sums per lane
-  constexpr int PASS = GW * U;
-  constexpr int NG = 256 / GW;  // groups per block
-  constexpr int NP = 12;        // floats per candidate constant set (3 x float4)
-  static_assert(PASS % 16 == 0, "a pass must cover whole classes");
-  static_assert(NG <= kLLsv || NG % kLLsv == 0, "groups per block");
-  __shared__ long long h_beg[kLLsv];
-  __shared__ int h_n[kLLsv];
-  __shared__ float h_a[kLLsv][3];
-  __shared__ int h_k[kLLsv][KM];
-  __shared__ float4 cp[kLLsv][KM][NP / 4];
-  __shared__ float4 vox[kLLcap];
-  __shared__ double svsum[kLLsv];
-  __shared__ long long s_vbase;
-  __shared__ int s_nv, s_last;
-  const long long sb = (long long)blockIdx.x * kLLsv;
-  const int t = threadIdx.x;
-  const int K = O.K;
+template <int KM>
+struct SvHead {
+  long long beg;
+  int n;
+  float a0, a1, a2;
+  int kk[KM];
+};
+
+template <int KM>
+__device__ inline SvHead<KM> ll_head(const DevPtrs& P, long long s, int f, int K) {
   const long long S = P.Scap;
+  SvHead<KM> h;
+  const int sf = P.sv_frame[s];
+  h.beg = P.sv_beg[s];
+  const int len = P.sv_len[s];
+  h.a0 = P.anchor[s];
+  h.a1 = P.anchor[S + s];
+  h.a2 = P.anchor[2 * S + s];
+#pragma unroll
+  for (int j = 0; j < KM; j++) h.kk[j] = (j < K) ? P.ind[(long long)j * S + s] : -1;
+  h.n = (sf == f) ? len : 0;
+  return h;
+}
+
+template <int KM, int GW>
+__global__ __launch_bounds__(64) void k_loglik(DevPtrs P, KOpts O) {
+  constexpr int NGW = 64 / GW;      // groups per wave
+  constexpr int SPG = kLLsv / NGW;  // supervoxels per group
+  __shared__ double svsum[kLLsv];
+  __shared__ int s_last;
+  const long long sb = (long long)blockIdx.x * kLLsv;
+  const int lane = threadIdx.x % GW, grp = threadIdx.x / GW;
+  const int K = O.K;
   const int f = P.sblk_frame[sb >> 8];
   if (f < 0) return;
   FrameDev* F = P.fr + f;
   if (!F->active) return;
-  // 1. headers
-  if (t < kLLsv) {
-    const long long s = sb + t;
-    const int sf = P.sv_frame[s];
-    const long long beg = P.sv_beg[s];
-    const int len = P.sv_len[s];
-    h_a[t][0] = P.anchor[s];
-    h_a[t][1] = P.anchor[S + s];
-    h_a[t][2] = P.anchor[2 * S + s];
-#pragma unroll
-    for (int j = 0; j < KM; j++) h_k[t][j] = (j < K) ? P.ind[(long long)j * S + s] : -1;
-    h_beg[t] = beg;
-    h_n[t] = (sf == f) ? len : 0;
-  }
-  __syncthreads();
-  if (t == 0) {
-    // the frame's supervoxels are consecutive in the voxel stream; padding
-    // entries (n = 0) come after the last one
-    int last = -1;
-    for (int j = 0; j < kLLsv; j++)
-      if (h_n[j] > 0) last = j;
-    const long long vb = last >= 0 ? h_beg[0] : 0;
-    s_vbase = vb;
-    s_nv = last >= 0 ? (int)min(h_beg[last] + h_n[last] - vb, (long long)kLLcap + 1) : 0;
-  }
-  __syncthreads();
-  // 2. voxels into LDS, candidate constants: all of the thread's loads are
-  // issued before the first LDS store
-  const long long vbase = s_vbase;
-  const int nv = s_nv;
-  const bool staged = nv <= kLLcap;
-  constexpr int VR = kLLcap / 256;
-  float4 tv[VR];
-#pragma unroll
-  for (int r = 0; r < VR; r++) {
-    const int i = t + 256 * r;
-    tv[r] = make_float4(0.f, 0.f, 0.f, 0.f);
-    if (staged && i < nv)
-      tv[r] = make_float4(P.vx[vbase + i], P.vy[vbase + i], P.vz[vbase + i], P.vimg[vbase + i]);
-  }
   const double lsa2 = F->log_sum_alpha * 1.4426950408889634074;
-  for (int e = t; e < kLLsv * KM; e += 256) {
-    const int sv = e / KM, j = e % KM;
-    const int k = h_k[sv][j];
-    float c[NP];
-#pragma unroll
-    for (int i = 0; i < NP; i++) c[i] = 0.0f;
-    c[9] = -INFINITY;
-    if (h_n[sv] > 0 && k >= 0) {
-      const LLPack lp = P.llpack[k];
-      c[0] = (float)((double)h_a[sv][0] - lp.m[0]);
-      c[1] = (float)((double)h_a[sv][1] - lp.m[1]);
-      c[2] = (float)((double)h_a[sv][2] - lp.m[2]);
-#pragma unroll
-      for (int i = 0; i < 6; i++) c[3 + i] = lp.M[i];
-      c[9] = (float)(lp.lg2 - lsa2);
-    }
-    cp[sv][j][0] = make_float4(c[0], c[1], c[2], c[3]);
-    cp[sv][j][1] = make_float4(c[4], c[5], c[6], c[7]);
-    cp[sv][j][2] = make_float4(c[8], c[9], c[10], c[11]);
-  }
-#pragma unroll
-  for (int r = 0; r < VR; r++) {
-    const int i = t + 256 * r;
-    if (staged && i < nv) vox[i] = tv[r];
-  }
-  __syncthreads();
-  // 3. one supervoxel per group at a time
-  const int lane = t % GW, grp = t / GW;
-  for (int sv = grp; sv < kLLsv; sv += NG) {
-    const int n = h_n[sv];
-    const float a0 = h_a[sv][0], a1 = h_a[sv][1], a2 = h_a[sv][2];
-    const int off = (int)(h_beg[sv] - vbase);
+  SvHead<KM> h = ll_head<KM>(P, sb + grp, f, K);
+  for (int it = 0; it < SPG; it++) {
+    const int sv = grp + NGW * it;
+    const SvHead<KM> c = h;
+    if (it + 1 < SPG) h = ll_head<KM>(P, sb + sv + NGW, f, K);
+    const int n = c.n;
     float cd0[KM], cd1[KM], cd2[KM], cw[KM][6], cg[KM];
 #pragma unroll
     for (int j = 0; j < KM; j++) {
-      const float4 p0 = cp[sv][j][0], p1 = cp[sv][j][1], p2 = cp[sv][j][2];
-      cd0[j] = p0.x;
-      cd1[j] = p0.y;
-      cd2[j] = p0.z;
-      cw[j][0] = p0.w;
-      cw[j][1] = p1.x;
-      cw[j][2] = p1.y;
-      cw[j][3] = p1.z;
-      cw[j][4] = p1.w;
-      cw[j][5] = p2.x;
-      cg[j] = (j < K) ? p2.y : -INFINITY;
+      cd0[j] = cd1[j] = cd2[j] = 0.0f;
+#pragma unroll
+      for (int i = 0; i < 6; i++) cw[j][i] = 0.0f;
+      cg[j] = -INFINITY;
+      if (n > 0 && c.kk[j] >= 0) {
+        const LLPack lp = P.llpack[c.kk[j]];
+        cd0[j] = (float)((double)c.a0 - lp.m[0]);
+        cd1[j] = (float)((double)c.a1 - lp.m[1]);
+        cd2[j] = (float)((double)c.a2 - lp.m[2]);
+#pragma unroll
+        for (int i = 0; i < 6; i++) cw[j][i] = lp.M[i];
+        cg[j] = (float)(lp.lg2 - lsa2);
+      }
     }
-    double cls[NC];
-#pragma unroll
-    for (int q = 0; q < NC; q++) cls[q] = 0.0;
-    // one voxel's term log2(sum_j 2^t_j), added to its class sum
-    auto term = [&](const float4 v, double& acc) {
+    double cls = 0.0;  // class sum lane (lanes 0..15 of the group)
+    for (int base = 0; base < n; base += GW) {
+      const int i = base + lane;
+      double v = 0.0;
+      if (i < n) {
 #pragma clang fp contract(off)
-      const float x0 = v.x - a0, x1 = v.y - a1, x2 = v.z - a2;
-      float tj[KM];
-      float mx = -INFINITY;
+        const long long g = c.beg + i;
+        const float x0 = P.vx[g] - c.a0, x1 = P.vy[g] - c.a1, x2 = P.vz[g] - c.a2;
+        const float w = P.vimg[g];
+        float tj[KM];
+        float mx = -INFINITY;
 #pragma unroll
-      for (int j = 0; j < KM; j++) {
-        const float d0 = x0 + cd0[j], d1 = x1 + cd1[j], d2 = x2 + cd2[j];
-        float q = __builtin_fmaf(cw[j][0] * d0, d0, 0.0f);
-        q = __builtin_fmaf(cw[j][3] * d1, d1, q);
-        q = __builtin_fmaf(cw[j][5] * d2, d2, q);
-        float o = cw[j][1] * d0 * d1;
-        o = __builtin_fmaf(cw[j][2] * d0, d2, o);
-        o = __builtin_fmaf(cw[j][4] * d1, d2, o);
-        q = __builtin_fmaf(2.0f, o, q);
-        tj[j] = cg[j] - q;
-        mx = fmaxf(mx, tj[j]);
-      }
-      if (mx > -1074.0f) {
-        float sum = 0.0f;
+        for (int j = 0; j < KM; j++) {
+          const float d0 = x0 + cd0[j], d1 = x1 + cd1[j], d2 = x2 + cd2[j];
+          float q = __builtin_fmaf(cw[j][0] * d0, d0, 0.0f);
+          q = __builtin_fmaf(cw[j][3] * d1, d1, q);
+          q = __builtin_fmaf(cw[j][5] * d2, d2, q);
+          float o = cw[j][1] * d0 * d1;
+          o = __builtin_fmaf(cw[j][2] * d0, d2, o);
+          o = __builtin_fmaf(cw[j][4] * d1, d2, o);
+          q = __builtin_fmaf(2.0f, o, q);
+          tj[j] = cg[j] - q;
+          mx = fmaxf(mx, tj[j]);
+        }
+        if (mx > -1074.0f) {
+          float sum = 0.0f;
 #pragma unroll
-        for (int j = 0; j < KM; j++) sum += __builtin_amdgcn_exp2f(tj[j] - mx);
-        acc += (double)v.w * (double)(mx + __builtin_amdgcn_logf(sum));
-      }
-    };
-    for (int base = 0; base < n; base += PASS) {
-      float4 v[U];
-#pragma unroll
-      for (int u = 0; u < U; u++) {
-        const int i = base + lane + GW * u;
-        v[u] = make_float4(0.f, 0.f, 0.f, 0.f);
-        if (i < n) {
-          if (staged) {
-            v[u] = vox[off + i];
-          } else {
-            const long long g = h_beg[sv] + i;
-            v[u] = make_float4(P.vx[g], P.vy[g], P.vz[g], P.vimg[g]);
-          }
+          for (int j = 0; j < KM; j++) sum += __builtin_amdgcn_exp2f(tj[j] - mx);
+          v = (double)w * (double)(mx + __builtin_amdgcn_logf(sum));
         }
       }
-#pragma unroll
-      for (int u = 0; u < U; u++)
-        if (base + lane + GW * u < n) term(v[u], cls[u % NC]);
+      // this pass's voxels of class (lane mod 16), in increasing index
+      {
+#pragma clang fp contract(off)
+        cls += v;
+        if (GW >= 32) cls += __shfl_down(v, 16, GW);
+        if (GW >= 64) {
+          cls += __shfl_down(v, 32, GW);
+          cls += __shfl_down(v, 48, GW);
+        }
+      }
     }
-    // class sums: in-lane stages (strides >= GW), then lane butterflies
+    double tot = cls;
 #pragma unroll
-    for (int m = 8; m >= GW; m >>= 1) {
-      const int qm = m / GW;
-#pragma unroll
-      for (int q = 0; q < qm; q++) cls[q] += cls[q + qm];
-    }
-    double tot = cls[0];
-#pragma unroll
-    for (int m = GW / 2; m >= 1; m >>= 1) tot += __shfl_xor(tot, m, GW);
+    for (int m = 8; m >= 1; m >>= 1) tot += __shfl_xor(tot, m, 16);
     if (lane == 0) svsum[sv] = tot * 0.69314718055994530942;
   }
   __syncthreads();
   // block partial, then the 8-block group, then the frame
-  if (t == 0) {
+  if (threadIdx.x == 0) {
     double tr[kLLsv / 2];
 #pragma unroll
     for (int i = 0; i < kLLsv / 2; i++) tr[i] = svsum[2 * i] + svsum[2 * i + 1];
@@ -1273,10 +1205,10 @@ __global__ __launch_bounds__(256) void k_loglik(DevPtrs P, KOpts O) {
     s_last = last;
   }
   __syncthreads();
-  if (!s_last || t >= 64) return;
+  if (!s_last) return;
   __threadfence();
   const double ll = frame_sum_w64(P.part_sg + F->s0 / (kLLsv * kLLgrp), F->nbl / kLLgrp);
-  if (t == 0) {
+  if (threadIdx.x == 0) {
     P.done[4 * f + 3] = 0u;
     F->ll3 = F->ll2;
     F->ll2 = F->ll1;
@@ -1343,12 +1275,12 @@ hipError_t launch_next_priors(const DevPtrs& P, long long g0, long long ng, doub
 template <int KM>
 static void launch_loglik(const DevPtrs& P, const KOpts& O, hipStream_t st) {
   const unsigned g = nb(P.Scap, kLLsv);
-  if (O.tiles.ll_gw == 16)
-    k_loglik<KM, 16, 4><<<g, 256, 0, st>>>(P, O);
-  else if (O.tiles.ll_u == 4)
-    k_loglik<KM, 8, 4><<<g, 256, 0, st>>>(P, O);
+  if (O.tiles.ll_gw == 64)
+    k_loglik<KM, 64><<<g, 64, 0, st>>>(P, O);
+  else if (O.tiles.ll_gw == 32)
+    k_loglik<KM, 32><<<g, 64, 0, st>>>(P, O);
   else
-    k_loglik<KM, 8, 8><<<g, 256, 0, st>>>(P, O);
+    k_loglik<KM, 16><<<g, 64, 0, st>>>(P, O);
 }
 
 static void launch_mstep_stats(const DevPtrs& P, const KOpts& O, hipStream_t st) {

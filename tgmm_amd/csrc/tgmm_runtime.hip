@@ -182,8 +182,7 @@ KOpts make_kopts(const tgmm_em_opts& o, const float* scale_reg) {
 
 bool tgmm::tiles_valid(const Tiles& t) {
   auto in = [](int v, int a, int b, int c) { return v == a || v == b || v == c; };
-  return ((t.ll_gw == 16 && t.ll_u == 4) || (t.ll_gw == 8 && (t.ll_u == 4 || t.ll_u == 8))) &&
-         in(t.ms_lanes, 4, 8, 16) &&
+  return in(t.ll_gw, 16, 32, 64) && t.ll_u == 1 && in(t.ms_lanes, 4, 8, 16) &&
          in(t.sv_block, 64, 128, 256) && in(t.knn_block, 64, 128, 256) &&
          (t.knn_chunk == 4 || t.knn_chunk == 8) && in(t.knn_cell, 2, 4, 6) &&
          in(t.knn_lanes, 1, 2, 4);
@@ -694,7 +693,7 @@ extern "C" int tgmm_batch_autotune(tgmm_batch* b, int reps, tgmm_tiles* chosen, 
     int v[18][3];
   };
   const Knob knobs[5] = {
-      {"loglik", TK_LOGLIK, 3, {{16, 4}, {8, 4}, {8, 8}}},
+      {"loglik", TK_LOGLIK, 3, {{16, 1}, {32, 1}, {64, 1}}},
       {"mstep", TK_MSTEP, 3, {{4, 0}, {8, 0}, {16, 0}}},
       {"estep", TK_ESTEP, 3, {{64, 0}, {128, 0}, {256, 0}}},
       {"knn_cell", TK_KNN, 3, {{2, 0}, {4, 0}, {6, 0}}},
