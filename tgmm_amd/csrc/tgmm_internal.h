@@ -22,8 +22,7 @@ namespace tgmm {
 constexpr int kAlign = 256;      // supervoxel / Gaussian range alignment
 constexpr int kGB = 64;          // threads per block, Gaussian-lane kernels
 constexpr int kSB = 256;         // threads per block, supervoxel-lane kernels
-constexpr int kLLsv = 32;        // supervoxels per log-likelihood block
-constexpr int kLLgrp = 8;        // log-likelihood blocks per partial group (256 supervoxels)
+constexpr int kLLminB = 4;       // fewest supervoxels per log-likelihood block
 constexpr int kPairShift = 4;    // pair code = (s << 4) | j, j < 16
 
 // state field indices (record order, GMEMupdateCUDA.h:22-34)
@@ -42,7 +41,7 @@ struct FrameDev {
   float h;            // cell edge
   int dims[3];
   int nbg;            // kGB-blocks over the 256-aligned Gaussian range
-  int nbl;            // kLLsv-supervoxel blocks over the 256-aligned range
+  int nbl;            // 256-supervoxel groups over the aligned range (log-likelihood)
   int max_iter;
   double tol;
   double sum_alpha, psi_sum_alpha, log_sum_alpha, sum_alpha2;
@@ -76,7 +75,7 @@ struct alignas(32) EPack {
 // independently of the shape.
 struct Tiles {
   int ll_gw;      // lanes per supervoxel, log-likelihood: 16, 32, 64
-  int ll_u;       // voxels per lane per pass, log-likelihood: 1
+  int ll_u;       // supervoxels per lane group, log-likelihood: 1, 2 (4 with ll_gw 16)
   int ms_lanes;   // lanes per Gaussian, M-step statistics: 4, 8, 16
   int sv_block;   // threads per block, E-step and pair scatter: 64, 128, 256
   int knn_block;  // threads per block, kNN search: 64, 128, 256
@@ -85,7 +84,7 @@ struct Tiles {
                   // when its grid is built)
   int knn_lanes;  // lanes per query, kNN search: 1, 2, 4
 };
-constexpr Tiles kDefaultTiles = {64, 1, 16, 256, 256, 8, 4, 1};
+constexpr Tiles kDefaultTiles = {16, 1, 16, 256, 256, 8, 4, 1};
 bool tiles_valid(const Tiles& t);
 
 struct KOpts {
@@ -137,8 +136,8 @@ struct DevPtrs {
   float4* cg4;      // Gcap: cell-sorted references (x, y, z, index bits)
   // reductions
   double* part_g;   // Gcap/kGB
-  double* part_s;   // Scap/kLLsv: log-likelihood block partials
-  double* part_sg;  // Scap/256: partials of groups of kLLgrp blocks
+  double* part_s;   // Scap/kLLminB: log-likelihood block partials
+  double* part_sg;  // Scap/256: partials of 256-supervoxel groups
   unsigned* grp_cnt;  // Scap/256: arrivals per group (zero between launches)
   unsigned* done;   // 4*Fcap
   FrameDev* fr;

@@ -1043,12 +1043,11 @@ __device__ inline double frame_sum_w64(const double* part, int nblk) {
   return wave_sum((a4[0] + a4[1]) + (a4[2] + a4[3]));
 }
 
-// Log-likelihood over the voxel stream (GMEM.cu:1913-1978). A block is one
-// wave and covers kLLsv = 32 consecutive supervoxels of one frame; groups of
-// GW lanes (GW = 64: the whole wave, so the supervoxel's header and its
-// candidates' parameters are wave-uniform) take one supervoxel at a time,
-// one voxel per lane per pass, with the next supervoxel's header loaded while
-// the current one is evaluated.
+// Log-likelihood over the voxel stream (GMEM.cu:1913-1978). A block has 256
+// threads in groups of GW lanes; each group takes SPG supervoxels in turn
+// (one voxel per lane per pass), with the next one's header loaded while the
+// current one is evaluated, so a block covers B = (256/GW)*SPG consecutive
+// supervoxels of one frame.
 // Per voxel, d = (x - a_s) + (a_s - m_k) with the anchor a_s a voxel of the
 // supervoxel (exact in f32) and (a_s - m_k) rounded once from FP64, so the
 // f32 quadratic form keeps full relative precision far from the origin.
@@ -1058,12 +1057,12 @@ __device__ inline double frame_sum_w64(const double* part, int nblk) {
 // FP64 underflow of the reference's direct sum (2^-1074) contributes nothing,
 // as in the reference.
 //
-// Summation order, identical for every GW: voxel i of a supervoxel is added,
-// in increasing i, to class sum i mod 16; the 16 class sums are added as
-// ((c + c^8) + ...^4) ...^1; a block's 32 supervoxel sums by a fixed
-// pairwise tree; the partials of 8 consecutive blocks (256 supervoxels) by a
-// fixed tree in the last of them to finish; the frame's group partials by
-// frame_sum_w64 in the last group to finish.
+// Summation order, identical for every (GW, SPG): voxel i of a supervoxel is
+// added, in increasing i, to class sum i mod 16; the 16 class sums are added
+// as ((c + c^8) + ...^4) ...^1; the sums of each 256 consecutive supervoxels
+// by one pairwise tree (a block forms the subtree over its B leaves, the last
+// block of the 256 to finish the levels above); the frame's 256-supervoxel
+// partials by frame_sum_w64 in the last of them to finish.
 template <int KM>
 struct SvHead {
   long long beg;
@@ -1088,13 +1087,24 @@ __device__ inline SvHead<KM> ll_head(const DevPtrs& P, long long s, int f, int K
   return h;
 }
 
-template <int KM, int GW>
-__global__ __launch_bounds__(64) void k_loglik(DevPtrs P, KOpts O) {
-  constexpr int NGW = 64 / GW;      // groups per wave
-  constexpr int SPG = kLLsv / NGW;  // supervoxels per group
-  __shared__ double svsum[kLLsv];
+template <int N>
+__device__ inline double pairwise_tree(double* v) {  // in place, v[0] = result
+#pragma unroll
+  for (int w = N / 2; w >= 1; w >>= 1)
+#pragma unroll
+    for (int i = 0; i < w; i++) v[i] = v[2 * i] + v[2 * i + 1];
+  return v[0];
+}
+
+template <int KM, int GW, int SPG>
+__global__ __launch_bounds__(256) void k_loglik(DevPtrs P, KOpts O) {
+  constexpr int NGB = 256 / GW;  // groups per block
+  constexpr int B = NGB * SPG;   // supervoxels per block
+  constexpr int BPG = 256 / B;   // blocks per 256 supervoxels
+  static_assert(B <= 256 && 256 % B == 0, "block covers a divisor of 256 supervoxels");
+  __shared__ double svsum[B];
   __shared__ int s_last;
-  const long long sb = (long long)blockIdx.x * kLLsv;
+  const long long sb = (long long)blockIdx.x * B;
   const int lane = threadIdx.x % GW, grp = threadIdx.x / GW;
   const int K = O.K;
   const int f = P.sblk_frame[sb >> 8];
@@ -1104,9 +1114,9 @@ __global__ __launch_bounds__(64) void k_loglik(DevPtrs P, KOpts O) {
   const double lsa2 = F->log_sum_alpha * 1.4426950408889634074;
   SvHead<KM> h = ll_head<KM>(P, sb + grp, f, K);
   for (int it = 0; it < SPG; it++) {
-    const int sv = grp + NGW * it;
+    const int sv = grp + NGB * it;
     const SvHead<KM> c = h;
-    if (it + 1 < SPG) h = ll_head<KM>(P, sb + sv + NGW, f, K);
+    if (it + 1 < SPG) h = ll_head<KM>(P, sb + sv + NGB, f, K);
     const int n = c.n;
     float cd0[KM], cd1[KM], cd2[KM], cw[KM][6], cg[KM];
 #pragma unroll
@@ -1125,7 +1135,7 @@ __global__ __launch_bounds__(64) void k_loglik(DevPtrs P, KOpts O) {
         cg[j] = (float)(lp.lg2 - lsa2);
       }
     }
-    double cls = 0.0;  // class sum lane (lanes 0..15 of the group)
+    double cls = 0.0;  // class sum (lanes 0..15 of the group)
     for (int base = 0; base < n; base += GW) {
       const int i = base + lane;
       double v = 0.0;
@@ -1173,41 +1183,34 @@ __global__ __launch_bounds__(64) void k_loglik(DevPtrs P, KOpts O) {
     if (lane == 0) svsum[sv] = tot * 0.69314718055994530942;
   }
   __syncthreads();
-  // block partial, then the 8-block group, then the frame
+  // block subtree, then the levels above it in the 256-supervoxel group, then
+  // the frame
   if (threadIdx.x == 0) {
-    double tr[kLLsv / 2];
+    double tr[B];
 #pragma unroll
-    for (int i = 0; i < kLLsv / 2; i++) tr[i] = svsum[2 * i] + svsum[2 * i + 1];
-#pragma unroll
-    for (int w = kLLsv / 4; w >= 1; w >>= 1)
-#pragma unroll
-      for (int i = 0; i < w; i++) tr[i] = tr[2 * i] + tr[2 * i + 1];
-    P.part_s[blockIdx.x] = tr[0];
+    for (int i = 0; i < B; i++) tr[i] = svsum[i];
+    P.part_s[blockIdx.x] = pairwise_tree<B>(tr);
     __threadfence();
-    const long long g = blockIdx.x / kLLgrp;
+    const long long g = sb >> 8;
     const unsigned prev = atomicAdd(P.grp_cnt + g, 1u);
     int last = 0;
-    if (prev == (unsigned)(kLLgrp - 1)) {
+    if (prev == (unsigned)(BPG - 1)) {
       __threadfence();
       P.grp_cnt[g] = 0u;
-      double q8[kLLgrp];
+      double q[BPG];
 #pragma unroll
-      for (int i = 0; i < kLLgrp; i++) q8[i] = load_coherent(P.part_s + g * kLLgrp + i);
-#pragma unroll
-      for (int w = kLLgrp / 2; w >= 1; w >>= 1)
-#pragma unroll
-        for (int i = 0; i < w; i++) q8[i] = q8[2 * i] + q8[2 * i + 1];
-      P.part_sg[g] = q8[0];
+      for (int i = 0; i < BPG; i++) q[i] = load_coherent(P.part_s + g * BPG + i);
+      P.part_sg[g] = pairwise_tree<BPG>(q);
       __threadfence();
       const unsigned pf = atomicAdd(P.done + 4 * f + 3, 1u);
-      last = (pf == (unsigned)(F->nbl / kLLgrp - 1));
+      last = (pf == (unsigned)(F->nbl - 1));
     }
     s_last = last;
   }
   __syncthreads();
-  if (!s_last) return;
+  if (!s_last || threadIdx.x >= 64) return;
   __threadfence();
-  const double ll = frame_sum_w64(P.part_sg + F->s0 / (kLLsv * kLLgrp), F->nbl / kLLgrp);
+  const double ll = frame_sum_w64(P.part_sg + (F->s0 >> 8), F->nbl);
   if (threadIdx.x == 0) {
     P.done[4 * f + 3] = 0u;
     F->ll3 = F->ll2;
@@ -1274,13 +1277,22 @@ hipError_t launch_next_priors(const DevPtrs& P, long long g0, long long ng, doub
 
 template <int KM>
 static void launch_loglik(const DevPtrs& P, const KOpts& O, hipStream_t st) {
-  const unsigned g = nb(P.Scap, kLLsv);
-  if (O.tiles.ll_gw == 64)
-    k_loglik<KM, 64><<<g, 64, 0, st>>>(P, O);
-  else if (O.tiles.ll_gw == 32)
-    k_loglik<KM, 32><<<g, 64, 0, st>>>(P, O);
-  else
-    k_loglik<KM, 16><<<g, 64, 0, st>>>(P, O);
+  const int gw = O.tiles.ll_gw, spg = O.tiles.ll_u;
+  const long long B = (256 / gw) * spg;
+  const unsigned g = nb(P.Scap, B);
+#define TGMM_LL(GW_, SPG_)                                    \
+  if (gw == GW_ && spg == SPG_) {                             \
+    k_loglik<KM, GW_, SPG_><<<g, 256, 0, st>>>(P, O);         \
+    return;                                                   \
+  }
+  TGMM_LL(16, 1)
+  TGMM_LL(16, 2)
+  TGMM_LL(16, 4)
+  TGMM_LL(32, 1)
+  TGMM_LL(32, 2)
+  TGMM_LL(64, 1)
+  TGMM_LL(64, 2)
+#undef TGMM_LL
 }
 
 static void launch_mstep_stats(const DevPtrs& P, const KOpts& O, hipStream_t st) {

@@ -112,7 +112,7 @@ int ws_alloc(Workspace* w, const WsShape& sh, int K) {
   AL(P.cell_start, C + 1);
   AL(P.cg4, G);
   AL(P.part_g, G / kGB);
-  AL(P.part_s, S / kLLsv);
+  AL(P.part_s, S / kLLminB);
   AL(P.part_sg, S / 256);
   AL(P.grp_cnt, S / 256);
   AL(P.done, 4 * F);
@@ -182,7 +182,8 @@ KOpts make_kopts(const tgmm_em_opts& o, const float* scale_reg) {
 
 bool tgmm::tiles_valid(const Tiles& t) {
   auto in = [](int v, int a, int b, int c) { return v == a || v == b || v == c; };
-  return in(t.ll_gw, 16, 32, 64) && t.ll_u == 1 && in(t.ms_lanes, 4, 8, 16) &&
+  return in(t.ll_gw, 16, 32, 64) && (t.ll_u == 1 || t.ll_u == 2 || (t.ll_u == 4 && t.ll_gw == 16)) &&
+         in(t.ms_lanes, 4, 8, 16) &&
          in(t.sv_block, 64, 128, 256) && in(t.knn_block, 64, 128, 256) &&
          (t.knn_chunk == 4 || t.knn_chunk == 8) && in(t.knn_cell, 2, 4, 6) &&
          in(t.knn_lanes, 1, 2, 4);
@@ -242,7 +243,7 @@ int batch_build(int device, const std::vector<FrameSrc>& fs, const tgmm_em_opts&
     // per-frame reductions, including trailing blocks with no valid element;
     // the Gaussian range has at least one padding entry (segment end)
     F.nbg = (int)(round_up(f.G + 1, kAlign) / kGB);
-    F.nbl = (int)(round_up(std::max<long long>(f.S, 1), kAlign) / kLLsv);
+    F.nbl = (int)(round_up(std::max<long long>(f.S, 1), kAlign) / 256);
     F.max_iter = o.max_iter;
     F.tol = o.tol;
     v += f.N;
@@ -693,7 +694,7 @@ extern "C" int tgmm_batch_autotune(tgmm_batch* b, int reps, tgmm_tiles* chosen, 
     int v[18][3];
   };
   const Knob knobs[5] = {
-      {"loglik", TK_LOGLIK, 3, {{16, 1}, {32, 1}, {64, 1}}},
+      {"loglik", TK_LOGLIK, 7, {{16, 1}, {16, 2}, {16, 4}, {32, 1}, {32, 2}, {64, 1}, {64, 2}}},
       {"mstep", TK_MSTEP, 3, {{4, 0}, {8, 0}, {16, 0}}},
       {"estep", TK_ESTEP, 3, {{64, 0}, {128, 0}, {256, 0}}},
       {"knn_cell", TK_KNN, 3, {{2, 0}, {4, 0}, {6, 0}}},

@@ -363,7 +363,7 @@ static int admit(tgmm_server* s, Ticket& t, int f) {
   F.c0 = t.c0;
   F.ncap = t.ncap;
   F.nbg = (int)(t.ga / kGB);
-  F.nbl = (int)(t.sa / kLLsv);
+  F.nbl = (int)(t.sa / 256);
   F.max_iter = s->o.em.max_iter;
   F.tol = s->o.em.tol;
   HIPCHK(hipMemcpyAsync(P.fr + f, &F, sizeof(FrameDev), hipMemcpyHostToDevice, st));
