@@ -1063,28 +1063,34 @@ __device__ inline double frame_sum_w64(const double* part, int nblk) {
 // by one pairwise tree (a block forms the subtree over its B leaves, the last
 // block of the 256 to finish the levels above); the frame's 256-supervoxel
 // partials by frame_sum_w64 in the last of them to finish.
-template <int KM>
+// a supervoxel's header as one lane of its group holds it: lane j < K also
+// holds candidate j's index
 struct SvHead {
   long long beg;
   int n;
   float a0, a1, a2;
-  int kk[KM];
+  int kk;
 };
 
-template <int KM>
-__device__ inline SvHead<KM> ll_head(const DevPtrs& P, long long s, int f, int K) {
+__device__ inline SvHead ll_head(const DevPtrs& P, long long s, int f, int lane, int K) {
   const long long S = P.Scap;
-  SvHead<KM> h;
+  SvHead h;
   const int sf = P.sv_frame[s];
   h.beg = P.sv_beg[s];
   const int len = P.sv_len[s];
   h.a0 = P.anchor[s];
   h.a1 = P.anchor[S + s];
   h.a2 = P.anchor[2 * S + s];
-#pragma unroll
-  for (int j = 0; j < KM; j++) h.kk[j] = (j < K) ? P.ind[(long long)j * S + s] : -1;
+  h.kk = (lane < K) ? P.ind[(long long)lane * S + s] : -1;
   h.n = (sf == f) ? len : 0;
   return h;
+}
+
+// value of lane j of the group (a whole-wave group reads it into a scalar)
+template <int GW>
+__device__ inline float group_bcast(float x, int j) {
+  if (GW == 64) return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), j));
+  return __shfl(x, j, GW);
 }
 
 template <int N>
@@ -1112,28 +1118,33 @@ __global__ __launch_bounds__(256) void k_loglik(DevPtrs P, KOpts O) {
   FrameDev* F = P.fr + f;
   if (!F->active) return;
   const double lsa2 = F->log_sum_alpha * 1.4426950408889634074;
-  SvHead<KM> h = ll_head<KM>(P, sb + grp, f, K);
+  static_assert(GW >= KM, "a lane per candidate");
+  SvHead h = ll_head(P, sb + grp, f, lane, K);
   for (int it = 0; it < SPG; it++) {
     const int sv = grp + NGB * it;
-    const SvHead<KM> c = h;
-    if (it + 1 < SPG) h = ll_head<KM>(P, sb + sv + NGB, f, K);
+    const SvHead c = h;
+    if (it + 1 < SPG) h = ll_head(P, sb + sv + NGB, f, lane, K);
     const int n = c.n;
+    // lane j < K reads candidate j's pack; the constants are broadcast
+    float pd0 = 0.0f, pd1 = 0.0f, pd2 = 0.0f, pw[6] = {0, 0, 0, 0, 0, 0}, pg = -INFINITY;
+    if (n > 0 && c.kk >= 0) {
+      const LLPack lp = P.llpack[c.kk];
+      pd0 = (float)((double)c.a0 - lp.m[0]);
+      pd1 = (float)((double)c.a1 - lp.m[1]);
+      pd2 = (float)((double)c.a2 - lp.m[2]);
+#pragma unroll
+      for (int i = 0; i < 6; i++) pw[i] = lp.M[i];
+      pg = (float)(lp.lg2 - lsa2);
+    }
     float cd0[KM], cd1[KM], cd2[KM], cw[KM][6], cg[KM];
 #pragma unroll
     for (int j = 0; j < KM; j++) {
-      cd0[j] = cd1[j] = cd2[j] = 0.0f;
+      cd0[j] = group_bcast<GW>(pd0, j);
+      cd1[j] = group_bcast<GW>(pd1, j);
+      cd2[j] = group_bcast<GW>(pd2, j);
 #pragma unroll
-      for (int i = 0; i < 6; i++) cw[j][i] = 0.0f;
-      cg[j] = -INFINITY;
-      if (n > 0 && c.kk[j] >= 0) {
-        const LLPack lp = P.llpack[c.kk[j]];
-        cd0[j] = (float)((double)c.a0 - lp.m[0]);
-        cd1[j] = (float)((double)c.a1 - lp.m[1]);
-        cd2[j] = (float)((double)c.a2 - lp.m[2]);
-#pragma unroll
-        for (int i = 0; i < 6; i++) cw[j][i] = lp.M[i];
-        cg[j] = (float)(lp.lg2 - lsa2);
-      }
+      for (int i = 0; i < 6; i++) cw[j][i] = group_bcast<GW>(pw[i], j);
+      cg[j] = (j < K) ? group_bcast<GW>(pg, j) : -INFINITY;
     }
     double cls = 0.0;  // class sum (lanes 0..15 of the group)
     for (int base = 0; base < n; base += GW) {
