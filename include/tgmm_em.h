@@ -183,11 +183,11 @@ int tgmm_batch_solve(tgmm_batch* b, int mode);
 int tgmm_batch_fetch(tgmm_batch* b, tgmm_frame_desc* frames, int nframes);
 /* Launch shapes of the tunable kernels. Every valid choice gives bitwise the
  * same results (the kernels fix their summation order independently of the
- * shape); only the speed changes. Defaults: {16, 1, 16, 256, 256, 8, 4, 1}. */
+ * shape); only the speed changes. Defaults: {16, 4, 16, 256, 256, 8, 4, 1}. */
 typedef struct tgmm_tiles {
   int ll_group;    /* lanes per supervoxel, log-likelihood: 16, 32 or 64 */
   int ll_unroll;   /* supervoxels per lane group in turn, log-likelihood:
-                      1 or 2 (4 with ll_group 16) */
+                      1, 2 or 4 (8 with ll_group 16) */
   int mstep_lanes; /* lanes per Gaussian, M-step statistics: 4, 8 or 16 */
   int sv_block;    /* threads per block, E-step and pair scatter: 64, 128, 256 */
   int knn_block;   /* threads per block, kNN search: 64, 128 or 256 */

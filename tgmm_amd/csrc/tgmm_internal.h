@@ -75,7 +75,7 @@ struct alignas(32) EPack {
 // independently of the shape.
 struct Tiles {
   int ll_gw;      // lanes per supervoxel, log-likelihood: 16, 32, 64
-  int ll_u;       // supervoxels per lane group, log-likelihood: 1, 2 (4 with ll_gw 16)
+  int ll_u;       // supervoxels per lane group, log-likelihood: 1, 2, 4 (8 with ll_gw 16)
   int ms_lanes;   // lanes per Gaussian, M-step statistics: 4, 8, 16
   int sv_block;   // threads per block, E-step and pair scatter: 64, 128, 256
   int knn_block;  // threads per block, kNN search: 64, 128, 256
@@ -84,7 +84,7 @@ struct Tiles {
                   // when its grid is built)
   int knn_lanes;  // lanes per query, kNN search: 1, 2, 4
 };
-constexpr Tiles kDefaultTiles = {16, 1, 16, 256, 256, 8, 4, 1};
+constexpr Tiles kDefaultTiles = {16, 4, 16, 256, 256, 8, 4, 1};
 bool tiles_valid(const Tiles& t);
 
 struct KOpts {
@@ -138,7 +138,6 @@ struct DevPtrs {
   double* part_g;   // Gcap/kGB
   double* part_s;   // Scap/kLLminB: log-likelihood block partials
   double* part_sg;  // Scap/256: partials of 256-supervoxel groups
-  unsigned* grp_cnt;  // Scap/256: arrivals per group (zero between launches)
   unsigned* done;   // 4*Fcap
   FrameDev* fr;
   long long Scap, Gcap, Ccap, Vcap;

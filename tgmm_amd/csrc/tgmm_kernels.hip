@@ -1044,10 +1044,12 @@ __device__ inline double frame_sum_w64(const double* part, int nblk) {
 }
 
 // Log-likelihood over the voxel stream (GMEM.cu:1913-1978). A block has 256
-// threads in groups of GW lanes; each group takes SPG supervoxels in turn
-// (one voxel per lane per pass), with the next one's header loaded while the
-// current one is evaluated, so a block covers B = (256/GW)*SPG consecutive
-// supervoxels of one frame.
+// threads in groups of GW lanes; each group takes SPG supervoxels in turn,
+// 64 voxels per pass (U = 64/GW per lane, all loads of a pass issued
+// together), with the next supervoxel's header loaded while the current one
+// is evaluated; a block covers B = (256/GW)*SPG consecutive supervoxels of
+// one frame. Lane j < K of a group reads candidate j's parameter pack; the
+// constants are broadcast to the group (into scalars for a whole-wave group).
 // Per voxel, d = (x - a_s) + (a_s - m_k) with the anchor a_s a voxel of the
 // supervoxel (exact in f32) and (a_s - m_k) rounded once from FP64, so the
 // f32 quadratic form keeps full relative precision far from the origin.
@@ -1060,16 +1062,15 @@ __device__ inline double frame_sum_w64(const double* part, int nblk) {
 // Summation order, identical for every (GW, SPG): voxel i of a supervoxel is
 // added, in increasing i, to class sum i mod 16; the 16 class sums are added
 // as ((c + c^8) + ...^4) ...^1; the sums of each 256 consecutive supervoxels
-// by one pairwise tree (a block forms the subtree over its B leaves, the last
-// block of the 256 to finish the levels above); the frame's 256-supervoxel
-// partials by frame_sum_w64 in the last of them to finish.
-// a supervoxel's header as one lane of its group holds it: lane j < K also
-// holds candidate j's index
+// by one pairwise tree (a block forms the subtree over its B leaves,
+// k_ll_final the levels above); the frame's 256-supervoxel partials by
+// frame_sum_w64. No block waits for another: the frame total is formed by
+// k_ll_final after this kernel.
 struct SvHead {
   long long beg;
   int n;
   float a0, a1, a2;
-  int kk;
+  int kk;  // lane j < K: candidate j
 };
 
 __device__ inline SvHead ll_head(const DevPtrs& P, long long s, int f, int lane, int K) {
@@ -1106,26 +1107,35 @@ template <int KM, int GW, int SPG>
 __global__ __launch_bounds__(256) void k_loglik(DevPtrs P, KOpts O) {
   constexpr int NGB = 256 / GW;  // groups per block
   constexpr int B = NGB * SPG;   // supervoxels per block
-  constexpr int BPG = 256 / B;   // blocks per 256 supervoxels
+  constexpr int U = 64 / GW;     // voxels per lane per pass
   static_assert(B <= 256 && 256 % B == 0, "block covers a divisor of 256 supervoxels");
+  static_assert(GW >= KM, "a lane per candidate");
   __shared__ double svsum[B];
-  __shared__ int s_last;
   const long long sb = (long long)blockIdx.x * B;
   const int lane = threadIdx.x % GW, grp = threadIdx.x / GW;
   const int K = O.K;
   const int f = P.sblk_frame[sb >> 8];
   if (f < 0) return;
-  FrameDev* F = P.fr + f;
+  const FrameDev* F = P.fr + f;
   if (!F->active) return;
   const double lsa2 = F->log_sum_alpha * 1.4426950408889634074;
-  static_assert(GW >= KM, "a lane per candidate");
   SvHead h = ll_head(P, sb + grp, f, lane, K);
   for (int it = 0; it < SPG; it++) {
     const int sv = grp + NGB * it;
     const SvHead c = h;
     if (it + 1 < SPG) h = ll_head(P, sb + sv + NGB, f, lane, K);
     const int n = c.n;
-    // lane j < K reads candidate j's pack; the constants are broadcast
+    // the first pass's voxels, then the candidates' constants
+    float4 vv[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const int i = lane + GW * u;
+      vv[u] = make_float4(0.f, 0.f, 0.f, 0.f);
+      if (i < n) {
+        const long long g = c.beg + i;
+        vv[u] = make_float4(P.vx[g], P.vy[g], P.vz[g], P.vimg[g]);
+      }
+    }
     float pd0 = 0.0f, pd1 = 0.0f, pd2 = 0.0f, pw[6] = {0, 0, 0, 0, 0, 0}, pg = -INFINITY;
     if (n > 0 && c.kk >= 0) {
       const LLPack lp = P.llpack[c.kk];
@@ -1146,45 +1156,56 @@ __global__ __launch_bounds__(256) void k_loglik(DevPtrs P, KOpts O) {
       for (int i = 0; i < 6; i++) cw[j][i] = group_bcast<GW>(pw[i], j);
       cg[j] = (j < K) ? group_bcast<GW>(pg, j) : -INFINITY;
     }
-    double cls = 0.0;  // class sum (lanes 0..15 of the group)
-    for (int base = 0; base < n; base += GW) {
-      const int i = base + lane;
-      double v = 0.0;
-      if (i < n) {
+    // img * log2(sum_j 2^t_j) of one voxel, rounded to FP64
+    auto term = [&](const float4 x) -> double {
 #pragma clang fp contract(off)
-        const long long g = c.beg + i;
-        const float x0 = P.vx[g] - c.a0, x1 = P.vy[g] - c.a1, x2 = P.vz[g] - c.a2;
-        const float w = P.vimg[g];
-        float tj[KM];
-        float mx = -INFINITY;
+      const float x0 = x.x - c.a0, x1 = x.y - c.a1, x2 = x.z - c.a2;
+      float tj[KM];
+      float mx = -INFINITY;
 #pragma unroll
-        for (int j = 0; j < KM; j++) {
-          const float d0 = x0 + cd0[j], d1 = x1 + cd1[j], d2 = x2 + cd2[j];
-          float q = __builtin_fmaf(cw[j][0] * d0, d0, 0.0f);
-          q = __builtin_fmaf(cw[j][3] * d1, d1, q);
-          q = __builtin_fmaf(cw[j][5] * d2, d2, q);
-          float o = cw[j][1] * d0 * d1;
-          o = __builtin_fmaf(cw[j][2] * d0, d2, o);
-          o = __builtin_fmaf(cw[j][4] * d1, d2, o);
-          q = __builtin_fmaf(2.0f, o, q);
-          tj[j] = cg[j] - q;
-          mx = fmaxf(mx, tj[j]);
-        }
-        if (mx > -1074.0f) {
-          float sum = 0.0f;
+      for (int j = 0; j < KM; j++) {
+        const float d0 = x0 + cd0[j], d1 = x1 + cd1[j], d2 = x2 + cd2[j];
+        float q = __builtin_fmaf(cw[j][0] * d0, d0, 0.0f);
+        q = __builtin_fmaf(cw[j][3] * d1, d1, q);
+        q = __builtin_fmaf(cw[j][5] * d2, d2, q);
+        float o = cw[j][1] * d0 * d1;
+        o = __builtin_fmaf(cw[j][2] * d0, d2, o);
+        o = __builtin_fmaf(cw[j][4] * d1, d2, o);
+        q = __builtin_fmaf(2.0f, o, q);
+        tj[j] = cg[j] - q;
+        mx = fmaxf(mx, tj[j]);
+      }
+      if (!(mx > -1074.0f)) return 0.0;
+      float sum = 0.0f;
 #pragma unroll
-          for (int j = 0; j < KM; j++) sum += __builtin_amdgcn_exp2f(tj[j] - mx);
-          v = (double)w * (double)(mx + __builtin_amdgcn_logf(sum));
+      for (int j = 0; j < KM; j++) sum += __builtin_amdgcn_exp2f(tj[j] - mx);
+      return (double)x.w * (double)(mx + __builtin_amdgcn_logf(sum));
+    };
+    double cls = 0.0;  // class sum (lanes 0..15 of the group)
+    for (int base = 0; base < n; base += 64) {
+      if (base > 0) {
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+          const int i = base + lane + GW * u;
+          vv[u] = make_float4(0.f, 0.f, 0.f, 0.f);
+          if (i < n) {
+            const long long g = c.beg + i;
+            vv[u] = make_float4(P.vx[g], P.vy[g], P.vz[g], P.vimg[g]);
+          }
         }
       }
-      // this pass's voxels of class (lane mod 16), in increasing index
-      {
+      double v[U];
+#pragma unroll
+      for (int u = 0; u < U; u++) v[u] = (base + lane + GW * u < n) ? term(vv[u]) : 0.0;
+      // the pass's voxels of class (lane mod 16), in increasing index
+#pragma unroll
+      for (int u = 0; u < U; u++) {
 #pragma clang fp contract(off)
-        cls += v;
-        if (GW >= 32) cls += __shfl_down(v, 16, GW);
+        cls += v[u];
+        if (GW >= 32) cls += __shfl_down(v[u], 16, GW);
         if (GW >= 64) {
-          cls += __shfl_down(v, 32, GW);
-          cls += __shfl_down(v, 48, GW);
+          cls += __shfl_down(v[u], 32, GW);
+          cls += __shfl_down(v[u], 48, GW);
         }
       }
     }
@@ -1194,36 +1215,36 @@ __global__ __launch_bounds__(256) void k_loglik(DevPtrs P, KOpts O) {
     if (lane == 0) svsum[sv] = tot * 0.69314718055994530942;
   }
   __syncthreads();
-  // block subtree, then the levels above it in the 256-supervoxel group, then
-  // the frame
   if (threadIdx.x == 0) {
     double tr[B];
 #pragma unroll
     for (int i = 0; i < B; i++) tr[i] = svsum[i];
     P.part_s[blockIdx.x] = pairwise_tree<B>(tr);
-    __threadfence();
-    const long long g = sb >> 8;
-    const unsigned prev = atomicAdd(P.grp_cnt + g, 1u);
-    int last = 0;
-    if (prev == (unsigned)(BPG - 1)) {
-      __threadfence();
-      P.grp_cnt[g] = 0u;
-      double q[BPG];
-#pragma unroll
-      for (int i = 0; i < BPG; i++) q[i] = load_coherent(P.part_s + g * BPG + i);
-      P.part_sg[g] = pairwise_tree<BPG>(q);
-      __threadfence();
-      const unsigned pf = atomicAdd(P.done + 4 * f + 3, 1u);
-      last = (pf == (unsigned)(F->nbl - 1));
-    }
-    s_last = last;
   }
+}
+
+// Frame totals of the log-likelihood from the block partials of k_loglik (B
+// supervoxels per block), one block per frame: the upper levels of each
+// 256-supervoxel tree, then frame_sum_w64 over the frame's groups; the
+// convergence test (GMEM.cu:4145-4148) and the iteration count.
+template <int BPG>
+__global__ __launch_bounds__(256) void k_ll_final(DevPtrs P) {
+  const int f = blockIdx.x;
+  FrameDev* F = P.fr + f;
+  if (!F->active) return;
+  const long long g0 = F->s0 >> 8;
+  const int ng = F->nbl;
+  for (int g = threadIdx.x; g < ng; g += 256) {
+    double q[BPG];
+#pragma unroll
+    for (int i = 0; i < BPG; i++) q[i] = P.part_s[(g0 + g) * BPG + i];
+    P.part_sg[g0 + g] = pairwise_tree<BPG>(q);
+  }
+  __threadfence_block();
   __syncthreads();
-  if (!s_last || threadIdx.x >= 64) return;
-  __threadfence();
-  const double ll = frame_sum_w64(P.part_sg + (F->s0 >> 8), F->nbl);
+  if (threadIdx.x >= 64) return;
+  const double ll = frame_sum_w64(P.part_sg + g0, ng);
   if (threadIdx.x == 0) {
-    P.done[4 * f + 3] = 0u;
     F->ll3 = F->ll2;
     F->ll2 = F->ll1;
     F->ll1 = F->ll;
@@ -1294,15 +1315,19 @@ static void launch_loglik(const DevPtrs& P, const KOpts& O, hipStream_t st) {
 #define TGMM_LL(GW_, SPG_)                                    \
   if (gw == GW_ && spg == SPG_) {                             \
     k_loglik<KM, GW_, SPG_><<<g, 256, 0, st>>>(P, O);         \
+    k_ll_final<256 / ((256 / GW_) * SPG_)><<<P.nframes, 256, 0, st>>>(P); \
     return;                                                   \
   }
   TGMM_LL(16, 1)
   TGMM_LL(16, 2)
   TGMM_LL(16, 4)
+  TGMM_LL(16, 8)
   TGMM_LL(32, 1)
   TGMM_LL(32, 2)
+  TGMM_LL(32, 4)
   TGMM_LL(64, 1)
   TGMM_LL(64, 2)
+  TGMM_LL(64, 4)
 #undef TGMM_LL
 }
 

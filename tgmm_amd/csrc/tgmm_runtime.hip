@@ -114,7 +114,6 @@ int ws_alloc(Workspace* w, const WsShape& sh, int K) {
   AL(P.part_g, G / kGB);
   AL(P.part_s, S / kLLminB);
   AL(P.part_sg, S / 256);
-  AL(P.grp_cnt, S / 256);
   AL(P.done, 4 * F);
   AL(P.fr, F);
   AL(w->gs_init, GS_NFIELDS * G);
@@ -182,7 +181,7 @@ KOpts make_kopts(const tgmm_em_opts& o, const float* scale_reg) {
 
 bool tgmm::tiles_valid(const Tiles& t) {
   auto in = [](int v, int a, int b, int c) { return v == a || v == b || v == c; };
-  return in(t.ll_gw, 16, 32, 64) && (t.ll_u == 1 || t.ll_u == 2 || (t.ll_u == 4 && t.ll_gw == 16)) &&
+  return in(t.ll_gw, 16, 32, 64) && (in(t.ll_u, 1, 2, 4) || (t.ll_u == 8 && t.ll_gw == 16)) &&
          in(t.ms_lanes, 4, 8, 16) &&
          in(t.sv_block, 64, 128, 256) && in(t.knn_block, 64, 128, 256) &&
          (t.knn_chunk == 4 || t.knn_chunk == 8) && in(t.knn_cell, 2, 4, 6) &&
@@ -354,7 +353,6 @@ int batch_upload(tgmm_batch* b, const std::vector<FrameSrc>& fs) {
   HIPCHK(hipMemsetAsync(P.seg_cnt, 0, (G + 1) * 4, st));
   HIPCHK(hipMemsetAsync(P.cell_cnt, 0, (sh.Ccap + 1) * 4, st));
   HIPCHK(hipMemsetAsync(P.done, 0, 4 * sh.Fcap * 4, st));
-  HIPCHK(hipMemsetAsync(P.grp_cnt, 0, (sh.Scap / 256) * 4, st));
   HIPCHK(launch_prep(P, b->ko, st));
   b->grid_cell = b->ko.tiles.knn_cell;
   HIPCHK(launch_reset(P, b->ko, st));
@@ -694,7 +692,8 @@ extern "C" int tgmm_batch_autotune(tgmm_batch* b, int reps, tgmm_tiles* chosen, 
     int v[18][3];
   };
   const Knob knobs[5] = {
-      {"loglik", TK_LOGLIK, 7, {{16, 1}, {16, 2}, {16, 4}, {32, 1}, {32, 2}, {64, 1}, {64, 2}}},
+      {"loglik", TK_LOGLIK, 10, {{16, 1}, {16, 2}, {16, 4}, {16, 8}, {32, 1}, {32, 2}, {32, 4},
+                                 {64, 1}, {64, 2}, {64, 4}}},
       {"mstep", TK_MSTEP, 3, {{4, 0}, {8, 0}, {16, 0}}},
       {"estep", TK_ESTEP, 3, {{64, 0}, {128, 0}, {256, 0}}},
       {"knn_cell", TK_KNN, 3, {{2, 0}, {4, 0}, {6, 0}}},

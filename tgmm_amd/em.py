@@ -123,7 +123,7 @@ class Tiles(ctypes.Structure):
         return t
 
 
-DEFAULT_TILES = dict(ll_group=16, ll_unroll=1, mstep_lanes=16, sv_block=256, knn_block=256,
+DEFAULT_TILES = dict(ll_group=16, ll_unroll=4, mstep_lanes=16, sv_block=256, knn_block=256,
                      knn_chunk=8, knn_cell=4, knn_lanes=1)
 
 
