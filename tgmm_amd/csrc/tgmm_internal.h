@@ -138,7 +138,6 @@ struct DevPtrs {
   double* part_g;   // Gcap/kGB
   double* part_s;   // Scap/kLLminB: log-likelihood block partials
   double* part_sg;  // Scap/256: partials of 256-supervoxel groups
-  unsigned* done;   // 4*Fcap
   FrameDev* fr;
   long long Scap, Gcap, Ccap, Vcap;
   int Fcap;

@@ -58,39 +58,19 @@ __device__ inline double load_coherent(const double* p) {
   return __longlong_as_double((long long)u);
 }
 
-// Per-frame "last block" reduction: every block of the frame stores its
-// partial, the block that arrives last sums all partials in index order and
-// resets the arrival counter. Returns the frame total in thread 0 of the
-// last block; *is_last tells every thread of the block whether it was last.
-template <int NW>
-__device__ inline double frame_reduce(double v, double* part, long long pidx, long long pbase,
-                                      int nblk, unsigned* done, double* red, int* is_last) {
-  __shared__ int s_last;
-  const double t = block_sum<NW>(v, red);
-  if (threadIdx.x == 0) {
-    part[pidx] = t;
-    __threadfence();
-    const unsigned prev = atomicAdd(done, 1u);
-    s_last = (prev == (unsigned)(nblk - 1));
-  }
-  __syncthreads();
-  *is_last = s_last;
-  if (!s_last) return 0.0;
-  __threadfence();
-  // four independent partial sums per thread keep several loads in flight;
-  // the assignment of partials to (thread, slot) is fixed, so is the order
+// Sum of a frame's per-block partials by one wave, in an order fixed by the
+// partial index alone (lane l takes partials l, l+64, ... into four
+// interleaved sums, then a fixed butterfly). Every lane returns the total.
+__device__ inline double frame_sum_w64(const double* part, int nblk) {
   double a4[4] = {0.0, 0.0, 0.0, 0.0};
-  const int stride = NW * 64;
-  int i = threadIdx.x;
-  for (; i + 3 * stride < nblk; i += 4 * stride) {
+  const int l = threadIdx.x;
+  int i = l;
+  for (; i + 192 < nblk; i += 256) {
 #pragma unroll
-    for (int q = 0; q < 4; q++) a4[q] += load_coherent(part + pbase + i + q * stride);
+    for (int q = 0; q < 4; q++) a4[q] += load_coherent(part + i + 64 * q);
   }
-  for (int q = 0; i < nblk; i += stride, q++) a4[q & 3] += load_coherent(part + pbase + i);
-  const double acc = (a4[0] + a4[1]) + (a4[2] + a4[3]);
-  const double tot = block_sum<NW>(acc, red);
-  if (threadIdx.x == 0) *done = 0u;
-  return tot;
+  for (int q = 0; i < nblk; i += 64, q++) a4[q & 3] += load_coherent(part + i);
+  return wave_sum((a4[0] + a4[1]) + (a4[2] + a4[3]));
 }
 
 __device__ inline double gsv(const DevPtrs& P, int f, long long k) { return P.gs[f * P.Gcap + k]; }
@@ -345,14 +325,8 @@ __global__ __launch_bounds__(kGB) void k_alpha_init(DevPtrs P) {
   if (!F->active) return;
   const long long k = (long long)blockIdx.x * kGB + threadIdx.x;
   const double a = (P.g_frame[k] == f) ? gsv(P, GS_ALPHA, k) : 0.0;
-  int last;
-  const double tot = frame_reduce<1>(a, P.part_g, blockIdx.x, F->g0 / kGB, F->nbg,
-                                     P.done + 4 * f + 0, red, &last);
-  if (last && threadIdx.x == 0) {
-    F->sum_alpha = tot;
-    F->psi_sum_alpha = digamma(tot);
-    F->log_sum_alpha = log(tot);
-  }
+  const double t = block_sum<1>(a, red);
+  if (threadIdx.x == 0) P.part_g[blockIdx.x] = t;
 }
 
 // ---------------------------------------------------------------- scans --
@@ -957,10 +931,8 @@ __global__ __launch_bounds__(kGB) void k_mstep_update(DevPtrs P, KOpts O) {
       alpha_out = gsv(P, GS_ALPHA, k);
     }
   }
-  int last;
-  const double tot = frame_reduce<1>(alpha_out, P.part_g, blockIdx.x, F->g0 / kGB, F->nbg,
-                                     P.done + 4 * f + 1, red, &last);
-  if (last && threadIdx.x == 0) F->sum_alpha2 = tot;
+  const double t = block_sum<1>(alpha_out, red);
+  if (threadIdx.x == 0) P.part_g[blockIdx.x] = t;
 }
 
 // Dead-cell rule (GMEM.cu:1813-1843), then the log-likelihood parameters and
@@ -1016,32 +988,32 @@ __global__ __launch_bounds__(kGB) void k_death(DevPtrs P, KOpts O) {
     P.llpack[k] = lp;
     alpha_out = alpha;
   }
-  int last;
-  const double tot = frame_reduce<1>(alpha_out, P.part_g, blockIdx.x, F->g0 / kGB, F->nbg,
-                                     P.done + 4 * f + 2, red, &last);
-  if (last && threadIdx.x == 0) {
-    F->sum_alpha = tot;
-    F->psi_sum_alpha = digamma(tot);
-    F->log_sum_alpha = log(tot);
+  const double t = block_sum<1>(alpha_out, red);
+  if (threadIdx.x == 0) P.part_g[blockIdx.x] = t;
+}
+
+// A frame's sum of alpha from the Gaussian-block partials of the kernel
+// before, one wave per frame, partials in index order (lane l takes l, l+64,
+// ... into four interleaved sums, then a fixed butterfly). which 0: the sum
+// used by E[ln pi] and the log-likelihood (with its digamma and log);
+// which 1: the sum before the death test.
+__global__ __launch_bounds__(64) void k_alpha_sum(DevPtrs P, int which, int need_active) {
+  const int f = (P.only_frame >= 0) ? P.only_frame : (int)blockIdx.x;
+  if (f >= P.Fcap || (P.only_frame < 0 && f >= P.nframes)) return;
+  FrameDev* F = P.fr + f;
+  if (F->ncap == 0 || (need_active && !F->active)) return;
+  const double tot = frame_sum_w64(P.part_g + F->g0 / kGB, F->nbg);
+  if (threadIdx.x == 0) {
+    if (which == 1) {
+      F->sum_alpha2 = tot;
+    } else {
+      F->sum_alpha = tot;
+      F->psi_sum_alpha = digamma(tot);
+      F->log_sum_alpha = log(tot);
+    }
   }
 }
 
-// Sum of a frame's per-block partials by wave 0 of the frame's last block, in
-// an order fixed by the partial index alone (lane l takes partials l, l+64,
-// ... into four interleaved sums, then a fixed butterfly), so the result does
-// not depend on the block shape of the kernel that produced the partials.
-// Every lane of wave 0 returns the total.
-__device__ inline double frame_sum_w64(const double* part, int nblk) {
-  double a4[4] = {0.0, 0.0, 0.0, 0.0};
-  const int l = threadIdx.x;
-  int i = l;
-  for (; i + 192 < nblk; i += 256) {
-#pragma unroll
-    for (int q = 0; q < 4; q++) a4[q] += load_coherent(part + i + 64 * q);
-  }
-  for (int q = 0; i < nblk; i += 64, q++) a4[q & 3] += load_coherent(part + i);
-  return wave_sum((a4[0] + a4[1]) + (a4[2] + a4[3]));
-}
 
 // Log-likelihood over the voxel stream (GMEM.cu:1913-1978). A block has 256
 // threads in groups of GW lanes; each group takes SPG supervoxels in turn,
@@ -1272,6 +1244,7 @@ hipError_t launch_prep(const DevPtrs& P, const KOpts& O, hipStream_t st) {
 hipError_t launch_reset(const DevPtrs& P, const KOpts& O, hipStream_t st) {
   k_iter_init<<<nb(P.nframes, 64), 64, 0, st>>>(P);
   k_alpha_init<<<nb(P.Gcap, kGB), kGB, 0, st>>>(P);
+  k_alpha_sum<<<P.only_frame >= 0 ? 1 : P.nframes, 64, 0, st>>>(P, 0, 1);
   k_gauss_prep<<<nb(P.Gcap, kGB), kGB, 0, st>>>(P, O);
   return hipGetLastError();
 }
@@ -1375,8 +1348,10 @@ static hipError_t iteration_t(const DevPtrs& P, const KOpts& O, hipStream_t st, 
   if (tk == TK_MSTEP && e0) (void)hipEventRecord(e0, st);
   launch_mstep_stats(P, O, st);
   k_mstep_update<<<gb, kGB, 0, st>>>(P, O);
+  k_alpha_sum<<<P.nframes, 64, 0, st>>>(P, 1, 1);
   if (tk == TK_MSTEP && e1) (void)hipEventRecord(e1, st);
   k_death<<<gb, kGB, 0, st>>>(P, O);
+  k_alpha_sum<<<P.nframes, 64, 0, st>>>(P, 0, 1);
   if (tk == TK_LOGLIK && e0) (void)hipEventRecord(e0, st);
   launch_loglik<KM>(P, O, st);
   if (tk == TK_LOGLIK && e1) (void)hipEventRecord(e1, st);

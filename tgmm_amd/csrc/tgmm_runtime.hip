@@ -114,7 +114,6 @@ int ws_alloc(Workspace* w, const WsShape& sh, int K) {
   AL(P.part_g, G / kGB);
   AL(P.part_s, S / kLLminB);
   AL(P.part_sg, S / 256);
-  AL(P.done, 4 * F);
   AL(P.fr, F);
   AL(w->gs_init, GS_NFIELDS * G);
 #undef AL
@@ -352,7 +351,6 @@ int batch_upload(tgmm_batch* b, const std::vector<FrameSrc>& fs) {
   HIPCHK(hipMemsetAsync(P.rnk, 0, (size_t)b->ws.K * S * 4, st));
   HIPCHK(hipMemsetAsync(P.seg_cnt, 0, (G + 1) * 4, st));
   HIPCHK(hipMemsetAsync(P.cell_cnt, 0, (sh.Ccap + 1) * 4, st));
-  HIPCHK(hipMemsetAsync(P.done, 0, 4 * sh.Fcap * 4, st));
   HIPCHK(launch_prep(P, b->ko, st));
   b->grid_cell = b->ko.tiles.knn_cell;
   HIPCHK(launch_reset(P, b->ko, st));

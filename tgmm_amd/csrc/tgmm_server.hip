@@ -231,7 +231,6 @@ extern "C" int tgmm_server_create(int device, const tgmm_server_opts* opts, tgmm
       (er = hipMemsetAsync(P.rnk, 0, (size_t)e.K * S * 4, s->st)) ||
       (er = hipMemsetAsync(P.seg_cnt, 0, (G + 1) * 4, s->st)) ||
       (er = hipMemsetAsync(P.cell_cnt, 0, (sh.Ccap + 1) * 4, s->st)) ||
-      (er = hipMemsetAsync(P.done, 0, 4 * sh.Fcap * 4, s->st)) ||
       (er = hipMemcpyAsync(P.fr, s->fh.data(), sh.Fcap * sizeof(FrameDev),
                            hipMemcpyHostToDevice, s->st)) ||
       (er = hipStreamSynchronize(s->st)))
