@@ -114,6 +114,8 @@ def main():
                          "closed-loop throughput)")
     ap.add_argument("--hbm-budget-gb", type=float, default=24.0,
                     help="config 4: server workspace budget")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the default launch shapes (for counter runs)")
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
                     help="also run one device-to-device copy of this many MiB (hipMemcpy), "
                          "the known-size reference for HBM traffic counters")
@@ -174,9 +176,12 @@ def main():
 
     # launch shapes: the autotuner times the candidates on this batch (not
     # part of the timed steps; results do not depend on the choice)
-    tiles, tune_report = batch.autotune(reps=3, cache=False)
-    print("# autotuned tiles %s\n%s" % (json.dumps(tiles), tune_report or ""),
-          file=sys.stderr, flush=True)
+    if args.no_autotune:
+        tiles = batch.get_tiles()
+    else:
+        tiles, tune_report = batch.autotune(reps=3, cache=False)
+        print("# autotuned tiles %s\n%s" % (json.dumps(tiles), tune_report or ""),
+              file=sys.stderr, flush=True)
 
     # warmup + timed steps
     for _ in range(args.warmup):
