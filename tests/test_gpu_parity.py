@@ -42,10 +42,8 @@ def test_knn_bit_exact_first_iteration(c1):
     assert np.array_equal(out["ind"], ind_ref)
 
 
-@pytest.mark.parametrize("K", [5, 8])
-def test_knn_bit_exact_dense(c1, K):
-    """Many Gaussians per cell (ties and near ties): 4 copies of each mean
-    with sub-voxel offsets, kNN must equal brute force exactly."""
+def _dense(c1):
+    """4 copies of each mean with quarter-voxel offsets: ties and near ties."""
     f = copy.copy(c1)
     rng = np.random.default_rng(3)
     gms = [c1.gm]
@@ -55,11 +53,36 @@ def test_knn_bit_exact_dense(c1, K):
         g["m_o"] = g["m_k"]
         gms.append(g)
     f.gm = np.concatenate(gms).astype(GMM_DTYPE)
-    out = _run_batch([f], _opts(K=K, iters=1))[0]
+    return f
+
+
+def _brute_knn(f, gm, K):
     ind_ref = np.zeros(K * f.S, np.int32)
-    O.oracle().orc_knn(f.S, O.p(f.centroids, O._fp), f.G, f.gm.ctypes.data,
+    O.oracle().orc_knn(f.S, O.p(f.centroids, O._fp), gm.shape[0], gm.ctypes.data,
                        O.p(np.array([1, 1, 5], np.float32), O._fp), K, O.p(ind_ref, O._ip))
-    assert np.array_equal(out["ind"], ind_ref)
+    return ind_ref
+
+
+@pytest.mark.parametrize("K", [5, 8])
+def test_knn_bit_exact_dense(c1, K):
+    """Many Gaussians per cell (ties and near ties): kNN must equal brute
+    force exactly."""
+    f = _dense(c1)
+    out = _run_batch([f], _opts(K=K, iters=1))[0]
+    assert np.array_equal(out["ind"], _brute_knn(f, f.gm, K))
+
+
+@pytest.mark.parametrize("K", [5, 8])
+def test_knn_bit_exact_with_previous_candidates(c1, K):
+    """From the second iteration on, the previous candidates bound the search
+    and prune grid cells: the candidates of iteration n equal brute force on
+    the means iteration n starts from (the final means of an n-1 iteration
+    run of the same frame)."""
+    f = _dense(c1)
+    for n in (2, 4):
+        prev = _run_batch([f], _opts(K=K, iters=n - 1))[0]
+        cur = _run_batch([f], _opts(K=K, iters=n))[0]
+        assert np.array_equal(cur["ind"], _brute_knn(f, np.ascontiguousarray(prev["gm"]), K))
 
 
 def test_first_iteration_matches_oracle(c1):

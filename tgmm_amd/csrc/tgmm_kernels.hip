@@ -581,20 +581,63 @@ __global__ __launch_bounds__(256) void k_knn(DevPtrs P, KOpts O) {
       }
     }
   };
-  // shells 0 and 1 together: the 9 rows of the 3x3x3 block. All 18 row
-  // bounds are loaded first; the rows' references are then walked as one
-  // flat list, KC loads per lane in flight at a time.
+  // An upper bound of the K-th distance: the previous iteration's candidates
+  // of this supervoxel, if all K are still means of this frame that were
+  // binned this iteration (a mean's distance is a real candidate distance).
+  float kbound = INFINITY;
   {
-    const int xlo = max(c[0] - 1, 0), xhi = min(c[0] + 1, nx - 1);
+    float mx = -INFINITY;
+    bool all = true;
+#pragma unroll
+    for (int j = 0; j < KM; j++) {
+      if (j >= K) continue;
+      const int k = P.ind[(long long)j * S + s];
+      const bool ok = k >= F->g0 && k < F->g0 + F->ng && P.gcell[k] >= 0;
+      all &= ok;
+      if (ok) {
+        const float d = knn_dist(q0, q1, q2, P.mf[k], P.mf[P.Gcap + k], P.mf[2 * P.Gcap + k],
+                                 s0, s1, s2);
+        mx = fmaxf(mx, d);
+      }
+    }
+    if (all) kbound = mx;
+  }
+  // squared distance from the query to the box of cell index ci along axis a
+  // (FP64 from the f32 grid values), shrunk by the same relative margin as
+  // the shell bound; 0 inside. A mean binned into a cell (an edge cell holds
+  // the means beyond the grid on its outer side) is at least that far.
+  auto box_d2 = [&](int a, int ci) -> double {
+    const double lo = org[a] + (double)ci * h, hi = org[a] + (double)(ci + 1) * h;
+    const double d = fmax(fmax(lo - u[a], u[a] - hi), 0.0) * (1.0 - 1e-5);
+    return d * d;
+  };
+  // shells 0 and 1 together: the 9 rows of the 3x3x3 block, each trimmed to
+  // the cells whose box is within the bound (a cell farther than the bound
+  // holds no candidate that can beat the K-th). All row bounds are loaded
+  // first; the rows' references are then walked as one flat list, KC loads
+  // per lane in flight at a time.
+  {
+    double dx2[3], dy2[3], dz2[3];
+#pragma unroll
+    for (int o = 0; o < 3; o++) {
+      dx2[o] = box_d2(0, c[0] + o - 1);
+      dy2[o] = box_d2(1, c[1] + o - 1);
+      dz2[o] = box_d2(2, c[2] + o - 1);
+    }
+    const double thr = (double)kbound;
     int pb[9], cum[10];
     cum[0] = 0;
 #pragma unroll
     for (int t = 0; t < 9; t++) {
       const int z = c[2] + t / 3 - 1, y = c[1] + t % 3 - 1;
-      const bool ok = z >= 0 && z < nz && y >= 0 && y < ny;
+      const double base = dz2[t / 3] + dy2[t % 3];
+      int xa = max(c[0] - 1, 0), xb = min(c[0] + 1, nx - 1);
+      if (xa < c[0] && base + dx2[0] > thr) xa = c[0];
+      if (xb > c[0] && base + dx2[2] > thr) xb = c[0];
+      const bool ok = z >= 0 && z < nz && y >= 0 && y < ny && base + dx2[1] <= thr;
       const long long row = ok ? c0 + ((long long)z * ny + y) * nx : c0;
-      const int b0 = ok ? P.cell_start[row + xlo] : 0;
-      const int e0 = ok ? P.cell_start[row + xhi + 1] : 0;
+      const int b0 = ok ? P.cell_start[row + xa] : 0;
+      const int e0 = ok ? P.cell_start[row + xb + 1] : 0;
       pb[t] = b0;
       cum[t + 1] = cum[t] + (e0 - b0);
     }
@@ -1280,14 +1323,18 @@ hipError_t launch_next_priors(const DevPtrs& P, long long g0, long long ng, doub
   return hipGetLastError();
 }
 
+// ev_after (optional) is recorded between the log-likelihood kernel and the
+// per-frame totals
 template <int KM>
-static void launch_loglik(const DevPtrs& P, const KOpts& O, hipStream_t st) {
+static void launch_loglik(const DevPtrs& P, const KOpts& O, hipStream_t st,
+                          hipEvent_t ev_after) {
   const int gw = O.tiles.ll_gw, spg = O.tiles.ll_u;
   const long long B = (256 / gw) * spg;
   const unsigned g = nb(P.Scap, B);
 #define TGMM_LL(GW_, SPG_)                                    \
   if (gw == GW_ && spg == SPG_) {                             \
     k_loglik<KM, GW_, SPG_><<<g, 256, 0, st>>>(P, O);         \
+    if (ev_after) (void)hipEventRecord(ev_after, st);         \
     k_ll_final<256 / ((256 / GW_) * SPG_)><<<P.nframes, 256, 0, st>>>(P); \
     return;                                                   \
   }
@@ -1353,8 +1400,7 @@ static hipError_t iteration_t(const DevPtrs& P, const KOpts& O, hipStream_t st, 
   k_death<<<gb, kGB, 0, st>>>(P, O);
   k_alpha_sum<<<P.nframes, 64, 0, st>>>(P, 0, 1);
   if (tk == TK_LOGLIK && e0) (void)hipEventRecord(e0, st);
-  launch_loglik<KM>(P, O, st);
-  if (tk == TK_LOGLIK && e1) (void)hipEventRecord(e1, st);
+  launch_loglik<KM>(P, O, st, tk == TK_LOGLIK ? e1 : nullptr);
   return hipGetLastError();
 }
 
