@@ -21,6 +21,7 @@ namespace tgmm {
 
 constexpr int kAlign = 256;      // supervoxel / Gaussian range alignment
 constexpr int kGB = 64;          // threads per block, Gaussian-lane kernels
+constexpr int kCB = 64;          // grid-cell range alignment (cell block -> frame table)
 constexpr int kSB = 256;         // threads per block, supervoxel-lane kernels
 constexpr int kLLminB = 4;       // fewest supervoxels per log-likelihood block
 constexpr int kPairShift = 4;    // pair code = (s << 4) | j, j < 16
@@ -82,9 +83,11 @@ struct Tiles {
   int knn_chunk;  // candidate references loaded together, kNN search: 4, 8
   int knn_cell;   // mean Gaussians per kNN grid cell: 2, 4, 6 (set per frame
                   // when its grid is built)
-  int knn_lanes;  // lanes per query, kNN search: 1, 2, 4
+  int knn_lanes;  // kNN search: 0 = one wave per grid cell (its 3x3x3 block
+                  // staged in LDS for all of the cell's queries); 1, 2, 4 =
+                  // lanes per query
 };
-constexpr Tiles kDefaultTiles = {16, 4, 16, 256, 256, 8, 4, 1};
+constexpr Tiles kDefaultTiles = {16, 4, 16, 256, 256, 8, 4, 0};
 bool tiles_valid(const Tiles& t);
 
 struct KOpts {
@@ -133,6 +136,8 @@ struct DevPtrs {
   // grid
   int* cell_cnt;    // Ccap+1
   int* cell_start;  // Ccap+1
+  int* qcell_start; // Ccap+1: first kNN query of each cell (positions in q4)
+  int* cblk_frame;  // Ccap/kCB: frame of each 64-cell block, -1 if none
   float4* cg4;      // Gcap: cell-sorted references (x, y, z, index bits)
   // reductions
   double* part_g;   // Gcap/kGB

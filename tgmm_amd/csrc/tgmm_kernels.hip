@@ -290,8 +290,7 @@ __global__ __launch_bounds__(256) void k_qcell_scatter(DevPtrs P) {
   if (P.sv_frame[s] != f) return;
   const FrameDev* F = P.fr + f;
   const int c = P.qcell[s];
-  const long long pos = F->s0 + (P.cell_start[c] - P.cell_start[F->c0]) +
-                        atomicSub(P.cell_cnt + c, 1) - 1;
+  const long long pos = P.qcell_start[c] + atomicSub(P.cell_cnt + c, 1) - 1;
   const long long S = P.Scap;
   P.q4[pos] = make_float4(P.sv_cent[s], P.sv_cent[S + s], P.sv_cent[2 * S + s],
                           __int_as_float((int)s));
@@ -358,7 +357,9 @@ __device__ inline int block_exclusive_scan1024(int v, int* lds) {
 //   which 0: the frame's kNN grid cells (off c0, n cells, base g0: the
 //            references of a frame's grid go to its Gaussian range),
 //   which 1: the frame's Gaussians (off g0, n ng, base K*s0: a frame has at
-//            most K*ns significant pairs).
+//            most K*ns significant pairs),
+//   which 2: the frame's grid cells, counts of kNN queries (base s0: the
+//            positions of the frame's queries in cell order).
 // Both ranges hold n+1 entries (the grid has fewer cells than its range, the
 // Gaussian range at least one padding entry), so frames never share an entry.
 // Tiles of 16384 entries, 16 consecutive per thread as four int4 loads (the
@@ -375,11 +376,11 @@ __global__ __launch_bounds__(1024) void k_scan_frames(DevPtrs P, int which, int 
   int* out;
   long long n;
   int base;
-  if (which == 0) {
+  if (which == 0 || which == 2) {
     in = P.cell_cnt + F->c0;
-    out = P.cell_start + F->c0;
+    out = (which == 0 ? P.cell_start : P.qcell_start) + F->c0;
     n = (long long)F->dims[0] * F->dims[1] * F->dims[2];
-    base = (int)F->g0;
+    base = (int)(which == 0 ? F->g0 : F->s0);
   } else {
     in = P.seg_cnt + F->g0;
     out = P.seg_start + F->g0;
@@ -721,6 +722,166 @@ __global__ __launch_bounds__(256) void k_knn(DevPtrs P, KOpts O) {
     }
   }
   if (sub == 0) {
+#pragma unroll
+    for (int j = 0; j < KM; j++)
+      if (j < K) P.ind[(long long)j * S + s] = (bd[j] <= 1e32f) ? bi[j] : -1;
+  }
+}
+
+// Exact K nearest means, one wave per grid cell (the same search as k_knn,
+// organised by cell): every query of the cell has the same 3x3x3 block, so
+// the wave stages the block's references in LDS once (kKnnCap at a time) and
+// each lane, one query, reads them back as broadcasts. A query whose K-th
+// best is not below the block's open-face bound continues with the further
+// shells on its own, from HBM, as in k_knn.
+constexpr int kKnnCap = 512;
+
+template <int KM>
+__global__ __launch_bounds__(64) void k_knn_cell(DevPtrs P, KOpts O) {
+  __shared__ float4 cand[kKnnCap];
+  const long long cc = blockIdx.x;  // global cell index
+  const int lane = threadIdx.x;
+  const int f = P.cblk_frame[cc / kCB];
+  if (f < 0) return;
+  const FrameDev* F = P.fr + f;
+  if (!F->active) return;
+  const int nx = F->dims[0], ny = F->dims[1], nz = F->dims[2];
+  const long long lc = cc - F->c0;
+  if (lc >= (long long)nx * ny * nz) return;
+  const int qs = P.qcell_start[cc], qe = P.qcell_start[cc + 1];
+  if (qs >= qe) return;
+  const long long c0 = F->c0;
+  const int c[3] = {(int)(lc % nx), (int)((lc / nx) % ny), (int)(lc / ((long long)nx * ny))};
+  // the 9 rows of the block: lanes 0..8 read the row starts, 9..17 the ends
+  const int xlo = max(c[0] - 1, 0), xhi = min(c[0] + 1, nx - 1);
+  int rv = 0;
+  if (lane < 18) {
+    const int t = lane % 9;
+    const int z = c[2] + t / 3 - 1, y = c[1] + t % 3 - 1;
+    if (z >= 0 && z < nz && y >= 0 && y < ny)
+      rv = P.cell_start[c0 + ((long long)z * ny + y) * nx + (lane < 9 ? xlo : xhi + 1)];
+  }
+  int pb[9], cum[10];
+  cum[0] = 0;
+#pragma unroll
+  for (int t = 0; t < 9; t++) {
+    const int b0 = __builtin_amdgcn_readlane(rv, t), e0 = __builtin_amdgcn_readlane(rv, t + 9);
+    pb[t] = b0;
+    cum[t + 1] = cum[t] + (e0 - b0);
+  }
+  const int T = cum[9];
+  const long long S = P.Scap;
+  const int K = O.K;
+  const float s0 = O.scale[0], s1 = O.scale[1], s2 = O.scale[2];
+  const double h = F->h;
+  const double org[3] = {F->org[0], F->org[1], F->org[2]};
+  const int dims[3] = {nx, ny, nz};
+  for (int qb = qs; qb < qe; qb += 64) {
+    const int qi = qb + lane;
+    const bool act = qi < qe;
+    const float4 qq = act ? P.q4[qi] : make_float4(0.f, 0.f, 0.f, 0.f);
+    const long long s = __float_as_int(qq.w);
+    const float q0 = qq.x, q1 = qq.y, q2 = qq.z;
+    float bd[KM];
+    int bi[KM];
+#pragma unroll
+    for (int j = 0; j < KM; j++) {
+      bd[j] = INFINITY;
+      bi[j] = 0x7fffffff;
+    }
+    auto consider = [&](const float4 cur) {
+      const float d = knn_dist(q0, q1, q2, cur.x, cur.y, cur.z, s0, s1, s2);
+      const int idx = __float_as_int(cur.w);
+      if ((d <= 1e32f) && lex_less(d, idx, bd[KM - 1], bi[KM - 1])) {
+        float cd = d;
+        int ci2 = idx;
+#pragma unroll
+        for (int j = 0; j < KM; j++) {
+          if (lex_less(cd, ci2, bd[j], bi[j])) {
+            const float td = bd[j];
+            const int ti = bi[j];
+            bd[j] = cd;
+            bi[j] = ci2;
+            cd = td;
+            ci2 = ti;
+          }
+        }
+      }
+    };
+    for (int cb = 0; cb < T; cb += kKnnCap) {
+      const int n = min(kKnnCap, T - cb);
+      __syncthreads();  // the previous chunk has been read
+      for (int x = lane; x < n; x += 64) {
+        const int y = cb + x;
+        int addr = pb[0] + y;
+#pragma unroll
+        for (int t = 1; t < 9; t++)
+          if (y >= cum[t]) addr = pb[t] + (y - cum[t]);
+        cand[x] = P.cg4[addr];
+      }
+      __syncthreads();
+      if (act)
+        for (int x = 0; x < n; x++) consider(cand[x]);
+    }
+    if (!act) continue;
+    // further shells for this query alone (same bound as k_knn)
+    const double u[3] = {(double)q0 * s0, (double)q1 * s1, (double)q2 * s2};
+    auto lower_bound_sq = [&](int r) -> double {
+      double lb = INFINITY;
+      for (int a = 0; a < 3; a++) {
+        if (c[a] - r > 0) lb = fmin(lb, u[a] - (org[a] + (double)(c[a] - r) * h));
+        if (c[a] + r < dims[a] - 1) lb = fmin(lb, (org[a] + (double)(c[a] + r + 1) * h) - u[a]);
+      }
+      if (lb == INFINITY) return INFINITY;
+      lb = fmax(lb, 0.0) * (1.0 - 1e-5);
+      return lb * lb;
+    };
+    auto done_at = [&](int r) -> bool {
+      const double lb2 = lower_bound_sq(r);
+      if (lb2 == INFINITY) return true;
+      float kth = INFINITY;
+#pragma unroll
+      for (int t = 0; t < KM; t++)
+        if (t == K - 1) kth = bd[t];
+      return (double)kth < lb2;
+    };
+    const int rmax = max(nx, max(ny, nz));
+    if (!done_at(1)) {
+      for (int r = 2; r <= rmax; r++) {
+        const int zlo = max(c[2] - r, 0), zhi = min(c[2] + r, nz - 1);
+        const int ylo = max(c[1] - r, 0), yhi = min(c[1] + r, ny - 1);
+        for (int z = zlo; z <= zhi; z++) {
+          const bool zface = (z == c[2] - r) || (z == c[2] + r);
+          for (int y = ylo; y <= yhi; y++) {
+            const bool face = zface || (y == c[1] - r) || (y == c[1] + r);
+            const long long row = c0 + ((long long)z * ny + y) * nx;
+#pragma unroll 1
+            for (int g = 0; g < 2; g++) {
+              int xa, xb;
+              if (face) {
+                xa = g == 0 ? max(c[0] - r, 0) : 1;
+                xb = g == 0 ? min(c[0] + r, nx - 1) : 0;
+              } else {
+                xa = xb = g == 0 ? c[0] - r : c[0] + r;
+                if (xa < 0 || xa >= nx) xb = xa - 1;
+              }
+              if (xa > xb) continue;
+              const int rb = P.cell_start[row + xa], re = P.cell_start[row + xb + 1];
+              for (int p = rb; p < re; p += 4) {
+                float4 v[4];
+#pragma unroll
+                for (int w = 0; w < 4; w++)
+                  v[w] = (p + w < re) ? P.cg4[p + w] : make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+                for (int w = 0; w < 4; w++)
+                  if (p + w < re) consider(v[w]);
+              }
+            }
+          }
+        }
+        if (done_at(r)) break;
+      }
+    }
 #pragma unroll
     for (int j = 0; j < KM; j++)
       if (j < K) P.ind[(long long)j * S + s] = (bd[j] <= 1e32f) ? bi[j] : -1;
@@ -1279,7 +1440,7 @@ hipError_t launch_prep(const DevPtrs& P, const KOpts& O, hipStream_t st) {
   k_moments<<<nb(P.Scap, 4), 256, 0, st>>>(P);
   k_grid_setup<<<P.only_frame >= 0 ? 1 : P.nframes, 1024, 0, st>>>(P, O);
   k_qcell_count<<<nb(P.Scap, 256), 256, 0, st>>>(P, O);
-  k_scan_frames<<<P.only_frame >= 0 ? 1 : P.nframes, 1024, 0, st>>>(P, 0, 0, O.K);
+  k_scan_frames<<<P.only_frame >= 0 ? 1 : P.nframes, 1024, 0, st>>>(P, 2, 0, O.K);
   k_qcell_scatter<<<nb(P.Scap, 256), 256, 0, st>>>(P);
   return hipGetLastError();
 }
@@ -1363,6 +1524,10 @@ static void launch_mstep_stats(const DevPtrs& P, const KOpts& O, hipStream_t st)
 
 template <int KM>
 static void launch_knn(const DevPtrs& P, const KOpts& O, hipStream_t st) {
+  if (O.tiles.knn_lanes == 0) {
+    k_knn_cell<KM><<<(unsigned)P.Ccap, 64, 0, st>>>(P, O);
+    return;
+  }
   const int bk = O.tiles.knn_block, ql = O.tiles.knn_lanes;
   const unsigned g = nb(P.Scap, bk / ql);
   if (O.tiles.knn_chunk == 4) {

@@ -110,6 +110,8 @@ int ws_alloc(Workspace* w, const WsShape& sh, int K) {
   AL(P.mstat, 10 * G);
   AL(P.cell_cnt, C + 1);
   AL(P.cell_start, C + 1);
+  AL(P.qcell_start, C + 1);
+  AL(P.cblk_frame, C / kCB + 1);
   AL(P.cg4, G);
   AL(P.part_g, G / kGB);
   AL(P.part_s, S / kLLminB);
@@ -184,7 +186,7 @@ bool tgmm::tiles_valid(const Tiles& t) {
          in(t.ms_lanes, 4, 8, 16) &&
          in(t.sv_block, 64, 128, 256) && in(t.knn_block, 64, 128, 256) &&
          (t.knn_chunk == 4 || t.knn_chunk == 8) && in(t.knn_cell, 2, 4, 6) &&
-         in(t.knn_lanes, 1, 2, 4);
+         (t.knn_lanes == 0 || in(t.knn_lanes, 1, 2, 4));
 }
 
 // records (AoS, host) -> 27 x G SoA columns at [g0, g0+G) of a Gcap table
@@ -235,8 +237,8 @@ int batch_build(int device, const std::vector<FrameSrc>& fs, const tgmm_em_opts&
     F.ng = f.G;
     F.c0 = c;
     // grid-cell range: more entries than cells (the per-frame scan writes one
-    // past the last cell), a multiple of 4 (vector loads)
-    F.ncap = round_up(std::max<long long>(8, 2LL * f.G), 4);
+    // past the last cell), a multiple of 64 (cell block -> frame table)
+    F.ncap = round_up(std::max<long long>(8, 2LL * f.G), kCB);
     // every block mapped to the frame's 256-aligned ranges takes part in the
     // per-frame reductions, including trailing blocks with no valid element;
     // the Gaussian range has at least one padding entry (segment end)
@@ -252,7 +254,7 @@ int batch_build(int device, const std::vector<FrameSrc>& fs, const tgmm_em_opts&
   sh.Vcap = std::max<long long>(v, 1);
   sh.Scap = std::max<long long>(s, kAlign);
   sh.Gcap = std::max<long long>(g, kAlign);
-  sh.Ccap = std::max<long long>(c, 8);
+  sh.Ccap = std::max<long long>(c, kCB);
   sh.Fcap = std::max<int>((int)fs.size(), 1);
   if ((sh.Scap << kPairShift) >= (1LL << 31) || (long long)o.K * sh.Scap >= (1LL << 31) ||
       sh.Gcap >= (1LL << 31) || sh.Ccap >= (1LL << 31)) {
@@ -296,7 +298,7 @@ int batch_upload(tgmm_batch* b, const std::vector<FrameSrc>& fs) {
   const long long S = sh.Scap, G = sh.Gcap;
   std::vector<int> sv_frame(S, -1), sv_len(S, 0), g_frame(G, -1);
   std::vector<long long> sv_beg(S, 0);
-  std::vector<int> sblk(S / kAlign, -1), gblk(G / kGB, -1);
+  std::vector<int> sblk(S / kAlign, -1), gblk(G / kGB, -1), cblk(b->ws.sh.Ccap / kCB + 1, -1);
   std::vector<double> gs((size_t)GS_NFIELDS * G, 0.0);
   std::vector<unsigned char> fixed(G, 0);
   for (size_t i = 0; i < fs.size(); i++) {
@@ -319,6 +321,7 @@ int batch_upload(tgmm_batch* b, const std::vector<FrameSrc>& fs) {
     for (long long j = 0; j < round_up(std::max<long long>(f.S, 1), kAlign) / kAlign; j++)
       sblk[F.s0 / kAlign + j] = (int)i;
     for (long long j = 0; j < F.nbg; j++) gblk[F.g0 / kGB + j] = (int)i;
+    for (long long j = 0; j < F.ncap / kCB; j++) cblk[F.c0 / kCB + j] = (int)i;
     for (long long k = 0; k < f.G; k++) g_frame[F.g0 + k] = (int)i;
     if (f.G > 0) records_to_soa(f.gm, f.G, F.g0, G, gs.data(), fixed.data());
   }
@@ -340,6 +343,7 @@ int batch_upload(tgmm_batch* b, const std::vector<FrameSrc>& fs) {
   HIPCHK(hipMemcpyAsync(P.sv_beg, sv_beg.data(), S * 8, hipMemcpyHostToDevice, st));
   HIPCHK(hipMemcpyAsync(P.sv_len, sv_len.data(), S * 4, hipMemcpyHostToDevice, st));
   HIPCHK(hipMemcpyAsync(P.sblk_frame, sblk.data(), sblk.size() * 4, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(P.cblk_frame, cblk.data(), cblk.size() * 4, hipMemcpyHostToDevice, st));
   HIPCHK(hipMemcpyAsync(P.g_frame, g_frame.data(), G * 4, hipMemcpyHostToDevice, st));
   HIPCHK(hipMemcpyAsync(P.gblk_frame, gblk.data(), gblk.size() * 4, hipMemcpyHostToDevice, st));
   HIPCHK(hipMemcpyAsync(P.gs, gs.data(), gs.size() * 8, hipMemcpyHostToDevice, st));
@@ -687,7 +691,7 @@ extern "C" int tgmm_batch_autotune(tgmm_batch* b, int reps, tgmm_tiles* chosen, 
     const char* name;
     int tk;
     int n;
-    int v[18][3];
+    int v[13][3];
   };
   const Knob knobs[5] = {
       {"loglik", TK_LOGLIK, 10, {{16, 1}, {16, 2}, {16, 4}, {16, 8}, {32, 1}, {32, 2}, {32, 4},
@@ -695,10 +699,9 @@ extern "C" int tgmm_batch_autotune(tgmm_batch* b, int reps, tgmm_tiles* chosen, 
       {"mstep", TK_MSTEP, 3, {{4, 0}, {8, 0}, {16, 0}}},
       {"estep", TK_ESTEP, 3, {{64, 0}, {128, 0}, {256, 0}}},
       {"knn_cell", TK_KNN, 3, {{2, 0}, {4, 0}, {6, 0}}},
-      {"knn", TK_KNN, 18, {{64, 4, 1},  {128, 4, 1}, {256, 4, 1}, {64, 8, 1},  {128, 8, 1},
-                           {256, 8, 1}, {64, 4, 2},  {128, 4, 2}, {256, 4, 2}, {64, 8, 2},
-                           {128, 8, 2}, {256, 8, 2}, {64, 4, 4},  {128, 4, 4}, {256, 4, 4},
-                           {64, 8, 4},  {128, 8, 4}, {256, 8, 4}}},
+      {"knn", TK_KNN, 13, {{64, 8, 0},  {64, 4, 1},  {256, 4, 1}, {64, 8, 1},  {256, 8, 1},
+                           {64, 4, 2},  {256, 4, 2}, {64, 8, 2},  {256, 8, 2}, {64, 4, 4},
+                           {256, 4, 4}, {64, 8, 4},  {256, 8, 4}}},
   };
   int rc = TGMM_OK;
   for (int q = 0; q < 5 && rc == TGMM_OK; q++) {
@@ -720,7 +723,9 @@ extern "C" int tgmm_batch_autotune(tgmm_batch* b, int reps, tgmm_tiles* chosen, 
       }
       double ms = 0.0;
       if ((rc = time_shape(b, t, kb.tk, iters, e0, e1, &ms)) != TGMM_OK) break;
-      if (kb.v[c][2])
+      if (q == 4 && kb.v[c][2] == 0)
+        snprintf(line, sizeof(line), "%s cell %.2f\n", kb.name, 1000.0 * ms);
+      else if (kb.v[c][2])
         snprintf(line, sizeof(line), "%s %dx%dx%d %.2f\n", kb.name, kb.v[c][0], kb.v[c][1],
                  kb.v[c][2], 1000.0 * ms);
       else if (kb.v[c][1])

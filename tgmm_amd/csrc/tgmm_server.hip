@@ -187,14 +187,14 @@ extern "C" int tgmm_server_create(int device, const tgmm_server_opts* opts, tgmm
     t.Vcap = V2;
     t.Scap = rup(V2 / 8 + 256LL * t.Fcap, kAlign);
     t.Gcap = rup(V2 / 32 + 256LL * t.Fcap, kAlign);
-    t.Ccap = 2 * t.Gcap + 8LL * t.Fcap;
+    t.Ccap = rup(2 * t.Gcap + 2LL * kCB * t.Fcap, kCB);
     if (ws_bytes_estimate(t, e.K) > opts->hbm_budget_bytes || V2 > (1LL << 31)) break;
     V = V2;
   }
   sh.Vcap = V;
   sh.Scap = rup(V / 8 + 256LL * sh.Fcap, kAlign);
   sh.Gcap = rup(V / 32 + 256LL * sh.Fcap, kAlign);
-  sh.Ccap = 2 * sh.Gcap + 8LL * sh.Fcap;
+  sh.Ccap = rup(2 * sh.Gcap + 2LL * kCB * sh.Fcap, kCB);
   if (int rc = ws_alloc(&s->ws, sh, e.K)) {
     ws_free(&s->ws);
     delete s;
@@ -225,6 +225,7 @@ extern "C" int tgmm_server_create(int device, const tgmm_server_opts* opts, tgmm
   hipError_t er;
   if ((er = hipMemsetAsync(P.sv_frame, 0xFF, S * 4, s->st)) ||
       (er = hipMemsetAsync(P.sblk_frame, 0xFF, (S / kAlign) * 4, s->st)) ||
+      (er = hipMemsetAsync(P.cblk_frame, 0xFF, (sh.Ccap / kCB + 1) * 4, s->st)) ||
       (er = hipMemsetAsync(P.g_frame, 0xFF, G * 4, s->st)) ||
       (er = hipMemsetAsync(P.gblk_frame, 0xFF, (G / kGB) * 4, s->st)) ||
       (er = hipMemsetAsync(P.ind, 0xFF, (size_t)e.K * S * 4, s->st)) ||
@@ -293,7 +294,8 @@ extern "C" int tgmm_server_submit(tgmm_server* s, int stream_id, const tgmm_fram
 static int map_ranges(tgmm_server* s, const Ticket& t, int f) {
   DevPtrs& P = s->ws.P;
   hipStream_t st = s->st;
-  std::vector<int> sv(t.sa, f), sb(t.sa / kAlign, f), gv(t.ga, f), gb(t.ga / kGB, f);
+  std::vector<int> sv(t.sa, f), sb(t.sa / kAlign, f), gv(t.ga, f), gb(t.ga / kGB, f),
+      cb(t.ncap / kCB, f);
   if (f >= 0) {
     for (long long j = t.S; j < t.sa; j++) sv[j] = -1;
     for (long long j = t.G; j < t.ga; j++) gv[j] = -1;
@@ -303,6 +305,8 @@ static int map_ranges(tgmm_server* s, const Ticket& t, int f) {
                         hipMemcpyHostToDevice, st));
   HIPCHK(hipMemcpyAsync(P.g_frame + t.g0, gv.data(), t.ga * 4, hipMemcpyHostToDevice, st));
   HIPCHK(hipMemcpyAsync(P.gblk_frame + t.g0 / kGB, gb.data(), gb.size() * 4,
+                        hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(P.cblk_frame + t.c0 / kCB, cb.data(), cb.size() * 4,
                         hipMemcpyHostToDevice, st));
   HIPCHK(hipStreamSynchronize(st));  // host vectors go out of scope
   return TGMM_OK;
@@ -407,7 +411,7 @@ static int try_admit(tgmm_server* s) {
     if (f < 0) break;
     t.sa = rup(std::max<long long>(t.S, 1), kAlign);
     t.ga = rup(t.G + 1, kAlign);  // at least one padding entry (segment end)
-    t.ncap = rup(std::max<long long>(8, 2 * t.G), 4);
+    t.ncap = rup(std::max<long long>(8, 2 * t.G), kCB);
     const bool reuse_g = t.gm_in.empty();  // priors from the resident range
     long long v0, s0, g0 = 0, c0;
     if (!s->av.alloc(t.N, 1, &v0)) {
@@ -419,7 +423,7 @@ static int try_admit(tgmm_server* s) {
       ++it;
       continue;
     }
-    if (!s->ac.alloc(t.ncap, 4, &c0)) {
+    if (!s->ac.alloc(t.ncap, kCB, &c0)) {
       s->av.release(v0, t.N);
       s->as.release(s0, t.sa);
       ++it;
