@@ -183,7 +183,7 @@ int tgmm_batch_solve(tgmm_batch* b, int mode);
 int tgmm_batch_fetch(tgmm_batch* b, tgmm_frame_desc* frames, int nframes);
 /* Launch shapes of the tunable kernels. Every valid choice gives bitwise the
  * same results (the kernels fix their summation order independently of the
- * shape); only the speed changes. Defaults: {16, 4, 16, 256, 256, 8, 4, 0}. */
+ * shape); only the speed changes. Defaults: {16, 4, 16, 256, 256, 8, 6, 2}. */
 typedef struct tgmm_tiles {
   int ll_group;    /* lanes per supervoxel, log-likelihood: 16, 32 or 64 */
   int ll_unroll;   /* supervoxels per lane group in turn, log-likelihood:
@@ -193,9 +193,9 @@ typedef struct tgmm_tiles {
   int knn_block;   /* threads per block, kNN search: 64, 128 or 256 */
   int knn_chunk;   /* candidate references loaded together, kNN search: 4 or 8 */
   int knn_cell;    /* mean Gaussians per kNN grid cell: 2, 4 or 6 */
-  int knn_lanes;   /* kNN search: 0 = one wave per grid cell, its 3x3x3 block
-                      staged in LDS for all of the cell's queries; 1, 2 or 4 =
-                      lanes per query */
+  int knn_lanes;   /* kNN search: 0 = one block of knn_block/64 waves per grid
+                      cell, its 3x3x3 block staged in LDS for all of the
+                      cell's queries; 1, 2 or 4 = lanes per query */
 } tgmm_tiles;
 
 /* A change of knn_cell rebuilds the kNN grids and resets the batch to its

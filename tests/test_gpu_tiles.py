@@ -11,7 +11,7 @@ from tgmm_amd import em, synth
 pytestmark = pytest.mark.gpu
 
 SHAPES = [
-    dict(ll_group=16, ll_unroll=4, mstep_lanes=16, sv_block=256, knn_block=256, knn_chunk=4,
+    dict(ll_group=16, ll_unroll=4, mstep_lanes=16, sv_block=256, knn_block=128, knn_chunk=4,
          knn_cell=2, knn_lanes=0),
     dict(ll_group=32, ll_unroll=2, mstep_lanes=8, sv_block=128, knn_block=64, knn_chunk=8,
          knn_cell=6, knn_lanes=2),
@@ -19,7 +19,7 @@ SHAPES = [
          knn_cell=4, knn_lanes=4),
     dict(ll_group=16, ll_unroll=8, mstep_lanes=16, sv_block=128, knn_block=256, knn_chunk=8,
          knn_cell=2, knn_lanes=2),
-    dict(ll_group=64, ll_unroll=2, mstep_lanes=4, sv_block=256, knn_block=64, knn_chunk=4,
+    dict(ll_group=64, ll_unroll=2, mstep_lanes=4, sv_block=256, knn_block=256, knn_chunk=4,
          knn_cell=6, knn_lanes=0),
 ]
 
@@ -77,7 +77,7 @@ def test_autotune_choice_is_valid_and_changes_nothing(frames):
         tiles, report = b.autotune(reps=2, cache=False)
         print(tiles, "\n", report, flush=True)
         assert b.get_tiles() == tiles
-        assert len(report.strip().splitlines()) == 32
+        assert len(report.strip().splitlines()) == 34
         b.solve(em.SOLVE_GRAPH)
         out = b.fetch()
     _same(ref, out)

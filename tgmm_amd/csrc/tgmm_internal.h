@@ -83,11 +83,11 @@ struct Tiles {
   int knn_chunk;  // candidate references loaded together, kNN search: 4, 8
   int knn_cell;   // mean Gaussians per kNN grid cell: 2, 4, 6 (set per frame
                   // when its grid is built)
-  int knn_lanes;  // kNN search: 0 = one wave per grid cell (its 3x3x3 block
-                  // staged in LDS for all of the cell's queries); 1, 2, 4 =
-                  // lanes per query
+  int knn_lanes;  // kNN search: 0 = one block of knn_block/64 waves per grid
+                  // cell (its 3x3x3 block staged in LDS for all of the cell's
+                  // queries); 1, 2, 4 = lanes per query
 };
-constexpr Tiles kDefaultTiles = {16, 4, 16, 256, 256, 8, 4, 0};
+constexpr Tiles kDefaultTiles = {16, 4, 16, 256, 256, 8, 6, 2};
 bool tiles_valid(const Tiles& t);
 
 struct KOpts {

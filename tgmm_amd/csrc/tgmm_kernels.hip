@@ -728,19 +728,39 @@ __global__ __launch_bounds__(256) void k_knn(DevPtrs P, KOpts O) {
   }
 }
 
-// Exact K nearest means, one wave per grid cell (the same search as k_knn,
+// Exact K nearest means, one block per grid cell (the same search as k_knn,
 // organised by cell): every query of the cell has the same 3x3x3 block, so
-// the wave stages the block's references in LDS once (kKnnCap at a time) and
-// each lane, one query, reads them back as broadcasts. A query whose K-th
-// best is not below the block's open-face bound continues with the further
-// shells on its own, from HBM, as in k_knn.
+// the block stages the block's references in LDS once (kKnnCap at a time).
+// Lane l of each of the NW waves takes query l of the cell; wave w scans the
+// references w, w+NW, ... into its own sorted list (LDS reads are
+// broadcasts), the NW lists of a query are merged by wave 0, and a query
+// whose K-th best is not below the block's open-face bound continues with the
+// further shells on its own, from HBM, as in k_knn.
 constexpr int kKnnCap = 512;
 
 template <int KM>
-__global__ __launch_bounds__(64) void k_knn_cell(DevPtrs P, KOpts O) {
+__device__ inline void knn_insert(float (&bd)[KM], int (&bi)[KM], float d, int idx) {
+  if (!lex_less(d, idx, bd[KM - 1], bi[KM - 1])) return;
+#pragma unroll
+  for (int j = 0; j < KM; j++) {
+    if (lex_less(d, idx, bd[j], bi[j])) {
+      const float td = bd[j];
+      const int ti = bi[j];
+      bd[j] = d;
+      bi[j] = idx;
+      d = td;
+      idx = ti;
+    }
+  }
+}
+
+template <int KM, int NW>
+__global__ __launch_bounds__(64 * NW) void k_knn_cell(DevPtrs P, KOpts O) {
   __shared__ float4 cand[kKnnCap];
+  __shared__ float ld[NW > 1 ? NW - 1 : 1][KM][64];
+  __shared__ int li[NW > 1 ? NW - 1 : 1][KM][64];
   const long long cc = blockIdx.x;  // global cell index
-  const int lane = threadIdx.x;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int f = P.cblk_frame[cc / kCB];
   if (f < 0) return;
   const FrameDev* F = P.fr + f;
@@ -791,27 +811,12 @@ __global__ __launch_bounds__(64) void k_knn_cell(DevPtrs P, KOpts O) {
     }
     auto consider = [&](const float4 cur) {
       const float d = knn_dist(q0, q1, q2, cur.x, cur.y, cur.z, s0, s1, s2);
-      const int idx = __float_as_int(cur.w);
-      if ((d <= 1e32f) && lex_less(d, idx, bd[KM - 1], bi[KM - 1])) {
-        float cd = d;
-        int ci2 = idx;
-#pragma unroll
-        for (int j = 0; j < KM; j++) {
-          if (lex_less(cd, ci2, bd[j], bi[j])) {
-            const float td = bd[j];
-            const int ti = bi[j];
-            bd[j] = cd;
-            bi[j] = ci2;
-            cd = td;
-            ci2 = ti;
-          }
-        }
-      }
+      if (d <= 1e32f) knn_insert<KM>(bd, bi, d, __float_as_int(cur.w));
     };
     for (int cb = 0; cb < T; cb += kKnnCap) {
       const int n = min(kKnnCap, T - cb);
       __syncthreads();  // the previous chunk has been read
-      for (int x = lane; x < n; x += 64) {
+      for (int x = threadIdx.x; x < n; x += 64 * NW) {
         const int y = cb + x;
         int addr = pb[0] + y;
 #pragma unroll
@@ -820,8 +825,37 @@ __global__ __launch_bounds__(64) void k_knn_cell(DevPtrs P, KOpts O) {
         cand[x] = P.cg4[addr];
       }
       __syncthreads();
-      if (act)
-        for (int x = 0; x < n; x++) consider(cand[x]);
+      if (act) {
+        int x = w;
+        for (; x + 3 * NW < n; x += 4 * NW) {
+          const float4 a0 = cand[x], a1 = cand[x + NW], a2 = cand[x + 2 * NW],
+                       a3 = cand[x + 3 * NW];
+          consider(a0);
+          consider(a1);
+          consider(a2);
+          consider(a3);
+        }
+        for (; x < n; x += NW) consider(cand[x]);
+      }
+    }
+    if (NW > 1) {
+      __syncthreads();
+      if (w > 0) {
+#pragma unroll
+        for (int j = 0; j < KM; j++) {
+          ld[w - 1][j][lane] = bd[j];
+          li[w - 1][j][lane] = bi[j];
+        }
+      }
+      __syncthreads();
+      if (w > 0) continue;  // wave 0 finishes the cell's queries
+#pragma unroll
+      for (int v = 0; v < NW - 1; v++)
+#pragma unroll
+        for (int j = 0; j < KM; j++) {
+          const float d = ld[v][j][lane];
+          if (d <= 1e32f) knn_insert<KM>(bd, bi, d, li[v][j][lane]);
+        }
     }
     if (!act) continue;
     // further shells for this query alone (same bound as k_knn)
@@ -870,11 +904,11 @@ __global__ __launch_bounds__(64) void k_knn_cell(DevPtrs P, KOpts O) {
               for (int p = rb; p < re; p += 4) {
                 float4 v[4];
 #pragma unroll
-                for (int w = 0; w < 4; w++)
-                  v[w] = (p + w < re) ? P.cg4[p + w] : make_float4(0.f, 0.f, 0.f, 0.f);
+                for (int k4 = 0; k4 < 4; k4++)
+                  v[k4] = (p + k4 < re) ? P.cg4[p + k4] : make_float4(0.f, 0.f, 0.f, 0.f);
 #pragma unroll
-                for (int w = 0; w < 4; w++)
-                  if (p + w < re) consider(v[w]);
+                for (int k4 = 0; k4 < 4; k4++)
+                  if (p + k4 < re) consider(v[k4]);
               }
             }
           }
@@ -1524,8 +1558,14 @@ static void launch_mstep_stats(const DevPtrs& P, const KOpts& O, hipStream_t st)
 
 template <int KM>
 static void launch_knn(const DevPtrs& P, const KOpts& O, hipStream_t st) {
-  if (O.tiles.knn_lanes == 0) {
-    k_knn_cell<KM><<<(unsigned)P.Ccap, 64, 0, st>>>(P, O);
+  if (O.tiles.knn_lanes == 0) {  // one block of knn_block/64 waves per grid cell
+    const unsigned g = (unsigned)P.Ccap;
+    if (O.tiles.knn_block == 64)
+      k_knn_cell<KM, 1><<<g, 64, 0, st>>>(P, O);
+    else if (O.tiles.knn_block == 128)
+      k_knn_cell<KM, 2><<<g, 128, 0, st>>>(P, O);
+    else
+      k_knn_cell<KM, 4><<<g, 256, 0, st>>>(P, O);
     return;
   }
   const int bk = O.tiles.knn_block, ql = O.tiles.knn_lanes;

@@ -691,7 +691,7 @@ extern "C" int tgmm_batch_autotune(tgmm_batch* b, int reps, tgmm_tiles* chosen, 
     const char* name;
     int tk;
     int n;
-    int v[13][3];
+    int v[15][3];
   };
   const Knob knobs[5] = {
       {"loglik", TK_LOGLIK, 10, {{16, 1}, {16, 2}, {16, 4}, {16, 8}, {32, 1}, {32, 2}, {32, 4},
@@ -699,9 +699,9 @@ extern "C" int tgmm_batch_autotune(tgmm_batch* b, int reps, tgmm_tiles* chosen, 
       {"mstep", TK_MSTEP, 3, {{4, 0}, {8, 0}, {16, 0}}},
       {"estep", TK_ESTEP, 3, {{64, 0}, {128, 0}, {256, 0}}},
       {"knn_cell", TK_KNN, 3, {{2, 0}, {4, 0}, {6, 0}}},
-      {"knn", TK_KNN, 13, {{64, 8, 0},  {64, 4, 1},  {256, 4, 1}, {64, 8, 1},  {256, 8, 1},
-                           {64, 4, 2},  {256, 4, 2}, {64, 8, 2},  {256, 8, 2}, {64, 4, 4},
-                           {256, 4, 4}, {64, 8, 4},  {256, 8, 4}}},
+      {"knn", TK_KNN, 15, {{64, 8, 0},  {128, 8, 0}, {256, 8, 0}, {64, 4, 1},  {256, 4, 1},
+                           {64, 8, 1},  {256, 8, 1}, {64, 4, 2},  {256, 4, 2}, {64, 8, 2},
+                           {256, 8, 2}, {64, 4, 4},  {256, 4, 4}, {64, 8, 4},  {256, 8, 4}}},
   };
   int rc = TGMM_OK;
   for (int q = 0; q < 5 && rc == TGMM_OK; q++) {
@@ -724,7 +724,7 @@ extern "C" int tgmm_batch_autotune(tgmm_batch* b, int reps, tgmm_tiles* chosen, 
       double ms = 0.0;
       if ((rc = time_shape(b, t, kb.tk, iters, e0, e1, &ms)) != TGMM_OK) break;
       if (q == 4 && kb.v[c][2] == 0)
-        snprintf(line, sizeof(line), "%s cell %.2f\n", kb.name, 1000.0 * ms);
+        snprintf(line, sizeof(line), "%s cell%d %.2f\n", kb.name, kb.v[c][0] / 64, 1000.0 * ms);
       else if (kb.v[c][2])
         snprintf(line, sizeof(line), "%s %dx%dx%d %.2f\n", kb.name, kb.v[c][0], kb.v[c][1],
                  kb.v[c][2], 1000.0 * ms);

@@ -124,7 +124,7 @@ class Tiles(ctypes.Structure):
 
 
 DEFAULT_TILES = dict(ll_group=16, ll_unroll=4, mstep_lanes=16, sv_block=256, knn_block=256,
-                     knn_chunk=8, knn_cell=4, knn_lanes=0)
+                     knn_chunk=8, knn_cell=6, knn_lanes=2)
 
 
 
