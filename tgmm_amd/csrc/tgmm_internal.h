@@ -59,7 +59,8 @@ struct FrameDev {
 struct alignas(64) LLPack {
   double m[3];  // mean
   double lg2;   // (log alpha + log det(nu W)/2 - 1.5 log 2pi) * log2(e)
-  float M[6];   // nu W * log2(e) / 2, packed symmetric
+  float U[6];   // upper Cholesky factor of nu W * log2(e) / 2:
+                // u00 u01 u02 u11 u12 u22, so d^T (nu W log2(e)/2) d = |U d|^2
   float pad[2];
 };
 struct alignas(32) EPack {

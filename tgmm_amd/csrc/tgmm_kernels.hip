@@ -1217,10 +1217,25 @@ __global__ __launch_bounds__(kGB) void k_death(DevPtrs P, KOpts O) {
     gauss_prep_one(P, O, F, k, alive, m, W, nu, gsv(P, GS_BETA, k), alpha);
     LLPack lp;
     for (int i = 0; i < 3; i++) lp.m[i] = m[i];
-    for (int i = 0; i < 6; i++) lp.M[i] = (float)(nu * W[i] * (0.5 * LOG2E));
+    // Cholesky factor (FP64) of M = nu W log2(e)/2; a precision matrix that
+    // is not positive definite has no density, its terms are dropped
+    double M[6];
+    for (int i = 0; i < 6; i++) M[i] = nu * W[i] * (0.5 * LOG2E);
+    const double u00 = sqrt(M[0]);
+    const double u01 = M[1] / u00, u02 = M[2] / u00;
+    const double u11 = sqrt(M[3] - u01 * u01);
+    const double u12 = (M[4] - u01 * u02) / u11;
+    const double u22 = sqrt(M[5] - u02 * u02 - u12 * u12);
+    const bool pd = u00 > 0.0 && u11 > 0.0 && u22 > 0.0;  // false for NaN too
+    lp.U[0] = pd ? (float)u00 : 0.0f;
+    lp.U[1] = pd ? (float)u01 : 0.0f;
+    lp.U[2] = pd ? (float)u02 : 0.0f;
+    lp.U[3] = pd ? (float)u11 : 0.0f;
+    lp.U[4] = pd ? (float)u12 : 0.0f;
+    lp.U[5] = pd ? (float)u22 : 0.0f;
     lp.pad[0] = lp.pad[1] = 0.0f;
     const double dt = det3(W) * nu * nu * nu;
-    lp.lg2 = (alive && alpha > 0.0 && dt > 0.0)
+    lp.lg2 = (alive && alpha > 0.0 && dt > 0.0 && pd)
                  ? (log(alpha) + 0.5 * log(dt) - 1.5 * 1.8378770664093454836) * LOG2E
                  : -INFINITY;
     P.llpack[k] = lp;
@@ -1260,12 +1275,14 @@ __global__ __launch_bounds__(64) void k_alpha_sum(DevPtrs P, int which, int need
 // is evaluated; a block covers B = (256/GW)*SPG consecutive supervoxels of
 // one frame. Lane j < K of a group reads candidate j's parameter pack; the
 // constants are broadcast to the group (into scalars for a whole-wave group).
-// Per voxel, d = (x - a_s) + (a_s - m_k) with the anchor a_s a voxel of the
-// supervoxel (exact in f32) and (a_s - m_k) rounded once from FP64, so the
-// f32 quadratic form keeps full relative precision far from the origin.
-// Terms are evaluated in base 2 (hardware exp2/log2), log-sum-exp with the
-// largest term factored out; a voxel's contribution img * term is rounded
-// to FP64 and accumulated in FP64. A voxel whose largest term is below the
+// Per voxel and candidate, the scaled quadratic form is |y|^2 with
+// y = U (x - a_s) + e: U the candidate's Cholesky factor, a_s an anchor voxel
+// of the supervoxel (x - a_s is exact in f32) and e = U (a_s - m_k) formed in
+// FP64 and rounded once, so the f32 form keeps full relative precision far
+// from the origin (9 FMA-class operations per candidate). Terms are evaluated
+// in base 2 (hardware exp2/log2), log-sum-exp with the largest term factored
+// out; a voxel's contribution img * term is rounded to FP64 and accumulated
+// in FP64. A voxel whose largest term is below the
 // FP64 underflow of the reference's direct sum (2^-1074) contributes nothing,
 // as in the reference.
 //
@@ -1346,24 +1363,28 @@ __global__ __launch_bounds__(256) void k_loglik(DevPtrs P, KOpts O) {
         vv[u] = make_float4(P.vx[g], P.vy[g], P.vz[g], P.vimg[g]);
       }
     }
-    float pd0 = 0.0f, pd1 = 0.0f, pd2 = 0.0f, pw[6] = {0, 0, 0, 0, 0, 0}, pg = -INFINITY;
+    // candidate constants: the Cholesky factor U and e = U (a_s - m_k), FP64
+    // from the FP64 mean then rounded, so y = U (x - a_s) + e = U (x - m_k)
+    float pe0 = 0.0f, pe1 = 0.0f, pe2 = 0.0f, pu[6] = {0, 0, 0, 0, 0, 0}, pg = -INFINITY;
     if (n > 0 && c.kk >= 0) {
       const LLPack lp = P.llpack[c.kk];
-      pd0 = (float)((double)c.a0 - lp.m[0]);
-      pd1 = (float)((double)c.a1 - lp.m[1]);
-      pd2 = (float)((double)c.a2 - lp.m[2]);
+      const double d0 = (double)c.a0 - lp.m[0], d1 = (double)c.a1 - lp.m[1],
+                   d2 = (double)c.a2 - lp.m[2];
 #pragma unroll
-      for (int i = 0; i < 6; i++) pw[i] = lp.M[i];
+      for (int i = 0; i < 6; i++) pu[i] = lp.U[i];
+      pe0 = (float)((double)pu[0] * d0 + (double)pu[1] * d1 + (double)pu[2] * d2);
+      pe1 = (float)((double)pu[3] * d1 + (double)pu[4] * d2);
+      pe2 = (float)((double)pu[5] * d2);
       pg = (float)(lp.lg2 - lsa2);
     }
-    float cd0[KM], cd1[KM], cd2[KM], cw[KM][6], cg[KM];
+    float ce0[KM], ce1[KM], ce2[KM], cu[KM][6], cg[KM];
 #pragma unroll
     for (int j = 0; j < KM; j++) {
-      cd0[j] = group_bcast<GW>(pd0, j);
-      cd1[j] = group_bcast<GW>(pd1, j);
-      cd2[j] = group_bcast<GW>(pd2, j);
+      ce0[j] = group_bcast<GW>(pe0, j);
+      ce1[j] = group_bcast<GW>(pe1, j);
+      ce2[j] = group_bcast<GW>(pe2, j);
 #pragma unroll
-      for (int i = 0; i < 6; i++) cw[j][i] = group_bcast<GW>(pw[i], j);
+      for (int i = 0; i < 6; i++) cu[j][i] = group_bcast<GW>(pu[i], j);
       cg[j] = (j < K) ? group_bcast<GW>(pg, j) : -INFINITY;
     }
     // img * log2(sum_j 2^t_j) of one voxel, rounded to FP64
@@ -1374,14 +1395,12 @@ __global__ __launch_bounds__(256) void k_loglik(DevPtrs P, KOpts O) {
       float mx = -INFINITY;
 #pragma unroll
       for (int j = 0; j < KM; j++) {
-        const float d0 = x0 + cd0[j], d1 = x1 + cd1[j], d2 = x2 + cd2[j];
-        float q = __builtin_fmaf(cw[j][0] * d0, d0, 0.0f);
-        q = __builtin_fmaf(cw[j][3] * d1, d1, q);
-        q = __builtin_fmaf(cw[j][5] * d2, d2, q);
-        float o = cw[j][1] * d0 * d1;
-        o = __builtin_fmaf(cw[j][2] * d0, d2, o);
-        o = __builtin_fmaf(cw[j][4] * d1, d2, o);
-        q = __builtin_fmaf(2.0f, o, q);
+        const float y0 = __builtin_fmaf(cu[j][0], x0,
+                                        __builtin_fmaf(cu[j][1], x1,
+                                                       __builtin_fmaf(cu[j][2], x2, ce0[j])));
+        const float y1 = __builtin_fmaf(cu[j][3], x1, __builtin_fmaf(cu[j][4], x2, ce1[j]));
+        const float y2 = __builtin_fmaf(cu[j][5], x2, ce2[j]);
+        const float q = __builtin_fmaf(y0, y0, __builtin_fmaf(y1, y1, y2 * y2));
         tj[j] = cg[j] - q;
         mx = fmaxf(mx, tj[j]);
       }

@@ -236,9 +236,10 @@ int batch_build(int device, const std::vector<FrameSrc>& fs, const tgmm_em_opts&
     F.g0 = g;
     F.ng = f.G;
     F.c0 = c;
-    // grid-cell range: more entries than cells (the per-frame scan writes one
-    // past the last cell), a multiple of 64 (cell block -> frame table)
-    F.ncap = round_up(std::max<long long>(8, 2LL * f.G), kCB);
+    // grid-cell range: room for the densest grid (two means per cell) plus
+    // one entry past the last cell (the per-frame scan writes it), a multiple
+    // of 64 (cell block -> frame table)
+    F.ncap = round_up(f.G / 2 + kCB, kCB);
     // every block mapped to the frame's 256-aligned ranges takes part in the
     // per-frame reductions, including trailing blocks with no valid element;
     // the Gaussian range has at least one padding entry (segment end)

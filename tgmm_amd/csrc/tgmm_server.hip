@@ -187,14 +187,14 @@ extern "C" int tgmm_server_create(int device, const tgmm_server_opts* opts, tgmm
     t.Vcap = V2;
     t.Scap = rup(V2 / 8 + 256LL * t.Fcap, kAlign);
     t.Gcap = rup(V2 / 32 + 256LL * t.Fcap, kAlign);
-    t.Ccap = rup(2 * t.Gcap + 2LL * kCB * t.Fcap, kCB);
+    t.Ccap = rup(t.Gcap / 2 + 2LL * kCB * t.Fcap, kCB);
     if (ws_bytes_estimate(t, e.K) > opts->hbm_budget_bytes || V2 > (1LL << 31)) break;
     V = V2;
   }
   sh.Vcap = V;
   sh.Scap = rup(V / 8 + 256LL * sh.Fcap, kAlign);
   sh.Gcap = rup(V / 32 + 256LL * sh.Fcap, kAlign);
-  sh.Ccap = rup(2 * sh.Gcap + 2LL * kCB * sh.Fcap, kCB);
+  sh.Ccap = rup(sh.Gcap / 2 + 2LL * kCB * sh.Fcap, kCB);
   if (int rc = ws_alloc(&s->ws, sh, e.K)) {
     ws_free(&s->ws);
     delete s;
@@ -411,7 +411,7 @@ static int try_admit(tgmm_server* s) {
     if (f < 0) break;
     t.sa = rup(std::max<long long>(t.S, 1), kAlign);
     t.ga = rup(t.G + 1, kAlign);  // at least one padding entry (segment end)
-    t.ncap = rup(std::max<long long>(8, 2 * t.G), kCB);
+    t.ncap = rup(t.G / 2 + kCB, kCB);
     const bool reuse_g = t.gm_in.empty();  // priors from the resident range
     long long v0, s0, g0 = 0, c0;
     if (!s->av.alloc(t.N, 1, &v0)) {
