@@ -615,8 +615,7 @@ __global__ __launch_bounds__(256) void k_knn(DevPtrs P, KOpts O) {
   // shells 0 and 1 together: the 9 rows of the 3x3x3 block, each trimmed to
   // the cells whose box is within the bound (a cell farther than the bound
   // holds no candidate that can beat the K-th). All row bounds are loaded
-  // first; the rows' references are then walked as one flat list, KC loads
-  // per lane in flight at a time.
+  // first.
   {
     double dx2[3], dy2[3], dz2[3];
 #pragma unroll
@@ -642,21 +641,27 @@ __global__ __launch_bounds__(256) void k_knn(DevPtrs P, KOpts O) {
       pb[t] = b0;
       cum[t + 1] = cum[t] + (e0 - b0);
     }
-    const int T = cum[9];
-    for (int q = 0; q < T; q += KC * QL) {
-      float4 v[KC];
+    // row by row (the row's bounds picked once per row), KC references per
+    // lane in flight
+#pragma unroll 1
+    for (int t = 0; t < 9; t++) {
+      int rb = pb[0], len = cum[1] - cum[0];
 #pragma unroll
-      for (int w = 0; w < KC; w++) {
-        const int x = q + w * QL + sub;
-        int addr = pb[0] + x;
+      for (int r = 1; r < 9; r++)
+        if (t == r) {
+          rb = pb[r];
+          len = cum[r + 1] - cum[r];
+        }
+      const int re = rb + len;
+      for (int p = rb + sub; p < re; p += KC * QL) {
+        float4 v[KC];
 #pragma unroll
-        for (int t = 1; t < 9; t++)
-          if (x >= cum[t]) addr = pb[t] + (x - cum[t]);
-        v[w] = (x < T) ? P.cg4[addr] : make_float4(0.f, 0.f, 0.f, 0.f);
+        for (int w = 0; w < KC; w++)
+          v[w] = (p + w * QL < re) ? P.cg4[p + w * QL] : make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+        for (int w = 0; w < KC; w++)
+          if (p + w * QL < re) consider(v[w]);
       }
-#pragma unroll
-      for (int w = 0; w < KC; w++)
-        if (q + w * QL + sub < T) consider(v[w]);
     }
   }
   if (QL > 1) knn_merge<KM, QL>(bd, bi);
