@@ -489,16 +489,6 @@ __global__ __launch_bounds__(kGB) void k_grid_scatter(DevPtrs P) {
   P.cg4[pos] = make_float4(P.mf[k], P.mf[G + k], P.mf[2 * G + k], __int_as_float((int)k));
 }
 
-// Exact K nearest means by expanding shells of grid cells. After shell r,
-// every unvisited mean lies beyond an open face of the visited block, so its
-// scaled distance is at least LB; the search stops once the K-th best
-// distance is below LB^2 with a relative margin far above f32 rounding, so
-// the result equals brute force (ties broken by the lower index). Queries are
-// taken in grid-cell order (q4). QL lanes share a query: each scans every
-// QL-th reference of the visited cells into its own sorted list, and the
-// lists are merged by a butterfly after each shell (a reference seen by two
-// lanes of a merged list is kept once), so every lane of the group holds the
-// K nearest of all references seen.
 template <int KM, int QL>
 __device__ inline void knn_merge(float (&bd)[KM], int (&bi)[KM]) {
 #pragma unroll
@@ -532,6 +522,103 @@ __device__ inline void knn_merge(float (&bd)[KM], int (&bi)[KM]) {
     }
   }
 }
+
+// Shells r >= 2 of the exact search around query cell c (scaled query u),
+// until the K-th best is below the open-face bound of the visited block.
+// Cells are pruned by the current K-th best: a row whose y/z boxes, or a cell
+// whose box, lies farther than it (with the bound's relative margin) holds no
+// reference that can enter the K best. QL lanes share a query (lane `sub`
+// takes every QL-th reference of a range); their lists are merged after each
+// shell.
+template <int KM, int QL, typename Consider>
+__device__ inline void knn_shells(const DevPtrs& P, const FrameDev* F, const int* c,
+                                  const double* u, int K, int sub, float (&bd)[KM],
+                                  int (&bi)[KM], Consider consider) {
+  const int nx = F->dims[0], ny = F->dims[1], nz = F->dims[2];
+  const int dims[3] = {nx, ny, nz};
+  const double h = F->h;
+  const double org[3] = {F->org[0], F->org[1], F->org[2]};
+  const long long c0 = F->c0;
+  auto kth = [&]() -> double {
+    float v = INFINITY;
+#pragma unroll
+    for (int t = 0; t < KM; t++)
+      if (t == K - 1) v = bd[t];
+    return (double)v;
+  };
+  auto lower_bound_sq = [&](int r) -> double {
+    double lb = INFINITY;
+    for (int a = 0; a < 3; a++) {
+      if (c[a] - r > 0) lb = fmin(lb, u[a] - (org[a] + (double)(c[a] - r) * h));
+      if (c[a] + r < dims[a] - 1) lb = fmin(lb, (org[a] + (double)(c[a] + r + 1) * h) - u[a]);
+    }
+    if (lb == INFINITY) return INFINITY;
+    lb = fmax(lb, 0.0) * (1.0 - 1e-5);
+    return lb * lb;
+  };
+  auto box_d2 = [&](int a, int ci) -> double {
+    const double lo = org[a] + (double)ci * h, hi = org[a] + (double)(ci + 1) * h;
+    const double d = fmax(fmax(lo - u[a], u[a] - hi), 0.0) * (1.0 - 1e-5);
+    return d * d;
+  };
+  const int rmax = max(nx, max(ny, nz));
+  for (int r = 1; r <= rmax; r++) {
+    if (r > 1) {
+      const double bound = kth();
+      const int zlo = max(c[2] - r, 0), zhi = min(c[2] + r, nz - 1);
+      const int ylo = max(c[1] - r, 0), yhi = min(c[1] + r, ny - 1);
+      for (int z = zlo; z <= zhi; z++) {
+        const double dz = box_d2(2, z);
+        if (dz > bound) continue;
+        const bool zface = (z == c[2] - r) || (z == c[2] + r);
+        for (int y = ylo; y <= yhi; y++) {
+          const double base = dz + box_d2(1, y);
+          if (base > bound) continue;
+          const bool face = zface || (y == c[1] - r) || (y == c[1] + r);
+          const long long row = c0 + ((long long)z * ny + y) * nx;
+#pragma unroll 1
+          for (int g = 0; g < 2; g++) {
+            int xa, xb;
+            if (face) {
+              if (g == 1) break;
+              xa = max(c[0] - r, 0);
+              xb = min(c[0] + r, nx - 1);
+              while (xa < c[0] && base + box_d2(0, xa) > bound) xa++;
+              while (xb > c[0] && base + box_d2(0, xb) > bound) xb--;
+            } else {
+              xa = xb = g == 0 ? c[0] - r : c[0] + r;
+              if (xa < 0 || xa >= nx || base + box_d2(0, xa) > bound) continue;
+            }
+            const int pb = P.cell_start[row + xa], pe = P.cell_start[row + xb + 1];
+            for (int p = pb + sub; p < pe; p += 4 * QL) {
+              float4 v[4];
+#pragma unroll
+              for (int w = 0; w < 4; w++)
+                v[w] = (p + w * QL < pe) ? P.cg4[p + w * QL] : make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+              for (int w = 0; w < 4; w++)
+                if (p + w * QL < pe) consider(v[w]);
+            }
+          }
+        }
+      }
+      if (QL > 1) knn_merge<KM, QL>(bd, bi);
+    }
+    const double lb2 = lower_bound_sq(r);
+    if (lb2 == INFINITY || kth() < lb2) break;  // every cell visited, or done
+  }
+}
+
+// Exact K nearest means by expanding shells of grid cells. After shell r,
+// every unvisited mean lies beyond an open face of the visited block, so its
+// scaled distance is at least LB; the search stops once the K-th best
+// distance is below LB^2 with a relative margin far above f32 rounding, so
+// the result equals brute force (ties broken by the lower index). Queries are
+// taken in grid-cell order (q4). QL lanes share a query: each scans every
+// QL-th reference of the visited cells into its own sorted list, and the
+// lists are merged by a butterfly after each shell (a reference seen by two
+// lanes of a merged list is kept once), so every lane of the group holds the
+// K nearest of all references seen.
 
 template <int KM, int KC, int QL>
 __global__ __launch_bounds__(256) void k_knn(DevPtrs P, KOpts O) {
@@ -665,67 +752,7 @@ __global__ __launch_bounds__(256) void k_knn(DevPtrs P, KOpts O) {
     }
   }
   if (QL > 1) knn_merge<KM, QL>(bd, bi);
-  auto lower_bound_sq = [&](int r) -> double {
-    double lb = INFINITY;
-    for (int a = 0; a < 3; a++) {
-      if (c[a] - r > 0) lb = fmin(lb, u[a] - (org[a] + (double)(c[a] - r) * h));
-      if (c[a] + r < dims[a] - 1) lb = fmin(lb, (org[a] + (double)(c[a] + r + 1) * h) - u[a]);
-    }
-    if (lb == INFINITY) return INFINITY;
-    lb = fmax(lb, 0.0) * (1.0 - 1e-5);
-    return lb * lb;
-  };
-  // done once the K-th best is below the bound (a finite K-th best means K
-  // references were found)
-  auto done_at = [&](int r) -> bool {
-    const double lb2 = lower_bound_sq(r);
-    if (lb2 == INFINITY) return true;  // every cell visited
-    float kth = INFINITY;
-#pragma unroll
-    for (int t = 0; t < KM; t++)
-      if (t == K - 1) kth = bd[t];
-    return (double)kth < lb2;
-  };
-  // further shells: the rows of shell r; a row on a z or y face of the
-  // shell contributes its whole x range, any other row its two end cells
-  const int rmax = max(nx, max(ny, nz));
-  if (!done_at(1)) {
-    for (int r = 2; r <= rmax; r++) {
-      const int zlo = max(c[2] - r, 0), zhi = min(c[2] + r, nz - 1);
-      const int ylo = max(c[1] - r, 0), yhi = min(c[1] + r, ny - 1);
-      for (int z = zlo; z <= zhi; z++) {
-        const bool zface = (z == c[2] - r) || (z == c[2] + r);
-        for (int y = ylo; y <= yhi; y++) {
-          const bool face = zface || (y == c[1] - r) || (y == c[1] + r);
-          const long long row = c0 + ((long long)z * ny + y) * nx;
-#pragma unroll 1
-          for (int g = 0; g < 2; g++) {
-            int xa, xb;
-            if (face) {
-              xa = g == 0 ? max(c[0] - r, 0) : 1;
-              xb = g == 0 ? min(c[0] + r, nx - 1) : 0;
-            } else {
-              xa = xb = g == 0 ? c[0] - r : c[0] + r;
-              if (xa < 0 || xa >= nx) xb = xa - 1;
-            }
-            if (xa > xb) continue;
-            const int pb = P.cell_start[row + xa], pe = P.cell_start[row + xb + 1];
-            for (int p = pb + sub; p < pe; p += 4 * QL) {
-              float4 v[4];
-#pragma unroll
-              for (int w = 0; w < 4; w++)
-                v[w] = (p + w * QL < pe) ? P.cg4[p + w * QL] : make_float4(0.f, 0.f, 0.f, 0.f);
-#pragma unroll
-              for (int w = 0; w < 4; w++)
-                if (p + w * QL < pe) consider(v[w]);
-            }
-          }
-        }
-      }
-      if (QL > 1) knn_merge<KM, QL>(bd, bi);
-      if (done_at(r)) break;
-    }
-  }
+  knn_shells<KM, QL>(P, F, c, u, K, sub, bd, bi, consider);
   if (sub == 0) {
 #pragma unroll
     for (int j = 0; j < KM; j++)
@@ -863,64 +890,8 @@ __global__ __launch_bounds__(64 * NW) void k_knn_cell(DevPtrs P, KOpts O) {
         }
     }
     if (!act) continue;
-    // further shells for this query alone (same bound as k_knn)
     const double u[3] = {(double)q0 * s0, (double)q1 * s1, (double)q2 * s2};
-    auto lower_bound_sq = [&](int r) -> double {
-      double lb = INFINITY;
-      for (int a = 0; a < 3; a++) {
-        if (c[a] - r > 0) lb = fmin(lb, u[a] - (org[a] + (double)(c[a] - r) * h));
-        if (c[a] + r < dims[a] - 1) lb = fmin(lb, (org[a] + (double)(c[a] + r + 1) * h) - u[a]);
-      }
-      if (lb == INFINITY) return INFINITY;
-      lb = fmax(lb, 0.0) * (1.0 - 1e-5);
-      return lb * lb;
-    };
-    auto done_at = [&](int r) -> bool {
-      const double lb2 = lower_bound_sq(r);
-      if (lb2 == INFINITY) return true;
-      float kth = INFINITY;
-#pragma unroll
-      for (int t = 0; t < KM; t++)
-        if (t == K - 1) kth = bd[t];
-      return (double)kth < lb2;
-    };
-    const int rmax = max(nx, max(ny, nz));
-    if (!done_at(1)) {
-      for (int r = 2; r <= rmax; r++) {
-        const int zlo = max(c[2] - r, 0), zhi = min(c[2] + r, nz - 1);
-        const int ylo = max(c[1] - r, 0), yhi = min(c[1] + r, ny - 1);
-        for (int z = zlo; z <= zhi; z++) {
-          const bool zface = (z == c[2] - r) || (z == c[2] + r);
-          for (int y = ylo; y <= yhi; y++) {
-            const bool face = zface || (y == c[1] - r) || (y == c[1] + r);
-            const long long row = c0 + ((long long)z * ny + y) * nx;
-#pragma unroll 1
-            for (int g = 0; g < 2; g++) {
-              int xa, xb;
-              if (face) {
-                xa = g == 0 ? max(c[0] - r, 0) : 1;
-                xb = g == 0 ? min(c[0] + r, nx - 1) : 0;
-              } else {
-                xa = xb = g == 0 ? c[0] - r : c[0] + r;
-                if (xa < 0 || xa >= nx) xb = xa - 1;
-              }
-              if (xa > xb) continue;
-              const int rb = P.cell_start[row + xa], re = P.cell_start[row + xb + 1];
-              for (int p = rb; p < re; p += 4) {
-                float4 v[4];
-#pragma unroll
-                for (int k4 = 0; k4 < 4; k4++)
-                  v[k4] = (p + k4 < re) ? P.cg4[p + k4] : make_float4(0.f, 0.f, 0.f, 0.f);
-#pragma unroll
-                for (int k4 = 0; k4 < 4; k4++)
-                  if (p + k4 < re) consider(v[k4]);
-              }
-            }
-          }
-        }
-        if (done_at(r)) break;
-      }
-    }
+    knn_shells<KM, 1>(P, F, c, u, K, 0, bd, bi, consider);
 #pragma unroll
     for (int j = 0; j < KM; j++)
       if (j < K) P.ind[(long long)j * S + s] = (bd[j] <= 1e32f) ? bi[j] : -1;
