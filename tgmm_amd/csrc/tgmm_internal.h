@@ -125,7 +125,7 @@ struct DevPtrs {
   double* eldn;     // Gcap: E[ln det] of the next iteration (committed to the state
                     // when that iteration's M-step runs)
   LLPack* llpack;   // Gcap: log-likelihood parameters (written by death)
-  float* mf;        // 3*Gcap f32 means (kNN references)
+  float4* m4;       // Gcap: f32 mean and index bits (kNN references; NaN: dead)
   int* gcell;
   int* gblk_frame;  // Gcap/kGB
   // M-step segments

@@ -438,6 +438,7 @@ __device__ inline void gauss_prep_one(const DevPtrs& P, const KOpts& O, const Fr
   if (!alive) {
     P.eldn[k] = -1e32;
     P.gcell[k] = -1;
+    P.m4[k] = make_float4(NAN, NAN, NAN, __int_as_float((int)k));  // no kNN distance
     return;
   }
   double eld = 3.0 * 0.69314718055994530942 + log(det3(W));
@@ -450,12 +451,12 @@ __device__ inline void gauss_prep_one(const DevPtrs& P, const KOpts& O, const Fr
   ep.kappa0 = 0.5 * eld - 1.5 / beta;
   ep.dpsi = digamma(alpha);
   P.epack[k] = ep;
+  const float4 mf = make_float4((float)m[0], (float)m[1], (float)m[2], __int_as_float((int)k));
+  P.m4[k] = mf;
   int c[3];
-  for (int a = 0; a < 3; a++) {
-    const float mf = (float)m[a];
-    P.mf[a * G + k] = mf;
-    c[a] = axis_cell((double)mf * (double)O.scale[a], F->org[a], F->h, F->dims[a]);
-  }
+  c[0] = axis_cell((double)mf.x * (double)O.scale[0], F->org[0], F->h, F->dims[0]);
+  c[1] = axis_cell((double)mf.y * (double)O.scale[1], F->org[1], F->h, F->dims[1]);
+  c[2] = axis_cell((double)mf.z * (double)O.scale[2], F->org[2], F->h, F->dims[2]);
   const long long cell = F->c0 + ((long long)c[2] * F->dims[1] + c[1]) * F->dims[0] + c[0];
   P.gcell[k] = (int)cell;
   atomicAdd(P.cell_cnt + cell, 1);
@@ -486,7 +487,7 @@ __global__ __launch_bounds__(kGB) void k_grid_scatter(DevPtrs P) {
   if (c < 0) return;
   const int pos = P.cell_start[c] + atomicSub(P.cell_cnt + c, 1) - 1;
   const long long G = P.Gcap;
-  P.cg4[pos] = make_float4(P.mf[k], P.mf[G + k], P.mf[2 * G + k], __int_as_float((int)k));
+  P.cg4[pos] = P.m4[k];
 }
 
 template <int KM, int QL>
@@ -670,8 +671,8 @@ __global__ __launch_bounds__(256) void k_knn(DevPtrs P, KOpts O) {
     }
   };
   // An upper bound of the K-th distance: the previous iteration's candidates
-  // of this supervoxel, if all K are still means of this frame that were
-  // binned this iteration (a mean's distance is a real candidate distance).
+  // of this supervoxel, if all K are still living means of this frame (a
+  // mean's distance is a real candidate distance; dead means are NaN).
   float kbound = INFINITY;
   {
     float mx = -INFINITY;
@@ -680,13 +681,11 @@ __global__ __launch_bounds__(256) void k_knn(DevPtrs P, KOpts O) {
     for (int j = 0; j < KM; j++) {
       if (j >= K) continue;
       const int k = P.ind[(long long)j * S + s];
-      const bool ok = k >= F->g0 && k < F->g0 + F->ng && P.gcell[k] >= 0;
-      all &= ok;
-      if (ok) {
-        const float d = knn_dist(q0, q1, q2, P.mf[k], P.mf[P.Gcap + k], P.mf[2 * P.Gcap + k],
-                                 s0, s1, s2);
-        mx = fmaxf(mx, d);
-      }
+      const bool in = k >= F->g0 && k < F->g0 + F->ng;
+      const float4 m = in ? P.m4[k] : make_float4(NAN, NAN, NAN, 0.f);
+      const float d = knn_dist(q0, q1, q2, m.x, m.y, m.z, s0, s1, s2);
+      all &= (d <= 1e32f);
+      mx = fmaxf(mx, d);
     }
     if (all) kbound = mx;
   }

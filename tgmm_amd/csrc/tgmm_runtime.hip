@@ -100,7 +100,7 @@ int ws_alloc(Workspace* w, const WsShape& sh, int K) {
   AL(P.epack, G);
   AL(P.eldn, G);
   AL(P.llpack, G);
-  AL(P.mf, 3 * G);
+  AL(P.m4, G);
   AL(P.gcell, G);
   AL(P.gblk_frame, G / kGB);
   AL(P.seg_cnt, G + 1);
