@@ -146,6 +146,8 @@ struct DevPtrs {
   double* part_sg;  // Scap/256: partials of 256-supervoxel groups
   FrameDev* fr;
   long long Scap, Gcap, Ccap, Vcap;
+  long long Sext, Gext, Cext;  // launch extents (<= capacities): grids cover
+                               // only the ranges in use
   int Fcap;
   int nframes;
   int only_frame;   // prep/reset kernels: -1 = every frame, else this frame

@@ -1465,19 +1465,19 @@ static inline unsigned nb(long long n, int per) { return (unsigned)((n + per - 1
 
 
 hipError_t launch_prep(const DevPtrs& P, const KOpts& O, hipStream_t st) {
-  k_moments<<<nb(P.Scap, 4), 256, 0, st>>>(P);
+  k_moments<<<nb(P.Sext, 4), 256, 0, st>>>(P);
   k_grid_setup<<<P.only_frame >= 0 ? 1 : P.nframes, 1024, 0, st>>>(P, O);
-  k_qcell_count<<<nb(P.Scap, 256), 256, 0, st>>>(P, O);
+  k_qcell_count<<<nb(P.Sext, 256), 256, 0, st>>>(P, O);
   k_scan_frames<<<P.only_frame >= 0 ? 1 : P.nframes, 1024, 0, st>>>(P, 2, 0, O.K);
-  k_qcell_scatter<<<nb(P.Scap, 256), 256, 0, st>>>(P);
+  k_qcell_scatter<<<nb(P.Sext, 256), 256, 0, st>>>(P);
   return hipGetLastError();
 }
 
 hipError_t launch_reset(const DevPtrs& P, const KOpts& O, hipStream_t st) {
   k_iter_init<<<nb(P.nframes, 64), 64, 0, st>>>(P);
-  k_alpha_init<<<nb(P.Gcap, kGB), kGB, 0, st>>>(P);
+  k_alpha_init<<<nb(P.Gext, kGB), kGB, 0, st>>>(P);
   k_alpha_sum<<<P.only_frame >= 0 ? 1 : P.nframes, 64, 0, st>>>(P, 0, 1);
-  k_gauss_prep<<<nb(P.Gcap, kGB), kGB, 0, st>>>(P, O);
+  k_gauss_prep<<<nb(P.Gext, kGB), kGB, 0, st>>>(P, O);
   return hipGetLastError();
 }
 
@@ -1519,7 +1519,7 @@ static void launch_loglik(const DevPtrs& P, const KOpts& O, hipStream_t st,
                           hipEvent_t ev_after) {
   const int gw = O.tiles.ll_gw, spg = O.tiles.ll_u;
   const long long B = (256 / gw) * spg;
-  const unsigned g = nb(P.Scap, B);
+  const unsigned g = nb(P.Sext, B);
 #define TGMM_LL(GW_, SPG_)                                    \
   if (gw == GW_ && spg == SPG_) {                             \
     k_loglik<KM, GW_, SPG_><<<g, 256, 0, st>>>(P, O);         \
@@ -1541,7 +1541,7 @@ static void launch_loglik(const DevPtrs& P, const KOpts& O, hipStream_t st,
 }
 
 static void launch_mstep_stats(const DevPtrs& P, const KOpts& O, hipStream_t st) {
-  const unsigned g = nb(P.Gcap, 16);
+  const unsigned g = nb(P.Gext, 16);
   if (O.tiles.ms_lanes == 4)
     k_mstep_stats<4><<<g, 64, 0, st>>>(P);
   else if (O.tiles.ms_lanes == 8)
@@ -1553,7 +1553,7 @@ static void launch_mstep_stats(const DevPtrs& P, const KOpts& O, hipStream_t st)
 template <int KM>
 static void launch_knn(const DevPtrs& P, const KOpts& O, hipStream_t st) {
   if (O.tiles.knn_lanes == 0) {  // one block of knn_block/64 waves per grid cell
-    const unsigned g = (unsigned)P.Ccap;
+    const unsigned g = (unsigned)P.Cext;
     if (O.tiles.knn_block == 64)
       k_knn_cell<KM, 1><<<g, 64, 0, st>>>(P, O);
     else if (O.tiles.knn_block == 128)
@@ -1563,7 +1563,7 @@ static void launch_knn(const DevPtrs& P, const KOpts& O, hipStream_t st) {
     return;
   }
   const int bk = O.tiles.knn_block, ql = O.tiles.knn_lanes;
-  const unsigned g = nb(P.Scap, bk / ql);
+  const unsigned g = nb(P.Sext, bk / ql);
   if (O.tiles.knn_chunk == 4) {
     if (ql == 1) k_knn<KM, 4, 1><<<g, bk, 0, st>>>(P, O);
     else if (ql == 2) k_knn<KM, 4, 2><<<g, bk, 0, st>>>(P, O);
@@ -1578,7 +1578,7 @@ static void launch_knn(const DevPtrs& P, const KOpts& O, hipStream_t st) {
 template <int KM>
 static hipError_t iteration_t(const DevPtrs& P, const KOpts& O, hipStream_t st, hipEvent_t e0,
                               hipEvent_t e1, int tk) {
-  const unsigned gb = nb(P.Gcap, kGB);
+  const unsigned gb = nb(P.Gext, kGB);
   const int bs = O.tiles.sv_block;
   k_scan_frames<<<P.nframes, 1024, 0, st>>>(P, 0, 1, O.K);
   k_grid_scatter<<<gb, kGB, 0, st>>>(P);
@@ -1586,11 +1586,11 @@ static hipError_t iteration_t(const DevPtrs& P, const KOpts& O, hipStream_t st, 
   launch_knn<KM>(P, O, st);
   if (tk == TK_KNN && e1) (void)hipEventRecord(e1, st);
   if (tk == TK_ESTEP && e0) (void)hipEventRecord(e0, st);
-  k_estep<KM><<<nb(P.Scap, bs), bs, 0, st>>>(P, O);
+  k_estep<KM><<<nb(P.Sext, bs), bs, 0, st>>>(P, O);
   if (tk == TK_ESTEP && e1) (void)hipEventRecord(e1, st);
   k_scan_frames<<<P.nframes, 1024, 0, st>>>(P, 1, 1, O.K);
-  k_pair_scatter<<<nb(P.Scap, bs), bs, 0, st>>>(P, O);
-  k_seg_sort<<<nb(P.Gcap, 16), 256, 0, st>>>(P);
+  k_pair_scatter<<<nb(P.Sext, bs), bs, 0, st>>>(P, O);
+  k_seg_sort<<<nb(P.Gext, 16), 256, 0, st>>>(P);
   if (tk == TK_MSTEP && e0) (void)hipEventRecord(e0, st);
   launch_mstep_stats(P, O, st);
   k_mstep_update<<<gb, kGB, 0, st>>>(P, O);

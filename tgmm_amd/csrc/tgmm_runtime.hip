@@ -62,6 +62,9 @@ int ws_alloc(Workspace* w, const WsShape& sh, int K) {
   P.Scap = sh.Scap;
   P.Gcap = sh.Gcap;
   P.Ccap = sh.Ccap;
+  P.Sext = sh.Scap;
+  P.Gext = sh.Gcap;
+  P.Cext = sh.Ccap;
   P.Vcap = sh.Vcap;
   P.Fcap = sh.Fcap;
   auto A = [&](void** p, size_t bytes) -> int {

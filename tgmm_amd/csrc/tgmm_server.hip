@@ -51,8 +51,10 @@ class RangeAlloc {
  public:
   void init(long long cap) {
     free_.clear();
+    hwm_ = 0;
     if (cap > 0) free_[0] = cap;
   }
+  long long high_water() const { return hwm_; }  // largest end ever allocated
   bool alloc(long long n, long long align, long long* out) {
     if (n <= 0) {
       *out = 0;
@@ -67,6 +69,7 @@ class RangeAlloc {
         if (st > a) free_[a] = st - a;
         if (st + n < b) free_[st + n] = b - (st + n);
         *out = st;
+        hwm_ = std::max(hwm_, st + n);
         return true;
       }
     }
@@ -91,6 +94,7 @@ class RangeAlloc {
 
  private:
   std::map<long long, long long> free_;
+  long long hwm_ = 0;
 };
 
 struct Ticket {
@@ -203,6 +207,9 @@ extern "C" int tgmm_server_create(int device, const tgmm_server_opts* opts, tgmm
   DevPtrs& P = s->ws.P;
   P.nframes = sh.Fcap;
   P.only_frame = -1;
+  P.Sext = kAlign;  // grown to the allocated ranges as frames are admitted
+  P.Gext = kAlign;
+  P.Cext = kCB;
   if (hipStreamCreateWithFlags(&s->st, hipStreamNonBlocking) != hipSuccess) {
     ws_free(&s->ws);
     delete s;
@@ -290,6 +297,25 @@ extern "C" int tgmm_server_submit(tgmm_server* s, int stream_id, const tgmm_fram
   return TGMM_OK;
 }
 
+// Kernel grids cover the ranges allocated so far (high-water marks), not the
+// whole budget-sized workspace; the captured iteration is re-captured when
+// they grow.
+static int update_extents(tgmm_server* s) {
+  DevPtrs& P = s->ws.P;
+  const long long se = std::max<long long>(rup(s->as.high_water(), kAlign), kAlign);
+  const long long ge = std::max<long long>(rup(s->ag.high_water(), kAlign), kAlign);
+  const long long ce = std::max<long long>(rup(s->ac.high_water(), kCB), kCB);
+  if (se == P.Sext && ge == P.Gext && ce == P.Cext) return TGMM_OK;
+  P.Sext = std::min(se, P.Scap);
+  P.Gext = std::min(ge, P.Gcap);
+  P.Cext = std::min(ce, P.Ccap);
+  if (s->gexec) (void)hipGraphExecDestroy(s->gexec);
+  if (s->graph) (void)hipGraphDestroy(s->graph);
+  s->gexec = nullptr;
+  s->graph = nullptr;
+  return TGMM_OK;
+}
+
 // Map the frame's ranges to slot f (or -1) in the per-element frame tables.
 static int map_ranges(tgmm_server* s, const Ticket& t, int f) {
   DevPtrs& P = s->ws.P;
@@ -370,6 +396,7 @@ static int admit(tgmm_server* s, Ticket& t, int f) {
   F.tol = s->o.em.tol;
   HIPCHK(hipMemcpyAsync(P.fr + f, &F, sizeof(FrameDev), hipMemcpyHostToDevice, st));
   if (int rc = map_ranges(s, t, f)) return rc;
+  if (int rc = update_extents(s)) return rc;  // grids must cover the new ranges
   DevPtrs Q = P;
   Q.only_frame = f;
   // the cell range may hold counts left by an earlier frame's last binning
@@ -534,6 +561,7 @@ extern "C" int tgmm_server_step(tgmm_server* s, int* pending) {
       if (int rc = retire(s, f)) return rc;
   const double t1 = now_s();
   if (int rc = try_admit(s)) return rc;
+  if (int rc = update_extents(s)) return rc;
   const double t2 = now_s();
   if (s->in_flight > 0) {
     if (int rc = server_capture(s)) return rc;
