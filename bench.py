@@ -306,7 +306,7 @@ def _serve(streams, opts, budget, order, arrivals=None, pct=(0.5, 0.5, 0.8), til
     ns = len(streams)
     G = [fr[0].G for fr in streams]
     srv = em.Server(opts, hbm_budget_bytes=budget, max_frames_in_flight=2 * ns,
-                    beta_pct=pct[0], nu_pct=pct[1], alpha_pct=pct[2])
+                    beta_pct=pct[0], nu_pct=pct[1], alpha_pct=pct[2], borrow=True)
     if tiles is not None:
         srv.set_tiles(tiles)
     nxt = [0] * ns
@@ -361,6 +361,15 @@ def serve_bench(args):
     streams = synth.config4_streams(n_frames=args.stream_frames)
     n_frames = sum(len(fr) for fr in streams)
     n_vox = sum(f.N for fr in streams for f in fr)
+    # the voxel streams in page-locked host memory, as a camera pipeline would
+    # deliver them, so that uploads overlap the iterations
+    t1 = time.time()
+    for fr in streams:
+        for f in fr:
+            f.query = em.pinned(f.query)
+            f.img = em.pinned(f.img)
+    print("# page-locked %d frames in %.1fs" % (n_frames, time.time() - t1), file=sys.stderr,
+          flush=True)
     print("# generated %d streams x %d frames in %.1fs: %d voxels, cells %d-%d per frame" % (
         len(streams), args.stream_frames, time.time() - t0, n_vox,
         min(f.meta["cells"] for fr in streams for f in fr),

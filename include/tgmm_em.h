@@ -235,6 +235,10 @@ typedef struct tgmm_server_opts {
   double beta_percentage_of_nk;  /* updatePriors arguments */
   double nu_percentage_of_nk;
   double alpha_percentage;
+  int borrow_inputs; /* nonzero: submit keeps pointers to the caller's query,
+                        img, centroids and sv_ptr arrays (which must stay valid
+                        until the ticket is fetched) instead of copying them;
+                        page-locked arrays then upload without blocking */
 } tgmm_server_opts;
 
 int tgmm_server_create(int device, const tgmm_server_opts* opts,
@@ -242,7 +246,8 @@ int tgmm_server_create(int device, const tgmm_server_opts* opts,
 /* Queue a frame. stream_id >= 0 names a time-lapse; when the stream already
  * has device-resident Gaussians and frame->gm is NULL, the priors come from
  * the stream's previous posterior. Returns a ticket >= 0 or an error. Input
- * host arrays are copied before return. */
+ * host arrays are copied before return unless borrow_inputs is set; records
+ * (gm) are always copied. */
 int tgmm_server_submit(tgmm_server* s, int stream_id,
                        const tgmm_frame_desc* frame, long long* ticket);
 /* Runs scheduling steps until the ticket completes (or, with ticket < 0,

@@ -5,9 +5,13 @@
 // One device workspace of fixed capacity, sized from an HBM budget, holds
 // every in-flight frame. Each scheduling step
 //   1. retires frames whose EM has stopped (results copied to the host),
-//   2. admits queued frames whose voxel, supervoxel, Gaussian and grid-cell
-//      ranges fit (first fit), in arrival order, one frame per stream at a time,
-//   3. replays one captured EM iteration over every in-flight frame.
+//   2. stages queued frames whose voxel, supervoxel, Gaussian and grid-cell
+//      ranges fit (first fit, in arrival order): their inputs are uploaded on
+//      a copy stream, so the upload of a stream's next frame overlaps the
+//      iterations of the frames in flight,
+//   3. admits staged frames, one frame per stream at a time (the compute
+//      stream waits for the frame's upload on the device, not on the host),
+//   4. replays one captured EM iteration over every in-flight frame.
 // Frames join and leave at iteration boundaries, so requests that arrive
 // while others are on the GPU start as soon as capacity frees up and their
 // iterations share launches with the frames already running. A stream's
@@ -101,14 +105,22 @@ struct Ticket {
   long long id = 0;
   int stream = -1;
   long long N = 0, S = 0, G = 0;
-  std::vector<float> query, img, cent;
-  std::vector<long long> ptr;
+  // inputs: the caller's arrays (borrowed) or the copies below
+  const float *q = nullptr, *im = nullptr, *ce = nullptr;
+  const long long* pt = nullptr;
+  std::vector<float> q_own, im_own, ce_own;
+  std::vector<long long> pt_own;
   std::vector<GaussianMixtureModelCUDA> gm_in;  // empty: priors from the stream
-  int state = 0;                                // 0 queued, 1 running, 2 done
+  int state = 0;  // 0 queued, 3 staged (ranges held, upload issued), 1 running, 2 done
   double t_submit = 0, t_admit = 0, t_done = 0;
   int slot = -1;
   long long v0 = 0, s0 = 0, sa = 0, g0 = 0, ga = 0, c0 = 0, ncap = 0;
-  bool g_resident = false;
+  bool own_g = false;         // the Gaussian range was allocated for this ticket
+  hipEvent_t up = nullptr;    // the upload has completed on the copy stream
+  std::vector<long long> beg; // staged supervoxel tables and records
+  std::vector<int> len;
+  std::vector<double> gs_in;
+  std::vector<unsigned char> fx_in;
   std::vector<GaussianMixtureModelCUDA> gm_out;
   std::vector<float> rnk;
   std::vector<int> ind;
@@ -128,7 +140,8 @@ struct Stream {
 
 struct tgmm_server {
   int device = 0;
-  hipStream_t st = nullptr;
+  hipStream_t st = nullptr;   // compute
+  hipStream_t cst = nullptr;  // uploads
   tgmm_server_opts o;
   KOpts ko;
   Workspace ws;
@@ -210,7 +223,9 @@ extern "C" int tgmm_server_create(int device, const tgmm_server_opts* opts, tgmm
   P.Sext = kAlign;  // grown to the allocated ranges as frames are admitted
   P.Gext = kAlign;
   P.Cext = kCB;
-  if (hipStreamCreateWithFlags(&s->st, hipStreamNonBlocking) != hipSuccess) {
+  if (hipStreamCreateWithFlags(&s->st, hipStreamNonBlocking) != hipSuccess ||
+      hipStreamCreateWithFlags(&s->cst, hipStreamNonBlocking) != hipSuccess) {
+    if (s->st) (void)hipStreamDestroy(s->st);
     ws_free(&s->ws);
     delete s;
     tgmm_set_error("hipStreamCreate failed");
@@ -261,12 +276,23 @@ extern "C" int tgmm_server_submit(tgmm_server* s, int stream_id, const tgmm_fram
   t.stream = stream_id;
   t.N = d->n_voxels;
   t.S = d->n_supervoxels;
-  t.query.assign(d->query, d->query + 3 * t.N);
-  t.img.assign(d->img, d->img + t.N);
-  t.cent.assign(d->centroids, d->centroids + 3 * t.S);
-  t.ptr.assign(d->sv_ptr, d->sv_ptr + t.S + 1);
+  if (s->o.borrow_inputs) {
+    t.q = d->query;
+    t.im = d->img;
+    t.ce = d->centroids;
+    t.pt = d->sv_ptr;
+  } else {
+    t.q_own.assign(d->query, d->query + 3 * t.N);
+    t.im_own.assign(d->img, d->img + t.N);
+    t.ce_own.assign(d->centroids, d->centroids + 3 * t.S);
+    t.pt_own.assign(d->sv_ptr, d->sv_ptr + t.S + 1);
+    t.q = t.q_own.data();
+    t.im = t.im_own.data();
+    t.ce = t.ce_own.data();
+    t.pt = t.pt_own.data();
+  }
   for (long long j = 0; j < t.S; j++)
-    if (t.ptr[j] < 0 || t.ptr[j + 1] < t.ptr[j] || t.ptr[j + 1] > t.N) {
+    if (t.pt[j] < 0 || t.pt[j + 1] < t.pt[j] || t.pt[j + 1] > t.N) {
       tgmm_set_error("tgmm_server_submit: supervoxel pointer array not monotone in [0, N]");
       return TGMM_ERR_ARG;
     }
@@ -338,48 +364,94 @@ static int map_ranges(tgmm_server* s, const Ticket& t, int f) {
   return TGMM_OK;
 }
 
-static int admit(tgmm_server* s, Ticket& t, int f) {
+static void release_ranges(tgmm_server* s, Ticket& t) {
+  s->av.release(t.v0, t.N);
+  s->as.release(t.s0, t.sa);
+  s->ac.release(t.c0, t.ncap);
+  if (t.own_g) s->ag.release(t.g0, t.ga);
+  t.own_g = false;
+}
+
+// Allocate the ticket's ranges and issue its uploads on the copy stream.
+// Returns TGMM_OK with t.state = 3, or TGMM_ERR_BUSY if a range does not fit.
+static int stage(tgmm_server* s, Ticket& t) {
   DevPtrs& P = s->ws.P;
-  hipStream_t st = s->st;
+  hipStream_t cst = s->cst;
   const long long S = s->ws.sh.Scap, G = s->ws.sh.Gcap;
-  // voxels and supervoxels
-  if (t.N > 0) {
-    HIPCHK(hipMemcpyAsync(P.vx + t.v0, t.query.data(), t.N * 4, hipMemcpyHostToDevice, st));
-    HIPCHK(hipMemcpyAsync(P.vy + t.v0, t.query.data() + t.N, t.N * 4, hipMemcpyHostToDevice, st));
-    HIPCHK(hipMemcpyAsync(P.vz + t.v0, t.query.data() + 2 * t.N, t.N * 4, hipMemcpyHostToDevice,
-                          st));
-    HIPCHK(hipMemcpyAsync(P.vimg + t.v0, t.img.data(), t.N * 4, hipMemcpyHostToDevice, st));
+  t.sa = rup(std::max<long long>(t.S, 1), kAlign);
+  t.ga = rup(t.G + 1, kAlign);  // at least one padding entry (segment end)
+  t.ncap = rup(t.G / 2 + kCB, kCB);
+  t.own_g = false;
+  if (!s->av.alloc(t.N, 1, &t.v0)) return TGMM_ERR_BUSY;
+  if (!s->as.alloc(t.sa, kAlign, &t.s0)) {
+    s->av.release(t.v0, t.N);
+    return TGMM_ERR_BUSY;
   }
-  std::vector<long long> beg(std::max<long long>(t.S, 1));
-  std::vector<int> len(std::max<long long>(t.S, 1));
+  if (!s->ac.alloc(t.ncap, kCB, &t.c0)) {
+    s->av.release(t.v0, t.N);
+    s->as.release(t.s0, t.sa);
+    return TGMM_ERR_BUSY;
+  }
+  if (!t.gm_in.empty()) {  // records given: a range of its own
+    if (!s->ag.alloc(t.ga, kAlign, &t.g0)) {
+      release_ranges(s, t);
+      return TGMM_ERR_BUSY;
+    }
+    t.own_g = true;
+  }
+  if (t.N > 0) {
+    HIPCHK(hipMemcpyAsync(P.vx + t.v0, t.q, t.N * 4, hipMemcpyHostToDevice, cst));
+    HIPCHK(hipMemcpyAsync(P.vy + t.v0, t.q + t.N, t.N * 4, hipMemcpyHostToDevice, cst));
+    HIPCHK(hipMemcpyAsync(P.vz + t.v0, t.q + 2 * t.N, t.N * 4, hipMemcpyHostToDevice, cst));
+    HIPCHK(hipMemcpyAsync(P.vimg + t.v0, t.im, t.N * 4, hipMemcpyHostToDevice, cst));
+  }
+  t.beg.resize(std::max<long long>(t.S, 1));
+  t.len.resize(std::max<long long>(t.S, 1));
   for (long long j = 0; j < t.S; j++) {
-    beg[j] = t.v0 + t.ptr[j];
-    len[j] = (int)(t.ptr[j + 1] - t.ptr[j]);
+    t.beg[j] = t.v0 + t.pt[j];
+    t.len[j] = (int)(t.pt[j + 1] - t.pt[j]);
   }
   if (t.S > 0) {
     for (int a = 0; a < 3; a++)
-      HIPCHK(hipMemcpyAsync(P.sv_cent + a * S + t.s0, t.cent.data() + a * t.S, t.S * 4,
-                            hipMemcpyHostToDevice, st));
-    HIPCHK(hipMemcpyAsync(P.sv_beg + t.s0, beg.data(), t.S * 8, hipMemcpyHostToDevice, st));
-    HIPCHK(hipMemcpyAsync(P.sv_len + t.s0, len.data(), t.S * 4, hipMemcpyHostToDevice, st));
+      HIPCHK(hipMemcpyAsync(P.sv_cent + a * S + t.s0, t.ce + a * t.S, t.S * 4,
+                            hipMemcpyHostToDevice, cst));
+    HIPCHK(hipMemcpyAsync(P.sv_beg + t.s0, t.beg.data(), t.S * 8, hipMemcpyHostToDevice, cst));
+    HIPCHK(hipMemcpyAsync(P.sv_len + t.s0, t.len.data(), t.S * 4, hipMemcpyHostToDevice, cst));
   }
-  // Gaussians: given records, or the stream's posterior turned into priors
-  Stream& sr = s->streams[t.stream];
-  if (!t.gm_in.empty()) {
-    std::vector<double> gs((size_t)GS_NFIELDS * t.G);
-    std::vector<unsigned char> fx(t.G);
-    records_to_soa(t.gm_in.data(), t.G, 0, t.G, gs.data(), fx.data());
+  if (t.own_g) {
+    t.gs_in.resize((size_t)GS_NFIELDS * std::max<long long>(t.G, 1));
+    t.fx_in.resize(std::max<long long>(t.G, 1));
+    records_to_soa(t.gm_in.data(), t.G, 0, t.G, t.gs_in.data(), t.fx_in.data());
     for (int fi = 0; fi < GS_NFIELDS; fi++)
-      HIPCHK(hipMemcpyAsync(P.gs + fi * G + t.g0, gs.data() + fi * t.G, t.G * 8,
-                            hipMemcpyHostToDevice, st));
-    HIPCHK(hipMemcpyAsync(P.fixed + t.g0, fx.data(), t.G, hipMemcpyHostToDevice, st));
-    HIPCHK(hipStreamSynchronize(st));
+      HIPCHK(hipMemcpyAsync(P.gs + fi * G + t.g0, t.gs_in.data() + fi * t.G, t.G * 8,
+                            hipMemcpyHostToDevice, cst));
+    HIPCHK(hipMemcpyAsync(P.fixed + t.g0, t.fx_in.data(), t.G, hipMemcpyHostToDevice, cst));
+  }
+  if (!t.up) HIPCHK(hipEventCreateWithFlags(&t.up, hipEventDisableTiming));
+  HIPCHK(hipEventRecord(t.up, cst));
+  t.state = 3;
+  return TGMM_OK;
+}
+
+// Put a staged ticket into slot f: the compute stream waits for its upload,
+// the stream's resident posterior becomes its priors (or its own records
+// replace the resident state), then the frame's set-up kernels.
+static int activate(tgmm_server* s, Ticket& t, int f) {
+  DevPtrs& P = s->ws.P;
+  hipStream_t st = s->st;
+  Stream& sr = s->streams[t.stream];
+  HIPCHK(hipStreamWaitEvent(st, t.up, 0));
+  if (t.own_g) {
+    if (t.stream >= 0 && sr.g0 >= 0) {  // new records replace the resident state
+      s->ag.release(sr.g0, sr.ga);
+      sr.g0 = -1;
+    }
   } else {
+    t.g0 = sr.g0;
     HIPCHK(launch_next_priors(P, t.g0, t.G, s->o.beta_percentage_of_nk,
                               s->o.nu_percentage_of_nk, s->o.alpha_percentage, sr.alpha_total,
                               st));
   }
-  // frame descriptor
   FrameDev& F = s->fh[f];
   std::memset(&F, 0, sizeof(F));
   F.v0 = t.v0;
@@ -412,23 +484,40 @@ static int admit(tgmm_server* s, Ticket& t, int f) {
   return TGMM_OK;
 }
 
-static int try_admit(tgmm_server* s) {
-  // streams with a frame in flight or an earlier frame still queued; frames of
-  // one stream run one after another (each needs the previous posterior),
-  // frames of other streams may pass them
-  std::set<int> blocked;
-  for (const auto& kv : s->streams)
-    if (kv.second.busy) blocked.insert(kv.first);
+// Stage queued frames (a stream's frames strictly in order, at most its next
+// one staged ahead), then admit staged frames whose stream is free, in
+// arrival order.
+static int schedule(tgmm_server* s) {
+  std::set<int> ahead;  // streams whose earliest unfinished frame was passed
+  int staged = 0;
+  for (auto it = s->queue.begin(); it != s->queue.end(); ++it) {
+    Ticket& t = s->tickets[*it];
+    if (t.state == 3) {
+      staged++;
+      if (t.stream >= 0) ahead.insert(t.stream);
+      continue;
+    }
+    if (t.stream >= 0 && ahead.count(t.stream)) continue;  // not this stream's next
+    if (t.stream >= 0) ahead.insert(t.stream);
+    if (staged >= s->ws.sh.Fcap) continue;
+    const int rc = stage(s, t);
+    if (rc == TGMM_OK) {
+      staged++;
+    } else if (rc != TGMM_ERR_BUSY) {
+      return rc;
+    }
+  }
   for (auto it = s->queue.begin(); it != s->queue.end();) {
     if (s->in_flight >= s->ws.sh.Fcap) break;
-    if (s->anon_queued == 0 && blocked.size() >= s->streams.size()) break;
     Ticket& t = s->tickets[*it];
-    Stream& sr = s->streams[t.stream];
-    if (t.stream >= 0 && blocked.count(t.stream)) {
+    if (t.state != 3 || (t.stream >= 0 && s->streams[t.stream].busy)) {
       ++it;
       continue;
     }
-    if (t.stream >= 0) blocked.insert(t.stream);  // later frames of it wait
+    if (!t.own_g && s->streams[t.stream].g0 < 0) {  // no resident state (cannot happen)
+      ++it;
+      continue;
+    }
     int f = -1;
     for (int i = 0; i < (int)s->slot_ticket.size(); i++)
       if (s->slot_ticket[i] < 0) {
@@ -436,44 +525,7 @@ static int try_admit(tgmm_server* s) {
         break;
       }
     if (f < 0) break;
-    t.sa = rup(std::max<long long>(t.S, 1), kAlign);
-    t.ga = rup(t.G + 1, kAlign);  // at least one padding entry (segment end)
-    t.ncap = rup(t.G / 2 + kCB, kCB);
-    const bool reuse_g = t.gm_in.empty();  // priors from the resident range
-    long long v0, s0, g0 = 0, c0;
-    if (!s->av.alloc(t.N, 1, &v0)) {
-      ++it;
-      continue;
-    }
-    if (!s->as.alloc(t.sa, kAlign, &s0)) {
-      s->av.release(v0, t.N);
-      ++it;
-      continue;
-    }
-    if (!s->ac.alloc(t.ncap, kCB, &c0)) {
-      s->av.release(v0, t.N);
-      s->as.release(s0, t.sa);
-      ++it;
-      continue;
-    }
-    if (reuse_g) {
-      g0 = sr.g0;
-    } else if (!s->ag.alloc(t.ga, kAlign, &g0)) {
-      s->av.release(v0, t.N);
-      s->as.release(s0, t.sa);
-      s->ac.release(c0, t.ncap);
-      ++it;
-      continue;
-    }
-    if (!reuse_g && t.stream >= 0 && sr.g0 >= 0) {  // new records replace the resident state
-      s->ag.release(sr.g0, sr.ga);
-      sr.g0 = -1;
-    }
-    t.v0 = v0;
-    t.s0 = s0;
-    t.g0 = g0;
-    t.c0 = c0;
-    if (int rc = admit(s, t, f)) return rc;
+    if (int rc = activate(s, t, f)) return rc;
     if (t.stream < 0) s->anon_queued--;
     it = s->queue.erase(it);
   }
@@ -544,10 +596,9 @@ static int retire(tgmm_server* s, int f) {
   sr.busy = false;
   s->slot_ticket[f] = -1;
   s->in_flight--;
-  t.query.clear();
-  t.query.shrink_to_fit();
-  t.img.clear();
-  t.img.shrink_to_fit();
+  std::vector<float>().swap(t.q_own);
+  std::vector<float>().swap(t.im_own);
+  std::vector<double>().swap(t.gs_in);
   return TGMM_OK;
 }
 
@@ -560,7 +611,7 @@ extern "C" int tgmm_server_step(tgmm_server* s, int* pending) {
     if (s->slot_ticket[f] >= 0 && !s->fh[f].active)
       if (int rc = retire(s, f)) return rc;
   const double t1 = now_s();
-  if (int rc = try_admit(s)) return rc;
+  if (int rc = schedule(s)) return rc;
   if (int rc = update_extents(s)) return rc;
   const double t2 = now_s();
   if (s->in_flight > 0) {
@@ -591,11 +642,12 @@ extern "C" int tgmm_server_wait(tgmm_server* s, long long ticket) {
     if (ticket >= 0 && s->tickets[ticket].state == 2) return TGMM_OK;
     if (int rc = tgmm_server_step(s, &pending)) return rc;
     if (ticket < 0 && pending == 0) return TGMM_OK;
-    if (ticket >= 0 && s->tickets[ticket].state == 0 && s->in_flight == 0) {
-      // nothing in flight and the ticket cannot be admitted: it never fits
+    if (ticket >= 0 && s->tickets[ticket].state != 1 && s->in_flight == 0) {
+      // nothing in flight and the ticket was not admitted: it never fits
       int again = 0;
       if (int rc = tgmm_server_step(s, &again)) return rc;
-      if (s->tickets[ticket].state == 0 && s->in_flight == 0) {
+      if (s->tickets[ticket].state != 1 && s->tickets[ticket].state != 2 &&
+          s->in_flight == 0) {
         tgmm_set_error("ticket %lld cannot be admitted (capacity)", ticket);
         return TGMM_ERR_BUSY;
       }
@@ -608,7 +660,8 @@ extern "C" int tgmm_server_query(tgmm_server* s, long long ticket, int* state) {
     tgmm_set_error("tgmm_server_query: bad arguments");
     return TGMM_ERR_ARG;
   }
-  *state = s->tickets[ticket].state;
+  const int st = s->tickets[ticket].state;
+  *state = (st == 3) ? 0 : st;  // staged frames are still queued
   return TGMM_OK;
 }
 
@@ -633,6 +686,7 @@ extern "C" int tgmm_server_fetch(tgmm_server* s, long long ticket, tgmm_frame_de
   if (t_submit) *t_submit = t.t_submit;
   if (t_admit) *t_admit = t.t_admit;
   if (t_done) *t_done = t.t_done;
+  if (t.up) (void)hipEventDestroy(t.up);
   s->tickets.erase(ticket);
   return TGMM_OK;
 }
@@ -676,11 +730,15 @@ extern "C" int tgmm_server_set_tiles(tgmm_server* s, const tgmm_tiles* t) {
 extern "C" int tgmm_server_destroy(tgmm_server* s) {
   if (!s) return TGMM_ERR_ARG;
   (void)hipSetDevice(s->device);
+  if (s->cst) (void)hipStreamSynchronize(s->cst);
   if (s->st) (void)hipStreamSynchronize(s->st);
   if (s->gexec) (void)hipGraphExecDestroy(s->gexec);
   if (s->graph) (void)hipGraphDestroy(s->graph);
+  for (auto& kv : s->tickets)
+    if (kv.second.up) (void)hipEventDestroy(kv.second.up);
   ws_free(&s->ws);
   if (s->st) (void)hipStreamDestroy(s->st);
+  if (s->cst) (void)hipStreamDestroy(s->cst);
   delete s;
   return TGMM_OK;
 }

@@ -94,6 +94,7 @@ class ServerOpts(ctypes.Structure):
         ("beta_percentage_of_nk", ctypes.c_double),
         ("nu_percentage_of_nk", ctypes.c_double),
         ("alpha_percentage", ctypes.c_double),
+        ("borrow_inputs", ctypes.c_int),
     ]
 
 
@@ -182,6 +183,34 @@ def load():
     L.tgmm_server_destroy.argtypes = [_vp]
     _lib = L
     return L
+
+
+_hip = None
+
+
+def pinned(a):
+    """A page-locked (hipHostMalloc) copy of numpy array `a`, or `a` itself if
+    page-locked memory cannot be had. The memory is freed with the array."""
+    global _hip
+    import weakref
+    a = np.ascontiguousarray(a)
+    try:
+        if _hip is None:
+            _hip = ctypes.CDLL("libamdhip64.so")
+            _hip.hipHostMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t,
+                                           ctypes.c_uint]
+            _hip.hipHostFree.argtypes = [ctypes.c_void_p]
+        p = ctypes.c_void_p()
+        if _hip.hipHostMalloc(ctypes.byref(p), ctypes.c_size_t(max(a.nbytes, 1)), 0) != 0:
+            return a
+    except OSError:
+        return a
+    buf = (ctypes.c_char * max(a.nbytes, 1)).from_address(p.value)
+    out = np.frombuffer(buf, dtype=a.dtype, count=a.size).reshape(a.shape)
+    out[...] = a
+    # freed when the buffer object goes, i.e. after the last array viewing it
+    weakref.finalize(buf, _hip.hipHostFree, ctypes.c_void_p(p.value))
+    return out
 
 
 def _tiles_cache_path():
@@ -442,9 +471,13 @@ class Server:
     (tgmm_server_* in include/tgmm_em.h)."""
 
     def __init__(self, opts, hbm_budget_bytes=8 << 30, max_frames_in_flight=16,
-                 beta_pct=0.5, nu_pct=0.5, alpha_pct=0.8, device=0):
+                 beta_pct=0.5, nu_pct=0.5, alpha_pct=0.8, device=0, borrow=False):
+        """borrow=True: the server reads the frames' arrays in place (they are
+        kept referenced here until fetched); page-locked arrays (pinned())
+        then upload without blocking the host."""
         self.L = load()
         self.opts = opts
+        self.borrow = bool(borrow)
         so = ServerOpts()
         so.em = opts
         so.hbm_budget_bytes = int(hbm_budget_bytes)
@@ -452,9 +485,11 @@ class Server:
         so.beta_percentage_of_nk = beta_pct
         so.nu_percentage_of_nk = nu_pct
         so.alpha_percentage = alpha_pct
+        so.borrow_inputs = 1 if self.borrow else 0
         self.h = ctypes.c_void_p()
         _check(self.L.tgmm_server_create(device, ctypes.byref(so), ctypes.byref(self.h)))
         self.shapes = {}
+        self._held = {}
 
     def submit(self, frame, stream=-1, with_gaussians=True):
         """Queue a frame (host arrays are copied); returns a ticket. With
@@ -477,6 +512,8 @@ class Server:
         t = ctypes.c_longlong()
         _check(self.L.tgmm_server_submit(self.h, int(stream), ctypes.byref(d), ctypes.byref(t)))
         self.shapes[t.value] = int(d.n_supervoxels)
+        if self.borrow:
+            self._held[t.value] = (q, im, ce, pt)
         return t.value
 
     def set_tiles(self, tiles):
@@ -504,7 +541,7 @@ class Server:
         return st.value
 
     def fetch(self, ticket, n_gaussians):
-        S = self.shapes.pop(ticket)
+        S = self.shapes[ticket]
         K = self.opts.K
         gm = np.zeros(int(n_gaussians), GMM_DTYPE)
         rnk = np.zeros(K * S, np.float32)
@@ -514,6 +551,8 @@ class Server:
         ts, ta, td = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
         _check(self.L.tgmm_server_fetch(self.h, int(ticket), ctypes.byref(d), ctypes.byref(ts),
                                         ctypes.byref(ta), ctypes.byref(td)))
+        del self.shapes[ticket]
+        self._held.pop(ticket, None)
         return dict(gm=gm, rnk=rnk, ind=ind, ll=d.ll, iterations=d.iterations,
                     t_submit=ts.value, t_admit=ta.value, t_done=td.value)
 
