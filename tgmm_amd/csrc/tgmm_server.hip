@@ -4,7 +4,11 @@
 //
 // One device workspace of fixed capacity, sized from an HBM budget, holds
 // every in-flight frame. Each scheduling step
-//   1. retires frames whose EM has stopped (results copied to the host),
+//   1. retires frames whose EM has stopped: a kernel gathers each one's
+//      results into a staging block that is copied back to page-locked host
+//      memory in one transfer, in stream order, while the step goes on; the
+//      ranges are released at once (later uploads wait for those copies) and
+//      the ticket is done when its copy has landed,
 //   2. stages queued frames whose voxel, supervoxel, Gaussian and grid-cell
 //      ranges fit (first fit, in arrival order): their inputs are uploaded on
 //      a copy stream, so the upload of a stream's next frame overlaps the
@@ -101,6 +105,48 @@ class RangeAlloc {
   long long hwm_ = 0;
 };
 
+// Recycled result buffers: a device staging area and its page-locked host
+// mirror, of the same power-of-two capacity.
+struct ResBuf {
+  void* h = nullptr;
+  void* d = nullptr;
+  size_t cap = 0;
+};
+
+class ResPool {
+ public:
+  ~ResPool() {
+    for (auto& kv : free_) {
+      (void)hipHostFree(kv.second.h);
+      (void)hipFree(kv.second.d);
+    }
+  }
+  bool get(size_t bytes, ResBuf* r) {
+    size_t c = 1 << 16;
+    while (c < bytes) c <<= 1;
+    auto it = free_.find(c);
+    if (it != free_.end()) {
+      *r = it->second;
+      free_.erase(it);
+      return true;
+    }
+    r->cap = c;
+    if (hipHostMalloc(&r->h, c, 0) != hipSuccess) return false;
+    if (hipMalloc(&r->d, c) != hipSuccess) {
+      (void)hipHostFree(r->h);
+      return false;
+    }
+    return true;
+  }
+  void put(ResBuf& r) {
+    if (r.h) free_.emplace(r.cap, r);
+    r = ResBuf{};
+  }
+
+ private:
+  std::multimap<size_t, ResBuf> free_;
+};
+
 struct Ticket {
   long long id = 0;
   int stream = -1;
@@ -111,7 +157,8 @@ struct Ticket {
   std::vector<float> q_own, im_own, ce_own;
   std::vector<long long> pt_own;
   std::vector<GaussianMixtureModelCUDA> gm_in;  // empty: priors from the stream
-  int state = 0;  // 0 queued, 3 staged (ranges held, upload issued), 1 running, 2 done
+  int state = 0;  // 0 queued, 3 staged (ranges held, upload issued), 1 running,
+                  // 4 retiring (results being copied back), 2 done
   double t_submit = 0, t_admit = 0, t_done = 0;
   int slot = -1;
   long long v0 = 0, s0 = 0, sa = 0, g0 = 0, ga = 0, c0 = 0, ncap = 0;
@@ -121,9 +168,10 @@ struct Ticket {
   std::vector<int> len;
   std::vector<double> gs_in;
   std::vector<unsigned char> fx_in;
-  std::vector<GaussianMixtureModelCUDA> gm_out;
-  std::vector<float> rnk;
-  std::vector<int> ind;
+  // results: copied into a page-locked buffer on the compute stream at
+  // retirement (state 4 until `down` completes), turned into records at fetch
+  ResBuf res;
+  hipEvent_t down = nullptr;
   double ll = 0;
   int iters = 0;
   int status = TGMM_OK;
@@ -156,6 +204,10 @@ struct tgmm_server {
   long long next_id = 0;
   int in_flight = 0;
   long long anon_queued = 0;  // queued frames with no stream
+  ResPool pool;
+  std::vector<long long> retiring;
+  hipEvent_t released = nullptr;  // recorded after the last retirement's copies
+  bool release_pending = false;   // uploads must wait for `released`
   // cumulative host seconds: submit, admit, iterate (launch + wait), retire
   double t_phase[4] = {0, 0, 0, 0};
   long long n_steps = 0, n_iter_frames = 0;
@@ -342,25 +394,36 @@ static int update_extents(tgmm_server* s) {
   return TGMM_OK;
 }
 
-// Map the frame's ranges to slot f (or -1) in the per-element frame tables.
-static int map_ranges(tgmm_server* s, const Ticket& t, int f) {
+// One per-element frame table range: n entries from p, the first `valid` of
+// them set to the slot and the rest to -1.
+struct MapSeg {
+  int* p;
+  long long n, valid;
+};
+
+__global__ void __launch_bounds__(256) k_map_ranges(MapSeg s0, MapSeg s1, MapSeg s2, MapSeg s3,
+                                                    MapSeg s4, int f) {
+  const MapSeg m = blockIdx.y == 0 ? s0 : blockIdx.y == 1 ? s1 : blockIdx.y == 2 ? s2
+                   : blockIdx.y == 3 ? s3 : s4;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < m.n;
+       i += (long long)gridDim.x * blockDim.x)
+    m.p[i] = i < m.valid ? f : -1;
+}
+
+// Map the frame's ranges to slot f (or -1) in the per-element frame tables
+// (on the compute stream, so ordered with every kernel that reads them).
+static int map_ranges(tgmm_server* s, Ticket& t, int f) {
   DevPtrs& P = s->ws.P;
-  hipStream_t st = s->st;
-  std::vector<int> sv(t.sa, f), sb(t.sa / kAlign, f), gv(t.ga, f), gb(t.ga / kGB, f),
-      cb(t.ncap / kCB, f);
-  if (f >= 0) {
-    for (long long j = t.S; j < t.sa; j++) sv[j] = -1;
-    for (long long j = t.G; j < t.ga; j++) gv[j] = -1;
-  }
-  HIPCHK(hipMemcpyAsync(P.sv_frame + t.s0, sv.data(), t.sa * 4, hipMemcpyHostToDevice, st));
-  HIPCHK(hipMemcpyAsync(P.sblk_frame + t.s0 / kAlign, sb.data(), sb.size() * 4,
-                        hipMemcpyHostToDevice, st));
-  HIPCHK(hipMemcpyAsync(P.g_frame + t.g0, gv.data(), t.ga * 4, hipMemcpyHostToDevice, st));
-  HIPCHK(hipMemcpyAsync(P.gblk_frame + t.g0 / kGB, gb.data(), gb.size() * 4,
-                        hipMemcpyHostToDevice, st));
-  HIPCHK(hipMemcpyAsync(P.cblk_frame + t.c0 / kCB, cb.data(), cb.size() * 4,
-                        hipMemcpyHostToDevice, st));
-  HIPCHK(hipStreamSynchronize(st));  // host vectors go out of scope
+  const long long vs = f >= 0 ? t.S : 0, vg = f >= 0 ? t.G : 0;
+  MapSeg a{P.sv_frame + t.s0, t.sa, vs};
+  MapSeg b{P.sblk_frame + t.s0 / kAlign, t.sa / kAlign, f >= 0 ? t.sa / kAlign : 0};
+  MapSeg c{P.g_frame + t.g0, t.ga, vg};
+  MapSeg d{P.gblk_frame + t.g0 / kGB, t.ga / kGB, f >= 0 ? t.ga / kGB : 0};
+  MapSeg e{P.cblk_frame + t.c0 / kCB, t.ncap / kCB, f >= 0 ? t.ncap / kCB : 0};
+  const long long mx = std::max(std::max(t.sa, t.ga), t.ncap / kCB);
+  const unsigned gx = (unsigned)std::max(1LL, std::min(1024LL, (mx + 255) / 256));
+  k_map_ranges<<<dim3(gx, 5), 256, 0, s->st>>>(a, b, c, d, e, f);
+  HIPCHK(hipGetLastError());
   return TGMM_OK;
 }
 
@@ -398,6 +461,10 @@ static int stage(tgmm_server* s, Ticket& t) {
       return TGMM_ERR_BUSY;
     }
     t.own_g = true;
+  }
+  if (s->release_pending) {  // ranges released by retirements are read back first
+    HIPCHK(hipStreamWaitEvent(cst, s->released, 0));
+    s->release_pending = false;
   }
   if (t.N > 0) {
     HIPCHK(hipMemcpyAsync(P.vx + t.v0, t.q, t.N * 4, hipMemcpyHostToDevice, cst));
@@ -532,6 +599,47 @@ static int schedule(tgmm_server* s) {
   return TGMM_OK;
 }
 
+// A retired frame's results gathered into one contiguous block: the Gaussian
+// fields [GS_NFIELDS][G] (f64), the fixed flags [G rounded up to 8] (u8), the
+// responsibilities [K][S] (f32) and the candidate indices [K][S] (i32, made
+// frame-local; -1 stays -1).
+__global__ void __launch_bounds__(256) k_gather_results(const double* __restrict__ gs,
+    const unsigned char* __restrict__ fixed, const float* __restrict__ rnk,
+    const int* __restrict__ ind, long long Gcap, long long Scap, long long g0, long long G,
+    long long s0, long long S, int K, char* __restrict__ out) {
+  const long long gsb = (long long)GS_NFIELDS * G * 8, fxb = (G + 7) / 8 * 8, rb = K * S * 4;
+  const long long step = (long long)gridDim.x * blockDim.x;
+  const long long i0 = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  switch (blockIdx.y) {
+    case 0:
+      for (long long i = i0; i < GS_NFIELDS * G; i += step) {
+        const long long f = i / G, k = i - f * G;
+        ((double*)out)[i] = gs[f * Gcap + g0 + k];
+      }
+      break;
+    case 1:
+      for (long long i = i0; i < G; i += step) ((unsigned char*)(out + gsb))[i] = fixed[g0 + i];
+      break;
+    case 2:
+      for (long long i = i0; i < K * S; i += step) {
+        const long long j = i / S, v = i - j * S;
+        ((float*)(out + gsb + fxb))[i] = rnk[j * Scap + s0 + v];
+      }
+      break;
+    default:
+      for (long long i = i0; i < K * S; i += step) {
+        const long long j = i / S, v = i - j * S;
+        const int k = ind[j * Scap + s0 + v];
+        ((int*)(out + gsb + fxb + rb))[i] = k >= 0 ? k - (int)g0 : k;
+      }
+  }
+}
+
+// Retire the frame of slot f: its results are copied into a page-locked
+// buffer on the compute stream (ordered before any later use of its ranges
+// there; uploads on the copy stream wait for `released`), its ranges are
+// unmapped and released, and the stream keeps its Gaussian range for the
+// next frame. The ticket is done once the copies have landed.
 static int retire(tgmm_server* s, int f) {
   const long long id = s->slot_ticket[f];
   Ticket& t = s->tickets[id];
@@ -539,48 +647,32 @@ static int retire(tgmm_server* s, int f) {
   hipStream_t st = s->st;
   const long long S = s->ws.sh.Scap, G = s->ws.sh.Gcap;
   const int K = s->ws.K;
-  std::vector<double> gs((size_t)GS_NFIELDS * std::max<long long>(t.G, 1));
-  for (int fi = 0; fi < GS_NFIELDS; fi++)
-    HIPCHK(hipMemcpyAsync(gs.data() + fi * t.G, P.gs + fi * G + t.g0, t.G * 8,
-                          hipMemcpyDeviceToHost, st));
-  std::vector<unsigned char> fx(std::max<long long>(t.G, 1));
-  HIPCHK(hipMemcpyAsync(fx.data(), P.fixed + t.g0, t.G, hipMemcpyDeviceToHost, st));
-  t.rnk.resize((size_t)K * t.S);
-  t.ind.resize((size_t)K * t.S);
-  if (t.S > 0) {
-    HIPCHK(hipMemcpy2DAsync(t.rnk.data(), t.S * 4, P.rnk + t.s0, S * 4, t.S * 4, K,
-                            hipMemcpyDeviceToHost, st));
-    HIPCHK(hipMemcpy2DAsync(t.ind.data(), t.S * 4, P.ind + t.s0, S * 4, t.S * 4, K,
-                            hipMemcpyDeviceToHost, st));
+  const size_t gsb = (size_t)GS_NFIELDS * t.G * 8, fxb = rup(t.G, 8),
+               rb = (size_t)K * t.S * 4;
+  const size_t bytes = gsb + fxb + 2 * rb;
+  if (!s->pool.get(bytes + 64, &t.res)) {
+    tgmm_set_error("result buffers of %zu bytes not available", bytes);
+    return TGMM_ERR_OOM;
   }
-  HIPCHK(hipStreamSynchronize(st));
-  t.gm_out.resize(t.G);
-  soa_to_records(gs.data(), t.G, 0, t.G, t.gm_out.data());
-  for (long long k = 0; k < t.G; k++) {
-    t.gm_out[k].fixed = fx[k];
-    t.gm_out[k].supervoxelNum = 0;
-  }
-  const int cap = s->o.em.max_sv_per_gaussian;
-  for (int j = 0; j < K; j++)
-    for (long long sv = 0; sv < t.S; sv++) {
-      int& k = t.ind[j * t.S + sv];
-      if (k >= 0) k -= (int)t.g0;
-      if (t.rnk[j * t.S + sv] > 0.5f && k >= 0) {
-        GaussianMixtureModelCUDA& g = t.gm_out[k];
-        if (g.supervoxelNum < cap) g.supervoxelIdx[g.supervoxelNum++] = (int)sv;
-      }
-    }
+  const long long mx = std::max<long long>((long long)GS_NFIELDS * t.G, (long long)K * t.S);
+  const unsigned gx = (unsigned)std::max(1LL, std::min(2048LL, (mx + 255) / 256));
+  k_gather_results<<<dim3(gx, 4), 256, 0, st>>>(P.gs, P.fixed, P.rnk, P.ind, G, S, t.g0, t.G,
+                                                 t.s0, t.S, K, (char*)t.res.d);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(t.res.h, t.res.d, bytes, hipMemcpyDeviceToHost, st));
+  if (!t.down) HIPCHK(hipEventCreateWithFlags(&t.down, hipEventDisableTiming));
+  HIPCHK(hipEventRecord(t.down, st));
   const FrameDev F = s->fh[f];  // a copy: the slot's entry is cleared below
   t.ll = F.ll;
   t.iters = F.iter - 1;
-  t.state = 2;
-  t.t_done = now_s();
-  // unmap and release; the stream keeps its Gaussian range for the next frame
+  t.state = 4;
+  s->retiring.push_back(t.id);
+  // unmap and release
   if (int rc = map_ranges(s, t, -1)) return rc;
   FrameDev z;
   std::memset(&z, 0, sizeof(z));
   s->fh[f] = z;
-  HIPCHK(hipMemcpyAsync(P.fr + f, &z, sizeof(FrameDev), hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(P.fr + f, &s->fh[f], sizeof(FrameDev), hipMemcpyHostToDevice, st));
   s->av.release(t.v0, t.N);
   s->as.release(t.s0, t.sa);
   s->ac.release(t.c0, t.ncap);
@@ -596,9 +688,33 @@ static int retire(tgmm_server* s, int f) {
   sr.busy = false;
   s->slot_ticket[f] = -1;
   s->in_flight--;
-  std::vector<float>().swap(t.q_own);
-  std::vector<float>().swap(t.im_own);
-  std::vector<double>().swap(t.gs_in);
+  s->release_pending = true;
+  return TGMM_OK;
+}
+
+// Retiring tickets whose results have landed become done.
+static int collect(tgmm_server* s, bool block) {
+  for (size_t i = 0; i < s->retiring.size();) {
+    Ticket& t = s->tickets[s->retiring[i]];
+    hipError_t q = block ? hipEventSynchronize(t.down) : hipEventQuery(t.down);
+    if (q == hipErrorNotReady) {
+      ++i;
+      continue;
+    }
+    HIPCHK(q);
+    t.state = 2;
+    t.t_done = now_s();
+    std::vector<float>().swap(t.q_own);
+    std::vector<float>().swap(t.im_own);
+    std::vector<float>().swap(t.ce_own);
+    std::vector<long long>().swap(t.pt_own);
+    std::vector<GaussianMixtureModelCUDA>().swap(t.gm_in);
+    std::vector<long long>().swap(t.beg);
+    std::vector<int>().swap(t.len);
+    std::vector<double>().swap(t.gs_in);
+    std::vector<unsigned char>().swap(t.fx_in);
+    s->retiring.erase(s->retiring.begin() + i);
+  }
   return TGMM_OK;
 }
 
@@ -610,6 +726,10 @@ extern "C" int tgmm_server_step(tgmm_server* s, int* pending) {
   for (int f = 0; f < (int)s->slot_ticket.size(); f++)
     if (s->slot_ticket[f] >= 0 && !s->fh[f].active)
       if (int rc = retire(s, f)) return rc;
+  if (s->release_pending) {
+    if (!s->released) HIPCHK(hipEventCreateWithFlags(&s->released, hipEventDisableTiming));
+    HIPCHK(hipEventRecord(s->released, s->st));
+  }
   const double t1 = now_s();
   if (int rc = schedule(s)) return rc;
   if (int rc = update_extents(s)) return rc;
@@ -627,7 +747,9 @@ extern "C" int tgmm_server_step(tgmm_server* s, int* pending) {
   s->t_phase[3] += t1 - t0;
   s->t_phase[1] += t2 - t1;
   s->t_phase[2] += t3 - t2;
-  if (pending) *pending = s->in_flight + (int)s->queue.size();
+  // results that have landed; with nothing left to run, wait for them
+  if (int rc = collect(s, s->in_flight == 0)) return rc;
+  if (pending) *pending = s->in_flight + (int)s->queue.size() + (int)s->retiring.size();
   return TGMM_OK;
 }
 
@@ -642,12 +764,13 @@ extern "C" int tgmm_server_wait(tgmm_server* s, long long ticket) {
     if (ticket >= 0 && s->tickets[ticket].state == 2) return TGMM_OK;
     if (int rc = tgmm_server_step(s, &pending)) return rc;
     if (ticket < 0 && pending == 0) return TGMM_OK;
-    if (ticket >= 0 && s->tickets[ticket].state != 1 && s->in_flight == 0) {
+    const int ts = ticket >= 0 ? s->tickets[ticket].state : 0;
+    if (ticket >= 0 && (ts == 0 || ts == 3) && s->in_flight == 0) {
       // nothing in flight and the ticket was not admitted: it never fits
       int again = 0;
       if (int rc = tgmm_server_step(s, &again)) return rc;
-      if (s->tickets[ticket].state != 1 && s->tickets[ticket].state != 2 &&
-          s->in_flight == 0) {
+      const int t2 = s->tickets[ticket].state;
+      if ((t2 == 0 || t2 == 3) && s->in_flight == 0) {
         tgmm_set_error("ticket %lld cannot be admitted (capacity)", ticket);
         return TGMM_ERR_BUSY;
       }
@@ -661,7 +784,7 @@ extern "C" int tgmm_server_query(tgmm_server* s, long long ticket, int* state) {
     return TGMM_ERR_ARG;
   }
   const int st = s->tickets[ticket].state;
-  *state = (st == 3) ? 0 : st;  // staged frames are still queued
+  *state = (st == 3) ? 0 : (st == 4) ? 1 : st;  // staged: queued; retiring: running
   return TGMM_OK;
 }
 
@@ -676,9 +799,35 @@ extern "C" int tgmm_server_fetch(tgmm_server* s, long long ticket, tgmm_frame_de
     tgmm_set_error("ticket %lld not finished", ticket);
     return TGMM_ERR_BUSY;
   }
-  if (d->gm) std::memcpy(d->gm, t.gm_out.data(), t.gm_out.size() * sizeof(GaussianMixtureModelCUDA));
-  if (d->rnk) std::memcpy(d->rnk, t.rnk.data(), t.rnk.size() * 4);
-  if (d->ind) std::memcpy(d->ind, t.ind.data(), t.ind.size() * 4);
+  const int K = s->ws.K;
+  const size_t gsb = (size_t)GS_NFIELDS * t.G * 8, fxb = rup(t.G, 8),
+               rb = (size_t)K * t.S * 4;
+  const char* base = (const char*)t.res.h;
+  const unsigned char* fx = (const unsigned char*)(base + gsb);
+  const float* rnk = (const float*)(base + gsb + fxb);
+  const int* ind = (const int*)(base + gsb + fxb + rb);
+  std::vector<GaussianMixtureModelCUDA> gm(std::max<long long>(t.G, 1));
+  soa_to_records((const double*)base, t.G, 0, t.G, gm.data());
+  for (long long k = 0; k < t.G; k++) {
+    gm[k].fixed = fx[k];
+    gm[k].supervoxelNum = 0;
+  }
+  // supervoxels assigned with r > 0.5, slot-major, at most max_sv_per_gaussian
+  // (GMEM.cu:4328-4350)
+  const int cap = s->o.em.max_sv_per_gaussian;
+  for (int j = 0; j < K; j++)
+    for (long long sv = 0; sv < t.S; sv++) {
+      const int k = ind[j * t.S + sv];
+      if (rnk[j * t.S + sv] > 0.5f && k >= 0) {
+        GaussianMixtureModelCUDA& g = gm[k];
+        if (g.supervoxelNum < cap) g.supervoxelIdx[g.supervoxelNum++] = (int)sv;
+      }
+    }
+  if (d->gm) std::memcpy(d->gm, gm.data(), t.G * sizeof(GaussianMixtureModelCUDA));
+  if (d->rnk) std::memcpy(d->rnk, rnk, rb);
+  if (d->ind) std::memcpy(d->ind, ind, rb);
+  s->pool.put(t.res);
+  if (t.down) (void)hipEventDestroy(t.down);
   d->ll = t.ll;
   d->iterations = t.iters;
   d->status = t.status;
@@ -734,8 +883,12 @@ extern "C" int tgmm_server_destroy(tgmm_server* s) {
   if (s->st) (void)hipStreamSynchronize(s->st);
   if (s->gexec) (void)hipGraphExecDestroy(s->gexec);
   if (s->graph) (void)hipGraphDestroy(s->graph);
-  for (auto& kv : s->tickets)
+  for (auto& kv : s->tickets) {
     if (kv.second.up) (void)hipEventDestroy(kv.second.up);
+    if (kv.second.down) (void)hipEventDestroy(kv.second.down);
+    s->pool.put(kv.second.res);
+  }
+  if (s->released) (void)hipEventDestroy(s->released);
   ws_free(&s->ws);
   if (s->st) (void)hipStreamDestroy(s->st);
   if (s->cst) (void)hipStreamDestroy(s->cst);
