@@ -379,8 +379,11 @@ def serve_bench(args):
     pct = (0.5, 0.5, 0.8)
 
     # launch shapes: autotuned on a batch of the streams' first frames
-    with em.Batch([fr[0] for fr in streams], opts) as tb:
-        tiles, _ = tb.autotune(reps=3, cache=False)
+    if args.no_autotune:
+        tiles = dict(em.DEFAULT_TILES)
+    else:
+        with em.Batch([fr[0] for fr in streams], opts) as tb:
+            tiles, _ = tb.autotune(reps=3, cache=False)
     print("# autotuned tiles %s" % json.dumps(tiles), file=sys.stderr, flush=True)
 
     # warm-up: two frames per stream on a throw-away server

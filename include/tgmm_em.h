@@ -264,15 +264,17 @@ int tgmm_server_query(tgmm_server* s, long long ticket, int* state);
  * completion wall times (seconds, monotonic clock). */
 int tgmm_server_fetch(tgmm_server* s, long long ticket, tgmm_frame_desc* frame,
                       double* t_submit, double* t_admit, double* t_done);
-/* Cumulative host-side accounting of a server: seconds spent in submit
- * (input copies), admission (uploads, prior update, per-frame set-up),
- * iteration (graph launch and wait) and retirement (results to the host);
- * scheduling steps that ran an iteration, frame-iterations run, frames in
- * flight and queued now. */
+/* Cumulative accounting of a server: host seconds spent in submit (input
+ * copies), admission (uploads, prior update, per-frame set-up), iteration
+ * (graph launch and wait) and retirement (results gathered and queued for
+ * copy-back); scheduling steps that ran an iteration, frame-iterations run,
+ * frames in flight and queued now; device seconds of the iteration graph
+ * replays (HIP events around each replay). */
 typedef struct tgmm_server_stats_t {
   double submit_s, admit_s, iterate_s, retire_s;
   long long steps, frame_iterations;
   long long in_flight, queued;
+  double iterate_gpu_s;
 } tgmm_server_stats_t;
 int tgmm_server_stats(tgmm_server* s, tgmm_server_stats_t* stats);
 

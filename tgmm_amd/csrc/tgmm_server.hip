@@ -211,6 +211,11 @@ struct tgmm_server {
   // cumulative host seconds: submit, admit, iterate (launch + wait), retire
   double t_phase[4] = {0, 0, 0, 0};
   long long n_steps = 0, n_iter_frames = 0;
+  hipEvent_t ev_it[2] = {nullptr, nullptr};  // around each graph replay
+  hipEvent_t ev_fh = nullptr;                // frame states copied back
+  FrameDev* fh_pin = nullptr;                // page-locked landing area for them
+  bool launched = false;                     // an iteration is in flight
+  double t_gpu = 0;
 };
 
 static int server_capture(tgmm_server* s) {
@@ -290,6 +295,13 @@ extern "C" int tgmm_server_create(int device, const tgmm_server_opts* opts, tgmm
   s->slot_ticket.assign(sh.Fcap, -1);
   s->fh.assign(sh.Fcap, FrameDev());
   std::memset(s->fh.data(), 0, sizeof(FrameDev) * sh.Fcap);
+  if (hipHostMalloc((void**)&s->fh_pin, sizeof(FrameDev) * sh.Fcap, 0) != hipSuccess ||
+      hipEventCreate(&s->ev_it[0]) != hipSuccess || hipEventCreate(&s->ev_it[1]) != hipSuccess ||
+      hipEventCreateWithFlags(&s->ev_fh, hipEventDisableTiming) != hipSuccess) {
+    tgmm_set_error("server: page-locked frame table or events not available");
+    tgmm_server_destroy(s);
+    return TGMM_ERR_OOM;
+  }
   const long long S = sh.Scap, G = sh.Gcap;
   auto fail = [&](hipError_t e2) {
     tgmm_set_error("server init: %s", hipGetErrorString(e2));
@@ -718,9 +730,25 @@ static int collect(tgmm_server* s, bool block) {
   return TGMM_OK;
 }
 
+// Wait for the iteration launched by the previous step and take the frame
+// states it copied back.
+static int drain(tgmm_server* s) {
+  if (!s->launched) return TGMM_OK;
+  const double t = now_s();
+  HIPCHK(hipEventSynchronize(s->ev_fh));
+  std::memcpy(s->fh.data(), s->fh_pin, s->fh.size() * sizeof(FrameDev));
+  float ms = 0;
+  HIPCHK(hipEventElapsedTime(&ms, s->ev_it[0], s->ev_it[1]));
+  s->t_gpu += 1e-3 * ms;
+  s->launched = false;
+  s->t_phase[2] += now_s() - t;
+  return TGMM_OK;
+}
+
 extern "C" int tgmm_server_step(tgmm_server* s, int* pending) {
   if (!s) return TGMM_ERR_ARG;
   HIPCHK(hipSetDevice(s->device));
+  if (int rc = drain(s)) return rc;
   // retire frames that stopped in the previous iteration
   const double t0 = now_s();
   for (int f = 0; f < (int)s->slot_ticket.size(); f++)
@@ -736,10 +764,13 @@ extern "C" int tgmm_server_step(tgmm_server* s, int* pending) {
   const double t2 = now_s();
   if (s->in_flight > 0) {
     if (int rc = server_capture(s)) return rc;
+    HIPCHK(hipEventRecord(s->ev_it[0], s->st));
     HIPCHK(hipGraphLaunch(s->gexec, s->st));
-    HIPCHK(hipMemcpyAsync(s->fh.data(), s->ws.P.fr, s->fh.size() * sizeof(FrameDev),
+    HIPCHK(hipEventRecord(s->ev_it[1], s->st));
+    HIPCHK(hipMemcpyAsync(s->fh_pin, s->ws.P.fr, s->fh.size() * sizeof(FrameDev),
                           hipMemcpyDeviceToHost, s->st));
-    HIPCHK(hipStreamSynchronize(s->st));
+    HIPCHK(hipEventRecord(s->ev_fh, s->st));
+    s->launched = true;  // the next step (or drain) waits for it
     s->n_steps++;
     s->n_iter_frames += s->in_flight;
   }
@@ -748,7 +779,7 @@ extern "C" int tgmm_server_step(tgmm_server* s, int* pending) {
   s->t_phase[1] += t2 - t1;
   s->t_phase[2] += t3 - t2;
   // results that have landed; with nothing left to run, wait for them
-  if (int rc = collect(s, s->in_flight == 0)) return rc;
+  if (int rc = collect(s, !s->launched)) return rc;
   if (pending) *pending = s->in_flight + (int)s->queue.size() + (int)s->retiring.size();
   return TGMM_OK;
 }
@@ -846,6 +877,7 @@ extern "C" int tgmm_server_stats(tgmm_server* s, tgmm_server_stats_t* st) {
   st->admit_s = s->t_phase[1];
   st->iterate_s = s->t_phase[2];
   st->retire_s = s->t_phase[3];
+  st->iterate_gpu_s = s->t_gpu;
   st->steps = s->n_steps;
   st->frame_iterations = s->n_iter_frames;
   st->in_flight = s->in_flight;
@@ -867,6 +899,7 @@ extern "C" int tgmm_server_set_tiles(tgmm_server* s, const tgmm_tiles* t) {
     return TGMM_ERR_BUSY;
   }
   HIPCHK(hipSetDevice(s->device));
+  if (int rc = drain(s)) return rc;
   HIPCHK(hipStreamSynchronize(s->st));
   s->ko.tiles = x;
   if (s->gexec) (void)hipGraphExecDestroy(s->gexec);
@@ -889,6 +922,10 @@ extern "C" int tgmm_server_destroy(tgmm_server* s) {
     s->pool.put(kv.second.res);
   }
   if (s->released) (void)hipEventDestroy(s->released);
+  for (hipEvent_t e : s->ev_it)
+    if (e) (void)hipEventDestroy(e);
+  if (s->ev_fh) (void)hipEventDestroy(s->ev_fh);
+  if (s->fh_pin) (void)hipHostFree(s->fh_pin);
   ws_free(&s->ws);
   if (s->st) (void)hipStreamDestroy(s->st);
   if (s->cst) (void)hipStreamDestroy(s->cst);

@@ -102,7 +102,8 @@ class ServerStats(ctypes.Structure):
     _fields_ = [("submit_s", ctypes.c_double), ("admit_s", ctypes.c_double),
                 ("iterate_s", ctypes.c_double), ("retire_s", ctypes.c_double),
                 ("steps", ctypes.c_longlong), ("frame_iterations", ctypes.c_longlong),
-                ("in_flight", ctypes.c_longlong), ("queued", ctypes.c_longlong)]
+                ("in_flight", ctypes.c_longlong), ("queued", ctypes.c_longlong),
+                ("iterate_gpu_s", ctypes.c_double)]
 
 
 class Tiles(ctypes.Structure):
