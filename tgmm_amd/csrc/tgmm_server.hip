@@ -5,10 +5,10 @@
 // One device workspace of fixed capacity, sized from an HBM budget, holds
 // every in-flight frame. Each scheduling step
 //   1. retires frames whose EM has stopped: a kernel gathers each one's
-//      results into a staging block that is copied back to page-locked host
-//      memory in one transfer, in stream order, while the step goes on; the
-//      ranges are released at once (later uploads wait for those copies) and
-//      the ticket is done when its copy has landed,
+//      results into a staging block that a third stream copies back to
+//      page-locked host memory in one transfer while the iterations go on;
+//      the ranges are released at once (later uploads wait for the gathers)
+//      and the ticket is done when its copy has landed,
 //   2. stages queued frames whose voxel, supervoxel, Gaussian and grid-cell
 //      ranges fit (first fit, in arrival order): their inputs are uploaded on
 //      a copy stream, so the upload of a stream's next frame overlaps the
@@ -171,6 +171,7 @@ struct Ticket {
   // results: copied into a page-locked buffer on the compute stream at
   // retirement (state 4 until `down` completes), turned into records at fetch
   ResBuf res;
+  size_t res_bytes = 0;
   hipEvent_t down = nullptr;
   double ll = 0;
   int iters = 0;
@@ -190,6 +191,9 @@ struct tgmm_server {
   int device = 0;
   hipStream_t st = nullptr;   // compute
   hipStream_t cst = nullptr;  // uploads
+  hipStream_t dst = nullptr;  // result downloads
+  hipEvent_t gathered = nullptr;        // the step's result gathers are done
+  std::vector<long long> to_download;   // retired this step, copies not yet queued
   tgmm_server_opts o;
   KOpts ko;
   Workspace ws;
@@ -281,8 +285,10 @@ extern "C" int tgmm_server_create(int device, const tgmm_server_opts* opts, tgmm
   P.Gext = kAlign;
   P.Cext = kCB;
   if (hipStreamCreateWithFlags(&s->st, hipStreamNonBlocking) != hipSuccess ||
-      hipStreamCreateWithFlags(&s->cst, hipStreamNonBlocking) != hipSuccess) {
+      hipStreamCreateWithFlags(&s->cst, hipStreamNonBlocking) != hipSuccess ||
+      hipStreamCreateWithFlags(&s->dst, hipStreamNonBlocking) != hipSuccess) {
     if (s->st) (void)hipStreamDestroy(s->st);
+    if (s->cst) (void)hipStreamDestroy(s->cst);
     ws_free(&s->ws);
     delete s;
     tgmm_set_error("hipStreamCreate failed");
@@ -297,7 +303,8 @@ extern "C" int tgmm_server_create(int device, const tgmm_server_opts* opts, tgmm
   std::memset(s->fh.data(), 0, sizeof(FrameDev) * sh.Fcap);
   if (hipHostMalloc((void**)&s->fh_pin, sizeof(FrameDev) * sh.Fcap, 0) != hipSuccess ||
       hipEventCreate(&s->ev_it[0]) != hipSuccess || hipEventCreate(&s->ev_it[1]) != hipSuccess ||
-      hipEventCreateWithFlags(&s->ev_fh, hipEventDisableTiming) != hipSuccess) {
+      hipEventCreateWithFlags(&s->ev_fh, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&s->gathered, hipEventDisableTiming) != hipSuccess) {
     tgmm_set_error("server: page-locked frame table or events not available");
     tgmm_server_destroy(s);
     return TGMM_ERR_OOM;
@@ -647,11 +654,12 @@ __global__ void __launch_bounds__(256) k_gather_results(const double* __restrict
   }
 }
 
-// Retire the frame of slot f: its results are copied into a page-locked
-// buffer on the compute stream (ordered before any later use of its ranges
-// there; uploads on the copy stream wait for `released`), its ranges are
-// unmapped and released, and the stream keeps its Gaussian range for the
-// next frame. The ticket is done once the copies have landed.
+// Retire the frame of slot f: its results are gathered into a staging block
+// on the compute stream (ordered before any later use of its ranges there;
+// uploads on the copy stream wait for `released`), to be copied back on the
+// download stream (queue_downloads), its ranges are unmapped and released,
+// and the stream keeps its Gaussian range for the next frame. The ticket is
+// done once its copy has landed.
 static int retire(tgmm_server* s, int f) {
   const long long id = s->slot_ticket[f];
   Ticket& t = s->tickets[id];
@@ -671,9 +679,8 @@ static int retire(tgmm_server* s, int f) {
   k_gather_results<<<dim3(gx, 4), 256, 0, st>>>(P.gs, P.fixed, P.rnk, P.ind, G, S, t.g0, t.G,
                                                  t.s0, t.S, K, (char*)t.res.d);
   HIPCHK(hipGetLastError());
-  HIPCHK(hipMemcpyAsync(t.res.h, t.res.d, bytes, hipMemcpyDeviceToHost, st));
-  if (!t.down) HIPCHK(hipEventCreateWithFlags(&t.down, hipEventDisableTiming));
-  HIPCHK(hipEventRecord(t.down, st));
+  t.res_bytes = bytes;
+  s->to_download.push_back(t.id);
   const FrameDev F = s->fh[f];  // a copy: the slot's entry is cleared below
   t.ll = F.ll;
   t.iters = F.iter - 1;
@@ -701,6 +708,23 @@ static int retire(tgmm_server* s, int f) {
   s->slot_ticket[f] = -1;
   s->in_flight--;
   s->release_pending = true;
+  return TGMM_OK;
+}
+
+// Copy the staging blocks gathered in this step back to page-locked host
+// memory on the download stream, after the gathers, overlapping the
+// iterations that follow on the compute stream.
+static int queue_downloads(tgmm_server* s) {
+  if (s->to_download.empty()) return TGMM_OK;
+  HIPCHK(hipEventRecord(s->gathered, s->st));
+  HIPCHK(hipStreamWaitEvent(s->dst, s->gathered, 0));
+  for (long long id : s->to_download) {
+    Ticket& t = s->tickets[id];
+    HIPCHK(hipMemcpyAsync(t.res.h, t.res.d, t.res_bytes, hipMemcpyDeviceToHost, s->dst));
+    if (!t.down) HIPCHK(hipEventCreateWithFlags(&t.down, hipEventDisableTiming));
+    HIPCHK(hipEventRecord(t.down, s->dst));
+  }
+  s->to_download.clear();
   return TGMM_OK;
 }
 
@@ -754,6 +778,7 @@ extern "C" int tgmm_server_step(tgmm_server* s, int* pending) {
   for (int f = 0; f < (int)s->slot_ticket.size(); f++)
     if (s->slot_ticket[f] >= 0 && !s->fh[f].active)
       if (int rc = retire(s, f)) return rc;
+  if (int rc = queue_downloads(s)) return rc;
   if (s->release_pending) {
     if (!s->released) HIPCHK(hipEventCreateWithFlags(&s->released, hipEventDisableTiming));
     HIPCHK(hipEventRecord(s->released, s->st));
@@ -914,6 +939,7 @@ extern "C" int tgmm_server_destroy(tgmm_server* s) {
   (void)hipSetDevice(s->device);
   if (s->cst) (void)hipStreamSynchronize(s->cst);
   if (s->st) (void)hipStreamSynchronize(s->st);
+  if (s->dst) (void)hipStreamSynchronize(s->dst);
   if (s->gexec) (void)hipGraphExecDestroy(s->gexec);
   if (s->graph) (void)hipGraphDestroy(s->graph);
   for (auto& kv : s->tickets) {
@@ -929,6 +955,8 @@ extern "C" int tgmm_server_destroy(tgmm_server* s) {
   ws_free(&s->ws);
   if (s->st) (void)hipStreamDestroy(s->st);
   if (s->cst) (void)hipStreamDestroy(s->cst);
+  if (s->dst) (void)hipStreamDestroy(s->dst);
+  if (s->gathered) (void)hipEventDestroy(s->gathered);
   delete s;
   return TGMM_OK;
 }
