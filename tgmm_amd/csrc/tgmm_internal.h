@@ -151,6 +151,9 @@ struct DevPtrs {
   int Fcap;
   int nframes;
   int only_frame;   // prep/reset kernels: -1 = every frame, else this frame
+  // prep/reset kernels for one frame: its supervoxel and Gaussian ranges
+  // start here (256-aligned) and Sext/Gext count from these bases
+  long long s_lo = 0, g_lo = 0;
 };
 
 // kernel launchers (tgmm_kernels.hip); all enqueue on `st`

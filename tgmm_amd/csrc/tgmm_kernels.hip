@@ -112,7 +112,7 @@ __device__ inline int axis_cell(double u, float org, float h, int dim) {
 // One wave per supervoxel: unweighted moments (E-step) and intensity-weighted
 // moments (M-step), FP64, centred about the supervoxel's own means.
 __global__ __launch_bounds__(256) void k_moments(DevPtrs P) {
-  const long long s = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
+  const long long s = P.s_lo + (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
   if (s >= P.Scap) return;
   const int f = P.sv_frame[s];
@@ -268,9 +268,10 @@ __global__ __launch_bounds__(1024) void k_grid_setup(DevPtrs P, KOpts O) {
 // Supervoxels sorted by the grid cell of their centroid (once per frame), so
 // that neighbouring lanes of the kNN kernel walk the same cells.
 __global__ __launch_bounds__(256) void k_qcell_count(DevPtrs P, KOpts O) {
-  const int f = P.sblk_frame[blockIdx.x];
+  const long long sb = P.s_lo / 256 + blockIdx.x;
+  const int f = P.sblk_frame[sb];
   if (f < 0 || (P.only_frame >= 0 && f != P.only_frame)) return;
-  const long long s = (long long)blockIdx.x * 256 + threadIdx.x;
+  const long long s = sb * 256 + threadIdx.x;
   if (P.sv_frame[s] != f) return;
   const FrameDev* F = P.fr + f;
   const long long S = P.Scap;
@@ -284,9 +285,10 @@ __global__ __launch_bounds__(256) void k_qcell_count(DevPtrs P, KOpts O) {
 }
 
 __global__ __launch_bounds__(256) void k_qcell_scatter(DevPtrs P) {
-  const int f = P.sblk_frame[blockIdx.x];
+  const long long sb = P.s_lo / 256 + blockIdx.x;
+  const int f = P.sblk_frame[sb];
   if (f < 0 || (P.only_frame >= 0 && f != P.only_frame)) return;
-  const long long s = (long long)blockIdx.x * 256 + threadIdx.x;
+  const long long s = sb * 256 + threadIdx.x;
   if (P.sv_frame[s] != f) return;
   const FrameDev* F = P.fr + f;
   const int c = P.qcell[s];
@@ -318,14 +320,15 @@ __global__ void k_iter_init(DevPtrs P) {
 // Sum of alpha per frame before the first iteration (GMEM.cu:4167).
 __global__ __launch_bounds__(kGB) void k_alpha_init(DevPtrs P) {
   __shared__ double red[1];
-  const int f = P.gblk_frame[blockIdx.x];
+  const long long gb = P.g_lo / kGB + blockIdx.x;
+  const int f = P.gblk_frame[gb];
   if (f < 0 || (P.only_frame >= 0 && f != P.only_frame)) return;
   FrameDev* F = P.fr + f;
   if (!F->active) return;
-  const long long k = (long long)blockIdx.x * kGB + threadIdx.x;
+  const long long k = gb * kGB + threadIdx.x;
   const double a = (P.g_frame[k] == f) ? gsv(P, GS_ALPHA, k) : 0.0;
   const double t = block_sum<1>(a, red);
-  if (threadIdx.x == 0) P.part_g[blockIdx.x] = t;
+  if (threadIdx.x == 0) P.part_g[gb] = t;
 }
 
 // ---------------------------------------------------------------- scans --
@@ -465,11 +468,12 @@ __device__ inline void gauss_prep_one(const DevPtrs& P, const KOpts& O, const Fr
 // The first iteration's E-step constants and grid binning, from the initial
 // state (later iterations get theirs from k_death).
 __global__ __launch_bounds__(kGB) void k_gauss_prep(DevPtrs P, KOpts O) {
-  const int f = P.gblk_frame[blockIdx.x];
+  const long long gb = P.g_lo / kGB + blockIdx.x;
+  const int f = P.gblk_frame[gb];
   if (f < 0 || (P.only_frame >= 0 && f != P.only_frame)) return;
   const FrameDev* F = P.fr + f;
   if (!F->active) return;
-  const long long k = (long long)blockIdx.x * kGB + threadIdx.x;
+  const long long k = gb * kGB + threadIdx.x;
   if (P.g_frame[k] != f) return;
   double m[3], W[6];
   for (int i = 0; i < 3; i++) m[i] = gsv(P, GS_M + i, k);

@@ -9,13 +9,15 @@
 //      page-locked host memory in one transfer while the iterations go on;
 //      the ranges are released at once (later uploads wait for the gathers)
 //      and the ticket is done when its copy has landed,
-//   2. stages queued frames whose voxel, supervoxel, Gaussian and grid-cell
-//      ranges fit (first fit, in arrival order): their inputs are uploaded on
-//      a copy stream, so the upload of a stream's next frame overlaps the
-//      iterations of the frames in flight,
-//   3. admits staged frames, one frame per stream at a time (the compute
+//   2. admits staged frames, one frame per stream at a time (the compute
 //      stream waits for the frame's upload on the device, not on the host),
-//   4. replays one captured EM iteration over every in-flight frame.
+//   3. replays one captured EM iteration over every in-flight frame, without
+//      waiting for it (the next step does),
+//   4. stages queued frames whose voxel, supervoxel, Gaussian and grid-cell
+//      ranges fit (first fit, in arrival order) while that iteration runs:
+//      their inputs are uploaded on copy streams, so the upload of a
+//      stream's next frame overlaps the iterations of the frames in flight.
+//      With nothing in flight, frames are staged and admitted at once.
 // Frames join and leave at iteration boundaries, so requests that arrive
 // while others are on the GPU start as soon as capacity frees up and their
 // iterations share launches with the frames already running. A stream's
@@ -163,7 +165,7 @@ struct Ticket {
   int slot = -1;
   long long v0 = 0, s0 = 0, sa = 0, g0 = 0, ga = 0, c0 = 0, ncap = 0;
   bool own_g = false;         // the Gaussian range was allocated for this ticket
-  hipEvent_t up = nullptr;    // the upload has completed on the copy stream
+  hipEvent_t up = nullptr;    // the upload has completed on the copy streams
   std::vector<long long> beg; // staged supervoxel tables and records
   std::vector<int> len;
   std::vector<double> gs_in;
@@ -190,7 +192,13 @@ struct Stream {
 struct tgmm_server {
   int device = 0;
   hipStream_t st = nullptr;   // compute
-  hipStream_t cst = nullptr;  // uploads
+  // uploads: a frame's voxel arrays go out on several copy streams at once
+  // (each stream's copies run on one DMA engine, which alone does not fill
+  // the host link); the first stream joins the others before the frame's
+  // upload event
+  static constexpr int kUp = 4;
+  hipStream_t cst[kUp] = {};
+  hipEvent_t up_join[kUp] = {};
   hipStream_t dst = nullptr;  // result downloads
   hipEvent_t gathered = nullptr;        // the step's result gathers are done
   std::vector<long long> to_download;   // retired this step, copies not yet queued
@@ -284,11 +292,18 @@ extern "C" int tgmm_server_create(int device, const tgmm_server_opts* opts, tgmm
   P.Sext = kAlign;  // grown to the allocated ranges as frames are admitted
   P.Gext = kAlign;
   P.Cext = kCB;
-  if (hipStreamCreateWithFlags(&s->st, hipStreamNonBlocking) != hipSuccess ||
-      hipStreamCreateWithFlags(&s->cst, hipStreamNonBlocking) != hipSuccess ||
-      hipStreamCreateWithFlags(&s->dst, hipStreamNonBlocking) != hipSuccess) {
+  bool streams_ok = hipStreamCreateWithFlags(&s->st, hipStreamNonBlocking) == hipSuccess &&
+                    hipStreamCreateWithFlags(&s->dst, hipStreamNonBlocking) == hipSuccess;
+  for (int i = 0; i < tgmm_server::kUp && streams_ok; i++)
+    streams_ok = hipStreamCreateWithFlags(&s->cst[i], hipStreamNonBlocking) == hipSuccess &&
+                 hipEventCreateWithFlags(&s->up_join[i], hipEventDisableTiming) == hipSuccess;
+  if (!streams_ok) {
     if (s->st) (void)hipStreamDestroy(s->st);
-    if (s->cst) (void)hipStreamDestroy(s->cst);
+    if (s->dst) (void)hipStreamDestroy(s->dst);
+    for (int i = 0; i < tgmm_server::kUp; i++) {
+      if (s->cst[i]) (void)hipStreamDestroy(s->cst[i]);
+      if (s->up_join[i]) (void)hipEventDestroy(s->up_join[i]);
+    }
     ws_free(&s->ws);
     delete s;
     tgmm_set_error("hipStreamCreate failed");
@@ -413,6 +428,26 @@ static int update_extents(tgmm_server* s) {
   return TGMM_OK;
 }
 
+// The compute stream issues no copies of its own while serving: the copy
+// engines are kept busy by the uploads and downloads, and a small copy queued
+// behind them would hold up the iterations. Frame-table entries travel as
+// kernel arguments and the table comes back through page-locked host memory
+// written by a kernel.
+__global__ void k_set_frame(FrameDev* fr, FrameDev F) {
+  if (threadIdx.x == 0) *fr = F;
+}
+
+__global__ void __launch_bounds__(256) k_frames_to_host(int* __restrict__ out,
+                                                        const int* __restrict__ fr, int words) {
+  for (int i = threadIdx.x; i < words; i += blockDim.x) out[i] = fr[i];
+}
+
+__global__ void __launch_bounds__(256) k_fill_i32(int* p, long long n, int v) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
 // One per-element frame table range: n entries from p, the first `valid` of
 // them set to the slot and the rest to -1.
 struct MapSeg {
@@ -454,11 +489,12 @@ static void release_ranges(tgmm_server* s, Ticket& t) {
   t.own_g = false;
 }
 
-// Allocate the ticket's ranges and issue its uploads on the copy stream.
+// Allocate the ticket's ranges and issue its uploads on the copy streams.
 // Returns TGMM_OK with t.state = 3, or TGMM_ERR_BUSY if a range does not fit.
 static int stage(tgmm_server* s, Ticket& t) {
   DevPtrs& P = s->ws.P;
-  hipStream_t cst = s->cst;
+  hipStream_t* cs = s->cst;
+  constexpr int nu = tgmm_server::kUp;
   const long long S = s->ws.sh.Scap, G = s->ws.sh.Gcap;
   t.sa = rup(std::max<long long>(t.S, 1), kAlign);
   t.ga = rup(t.G + 1, kAlign);  // at least one padding entry (segment end)
@@ -482,14 +518,15 @@ static int stage(tgmm_server* s, Ticket& t) {
     t.own_g = true;
   }
   if (s->release_pending) {  // ranges released by retirements are read back first
-    HIPCHK(hipStreamWaitEvent(cst, s->released, 0));
+    for (int i = 0; i < nu; i++) HIPCHK(hipStreamWaitEvent(cs[i], s->released, 0));
     s->release_pending = false;
   }
   if (t.N > 0) {
-    HIPCHK(hipMemcpyAsync(P.vx + t.v0, t.q, t.N * 4, hipMemcpyHostToDevice, cst));
-    HIPCHK(hipMemcpyAsync(P.vy + t.v0, t.q + t.N, t.N * 4, hipMemcpyHostToDevice, cst));
-    HIPCHK(hipMemcpyAsync(P.vz + t.v0, t.q + 2 * t.N, t.N * 4, hipMemcpyHostToDevice, cst));
-    HIPCHK(hipMemcpyAsync(P.vimg + t.v0, t.im, t.N * 4, hipMemcpyHostToDevice, cst));
+    HIPCHK(hipMemcpyAsync(P.vx + t.v0, t.q, t.N * 4, hipMemcpyHostToDevice, cs[0]));
+    HIPCHK(hipMemcpyAsync(P.vy + t.v0, t.q + t.N, t.N * 4, hipMemcpyHostToDevice, cs[1 % nu]));
+    HIPCHK(hipMemcpyAsync(P.vz + t.v0, t.q + 2 * t.N, t.N * 4, hipMemcpyHostToDevice,
+                          cs[2 % nu]));
+    HIPCHK(hipMemcpyAsync(P.vimg + t.v0, t.im, t.N * 4, hipMemcpyHostToDevice, cs[3 % nu]));
   }
   t.beg.resize(std::max<long long>(t.S, 1));
   t.len.resize(std::max<long long>(t.S, 1));
@@ -500,9 +537,11 @@ static int stage(tgmm_server* s, Ticket& t) {
   if (t.S > 0) {
     for (int a = 0; a < 3; a++)
       HIPCHK(hipMemcpyAsync(P.sv_cent + a * S + t.s0, t.ce + a * t.S, t.S * 4,
-                            hipMemcpyHostToDevice, cst));
-    HIPCHK(hipMemcpyAsync(P.sv_beg + t.s0, t.beg.data(), t.S * 8, hipMemcpyHostToDevice, cst));
-    HIPCHK(hipMemcpyAsync(P.sv_len + t.s0, t.len.data(), t.S * 4, hipMemcpyHostToDevice, cst));
+                            hipMemcpyHostToDevice, cs[0]));
+    HIPCHK(hipMemcpyAsync(P.sv_beg + t.s0, t.beg.data(), t.S * 8, hipMemcpyHostToDevice,
+                          cs[0]));
+    HIPCHK(hipMemcpyAsync(P.sv_len + t.s0, t.len.data(), t.S * 4, hipMemcpyHostToDevice,
+                          cs[0]));
   }
   if (t.own_g) {
     t.gs_in.resize((size_t)GS_NFIELDS * std::max<long long>(t.G, 1));
@@ -510,11 +549,15 @@ static int stage(tgmm_server* s, Ticket& t) {
     records_to_soa(t.gm_in.data(), t.G, 0, t.G, t.gs_in.data(), t.fx_in.data());
     for (int fi = 0; fi < GS_NFIELDS; fi++)
       HIPCHK(hipMemcpyAsync(P.gs + fi * G + t.g0, t.gs_in.data() + fi * t.G, t.G * 8,
-                            hipMemcpyHostToDevice, cst));
-    HIPCHK(hipMemcpyAsync(P.fixed + t.g0, t.fx_in.data(), t.G, hipMemcpyHostToDevice, cst));
+                            hipMemcpyHostToDevice, cs[0]));
+    HIPCHK(hipMemcpyAsync(P.fixed + t.g0, t.fx_in.data(), t.G, hipMemcpyHostToDevice, cs[0]));
   }
   if (!t.up) HIPCHK(hipEventCreateWithFlags(&t.up, hipEventDisableTiming));
-  HIPCHK(hipEventRecord(t.up, cst));
+  for (int i = 1; i < nu; i++) {
+    HIPCHK(hipEventRecord(s->up_join[i], cs[i]));
+    HIPCHK(hipStreamWaitEvent(cs[0], s->up_join[i], 0));
+  }
+  HIPCHK(hipEventRecord(t.up, cs[0]));
   t.state = 3;
   return TGMM_OK;
 }
@@ -552,13 +595,20 @@ static int activate(tgmm_server* s, Ticket& t, int f) {
   F.nbl = (int)(t.sa / 256);
   F.max_iter = s->o.em.max_iter;
   F.tol = s->o.em.tol;
-  HIPCHK(hipMemcpyAsync(P.fr + f, &F, sizeof(FrameDev), hipMemcpyHostToDevice, st));
+  k_set_frame<<<1, 64, 0, st>>>(P.fr + f, F);
+  HIPCHK(hipGetLastError());
   if (int rc = map_ranges(s, t, f)) return rc;
   if (int rc = update_extents(s)) return rc;  // grids must cover the new ranges
-  DevPtrs Q = P;
+  DevPtrs Q = P;  // set-up kernels over this frame's ranges only
   Q.only_frame = f;
+  Q.s_lo = t.s0;
+  Q.Sext = t.sa;
+  Q.g_lo = t.g0;
+  Q.Gext = t.ga;
   // the cell range may hold counts left by an earlier frame's last binning
-  HIPCHK(hipMemsetAsync(P.cell_cnt + t.c0, 0, t.ncap * 4, st));
+  k_fill_i32<<<(unsigned)std::min<long long>(1024, (t.ncap + 255) / 256), 256, 0, st>>>(
+      P.cell_cnt + t.c0, t.ncap, 0);
+  HIPCHK(hipGetLastError());
   HIPCHK(launch_prep(Q, s->ko, st));
   HIPCHK(launch_reset(Q, s->ko, st));
   t.slot = f;
@@ -571,9 +621,8 @@ static int activate(tgmm_server* s, Ticket& t, int f) {
 }
 
 // Stage queued frames (a stream's frames strictly in order, at most its next
-// one staged ahead), then admit staged frames whose stream is free, in
-// arrival order.
-static int schedule(tgmm_server* s) {
+// one staged ahead).
+static int stage_queued(tgmm_server* s) {
   std::set<int> ahead;  // streams whose earliest unfinished frame was passed
   int staged = 0;
   for (auto it = s->queue.begin(); it != s->queue.end(); ++it) {
@@ -593,6 +642,11 @@ static int schedule(tgmm_server* s) {
       return rc;
     }
   }
+  return TGMM_OK;
+}
+
+// Admit staged frames whose stream is free, in arrival order.
+static int admit_staged(tgmm_server* s) {
   for (auto it = s->queue.begin(); it != s->queue.end();) {
     if (s->in_flight >= s->ws.sh.Fcap) break;
     Ticket& t = s->tickets[*it];
@@ -656,7 +710,7 @@ __global__ void __launch_bounds__(256) k_gather_results(const double* __restrict
 
 // Retire the frame of slot f: its results are gathered into a staging block
 // on the compute stream (ordered before any later use of its ranges there;
-// uploads on the copy stream wait for `released`), to be copied back on the
+// uploads on the copy streams wait for `released`), to be copied back on the
 // download stream (queue_downloads), its ranges are unmapped and released,
 // and the stream keeps its Gaussian range for the next frame. The ticket is
 // done once its copy has landed.
@@ -691,7 +745,8 @@ static int retire(tgmm_server* s, int f) {
   FrameDev z;
   std::memset(&z, 0, sizeof(z));
   s->fh[f] = z;
-  HIPCHK(hipMemcpyAsync(P.fr + f, &s->fh[f], sizeof(FrameDev), hipMemcpyHostToDevice, st));
+  k_set_frame<<<1, 64, 0, st>>>(P.fr + f, z);
+  HIPCHK(hipGetLastError());
   s->av.release(t.v0, t.N);
   s->as.release(t.s0, t.sa);
   s->ac.release(t.c0, t.ncap);
@@ -784,7 +839,11 @@ extern "C" int tgmm_server_step(tgmm_server* s, int* pending) {
     HIPCHK(hipEventRecord(s->released, s->st));
   }
   const double t1 = now_s();
-  if (int rc = schedule(s)) return rc;
+  if (int rc = admit_staged(s)) return rc;
+  if (s->in_flight == 0) {  // idle: stage and admit at once
+    if (int rc = stage_queued(s)) return rc;
+    if (int rc = admit_staged(s)) return rc;
+  }
   if (int rc = update_extents(s)) return rc;
   const double t2 = now_s();
   if (s->in_flight > 0) {
@@ -792,16 +851,22 @@ extern "C" int tgmm_server_step(tgmm_server* s, int* pending) {
     HIPCHK(hipEventRecord(s->ev_it[0], s->st));
     HIPCHK(hipGraphLaunch(s->gexec, s->st));
     HIPCHK(hipEventRecord(s->ev_it[1], s->st));
-    HIPCHK(hipMemcpyAsync(s->fh_pin, s->ws.P.fr, s->fh.size() * sizeof(FrameDev),
-                          hipMemcpyDeviceToHost, s->st));
+    static_assert(sizeof(FrameDev) % 4 == 0, "frame table copied in words");
+    k_frames_to_host<<<1, 256, 0, s->st>>>((int*)s->fh_pin, (const int*)s->ws.P.fr,
+                                           (int)(s->fh.size() * sizeof(FrameDev) / 4));
+    HIPCHK(hipGetLastError());
     HIPCHK(hipEventRecord(s->ev_fh, s->st));
     s->launched = true;  // the next step (or drain) waits for it
     s->n_steps++;
     s->n_iter_frames += s->in_flight;
   }
   const double t3 = now_s();
+  // staging (host tables, uploads) overlaps the iteration just launched;
+  // these frames are admitted by the next step
+  if (int rc = stage_queued(s)) return rc;
+  const double t4 = now_s();
   s->t_phase[3] += t1 - t0;
-  s->t_phase[1] += t2 - t1;
+  s->t_phase[1] += (t2 - t1) + (t4 - t3);
   s->t_phase[2] += t3 - t2;
   // results that have landed; with nothing left to run, wait for them
   if (int rc = collect(s, !s->launched)) return rc;
@@ -937,7 +1002,8 @@ extern "C" int tgmm_server_set_tiles(tgmm_server* s, const tgmm_tiles* t) {
 extern "C" int tgmm_server_destroy(tgmm_server* s) {
   if (!s) return TGMM_ERR_ARG;
   (void)hipSetDevice(s->device);
-  if (s->cst) (void)hipStreamSynchronize(s->cst);
+  for (hipStream_t c : s->cst)
+    if (c) (void)hipStreamSynchronize(c);
   if (s->st) (void)hipStreamSynchronize(s->st);
   if (s->dst) (void)hipStreamSynchronize(s->dst);
   if (s->gexec) (void)hipGraphExecDestroy(s->gexec);
@@ -954,7 +1020,10 @@ extern "C" int tgmm_server_destroy(tgmm_server* s) {
   if (s->fh_pin) (void)hipHostFree(s->fh_pin);
   ws_free(&s->ws);
   if (s->st) (void)hipStreamDestroy(s->st);
-  if (s->cst) (void)hipStreamDestroy(s->cst);
+  for (int i = 0; i < tgmm_server::kUp; i++) {
+    if (s->cst[i]) (void)hipStreamDestroy(s->cst[i]);
+    if (s->up_join[i]) (void)hipEventDestroy(s->up_join[i]);
+  }
   if (s->dst) (void)hipStreamDestroy(s->dst);
   if (s->gathered) (void)hipEventDestroy(s->gathered);
   delete s;
