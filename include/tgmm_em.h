@@ -117,6 +117,23 @@ int tgmm_GMEMvariationalInferenceCUDAWithSupervoxels(
     double tolLikelihood, int devCUDA, int frame, int W4DOF,
     const char* debugPath);
 
+/* Replaces GMEMvariationalInferenceCUDA (GMEMupdateCUDA.h:87-91,
+ * GMEMupdateCUDA.cu:3420-3973), the per-voxel form of the EM: device inputs
+ * queryCUDA [3*query_nb] SoA and imgDataCUDA [query_nb]; vecGMtemp is host
+ * in/out [ref_nb]. Every voxel takes its K = 5 nearest Gaussians; its
+ * responsibilities are cut to zero where the log score is <= -100 and are
+ * multiplied by the voxel's intensity (GMEMupdateCUDA.cu:2177-2242); the
+ * M-step keeps pairs whose stored value exceeds 1e-3. On return: posterior
+ * and expectation fields updated, supervoxelIdx/supervoxelNum as given,
+ * rnkCUDA/indCUDA (device, slot-major v + j*query_nb) hold the last
+ * iteration. rnkCUDAtr may be NULL; indCUDAtr (may be NULL) receives a copy
+ * of indCUDA. debugPath is accepted and unused, as in the reference. */
+int tgmm_GMEMvariationalInferenceCUDA(float* queryCUDA, float* imgDataCUDA, float* rnkCUDA,
+                                      float* rnkCUDAtr, int* indCUDA, int* indCUDAtr,
+                                      GaussianMixtureModelCUDA* vecGMtemp, long long query_nb,
+                                      int ref_nb, int maxIterEM, double tolLikelihood,
+                                      int devCUDA, int frame, int W4DOF, const char* debugPath);
+
 /* Replaces GMEMreleaseMemoryWithSupervoxels (GMEMupdateCUDA.h:140-144,
  * GMEMupdateCUDA.cu:4654-4676): frees and nulls every pointer. */
 int tgmm_GMEMreleaseMemoryWithSupervoxels(float** queryCUDA,
@@ -139,6 +156,13 @@ typedef struct tgmm_em_opts {
   int w4dof;               /* zero xz/yz of W in regularisation */
   float scale[TGMM_DIMS];  /* anisotropy for kNN and regularisation */
   int max_sv_per_gaussian; /* cap of supervoxelIdx, 0..16 (16 = reference) */
+  int voxel_mode;          /* 1: per-voxel EM (GMEMvariationalInferenceCUDA,
+                              GMEMupdateCUDA.cu:3420-3973): every voxel is its
+                              own query with K candidates; n_supervoxels,
+                              sv_ptr and centroids are ignored, rnk/ind have
+                              K*n_voxels entries and rnk is multiplied by the
+                              voxel's intensity; supervoxelIdx/Num are left as
+                              given */
 } tgmm_em_opts;
 
 /* One frame. Input arrays are HOST memory with the reference's boundary

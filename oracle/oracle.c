@@ -15,6 +15,15 @@
  *   death          GMEM.cu:1813-1843 (K16)
  *   log-likelihood GMEM.cu:1913-1978 (K17), per voxel, FP64
  *   convergence    GMEM.cu:4143-4148; supervoxel assignment GMEM.cu:4328-4350
+ * and, with tgmm_em_opts.voxel_mode, the per-voxel EM of
+ * GMEMvariationalInferenceCUDA (GMEM.cu:3420-3973):
+ *   kNN            each voxel is a query (knnCUDAinPlace on queryCUDA, :3576)
+ *   E-step         GMEM.cu:2177-2242 (GMEMcomputeRnkKernel): cut at -100,
+ *                  stored r multiplied by the voxel's intensity
+ *   M-step         GMEM.cu:1142-1284 (GMEMcomputeXkNkSkKernelTr): the stored
+ *                  value is the weight, pairs with weight > 1e-3
+ *   log-likelihood GMEM.cu:1845-1912 (GMEMcomputeLikelihoodKernel)
+ *   no supervoxel assignment (the records keep supervoxelIdx/Num)
  *
  * Declared deviations from the reference (identical in the HIP code):
  *   - kNN candidates are the K smallest (distance, index) pairs, output sorted
@@ -363,6 +372,45 @@ void orc_estep(long long N, long long S, const float* query,
   }
 }
 
+/* GMEM.cu:2177-2242, per voxel in FP64. A candidate whose log score
+ * aux = -0.5 (3/beta + nu d^T W d) + E[ln pi] + 0.5 E[ln det] is <= -100
+ * counts as zero (:2208); the others are normalised among themselves (the
+ * reference's exp(aux) / sum exp(aux), evaluated max-subtracted) and the
+ * stored value is r * img (:2216-2222). A voxel with no candidate above the
+ * cut keeps r = 0 (the reference's memset, :3104-3106). */
+void orc_estep_voxel(long long N, const float* query, const float* img, int G,
+                     const GaussianMixtureModelCUDA* gm, int K, const int* ind,
+                     float* rnk) {
+  (void)G;
+#pragma omp parallel for schedule(static, 256)
+  for (long long v = 0; v < N; v++) {
+    double aux[TGMM_KMAX];
+    int keep[TGMM_KMAX];
+    double mx = -INFINITY;
+    for (int j = 0; j < K; j++) {
+      const int k = ind[v + (long long)j * N];
+      keep[j] = 0;
+      if (k < 0) continue;
+      const GaussianMixtureModelCUDA* g = &gm[k];
+      const double d0 = (double)query[v] - g->m_k[0];
+      const double d1 = (double)query[v + N] - g->m_k[1];
+      const double d2 = (double)query[v + 2 * N] - g->m_k[2];
+      aux[j] = -0.5 * (3.0 / g->beta_k + g->nu_k * quad(g->W_k, d0, d1, d2)) +
+               g->expectedLogResponsivityCUDA + 0.5 * g->expectedLogDetCovarianceCUDA;
+      keep[j] = aux[j] > -100.0;
+      if (keep[j] && aux[j] > mx) mx = aux[j];
+    }
+    double den = 0.0;
+    for (int j = 0; j < K; j++)
+      if (keep[j]) den += exp(aux[j] - mx);
+    for (int j = 0; j < K; j++) {
+      double r = 0.0;
+      if (keep[j] && den > 0.0) r = (double)img[v] * (exp(aux[j] - mx) / den);
+      rnk[v + (long long)j * N] = (float)r;
+    }
+  }
+}
+
 /* GMEM.cu:1286-1455: weights r_sk * img_v over pairs with r > 1e-3; centred
  * second pass in FP64 (Ck = sum w (x - xk)(x - xk)^T, unnormalised). */
 void orc_mstep(long long N, long long S, const float* query, const float* img,
@@ -525,6 +573,20 @@ static int em_loop(long long N, long long S, int G, const float* query,
                    int* iters, double* ll_final) {
   const int K = o->K;
   if (K < 1 || K > TGMM_KMAX || G < 0 || S < 0 || N < 0) return TGMM_ERR_ARG;
+  /* voxel mode: every voxel is its own query and its own "supervoxel" for
+   * the kNN and the log-likelihood; the M-step weight is the stored value */
+  const int vox = o->voxel_mode != 0;
+  long long* vptr = NULL;
+  float* ones = NULL;
+  if (vox) {
+    vptr = (long long*)malloc(sizeof(long long) * (size_t)(N + 1));
+    ones = (float*)malloc(sizeof(float) * (size_t)(N > 0 ? N : 1));
+    for (long long v = 0; v <= N; v++) vptr[v] = v;
+    for (long long v = 0; v < N; v++) ones[v] = 1.0f;
+    S = N;
+    cent = query;
+    ptr = vptr;
+  }
   double* Nk = (double*)malloc(sizeof(double) * (size_t)(G > 0 ? G : 1));
   double* xk = (double*)malloc(sizeof(double) * 3 * (size_t)(G > 0 ? G : 1));
   double* Ck = (double*)malloc(sizeof(double) * 6 * (size_t)(G > 0 ? G : 1));
@@ -533,8 +595,13 @@ static int em_loop(long long N, long long S, int G, const float* query,
   while (!orc_converged(ll, ll1, ll2, ll3, o->tol) && it <= o->max_iter) {
     orc_knn(S, cent, G, gm, o->scale, K, ind);
     orc_expectations(G, gm, orc_sum_alpha(G, gm));
-    orc_estep(N, S, query, ptr, G, gm, K, ind, rnk);
-    orc_mstep(N, S, query, img, ptr, G, K, ind, rnk, Nk, xk, Ck);
+    if (vox) {
+      orc_estep_voxel(N, query, img, G, gm, K, ind, rnk);
+      orc_mstep(N, S, query, ones, ptr, G, K, ind, rnk, Nk, xk, Ck);
+    } else {
+      orc_estep(N, S, query, ptr, G, gm, K, ind, rnk);
+      orc_mstep(N, S, query, img, ptr, G, K, ind, rnk, Nk, xk, Ck);
+    }
     if (stop_after > 0 && it == stop_after) {
       memcpy(Nk_out, Nk, sizeof(double) * G);
       memcpy(xk_out, xk, sizeof(double) * 3 * G);
@@ -552,6 +619,8 @@ static int em_loop(long long N, long long S, int G, const float* query,
   }
   if (iters) *iters = it - 1;
   if (ll_final) *ll_final = ll;
+  free(vptr);
+  free(ones);
   free(Nk);
   free(xk);
   free(Ck);
@@ -564,7 +633,7 @@ int orc_em_run(long long N, long long S, int G, const float* query,
                int* ind, double* ll_hist, int* iters, double* ll_final) {
   int rc = em_loop(N, S, G, query, img, cent, ptr, gm, o, 0, rnk, ind, NULL,
                    NULL, NULL, ll_hist, iters, ll_final);
-  if (rc == TGMM_OK) {
+  if (rc == TGMM_OK && !o->voxel_mode) {
     const int cap = o->max_sv_per_gaussian > 0 ? o->max_sv_per_gaussian : 0;
     orc_assign_supervoxels(S, G, gm, o->K, rnk, ind, cap);
   }

@@ -2,7 +2,8 @@
  * oracle.h -- CPU restatement (FP64, plain C, OpenMP) of the reference's
  * supervoxel-mode variational-inference EM
  * (nmeth.3036-S2/src/UtilsCUDA/GMEMupdateCUDA.cu:3985-4421 and the kernels it
- * launches, knnCuda.cu:61-246).
+ * launches, knnCuda.cu:61-246) and, with tgmm_em_opts.voxel_mode, of the
+ * per-voxel EM (GMEMupdateCUDA.cu:3420-3973).
  *
  * TEST INFRASTRUCTURE ONLY. This library is the checker: it is loaded by the
  * tests, by __graft_entry__.smoke() and by bench.py's CPU-baseline leg, and by
@@ -42,6 +43,10 @@ void orc_expectations(int G, GaussianMixtureModelCUDA* gm, double sum_alpha);
 void orc_estep(long long N, long long S, const float* query,
                const long long* ptr, int G, const GaussianMixtureModelCUDA* gm,
                int K, const int* ind, float* rnk);
+/* voxel mode: N voxels, rnk/ind [K*N] slot-major */
+void orc_estep_voxel(long long N, const float* query, const float* img, int G,
+                     const GaussianMixtureModelCUDA* gm, int K, const int* ind,
+                     float* rnk);
 void orc_mstep(long long N, long long S, const float* query, const float* img,
                const long long* ptr, int G, int K, const int* ind,
                const float* rnk, double* Nk, double* xk, double* Ck);

@@ -29,17 +29,19 @@ class EmOpts(ctypes.Structure):
         ("w4dof", ctypes.c_int),
         ("scale", ctypes.c_float * 3),
         ("max_sv_per_gaussian", ctypes.c_int),
+        ("voxel_mode", ctypes.c_int),
     ]
 
 
 def make_opts(K=5, max_iter=10, tol=-1.0, lmin=0.02, lmax=0.2222, emax=9.0,
-              w4dof=1, scale=(1.0, 1.0, 5.0), cap=16):
+              w4dof=1, scale=(1.0, 1.0, 5.0), cap=16, voxel=False):
     o = EmOpts()
     o.K, o.max_iter, o.tol = K, max_iter, tol
     o.lambda_min, o.lambda_max, o.max_excentricity = lmin, lmax, emax
     o.w4dof = w4dof
     o.scale[:] = list(scale)
     o.max_sv_per_gaussian = cap
+    o.voxel_mode = 1 if voxel else 0
     return o
 
 
@@ -72,6 +74,8 @@ def oracle():
         L.orc_expectations.argtypes = [ctypes.c_int, _vp, ctypes.c_double]
         L.orc_estep.argtypes = [_ll, _ll, _fp, _lp, ctypes.c_int, _vp,
                                 ctypes.c_int, _ip, _fp]
+        L.orc_estep_voxel.argtypes = [_ll, _fp, _fp, ctypes.c_int, _vp, ctypes.c_int, _ip,
+                                      _fp]
         L.orc_mstep.argtypes = [_ll, _ll, _fp, _fp, _lp, ctypes.c_int,
                                 ctypes.c_int, _ip, _fp, _dp, _dp, _dp]
         L.orc_update.argtypes = [ctypes.c_int, _vp, _dp, _dp, _dp,
@@ -99,7 +103,7 @@ def em_run(frame, opts, gm=None):
     """Full oracle EM on a synth.Frame; returns (gm, rnk, ind, ll_hist, iters, ll)."""
     L = oracle()
     gm = (frame.gm if gm is None else gm).copy()
-    S, K = frame.S, opts.K
+    S, K = (frame.N if opts.voxel_mode else frame.S), opts.K
     rnk = np.zeros(S * K, np.float32)
     ind = np.zeros(S * K, np.int32)
     hist = np.zeros(max(opts.max_iter, 1), np.float64)
@@ -117,7 +121,7 @@ def em_partial(frame, opts, iters, gm=None):
     """Oracle EM stopped after the M-step statistics of iteration `iters`."""
     L = oracle()
     gm = (frame.gm if gm is None else gm).copy()
-    S, K, G = frame.S, opts.K, frame.G
+    S, K, G = (frame.N if opts.voxel_mode else frame.S), opts.K, frame.G
     rnk = np.zeros(S * K, np.float32)
     ind = np.zeros(S * K, np.int32)
     Nk = np.zeros(G)

@@ -1,0 +1,99 @@
+"""GPU parity of the voxel mode (GMEMvariationalInferenceCUDA,
+GMEMupdateCUDA.cu:3420-3973): the HIP library against the oracle's voxel
+mode, the drop-in entry point against the batch, and the identity with
+supervoxel mode on one-voxel supervoxels at unit intensity."""
+import numpy as np
+import pytest
+
+import oracle_lib as O
+from helpers import record_errors, rel_err
+from tgmm_amd import em, synth
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-5  # relative, means / precisions / nu / beta / alpha / log-likelihood
+
+
+@pytest.fixture(scope="module")
+def fr():
+    return synth.config1(seed=4242, n_cells=40, target_voxels=8000)
+
+
+def _run(frames, opts, mode=em.SOLVE_GRAPH):
+    with em.Batch(frames, opts) as b:
+        b.solve(mode)
+        return b.fetch()
+
+
+@pytest.mark.parametrize("iters", [1, 8])
+def test_voxel_mode_matches_oracle(fr, iters):
+    print("voxel mode, N=%d G=%d iters=%d" % (fr.N, fr.G, iters), flush=True)
+    out = _run([fr], em.make_opts(K=5, max_iter=iters, voxel=True))[0]
+    gm_o, rnk_o, ind_o, hist, it, ll = O.em_run(fr, O.make_opts(K=5, max_iter=iters,
+                                                                voxel=True))
+    assert out["iterations"] == it == iters
+    assert out["rnk"].shape == (5 * fr.N,)
+    if iters == 1:
+        assert np.array_equal(out["ind"], ind_o)
+    # stored values are r * img; float exponentials against FP64
+    assert np.max(np.abs(out["rnk"] - rnk_o)) < 2e-6
+    e = record_errors(out["gm"], gm_o)
+    assert max(e.values()) < TOL, e
+    assert rel_err(out["ll"], ll) < TOL
+    # the supervoxel lists come back as given
+    assert np.array_equal(out["gm"]["supervoxelIdx"], fr.gm["supervoxelIdx"])
+    assert np.array_equal(out["gm"]["supervoxelNum"], fr.gm["supervoxelNum"])
+
+
+def test_unit_intensity_equals_one_voxel_supervoxels(fr):
+    ones = np.ones(fr.N, np.float32)
+    f1 = synth.Frame(fr.query.copy(), ones, fr.query.copy(),
+                     np.arange(fr.N + 1, dtype=np.int64), fr.gm.copy())
+    v = _run([f1], em.make_opts(K=5, max_iter=6, voxel=True))[0]
+    s = _run([f1], em.make_opts(K=5, max_iter=6))[0]
+    for k in ("m_k", "W_k", "nu_k", "beta_k", "alpha_k", "expectedLogDetCovarianceCUDA",
+              "expectedLogResponsivityCUDA"):
+        assert np.array_equal(v["gm"][k], s["gm"][k]), k
+    assert v["ll"] == s["ll"]
+    assert np.array_equal(v["ind"], s["ind"])
+    assert np.allclose(v["rnk"], s["rnk"], rtol=0, atol=1e-30)
+
+
+def test_voxel_entry_point_matches_batch(fr):
+    em.regularizePrecisionMatrixConstants.setConstants(0.02, 0.2222, 9.0)
+    scale = np.array([1, 1, 5], np.float32)
+    # the kNN anisotropy is recorded by the init call of the supervoxel API;
+    # set it the same way here (GMEMinitializeMemoryWithSupervoxels with no
+    # supervoxels), then the regularisation anisotropy
+    d = em.GMEMinitializeMemoryWithSupervoxels(fr.query, fr.img, scale, fr.N, 0,
+                                               np.zeros(3, np.float32),
+                                               np.zeros(1, np.int64))
+    em.copyScaleToDvice(scale)
+    rnk = em.DeviceArray(nbytes=5 * fr.N * 4)
+    ind = em.DeviceArray(nbytes=5 * fr.N * 4)
+    indtr = em.DeviceArray(nbytes=5 * fr.N * 4)
+    gm = fr.gm.copy()
+    em.GMEMvariationalInferenceCUDA(d.queryCUDA, d.imgDataCUDA, rnk, ind, indtr, gm, fr.N,
+                                    fr.G, 8, -1.0)
+    em.GMEMreleaseMemoryWithSupervoxels(d)
+    ref = _run([fr], em.make_opts(K=5, max_iter=8, voxel=True))[0]
+    assert gm.tobytes() == ref["gm"].tobytes()
+    r, i = rnk.numpy(np.float32), ind.numpy(np.int32)
+    assert np.array_equal(r, ref["rnk"])
+    assert np.array_equal(i, ref["ind"])
+    assert np.array_equal(indtr.numpy(np.int32), i)
+
+
+def test_voxel_mode_alone_equals_in_the_mix(fr):
+    f2 = synth.config1(seed=99, n_cells=25, target_voxels=5000)
+    o = em.make_opts(K=5, max_iter=5, voxel=True)
+    mix = _run([fr, f2], o)
+    for f, m in zip([fr, f2], mix):
+        a = _run([f], o)[0]
+        assert a["gm"].tobytes() == m["gm"].tobytes()
+        assert np.array_equal(a["rnk"], m["rnk"]) and a["ll"] == m["ll"]
+
+
+def test_server_refuses_voxel_mode():
+    with pytest.raises(em.TgmmError):
+        em.Server(em.make_opts(K=5, voxel=True), hbm_budget_bytes=1 << 30)

@@ -98,6 +98,7 @@ struct KOpts {
   float scale_reg[3];  // regularisation anisotropy (copyScaleToDvice)
   double lmin, lmax, emax;
   Tiles tiles;
+  int voxel;  // per-voxel EM (one-voxel supervoxels, cut and intensity in the E-step)
 };
 
 struct DevPtrs {

@@ -192,6 +192,31 @@ __global__ __launch_bounds__(256) void k_moments(DevPtrs P) {
   }
 }
 
+// Voxel mode: every supervoxel is one voxel, so its moments are the voxel
+// itself (count 1, mean x, no scatter) and its M-step weight is the stored
+// responsibility, which already carries the intensity (GMEM.cu:2216-2222).
+__global__ __launch_bounds__(256) void k_moments_voxel(DevPtrs P) {
+  const long long s = P.s_lo + (long long)blockIdx.x * 256 + threadIdx.x;
+  if (s >= P.Scap) return;
+  const int f = P.sv_frame[s];
+  if (f < 0 || (P.only_frame >= 0 && f != P.only_frame)) return;
+  const long long S = P.Scap, v = P.sv_beg[s];
+  const float x[3] = {P.vx[v], P.vy[v], P.vz[v]};
+  P.mom[s] = 1.0;
+  P.wmom[s] = 1.0;
+#pragma unroll
+  for (int a = 0; a < 3; a++) {
+    P.mom[(1 + a) * S + s] = (double)x[a];
+    P.wmom[(1 + a) * S + s] = (double)x[a];
+    P.anchor[a * S + s] = x[a];
+  }
+#pragma unroll
+  for (int i = 4; i < 10; i++) {
+    P.mom[i * S + s] = 0.0;
+    P.wmom[i * S + s] = 0.0;
+  }
+}
+
 // One block per frame: bounding box of the scaled kNN centroids and the grid
 // shape (about knn_cell means per cell, at most ncap cells).
 __global__ __launch_bounds__(1024) void k_grid_setup(DevPtrs P, KOpts O) {
@@ -942,7 +967,9 @@ __global__ __launch_bounds__(256) void k_estep(DevPtrs P, KOpts O) {
                           2.0 * (W[1] * Q[1] + W[2] * Q[2] + W[4] * Q[4]);
         const double kappa = (ep.dpsi - psi_sum) + ep.kappa0;
         L[j] = n * kappa - 0.5 * ep.nu * (tq + n * qf);
-        mx = fmax(mx, L[j]);
+        // voxel mode: a score <= -100 counts as zero (GMEM.cu:2208)
+        if (O.voxel && !(L[j] > -100.0)) kk[j] = -2;
+        if (kk[j] >= 0) mx = fmax(mx, L[j]);
       }
     }
   }
@@ -956,10 +983,12 @@ __global__ __launch_bounds__(256) void k_estep(DevPtrs P, KOpts O) {
       den += e[j];
     }
   }
+  // voxel mode stores r times the voxel's intensity (GMEM.cu:2216-2222)
+  const float wv = O.voxel ? P.vimg[P.sv_beg[s]] : 1.0f;
 #pragma unroll
   for (int j = 0; j < KM; j++) {
     if (j < K) {
-      const float r = (den > 0.0f || den != den) ? e[j] / den : 0.0f;
+      const float r = ((den > 0.0f || den != den) ? e[j] / den : 0.0f) * wv;
       P.rnk[(long long)j * S + s] = r;
       if (kk[j] >= 0 && r > 1e-3f) atomicAdd(P.seg_cnt + kk[j], 1);
     }
@@ -1469,7 +1498,10 @@ static inline unsigned nb(long long n, int per) { return (unsigned)((n + per - 1
 
 
 hipError_t launch_prep(const DevPtrs& P, const KOpts& O, hipStream_t st) {
-  k_moments<<<nb(P.Sext, 4), 256, 0, st>>>(P);
+  if (O.voxel)
+    k_moments_voxel<<<nb(P.Sext, 256), 256, 0, st>>>(P);
+  else
+    k_moments<<<nb(P.Sext, 4), 256, 0, st>>>(P);
   k_grid_setup<<<P.only_frame >= 0 ? 1 : P.nframes, 1024, 0, st>>>(P, O);
   k_qcell_count<<<nb(P.Sext, 256), 256, 0, st>>>(P, O);
   k_scan_frames<<<P.only_frame >= 0 ? 1 : P.nframes, 1024, 0, st>>>(P, 2, 0, O.K);

@@ -89,6 +89,98 @@ extern "C" int tgmm_GMEMinitializeMemoryWithSupervoxels(
   return TGMM_OK;
 }
 
+// Voxel mode (GMEMvariationalInferenceCUDA, GMEMupdateCUDA.cu:3420-3973):
+// the same plain per-frame solve over one-voxel supervoxels, with the
+// reference's size limits (:3425-3440) and outputs of K x query_nb. The
+// reference writes no debug dump in this mode (its dump code is commented
+// out), so debugPath is ignored.
+extern "C" int tgmm_GMEMvariationalInferenceCUDA(float* queryCUDA, float* imgDataCUDA,
+                                                 float* rnkCUDA, float* rnkCUDAtr, int* indCUDA,
+                                                 int* indCUDAtr,
+                                                 GaussianMixtureModelCUDA* vecGMtemp,
+                                                 long long query_nb, int ref_nb, int maxIterEM,
+                                                 double tolLikelihood, int devCUDA, int frame,
+                                                 int W4DOF, const char* debugPath) {
+  (void)rnkCUDAtr;
+  (void)debugPath;
+  if (query_nb > 1024LL * 65535LL) {
+    tgmm_set_error("Too many query points for the GPU resources (%lld)", query_nb);
+    return TGMM_ERR_LIMIT;
+  }
+  if (ref_nb > 65535) {
+    tgmm_set_error("Too many reference points for the GPU resources (%d)", ref_nb);
+    return TGMM_ERR_LIMIT;
+  }
+  if (query_nb < 0 || ref_nb < 0 || (ref_nb > 0 && !vecGMtemp) ||
+      (query_nb > 0 && (!queryCUDA || !imgDataCUDA || !rnkCUDA || !indCUDA))) {
+    tgmm_set_error("GMEMvariationalInferenceCUDA: NULL argument");
+    return TGMM_ERR_ARG;
+  }
+  tgmm_em_opts o;
+  float sreg[3];
+  {
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (g_lmin < 0) {
+      tgmm_set_error(
+          "before GMEMregularizeWkKernel: regularizePrecisionMatrixConstants were not set");
+      return TGMM_ERR_NOT_SET;
+    }
+    std::memset(&o, 0, sizeof(o));
+    o.K = TGMM_LEGACY_K;
+    o.max_iter = maxIterEM;
+    o.tol = tolLikelihood;
+    o.lambda_min = g_lmin;
+    o.lambda_max = g_lmax;
+    o.max_excentricity = g_emax;
+    o.w4dof = W4DOF ? 1 : 0;
+    for (int a = 0; a < 3; a++) {
+      o.scale[a] = g_scale_knn[a];
+      sreg[a] = g_scale_reg[a];
+    }
+    o.max_sv_per_gaussian = TGMM_MAX_SUPERVOXELS_PER_GAUSSIAN;
+    o.voxel_mode = 1;
+  }
+  HIPCHK(hipSetDevice(devCUDA));
+  FrameSrc f;
+  f.N = query_nb;
+  f.G = ref_nb;
+  f.query = queryCUDA;
+  f.img = imgDataCUDA;
+  f.gm = vecGMtemp;
+  tgmm_batch* b = nullptr;
+  int rc = batch_build(devCUDA, std::vector<FrameSrc>{f}, o, sreg, nullptr, &b);
+  if (rc) return rc;
+  rc = tgmm_batch_solve(b, TGMM_SOLVE_HOSTSYNC);
+  std::vector<double> gs;
+  std::vector<float> rnk;
+  std::vector<int> ind;
+  if (!rc) rc = batch_fetch_host(b, gs, rnk, ind);
+  double ll = 0.0;
+  int iters = 0;
+  if (!rc) {
+    frame_results(b, 0, gs, rnk, ind, vecGMtemp, nullptr, nullptr, &ll, &iters);
+    const long long Sc = b->ws.sh.Scap;
+    const size_t w = (size_t)query_nb * sizeof(float);
+    if (query_nb > 0) {
+      hipError_t e = hipMemcpy2D(rnkCUDA, w, b->ws.P.rnk, Sc * sizeof(float), w, TGMM_LEGACY_K,
+                                 hipMemcpyDeviceToDevice);
+      if (e == hipSuccess)
+        e = hipMemcpy2D(indCUDA, w, b->ws.P.ind, Sc * sizeof(int), w, TGMM_LEGACY_K,
+                        hipMemcpyDeviceToDevice);
+      if (e == hipSuccess && indCUDAtr)
+        e = hipMemcpy(indCUDAtr, indCUDA, TGMM_LEGACY_K * w, hipMemcpyDeviceToDevice);
+      if (e != hipSuccess) {
+        tgmm_set_error("copy of rnk/ind to caller buffers failed: %s", hipGetErrorString(e));
+        rc = TGMM_ERR_DEVICE;
+      }
+    }
+  }
+  if (!rc && getenv("TGMM_VERBOSE"))
+    printf("Frame=%d;iter=%d;Log-likelihood =%16.12f\n", frame, iters + 1, ll);
+  tgmm_batch_destroy(b);
+  return rc;
+}
+
 extern "C" int tgmm_GMEMreleaseMemoryWithSupervoxels(float** queryCUDA, float** imgDataCUDA,
                                                      float** rnkCUDA, int** indCUDA,
                                                      int** indCUDAtr,

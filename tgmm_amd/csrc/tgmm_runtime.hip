@@ -164,6 +164,10 @@ static int validate_opts(const tgmm_em_opts* o) {
     tgmm_set_error("max_sv_per_gaussian outside [0, 16]");
     return TGMM_ERR_ARG;
   }
+  if (o->voxel_mode != 0 && o->voxel_mode != 1) {
+    tgmm_set_error("voxel_mode must be 0 or 1");
+    return TGMM_ERR_ARG;
+  }
   return TGMM_OK;
 }
 
@@ -180,6 +184,7 @@ KOpts make_kopts(const tgmm_em_opts& o, const float* scale_reg) {
   k.lmax = o.lambda_max;
   k.emax = o.max_excentricity;
   k.tiles = kDefaultTiles;
+  k.voxel = o.voxel_mode ? 1 : 0;
   return k;
 }
 
@@ -210,10 +215,30 @@ void soa_to_records(const double* gs, long long G, long long g0, long long Gcap,
   }
 }
 
-int batch_build(int device, const std::vector<FrameSrc>& fs, const tgmm_em_opts& o,
+int batch_build(int device, const std::vector<FrameSrc>& fs_in, const tgmm_em_opts& o,
                 const float* scale_reg, hipStream_t stream, tgmm_batch** out) {
   if (int rc = validate_opts(&o)) return rc;
   HIPCHK(hipSetDevice(device));
+  // voxel mode (GMEMupdateCUDA.cu:3420-3973): every voxel is its own kNN
+  // query and a one-voxel supervoxel whose centroid is the voxel itself
+  std::vector<FrameSrc> vfs;
+  std::vector<std::vector<long long>> vptr;
+  if (o.voxel_mode) {
+    vfs = fs_in;
+    vptr.resize(fs_in.size());
+    for (size_t i = 0; i < fs_in.size(); i++) {
+      if (fs_in[i].N < 0) {
+        tgmm_set_error("frame %zu: negative size", i);
+        return TGMM_ERR_ARG;
+      }
+      vptr[i].resize(fs_in[i].N + 1);
+      for (long long v = 0; v <= fs_in[i].N; v++) vptr[i][v] = v;
+      vfs[i].S = fs_in[i].N;
+      vfs[i].cent = fs_in[i].query;
+      vfs[i].ptr = vptr[i].data();
+    }
+  }
+  const std::vector<FrameSrc>& fs = o.voxel_mode ? vfs : fs_in;
   tgmm_batch* b = new tgmm_batch();
   b->device = device;
   b->opts = o;
@@ -290,6 +315,17 @@ int batch_build(int device, const std::vector<FrameSrc>& fs, const tgmm_em_opts&
   if (rc) {
     tgmm_batch_destroy(b);
     return rc;
+  }
+  if (o.voxel_mode) {  // the records keep their supervoxel lists
+    b->sv_keep.resize(fs.size());
+    for (size_t i = 0; i < fs.size(); i++) {
+      b->sv_keep[i].resize((size_t)fs[i].G * (TGMM_MAX_SUPERVOXELS_PER_GAUSSIAN + 1));
+      for (int k = 0; k < fs[i].G; k++) {
+        int* d = b->sv_keep[i].data() + (size_t)k * (TGMM_MAX_SUPERVOXELS_PER_GAUSSIAN + 1);
+        std::memcpy(d, fs[i].gm[k].supervoxelIdx, sizeof(int) * TGMM_MAX_SUPERVOXELS_PER_GAUSSIAN);
+        d[TGMM_MAX_SUPERVOXELS_PER_GAUSSIAN] = fs[i].gm[k].supervoxelNum;
+      }
+    }
   }
   *out = b;
   return TGMM_OK;
@@ -377,12 +413,14 @@ extern "C" int tgmm_batch_create(int device, const tgmm_frame_desc* frames, int 
     return TGMM_ERR_ARG;
   }
   if (int rc = validate_opts(opts)) return rc;
+  const bool vox = opts->voxel_mode != 0;  // supervoxel arrays are not used
   std::vector<FrameSrc> fs(nframes);
   for (int i = 0; i < nframes; i++) {
     const tgmm_frame_desc& d = frames[i];
     if (d.n_voxels < 0 || d.n_supervoxels < 0 || d.n_gaussians < 0 ||
         (d.n_voxels > 0 && (!d.query || !d.img)) ||
-        (d.n_supervoxels > 0 && (!d.centroids)) || !d.sv_ptr || (d.n_gaussians > 0 && !d.gm)) {
+        (!vox && ((d.n_supervoxels > 0 && !d.centroids) || !d.sv_ptr)) ||
+        (d.n_gaussians > 0 && !d.gm)) {
       tgmm_set_error("frame %d: NULL array or negative size", i);
       return TGMM_ERR_ARG;
     }
@@ -505,8 +543,18 @@ void frame_results(tgmm_batch* b, int i, const std::vector<double>& gs,
   const int K = b->ws.K;
   if (gm && F.ng > 0) {
     soa_to_records(gs.data(), F.ng, F.g0, G, gm);
-    const int cap = b->opts.max_sv_per_gaussian;
-    assign_supervoxels(gm, F.ng, F.ns, K, rnk.data() + F.s0, S, ind.data() + F.s0, F.g0, cap);
+    if (b->opts.voxel_mode) {  // voxel mode leaves the supervoxel lists as given
+      constexpr int M = TGMM_MAX_SUPERVOXELS_PER_GAUSSIAN;
+      for (long long k = 0; k < F.ng; k++) {
+        const int* d = b->sv_keep[i].data() + (size_t)k * (M + 1);
+        std::memcpy(gm[k].supervoxelIdx, d, sizeof(int) * M);
+        gm[k].supervoxelNum = d[M];
+      }
+    } else {
+      const int cap = b->opts.max_sv_per_gaussian;
+      assign_supervoxels(gm, F.ng, F.ns, K, rnk.data() + F.s0, S, ind.data() + F.s0, F.g0,
+                         cap);
+    }
   }
   for (int j = 0; j < K; j++)
     for (long long s = 0; s < F.ns; s++) {

@@ -52,6 +52,9 @@ struct tgmm_batch {
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
   int grid_cell = 0;  // knn_cell the kNN grids were last built with
+  // voxel mode: each frame's input supervoxelIdx[16] + supervoxelNum per
+  // Gaussian, returned unchanged by the fetch
+  std::vector<std::vector<int>> sv_keep;
 };
 
 void tgmm_set_error(const char* fmt, ...);

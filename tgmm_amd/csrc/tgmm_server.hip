@@ -256,6 +256,10 @@ extern "C" int tgmm_server_create(int device, const tgmm_server_opts* opts, tgmm
     tgmm_set_error("tgmm_server_create: bad EM options");
     return e.lambda_min < 0 ? TGMM_ERR_NOT_SET : TGMM_ERR_ARG;
   }
+  if (e.voxel_mode) {
+    tgmm_set_error("tgmm_server_create: voxel mode is served by batches, not by the server");
+    return TGMM_ERR_ARG;
+  }
   HIPCHK(hipSetDevice(device));
   tgmm_server* s = new tgmm_server();
   s->device = device;

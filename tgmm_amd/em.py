@@ -51,13 +51,15 @@ class EmOpts(ctypes.Structure):
         ("w4dof", ctypes.c_int),
         ("scale", ctypes.c_float * 3),
         ("max_sv_per_gaussian", ctypes.c_int),
+        ("voxel_mode", ctypes.c_int),
     ]
 
 
 def make_opts(K=5, max_iter=10, tol=-1.0, lambda_min=0.02, lambda_max=0.2222,
-              max_excentricity=9.0, w4dof=True, scale=(1.0, 1.0, 5.0), cap=16):
+              max_excentricity=9.0, w4dof=True, scale=(1.0, 1.0, 5.0), cap=16, voxel=False):
     """Defaults: parseConfigFile.h:193-195 constants, W4DOF as the tracker
-    (main.cpp:124), anisotropy (1, 1, 5)."""
+    (main.cpp:124), anisotropy (1, 1, 5). voxel=True selects the per-voxel
+    EM of GMEMvariationalInferenceCUDA (GMEMupdateCUDA.cu:3420-3973)."""
     o = EmOpts()
     o.K, o.max_iter, o.tol = int(K), int(max_iter), float(tol)
     o.lambda_min, o.lambda_max = float(lambda_min), float(lambda_max)
@@ -65,6 +67,7 @@ def make_opts(K=5, max_iter=10, tol=-1.0, lambda_min=0.02, lambda_max=0.2222,
     o.w4dof = 1 if w4dof else 0
     o.scale[:] = [float(x) for x in scale]
     o.max_sv_per_gaussian = int(cap)
+    o.voxel_mode = 1 if voxel else 0
     return o
 
 
@@ -156,6 +159,9 @@ def load():
         ctypes.c_ushort, ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int,
         ctypes.c_int, ctypes.c_char_p]
     L.tgmm_GMEMreleaseMemoryWithSupervoxels.argtypes = [_pp] * 7
+    L.tgmm_GMEMvariationalInferenceCUDA.argtypes = [
+        _vp, _vp, _vp, _vp, _vp, _vp, _vp, ctypes.c_longlong, ctypes.c_int, ctypes.c_int,
+        ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_char_p]
     L.tgmm_batch_create.argtypes = [ctypes.c_int, ctypes.POINTER(FrameDesc), ctypes.c_int,
                                     ctypes.POINTER(EmOpts), _pp]
     L.tgmm_batch_reset.argtypes = [_vp]
@@ -192,15 +198,10 @@ _hip = None
 def pinned(a):
     """A page-locked (hipHostMalloc) copy of numpy array `a`, or `a` itself if
     page-locked memory cannot be had. The memory is freed with the array."""
-    global _hip
     import weakref
     a = np.ascontiguousarray(a)
     try:
-        if _hip is None:
-            _hip = ctypes.CDLL("libamdhip64.so")
-            _hip.hipHostMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t,
-                                           ctypes.c_uint]
-            _hip.hipHostFree.argtypes = [ctypes.c_void_p]
+        _hip_lib()
         p = ctypes.c_void_p()
         if _hip.hipHostMalloc(ctypes.byref(p), ctypes.c_size_t(max(a.nbytes, 1)), 0) != 0:
             return a
@@ -309,6 +310,24 @@ def GMEMvariationalInferenceCUDAWithSupervoxels(d, vecGMtemp, query_nb, ref_nb, 
         debugPath.encode() if debugPath else None))
 
 
+def GMEMvariationalInferenceCUDA(queryCUDA, imgDataCUDA, rnkCUDA, indCUDA, indCUDAtr,
+                                 vecGMtemp, query_nb, ref_nb, maxIterEM, tolLikelihood,
+                                 devCUDA=0, frame=0, W4DOF=True, debugPath=""):
+    """One frame's per-voxel EM (GMEMupdateCUDA.h:87-91). Device arrays (see
+    DeviceArray): query f32 SoA [3N], img f32 [N], rnk f32 [5N], ind i32
+    [5N], indTr i32 [5N] or None. vecGMtemp: GMM_DTYPE array, updated in
+    place (supervoxelIdx/Num unchanged). rnk holds r * img of the last
+    iteration, slot-major."""
+    if vecGMtemp.dtype != GMM_DTYPE or not vecGMtemp.flags["C_CONTIGUOUS"]:
+        raise TypeError("vecGMtemp must be a C-contiguous GMM_DTYPE array (288-byte records)")
+    dp = lambda a: None if a is None else (a.ptr if isinstance(a, DeviceArray) else a)
+    _check(load().tgmm_GMEMvariationalInferenceCUDA(
+        dp(queryCUDA), dp(imgDataCUDA), dp(rnkCUDA), None, dp(indCUDA), dp(indCUDAtr),
+        vecGMtemp.ctypes.data, int(query_nb), int(ref_nb), int(maxIterEM),
+        float(tolLikelihood), int(devCUDA), int(frame), 1 if W4DOF else 0,
+        debugPath.encode() if debugPath else None))
+
+
 def GMEMreleaseMemoryWithSupervoxels(d):
     """Free the seven arrays and null them (GMEMupdateCUDA.h:140-144)."""
     _check(load().tgmm_GMEMreleaseMemoryWithSupervoxels(
@@ -317,15 +336,56 @@ def GMEMreleaseMemoryWithSupervoxels(d):
         ctypes.byref(d.centroidLabelPositionCUDA), ctypes.byref(d.labelListPtrCUDA)))
 
 
-_hip = None
+def _hip_lib():
+    """libamdhip64 through ctypes, for host-side buffer management only."""
+    global _hip
+    if _hip is None:
+        _hip = ctypes.CDLL("libamdhip64.so")
+    _hip.hipHostMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t,
+                                   ctypes.c_uint]
+    _hip.hipHostFree.argtypes = [ctypes.c_void_p]
+    _hip.hipMemcpy.argtypes = [_vp, _vp, ctypes.c_size_t, ctypes.c_int]
+    _hip.hipMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
+    _hip.hipFree.argtypes = [_vp]
+    return _hip
+
+
+class DeviceArray:
+    """A device buffer (hipMalloc) holding a copy of a numpy array, or
+    `nbytes` of zeros; freed with the object. For callers of the drop-in
+    entry points that take device pointers."""
+
+    def __init__(self, host=None, nbytes=None):
+        H = _hip_lib()
+        self.ptr = ctypes.c_void_p()
+        n = host.nbytes if host is not None else int(nbytes)
+        if H.hipMalloc(ctypes.byref(self.ptr), max(n, 1)) != 0:
+            raise TgmmError(-3, "hipMalloc of %d bytes failed" % n)
+        self.nbytes = n
+        if host is not None:
+            h = np.ascontiguousarray(host)
+            if H.hipMemcpy(self.ptr, h.ctypes.data, n, 1) != 0:  # HostToDevice
+                raise TgmmError(-4, "hipMemcpy H2D failed")
+        else:
+            z = np.zeros(max(n, 1), np.uint8)
+            if H.hipMemcpy(self.ptr, z.ctypes.data, max(n, 1), 1) != 0:
+                raise TgmmError(-4, "hipMemcpy H2D failed")
+
+    def numpy(self, dtype):
+        return copy_device_to_host(self.ptr, self.nbytes, dtype)
+
+    def __del__(self):
+        try:
+            if self.ptr:
+                _hip_lib().hipFree(self.ptr)
+                self.ptr = ctypes.c_void_p()
+        except Exception:
+            pass
 
 
 def copy_device_to_host(dev_ptr, nbytes, dtype):
     """hipMemcpy of a device buffer (e.g. rnkCUDA) into a new numpy array."""
-    global _hip
-    if _hip is None:
-        _hip = ctypes.CDLL("libamdhip64.so")
-        _hip.hipMemcpy.argtypes = [_vp, _vp, ctypes.c_size_t, ctypes.c_int]
+    _hip_lib()
     out = np.empty(nbytes // np.dtype(dtype).itemsize, dtype)
     rc = _hip.hipMemcpy(out.ctypes.data, dev_ptr, nbytes, 2)  # DeviceToHost
     if rc != 0:
@@ -347,16 +407,20 @@ class Batch:
         self._keep = []
         n = len(self.frames)
         self.desc = (FrameDesc * n)()
+        self.voxel = bool(opts.voxel_mode)
         for i, f in enumerate(self.frames):
-            arrs = [np.ascontiguousarray(f.query, np.float32),
-                    np.ascontiguousarray(f.img, np.float32),
-                    np.ascontiguousarray(f.centroids, np.float32),
-                    np.ascontiguousarray(f.sv_ptr, np.int64),
-                    as_records(f.gm)]
+            img = np.ascontiguousarray(f.img, np.float32)
+            if self.voxel:  # one-voxel supervoxels: the supervoxel arrays are unused
+                ce = np.zeros(3, np.float32)
+                pt = np.arange(2, dtype=np.int64)
+            else:
+                ce = np.ascontiguousarray(f.centroids, np.float32)
+                pt = np.ascontiguousarray(f.sv_ptr, np.int64)
+            arrs = [np.ascontiguousarray(f.query, np.float32), img, ce, pt, as_records(f.gm)]
             self._keep.append(arrs)
             d = self.desc[i]
             d.n_voxels = arrs[1].shape[0]
-            d.n_supervoxels = arrs[3].shape[0] - 1
+            d.n_supervoxels = arrs[1].shape[0] if self.voxel else arrs[3].shape[0] - 1
             d.n_gaussians = arrs[4].shape[0]
             d.query, d.img, d.centroids, d.sv_ptr, d.gm = [a.ctypes.data for a in arrs]
         self.h = ctypes.c_void_p()
@@ -371,11 +435,13 @@ class Batch:
         _check(self.L.tgmm_batch_solve(self.h, mode))
 
     def fetch(self):
-        """List of dicts: gm (records), rnk/ind (slot-major [K, S]), ll, iterations."""
+        """List of dicts: gm (records), rnk/ind (slot-major [K, S]; [K, N] in
+        voxel mode), ll, iterations."""
         K = self.opts.K
         outs = []
         for i, f in enumerate(self.frames):
-            S = int(np.asarray(f.sv_ptr).shape[0] - 1)
+            S = (int(np.asarray(f.img).shape[0]) if self.voxel
+                 else int(np.asarray(f.sv_ptr).shape[0] - 1))
             gm = np.zeros(int(np.asarray(f.gm).shape[0]), GMM_DTYPE)
             rnk = np.zeros(K * S, np.float32)
             ind = np.zeros(K * S, np.int32)
