@@ -116,6 +116,8 @@ def main():
                     help="config 4: server workspace budget")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default launch shapes (for counter runs)")
+    ap.add_argument("--voxel", action="store_true",
+                    help="per-voxel EM (GMEMvariationalInferenceCUDA) on the same frames")
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
                     help="also run one device-to-device copy of this many MiB (hipMemcpy), "
                          "the known-size reference for HBM traffic counters")
@@ -138,9 +140,12 @@ def main():
     print("# generated %d frame(s) in %.1fs: N=%d S=%d G=%d" % (
         len(frames), gen_s, sum(f.N for f in frames), sum(f.S for f in frames),
         sum(f.G for f in frames)), file=sys.stderr, flush=True)
-    opts = em.make_opts(K=K, max_iter=iters, tol=-1.0)
+    vox = bool(args.voxel)
+    if vox:
+        workload += "; voxel mode (every voxel its own kNN query)"
+    opts = em.make_opts(K=K, max_iter=iters, tol=-1.0, voxel=vox)
     batch = em.Batch(frames, opts)
-    S_tot = sum(f.S for f in frames)
+    S_tot = sum((f.N if vox else f.S) for f in frames)
     N_tot = sum(f.N for f in frames)
 
     # accuracy: one solve against the FP64 oracle (all frames when cheap)
@@ -150,7 +155,8 @@ def main():
         import oracle_lib as O
         from helpers import record_errors, rel_err
         if chk_iters != iters:
-            with em.Batch(frames, em.make_opts(K=K, max_iter=chk_iters, tol=-1.0)) as cb:
+            with em.Batch(frames, em.make_opts(K=K, max_iter=chk_iters, tol=-1.0,
+                                               voxel=vox)) as cb:
                 cb.solve(em.SOLVE_GRAPH)
                 outs = cb.fetch()
         else:
@@ -163,7 +169,8 @@ def main():
         sample = [frames[i] for i in idx]
         for i, f in zip(idx, sample):
             o = outs[i]
-            gm_o, rnk_o, ind_o, hist, it, ll = O.em_run(f, O.make_opts(K=K, max_iter=chk_iters))
+            gm_o, rnk_o, ind_o, hist, it, ll = O.em_run(f, O.make_opts(K=K, max_iter=chk_iters,
+                                                                        voxel=vox))
             e = record_errors(o["gm"], gm_o)
             e["ll"] = rel_err(o["ll"], ll)
             errs.append(e)
@@ -251,7 +258,7 @@ def main():
         idx = [0] if len(frames) == 1 else list(range(0, len(frames), 8))
         t1 = time.perf_counter()
         for i in idx:
-            O.em_run(frames[i], O.make_opts(K=K, max_iter=n_it))
+            O.em_run(frames[i], O.make_opts(K=K, max_iter=n_it, voxel=vox))
         dt = time.perf_counter() - t1
         per_step = dt / n_it * iters * len(frames) / len(idx)
         f = frames[0]

@@ -25,6 +25,8 @@
 //                  (1913-1978), per-frame total, convergence (4145-4148)
 #include <hip/hip_runtime.h>
 
+#include <climits>
+
 #include "tgmm_device.h"
 #include "tgmm_internal.h"
 
@@ -933,20 +935,73 @@ __global__ __launch_bounds__(64 * NW) void k_knn_cell(DevPtrs P, KOpts O) {
 // with n, c, Q the voxel count, mean and scatter of the supervoxel, so each
 // (s, k) score is one FP64 evaluation; normalisation is max-subtracted
 // log-sum-exp with FP32 exponentials.
+// Runs of equal keys in consecutive lanes of a wave (key < 0: no entry).
+// Neighbouring supervoxels, and in voxel mode neighbouring voxels, mostly
+// share their j-th candidate, so one atomic per run replaces one per lane.
+// Every lane of the wave must call these together.
+struct WaveRun {
+  int head;  // lane that starts this lane's run
+  int len;   // run length (valid on the head lane)
+};
+
+__device__ inline WaveRun wave_run(int key) {
+  const int lane = __lane_id();
+  const int prev = __shfl_up(key, 1, 64);
+  const bool cont = key >= 0 && lane > 0 && prev == key;
+  const unsigned long long m = __ballot(cont);
+  const unsigned long long heads = __ballot(key >= 0 && !cont);
+  WaveRun r;
+  const unsigned long long below = heads & (lane == 63 ? ~0ULL : ((2ULL << lane) - 1));
+  r.head = below ? 63 - __builtin_clzll(below) : lane;
+  const unsigned long long rest = lane == 63 ? ~0ULL : ~(m >> (lane + 1));
+  r.len = 1 + __builtin_ctzll(rest);
+  return r;
+}
+
+// cnt[key] += 1 for every lane with key >= 0.
+__device__ inline void wave_count(int* cnt, int key) {
+  const WaveRun r = wave_run(key);
+  if (key >= 0 && r.head == __lane_id()) atomicAdd(cnt + key, r.len);
+}
+
+// Takes one slot per lane with key >= 0 from the counter cnt[key] (counting
+// down); returns the slot (distinct per lane), or -1.
+__device__ inline int wave_take(int* cnt, int key) {
+  const WaveRun r = wave_run(key);
+  const int lane = __lane_id();
+  int old = 0;
+  if (key >= 0 && r.head == lane) old = atomicSub(cnt + key, r.len);
+  old = __shfl(old, r.head, 64);
+  return key >= 0 ? old - 1 - (lane - r.head) : -1;
+}
+
 template <int KM>
 __global__ __launch_bounds__(256) void k_estep(DevPtrs P, KOpts O) {
   const long long s = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   const int f = P.sblk_frame[s >> 8];
-  if (f < 0 || !P.fr[f].active) return;
-  if (P.sv_frame[s] != f) return;
+  // no early exit: every lane takes part in the wave-wide pair counting
+  const bool ok = f >= 0 && P.fr[f].active && P.sv_frame[s] == f;
   const long long S = P.Scap;
   const int K = O.K;
-  const double psi_sum = P.fr[f].psi_sum_alpha;
-  const double n = P.mom[s];
-  const double c0 = P.mom[S + s], c1 = P.mom[2 * S + s], c2 = P.mom[3 * S + s];
-  double Q[6];
+  const double psi_sum = ok ? P.fr[f].psi_sum_alpha : 0.0;
+  double n = 0.0, c0 = 0.0, c1 = 0.0, c2 = 0.0, Q[6] = {0, 0, 0, 0, 0, 0};
+  if (!ok) {
+  } else if (O.voxel) {  // the voxel itself: count 1, no scatter (the same values as mom)
+    const long long v = P.sv_beg[s];
+    n = 1.0;
+    c0 = (double)P.vx[v];
+    c1 = (double)P.vy[v];
+    c2 = (double)P.vz[v];
 #pragma unroll
-  for (int i = 0; i < 6; i++) Q[i] = P.mom[(4 + i) * S + s];
+    for (int i = 0; i < 6; i++) Q[i] = 0.0;
+  } else {
+    n = P.mom[s];
+    c0 = P.mom[S + s];
+    c1 = P.mom[2 * S + s];
+    c2 = P.mom[3 * S + s];
+#pragma unroll
+    for (int i = 0; i < 6; i++) Q[i] = P.mom[(4 + i) * S + s];
+  }
   double L[KM];
   int kk[KM];
   double mx = -INFINITY;
@@ -954,7 +1009,7 @@ __global__ __launch_bounds__(256) void k_estep(DevPtrs P, KOpts O) {
   for (int j = 0; j < KM; j++) {
     L[j] = -INFINITY;
     kk[j] = -1;
-    if (j < K) {
+    if (ok && j < K) {
       const int k = P.ind[(long long)j * S + s];
       kk[j] = k;
       if (k >= 0) {
@@ -984,13 +1039,13 @@ __global__ __launch_bounds__(256) void k_estep(DevPtrs P, KOpts O) {
     }
   }
   // voxel mode stores r times the voxel's intensity (GMEM.cu:2216-2222)
-  const float wv = O.voxel ? P.vimg[P.sv_beg[s]] : 1.0f;
+  const float wv = (ok && O.voxel) ? P.vimg[P.sv_beg[s]] : 1.0f;
 #pragma unroll
   for (int j = 0; j < KM; j++) {
-    if (j < K) {
+    if (j < K) {  // K is uniform: the whole wave runs this
       const float r = ((den > 0.0f || den != den) ? e[j] / den : 0.0f) * wv;
-      P.rnk[(long long)j * S + s] = r;
-      if (kk[j] >= 0 && r > 1e-3f) atomicAdd(P.seg_cnt + kk[j], 1);
+      if (ok) P.rnk[(long long)j * S + s] = r;
+      wave_count(P.seg_cnt, (ok && kk[j] >= 0 && r > 1e-3f) ? kk[j] : -1);
     }
   }
 }
@@ -998,34 +1053,80 @@ __global__ __launch_bounds__(256) void k_estep(DevPtrs P, KOpts O) {
 __global__ __launch_bounds__(256) void k_pair_scatter(DevPtrs P, KOpts O) {
   const long long s = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   const int f = P.sblk_frame[s >> 8];
-  if (f < 0 || !P.fr[f].active) return;
-  if (P.sv_frame[s] != f) return;
+  // no early exit: every lane takes part in the wave-wide slot reservation
+  const bool ok = f >= 0 && P.fr[f].active && P.sv_frame[s] == f;
   const long long S = P.Scap;
   for (int j = 0; j < O.K; j++) {
-    const int k = P.ind[(long long)j * S + s];
-    if (k < 0) continue;
-    if (P.rnk[(long long)j * S + s] > 1e-3f) {
-      const int pos = P.seg_start[k] + atomicSub(P.seg_cnt + k, 1) - 1;
-      P.pairs[pos] = (int)((s << kPairShift) | j);
-    }
+    const int k = ok ? P.ind[(long long)j * S + s] : -1;
+    const bool take = k >= 0 && P.rnk[(long long)j * S + s] > 1e-3f;
+    const int slot = wave_take(P.seg_cnt, take ? k : -1);
+    if (take) P.pairs[P.seg_start[k] + slot] = (int)((s << kPairShift) | j);
   }
 }
 
 // 16 lanes per Gaussian: order its pair list by code (= by supervoxel) with a
-// rank sort, so that the FP64 accumulation order is fixed.
-__global__ __launch_bounds__(256) void k_seg_sort(DevPtrs P) {
+// rank sort, so that the FP64 accumulation order is fixed. In voxel mode
+// (hundreds of voxels per Gaussian) lists longer than kSortShort are left to
+// k_seg_sort_long; both give the same ascending order.
+constexpr int kSortShort = 128;
+constexpr int kSortLds = 4096;  // longest list sorted in LDS
+
+__global__ __launch_bounds__(256) void k_seg_sort(DevPtrs P, int short_max) {
   const long long k = (long long)blockIdx.x * 16 + (threadIdx.x >> 4);
   const int lane = threadIdx.x & 15;
   const int f = P.gblk_frame[k / kGB];
   if (f < 0 || !P.fr[f].active) return;
   if (P.g_frame[k] != f) return;
   const int st = P.seg_start[k], L = P.seg_start[k + 1] - st;
+  if (L > short_max) return;
   for (int i = lane; i < L; i += 16) {
     const int e = P.pairs[st + i];
     int rank = 0;
     for (int j = 0; j < L; j++) rank += (P.pairs[st + j] < e) ? 1 : 0;
     P.pairs_sorted[st + rank] = e;
   }
+}
+
+// One block per Gaussian with a long pair list: bitonic sort in LDS (codes
+// are distinct: a supervoxel lists a Gaussian at most once), or a block-wide
+// rank sort beyond kSortLds entries.
+__global__ __launch_bounds__(256) void k_seg_sort_long(DevPtrs P) {
+  __shared__ int sh[kSortLds];
+  const long long k = blockIdx.x;
+  const int f = P.gblk_frame[k / kGB];
+  if (f < 0 || !P.fr[f].active) return;
+  if (P.g_frame[k] != f) return;
+  const int st = P.seg_start[k], L = P.seg_start[k + 1] - st;
+  if (L <= kSortShort) return;
+  const int t = threadIdx.x;
+  if (L > kSortLds) {
+    for (int i = t; i < L; i += 256) {
+      const int e = P.pairs[st + i];
+      int rank = 0;
+      for (int j = 0; j < L; j++) rank += (P.pairs[st + j] < e) ? 1 : 0;
+      P.pairs_sorted[st + rank] = e;
+    }
+    return;
+  }
+  int n = 1;
+  while (n < L) n <<= 1;
+  for (int i = t; i < n; i += 256) sh[i] = i < L ? P.pairs[st + i] : INT_MAX;
+  __syncthreads();
+  for (int w = 2; w <= n; w <<= 1)
+    for (int j = w >> 1; j > 0; j >>= 1) {
+      for (int i = t; i < n; i += 256) {
+        const int p = i ^ j;
+        if (p > i) {
+          const int a = sh[i], b = sh[p];
+          if ((a > b) == ((i & w) == 0)) {
+            sh[i] = b;
+            sh[p] = a;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  for (int i = t; i < L; i += 256) P.pairs_sorted[st + i] = sh[i];
 }
 
 // Segmented FP64 sufficient statistics over each Gaussian's significant
@@ -1626,7 +1727,8 @@ static hipError_t iteration_t(const DevPtrs& P, const KOpts& O, hipStream_t st, 
   if (tk == TK_ESTEP && e1) (void)hipEventRecord(e1, st);
   k_scan_frames<<<P.nframes, 1024, 0, st>>>(P, 1, 1, O.K);
   k_pair_scatter<<<nb(P.Sext, bs), bs, 0, st>>>(P, O);
-  k_seg_sort<<<nb(P.Gext, 16), 256, 0, st>>>(P);
+  k_seg_sort<<<nb(P.Gext, 16), 256, 0, st>>>(P, O.voxel ? kSortShort : INT_MAX);
+  if (O.voxel) k_seg_sort_long<<<(unsigned)P.Gext, 256, 0, st>>>(P);
   if (tk == TK_MSTEP && e0) (void)hipEventRecord(e0, st);
   launch_mstep_stats(P, O, st);
   k_mstep_update<<<gb, kGB, 0, st>>>(P, O);
