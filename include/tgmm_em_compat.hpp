@@ -1,9 +1,10 @@
-// tgmm_em_compat.hpp -- drop-in replacement, for the tracker's supervoxel EM
-// path, of the reference header nmeth.3036-S2/src/UtilsCUDA/GMEMupdateCUDA.h.
+// tgmm_em_compat.hpp -- drop-in replacement, for the tracker's EM path
+// (supervoxel and voxel mode), of the reference header
+// nmeth.3036-S2/src/UtilsCUDA/GMEMupdateCUDA.h.
 //
 // The tracker includes this file where it included GMEMupdateCUDA.h and links
 // libtgmm_em.so instead of the CUDA objects. The inline functions below carry
-// the reference's names and parameter lists (GMEMupdateCUDA.h:82, 92-98,
+// the reference's names and parameter lists (GMEMupdateCUDA.h:82, 87-98,
 // 132-144) and its error behaviour: print a message and exit(), with the
 // exit codes the reference uses (0 for the size limits, GMEM.cu:3992-4007;
 // 3 when the regularisation constants were never set, GMEM.cu:4217-4222;
@@ -67,6 +68,24 @@ inline void GMEMvariationalInferenceCUDAWithSupervoxels(
           centroidLabelPositionCUDA, labelListPtrCUDA, vecGMtemp, query_nb, ref_nb, maxLabels,
           maxIterEM, tolLikelihood, devCUDA, frame, W4DOF ? 1 : 0, debugPath.c_str()),
       "GMEMvariationalInferenceCUDAWithSupervoxels");
+}
+
+inline void GMEMvariationalInferenceCUDA(float* queryCUDA, float* imgDataCUDA, pxi* rnkCUDA,
+                                         pxi* rnkCUDAtr, int* indCUDA, int* indCUDAtr,
+                                         GaussianMixtureModelCUDA* vecGMtemp,
+                                         long long int query_nb, int ref_nb, int maxIterEM,
+                                         double tolLikelihood, int devCUDA, int frame, bool W4DOF,
+                                         std::string debugPath = "") {
+  if (regularizePrecisionMatrixConstants::lambdaMin >= 0.0)
+    tgmm_setRegularizePrecisionMatrixConstants(regularizePrecisionMatrixConstants::lambdaMin,
+                                               regularizePrecisionMatrixConstants::lambdaMax,
+                                               regularizePrecisionMatrixConstants::maxExcentricity);
+  tgmm_compat::check(
+      tgmm_GMEMvariationalInferenceCUDA(queryCUDA, imgDataCUDA, rnkCUDA, rnkCUDAtr, indCUDA,
+                                        indCUDAtr, vecGMtemp, query_nb, ref_nb, maxIterEM,
+                                        tolLikelihood, devCUDA, frame, W4DOF ? 1 : 0,
+                                        debugPath.c_str()),
+      "GMEMvariationalInferenceCUDA");
 }
 
 inline void GMEMreleaseMemoryWithSupervoxels(float** queryCUDA, float** imgDataCUDA,
