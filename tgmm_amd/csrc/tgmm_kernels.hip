@@ -758,27 +758,28 @@ __global__ __launch_bounds__(256) void k_knn(DevPtrs P, KOpts O) {
       pb[t] = b0;
       cum[t + 1] = cum[t] + (e0 - b0);
     }
-    // row by row (the row's bounds picked once per row), KC references per
-    // lane in flight
-#pragma unroll 1
-    for (int t = 0; t < 9; t++) {
-      int rb = pb[0], len = cum[1] - cum[0];
+    // the 9 trimmed rows as one range of cum[9] references: KC per lane in
+    // flight across row boundaries, so a lane waits for memory once per KC
+    // references rather than once per row (the K best do not depend on the
+    // order references are considered in)
+    const int tot = cum[9];
+    for (int p0 = sub; p0 < tot; p0 += KC * QL) {
+      float4 v[KC];
 #pragma unroll
-      for (int r = 1; r < 9; r++)
-        if (t == r) {
-          rb = pb[r];
-          len = cum[r + 1] - cum[r];
+      for (int w = 0; w < KC; w++) {
+        const int p = p0 + w * QL;
+        v[w] = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (p < tot) {
+          int a = pb[0] + p;  // row of p: the last t with cum[t] <= p
+#pragma unroll
+          for (int t = 1; t < 9; t++)
+            if (p >= cum[t]) a = pb[t] + (p - cum[t]);
+          v[w] = P.cg4[a];
         }
-      const int re = rb + len;
-      for (int p = rb + sub; p < re; p += KC * QL) {
-        float4 v[KC];
-#pragma unroll
-        for (int w = 0; w < KC; w++)
-          v[w] = (p + w * QL < re) ? P.cg4[p + w * QL] : make_float4(0.f, 0.f, 0.f, 0.f);
-#pragma unroll
-        for (int w = 0; w < KC; w++)
-          if (p + w * QL < re) consider(v[w]);
       }
+#pragma unroll
+      for (int w = 0; w < KC; w++)
+        if (p0 + w * QL < tot) consider(v[w]);
     }
   }
   if (QL > 1) knn_merge<KM, QL>(bd, bi);
