@@ -1447,7 +1447,9 @@ __global__ __launch_bounds__(256) void k_loglik(DevPtrs P, KOpts O) {
   constexpr int U = 64 / GW;     // voxels per lane per pass
   static_assert(B <= 256 && 256 % B == 0, "block covers a divisor of 256 supervoxels");
   static_assert(GW >= KM, "a lane per candidate");
+  constexpr bool kLds = GW < 64;  // candidate constants staged in LDS
   __shared__ double svsum[B];
+  __shared__ float4 cst[kLds ? NGB * KM * 3 : 1];
   const long long sb = (long long)blockIdx.x * B;
   const int lane = threadIdx.x % GW, grp = threadIdx.x / GW;
   const int K = O.K;
@@ -1487,38 +1489,45 @@ __global__ __launch_bounds__(256) void k_loglik(DevPtrs P, KOpts O) {
       pe2 = (float)((double)pu[5] * d2);
       pg = (float)(lp.lg2 - lsa2);
     }
-    float ce0[KM], ce1[KM], ce2[KM], cu[KM][6], cg[KM];
-#pragma unroll
-    for (int j = 0; j < KM; j++) {
-      ce0[j] = group_bcast<GW>(pe0, j);
-      ce1[j] = group_bcast<GW>(pe1, j);
-      ce2[j] = group_bcast<GW>(pe2, j);
-#pragma unroll
-      for (int i = 0; i < 6; i++) cu[j][i] = group_bcast<GW>(pu[i], j);
-      cg[j] = (j < K) ? group_bcast<GW>(pg, j) : -INFINITY;
-    }
-    // img * log2(sum_j 2^t_j) of one voxel, rounded to FP64
-    auto term = [&](const float4 x) -> double {
-#pragma clang fp contract(off)
-      const float x0 = x.x - c.a0, x1 = x.y - c.a1, x2 = x.z - c.a2;
-      float tj[KM];
-      float mx = -INFINITY;
+    // the group's candidate constants: a whole-wave group reads them into
+    // scalars; smaller groups stage them in LDS (12 floats per candidate,
+    // read back as broadcasts), so that only one candidate's constants are
+    // live in registers at a time
+    float ce0[kLds ? 1 : KM], ce1[kLds ? 1 : KM], ce2[kLds ? 1 : KM], cu[kLds ? 1 : KM][6],
+        cg[kLds ? 1 : KM];
+    if constexpr (kLds) {
+      __syncthreads();  // the previous supervoxel's constants are no longer read
+      if (lane < KM) {
+        float4* d = cst + (grp * KM + lane) * 3;
+        d[0] = make_float4(pu[0], pu[1], pu[2], pe0);
+        d[1] = make_float4(pu[3], pu[4], pe1, pu[5]);
+        d[2] = make_float4(pe2, lane < K ? pg : -INFINITY, 0.0f, 0.0f);
+      }
+      __syncthreads();
+    } else {
 #pragma unroll
       for (int j = 0; j < KM; j++) {
-        const float y0 = __builtin_fmaf(cu[j][0], x0,
-                                        __builtin_fmaf(cu[j][1], x1,
-                                                       __builtin_fmaf(cu[j][2], x2, ce0[j])));
-        const float y1 = __builtin_fmaf(cu[j][3], x1, __builtin_fmaf(cu[j][4], x2, ce1[j]));
-        const float y2 = __builtin_fmaf(cu[j][5], x2, ce2[j]);
-        const float q = __builtin_fmaf(y0, y0, __builtin_fmaf(y1, y1, y2 * y2));
-        tj[j] = cg[j] - q;
-        mx = fmaxf(mx, tj[j]);
-      }
-      if (!(mx > -1074.0f)) return 0.0;
-      float sum = 0.0f;
+        ce0[j] = group_bcast<GW>(pe0, j);
+        ce1[j] = group_bcast<GW>(pe1, j);
+        ce2[j] = group_bcast<GW>(pe2, j);
 #pragma unroll
-      for (int j = 0; j < KM; j++) sum += __builtin_amdgcn_exp2f(tj[j] - mx);
-      return (double)x.w * (double)(mx + __builtin_amdgcn_logf(sum));
+        for (int i = 0; i < 6; i++) cu[j][i] = group_bcast<GW>(pu[i], j);
+        cg[j] = (j < K) ? group_bcast<GW>(pg, j) : -INFINITY;
+      }
+    }
+    // candidate j's constants: U row-major (u00 u01 u02 u11 u12 u22), e, log2 weight
+    auto cand = [&](int j, float (&u)[6], float& e0, float& e1, float& e2, float& g) {
+      if constexpr (kLds) {
+        const float4* c4 = cst + (grp * KM + j) * 3;
+        const float4 A = c4[0], B = c4[1], C = c4[2];
+        u[0] = A.x; u[1] = A.y; u[2] = A.z; e0 = A.w;
+        u[3] = B.x; u[4] = B.y; e1 = B.z; u[5] = B.w;
+        e2 = C.x; g = C.y;
+      } else {
+#pragma unroll
+        for (int i = 0; i < 6; i++) u[i] = cu[j][i];
+        e0 = ce0[j]; e1 = ce1[j]; e2 = ce2[j]; g = cg[j];
+      }
     };
     double cls = 0.0;  // class sum (lanes 0..15 of the group)
     for (int base = 0; base < n; base += 64) {
@@ -1533,9 +1542,51 @@ __global__ __launch_bounds__(256) void k_loglik(DevPtrs P, KOpts O) {
           }
         }
       }
+      // per voxel and candidate t = g - |U (x - a) + e|^2, candidates outer
+      float tj[U][KM];
+      {
+#pragma clang fp contract(off)
+        float xd[U][3];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+          xd[u][0] = vv[u].x - c.a0;
+          xd[u][1] = vv[u].y - c.a1;
+          xd[u][2] = vv[u].z - c.a2;
+        }
+#pragma unroll
+        for (int j = 0; j < KM; j++) {
+          float cu6[6], e0, e1, e2, g;
+          cand(j, cu6, e0, e1, e2, g);
+#pragma unroll
+          for (int u = 0; u < U; u++) {
+            const float x0 = xd[u][0], x1 = xd[u][1], x2 = xd[u][2];
+            const float y0 = __builtin_fmaf(cu6[0], x0,
+                                            __builtin_fmaf(cu6[1], x1,
+                                                           __builtin_fmaf(cu6[2], x2, e0)));
+            const float y1 = __builtin_fmaf(cu6[3], x1, __builtin_fmaf(cu6[4], x2, e1));
+            const float y2 = __builtin_fmaf(cu6[5], x2, e2);
+            const float q = __builtin_fmaf(y0, y0, __builtin_fmaf(y1, y1, y2 * y2));
+            tj[u][j] = g - q;
+          }
+        }
+      }
+      // img * log2(sum_j 2^t_j) of each voxel, rounded to FP64
       double v[U];
 #pragma unroll
-      for (int u = 0; u < U; u++) v[u] = (base + lane + GW * u < n) ? term(vv[u]) : 0.0;
+      for (int u = 0; u < U; u++) {
+#pragma clang fp contract(off)
+        float mx = -INFINITY;
+#pragma unroll
+        for (int j = 0; j < KM; j++) mx = fmaxf(mx, tj[u][j]);
+        double t = 0.0;
+        if (mx > -1074.0f) {
+          float sum = 0.0f;
+#pragma unroll
+          for (int j = 0; j < KM; j++) sum += __builtin_amdgcn_exp2f(tj[u][j] - mx);
+          t = (double)vv[u].w * (double)(mx + __builtin_amdgcn_logf(sum));
+        }
+        v[u] = (base + lane + GW * u < n) ? t : 0.0;
+      }
       // the pass's voxels of class (lane mod 16), in increasing index
 #pragma unroll
       for (int u = 0; u < U; u++) {
