@@ -1380,8 +1380,8 @@ __global__ __launch_bounds__(64) void k_alpha_sum(DevPtrs P, int which, int need
 
 // Log-likelihood over the voxel stream (GMEM.cu:1913-1978). A block has 256
 // threads in groups of GW lanes; each group takes SPG supervoxels in turn,
-// 64 voxels per pass (U = 64/GW per lane, all loads of a pass issued
-// together), with the next supervoxel's header loaded while the current one
+// GW*U voxels per pass (U = 2 for 16-lane groups, else 64/GW, all loads of
+// a pass issued together), with the next supervoxel's header loaded while the current one
 // is evaluated; a block covers B = (256/GW)*SPG consecutive supervoxels of
 // one frame. Lane j < K of a group reads candidate j's parameter pack; the
 // constants are broadcast to the group (into scalars for a whole-wave group).
@@ -1444,7 +1444,7 @@ template <int KM, int GW, int SPG>
 __global__ __launch_bounds__(256) void k_loglik(DevPtrs P, KOpts O) {
   constexpr int NGB = 256 / GW;  // groups per block
   constexpr int B = NGB * SPG;   // supervoxels per block
-  constexpr int U = 64 / GW;     // voxels per lane per pass
+  constexpr int U = GW == 16 ? 2 : 64 / GW;  // voxels per lane per pass
   static_assert(B <= 256 && 256 % B == 0, "block covers a divisor of 256 supervoxels");
   static_assert(GW >= KM, "a lane per candidate");
   constexpr bool kLds = GW < 64;  // candidate constants staged in LDS
@@ -1530,7 +1530,7 @@ __global__ __launch_bounds__(256) void k_loglik(DevPtrs P, KOpts O) {
       }
     };
     double cls = 0.0;  // class sum (lanes 0..15 of the group)
-    for (int base = 0; base < n; base += 64) {
+    for (int base = 0; base < n; base += GW * U) {
       if (base > 0) {
 #pragma unroll
         for (int u = 0; u < U; u++) {
