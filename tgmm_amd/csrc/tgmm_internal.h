@@ -45,7 +45,7 @@ struct FrameDev {
   int nbl;            // 256-supervoxel groups over the aligned range (log-likelihood)
   int max_iter;
   double tol;
-  double sum_alpha, psi_sum_alpha, log_sum_alpha, sum_alpha2;
+  double sum_alpha, psi_sum_alpha, log_sum_alpha;
   double ll, ll1, ll2, ll3;
   int iter;           // reference numIterEM (starts at 1)
   int active;         // keeps iterating
@@ -143,6 +143,7 @@ struct DevPtrs {
   float4* cg4;      // Gcap: cell-sorted references (x, y, z, index bits)
   // reductions
   double* part_g;   // Gcap/kGB
+  double* part_g2;  // Gcap/kGB: sums of alpha after the posterior update (death test)
   double* part_s;   // Scap/kLLminB: log-likelihood block partials
   double* part_sg;  // Scap/256: partials of 256-supervoxel groups
   FrameDev* fr;

@@ -1280,7 +1280,7 @@ __global__ __launch_bounds__(kGB) void k_mstep_update(DevPtrs P, KOpts O) {
     }
   }
   const double t = block_sum<1>(alpha_out, red);
-  if (threadIdx.x == 0) P.part_g[blockIdx.x] = t;
+  if (threadIdx.x == 0) P.part_g2[blockIdx.x] = t;
 }
 
 // Dead-cell rule (GMEM.cu:1813-1843), then the log-likelihood parameters and
@@ -1293,6 +1293,9 @@ __global__ __launch_bounds__(kGB) void k_death(DevPtrs P, KOpts O) {
   if (f < 0) return;
   FrameDev* F = P.fr + f;
   if (!F->active) return;
+  // the frame's sum of alpha after the posterior update, from the blocks'
+  // partials in index order (every block of the frame forms the same value)
+  const double sum_alpha2 = frame_sum_w64(P.part_g2 + F->g0 / kGB, F->nbg);
   const long long k = (long long)blockIdx.x * kGB + threadIdx.x;
   const long long G = P.Gcap;
   double alpha_out = 0.0;
@@ -1306,7 +1309,7 @@ __global__ __launch_bounds__(kGB) void k_death(DevPtrs P, KOpts O) {
     double alpha = gsv(P, GS_ALPHA, k);
     bool alive = gsv(P, GS_M_O, k) > -1e31;
     const bool fixed = P.fixed[k] != 0;
-    if (nanW || (alpha / F->sum_alpha2 < 1e-5 && alive && !fixed)) {
+    if (nanW || (alpha / sum_alpha2 < 1e-5 && alive && !fixed)) {
       alpha = 0.0;
       P.gs[GS_ALPHA * G + k] = 0.0;
       for (int i = 0; i < 3; i++) {
@@ -1357,23 +1360,18 @@ __global__ __launch_bounds__(kGB) void k_death(DevPtrs P, KOpts O) {
 
 // A frame's sum of alpha from the Gaussian-block partials of the kernel
 // before, one wave per frame, partials in index order (lane l takes l, l+64,
-// ... into four interleaved sums, then a fixed butterfly). which 0: the sum
-// used by E[ln pi] and the log-likelihood (with its digamma and log);
-// which 1: the sum before the death test.
-__global__ __launch_bounds__(64) void k_alpha_sum(DevPtrs P, int which, int need_active) {
+// ... into four interleaved sums, then a fixed butterfly): the sum used by
+// E[ln pi] and the log-likelihood, with its digamma and log.
+__global__ __launch_bounds__(64) void k_alpha_sum(DevPtrs P, int need_active) {
   const int f = (P.only_frame >= 0) ? P.only_frame : (int)blockIdx.x;
   if (f >= P.Fcap || (P.only_frame < 0 && f >= P.nframes)) return;
   FrameDev* F = P.fr + f;
   if (F->ncap == 0 || (need_active && !F->active)) return;
   const double tot = frame_sum_w64(P.part_g + F->g0 / kGB, F->nbg);
   if (threadIdx.x == 0) {
-    if (which == 1) {
-      F->sum_alpha2 = tot;
-    } else {
-      F->sum_alpha = tot;
-      F->psi_sum_alpha = digamma(tot);
-      F->log_sum_alpha = log(tot);
-    }
+    F->sum_alpha = tot;
+    F->psi_sum_alpha = digamma(tot);
+    F->log_sum_alpha = log(tot);
   }
 }
 
@@ -1665,7 +1663,7 @@ hipError_t launch_prep(const DevPtrs& P, const KOpts& O, hipStream_t st) {
 hipError_t launch_reset(const DevPtrs& P, const KOpts& O, hipStream_t st) {
   k_iter_init<<<nb(P.nframes, 64), 64, 0, st>>>(P);
   k_alpha_init<<<nb(P.Gext, kGB), kGB, 0, st>>>(P);
-  k_alpha_sum<<<P.only_frame >= 0 ? 1 : P.nframes, 64, 0, st>>>(P, 0, 1);
+  k_alpha_sum<<<P.only_frame >= 0 ? 1 : P.nframes, 64, 0, st>>>(P, 1);
   k_gauss_prep<<<nb(P.Gext, kGB), kGB, 0, st>>>(P, O);
   return hipGetLastError();
 }
@@ -1784,10 +1782,9 @@ static hipError_t iteration_t(const DevPtrs& P, const KOpts& O, hipStream_t st, 
   if (tk == TK_MSTEP && e0) (void)hipEventRecord(e0, st);
   launch_mstep_stats(P, O, st);
   k_mstep_update<<<gb, kGB, 0, st>>>(P, O);
-  k_alpha_sum<<<P.nframes, 64, 0, st>>>(P, 1, 1);
   if (tk == TK_MSTEP && e1) (void)hipEventRecord(e1, st);
   k_death<<<gb, kGB, 0, st>>>(P, O);
-  k_alpha_sum<<<P.nframes, 64, 0, st>>>(P, 0, 1);
+  k_alpha_sum<<<P.nframes, 64, 0, st>>>(P, 1);
   if (tk == TK_LOGLIK && e0) (void)hipEventRecord(e0, st);
   launch_loglik<KM>(P, O, st, tk == TK_LOGLIK ? e1 : nullptr);
   return hipGetLastError();

@@ -117,6 +117,7 @@ int ws_alloc(Workspace* w, const WsShape& sh, int K) {
   AL(P.cblk_frame, C / kCB + 1);
   AL(P.cg4, G);
   AL(P.part_g, G / kGB);
+  AL(P.part_g2, G / kGB);
   AL(P.part_s, S / kLLminB);
   AL(P.part_sg, S / 256);
   AL(P.fr, F);
