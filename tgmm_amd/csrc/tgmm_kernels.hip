@@ -1390,9 +1390,9 @@ __global__ __launch_bounds__(64) void k_alpha_sum(DevPtrs P, int need_active) {
 // from the origin (9 FMA-class operations per candidate). Terms are evaluated
 // in base 2 (hardware exp2/log2), log-sum-exp with the largest term factored
 // out; a voxel's contribution img * term is rounded to FP64 and accumulated
-// in FP64. A voxel whose largest term is below the
-// FP64 underflow of the reference's direct sum (2^-1074) contributes nothing,
-// as in the reference.
+// in FP64. A candidate whose term underflows in the reference's double
+// evaluation (its exponential or the whole term below 2^-1075) adds nothing,
+// and a voxel with no term left contributes nothing (GMEM.cu:1958-1959).
 //
 // Summation order, identical for every (GW, SPG): voxel i of a supervoxel is
 // added, in increasing i, to class sum i mod 16; the 16 class sums are added
@@ -1564,7 +1564,10 @@ __global__ __launch_bounds__(256) void k_loglik(DevPtrs P, KOpts O) {
             const float y1 = __builtin_fmaf(cu6[3], x1, __builtin_fmaf(cu6[4], x2, e1));
             const float y2 = __builtin_fmaf(cu6[5], x2, e2);
             const float q = __builtin_fmaf(y0, y0, __builtin_fmaf(y1, y1, y2 * y2));
-            tj[u][j] = g - q;
+            // the reference's double term underflows to 0 when its exponential
+            // (2^-q) or the whole term (2^t) is below 2^-1075
+            const float t = g - q;
+            tj[u][j] = (q <= 1075.0f && t >= -1075.0f) ? t : -INFINITY;
           }
         }
       }
@@ -1577,7 +1580,7 @@ __global__ __launch_bounds__(256) void k_loglik(DevPtrs P, KOpts O) {
 #pragma unroll
         for (int j = 0; j < KM; j++) mx = fmaxf(mx, tj[u][j]);
         double t = 0.0;
-        if (mx > -1074.0f) {
+        if (mx > -INFINITY) {
           float sum = 0.0f;
 #pragma unroll
           for (int j = 0; j < KM; j++) sum += __builtin_amdgcn_exp2f(tj[u][j] - mx);
