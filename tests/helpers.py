@@ -97,3 +97,13 @@ def device_alpha_sum(alpha):
             q += 1
         acc.append((a4[0] + a4[1]) + (a4[2] + a4[3]))
     return _butterfly64(acc)
+
+
+def records_equal(a, b):
+    """Bitwise equality of every named field of two record arrays (the three
+    padding bytes after `fixed` are not compared: numpy's copy of a padded
+    structured array need not copy them)."""
+    if a.shape != b.shape:
+        return False
+    return all(np.ascontiguousarray(a[k]).tobytes() == np.ascontiguousarray(b[k]).tobytes()
+               for k in a.dtype.names if k)

@@ -71,7 +71,9 @@ def test_fixed_and_dead_input_gaussians(base):
     gm["alpha_k"][dead] = 0.0
     f = synth.Frame(base.query, base.img, base.centroids, base.sv_ptr, gm)
     out, _ = _check(f)
-    # fixed Gaussians keep their posterior (GMEM.cu:1610, the update skips them)
+    # the flags come back as given, and fixed Gaussians keep their posterior
+    # (GMEM.cu:1610-1611 and 1702-1703: the update and the regularisation skip them)
+    assert np.array_equal(out["gm"]["fixed"], gm["fixed"])
     fx = gm["fixed"] == 1
     live = fx & (gm["m_o"][:, 0] > -1e31)
     assert np.array_equal(out["gm"]["m_k"][live], gm["m_k"][live])

@@ -111,11 +111,20 @@ __device__ inline int axis_cell(double u, float org, float h, int dim) {
 }
 
 // ------------------------------------------------------------ prep kernels --
-// One wave per supervoxel: unweighted moments (E-step) and intensity-weighted
-// moments (M-step), FP64, centred about the supervoxel's own means.
+// 16 lanes per supervoxel: unweighted moments (E-step) and intensity-weighted
+// moments (M-step), FP64, centred about the supervoxel's own means (two
+// passes over its voxels; sums over the group by a 4-step butterfly).
+__device__ inline double group16_sum(double x) {
+#pragma unroll
+  for (int m = 8; m >= 1; m >>= 1) x += __shfl_xor(x, m, 16);
+  return x;
+}
+
 __global__ __launch_bounds__(256) void k_moments(DevPtrs P) {
-  const long long s = P.s_lo + (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
-  const int lane = threadIdx.x & 63;
+  // every exit below is uniform over a supervoxel's 16 lanes, which is all
+  // the group sums need
+  const long long s = P.s_lo + (long long)blockIdx.x * 16 + (threadIdx.x >> 4);
+  const int lane = threadIdx.x & 15;
   if (s >= P.Scap) return;
   const int f = P.sv_frame[s];
   if (f < 0 || (P.only_frame >= 0 && f != P.only_frame)) return;
@@ -132,7 +141,7 @@ __global__ __launch_bounds__(256) void k_moments(DevPtrs P) {
   }
   const float a0 = P.vx[beg], a1 = P.vy[beg], a2 = P.vz[beg];
   double sx = 0, sy = 0, sz = 0, sw = 0, swx = 0, swy = 0, swz = 0;
-  for (int i = lane; i < n; i += 64) {
+  for (int i = lane; i < n; i += 16) {
     const double x = (double)P.vx[beg + i] - a0, y = (double)P.vy[beg + i] - a1,
                  z = (double)P.vz[beg + i] - a2, w = (double)P.vimg[beg + i];
     sx += x;
@@ -143,18 +152,18 @@ __global__ __launch_bounds__(256) void k_moments(DevPtrs P) {
     swy += w * y;
     swz += w * z;
   }
-  sx = wave_sum(sx);
-  sy = wave_sum(sy);
-  sz = wave_sum(sz);
-  sw = wave_sum(sw);
-  swx = wave_sum(swx);
-  swy = wave_sum(swy);
-  swz = wave_sum(swz);
+  sx = group16_sum(sx);
+  sy = group16_sum(sy);
+  sz = group16_sum(sz);
+  sw = group16_sum(sw);
+  swx = group16_sum(swx);
+  swy = group16_sum(swy);
+  swz = group16_sum(swz);
   const double cx = sx / n, cy = sy / n, cz = sz / n;
   const double ux = sw > 0 ? swx / sw : cx, uy = sw > 0 ? swy / sw : cy,
                uz = sw > 0 ? swz / sw : cz;
   double q[6] = {0, 0, 0, 0, 0, 0}, c[6] = {0, 0, 0, 0, 0, 0};
-  for (int i = lane; i < n; i += 64) {
+  for (int i = lane; i < n; i += 16) {
     const double x = (double)P.vx[beg + i] - a0, y = (double)P.vy[beg + i] - a1,
                  z = (double)P.vz[beg + i] - a2, w = (double)P.vimg[beg + i];
     const double d0 = x - cx, d1 = y - cy, d2 = z - cz;
@@ -174,8 +183,8 @@ __global__ __launch_bounds__(256) void k_moments(DevPtrs P) {
   }
 #pragma unroll
   for (int i = 0; i < 6; i++) {
-    q[i] = wave_sum(q[i]);
-    c[i] = wave_sum(c[i]);
+    q[i] = group16_sum(q[i]);
+    c[i] = group16_sum(c[i]);
   }
   if (lane == 0) {
     P.mom[0 * S + s] = (double)n;
@@ -1655,7 +1664,7 @@ hipError_t launch_prep(const DevPtrs& P, const KOpts& O, hipStream_t st) {
   if (O.voxel)
     k_moments_voxel<<<nb(P.Sext, 256), 256, 0, st>>>(P);
   else
-    k_moments<<<nb(P.Sext, 4), 256, 0, st>>>(P);
+    k_moments<<<nb(P.Sext, 16), 256, 0, st>>>(P);
   k_grid_setup<<<P.only_frame >= 0 ? 1 : P.nframes, 1024, 0, st>>>(P, O);
   k_qcell_count<<<nb(P.Sext, 256), 256, 0, st>>>(P, O);
   k_scan_frames<<<P.only_frame >= 0 ? 1 : P.nframes, 1024, 0, st>>>(P, 2, 0, O.K);

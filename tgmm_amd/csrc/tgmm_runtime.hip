@@ -544,6 +544,7 @@ void frame_results(tgmm_batch* b, int i, const std::vector<double>& gs,
   const int K = b->ws.K;
   if (gm && F.ng > 0) {
     soa_to_records(gs.data(), F.ng, F.g0, G, gm);
+    for (long long k = 0; k < F.ng; k++) gm[k].fixed = b->fixed_host[F.g0 + k];  // as given
     if (b->opts.voxel_mode) {  // voxel mode leaves the supervoxel lists as given
       constexpr int M = TGMM_MAX_SUPERVOXELS_PER_GAUSSIAN;
       for (long long k = 0; k < F.ng; k++) {
