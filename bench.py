@@ -232,7 +232,8 @@ def main():
     dom = max(("loglik", "estep", "mstep"), key=lambda k: kern[k]["avg_ms"])
     achieved = kern[dom]["bytes"] / (kern[dom]["avg_ms"] * 1e-3)
     traffic = None
-    tpath = os.path.join(ROOT, "profiles", "traffic_%s_config%d.json" % (dom, args.config))
+    tpath = os.path.join(ROOT, "profiles", "traffic_%s_config%d%s.json" % (
+        dom, args.config, "_voxel" if vox else ""))
     if os.path.exists(tpath):
         with open(tpath) as fh:
             traffic = json.load(fh).get("hbm_bytes_per_launch")
