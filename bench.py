@@ -18,6 +18,11 @@ and the oracle timed on the host cores as the CPU baseline.
 
     python bench.py [--steps K] [--warmup W] [--config 1|2|3|5]
 
+With --local it measures the local log-likelihood entry point
+(calculateLocalLikelihood, the cell-division test's input) on the solved
+frame of --config (default 2): the tracker's per-Gaussian supervoxel lists,
+for 3% of the Gaussians (a division round) and for all of them.
+
 With --config 4 it measures the continuous-batching server instead: four
 synthetic time-lapse streams of 200 frames (2,000-15,000 cells, ~200 voxels
 per cell, K = 5, 10 iterations), each stream's posterior kept on the device as
@@ -130,6 +135,8 @@ def main():
                     help="keep the default launch shapes (for counter runs)")
     ap.add_argument("--voxel", action="store_true",
                     help="per-voxel EM (GMEMvariationalInferenceCUDA) on the same frames")
+    ap.add_argument("--local", action="store_true",
+                    help="measure the local log-likelihood entry point instead")
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
                     help="also run one device-to-device copy of this many MiB (hipMemcpy), "
                          "the known-size reference for HBM traffic counters")
@@ -145,6 +152,8 @@ def main():
 
     if args.config == 4:
         return serve_bench(args)
+    if args.local:
+        return local_bench(args)
 
     t0 = time.time()
     frames, K, iters, workload = _frames_for(args.config)
@@ -312,6 +321,81 @@ def main():
         line["plain_per_frame_form"] = plain
     if args.copy_calibration_mb > 0:
         line["copy_calibration_bytes"] = _copy_calibration(args.copy_calibration_mb << 20)
+    print(json.dumps(line), flush=True)
+
+
+def local_bench(args):
+    """calculateLocalLikelihood on one solved frame. Lists as the division test
+    builds them (mainTrackingGaussianMixtureModel.cpp:1312-1317): each chosen
+    Gaussian's supervoxelIdx[0 .. supervoxelNum). Timed per call from the host
+    through the C entry point with the lists already in CSR form (records and
+    lists uploaded, results downloaded inside the call); the per-kernel device
+    times come from a kernel trace of this run."""
+    import numpy as np
+    from tgmm_amd import em
+    frames, K, iters, workload = _frames_for(args.config)
+    if K != 5:
+        raise SystemExit("the local log-likelihood has K = 5 slots; use --config 1, 2 or 3")
+    f = frames[0]
+    with em.Batch([f], em.make_opts(K=K, max_iter=iters, tol=-1.0)) as b:
+        b.solve(em.SOLVE_GRAPH)
+        res = b.fetch()[0]
+    gm = np.ascontiguousarray(res["gm"])
+    q, im = em.DeviceArray(f.query), em.DeviceArray(f.img)
+    ind = em.DeviceArray(res["ind"].astype(np.int32))
+    ptr = em.DeviceArray(f.sv_ptr.astype(np.int64))
+    per_g = [list(gm["supervoxelIdx"][k][: gm["supervoxelNum"][k]]) for k in range(f.G)]
+    rng = np.random.default_rng(1)
+    div = sorted(rng.choice(f.G, max(1, f.G * 3 // 100), replace=False))
+    cases = [("divisions_3pct", [per_g[k] for k in div]), ("all_gaussians", per_g)]
+    sizes = np.diff(f.sv_ptr)
+    out = {}
+    L = em.load()
+    for name, lists in cases:
+        # the C entry point on prebuilt CSR arrays, as a C++ caller passes them
+        lptr = np.zeros(len(lists) + 1, np.int64)
+        lptr[1:] = np.cumsum([len(l) for l in lists])
+        lidx = np.concatenate([np.asarray(l, np.int32) for l in lists]).astype(np.int32)
+        res_ll = np.zeros(len(lists))
+
+        def call():
+            em._check(L.tgmm_calculateLocalLikelihood(
+                res_ll.ctypes.data, lptr.ctypes.data, lidx.ctypes.data, len(lists), q.ptr,
+                im.ptr, None, ind.ptr, gm.ctypes.data, ptr.ptr, f.N, f.G, f.S))
+        for _ in range(max(1, args.warmup)):
+            call()
+        got = list(res_ll)
+        assert got == em.calculateLocalLikelihood([], lists, q, im, None, ind, gm, ptr, f.N,
+                                                  f.G, f.S)
+        ts = []
+        for _ in range(args.steps):
+            t0 = time.perf_counter()
+            call()
+            ts.append(time.perf_counter() - t0)
+        ent = sum(len(l) for l in lists)
+        vox = int(sum(sizes[s] for l in lists for s in l))
+        rec = {"lists": len(lists), "entries": ent, "voxels": vox,
+               "ms_per_call_p50": 1e3 * _percentile(ts, 50),
+               "ms_per_call_min": 1e3 * min(ts)}
+        if not args.no_oracle:
+            import oracle_lib as O
+            t0 = time.perf_counter()
+            want = O.local_loglik(f, gm, res["ind"].astype(np.int32), K, lists)
+            cpu_s = time.perf_counter() - t0
+            rec["max_rel_err_vs_oracle"] = float(max(abs(a - b) / max(1.0, abs(b))
+                                                     for a, b in zip(got, want)))
+            rec["cpu_oracle_ms"] = 1e3 * cpu_s
+            rec["cpu_threads"] = O.oracle().orc_get_threads()
+        out[name] = rec
+        print("# %s: %s" % (name, json.dumps(rec)), file=sys.stderr, flush=True)
+    d = out["divisions_3pct"]
+    line = {"metric": "local log-likelihood calls/s (division lists)",
+            "value": 1e3 / d["ms_per_call_p50"], "unit": "calls/s", "n_gpus": 1,
+            "steps": args.steps, "warmup": args.warmup, "ms_per_step": d["ms_per_call_p50"],
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+            "data": "synthetic", "config": {"workload": workload + "; local log-likelihood "
+                                            "of the per-Gaussian supervoxel lists"},
+            "cases": out}
     print(json.dumps(line), flush=True)
 
 

@@ -134,6 +134,30 @@ int tgmm_GMEMvariationalInferenceCUDA(float* queryCUDA, float* imgDataCUDA, floa
                                       int ref_nb, int maxIterEM, double tolLikelihood,
                                       int devCUDA, int frame, int W4DOF, const char* debugPath);
 
+/* Replaces calculateLocalLikelihood (GMEMupdateCUDA.h:116-127,
+ * GMEMupdateCUDA.cu:2952-3050): for each list of supervoxel indices, the sum
+ * over its supervoxels s and their voxels v of
+ *   img_v * log( sum_j sqrt(det W_j nu_j^3) / (2 pi)^1.5
+ *                      * exp(-nu_j (x_v - m_j)^T W_j (x_v - m_j) / 2)
+ *                      * alpha_j / sum_alpha )
+ * over the K = 5 candidates j = indCUDA[s + j*numLabels] (slots < 0 skipped),
+ * sum_alpha over all ref_nb records of vecGMtemp (host, read only); a voxel
+ * whose density is not > 0 adds nothing; a supervoxel listed twice counts
+ * twice. The reference's vector<vector<int>> is passed in CSR form: list i is
+ * list_idx[list_ptr[i] .. list_ptr[i+1]), list_ptr[0] = 0, n_lists + 1
+ * entries. localLikelihood[i] (host) is written for every non-empty list and
+ * left as given for an empty one (the reference skips it). Device inputs
+ * queryCUDA, imgDataCUDA, indCUDA, labelListPtrCUDA as after
+ * tgmm_GMEMvariationalInferenceCUDAWithSupervoxels; rnkCUDA is unused, as in
+ * the reference. Indices outside [0, numLabels) are rejected (TGMM_ERR_ARG).
+ * Runs on the current device. */
+int tgmm_calculateLocalLikelihood(double* localLikelihood, const long long* list_ptr,
+                                  const int* list_idx, int n_lists, const float* queryCUDA,
+                                  const float* imgDataCUDA, const float* rnkCUDA,
+                                  const int* indCUDA, const GaussianMixtureModelCUDA* vecGMtemp,
+                                  const long long* labelListPtrCUDA, long long query_nb,
+                                  int ref_nb, int numLabels);
+
 /* Replaces GMEMreleaseMemoryWithSupervoxels (GMEMupdateCUDA.h:140-144,
  * GMEMupdateCUDA.cu:4654-4676): frees and nulls every pointer. */
 int tgmm_GMEMreleaseMemoryWithSupervoxels(float** queryCUDA,

@@ -5,7 +5,7 @@
 // The tracker includes this file where it included GMEMupdateCUDA.h and links
 // libtgmm_em.so instead of the CUDA objects. The inline functions below carry
 // the reference's names and parameter lists (GMEMupdateCUDA.h:82, 87-98,
-// 132-144) and its error behaviour: print a message and exit(), with the
+// 116-127, 132-144) and its error behaviour: print a message and exit(), with the
 // exit codes the reference uses (0 for the size limits, GMEM.cu:3992-4007;
 // 3 when the regularisation constants were never set, GMEM.cu:4217-4222;
 // EXIT_FAILURE for runtime errors, book.h:20-25).
@@ -19,6 +19,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <string>
+#include <vector>
 
 #include "tgmm_em.h"
 
@@ -86,6 +87,27 @@ inline void GMEMvariationalInferenceCUDA(float* queryCUDA, float* imgDataCUDA, p
                                         tolLikelihood, devCUDA, frame, W4DOF ? 1 : 0,
                                         debugPath.c_str()),
       "GMEMvariationalInferenceCUDA");
+}
+
+inline void calculateLocalLikelihood(std::vector<double>& localLikelihood,
+                                     const std::vector<std::vector<int> >& listSupervoxels,
+                                     float* queryCUDA, float* imgDataCUDA, pxi* rnkCUDA,
+                                     int* indCUDA, GaussianMixtureModelCUDA* vecGMtemp,
+                                     long long int* labelListPtrCUDA, long long int query_nb,
+                                     int ref_nb, int numLabels) {
+  localLikelihood.resize(listSupervoxels.size());
+  std::vector<long long> ptr(listSupervoxels.size() + 1, 0);
+  std::vector<int> idx;
+  for (size_t i = 0; i < listSupervoxels.size(); i++) {
+    ptr[i + 1] = ptr[i] + (long long)listSupervoxels[i].size();
+    idx.insert(idx.end(), listSupervoxels[i].begin(), listSupervoxels[i].end());
+  }
+  tgmm_compat::check(
+      tgmm_calculateLocalLikelihood(localLikelihood.data(), ptr.data(), idx.data(),
+                                    (int)listSupervoxels.size(), queryCUDA, imgDataCUDA, rnkCUDA,
+                                    indCUDA, vecGMtemp, labelListPtrCUDA, query_nb, ref_nb,
+                                    numLabels),
+      "calculateLocalLikelihood");
 }
 
 inline void GMEMreleaseMemoryWithSupervoxels(float** queryCUDA, float** imgDataCUDA,

@@ -15,6 +15,8 @@
  *   death          GMEM.cu:1813-1843 (K16)
  *   log-likelihood GMEM.cu:1913-1978 (K17), per voxel, FP64
  *   convergence    GMEM.cu:4143-4148; supervoxel assignment GMEM.cu:4328-4350
+ *   local LL       GMEM.cu:2952-3050 with the kernel :1989-2103 (lists of
+ *                  supervoxels, the cell-division likelihood-ratio input)
  * and, with tgmm_em_opts.voxel_mode, the per-voxel EM of
  * GMEMvariationalInferenceCUDA (GMEM.cu:3420-3973):
  *   kNN            each voxel is a query (knnCUDAinPlace on queryCUDA, :3576)
@@ -36,7 +38,9 @@
  *     float, GMEM.cu:1381-1394, 1614-1629).
  *   - log-likelihood sums every voxel of a supervoxel (the reference's
  *     `cache[tid] =` keeps only the last term per thread for supervoxels of
- *     more than 1024 voxels, GMEM.cu:1961-1962).
+ *     more than 1024 voxels, GMEM.cu:1961-1962; the same in the local
+ *     kernel, :2067-2068), and sums in FP64 (the reference's per-block and
+ *     per-list sums are FP32).
  *   - death NaN test reads W_k[0..5] and E[ln det Lambda] (the reference's
  *     7-element loop over W_k reaches that field, GMEM.cu:1823-1824).
  *
@@ -505,44 +509,84 @@ void orc_death(int G, GaussianMixtureModelCUDA* gm, double sum_alpha) {
   }
 }
 
+/* One supervoxel's term of the log-likelihood (GMEM.cu:1940-1959, the same
+ * per-voxel expression in the local kernel :2038-2078): the intensity-weighted
+ * log of the mixture density over the supervoxel's candidates, each term
+ * sqrt(det W nu^3) / (2 pi)^1.5 * exp(-nu d^T W d / 2) * alpha / sum_alpha,
+ * evaluated in that order. Slots with no candidate (ind < 0 or >= G) are
+ * skipped; a voxel whose density is not > 0 (all terms underflow, or NaN from
+ * an indefinite precision) adds nothing. */
+static double sv_loglik(long long N, long long S, long long s, const float* query,
+                        const float* img, const long long* ptr, int G,
+                        const GaussianMixtureModelCUDA* gm, int K, const int* ind,
+                        double sum_alpha) {
+  static const double two_pi_3_2 = 15.749609945722419; /* (2 pi)^1.5 */
+  double coef[TGMM_KMAX], ratio[TGMM_KMAX];
+  int kk[TGMM_KMAX];
+  for (int j = 0; j < K; j++) {
+    kk[j] = ind[s + (long long)j * S];
+    if (kk[j] < 0 || kk[j] >= G) {
+      kk[j] = -1;
+      continue;
+    }
+    const GaussianMixtureModelCUDA* g = &gm[kk[j]];
+    coef[j] = sqrt(orc_det3(g->W_k) * pow(g->nu_k, 3.0)) / two_pi_3_2;
+    ratio[j] = g->alpha_k / sum_alpha;
+  }
+  double acc = 0.0;
+  for (long long v = ptr[s]; v < ptr[s + 1]; v++) {
+    double like = 0.0;
+    for (int j = 0; j < K; j++) {
+      if (kk[j] < 0) continue;
+      const GaussianMixtureModelCUDA* g = &gm[kk[j]];
+      const double d0 = (double)query[v] - g->m_k[0];
+      const double d1 = (double)query[v + N] - g->m_k[1];
+      const double d2 = (double)query[v + 2 * N] - g->m_k[2];
+      double aux = coef[j];
+      aux *= exp(-0.5 * g->nu_k * quad(g->W_k, d0, d1, d2));
+      aux *= ratio[j];
+      like += aux;
+    }
+    if (like > 0.0) acc += (double)img[v] * log(like);
+  }
+  return acc;
+}
+
 /* GMEM.cu:1913-1978 + 2834-2873 */
 double orc_loglik(long long N, long long S, const float* query,
                   const float* img, const long long* ptr, int G,
                   const GaussianMixtureModelCUDA* gm, int K, const int* ind,
                   double sum_alpha) {
-  (void)G;
-  static const double two_pi_3_2 = 15.749609945722419; /* (2 pi)^1.5 */
   double* part = (double*)calloc((size_t)(S > 0 ? S : 1), sizeof(double));
 #pragma omp parallel for schedule(dynamic, 16)
-  for (long long s = 0; s < S; s++) {
-    double coef[TGMM_KMAX];
-    int kk[TGMM_KMAX];
-    for (int j = 0; j < K; j++) {
-      kk[j] = ind[s + (long long)j * S];
-      if (kk[j] < 0) continue;
-      const GaussianMixtureModelCUDA* g = &gm[kk[j]];
-      coef[j] = sqrt(orc_det3(g->W_k) * g->nu_k * g->nu_k * g->nu_k) / two_pi_3_2 *
-                (g->alpha_k / sum_alpha);
-    }
-    double acc = 0.0;
-    for (long long v = ptr[s]; v < ptr[s + 1]; v++) {
-      double like = 0.0;
-      for (int j = 0; j < K; j++) {
-        if (kk[j] < 0) continue;
-        const GaussianMixtureModelCUDA* g = &gm[kk[j]];
-        const double d0 = (double)query[v] - g->m_k[0];
-        const double d1 = (double)query[v + N] - g->m_k[1];
-        const double d2 = (double)query[v + 2 * N] - g->m_k[2];
-        like += coef[j] * exp(-0.5 * g->nu_k * quad(g->W_k, d0, d1, d2));
-      }
-      if (like > 0.0) acc += (double)img[v] * log(like);
-    }
-    part[s] = acc;
-  }
+  for (long long s = 0; s < S; s++)
+    part[s] = sv_loglik(N, S, s, query, img, ptr, G, gm, K, ind, sum_alpha);
   double ll = 0.0;
   for (long long s = 0; s < S; s++) ll += part[s];
   free(part);
   return ll;
+}
+
+/* calculateLocalLikelihood (GMEM.cu:2952-3050) with
+ * GMEMcomputeLocalLikelihoodKernelWithSupervoxels (:1989-2103) and its sum
+ * addLocalLikelihoodWithSupervoxels (:2876-2945): sum_alpha over all G
+ * records (calculateTotalAlpha, :2765-2799), then for each non-empty list
+ * the sum of its supervoxels' terms, a supervoxel listed twice counted
+ * twice; an empty list leaves out[i] as given (the reference skips it). */
+void orc_local_loglik(long long N, long long S, const float* query,
+                      const float* img, const long long* ptr, int G,
+                      const GaussianMixtureModelCUDA* gm, int K, const int* ind,
+                      int n_lists, const long long* list_ptr,
+                      const int* list_idx, double* out) {
+  const double sum_alpha = orc_sum_alpha(G, gm);
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int i = 0; i < n_lists; i++) {
+    if (list_ptr[i + 1] == list_ptr[i]) continue;
+    double ll = 0.0;
+    for (long long p = list_ptr[i]; p < list_ptr[i + 1]; p++)
+      ll += sv_loglik(N, S, list_idx[p], query, img, ptr, G, gm, K, ind, sum_alpha);
+    out[i] = ll;
+  }
 }
 
 /* loop condition GMEM.cu:4145-4148: 1 = keep iterating */

@@ -57,6 +57,13 @@ double orc_loglik(long long N, long long S, const float* query,
                   const float* img, const long long* ptr, int G,
                   const GaussianMixtureModelCUDA* gm, int K, const int* ind,
                   double sum_alpha);
+/* local log-likelihood over lists of supervoxels: list i is
+ * list_idx[list_ptr[i] .. list_ptr[i+1]); empty lists leave out[i] as given */
+void orc_local_loglik(long long N, long long S, const float* query,
+                      const float* img, const long long* ptr, int G,
+                      const GaussianMixtureModelCUDA* gm, int K, const int* ind,
+                      int n_lists, const long long* list_ptr,
+                      const int* list_idx, double* out);
 int orc_converged(double ll, double ll1, double ll2, double ll3, double tol);
 void orc_assign_supervoxels(long long S, int G, GaussianMixtureModelCUDA* gm,
                             int K, const float* rnk, const int* ind, int cap);

@@ -84,6 +84,8 @@ def oracle():
         L.orc_loglik.restype = ctypes.c_double
         L.orc_loglik.argtypes = [_ll, _ll, _fp, _fp, _lp, ctypes.c_int, _vp,
                                  ctypes.c_int, _ip, ctypes.c_double]
+        L.orc_local_loglik.argtypes = [_ll, _ll, _fp, _fp, _lp, ctypes.c_int, _vp,
+                                       ctypes.c_int, _ip, ctypes.c_int, _lp, _ip, _dp]
         L.orc_converged.argtypes = [ctypes.c_double] * 5
         L.orc_assign_supervoxels.argtypes = [_ll, ctypes.c_int, _vp,
                                              ctypes.c_int, _fp, _ip, ctypes.c_int]
@@ -133,6 +135,24 @@ def em_partial(frame, opts, iters, gm=None):
                           p(ind, _ip), p(Nk, _dp), p(xk, _dp), p(Ck, _dp))
     assert rc == 0
     return gm, rnk, ind, Nk, xk.reshape(G, 3), Ck.reshape(G, 6)
+
+
+def local_loglik(frame, gm, ind, K, lists, out=None):
+    """Oracle local log-likelihood of each list of supervoxel indices (lists:
+    sequence of integer sequences); empty lists keep `out`'s value (0 by
+    default)."""
+    L = oracle()
+    gm = np.ascontiguousarray(gm)
+    ind = np.ascontiguousarray(ind, np.int32)
+    ptr = np.zeros(len(lists) + 1, np.int64)
+    ptr[1:] = np.cumsum([len(x) for x in lists])
+    idx = np.ascontiguousarray(np.concatenate([np.asarray(x, np.int32) for x in lists])
+                               if ptr[-1] else np.zeros(1, np.int32), np.int32)
+    out = np.zeros(len(lists)) if out is None else np.array(out, np.float64)
+    L.orc_local_loglik(frame.N, frame.S, p(frame.query, _fp), p(frame.img, _fp),
+                       p(frame.sv_ptr, _lp), gm.shape[0], gm.ctypes.data, K, p(ind, _ip),
+                       len(lists), p(ptr, _lp), p(idx, _ip), p(out, _dp))
+    return out
 
 
 _twin = None

@@ -70,6 +70,31 @@ def test_legacy_limits_and_missing_constants_are_status_codes():
     assert b"Too many" in L.tgmm_last_error()
 
 
+def test_local_likelihood_argument_errors_are_status_codes():
+    """Host-side checks of tgmm_calculateLocalLikelihood, before any device
+    call: malformed list pointers and indices outside [0, numLabels)."""
+    L = em.load()
+    out = np.full(2, 5.0)
+    ll = ctypes.c_longlong
+
+    def call(ptr, idx, n, S=10, G=0):
+        ptr = np.asarray(ptr, np.int64)
+        idx = np.asarray(idx if len(idx) else [0], np.int32)
+        return L.tgmm_calculateLocalLikelihood(out.ctypes.data, ptr.ctypes.data,
+                                               idx.ctypes.data, n, None, None, None, None,
+                                               None, None, 100, G, S)
+    assert call([0, 1], [3], -1) == -1        # negative list count
+    assert call([1, 1, 1], [3], 2) == -1      # list_ptr[0] != 0
+    assert call([0, 2, 1], [3, 4], 2) == -1   # decreasing
+    assert call([0, 1, 2], [3, 10], 2) == -1  # index == numLabels
+    assert b"outside" in L.tgmm_last_error()
+    assert call([0, 1, 2], [-1, 2], 2) == -1
+    assert call([0, 1, 2], [1, 2], 2) == -1   # NULL device arrays
+    assert call([0, 0, 0], [], 2) == 0        # every list empty: nothing to do
+    assert call([0], [], 0) == 0
+    assert np.all(out == 5.0)
+
+
 def test_generator_is_deterministic():
     a = synth.config1()
     b = synth.config1()
@@ -109,6 +134,8 @@ def test_compat_header_compiles_and_links(tmp_path):
         "  GMEMvariationalInferenceCUDAWithSupervoxels(q, im, r, 0, i, it, c, p, g, 0, 0, 0,\n"
         "                                              10, 1e-5, 0, 0, true);\n"
         "  GMEMvariationalInferenceCUDA(q, im, r, 0, i, it, g, 0, 0, 10, 1e-5, 0, 0, true);\n"
+        "  std::vector<double> ll; std::vector<std::vector<int> > lists(2);\n"
+        "  calculateLocalLikelihood(ll, lists, q, im, r, i, g, p, 0, 0, 0);\n"
         "  GMEMreleaseMemoryWithSupervoxels(&q, &im, &r, &i, &it, &c, &p);\n"
         "  return 0;\n}\n")
     lib = os.path.join(ROOT, "tgmm_amd")

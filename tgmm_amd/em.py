@@ -162,6 +162,9 @@ def load():
     L.tgmm_GMEMvariationalInferenceCUDA.argtypes = [
         _vp, _vp, _vp, _vp, _vp, _vp, _vp, ctypes.c_longlong, ctypes.c_int, ctypes.c_int,
         ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_char_p]
+    L.tgmm_calculateLocalLikelihood.argtypes = [
+        _vp, _vp, _vp, ctypes.c_int, _vp, _vp, _vp, _vp, _vp, _vp, ctypes.c_longlong,
+        ctypes.c_int, ctypes.c_int]
     L.tgmm_batch_create.argtypes = [ctypes.c_int, ctypes.POINTER(FrameDesc), ctypes.c_int,
                                     ctypes.POINTER(EmOpts), _pp]
     L.tgmm_batch_reset.argtypes = [_vp]
@@ -326,6 +329,34 @@ def GMEMvariationalInferenceCUDA(queryCUDA, imgDataCUDA, rnkCUDA, indCUDA, indCU
         vecGMtemp.ctypes.data, int(query_nb), int(ref_nb), int(maxIterEM),
         float(tolLikelihood), int(devCUDA), int(frame), 1 if W4DOF else 0,
         debugPath.encode() if debugPath else None))
+
+
+def calculateLocalLikelihood(localLikelihood, listSupervoxels, queryCUDA, imgDataCUDA,
+                             rnkCUDA, indCUDA, vecGMtemp, labelListPtrCUDA, query_nb, ref_nb,
+                             numLabels):
+    """Local log-likelihood of each list of supervoxel indices
+    (GMEMupdateCUDA.h:116-127). localLikelihood: a Python list, resized to
+    len(listSupervoxels) (new entries 0.0) and written for every non-empty
+    list, as the reference does with its vector; also returned. Device
+    pointers (or DeviceArray) as after GMEMvariationalInferenceCUDAWithSupervoxels;
+    vecGMtemp: GMM_DTYPE host records [ref_nb]."""
+    if vecGMtemp.dtype != GMM_DTYPE or not vecGMtemp.flags["C_CONTIGUOUS"]:
+        raise TypeError("vecGMtemp must be a C-contiguous GMM_DTYPE array (288-byte records)")
+    n = len(listSupervoxels)
+    del localLikelihood[n:]
+    localLikelihood.extend([0.0] * (n - len(localLikelihood)))
+    ptr = np.zeros(n + 1, np.int64)
+    ptr[1:] = np.cumsum([len(x) for x in listSupervoxels])
+    idx = (np.concatenate([np.asarray(x, np.int32) for x in listSupervoxels])
+           if ptr[-1] else np.zeros(1, np.int32)).astype(np.int32)
+    out = np.array(localLikelihood + [0.0], np.float64)
+    dp = lambda a: None if a is None else (a.ptr if isinstance(a, DeviceArray) else a)
+    _check(load().tgmm_calculateLocalLikelihood(
+        out.ctypes.data, ptr.ctypes.data, idx.ctypes.data, n, dp(queryCUDA), dp(imgDataCUDA),
+        dp(rnkCUDA), dp(indCUDA), vecGMtemp.ctypes.data, dp(labelListPtrCUDA), int(query_nb),
+        int(ref_nb), int(numLabels)))
+    localLikelihood[:] = [float(x) for x in out[:n]]
+    return localLikelihood
 
 
 def GMEMreleaseMemoryWithSupervoxels(d):
