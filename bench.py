@@ -18,6 +18,12 @@ and the oracle timed on the host cores as the CPU baseline.
 
     python bench.py [--steps K] [--warmup W] [--config 1|2|3|5]
 
+With --marshal it measures the input marshalling on the device (raw uint16
+supervoxel voxels to the EM's float inputs and centroids) for the frame of
+--config (default 2), from inputs resident in HBM and from pinned host memory,
+beside the upload of the finished float arrays that the reference's host path
+needs and the oracle's host restatement of the same work.
+
 With --local it measures the local log-likelihood entry point
 (calculateLocalLikelihood, the cell-division test's input) on the solved
 frame of --config (default 2): the tracker's per-Gaussian supervoxel lists,
@@ -135,6 +141,8 @@ def main():
                     help="keep the default launch shapes (for counter runs)")
     ap.add_argument("--voxel", action="store_true",
                     help="per-voxel EM (GMEMvariationalInferenceCUDA) on the same frames")
+    ap.add_argument("--marshal", action="store_true",
+                    help="measure the input marshalling on the device instead")
     ap.add_argument("--local", action="store_true",
                     help="measure the local log-likelihood entry point instead")
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
@@ -154,6 +162,8 @@ def main():
         return serve_bench(args)
     if args.local:
         return local_bench(args)
+    if args.marshal:
+        return marshal_bench(args)
 
     t0 = time.time()
     frames, K, iters, workload = _frames_for(args.config)
@@ -321,6 +331,79 @@ def main():
         line["plain_per_frame_form"] = plain
     if args.copy_calibration_mb > 0:
         line["copy_calibration_bytes"] = _copy_calibration(args.copy_calibration_mb << 20)
+    print(json.dumps(line), flush=True)
+
+
+def marshal_bench(args):
+    """tgmm_marshal_frame on one frame: the frame's supervoxels as raw uint16
+    voxels (8-byte linear index and 2-byte intensity per voxel, a threshold
+    per supervoxel; intensities thr + round(img * 3000)). Timed per call
+    through the C entry point (input copies, both kernels, the range
+    read-back); outputs compared bitwise with the oracle."""
+    import ctypes
+    import numpy as np
+    from tgmm_amd import em
+    import oracle_lib as O
+    from helpers import raw_from_frame
+    frames, K, iters, workload = _frames_for(args.config)
+    f = frames[0]
+    dims = {1: (512, 512, 100), 2: (2048, 2048, 400), 5: (4096, 4096, 500)}[args.config]
+    idx, raw, ptr, thr, d = raw_from_frame(f, dims, seed=1)
+    N, S = f.N, f.S
+    L = em.load()
+    outs = (em.DeviceArray(nbytes=12 * N), em.DeviceArray(nbytes=4 * N),
+            em.DeviceArray(nbytes=12 * S), em.DeviceArray(nbytes=8 * (S + 1)))
+    sources = {
+        "device": (em.DeviceArray(idx), em.DeviceArray(raw), em.DeviceArray(thr)),
+        "pinned_host": (em.pinned(idx), em.pinned(raw), em.pinned(thr)),
+    }
+    res = {}
+    for name, (a_idx, a_raw, a_thr) in sources.items():
+        fr, keep = em.raw_frame(a_idx, a_raw, ptr, a_thr, d)
+
+        def call():
+            em._check(L.tgmm_marshal_frame(ctypes.byref(fr), outs[0].ptr, outs[1].ptr,
+                                           outs[2].ptr, outs[3].ptr))
+        for _ in range(max(1, args.warmup)):
+            call()
+        ts = []
+        for _ in range(args.steps):
+            t0 = time.perf_counter()
+            call()
+            ts.append(time.perf_counter() - t0)
+        res[name] = {"ms_per_frame_p50": 1e3 * _percentile(ts, 50),
+                     "ms_per_frame_min": 1e3 * min(ts)}
+        print("# %s: %s" % (name, json.dumps(res[name])), file=sys.stderr, flush=True)
+    # the reference's path uploads the finished float arrays instead
+    qh, imh, ch = O.marshal_frame(idx, raw, ptr, thr, d)
+    up = [em.pinned(qh), em.pinned(imh), em.pinned(ch)]
+    dst = [em.DeviceArray(nbytes=a.nbytes) for a in up]
+    H = em._hip_lib()
+    ts = []
+    for _ in range(args.steps):
+        t0 = time.perf_counter()
+        for a, b in zip(up, dst):
+            H.hipMemcpy(b.ptr, a.ctypes.data, a.nbytes, 1)
+        ts.append(time.perf_counter() - t0)
+    res["float_arrays_upload"] = {"ms_per_frame_p50": 1e3 * _percentile(ts, 50),
+                                  "bytes": int(sum(a.nbytes for a in up))}
+    got = [outs[0].numpy(np.float32)[: 3 * N], outs[1].numpy(np.float32)[:N],
+           outs[2].numpy(np.float32)[: 3 * S]]
+    exact = all(np.array_equal(g, w) for g, w in zip(got, (qh, imh, ch)))
+    cpu, cpu_reps = _cpu_sample(lambda: O.marshal_frame(idx, raw, ptr, thr, d))
+    t_dev = res["device"]["ms_per_frame_p50"] / 1e3
+    line = {"metric": "marshalled voxels/s (raw uint16 frame to EM inputs)",
+            "value": N / t_dev, "unit": "voxels/s", "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": 1e3 * t_dev, "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "u16->f32", "data": "synthetic",
+            "config": {"workload": workload + "; raw frame %dx%dx%d, %d voxels, %d supervoxels"
+                       % (dims[0], dims[1], dims[2], N, S)},
+            "bitwise_equal_to_oracle": bool(exact), "cases": res,
+            "input_bytes": {"raw": int(10 * N + 10 * S + 8),
+                            "float_arrays": int(16 * N + 12 * S)},
+            "cpu_baseline": {"value": N / cpu, "unit": "voxels/s", "cores": 1, "kind": "port",
+                             "sample": "the oracle's single-thread restatement of the whole "
+                                       "frame, %d calls (at least 2 s)" % cpu_reps}}
     print(json.dumps(line), flush=True)
 
 

@@ -332,6 +332,56 @@ int tgmm_server_stats(tgmm_server* s, tgmm_server_stats_t* stats);
 int tgmm_server_set_tiles(tgmm_server* s, const tgmm_tiles* tiles);
 int tgmm_server_destroy(tgmm_server* s);
 
+/* ---------------------------------------------------------------------------
+ * 3. Input marshalling on the device
+ * ------------------------------------------------------------------------ */
+
+/* One frame as raw supervoxel voxels: what the tracker holds after trimming
+ * (mainTrackingGaussianMixtureModel.cpp:556-649). Supervoxel s is
+ * pixel_idx[sv_ptr[s] .. sv_ptr[s+1]) (its PixelIdxList, linear indices
+ * x + d0 (y + d1 z)); supervoxels are disjoint. Intensities are the raw
+ * uint16 values of the listed voxels (raw), or, when raw is NULL, read from
+ * volume, the whole raw image in device memory. thr[s] is the supervoxel's
+ * local background (the trimSupervoxel threshold, :558-559). pixel_idx, raw
+ * and thr are host memory, or device memory read in place when
+ * device_inputs is 1; sv_ptr is host memory. Volumes up to 2^53 voxels. */
+typedef struct tgmm_raw_frame {
+  long long N;                         /* voxels listed */
+  long long S;                         /* supervoxels */
+  long long dims[TGMM_DIMS];           /* image size, x fastest */
+  const unsigned long long* pixel_idx; /* [N] */
+  const unsigned short* raw;           /* [N] or NULL */
+  const unsigned short* volume;        /* [d0 d1 d2] device, read where raw is NULL */
+  const long long* sv_ptr;             /* [S+1], sv_ptr[0] = 0, sv_ptr[S] = N */
+  const unsigned short* thr;           /* [S] */
+  int device_inputs;                   /* 1: pixel_idx, raw, thr are device memory */
+} tgmm_raw_frame;
+
+/* No single reference function: replaces the tracker's host construction of
+ * the EM inputs, mainTrackingGaussianMixtureModel.cpp:598-669 (local
+ * background subtraction in uint16, zeroing outside the supervoxels,
+ * Convert_Array_Inplace to FLOAT32 and Scale_Array_To_Range(img, 0, 1), mylib
+ * array.p:2708-2820) and :774-870 (the voxel SoA from Idx2CoordA), with the
+ * centroids of supervoxel::weightedGaussianStatistics<uint16>
+ * (supervoxel.cpp:1410-1449; raw intensities, float sums in list order).
+ * Writes, on the current device: queryCUDA [3N] SoA coordinates,
+ * imgDataCUDA [N] scaled intensities, centroidLabelPositionCUDA [3S] SoA and,
+ * if not NULL, labelListPtrCUDA [S+1] (a copy of sv_ptr). Returns
+ * TGMM_ERR_ARG for an index outside the volume (mylib exits there) and for
+ * more voxels than the volume holds (the lists overlap). */
+int tgmm_marshal_frame(const tgmm_raw_frame* frame, float* queryCUDA, float* imgDataCUDA,
+                       float* centroidLabelPositionCUDA, long long* labelListPtrCUDA);
+
+/* tgmm_GMEMinitializeMemoryWithSupervoxels (GMEMupdateCUDA.h:132-137) fed
+ * from a raw frame instead of the host float arrays: allocates the same
+ * seven device arrays (query_nb = frame->N, numLabels = frame->S <= 65535),
+ * fills them with tgmm_marshal_frame and records scale as the kNN
+ * anisotropy. Release with tgmm_GMEMreleaseMemoryWithSupervoxels. */
+int tgmm_GMEMinitializeMemoryFromRaw(float** queryCUDA, float** imgDataCUDA, const float* scale,
+                                     float** rnkCUDA, int** indCUDA, int** indCUDAtr,
+                                     float** centroidLabelPositionCUDA,
+                                     long long** labelListPtrCUDA, const tgmm_raw_frame* frame);
+
 #ifdef __cplusplus
 }
 #endif

@@ -17,6 +17,9 @@
  *   convergence    GMEM.cu:4143-4148; supervoxel assignment GMEM.cu:4328-4350
  *   local LL       GMEM.cu:2952-3050 with the kernel :1989-2103 (lists of
  *                  supervoxels, the cell-division likelihood-ratio input)
+ *   marshalling    mainTrackingGaussianMixtureModel.cpp:598-669, 774-870,
+ *                  supervoxel.cpp:1410-1449, mylib array.p:557-576,
+ *                  2708-2820 (raw uint16 voxels to the EM's float inputs)
  * and, with tgmm_em_opts.voxel_mode, the per-voxel EM of
  * GMEMvariationalInferenceCUDA (GMEM.cu:3420-3973):
  *   kNN            each voxel is a query (knnCUDAinPlace on queryCUDA, :3576)
@@ -587,6 +590,66 @@ void orc_local_loglik(long long N, long long S, const float* query,
       ll += sv_loglik(N, S, list_idx[p], query, img, ptr, G, gm, K, ind, sum_alpha);
     out[i] = ll;
   }
+}
+
+/* Input marshalling of one frame, from the supervoxels' raw uint16 voxels to
+ * the EM's inputs (mainTrackingGaussianMixtureModel.cpp:598-669, 774-870;
+ * supervoxel.cpp:1410-1449). Supervoxel s holds voxels
+ * sv_ptr[s] .. sv_ptr[s+1] with linear indices pixel_idx (x fastest, dims
+ * d0 d1 d2) and raw intensities raw. Per voxel:
+ *   query = (float) coordinates (coord4idx in mylib array.p:557-576);
+ *   img   = (float)(uint16)(raw - thr[s]) (local background subtraction,
+ *           main.cpp:618-627; Convert_Array_Inplace to FLOAT32, :663-664),
+ *           then Scale_Array_To_Range(img, 0, 1) over the whole volume
+ *           (mylib array.p:2708-2820): with M, m the volume's max and min
+ *           (voxels outside every supervoxel are 0, main.cpp:644-649),
+ *           img = (float)((1.0 / (M - m)) * (img - m)) in double, unless M == m.
+ * Per supervoxel the centroid of the RAW intensities, accumulated in float in
+ * list order (weightedGaussianStatistics, computed before the subtraction):
+ * sum(raw * coord) / sum(raw), or 0 when sum(raw) < 1e-5.
+ * vol_size = d0 d1 d2; supervoxels are disjoint, so a voxel outside every
+ * supervoxel exists iff N < vol_size. Returns 0, or -1 for an index outside
+ * the volume (mylib exits there). */
+int orc_marshal_frame(long long N, long long S, const unsigned long long* pixel_idx,
+                      const unsigned short* raw, const long long* sv_ptr,
+                      const unsigned short* thr, const long long dims[3], float* query,
+                      float* img, float* cent) {
+  const unsigned long long d0 = (unsigned long long)dims[0], d1 = (unsigned long long)dims[1],
+                           d2 = (unsigned long long)dims[2];
+  const unsigned long long vol = d0 * d1 * d2;
+  unsigned hi = 0, lo = 0xFFFFu;
+  for (long long s = 0; s < S; s++) {
+    float nk = 0.0f, c[3] = {0.0f, 0.0f, 0.0f};
+    for (long long v = sv_ptr[s]; v < sv_ptr[s + 1]; v++) {
+      unsigned long long id = pixel_idx[v];
+      const unsigned long long x = id % d0;
+      id /= d0;
+      const unsigned long long y = id % d1;
+      id /= d1;
+      const unsigned long long z = id % d2;
+      id /= d2;
+      if (id > 0) return -1;
+      query[v] = (float)x;
+      query[v + N] = (float)y;
+      query[v + 2 * N] = (float)z;
+      const unsigned short sub = (unsigned short)(raw[v] - thr[s]);
+      img[v] = (float)sub;
+      if (sub > hi) hi = sub;
+      if (sub < lo) lo = sub;
+      const float w = (float)raw[v];
+      nk += w;
+      c[0] += w * (float)x;
+      c[1] += w * (float)y;
+      c[2] += w * (float)z;
+    }
+    for (int a = 0; a < 3; a++) cent[s + a * S] = (nk < 1e-5) ? 0.0f : c[a] / nk;
+  }
+  if (N == 0) return 0;
+  if ((unsigned long long)N < vol) lo = 0; /* the zeroed background */
+  if (hi == lo) return 0;
+  const double f = 1.0 / ((double)hi - (double)lo);
+  for (long long v = 0; v < N; v++) img[v] = (float)(f * ((double)img[v] - (double)lo));
+  return 0;
 }
 
 /* loop condition GMEM.cu:4145-4148: 1 = keep iterating */

@@ -64,6 +64,12 @@ void orc_local_loglik(long long N, long long S, const float* query,
                       const GaussianMixtureModelCUDA* gm, int K, const int* ind,
                       int n_lists, const long long* list_ptr,
                       const int* list_idx, double* out);
+/* input marshalling of one frame from raw uint16 voxels: query [3N] SoA,
+ * img [N], centroids [3S] SoA; 0, or -1 for an index outside the volume */
+int orc_marshal_frame(long long N, long long S, const unsigned long long* pixel_idx,
+                      const unsigned short* raw, const long long* sv_ptr,
+                      const unsigned short* thr, const long long dims[3], float* query,
+                      float* img, float* cent);
 int orc_converged(double ll, double ll1, double ll2, double ll3, double tol);
 void orc_assign_supervoxels(long long S, int G, GaussianMixtureModelCUDA* gm,
                             int K, const float* rnk, const int* ind, int cap);

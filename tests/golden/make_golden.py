@@ -1,9 +1,14 @@
 """Writes tests/golden/reftwin_vectors.npz: input/output vectors of the
 reference's own CPU update equations (GaussianMixtureModel.h:229-368,
-GaussianMixtureModel.cpp:892-910, GSL psi.c), through the driver built in
-oracle/_ref (make -C oracle ref). Run in a tree where /root/reference exists:
+GaussianMixtureModel.cpp:892-910, GSL psi.c), and
+tests/golden/sv_centroid_vectors.npz: supervoxels over a uint16 volume with
+the centroid and intensity the reference's
+supervoxel::weightedGaussianStatistics<uint16> (supervoxel.cpp:1410-1449)
+gives them, through the driver built in oracle/_ref (make -C oracle ref).
+Run in a tree where /root/reference exists:
 
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py             # both files
+    python tests/golden/make_golden.py centroids   # the centroid file only
 """
 import os
 import sys
@@ -113,5 +118,60 @@ def rand_cov(rng, n):
     return out
 
 
+def centroid_vectors():
+    """Disjoint supervoxels (random blobs, voxels listed in random order) over
+    a 97 x 83 x 29 volume: intensities 100-4000, some near 65535 (sums beyond
+    2^24, so the float accumulation order shows), one all-zero supervoxel (the
+    N_k < 1e-5 branch) and single-voxel ones."""
+    T = O.reftwin()
+    if T is None or not hasattr(T, "ref_sv_centroid"):
+        raise SystemExit("oracle/_ref/libtgmm_reftwin.so not built (make -C oracle ref)")
+    rng = np.random.default_rng(20261203)
+    dims = np.array([97, 83, 29], np.uint64)
+    vol_size = int(np.prod(dims))
+    used = np.zeros(vol_size, bool)
+    vol = np.zeros(vol_size, np.uint16)
+    idx_all, ptr, thr = [], [0], []
+    for s in range(300):
+        c = rng.integers([0, 0, 0], dims.astype(np.int64))
+        ext = rng.integers(1, [9, 9, 4])
+        lo = np.maximum(c - ext, 0)
+        hi = np.minimum(c + ext + 1, dims.astype(np.int64))
+        xs, ys, zs = np.meshgrid(*[np.arange(a, b) for a, b in zip(lo, hi)], indexing="ij")
+        lin = (xs + dims[0] * (ys + dims[1] * zs)).reshape(-1).astype(np.int64)
+        lin = lin[~used[lin]]
+        n = int(rng.integers(1, 400)) if s % 7 else 1
+        if lin.size == 0:
+            continue
+        lin = rng.permutation(lin)[:n]
+        used[lin] = True
+        kind = s % 11
+        if kind == 3:
+            v = np.zeros(lin.size, np.uint16)
+        elif kind in (5, 9):
+            v = rng.integers(60000, 65536, lin.size).astype(np.uint16)
+        else:
+            v = rng.integers(100, 4001, lin.size).astype(np.uint16)
+        vol[lin] = v
+        idx_all.append(lin.astype(np.uint64))
+        ptr.append(ptr[-1] + lin.size)
+        thr.append(int(v.min()) - int(rng.integers(0, 2)) if v.min() > 0 else 0)
+    idx = np.concatenate(idx_all)
+    ptr = np.array(ptr, np.int64)
+    S = ptr.size - 1
+    cent = np.zeros((S, 3), np.float32)
+    inten = np.zeros(S, np.float32)
+    for s in range(S):
+        seg = np.ascontiguousarray(idx[ptr[s]:ptr[s + 1]])
+        T.ref_sv_centroid(vol.ctypes.data, dims.ctypes.data, seg.ctypes.data, seg.size,
+                          O.p(cent[s], O._fp), O.p(inten[s:s + 1], O._fp))
+    np.savez_compressed(os.path.join(HERE, "sv_centroid_vectors.npz"), dims=dims, idx=idx,
+                        raw=vol[idx.astype(np.int64)], ptr=ptr,
+                        thr=np.array(thr, np.uint16), centroid=cent, intensity=inten)
+    print("wrote sv_centroid_vectors.npz (%d supervoxels, %d voxels)" % (S, idx.size))
+
+
 if __name__ == "__main__":
-    main()
+    if sys.argv[1:] != ["centroids"]:
+        main()
+    centroid_vectors()

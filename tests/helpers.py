@@ -107,3 +107,18 @@ def records_equal(a, b):
         return False
     return all(np.ascontiguousarray(a[k]).tobytes() == np.ascontiguousarray(b[k]).tobytes()
                for k in a.dtype.names if k)
+
+
+def raw_from_frame(frame, dims, seed=0, bright=3000):
+    """A raw frame (pixel_idx, raw, sv_ptr, thr, dims) with the frame's
+    integer voxel coordinates and supervoxels: raw = thr + round(img * bright)
+    with thr drawn per supervoxel, voxels in their list order."""
+    rng = np.random.default_rng(seed)
+    N, S = frame.N, frame.S
+    q = frame.query.reshape(3, N).astype(np.int64)
+    d = np.asarray(dims, np.int64)
+    idx = (q[0] + d[0] * (q[1] + d[1] * q[2])).astype(np.uint64)
+    thr = rng.integers(50, 400, S).astype(np.uint16)
+    per = np.repeat(thr.astype(np.int64), np.diff(frame.sv_ptr))
+    raw = (per + np.round(frame.img.astype(np.float64) * bright)).astype(np.uint16)
+    return idx, raw, frame.sv_ptr.astype(np.int64), thr, d

@@ -86,6 +86,7 @@ def oracle():
                                  ctypes.c_int, _ip, ctypes.c_double]
         L.orc_local_loglik.argtypes = [_ll, _ll, _fp, _fp, _lp, ctypes.c_int, _vp,
                                        ctypes.c_int, _ip, ctypes.c_int, _lp, _ip, _dp]
+        L.orc_marshal_frame.argtypes = [_ll, _ll, _vp, _vp, _lp, _vp, _lp, _fp, _fp, _fp]
         L.orc_converged.argtypes = [ctypes.c_double] * 5
         L.orc_assign_supervoxels.argtypes = [_ll, ctypes.c_int, _vp,
                                              ctypes.c_int, _fp, _ip, ctypes.c_int]
@@ -155,6 +156,25 @@ def local_loglik(frame, gm, ind, K, lists, out=None):
     return out
 
 
+def marshal_frame(pixel_idx, raw, sv_ptr, thr, dims):
+    """Oracle input marshalling; returns (query [3N], img [N], centroids [3S])."""
+    L = oracle()
+    idx = np.ascontiguousarray(pixel_idx, np.uint64)
+    rw = np.ascontiguousarray(raw, np.uint16)
+    ptr = np.ascontiguousarray(sv_ptr, np.int64)
+    th = np.ascontiguousarray(thr, np.uint16)
+    dm = np.ascontiguousarray(dims, np.int64)
+    N, S = idx.shape[0], th.shape[0]
+    q = np.zeros(max(3 * N, 1), np.float32)
+    im = np.zeros(max(N, 1), np.float32)
+    c = np.zeros(max(3 * S, 1), np.float32)
+    rc = L.orc_marshal_frame(N, S, idx.ctypes.data, rw.ctypes.data, p(ptr, _lp), th.ctypes.data,
+                             p(dm, _lp), p(q, _fp), p(im, _fp), p(c, _fp))
+    if rc:
+        raise ValueError("index outside the volume")
+    return q[: 3 * N], im[:N], c[: 3 * S]
+
+
 _twin = None
 
 
@@ -180,5 +200,7 @@ def reftwin():
                                      ctypes.c_double]
         L.ref_update_priors.argtypes = [_dp, ctypes.c_double, ctypes.c_double,
                                         ctypes.c_double, ctypes.c_double, _dp]
+        if hasattr(L, "ref_sv_centroid"):
+            L.ref_sv_centroid.argtypes = [_vp, _vp, _vp, _ll, _fp, _fp]
         _twin = L
     return _twin

@@ -89,6 +89,54 @@ extern "C" int tgmm_GMEMinitializeMemoryWithSupervoxels(
   return TGMM_OK;
 }
 
+extern "C" int tgmm_GMEMinitializeMemoryFromRaw(float** queryCUDA, float** imgDataCUDA,
+                                               const float* scale, float** rnkCUDA,
+                                               int** indCUDA, int** indCUDAtr,
+                                               float** centroidLabelPositionCUDA,
+                                               long long** labelListPtrCUDA,
+                                               const tgmm_raw_frame* frame) {
+  if (!queryCUDA || !imgDataCUDA || !rnkCUDA || !indCUDA || !indCUDAtr ||
+      !centroidLabelPositionCUDA || !labelListPtrCUDA || !scale || !frame || frame->N < 0 ||
+      frame->S < 0) {
+    tgmm_set_error("GMEMinitializeMemoryFromRaw: NULL argument or negative size");
+    return TGMM_ERR_ARG;
+  }
+  if (frame->S > 65535) {
+    tgmm_set_error("GMEMinitializeMemoryFromRaw: %lld supervoxels (the drop-in counts them in "
+                   "an unsigned short)", frame->S);
+    return TGMM_ERR_LIMIT;
+  }
+  const long long N = frame->N, S = frame->S, K = TGMM_LEGACY_K;
+  float** outs_f[4] = {queryCUDA, imgDataCUDA, rnkCUDA, centroidLabelPositionCUDA};
+  for (float** p : outs_f) *p = nullptr;
+  *indCUDA = *indCUDAtr = nullptr;
+  *labelListPtrCUDA = nullptr;
+  auto M = [](void** p, size_t n) { return hipMalloc(p, n > 0 ? n : 4); };
+  int rc = TGMM_OK;
+  hipError_t e = hipSuccess;
+  if ((e = M((void**)queryCUDA, 3 * N * sizeof(float))) != hipSuccess ||
+      (e = M((void**)imgDataCUDA, N * sizeof(float))) != hipSuccess ||
+      (e = M((void**)rnkCUDA, K * S * sizeof(float))) != hipSuccess ||
+      (e = M((void**)indCUDA, K * S * sizeof(int))) != hipSuccess ||
+      (e = M((void**)indCUDAtr, K * S * sizeof(int))) != hipSuccess ||
+      (e = M((void**)centroidLabelPositionCUDA, 3 * S * sizeof(float))) != hipSuccess ||
+      (e = M((void**)labelListPtrCUDA, (S + 1) * sizeof(long long))) != hipSuccess) {
+    tgmm_set_error("GMEMinitializeMemoryFromRaw: hipMalloc failed: %s", hipGetErrorString(e));
+    rc = e == hipErrorOutOfMemory ? TGMM_ERR_OOM : TGMM_ERR_DEVICE;
+  }
+  if (!rc)
+    rc = tgmm_marshal_frame(frame, *queryCUDA, *imgDataCUDA, *centroidLabelPositionCUDA,
+                            *labelListPtrCUDA);
+  if (rc) {
+    tgmm_GMEMreleaseMemoryWithSupervoxels(queryCUDA, imgDataCUDA, rnkCUDA, indCUDA, indCUDAtr,
+                                          centroidLabelPositionCUDA, labelListPtrCUDA);
+    return rc;
+  }
+  std::lock_guard<std::mutex> lk(g_mu);
+  for (int a = 0; a < 3; a++) g_scale_knn[a] = scale[a];
+  return TGMM_OK;
+}
+
 // Voxel mode (GMEMvariationalInferenceCUDA, GMEMupdateCUDA.cu:3420-3973):
 // the same plain per-frame solve over one-voxel supervoxels, with the
 // reference's size limits (:3425-3440) and outputs of K x query_nb. The

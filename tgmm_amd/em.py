@@ -138,6 +138,81 @@ _vp = ctypes.c_void_p
 _pp = ctypes.POINTER(ctypes.c_void_p)
 
 
+class RawFrame(ctypes.Structure):
+    """tgmm_raw_frame: one frame as raw supervoxel voxels."""
+    _fields_ = [
+        ("N", ctypes.c_longlong),
+        ("S", ctypes.c_longlong),
+        ("dims", ctypes.c_longlong * 3),
+        ("pixel_idx", ctypes.c_void_p),
+        ("raw", ctypes.c_void_p),
+        ("volume", ctypes.c_void_p),
+        ("sv_ptr", ctypes.c_void_p),
+        ("thr", ctypes.c_void_p),
+        ("device_inputs", ctypes.c_int),
+    ]
+
+
+def raw_frame(pixel_idx, raw, sv_ptr, thr, dims, volume=None):
+    """(RawFrame, keep-alive list) from numpy arrays (host) or DeviceArrays
+    (device) for pixel_idx / raw / thr, all of one kind; volume: DeviceArray
+    of the whole uint16 image, read when raw is None."""
+    keep = []
+    kinds = {isinstance(a, DeviceArray) for a in (pixel_idx, raw, thr) if a is not None}
+    if len(kinds) > 1:
+        raise TypeError("pixel_idx, raw and thr must all be host arrays or all DeviceArrays")
+
+    def host_or_dev(a, dtype):
+        if a is None:
+            return None
+        if isinstance(a, DeviceArray):
+            keep.append(a)
+            return a.ptr.value
+        h = np.ascontiguousarray(a, dtype)
+        keep.append(h)
+        return h.ctypes.data
+    ptr = np.ascontiguousarray(sv_ptr, np.int64)
+    keep.append(ptr)
+    f = RawFrame()
+    f.N = int(ptr[-1])
+    f.S = ptr.shape[0] - 1
+    f.dims[:] = [int(d) for d in dims]
+    f.pixel_idx = host_or_dev(pixel_idx, np.uint64)
+    f.raw = host_or_dev(raw, np.uint16)
+    f.volume = host_or_dev(volume, np.uint16)
+    f.sv_ptr = ptr.ctypes.data
+    f.thr = host_or_dev(thr, np.uint16)
+    f.device_inputs = 1 if kinds == {True} else 0
+    return f, keep
+
+
+def marshal_frame(pixel_idx, raw, sv_ptr, thr, dims, volume=None):
+    """EM inputs built on the device from a raw frame (tgmm_marshal_frame).
+    Returns DeviceArrays (query [3N], img [N], centroids [3S], sv_ptr [S+1])."""
+    f, keep = raw_frame(pixel_idx, raw, sv_ptr, thr, dims, volume)
+    q = DeviceArray(nbytes=3 * f.N * 4)
+    im = DeviceArray(nbytes=f.N * 4)
+    c = DeviceArray(nbytes=3 * f.S * 4)
+    p = DeviceArray(nbytes=(f.S + 1) * 8)
+    _check(load().tgmm_marshal_frame(ctypes.byref(f), q.ptr, im.ptr, c.ptr, p.ptr))
+    return q, im, c, p
+
+
+def GMEMinitializeMemoryFromRaw(frame_arrays, scale):
+    """tgmm_GMEMinitializeMemoryFromRaw: the seven device arrays of
+    GMEMinitializeMemoryWithSupervoxels, filled from a raw frame given as
+    (pixel_idx, raw, sv_ptr, thr, dims[, volume]). Returns a DevicePointers."""
+    f, keep = raw_frame(*frame_arrays)
+    sc = np.ascontiguousarray(scale, np.float32)
+    d = DevicePointers()
+    _check(load().tgmm_GMEMinitializeMemoryFromRaw(
+        ctypes.byref(d.queryCUDA), ctypes.byref(d.imgDataCUDA), sc.ctypes.data,
+        ctypes.byref(d.rnkCUDA), ctypes.byref(d.indCUDA), ctypes.byref(d.indCUDAtr),
+        ctypes.byref(d.centroidLabelPositionCUDA), ctypes.byref(d.labelListPtrCUDA),
+        ctypes.byref(f)))
+    return d
+
+
 def load():
     """Load libtgmm_em.so (raises if it was not built)."""
     global _lib
@@ -165,6 +240,9 @@ def load():
     L.tgmm_calculateLocalLikelihood.argtypes = [
         _vp, _vp, _vp, ctypes.c_int, _vp, _vp, _vp, _vp, _vp, _vp, ctypes.c_longlong,
         ctypes.c_int, ctypes.c_int]
+    L.tgmm_marshal_frame.argtypes = [ctypes.POINTER(RawFrame), _vp, _vp, _vp, _vp]
+    L.tgmm_GMEMinitializeMemoryFromRaw.argtypes = [_pp, _pp, _vp, _pp, _pp, _pp, _pp, _pp,
+                                                   ctypes.POINTER(RawFrame)]
     L.tgmm_batch_create.argtypes = [ctypes.c_int, ctypes.POINTER(FrameDesc), ctypes.c_int,
                                     ctypes.POINTER(EmOpts), _pp]
     L.tgmm_batch_reset.argtypes = [_vp]
