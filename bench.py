@@ -24,6 +24,11 @@ supervoxel voxels to the EM's float inputs and centroids) for the frame of
 beside the upload of the finished float arrays that the reference's host path
 needs and the oracle's host restatement of the same work.
 
+With --svknn it measures the supervoxel nearest neighbours (knnCUDA_) on the
+frame's supervoxel centroids in raster order: the tracker's three searches
+per frame (in space, K = 11; forward and backward against the next frame's
+centroids, K = 10).
+
 With --local it measures the local log-likelihood entry point
 (calculateLocalLikelihood, the cell-division test's input) on the solved
 frame of --config (default 2): the tracker's per-Gaussian supervoxel lists,
@@ -143,6 +148,8 @@ def main():
                     help="per-voxel EM (GMEMvariationalInferenceCUDA) on the same frames")
     ap.add_argument("--marshal", action="store_true",
                     help="measure the input marshalling on the device instead")
+    ap.add_argument("--svknn", action="store_true",
+                    help="measure the supervoxel nearest-neighbour search instead")
     ap.add_argument("--local", action="store_true",
                     help="measure the local log-likelihood entry point instead")
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
@@ -164,6 +171,8 @@ def main():
         return local_bench(args)
     if args.marshal:
         return marshal_bench(args)
+    if args.svknn:
+        return svknn_bench(args)
 
     t0 = time.time()
     frames, K, iters, workload = _frames_for(args.config)
@@ -404,6 +413,67 @@ def marshal_bench(args):
             "cpu_baseline": {"value": N / cpu, "unit": "voxels/s", "cores": 1, "kind": "port",
                              "sample": "the oracle's single-thread restatement of the whole "
                                        "frame, %d calls (at least 2 s)" % cpu_reps}}
+    print(json.dumps(line), flush=True)
+
+
+def svknn_bench(args):
+    """tgmm_knnCUDA_ for one frame's three searches: centroids of the frame's
+    supervoxels sorted by (z, y, x) (the raster order of a segmentation's
+    labels), the next frame's centroids displaced by N(0, 1.5) voxels and
+    re-sorted. Timed per frame (the three calls, host in and out); every
+    slot of 2,000 sampled queries compared bitwise with the oracle; the
+    oracle's time on those queries, scaled to the frame, is the CPU
+    baseline."""
+    import numpy as np
+    from tgmm_amd import em
+    import oracle_lib as O
+    frames, K, iters, workload = _frames_for(args.config)
+    f = frames[0]
+    scale = np.array([1.0, 1.0, 5.0], np.float32)
+
+    def raster(c):
+        return c[:, np.lexsort((c[0], c[1], c[2]))]
+    cur = raster(f.centroids.reshape(3, f.S).astype(np.float64))
+    nxt = raster(cur + np.random.default_rng(2).normal(0, 1.5, cur.shape))
+    a = np.ascontiguousarray(cur.astype(np.float32).reshape(-1))
+    b = np.ascontiguousarray(nxt.astype(np.float32).reshape(-1))
+    calls = [("space", a, a, 11), ("forward", a, b, 10), ("backward", b, a, 10)]
+
+    def frame():
+        return [em.knnCUDA_(q, r, k, scale) for _, q, r, k in calls]
+    for _ in range(max(1, args.warmup)):
+        outs = frame()
+    ts = []
+    for _ in range(args.steps):
+        t0 = time.perf_counter()
+        frame()
+        ts.append(time.perf_counter() - t0)
+    S = f.S
+    rng = np.random.default_rng(3)
+    sample = np.sort(rng.choice(S, min(S, 2000), replace=False))
+    exact = True
+    cpu_s = 0.0
+    for (name, q, r, k), (gi, gd) in zip(calls, outs):
+        qs = np.ascontiguousarray(q.reshape(3, S)[:, sample].reshape(-1))
+        t0 = time.perf_counter()
+        oi, od = O.sv_knn(qs, r, k, scale)
+        cpu_s += time.perf_counter() - t0
+        gi2 = gi.reshape(k, S)[:, sample].reshape(-1)
+        gd2 = gd.reshape(k, S)[:, sample].reshape(-1)
+        exact &= bool(np.array_equal(gi2, oi) and np.array_equal(gd2, od))
+    cpu_frame = cpu_s * S / sample.size
+    t = _percentile(ts, 50)
+    line = {"metric": "supervoxel kNN frames/s (three searches per frame)", "value": 1.0 / t,
+            "unit": "frames/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+            "ms_per_step": 1e3 * t, "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+            "config": {"workload": workload + "; %d supervoxel centroids, raster order, "
+                       "searches in space (K=11), forward and backward (K=10)" % S},
+            "bitwise_equal_to_oracle_on_sample": exact, "sampled_queries": int(sample.size),
+            "cpu_baseline": {"value": 1.0 / cpu_frame, "unit": "frames/s",
+                             "cores": O.oracle().orc_get_threads(), "kind": "port",
+                             "sample": "the oracle's scan for %d sampled queries of each "
+                                       "search, scaled to all %d" % (sample.size, S)}}
     print(json.dumps(line), flush=True)
 
 

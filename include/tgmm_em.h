@@ -30,6 +30,7 @@ extern "C" {
 #define TGMM_MAX_SUPERVOXELS_PER_GAUSSIAN 16 /* GMEMupdateCUDA.h:20 */
 #define TGMM_KMAX 16                         /* widened API: K in [1, 16] */
 #define TGMM_LEGACY_K 5                      /* constants.h:15 maxGaussiansPerVoxel */
+#define TGMM_SV_KNN_MAX 11 /* temporalLogicalRules/knnCUDA/knnCuda.h:28 maxKNN */
 
 /* status codes */
 #define TGMM_OK 0
@@ -381,6 +382,27 @@ int tgmm_GMEMinitializeMemoryFromRaw(float** queryCUDA, float** imgDataCUDA, con
                                      float** rnkCUDA, int** indCUDA, int** indCUDAtr,
                                      float** centroidLabelPositionCUDA,
                                      long long** labelListPtrCUDA, const tgmm_raw_frame* frame);
+
+/* ---------------------------------------------------------------------------
+ * 4. Supervoxel nearest neighbours
+ * ------------------------------------------------------------------------ */
+
+/* Replaces knnCUDA_ (temporalLogicalRules/knnCUDA/knnCuda.h:56-58,
+ * knnCuda.cu:142-232), the tracker's supervoxel neighbour search in space and
+ * time (lineageHyperTree.cpp:2526-2775, supervoxel.cpp:700-1290). Host
+ * inputs query [3*query_nb] and ref [3*ref_nb] (SoA centroids), scale[3];
+ * host outputs ind and dist (may be NULL) [KNN*query_nb], slot-major
+ * (q + j*query_nb): for each query the references are scanned in index
+ * order and a reference whose scaled squared distance
+ * ((q-r)_x s_x)^2 + ((q-r)_y s_y)^2 + ((q-r)_z s_z)^2 (float, in that order)
+ * is not above the largest slot distance replaces the first slot holding it
+ * (slots start at 1e32), the reference's exact slot layout; slots never
+ * filled have ind = -1 (uninitialised in the reference) and dist = 1e32.
+ * ref_nb <= 0 sets dist to 1e32 and leaves ind as given. KNN outside
+ * 1..TGMM_SV_KNN_MAX returns TGMM_ERR_LIMIT (the reference's kernel writes
+ * nothing then). Runs on device devCUDA. */
+int tgmm_knnCUDA_(int* ind, float* dist, const float* query, const float* ref,
+                  long long query_nb, int ref_nb, int KNN, const float* scale, int devCUDA);
 
 #ifdef __cplusplus
 }

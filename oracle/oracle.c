@@ -17,6 +17,8 @@
  *   convergence    GMEM.cu:4143-4148; supervoxel assignment GMEM.cu:4328-4350
  *   local LL       GMEM.cu:2952-3050 with the kernel :1989-2103 (lists of
  *                  supervoxels, the cell-division likelihood-ratio input)
+ *   supervoxel kNN temporalLogicalRules/knnCUDA/knnCuda.cu:43-232 (the
+ *                  scan in reference order, slot replacement rule)
  *   marshalling    mainTrackingGaussianMixtureModel.cpp:598-669, 774-870,
  *                  supervoxel.cpp:1410-1449, mylib array.p:557-576,
  *                  2708-2820 (raw uint16 voxels to the EM's float inputs)
@@ -650,6 +652,62 @@ int orc_marshal_frame(long long N, long long S, const unsigned long long* pixel_
   const double f = 1.0 / ((double)hi - (double)lo);
   for (long long v = 0; v < N; v++) img[v] = (float)(f * ((double)img[v] - (double)lo));
   return 0;
+}
+
+/* Supervoxel nearest neighbours: knnCUDA_ (temporalLogicalRules/knnCUDA/
+ * knnCuda.cu:142-232) with knnKernelNoConstantMemory (:56-140). For each
+ * query the references are scanned in index order; K slots of (distance,
+ * index) start at (1e32, -1) and the threshold at 1e32. The scaled squared
+ * distance is ((q0 - r0) s0)^2 + ((q1 - r1) s1)^2 + ((q2 - r2) s2)^2 in float,
+ * added in that order (the reference's partial-sum early exits give the same
+ * decisions, the partial sums being non-decreasing). A reference whose
+ * distance is not above the threshold replaces the slot at minPos; then the
+ * threshold becomes the largest slot distance and minPos the first slot
+ * holding it (findMaxPosition, :43-53). Output slot-major, ind/dist
+ * [K * query_nb]; unfilled slots keep index -1 (the reference leaves them
+ * uninitialised) and distance 1e32. ref_nb <= 0: dist set to 1e32, ind left
+ * as given (:165-171). dist may be NULL. */
+void orc_sv_knn(int* ind, float* dist, const float* query, const float* ref,
+                long long query_nb, int ref_nb, int K, const float scale[3]) {
+  if (ref_nb <= 0) {
+    if (dist)
+      for (long long i = 0; i < query_nb * K; i++) dist[i] = 1e32f;
+    return;
+  }
+#pragma omp parallel for schedule(static)
+  for (long long q = 0; q < query_nb; q++) {
+    float md[TGMM_SV_KNN_MAX];
+    int mi[TGMM_SV_KNN_MAX];
+    for (int j = 0; j < K; j++) {
+      md[j] = 1e32f;
+      mi[j] = -1;
+    }
+    float thr = 1e32f;
+    int pos = 0;
+    const float q0 = query[q], q1 = query[q + query_nb], q2 = query[q + 2 * query_nb];
+    for (int r = 0; r < ref_nb; r++) {
+      float a = (q0 - ref[r]) * scale[0];
+      float d = a * a;
+      a = (q1 - ref[r + ref_nb]) * scale[1];
+      d += a * a;
+      a = (q2 - ref[r + 2 * ref_nb]) * scale[2];
+      d += a * a;
+      if (d > thr) continue;
+      md[pos] = d;
+      mi[pos] = r;
+      thr = md[0];
+      pos = 0;
+      for (int j = 1; j < K; j++)
+        if (thr < md[j]) {
+          thr = md[j];
+          pos = j;
+        }
+    }
+    for (int j = 0; j < K; j++) {
+      ind[q + j * query_nb] = mi[j];
+      if (dist) dist[q + j * query_nb] = md[j];
+    }
+  }
 }
 
 /* loop condition GMEM.cu:4145-4148: 1 = keep iterating */

@@ -70,6 +70,9 @@ int orc_marshal_frame(long long N, long long S, const unsigned long long* pixel_
                       const unsigned short* raw, const long long* sv_ptr,
                       const unsigned short* thr, const long long dims[3], float* query,
                       float* img, float* cent);
+/* supervoxel kNN (knnCUDA_): slot-major ind/dist [K * query_nb], K <= 11 */
+void orc_sv_knn(int* ind, float* dist, const float* query, const float* ref,
+                long long query_nb, int ref_nb, int K, const float scale[3]);
 int orc_converged(double ll, double ll1, double ll2, double ll3, double tol);
 void orc_assign_supervoxels(long long S, int G, GaussianMixtureModelCUDA* gm,
                             int K, const float* rnk, const int* ind, int cap);

@@ -87,6 +87,7 @@ def oracle():
         L.orc_local_loglik.argtypes = [_ll, _ll, _fp, _fp, _lp, ctypes.c_int, _vp,
                                        ctypes.c_int, _ip, ctypes.c_int, _lp, _ip, _dp]
         L.orc_marshal_frame.argtypes = [_ll, _ll, _vp, _vp, _lp, _vp, _lp, _fp, _fp, _fp]
+        L.orc_sv_knn.argtypes = [_ip, _fp, _fp, _fp, _ll, ctypes.c_int, ctypes.c_int, _fp]
         L.orc_converged.argtypes = [ctypes.c_double] * 5
         L.orc_assign_supervoxels.argtypes = [_ll, ctypes.c_int, _vp,
                                              ctypes.c_int, _fp, _ip, ctypes.c_int]
@@ -173,6 +174,20 @@ def marshal_frame(pixel_idx, raw, sv_ptr, thr, dims):
     if rc:
         raise ValueError("index outside the volume")
     return q[: 3 * N], im[:N], c[: 3 * S]
+
+
+def sv_knn(query, ref, K, scale, with_dist=True):
+    """Oracle supervoxel kNN (knnCUDA_): (ind [K*Q], dist [K*Q]) slot-major."""
+    L = oracle()
+    q = np.ascontiguousarray(query, np.float32)
+    r = np.ascontiguousarray(ref, np.float32)
+    Q, R = q.size // 3, r.size // 3
+    ind = np.full(max(K * Q, 1), -7, np.int32)
+    dist = np.zeros(max(K * Q, 1), np.float32)
+    sc = np.ascontiguousarray(scale, np.float32)
+    L.orc_sv_knn(p(ind, _ip), p(dist, _fp) if with_dist else None, p(q, _fp), p(r, _fp), Q, R,
+                 K, p(sc, _fp))
+    return ind[: K * Q], dist[: K * Q]
 
 
 _twin = None

@@ -106,7 +106,8 @@ def test_generator_is_deterministic():
 
 
 def test_compat_header_compiles_and_links(tmp_path):
-    """include/tgmm_em_compat.hpp gives the reference's C++ names: a tracker
+    """include/tgmm_em_compat.hpp and include/tgmm_knn_compat.hpp give the
+    reference's C++ names: a tracker
     translation unit that calls each of them compiles and links against the
     library. The tracker's constants.h is stood in for by a minimal struct
     with the same static members (constants.h:19-45)."""
@@ -123,6 +124,7 @@ def test_compat_header_compiles_and_links(tmp_path):
         "double regularizePrecisionMatrixConstants::maxExcentricity = -1.0;\n")
     (tmp_path / "tracker.cpp").write_text(
         '#include "constants.h"\n#include "tgmm_em_compat.hpp"\n'
+        '#include "tgmm_knn_compat.hpp"\n'
         "int main(int argc, char**) {\n"
         "  if (argc < 5) return 0;  // compiled and linked, never run\n"
         "  float *q = 0, *im = 0, *r = 0, *c = 0; int *i = 0, *it = 0; long long *p = 0;\n"
@@ -136,6 +138,8 @@ def test_compat_header_compiles_and_links(tmp_path):
         "  GMEMvariationalInferenceCUDA(q, im, r, 0, i, it, g, 0, 0, 10, 1e-5, 0, 0, true);\n"
         "  std::vector<double> ll; std::vector<std::vector<int> > lists(2);\n"
         "  calculateLocalLikelihood(ll, lists, q, im, r, i, g, p, 0, 0, 0);\n"
+        "  int nn = knnCUDA_(i, r, q, c, 0, 0, maxKNN, sc, 0);\n"
+        "  (void)nn;\n"
         "  GMEMreleaseMemoryWithSupervoxels(&q, &im, &r, &i, &it, &c, &p);\n"
         "  return 0;\n}\n")
     lib = os.path.join(ROOT, "tgmm_amd")

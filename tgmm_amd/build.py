@@ -18,7 +18,8 @@ CSRC = os.path.join(HERE, "csrc")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 HIP_SOURCES = ["tgmm_kernels.hip", "tgmm_runtime.hip", "tgmm_legacy.hip",
-               "tgmm_server.hip", "tgmm_local.hip", "tgmm_marshal.hip"]
+               "tgmm_server.hip", "tgmm_local.hip", "tgmm_marshal.hip",
+               "tgmm_svknn.hip"]
 HIP_FLAGS = [
     "-O3", "--offload-arch=gfx950", "-std=c++17", "-fPIC",
     # keep the compiler from forming packed FP32 instructions

@@ -243,6 +243,8 @@ def load():
     L.tgmm_marshal_frame.argtypes = [ctypes.POINTER(RawFrame), _vp, _vp, _vp, _vp]
     L.tgmm_GMEMinitializeMemoryFromRaw.argtypes = [_pp, _pp, _vp, _pp, _pp, _pp, _pp, _pp,
                                                    ctypes.POINTER(RawFrame)]
+    L.tgmm_knnCUDA_.argtypes = [_vp, _vp, _vp, _vp, ctypes.c_longlong, ctypes.c_int,
+                                ctypes.c_int, _vp, ctypes.c_int]
     L.tgmm_batch_create.argtypes = [ctypes.c_int, ctypes.POINTER(FrameDesc), ctypes.c_int,
                                     ctypes.POINTER(EmOpts), _pp]
     L.tgmm_batch_reset.argtypes = [_vp]
@@ -435,6 +437,23 @@ def calculateLocalLikelihood(localLikelihood, listSupervoxels, queryCUDA, imgDat
         int(ref_nb), int(numLabels)))
     localLikelihood[:] = [float(x) for x in out[:n]]
     return localLikelihood
+
+
+def knnCUDA_(query, ref, KNN, scale, devCUDA=0, with_dist=True):
+    """Supervoxel nearest neighbours (knnCuda.h:56-58): query [3Q], ref [3R]
+    SoA centroids on the host. Returns (ind, dist) [KNN*Q] slot-major, dist
+    None when with_dist is False. With no references ind is -1 throughout
+    (the C entry point leaves it as given) and dist 1e32."""
+    q = np.ascontiguousarray(query, np.float32)
+    r = np.ascontiguousarray(ref, np.float32)
+    sc = np.ascontiguousarray(scale, np.float32)
+    Q, R = q.size // 3, r.size // 3
+    ind = np.full(max(KNN * Q, 1), -1, np.int32)
+    dist = np.zeros(max(KNN * Q, 1), np.float32) if with_dist else None
+    _check(load().tgmm_knnCUDA_(ind.ctypes.data, dist.ctypes.data if with_dist else None,
+                                q.ctypes.data, r.ctypes.data, Q, R, int(KNN), sc.ctypes.data,
+                                int(devCUDA)))
+    return ind[: KNN * Q], (dist[: KNN * Q] if with_dist else None)
 
 
 def GMEMreleaseMemoryWithSupervoxels(d):
