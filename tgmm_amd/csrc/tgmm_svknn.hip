@@ -9,8 +9,13 @@
 // The slot layout is the outcome of a scan of the references in index
 // order: a reference whose distance is not above the current threshold (the
 // largest slot distance) replaces the first slot holding it. The kernel
-// replays that scan exactly, one lane per query, with the references read
-// as wave-uniform (scalar) loads. It skips whole blocks of 64 consecutive
+// replays that scan exactly, one lane per query. Per block of 64 references
+// each lane first marks, branch-free, the references not above its
+// threshold at the block's start (wave-uniform loads), a superset of the
+// ones the scan inserts since the threshold only falls; it then walks only
+// its marks, in index order, with the exact test against the current
+// threshold, so the wave pays the slot replacement for its busiest lane's
+// candidates rather than for every reference. It skips whole blocks of 64 consecutive
 // references that no lane of the wave can insert: a block's bounding box
 // gives, per axis, a gap no larger than any of its references' |q - r|, and
 // the bound is formed with the distance's own float operations in the same
@@ -33,7 +38,6 @@ namespace {
 
 constexpr int kKB = 64;  // references per bounding box
 constexpr int kKM = TGMM_SV_KNN_MAX;
-constexpr int kKU = 8;   // references whose distances are formed together
 constexpr int kKT = 64;  // threads (queries) per block
 
 // Bounding box of each block of 64 references: box[a][b] = min, box[3+a][b]
@@ -62,6 +66,19 @@ __global__ __launch_bounds__(kKB) void k_svknn_boxes(const float* __restrict__ r
       box[(long long)a * nblk + b] = any_nan ? -INFINITY : lo[a];
       box[(long long)(3 + a) * nblk + b] = any_nan ? INFINITY : hi[a];
     }
+}
+
+// the reference's scaled squared distance, float, x then y then z, no contraction
+__device__ inline float svdist(float q0, float q1, float q2, const float* __restrict__ ref,
+                               int r, int R, float s0, float s1, float s2) {
+#pragma clang fp contract(off)
+  float a = (q0 - ref[r]) * s0;
+  float d = a * a;
+  a = (q1 - ref[r + R]) * s1;
+  d += a * a;
+  a = (q2 - ref[r + 2 * R]) * s2;
+  d += a * a;
+  return d;
 }
 
 __global__ __launch_bounds__(kKT) void k_svknn(const float* __restrict__ query, long long Q,
@@ -96,40 +113,36 @@ __global__ __launch_bounds__(kKT) void k_svknn(const float* __restrict__ query, 
     lb += g1 * g1;
     lb += g2 * g2;
     if (!__any(qnan || !(lb > thr))) continue;  // no lane would insert from this block
-    const int r1 = (b + 1) * kKB < R ? (b + 1) * kKB : R;
-    for (int r8 = b * kKB; r8 < r1; r8 += kKU) {
-      // kKU distances computed independently, then the scan's tests in order
-      float dd[kKU];
+    const int rb = b * kKB;
+    const int n = R - rb < kKB ? R - rb : kKB;
+    // the references not above this lane's threshold at the block's start: a
+    // superset of those the scan inserts (the threshold only falls)
+    unsigned long long cand = 0ull;
+    const float t0 = thr;
+    for (int u = 0; u < n; u++) {
+      const float d = svdist(q0, q1, q2, ref, rb + u, R, s0, s1, s2);
+      cand |= (d > t0) ? 0ull : (1ull << u);
+    }
+    // the scan over the candidates, in index order, with the current threshold
+    while (cand) {
+      const int u = __builtin_ctzll(cand);
+      cand &= cand - 1ull;
+      const float d = svdist(q0, q1, q2, ref, rb + u, R, s0, s1, s2);
+      if (d > thr) continue;
 #pragma unroll
-      for (int u = 0; u < kKU; u++) {
-        const int r = r8 + u < r1 ? r8 + u : r1 - 1;
-        float a = (q0 - ref[r]) * s0;
-        float d = a * a;
-        a = (q1 - ref[r + R]) * s1;
-        d += a * a;
-        a = (q2 - ref[r + 2 * R]) * s2;
-        d += a * a;
-        dd[u] = d;
-      }
+      for (int j = 0; j < kKM; j++)
+        if (j == pos) {
+          md[j] = d;
+          mi[j] = rb + u;
+        }
+      thr = md[0];
+      pos = 0;
 #pragma unroll
-      for (int u = 0; u < kKU; u++) {
-        const float d = dd[u];
-        if (r8 + u >= r1 || d > thr) continue;
-#pragma unroll
-        for (int j = 0; j < kKM; j++)
-          if (j == pos) {
-            md[j] = d;
-            mi[j] = r8 + u;
-          }
-        thr = md[0];
-        pos = 0;
-#pragma unroll
-        for (int j = 1; j < kKM; j++)
-          if (j < K && thr < md[j]) {
-            thr = md[j];
-            pos = j;
-          }
-      }
+      for (int j = 1; j < kKM; j++)
+        if (j < K && thr < md[j]) {
+          thr = md[j];
+          pos = j;
+        }
     }
   }
   if (!live) return;
