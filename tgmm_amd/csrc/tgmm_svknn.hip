@@ -43,13 +43,13 @@ constexpr int kKT = 64;  // threads (queries) per block
 // Bounding box of each block of 64 references: box[a][b] = min, box[3+a][b]
 // = max of coordinate a (-inf, +inf when the block holds a NaN).
 __global__ __launch_bounds__(kKB) void k_svknn_boxes(const float* __restrict__ ref, int R,
-                                                     int nblk, float* __restrict__ box) {
+                                                     int Rp, int nblk, float* __restrict__ box) {
   const int b = blockIdx.x;
   const int r = b * kKB + threadIdx.x;
   float lo[3], hi[3];
   int nan = 0;
   for (int a = 0; a < 3; a++) {
-    const float v = r < R ? ref[r + (long long)a * R] : NAN;
+    const float v = r < R ? ref[r + (long long)a * Rp] : NAN;
     const bool in = r < R;
     nan |= (in && isnan(v)) ? 1 : 0;
     lo[a] = in ? v : INFINITY;
@@ -81,8 +81,10 @@ __device__ inline float svdist(float q0, float q1, float q2, const float* __rest
   return d;
 }
 
+// ref is SoA with axis stride Rp = nblk * 64 (padded with zeros past R)
 __global__ __launch_bounds__(kKT) void k_svknn(const float* __restrict__ query, long long Q,
-                                               const float* __restrict__ ref, int R, int K,
+                                               const float* __restrict__ ref, int R, int Rp,
+                                               int K,
                                                float s0, float s1, float s2,
                                                const float* __restrict__ box, int nblk,
                                                int* __restrict__ ind, float* __restrict__ dist) {
@@ -117,17 +119,21 @@ __global__ __launch_bounds__(kKT) void k_svknn(const float* __restrict__ query, 
     const int n = R - rb < kKB ? R - rb : kKB;
     // the references not above this lane's threshold at the block's start: a
     // superset of those the scan inserts (the threshold only falls)
+    // (the reference arrays are padded to whole blocks, so the 64 loads of a
+    // block are unconditional and batch; the padding's marks are cleared)
     unsigned long long cand = 0ull;
     const float t0 = thr;
-    for (int u = 0; u < n; u++) {
-      const float d = svdist(q0, q1, q2, ref, rb + u, R, s0, s1, s2);
+#pragma unroll
+    for (int u = 0; u < kKB; u++) {
+      const float d = svdist(q0, q1, q2, ref, rb + u, Rp, s0, s1, s2);
       cand |= (d > t0) ? 0ull : (1ull << u);
     }
+    if (n < kKB) cand &= (1ull << n) - 1ull;
     // the scan over the candidates, in index order, with the current threshold
     while (cand) {
       const int u = __builtin_ctzll(cand);
       cand &= cand - 1ull;
-      const float d = svdist(q0, q1, q2, ref, rb + u, R, s0, s1, s2);
+      const float d = svdist(q0, q1, q2, ref, rb + u, Rp, s0, s1, s2);
       if (d > thr) continue;
 #pragma unroll
       for (int j = 0; j < kKM; j++)
@@ -195,9 +201,10 @@ extern "C" int tgmm_knnCUDA_(int* ind, float* dist, const float* query, const fl
   if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
   const long long Q = query_nb, R = ref_nb;
   const int nblk = (int)((R + kKB - 1) / kKB);
+  const long long Rp = (long long)nblk * kKB;
   const size_t o_q = 0;
   const size_t o_r = o_q + up256((size_t)Q * 12);
-  const size_t o_box = o_r + up256((size_t)R * 12);
+  const size_t o_box = o_r + up256((size_t)Rp * 12);
   const size_t o_ind = o_box + up256((size_t)nblk * 24);
   const size_t o_dist = o_ind + up256((size_t)Q * KNN * 4);
   const size_t total = o_dist + up256((size_t)Q * KNN * 4);
@@ -217,11 +224,17 @@ extern "C" int tgmm_knnCUDA_(int* ind, float* dist, const float* query, const fl
   auto* dbox = (float*)(base + o_box);
   auto* dind = (int*)(base + o_ind);
   auto* ddist = (float*)(base + o_dist);
-  if ((e = hipMemcpyAsync(dq, query, (size_t)Q * 12, hipMemcpyHostToDevice, 0)) != hipSuccess ||
-      (e = hipMemcpyAsync(dr, ref, (size_t)R * 12, hipMemcpyHostToDevice, 0)) != hipSuccess)
-    return hip_fail(e, "upload of the centroids");
-  k_svknn_boxes<<<nblk, kKB, 0, 0>>>(dr, (int)R, nblk, dbox);
-  k_svknn<<<(unsigned)((Q + kKT - 1) / kKT), kKT, 0, 0>>>(dq, Q, dr, (int)R, KNN, scale[0], scale[1],
+  if ((e = hipMemcpyAsync(dq, query, (size_t)Q * 12, hipMemcpyHostToDevice, 0)) != hipSuccess)
+    return hip_fail(e, "upload of the queries");
+  for (int a = 0; a < 3; a++)
+    if ((e = hipMemcpyAsync(dr + a * Rp, ref + a * R, (size_t)R * 4, hipMemcpyHostToDevice, 0)) !=
+            hipSuccess ||
+        (Rp > R && (e = hipMemsetAsync(dr + a * Rp + R, 0, (size_t)(Rp - R) * 4, 0)) !=
+                       hipSuccess))
+      return hip_fail(e, "upload of the references");
+  k_svknn_boxes<<<nblk, kKB, 0, 0>>>(dr, (int)R, (int)Rp, nblk, dbox);
+  k_svknn<<<(unsigned)((Q + kKT - 1) / kKT), kKT, 0, 0>>>(dq, Q, dr, (int)R, (int)Rp, KNN,
+                                                        scale[0], scale[1],
                                                       scale[2], dbox, nblk, dind,
                                                       dist ? ddist : nullptr);
   if ((e = hipGetLastError()) != hipSuccess) return hip_fail(e, "kernel launch");
