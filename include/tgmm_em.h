@@ -234,7 +234,9 @@ int tgmm_batch_fetch(tgmm_batch* b, tgmm_frame_desc* frames, int nframes);
  * same results (the kernels fix their summation order independently of the
  * shape); only the speed changes. Defaults: {16, 4, 16, 256, 256, 8, 6, 2}. */
 typedef struct tgmm_tiles {
-  int ll_group;    /* lanes per supervoxel, log-likelihood: 16, 32 or 64 */
+  int ll_group;    /* lanes per supervoxel, log-likelihood: 16, 32 or 64;
+                      1 (with ll_unroll 1) = one lane per one-voxel
+                      supervoxel, voxel mode only, its default there */
   int ll_unroll;   /* supervoxels per lane group in turn, log-likelihood:
                       1, 2 or 4 (8 with ll_group 16) */
   int mstep_lanes; /* lanes per Gaussian, M-step statistics: 4, 8 or 16 */

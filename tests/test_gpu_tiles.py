@@ -81,3 +81,49 @@ def test_autotune_choice_is_valid_and_changes_nothing(frames):
         b.solve(em.SOLVE_GRAPH)
         out = b.fetch()
     _same(ref, out)
+
+
+def _run_voxel(frames, K, tiles, iters=4):
+    with em.Batch(frames, em.make_opts(K=K, max_iter=iters, tol=-1.0, voxel=True)) as b:
+        if tiles is not None:
+            b.set_tiles(tiles)
+            assert b.get_tiles() == tiles
+        else:
+            t = b.get_tiles()
+            assert (t["ll_group"], t["ll_unroll"]) == (1, 1)  # voxel mode's default
+        b.solve(em.SOLVE_GRAPH)
+        return b.fetch()
+
+
+@pytest.mark.parametrize("K", [5, 8])
+def test_voxel_mode_lane_per_voxel_equals_group_kernel(frames, K):
+    # the lane-per-voxel log-likelihood (the default in voxel mode) against
+    # the 16-, 32- and 64-lane group kernels on the same one-voxel supervoxels
+    vf = [frames[0], synth.config1(seed=5, n_cells=60, target_voxels=9000)]
+    ref = _run_voxel(vf, K, None)
+    for gw, u in ((16, 4), (32, 2), (64, 1)):
+        t = dict(em.DEFAULT_TILES, ll_group=gw, ll_unroll=u)
+        print("voxel mode, ll tiles", gw, u, flush=True)
+        _same(ref, _run_voxel(vf, K, t))
+
+
+def test_lane_per_voxel_needs_voxel_mode(frames):
+    with em.Batch(frames[:1], em.make_opts(K=5, max_iter=2)) as b:
+        with pytest.raises(em.TgmmError):
+            b.set_tiles(dict(em.DEFAULT_TILES, ll_group=1, ll_unroll=1))
+    with em.Batch(frames[:1], em.make_opts(K=5, max_iter=2, voxel=True)) as b:
+        with pytest.raises(em.TgmmError):
+            b.set_tiles(dict(em.DEFAULT_TILES, ll_group=1, ll_unroll=2))
+
+
+def test_voxel_mode_autotune_includes_lane_per_voxel(frames):
+    vf = frames[:1]
+    ref = _run_voxel(vf, 5, None, iters=3)
+    with em.Batch(vf, em.make_opts(K=5, max_iter=3, tol=-1.0, voxel=True)) as b:
+        tiles, report = b.autotune(reps=2, cache=False)
+        print(tiles, "\n", report, flush=True)
+        assert report.splitlines()[0].startswith("loglik 1x1")
+        assert len(report.strip().splitlines()) == 35
+        b.solve(em.SOLVE_GRAPH)
+        out = b.fetch()
+    _same(ref, out)

@@ -1623,6 +1623,86 @@ __global__ __launch_bounds__(256) void k_loglik(DevPtrs P, KOpts O) {
   }
 }
 
+// Log-likelihood in voxel mode, where every supervoxel is one voxel (or
+// none, as padding): one lane per supervoxel, 256 per block. A lane forms its
+// voxel's term with k_loglik's operations (the anchor is the voxel itself, so
+// y = U (x - a) + e = e), its supervoxel sum as k_loglik's class sums leave
+// it (0 + term), and the block the same pairwise tree over its 256
+// supervoxels: the partials, hence the totals and every convergence
+// decision, are bitwise those of k_loglik, without 15 idle lanes of 16.
+template <int KM>
+__global__ __launch_bounds__(256) void k_loglik_voxel(DevPtrs P, KOpts O) {
+  __shared__ double tree[256];
+  const int f = P.sblk_frame[blockIdx.x];
+  if (f < 0) return;
+  const FrameDev* F = P.fr + f;
+  if (!F->active) return;
+  const long long S = P.Scap;
+  const long long s = (long long)blockIdx.x * 256 + threadIdx.x;
+  const int K = O.K;
+  const double lsa2 = F->log_sum_alpha * 1.4426950408889634074;
+  const int n = (P.sv_frame[s] == f) ? P.sv_len[s] : 0;
+  double t = 0.0;
+  if (n > 0) {
+    const long long g = P.sv_beg[s];
+    const float4 vv = make_float4(P.vx[g], P.vy[g], P.vz[g], P.vimg[g]);
+    const float ca0 = P.anchor[s], ca1 = P.anchor[S + s], ca2 = P.anchor[2 * S + s];
+    float tj[KM];
+#pragma unroll
+    for (int j = 0; j < KM; j++) {
+      const int kk = (j < K) ? P.ind[(long long)j * S + s] : -1;
+      float pe0 = 0.0f, pe1 = 0.0f, pe2 = 0.0f, pu[6] = {0, 0, 0, 0, 0, 0}, pg = -INFINITY;
+      if (kk >= 0) {
+        const LLPack lp = P.llpack[kk];
+        const double d0 = (double)ca0 - lp.m[0], d1 = (double)ca1 - lp.m[1],
+                     d2 = (double)ca2 - lp.m[2];
+#pragma unroll
+        for (int i = 0; i < 6; i++) pu[i] = lp.U[i];
+        pe0 = (float)((double)pu[0] * d0 + (double)pu[1] * d1 + (double)pu[2] * d2);
+        pe1 = (float)((double)pu[3] * d1 + (double)pu[4] * d2);
+        pe2 = (float)((double)pu[5] * d2);
+        pg = (float)(lp.lg2 - lsa2);
+      }
+      {
+#pragma clang fp contract(off)
+        const float x0 = vv.x - ca0, x1 = vv.y - ca1, x2 = vv.z - ca2;
+        const float y0 = __builtin_fmaf(pu[0], x0, __builtin_fmaf(pu[1], x1,
+                                                                  __builtin_fmaf(pu[2], x2, pe0)));
+        const float y1 = __builtin_fmaf(pu[3], x1, __builtin_fmaf(pu[4], x2, pe1));
+        const float y2 = __builtin_fmaf(pu[5], x2, pe2);
+        const float q = __builtin_fmaf(y0, y0, __builtin_fmaf(y1, y1, y2 * y2));
+        const float tt = pg - q;
+        tj[j] = (q <= 1075.0f && tt >= -1075.0f) ? tt : -INFINITY;
+      }
+    }
+    {
+#pragma clang fp contract(off)
+      float mx = -INFINITY;
+#pragma unroll
+      for (int j = 0; j < KM; j++) mx = fmaxf(mx, tj[j]);
+      if (mx > -INFINITY) {
+        float sum = 0.0f;
+#pragma unroll
+        for (int j = 0; j < KM; j++) sum += __builtin_amdgcn_exp2f(tj[j] - mx);
+        t = (double)vv.w * (double)(mx + __builtin_amdgcn_logf(sum));
+      }
+    }
+  }
+  double cls = 0.0;
+  cls += t;
+  tree[threadIdx.x] = cls * 0.69314718055994530942;
+  __syncthreads();
+  // the pairwise tree of pairwise_tree<256>, one level per step
+  for (int w = 128; w >= 1; w >>= 1) {
+    double a = 0.0;
+    if (threadIdx.x < w) a = tree[2 * threadIdx.x] + tree[2 * threadIdx.x + 1];
+    __syncthreads();
+    if (threadIdx.x < w) tree[threadIdx.x] = a;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) P.part_s[blockIdx.x] = tree[0];
+}
+
 // Frame totals of the log-likelihood from the block partials of k_loglik (B
 // supervoxels per block), one block per frame: the upper levels of each
 // 256-supervoxel tree, then frame_sum_w64 over the frame's groups; the
@@ -1717,6 +1797,12 @@ template <int KM>
 static void launch_loglik(const DevPtrs& P, const KOpts& O, hipStream_t st,
                           hipEvent_t ev_after) {
   const int gw = O.tiles.ll_gw, spg = O.tiles.ll_u;
+  if (gw == 1) {  // one lane per one-voxel supervoxel (voxel mode)
+    k_loglik_voxel<KM><<<nb(P.Sext, 256), 256, 0, st>>>(P, O);
+    if (ev_after) (void)hipEventRecord(ev_after, st);
+    k_ll_final<1><<<P.nframes, 256, 0, st>>>(P);
+    return;
+  }
   const long long B = (256 / gw) * spg;
   const unsigned g = nb(P.Sext, B);
 #define TGMM_LL(GW_, SPG_)                                    \
@@ -1804,7 +1890,7 @@ static hipError_t iteration_t(const DevPtrs& P, const KOpts& O, hipStream_t st, 
 
 hipError_t launch_iteration(const DevPtrs& P, const KOpts& O, hipStream_t st, hipEvent_t e0,
                             hipEvent_t e1, int tk) {
-  if (!tiles_valid(O.tiles)) return hipErrorInvalidValue;
+  if (!tiles_valid(O.tiles) || (O.tiles.ll_gw == 1 && !O.voxel)) return hipErrorInvalidValue;
   switch (kmax_for(O.K)) {
     case 5:
       return iteration_t<5>(P, O, st, e0, e1, tk);

@@ -186,12 +186,17 @@ KOpts make_kopts(const tgmm_em_opts& o, const float* scale_reg) {
   k.emax = o.max_excentricity;
   k.tiles = kDefaultTiles;
   k.voxel = o.voxel_mode ? 1 : 0;
+  if (k.voxel) {  // the log-likelihood's lane-per-voxel form
+    k.tiles.ll_gw = 1;
+    k.tiles.ll_u = 1;
+  }
   return k;
 }
 
 bool tgmm::tiles_valid(const Tiles& t) {
   auto in = [](int v, int a, int b, int c) { return v == a || v == b || v == c; };
-  return in(t.ll_gw, 16, 32, 64) && (in(t.ll_u, 1, 2, 4) || (t.ll_u == 8 && t.ll_gw == 16)) &&
+  return ((in(t.ll_gw, 16, 32, 64) && (in(t.ll_u, 1, 2, 4) || (t.ll_u == 8 && t.ll_gw == 16))) ||
+          (t.ll_gw == 1 && t.ll_u == 1)) &&
          in(t.ms_lanes, 4, 8, 16) &&
          in(t.sv_block, 64, 128, 256) && in(t.knn_block, 64, 128, 256) &&
          (t.knn_chunk == 4 || t.knn_chunk == 8) && in(t.knn_cell, 2, 4, 6) &&
@@ -688,6 +693,10 @@ extern "C" int tgmm_batch_set_tiles(tgmm_batch* b, const tgmm_tiles* t) {
                    x.ll_u, x.ms_lanes, x.sv_block, x.knn_block);
     return TGMM_ERR_ARG;
   }
+  if (x.ll_gw == 1 && !b->ko.voxel) {
+    tgmm_set_error("tgmm_batch_set_tiles: ll_group 1 (a lane per voxel) needs voxel mode");
+    return TGMM_ERR_ARG;
+  }
   HIPCHK(hipSetDevice(b->device));
   HIPCHK(hipStreamSynchronize(b->st));
   b->ko.tiles = x;
@@ -762,8 +771,22 @@ extern "C" int tgmm_batch_autotune(tgmm_batch* b, int reps, tgmm_tiles* chosen, 
     const Knob& kb = knobs[q];
     double tbest = 1e300;
     Tiles pick = best;
-    for (int c = 0; c < kb.n; c++) {
+    const int c0 = (q == 0 && b->ko.voxel) ? -1 : 0;  // voxel mode: the lane-per-voxel form too
+    for (int c = c0; c < kb.n; c++) {
       Tiles t = best;
+      if (c < 0) {
+        t.ll_gw = 1;
+        t.ll_u = 1;
+        double ms = 0.0;
+        if ((rc = time_shape(b, t, kb.tk, iters, e0, e1, &ms)) != TGMM_OK) break;
+        snprintf(line, sizeof(line), "%s 1x1 %.2f\n", kb.name, 1000.0 * ms);
+        rep += line;
+        if (ms < tbest) {
+          tbest = ms;
+          pick = t;
+        }
+        continue;
+      }
       switch (q) {
         case 0: t.ll_gw = kb.v[c][0]; t.ll_u = kb.v[c][1]; break;
         case 1: t.ms_lanes = kb.v[c][0]; break;

@@ -983,7 +983,7 @@ extern "C" int tgmm_server_set_tiles(tgmm_server* s, const tgmm_tiles* t) {
   if (!s || !t) return TGMM_ERR_ARG;
   const Tiles x{t->ll_group,  t->ll_unroll, t->mstep_lanes, t->sv_block,
                 t->knn_block, t->knn_chunk, t->knn_cell,    t->knn_lanes};
-  if (!tiles_valid(x)) {
+  if (!tiles_valid(x) || x.ll_gw == 1) {  // the server runs supervoxel mode only
     tgmm_set_error("tgmm_server_set_tiles: invalid tile shape");
     return TGMM_ERR_ARG;
   }

@@ -592,14 +592,14 @@ class Batch:
         return t.as_dict()
 
     def signature(self):
-        """Workload key for the tile cache: K and log2 buckets of the total
-        supervoxels, Gaussians and voxels per supervoxel."""
+        """Workload key for the tile cache: the mode, K and log2 buckets of
+        the total supervoxels, Gaussians and voxels per supervoxel."""
         S = sum(int(np.asarray(f.sv_ptr).shape[0] - 1) for f in self.frames)
         G = sum(int(np.asarray(f.gm).shape[0]) for f in self.frames)
         N = sum(int(np.asarray(f.img).shape[0]) for f in self.frames)
         b = lambda x: int(round(np.log2(max(x, 1))))
-        return "K%d_S%d_G%d_v%d_f%d" % (self.opts.K, b(S), b(G), b(N / max(S, 1)),
-                                        b(len(self.frames)))
+        return "%sK%d_S%d_G%d_v%d_f%d" % ("vox_" if self.opts.voxel_mode else "", self.opts.K,
+                                          b(S), b(G), b(N / max(S, 1)), b(len(self.frames)))
 
     def autotune(self, reps=3, cache=True):
         """Pick the tile shapes by timing the candidates on this batch
