@@ -87,11 +87,27 @@ def build_devmath_host():
     return so
 
 
+def build_compat_frame():
+    """The C++ test program that runs a frame through the reference's names
+    (include/*_compat.hpp), for tests/test_gpu_compat.py."""
+    src = os.path.join(ROOT, "tests", "native", "compat_frame.cpp")
+    if not os.path.exists(src):
+        return None
+    exe = os.path.join(ROOT, "tests", "native", "compat_frame")
+    tmp = exe + ".tmp"
+    _run(["g++", "-O2", "-std=c++11", "-I", os.path.join(ROOT, "include"), "-o", tmp, src,
+          "-L", HERE, "-ltgmm_em", "-Wl,-rpath,$ORIGIN/../../tgmm_amd",
+          "-Wl,-rpath-link,/opt/rocm/lib"])
+    os.replace(tmp, exe)
+    return exe
+
+
 def build_all():
     build_synth()
     build_oracle()
     build_devmath_host()
     build_em()
+    build_compat_frame()
 
 
 if __name__ == "__main__":
