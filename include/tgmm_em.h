@@ -276,7 +276,7 @@ int tgmm_batch_destroy(tgmm_batch* b);
  * frames join as capacity frees up and leave when done; each stream's
  * posterior Gaussians stay on the device and become the next frame's priors
  * (GaussianMixtureModel::updatePriors, GaussianMixtureModel.cpp:892-910,
- * plus the driver's nu_k*2, W_k/2 step, main.cpp:961-969). */
+ * plus the tracker's nu_k*2, W_k/2 step, main.cpp:961-969). */
 typedef struct tgmm_server tgmm_server;
 
 typedef struct tgmm_server_opts {

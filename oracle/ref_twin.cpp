@@ -1,4 +1,4 @@
-// ref_twin.cpp -- C-ABI driver over the reference's own CPU twin of the
+// ref_twin.cpp -- C-ABI wrapper over the reference's own CPU twin of the
 // per-Gaussian update equations, compiled from the reference sources where
 // they lie (see oracle/Makefile target `ref`). TEST INFRASTRUCTURE ONLY: it
 // pins the oracle's scalar functions against the reference's code:

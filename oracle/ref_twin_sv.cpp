@@ -1,4 +1,4 @@
-// ref_twin_sv.cpp -- C-ABI driver over the reference's own supervoxel
+// ref_twin_sv.cpp -- C-ABI wrapper over the reference's own supervoxel
 // statistics, compiled from the reference sources where they lie (see
 // oracle/Makefile target `ref`). TEST INFRASTRUCTURE ONLY: it pins the
 // oracle's input marshalling against

@@ -4,7 +4,7 @@ GaussianMixtureModel.cpp:892-910, GSL psi.c), and
 tests/golden/sv_centroid_vectors.npz: supervoxels over a uint16 volume with
 the centroid and intensity the reference's
 supervoxel::weightedGaussianStatistics<uint16> (supervoxel.cpp:1410-1449)
-gives them, through the driver built in oracle/_ref (make -C oracle ref).
+gives them, through the wrapper built in oracle/_ref (make -C oracle ref).
 Run in a tree where /root/reference exists:
 
     python tests/golden/make_golden.py             # both files
