@@ -122,3 +122,26 @@ def test_em_from_raw_equals_em_from_marshalled_host_arrays():
     assert np.array_equal(m1, im) and np.array_equal(m2, im)
     assert records_equal(g1, g2)
     assert np.array_equal(i1, i2)
+
+
+def test_batch_from_raw_frames_equals_batch_from_host_arrays():
+    # a batch whose frames arrive raw (marshalled on the device) against one
+    # built from the oracle-marshalled float arrays
+    fa = synth.config1(seed=61, n_cells=40, target_voxels=10000)
+    fb = synth.config1(seed=62, n_cells=25, target_voxels=6000)
+    raws = [raw_from_frame(f, (512, 512, 100), seed=i) for i, f in enumerate((fa, fb))]
+    hosts = []
+    for f, r in zip((fa, fb), raws):
+        q, im, c = O.marshal_frame(*r)
+        hosts.append(synth.Frame(q, im, c, f.sv_ptr, f.gm))
+    opts = em.make_opts(K=5, max_iter=6, tol=-1.0)
+    with em.Batch.from_raw(raws, [fa.gm, fb.gm], opts) as b:
+        b.solve(em.SOLVE_GRAPH)
+        got = b.fetch()
+    with em.Batch(hosts, opts) as b:
+        b.solve(em.SOLVE_GRAPH)
+        want = b.fetch()
+    for g, w in zip(got, want):
+        assert records_equal(g["gm"], w["gm"])
+        assert np.array_equal(g["rnk"], w["rnk"]) and np.array_equal(g["ind"], w["ind"])
+        assert g["ll"] == w["ll"] and g["iterations"] == w["iterations"]

@@ -18,6 +18,7 @@ fallback).
 import ctypes
 import json
 import os
+import types
 
 import numpy as np
 
@@ -555,6 +556,42 @@ class Batch:
         _check(self.L.tgmm_batch_create(device, self.desc, n, ctypes.byref(opts),
                                         ctypes.byref(self.h)))
         self._keep = None  # host copies are no longer needed after create
+
+    @classmethod
+    def from_raw(cls, raw_frames, gms, opts, device=0):
+        """A supervoxel-mode batch whose frames arrive as raw uint16 supervoxel
+        voxels: each frame is built on the device (tgmm_marshal_frame) and
+        the workspace is filled from those device arrays (tgmm_batch_create
+        takes device pointers for query, img and centroids). raw_frames:
+        (pixel_idx, raw, sv_ptr, thr, dims[, volume]) per frame; gms: the
+        frames' Gaussian records."""
+        if opts.voxel_mode:
+            raise TgmmError(-1, "Batch.from_raw builds supervoxel-mode batches")
+        self = cls.__new__(cls)
+        self.L = load()
+        self.opts = opts
+        self.voxel = False
+        self.h = ctypes.c_void_p()
+        n = len(raw_frames)
+        self.desc = (FrameDesc * n)()
+        keep, frames = [], []
+        for i, (rf, gm) in enumerate(zip(raw_frames, gms)):
+            q, im, c, _ = marshal_frame(*rf)
+            ptr = np.ascontiguousarray(rf[2], np.int64)
+            g = as_records(gm)
+            keep += [q, im, c, ptr, g]
+            N, S = int(ptr[-1]), ptr.shape[0] - 1
+            d = self.desc[i]
+            d.n_voxels, d.n_supervoxels, d.n_gaussians = N, S, g.shape[0]
+            d.query, d.img, d.centroids = q.ptr.value, im.ptr.value, c.ptr.value
+            d.sv_ptr, d.gm = ptr.ctypes.data, g.ctypes.data
+            frames.append(types.SimpleNamespace(sv_ptr=ptr, gm=g,
+                                                img=np.broadcast_to(np.float32(0), (N,))))
+        self.frames = frames
+        _check(self.L.tgmm_batch_create(device, self.desc, n, ctypes.byref(opts),
+                                        ctypes.byref(self.h)))
+        self._keep = None  # the device inputs were copied into the workspace
+        return self
 
     def reset(self):
         _check(self.L.tgmm_batch_reset(self.h))
