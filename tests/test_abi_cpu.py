@@ -95,6 +95,53 @@ def test_local_likelihood_argument_errors_are_status_codes():
     assert np.all(out == 5.0)
 
 
+def test_marshal_and_knn_argument_errors_are_status_codes():
+    """Host-side checks of tgmm_marshal_frame, tgmm_GMEMinitializeMemoryFromRaw
+    and tgmm_knnCUDA_ that return before any device call."""
+    L = em.load()
+
+    def frame(N, ptr, dims=(4, 3, 2)):
+        idx = np.arange(max(N, 1), dtype=np.uint64)
+        raw = np.ones(max(N, 1), np.uint16)
+        thr = np.zeros(max(len(ptr) - 1, 1), np.uint16)
+        f, keep = em.raw_frame(idx[:N] if N else idx, raw[:N] if N else raw, ptr, thr, dims)
+        f.N = N
+        return f, keep + [idx, raw, thr]
+
+    out = np.zeros(64, np.float32)
+    ptr = out.ctypes.data
+    f, k = frame(3, [0, 2, 3])
+    f.dims[0] = 0
+    assert L.tgmm_marshal_frame(ctypes.byref(f), ptr, ptr, ptr, None) == -1  # dims
+    f, k = frame(3, [0, 2, 4])
+    assert L.tgmm_marshal_frame(ctypes.byref(f), ptr, ptr, ptr, None) == -1  # ptr[S] != N
+    f, k = frame(3, [0, 2, 1, 3])
+    assert L.tgmm_marshal_frame(ctypes.byref(f), ptr, ptr, ptr, None) == -1  # decreasing
+    f, k = frame(30, [0, 30])
+    assert L.tgmm_marshal_frame(ctypes.byref(f), ptr, ptr, ptr, None) == -1  # > 24 voxels
+    assert b"disjoint" in L.tgmm_last_error()
+    f, k = frame(3, [0, 3])
+    f.dims[:] = [1 << 20, 1 << 20, 1 << 14]
+    assert L.tgmm_marshal_frame(ctypes.byref(f), ptr, ptr, ptr, None) == -2  # 2^54 voxels
+    # the drop-in initialiser counts supervoxels in an unsigned short
+    big = np.zeros(70001, np.int64)
+    f, k = frame(0, big)
+    pp = [ctypes.c_void_p() for _ in range(7)]
+    sc = np.ones(3, np.float32)
+    rc = L.tgmm_GMEMinitializeMemoryFromRaw(*[ctypes.byref(x) for x in pp[:2]], sc.ctypes.data,
+                                            *[ctypes.byref(x) for x in pp[2:]], ctypes.byref(f))
+    assert rc == -2 and all(not x.value for x in pp)
+    # supervoxel kNN: K beyond maxKNN; no references fills the distances only
+    q = np.zeros(6, np.float32)
+    ind = np.full(8, 5, np.int32)
+    dist = np.zeros(8, np.float32)
+    assert L.tgmm_knnCUDA_(ind.ctypes.data, dist.ctypes.data, q.ctypes.data, q.ctypes.data, 2,
+                           2, 12, sc.ctypes.data, 0) == -2
+    assert L.tgmm_knnCUDA_(ind.ctypes.data, dist.ctypes.data, q.ctypes.data, q.ctypes.data, 2,
+                           0, 4, sc.ctypes.data, 0) == 0
+    assert np.all(dist == np.float32(1e32)) and np.all(ind == 5)
+
+
 def test_generator_is_deterministic():
     a = synth.config1()
     b = synth.config1()
