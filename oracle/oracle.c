@@ -421,7 +421,10 @@ void orc_estep_voxel(long long N, const float* query, const float* img, int G,
 }
 
 /* GMEM.cu:1286-1455: weights r_sk * img_v over pairs with r > 1e-3; centred
- * second pass in FP64 (Ck = sum w (x - xk)(x - xk)^T, unnormalised). */
+ * second pass in FP64 (Ck = sum w (x - xk)(x - xk)^T, unnormalised). The
+ * reference compares the float r with the double literal (`auxOrig > 1e-3`,
+ * :1350; voxel mode `aux > 1e-3`, :1195), so the comparison is in double: a
+ * pair with r == 1e-3f (0.00100000005) is kept. */
 void orc_mstep(long long N, long long S, const float* query, const float* img,
                const long long* ptr, int G, int K, const int* ind,
                const float* rnk, double* Nk, double* xk, double* Ck) {
@@ -430,7 +433,7 @@ void orc_mstep(long long N, long long S, const float* query, const float* img,
   for (int j = 0; j < K; j++)
     for (long long s = 0; s < S; s++) {
       const long long p = s + (long long)j * S;
-      if (ind[p] >= 0 && rnk[p] > 1e-3f) cnt[ind[p] + 1]++;
+      if (ind[p] >= 0 && (double)rnk[p] > 1e-3) cnt[ind[p] + 1]++;
     }
   for (int k = 0; k < G; k++) cnt[k + 1] += cnt[k];
   long long* list = (long long*)malloc(sizeof(long long) * (size_t)(cnt[G] > 0 ? cnt[G] : 1));
@@ -438,7 +441,7 @@ void orc_mstep(long long N, long long S, const float* query, const float* img,
   for (int j = 0; j < K; j++)
     for (long long s = 0; s < S; s++) {
       const long long p = s + (long long)j * S;
-      if (ind[p] >= 0 && rnk[p] > 1e-3f) list[cnt[ind[p]] + fill[ind[p]]++] = p;
+      if (ind[p] >= 0 && (double)rnk[p] > 1e-3) list[cnt[ind[p]] + fill[ind[p]]++] = p;
     }
 #pragma omp parallel for schedule(dynamic, 16)
   for (int k = 0; k < G; k++) {
@@ -659,11 +662,15 @@ int orc_marshal_frame(long long N, long long S, const unsigned long long* pixel_
  * query the references are scanned in index order; K slots of (distance,
  * index) start at (1e32, -1) and the threshold at 1e32. The scaled squared
  * distance is ((q0 - r0) s0)^2 + ((q1 - r1) s1)^2 + ((q2 - r2) s2)^2 in float,
- * added in that order (the reference's partial-sum early exits give the same
- * decisions, the partial sums being non-decreasing). A reference whose
- * distance is not above the threshold replaces the slot at minPos; then the
- * threshold becomes the largest slot distance and minPos the first slot
- * holding it (findMaxPosition, :43-53). Output slot-major, ind/dist
+ * added in that order, and the reference leaves a reference as soon as the
+ * x term, or the x + y partial sum, is above the threshold (:103-113). For
+ * finite coordinates the partial sums are non-decreasing, so only the full
+ * sum decides; with a NaN coordinate the full sum is NaN, which is never
+ * above the threshold, and the partial sums before it still skip. A
+ * reference that passes all three tests replaces the slot at minPos; then
+ * the threshold becomes the largest slot distance and minPos the first slot
+ * holding it (findMaxPosition, :43-53: a NaN in slot 0 makes the threshold
+ * NaN, so the next reference is inserted there). Output slot-major, ind/dist
  * [K * query_nb]; unfilled slots keep index -1 (the reference leaves them
  * uninitialised) and distance 1e32. ref_nb <= 0: dist set to 1e32, ind left
  * as given (:165-171). dist may be NULL. */
@@ -688,8 +695,10 @@ void orc_sv_knn(int* ind, float* dist, const float* query, const float* ref,
     for (int r = 0; r < ref_nb; r++) {
       float a = (q0 - ref[r]) * scale[0];
       float d = a * a;
+      if (d > thr) continue;
       a = (q1 - ref[r + ref_nb]) * scale[1];
       d += a * a;
+      if (d > thr) continue;
       a = (q2 - ref[r + 2 * ref_nb]) * scale[2];
       d += a * a;
       if (d > thr) continue;

@@ -11,4 +11,5 @@ void dm_regularize(double* W, double nu, int w4dof, const float* scale, double l
   tgmm::regularize_W(W, nu, w4dof, scale, lmin, lmax, emax);
 }
 void dm_inv3(const double* w, double* o) { tgmm::inv3(w, o); }
+int dm_pair_significant(float r) { return tgmm::pair_significant(r) ? 1 : 0; }
 }

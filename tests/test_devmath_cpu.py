@@ -59,3 +59,14 @@ def test_eig3_closed_form_order_and_vectors():
         V = v.reshape(3, 3).T
         assert np.allclose(V.T @ V, np.eye(3), atol=1e-9)
         assert np.allclose(V @ np.diag(d) @ V.T, M, atol=1e-7 * s[2])
+
+
+def test_pair_cut_compares_in_double():
+    """The kernels' M-step pair test (pair_significant) is the reference's
+    float-against-double comparison (GMEM.cu:1350, 1195)."""
+    D.dm_pair_significant.argtypes = [ctypes.c_float]
+    one = np.float32(1e-3)
+    assert D.dm_pair_significant(float(one)) == 1
+    assert D.dm_pair_significant(float(np.nextafter(one, np.float32(0)))) == 0
+    assert D.dm_pair_significant(0.5) == 1 and D.dm_pair_significant(0.0) == 0
+    assert D.dm_pair_significant(float("nan")) == 0

@@ -375,3 +375,180 @@ def test_next_priors_helper_vs_reference_twin_golden():
         assert abs(out["alpha_o"][0] - r[2]) <= 1e-12 * abs(r[2])
         assert np.array_equal(out["m_o"][0], r[3:6])
         assert np.allclose(out["W_o"][0], r[6:12], rtol=1e-12, atol=0)
+
+
+# --------------------------------------------------- E-step closed forms --
+def _spd(rng, lo, hi):
+    Q = rot(rng)
+    return Q @ np.diag(rng.uniform(lo, hi, 3)) @ Q.T
+
+
+def _distinct_gaussians(rng, G, centre):
+    """Gaussians whose beta, nu, alpha and W all differ, so that no
+    per-Gaussian term of the E-step cancels between candidates."""
+    gm = np.zeros(G, GMM_DTYPE)
+    gm["m_k"] = centre + rng.normal(size=(G, 3)) * 2.0
+    gm["m_o"] = gm["m_k"]
+    for k in range(G):
+        gm["W_k"][k] = packed(_spd(rng, 0.03, 0.6))
+    gm["W_o"] = gm["W_k"]
+    gm["beta_k"] = rng.uniform(0.4, 6.0, G)
+    gm["nu_k"] = rng.uniform(3.2, 14.0, G)
+    gm["alpha_k"] = rng.uniform(0.3, 25.0, G)
+    return gm
+
+
+def _bishop_expectations(gm):
+    """PRML eqs. 10.65 and 10.66 (D = 3) with scipy's digamma:
+    E[ln |Lambda_k|] = sum_i psi((nu_k + 1 - i)/2) + 3 ln 2 + ln |W_k|,
+    E[ln pi_k] = psi(alpha_k) - psi(sum_j alpha_j).
+    The reference sums i = 0, 1, 2 (GMEM.cu:569-572, and its CPU code
+    GaussianMixtureModel.h:230-235), where Bishop's i runs 1..3: its digamma
+    arguments are (nu + 1)/2, nu/2, (nu - 1)/2. That is the reference's
+    definition, which the oracle follows."""
+    from scipy.special import digamma
+    eld = np.array([sum(digamma((g["nu_k"] + 1 - i) / 2.0) for i in (0, 1, 2)) +
+                    3 * math.log(2.0) + math.log(np.linalg.det(w_full(g["W_k"])))
+                    for g in gm])
+    elp = digamma(gm["alpha_k"]) - digamma(gm["alpha_k"].sum())
+    return eld, elp
+
+
+def _bishop_log_rho(x, g, eld, elp):
+    """ln rho of one voxel and one Gaussian, PRML eq. 10.46 with eq. 10.64:
+    E[ln pi] + E[ln |Lambda|]/2 - (D/2) ln 2 pi - (D/beta + nu d^T W d)/2.
+    The (D/2) ln 2 pi term is common to all candidates (the reference omits
+    it, GMEM.cu:2281-2298); it cancels in the normalisation."""
+    d = np.asarray(x, np.float64) - g["m_k"]
+    maha = 3.0 / g["beta_k"] + g["nu_k"] * (d @ w_full(g["W_k"]) @ d)
+    return elp + 0.5 * eld - 1.5 * math.log(2 * math.pi) - 0.5 * maha
+
+
+def test_estep_matches_bishop_closed_form_with_distinct_gaussians():
+    """Supervoxels of 1 to 12 voxels, candidates with different beta, nu,
+    alpha and W: r_sk = exp(L_sk - logsumexp_j L_sj) with L_sk the sum over
+    the supervoxel's voxels of ln rho (PRML 10.46-10.49), the expectations
+    from scipy, independent of the oracle's own expectation code."""
+    from scipy.special import logsumexp
+    rng = np.random.default_rng(21)
+    G, S, K = 7, 40, 4
+    sizes = rng.integers(1, 13, S)
+    ptr = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)
+    N = int(ptr[-1])
+    X = 50.0 + rng.normal(size=(N, 3)) * 2.5
+    query = np.ascontiguousarray(X.T.reshape(-1), np.float32)
+    Xf = query.reshape(3, N).T.astype(np.float64)  # the voxels as the oracle sees them
+    gm = _distinct_gaussians(rng, G, 50.0)
+    eld, elp = _bishop_expectations(gm)
+    gm["expectedLogDetCovarianceCUDA"] = eld
+    gm["expectedLogResponsivityCUDA"] = elp
+    ind = np.full((K, S), -1, np.int32)
+    for s in range(S):
+        n_c = K if s % 5 else 2  # some supervoxels with padded slots
+        ind[:n_c, s] = rng.choice(G, n_c, replace=False)
+    ind = ind.reshape(-1)
+    rnk = np.zeros(K * S, np.float32)
+    L.orc_estep(N, S, O.p(query, fp), O.p(ptr, O._lp), G, gm.ctypes.data, K, O.p(ind, ip),
+                O.p(rnk, fp))
+    rnk = rnk.reshape(K, S)
+    indr = ind.reshape(K, S)
+    for s in range(S):
+        js = [j for j in range(K) if indr[j, s] >= 0]
+        Ls = np.array([sum(_bishop_log_rho(Xf[v], gm[indr[j, s]], eld[indr[j, s]],
+                                           elp[indr[j, s]])
+                           for v in range(ptr[s], ptr[s + 1])) for j in js])
+        want = np.exp(Ls - logsumexp(Ls))
+        assert np.allclose(rnk[js, s], want, rtol=1e-6, atol=1e-7), (s, rnk[js, s], want)
+        assert np.all(rnk[[j for j in range(K) if indr[j, s] < 0], s] == 0.0)
+    # the candidates' terms matter: responsibilities are far from uniform
+    assert np.mean(np.max(rnk, 0)) > 0.8
+
+
+def test_oracle_expectations_match_bishop():
+    rng = np.random.default_rng(22)
+    gm = _distinct_gaussians(rng, 9, 0.0)
+    eld, elp = _bishop_expectations(gm)
+    L.orc_expectations(9, gm.ctypes.data, float(gm["alpha_k"].sum()))
+    assert np.allclose(gm["expectedLogDetCovarianceCUDA"], eld, rtol=1e-13, atol=1e-13)
+    assert np.allclose(gm["expectedLogResponsivityCUDA"], elp, rtol=1e-13, atol=1e-13)
+
+
+def _hand_pair():
+    """Two Gaussians with literal parameters and expectations:
+      A: m = (1, 2, 4), W = 0.5 I, nu = 4, beta = 2, E[ln det] = 0.3,
+         E[ln pi] = -0.2
+      B: m = (2, 2, 3), W = diag(1, 0.5, 0.25), nu = 6, beta = 1,
+         E[ln det] = -1.0, E[ln pi] = -1.5"""
+    g = np.zeros(2, GMM_DTYPE)
+    g["m_k"] = [[1, 2, 4], [2, 2, 3]]
+    g["W_k"] = [[0.5, 0, 0, 0.5, 0, 0.5], [1, 0, 0, 0.5, 0, 0.25]]
+    g["nu_k"] = [4, 6]
+    g["beta_k"] = [2, 1]
+    g["alpha_k"] = [1, 1]
+    g["expectedLogDetCovarianceCUDA"] = [0.3, -1.0]
+    g["expectedLogResponsivityCUDA"] = [-0.2, -1.5]
+    return g
+
+
+def test_estep_hand_worked_two_gaussians():
+    """Supervoxel 0: one voxel x = (1, 2, 3).
+      A: d = (0, 0, -1), d^T W d = 0.5, 3/beta + nu 0.5 = 1.5 + 2 = 3.5,
+         L_A = -0.2 + 0.15 - 1.75 = -1.8
+      B: d = (-1, 0, 0), d^T W d = 1, 3/beta + nu 1 = 3 + 6 = 9,
+         L_B = -1.5 - 0.5 - 4.5 = -6.5
+      r_A = 1 / (1 + exp(-4.7)), r_B = 1 - r_A.
+    Supervoxel 1: voxels (1, 2, 3) and (1, 2, 5).
+      A at (1, 2, 5): d = (0, 0, 1), the same term -1.8: L_A = -3.6
+      B at (1, 2, 5): d = (-1, 0, 2), d^T W d = 1 + 0.25 * 4 = 2,
+         3 + 12 = 15, -1.5 - 0.5 - 7.5 = -9.5: L_B = -16
+      r_A = 1 / (1 + exp(-12.4)).
+    Candidate order is B, A in slot order for supervoxel 1."""
+    g = _hand_pair()
+    query = np.array([1, 1, 1, 2, 2, 2, 3, 3, 5], np.float32)  # SoA, N = 3
+    ptr = np.array([0, 1, 3], np.int64)
+    ind = np.array([0, 1, 1, 0], np.int32)  # slot 0: A, B; slot 1: B, A
+    rnk = np.zeros(4, np.float32)
+    L.orc_estep(3, 2, O.p(query, fp), O.p(ptr, O._lp), 2, g.ctypes.data, 2, O.p(ind, ip),
+                O.p(rnk, fp))
+    rA0 = 1.0 / (1.0 + math.exp(-4.7))
+    rA1 = 1.0 / (1.0 + math.exp(-12.4))
+    assert rnk[0] == np.float32(rA0) and rnk[2] == np.float32(1.0 - rA0)
+    assert rnk[3] == np.float32(rA1) and rnk[1] == np.float32(1.0 - rA1)
+
+
+def test_estep_voxel_hand_worked_two_gaussians():
+    """Voxel mode (GMEM.cu:2177-2242) on the voxels of the hand-worked case:
+    the scores are the per-voxel terms above; the stored value is r times the
+    intensity. Voxel (1, 2, 5): L_A = -1.8, L_B = -9.5. A third candidate far
+    away (score below -100) is cut and takes no share."""
+    g = np.concatenate([_hand_pair(), _hand_pair()[:1]]).astype(GMM_DTYPE)
+    g["m_k"][2] = [40, 2, 4]  # d = (-39, 0, -1): 3/2 + 4 * 0.5 * 1522 > 200
+    query = np.array([1, 1, 2, 2, 3, 5], np.float32)  # N = 2
+    img = np.array([0.5, 2.0], np.float32)
+    ind = np.array([0, 0, 1, 1, 2, 2], np.int32)
+    rnk = np.zeros(6, np.float32)
+    L.orc_estep_voxel(2, O.p(query, fp), O.p(img, fp), 3, g.ctypes.data, 3, O.p(ind, ip),
+                      O.p(rnk, fp))
+    r0 = 1.0 / (1.0 + math.exp(-4.7))
+    r1 = 1.0 / (1.0 + math.exp(-7.7))
+    assert rnk[0] == np.float32(0.5 * r0) and rnk[2] == np.float32(0.5 * (1 - r0))
+    assert rnk[1] == np.float32(2.0 * r1) and rnk[3] == np.float32(2.0 * (1 - r1))
+    assert rnk[4] == 0.0 and rnk[5] == 0.0
+
+
+def test_mstep_pair_cut_compares_in_double():
+    """The reference tests the float r against the double 1e-3 (GMEM.cu:1350,
+    1195): r == 1e-3f (0.00100000005) enters the M-step, the float below it
+    does not."""
+    rng = np.random.default_rng(23)
+    N, S, query, img, cent, ptr, gm = tiny_frame(rng, n_sv=2, G=1)
+    ind = np.zeros(S, np.int32)
+    for r, kept in ((np.float32(1e-3), True),
+                    (np.nextafter(np.float32(1e-3), np.float32(0)), False)):
+        rnk = np.array([r, 0.5], np.float32)
+        Nk, xk, Ck = np.zeros(1), np.zeros(3), np.zeros(6)
+        L.orc_mstep(N, S, O.p(query, fp), O.p(img, fp), O.p(ptr, O._lp), 1, 1, O.p(ind, ip),
+                    O.p(rnk, fp), O.p(Nk, dp), O.p(xk, dp), O.p(Ck, dp))
+        w1 = 0.5 * img[7:14].astype(np.float64).sum()
+        w0 = float(r) * img[:7].astype(np.float64).sum()
+        assert abs(Nk[0] - (w1 + (w0 if kept else 0.0))) < 1e-12 * w1, (r, Nk[0])

@@ -98,3 +98,97 @@ def test_self_search_finds_itself():
     for q in range(400):
         assert q in ind[:, q]
         assert dist[list(ind[:, q]).index(q), q] == 0.0
+
+
+# ------------------------------------------------------------------ NaN --
+def replay(query, ref, K, scale):
+    """A plain-Python replay of the reference's scan (knnCuda.cu:94-113 with
+    findMaxPosition :43-53), float32 operation by operation: the x term, the
+    x + y partial sum and the full sum are each tested against the threshold
+    (`dist > minDistThr` leaves the reference), then the slot at minPos is
+    replaced and the first slot holding the largest distance becomes minPos
+    (`minDist < distArray[ii]`, so a NaN in slot 0 keeps the threshold NaN)."""
+    f32 = np.float32
+    q = np.asarray(query, f32).reshape(3, -1)
+    r = np.asarray(ref, f32).reshape(3, -1)
+    s = [f32(x) for x in scale]
+    Q, R = q.shape[1], r.shape[1]
+    ind = np.full((K, Q), -1, np.int32)
+    dist = np.full((K, Q), f32(1e32), f32)
+    for i in range(Q):
+        md = [f32(1e32)] * K
+        mi = [-1] * K
+        thr, pos = f32(1e32), 0
+        for ii in range(R):
+            a = (q[0, i] - r[0, ii]) * s[0]
+            d = a * a
+            if d > thr:
+                continue
+            a = (q[1, i] - r[1, ii]) * s[1]
+            d = d + a * a
+            if d > thr:
+                continue
+            a = (q[2, i] - r[2, ii]) * s[2]
+            d = d + a * a
+            if d > thr:
+                continue
+            md[pos], mi[pos] = d, ii
+            thr, pos = md[0], 0
+            for j in range(1, K):
+                if thr < md[j]:
+                    thr, pos = md[j], j
+        ind[:, i] = mi
+        dist[:, i] = md
+    return ind.reshape(-1), dist.reshape(-1)
+
+
+def test_nan_partial_sums_leave_the_reference():
+    # K = 1, query at the origin, threshold 1 after reference 0
+    #   ref 1 (5, 0, NaN): x term 25 > 1, left before the NaN is reached
+    #   ref 2 (0.5, 3, NaN): x + y partial sum 9.25 > 1, left
+    #   ref 3 (0.5, 0.5, NaN): partial sums 0.25, 0.5; the NaN sum is taken
+    ref = np.array([[1, 5, 0.5, 0.5], [0, 0, 3, 0.5], [0, np.nan, np.nan, np.nan]],
+                   np.float32).reshape(-1)
+    for n, want in ((2, 0), (3, 0), (4, 3)):
+        r = ref.reshape(3, 4)[:, :n].reshape(-1)
+        ind, dist = O.sv_knn(ORIGIN, r, 1, (1, 1, 1))
+        assert ind[0] == want, (n, ind)
+        assert np.array_equal(ind, replay(ORIGIN, r, 1, (1, 1, 1))[0])
+
+
+def test_nan_in_slot_zero_takes_the_next_reference():
+    # K = 2: ref 0 (1, 0, NaN) goes to slot 0 with a NaN distance, so the
+    # threshold is NaN (findMaxPosition never moves off slot 0); ref 1, far
+    # away, is then taken into slot 0; the threshold becomes 1e32 (slot 1) and
+    # ref 2 fills slot 1
+    ref = np.array([[1, 100, 2], [0, 0, 0], [np.nan, 0, 0]], np.float32).reshape(-1)
+    ind, dist = O.sv_knn(ORIGIN, ref, 2, (1, 1, 1))
+    assert list(ind) == [1, 2] and list(dist) == [10000, 4]
+    ri, rd = replay(ORIGIN, ref, 2, (1, 1, 1))
+    assert np.array_equal(ind, ri) and np.array_equal(dist, rd)
+
+
+@pytest.mark.parametrize("K", [1, 3, 5])
+def test_nan_centroids_match_the_plain_replay(K):
+    rng = np.random.default_rng(100 + K)
+    q = rng.uniform(0, 20, (3, 40)).astype(np.float32)
+    r = rng.uniform(0, 20, (3, 150)).astype(np.float32)
+    for a, i in ((0, 3), (1, 17), (2, 30), (2, 0)):
+        r[a, i] = np.nan
+    q[1, 5] = np.nan
+    q[2, 9] = np.nan
+    q[0, 11] = np.nan
+    oi, od = O.sv_knn(q.reshape(-1), r.reshape(-1), K, (1, 1, 5))
+    ri, rd = replay(q.reshape(-1), r.reshape(-1), K, (1, 1, 5))
+    assert np.array_equal(oi, ri)
+    assert np.array_equal(od, rd, equal_nan=True)
+
+
+def test_plain_replay_matches_the_oracle_without_nan():
+    rng = np.random.default_rng(7)
+    q = np.round(rng.uniform(0, 12, (3, 30))).astype(np.float32).reshape(-1)
+    r = np.round(rng.uniform(0, 12, (3, 120))).astype(np.float32).reshape(-1)  # ties
+    for K in (1, 4, 11):
+        oi, od = O.sv_knn(q, r, K, (1, 1, 5))
+        ri, rd = replay(q, r, K, (1, 1, 5))
+        assert np.array_equal(oi, ri) and np.array_equal(od, rd)

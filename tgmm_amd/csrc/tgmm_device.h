@@ -34,6 +34,12 @@ __host__ __device__ inline double digamma(double x) {
   return digamma_pos(x);
 }
 
+// Whether a stored responsibility enters the M-step. The reference compares
+// the float value with the double literal 1e-3 (GMEM.cu:1350, and :1195 in
+// voxel mode), so the comparison is in double and r == 1e-3f (0.00100000005)
+// is kept.
+__host__ __device__ inline bool pair_significant(float r) { return (double)r > 1e-3; }
+
 __host__ __device__ inline double det3(const double* w) {
   return w[0] * (w[3] * w[5] - w[4] * w[4]) - w[1] * (w[1] * w[5] - w[2] * w[4]) +
          w[2] * (w[1] * w[4] - w[2] * w[3]);

@@ -1055,7 +1055,7 @@ __global__ __launch_bounds__(256) void k_estep(DevPtrs P, KOpts O) {
     if (j < K) {  // K is uniform: the whole wave runs this
       const float r = ((den > 0.0f || den != den) ? e[j] / den : 0.0f) * wv;
       if (ok) P.rnk[(long long)j * S + s] = r;
-      wave_count(P.seg_cnt, (ok && kk[j] >= 0 && r > 1e-3f) ? kk[j] : -1);
+      wave_count(P.seg_cnt, (ok && kk[j] >= 0 && pair_significant(r)) ? kk[j] : -1);
     }
   }
 }
@@ -1068,7 +1068,7 @@ __global__ __launch_bounds__(256) void k_pair_scatter(DevPtrs P, KOpts O) {
   const long long S = P.Scap;
   for (int j = 0; j < O.K; j++) {
     const int k = ok ? P.ind[(long long)j * S + s] : -1;
-    const bool take = k >= 0 && P.rnk[(long long)j * S + s] > 1e-3f;
+    const bool take = k >= 0 && pair_significant(P.rnk[(long long)j * S + s]);
     const int slot = wave_take(P.seg_cnt, take ? k : -1);
     if (take) P.pairs[P.seg_start[k] + slot] = (int)((s << kPairShift) | j);
   }

@@ -23,8 +23,10 @@
 // distance; a block is skipped only when the bound is above every lane's
 // threshold, i.e. when the scan would insert none of its references. A block
 // holding a NaN coordinate has an unbounded box and is never skipped, nor is
-// any block for a query with a NaN coordinate (a NaN distance is inserted by
-// the reference's `dist > thr` test).
+// any block for a query with a NaN coordinate. The exact test is the
+// reference's, with its early exits after the x term and the x + y partial
+// sum (a NaN full distance is taken only when those partial sums are not
+// above the threshold).
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -81,6 +83,25 @@ __device__ inline float svdist(float q0, float q1, float q2, const float* __rest
   return d;
 }
 
+// The reference's insertion test (knnCuda.cu:103-113): it leaves a reference
+// as soon as the x term, or the x + y partial sum, is above the threshold,
+// and takes it when the full sum is not. For finite coordinates the partial
+// sums are non-decreasing and only the full sum decides; a NaN makes the sums
+// after it NaN (never above the threshold), so the partial sums before it
+// still decide. Returns true with d = the full sum when the reference is
+// taken.
+__device__ inline bool svtake(float q0, float q1, float q2, const float* __restrict__ ref,
+                              int r, int R, float s0, float s1, float s2, float thr, float& d) {
+#pragma clang fp contract(off)
+  float a = (q0 - ref[r]) * s0;
+  const float p1 = a * a;
+  a = (q1 - ref[r + R]) * s1;
+  const float p2 = p1 + a * a;
+  a = (q2 - ref[r + 2 * R]) * s2;
+  d = p2 + a * a;
+  return !(p1 > thr) && !(p2 > thr) && !(d > thr);
+}
+
 // ref is SoA with axis stride Rp = nblk * 64 (padded with zeros past R)
 __global__ __launch_bounds__(kKT) void k_svknn(const float* __restrict__ query, long long Q,
                                                const float* __restrict__ ref, int R, int Rp,
@@ -117,8 +138,9 @@ __global__ __launch_bounds__(kKT) void k_svknn(const float* __restrict__ query, 
     if (!__any(qnan || !(lb > thr))) continue;  // no lane would insert from this block
     const int rb = b * kKB;
     const int n = R - rb < kKB ? R - rb : kKB;
-    // the references not above this lane's threshold at the block's start: a
-    // superset of those the scan inserts (the threshold only falls)
+    // the references whose full distance is not above this lane's threshold
+    // at the block's start: a superset of those the scan inserts (the
+    // threshold only falls, and a NaN distance is marked)
     // (the reference arrays are padded to whole blocks, so the 64 loads of a
     // block are unconditional and batch; the padding's marks are cleared)
     unsigned long long cand = 0ull;
@@ -128,13 +150,14 @@ __global__ __launch_bounds__(kKT) void k_svknn(const float* __restrict__ query, 
       const float d = svdist(q0, q1, q2, ref, rb + u, Rp, s0, s1, s2);
       cand |= (d > t0) ? 0ull : (1ull << u);
     }
-    if (n < kKB) cand &= (1ull << n) - 1ull;
+    const unsigned long long nmask = n < kKB ? (1ull << n) - 1ull : ~0ull;
+    cand &= nmask;
     // the scan over the candidates, in index order, with the current threshold
     while (cand) {
       const int u = __builtin_ctzll(cand);
       cand &= cand - 1ull;
-      const float d = svdist(q0, q1, q2, ref, rb + u, Rp, s0, s1, s2);
-      if (d > thr) continue;
+      float d;
+      if (!svtake(q0, q1, q2, ref, rb + u, Rp, s0, s1, s2, thr, d)) continue;
 #pragma unroll
       for (int j = 0; j < kKM; j++)
         if (j == pos) {
@@ -149,6 +172,10 @@ __global__ __launch_bounds__(kKT) void k_svknn(const float* __restrict__ query, 
           thr = md[j];
           pos = j;
         }
+      // a NaN in slot 0 makes the threshold NaN (findMaxPosition's `<` never
+      // moves off it): the scan then takes the next reference, whatever its
+      // distance, so the rest of the block is walked
+      if (thr != thr && u + 1 < kKB) cand |= (~0ull << (u + 1)) & nmask;
     }
   }
   if (!live) return;
