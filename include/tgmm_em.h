@@ -103,13 +103,16 @@ int tgmm_setRegularizePrecisionMatrixConstants(double lambdaMin,
 /* Replaces GMEMvariationalInferenceCUDAWithSupervoxels (GMEMupdateCUDA.h:92-98,
  * GMEMupdateCUDA.cu:3985-4421). Device inputs from the init call; vecGMtemp is
  * host in/out [ref_nb]. On return: posterior and expectation fields updated,
- * supervoxelIdx/supervoxelNum rebuilt from r > 0.5 (<= 16, slot-major scan),
- * rnkCUDA/indCUDA (device, slot-major s + j*S) hold the last iteration with
- * candidates sorted by (distance, index) and unused slots ind = -1, r = 0.
- * rnkCUDAtr may be NULL (unused, as in the reference); indCUDAtr is scratch.
- * debugPath NULL or "" disables the {int S; int K; float rnk[]; int ind[]}
- * dump. This entry point is the plain per-frame form: one iteration per host
- * round trip with the convergence test on the host, as the reference does. */
+ * supervoxelIdx/supervoxelNum rebuilt from r > 0.5 (<= 16, slot-major scan;
+ * entries past the count as given), rnkCUDA/indCUDA (device, slot-major
+ * s + j*S) hold the last iteration in the reference's slot layout (the
+ * outcome of its scan of the Gaussians in index order, knnCuda.cu:61-144)
+ * and unused slots ind = -1, r = 0. rnkCUDAtr may be NULL (unused, as in the
+ * reference); indCUDAtr receives a copy of indCUDA. debugPath NULL or ""
+ * disables the {int S; int K; float rnk[]; int ind[]} dump. The frame runs in
+ * a one-frame workspace kept per device between calls (grown when a frame
+ * does not fit, held until the process ends), its iteration replayed as a
+ * captured hipGraph, convergence polled every four iterations. */
 int tgmm_GMEMvariationalInferenceCUDAWithSupervoxels(
     float* queryCUDA, float* imgDataCUDA, float* rnkCUDA, float* rnkCUDAtr,
     int* indCUDA, int* indCUDAtr, float* centroidLabelPositionCUDA,
@@ -166,6 +169,35 @@ int tgmm_GMEMreleaseMemoryWithSupervoxels(float** queryCUDA,
                                           int** indCUDA, int** indCUDAtr,
                                           float** centroidLabelPositionCUDA,
                                           long long** labelListPtrCUDA);
+
+/* Replaces GMEMinitializeMemory (GMEMupdateCUDA.h:129-131,
+ * GMEMupdateCUDA.cu:4538-4579), the voxel-mode initialiser: allocates
+ * query[3N] f32 SoA, img[N], rnk[5N], ind[5N], indTr[5N] on the current
+ * device, copies the host arrays, and records `scale` as the kNN anisotropy
+ * (uploadScaleCUDA). */
+int tgmm_GMEMinitializeMemory(float** queryCUDA, const float* query, float** imgDataCUDA,
+                              const float* imgData, const float* scale, long long query_nb,
+                              float** rnkCUDA, int** indCUDA, int** indCUDAtr);
+
+/* Replaces GMEMreleaseMemory (GMEMupdateCUDA.h:138-139,
+ * GMEMupdateCUDA.cu:4637-4653): frees and nulls every pointer. */
+int tgmm_GMEMreleaseMemory(float** queryCUDA, float** imgDataCUDA, float** rnkCUDA,
+                           int** indCUDA, int** indCUDAtr);
+
+/* Replaces selectCUDAdeviceMaxMemory (UtilsCUDA/knnCuda.h:23,
+ * knnCuda.cu:27-47), called by the tracker at start-up (main.cpp:207): the
+ * visible device with the most global memory (the first of equals), or -1
+ * when there is none. */
+int tgmm_selectCUDAdeviceMaxMemory(int* device);
+
+/* Replaces setDeviceCUDA (UtilsCUDA/knnCuda.h:20, knnCuda.cu:737-741), called
+ * by the tracker before each frame (main.cpp:778): makes devCUDA current. */
+int tgmm_setDeviceCUDA(int devCUDA);
+
+/* Replaces uploadScaleCUDA (UtilsCUDA/knnCuda.h:21, knnCuda.cu:827-830): the
+ * kNN anisotropy used by the VI entry points (the init entry points also set
+ * it, as the reference's do). */
+int tgmm_uploadScaleCUDA(const float* scale);
 
 /* ---------------------------------------------------------------------------
  * 2. Widened API: batches of frames, graphs, serving
@@ -228,8 +260,23 @@ int tgmm_batch_reset(tgmm_batch* b);
 /* Runs EM on all frames; stream-ordered, returns after the last iteration has
  * been enqueued and completed (synchronises the batch stream). */
 int tgmm_batch_solve(tgmm_batch* b, int mode);
-/* Copies results into frames[i].gm/rnk/ind/ll/iterations (host). */
+/* Copies results into frames[i].gm/rnk/ind/ll/iterations (host); gm, rnk
+ * and ind may each be NULL. The records are formed on the device (state,
+ * fixed flag, the r > 0.5 supervoxel assignment; supervoxelIdx entries past
+ * supervoxelNum as given) and copied in one transfer per frame; rnk/ind are
+ * in the reference's slot layout with frame-local indices. */
 int tgmm_batch_fetch(tgmm_batch* b, tgmm_frame_desc* frames, int nframes);
+
+/* Diagnostics of the last solve: supervoxels whose K-th kNN distance was
+ * tied in the last iteration (searched again in the reference's scan order),
+ * supervoxels whose final slot layout found a different candidate set than
+ * the iterations' search (0 unless something is wrong), workspace bytes. */
+typedef struct tgmm_batch_stats_t {
+  int knn_ties_last_iteration;
+  int layout_mismatches;
+  long long workspace_bytes;
+} tgmm_batch_stats_t;
+int tgmm_batch_stats(tgmm_batch* b, tgmm_batch_stats_t* stats);
 /* Launch shapes of the tunable kernels. Every valid choice gives bitwise the
  * same results (the kernels fix their summation order independently of the
  * shape); only the speed changes. Defaults: {16, 4, 16, 256, 256, 8, 6, 2}. */

@@ -5,7 +5,8 @@
  * __graft_entry__.smoke() and by bench.py's cpu_baseline leg; never by the
  * product library. Written from the reference's equations, independently of
  * the HIP code:
- *   kNN            knnCuda.cu:61-144 (distance order), GMEM.cu:527-539
+ *   kNN            knnCuda.cu:61-144 (the scan in index order, early exits,
+ *                  slot replacement rule), GMEM.cu:527-539
  *   Sigma alpha    GMEM.cu:579-604, 2765-2799
  *   expectations   GMEM.cu:553-628 (K7, K8)
  *   E-step         GMEM.cu:2244-2336 (K9), per voxel, FP64
@@ -33,9 +34,8 @@
  *   no supervoxel assignment (the records keep supervoxelIdx/Num)
  *
  * Declared deviations from the reference (identical in the HIP code):
- *   - kNN candidates are the K smallest (distance, index) pairs, output sorted
- *     ascending; fewer than K finite candidates are padded with index -1 and
- *     r = 0 (the reference leaves such slots uninitialised, knnCuda.cu:74).
+ *   - kNN slots never filled (fewer than K living Gaussians) hold index -1
+ *     and r = 0 (the reference leaves them uninitialised, knnCuda.cu:74).
  *   - E-step normalisation is max-subtracted log-sum-exp (same value as
  *     1/sum_j exp(L_j - L_k), GMEM.cu:2327-2335, without overflow).
  *   - M-step statistics, N_k and the posterior are FP64 (the reference
@@ -271,42 +271,77 @@ void orc_posterior(GaussianMixtureModelCUDA* g, double Nk, const double xk[3],
 }
 
 /* ------------------------------------------------------------------------ */
-static int lex_less(float da, int ia, float db, int ib) {
-  return da < db || (da == db && ia < ib);
-}
-
-/* knnCuda.cu:104-130 distance order: ((c-m)*s)^2 summed x, y, z in FP32 */
+/* The EM's kNN, knnKernel / knnKernelNoConstantMemory (knnCuda.cu:61-144,
+ * 147-246) on the float means copied out of the records (GMEM.cu:527-539),
+ * replayed per query: the Gaussians are scanned in index order; K slots of
+ * (distance, index) start at 1e32 and the threshold at 1e32; the distance is
+ * ((c0 - m0) s0)^2 + ((c1 - m1) s1)^2 + ((c2 - m2) s2)^2 in float, added in
+ * that order, and the reference leaves a Gaussian as soon as the x term, the
+ * x + y partial sum or the full sum is above the threshold (:116-124); a
+ * Gaussian that passes replaces the slot at minPos, then the threshold
+ * becomes the largest slot distance and minPos the first slot holding it
+ * (findMaxPosition, :49-59). A dead Gaussian (m = -1e32) has an infinite
+ * distance and is never taken. Output slot-major ind [K * S] in the
+ * reference's slot layout; slots never filled (fewer than K living
+ * Gaussians) hold -1 (the reference leaves them uninitialised, :74).
+ *
+ * Each block of 256 Gaussians first forms every full distance without
+ * branches; the scan then applies the exact test only where the full
+ * distance is not above the current threshold or is NaN (for finite values
+ * the partial sums are non-decreasing, so no other Gaussian can pass). */
 void orc_knn(long long S, const float* cent, int G,
              const GaussianMixtureModelCUDA* gm, const float scale[3], int K,
              int* ind) {
   float* mf = (float*)malloc(sizeof(float) * 3 * (size_t)(G > 0 ? G : 1));
-  for (int k = 0; k < G; k++)
-    for (int i = 0; i < 3; i++) mf[3 * k + i] = (float)gm[k].m_k[i];
+  for (int i = 0; i < 3; i++)
+    for (int k = 0; k < G; k++) mf[(size_t)i * G + k] = (float)gm[k].m_k[i];
+  const float s0 = scale[0], s1 = scale[1], s2 = scale[2];
 #pragma omp parallel for schedule(dynamic, 64)
   for (long long s = 0; s < S; s++) {
-    float bd[TGMM_KMAX];
-    int bi[TGMM_KMAX];
-    int n = 0;
-    const float q0 = cent[s], q1 = cent[s + S], q2 = cent[s + 2 * S];
-    for (int k = 0; k < G; k++) {
-      const float a = (q0 - mf[3 * k]) * scale[0];
-      float dist = a * a;
-      const float b = (q1 - mf[3 * k + 1]) * scale[1];
-      dist += b * b;
-      const float e = (q2 - mf[3 * k + 2]) * scale[2];
-      dist += e * e;
-      if (!(dist <= 1e32f)) continue; /* dead Gaussians give inf */
-      if (n == K && !lex_less(dist, k, bd[K - 1], bi[K - 1])) continue;
-      int p = (n < K) ? n++ : K - 1;
-      while (p > 0 && lex_less(dist, k, bd[p - 1], bi[p - 1])) {
-        bd[p] = bd[p - 1];
-        bi[p] = bi[p - 1];
-        p--;
-      }
-      bd[p] = dist;
-      bi[p] = k;
+    float md[TGMM_KMAX];
+    int mi[TGMM_KMAX];
+    float d[256];
+    for (int j = 0; j < K; j++) {
+      md[j] = 1e32f;
+      mi[j] = -1;
     }
-    for (int j = 0; j < K; j++) ind[s + (long long)j * S] = (j < n) ? bi[j] : -1;
+    float thr = 1e32f;
+    int pos = 0;
+    const float q0 = cent[s], q1 = cent[s + S], q2 = cent[s + 2 * S];
+    for (int b0 = 0; b0 < G; b0 += 256) {
+      const int n = G - b0 < 256 ? G - b0 : 256;
+      for (int u = 0; u < n; u++) {
+        const float a = (q0 - mf[b0 + u]) * s0;
+        const float b = (q1 - mf[(size_t)G + b0 + u]) * s1;
+        const float e = (q2 - mf[2 * (size_t)G + b0 + u]) * s2;
+        float t = a * a;
+        t = t + b * b;
+        d[u] = t + e * e;
+      }
+      for (int u = 0; u < n; u++) {
+        if (d[u] > thr) continue; /* finite: the partial sums are not above it either */
+        const int k = b0 + u;
+        float a = (q0 - mf[k]) * s0;
+        float dist = a * a;
+        if (dist > thr) continue;
+        a = (q1 - mf[(size_t)G + k]) * s1;
+        dist += a * a;
+        if (dist > thr) continue;
+        a = (q2 - mf[2 * (size_t)G + k]) * s2;
+        dist += a * a;
+        if (dist > thr) continue;
+        md[pos] = dist;
+        mi[pos] = k;
+        thr = md[0];
+        pos = 0;
+        for (int j = 1; j < K; j++)
+          if (thr < md[j]) {
+            thr = md[j];
+            pos = j;
+          }
+      }
+    }
+    for (int j = 0; j < K; j++) ind[s + (long long)j * S] = mi[j];
   }
   free(mf);
 }

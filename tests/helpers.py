@@ -14,12 +14,17 @@ def rel_err(a, b, floor=0.0):
 
 
 def record_errors(gm, ref):
-    """Max relative errors over live Gaussians: means (per component, relative
-    to |m|), precision (Frobenius), nu, beta, alpha."""
-    live = ~is_dead(ref)
+    """Max relative errors over the Gaussians alive on either side (a
+    Gaussian one side killed and the other kept compares its -1e32 mean and
+    fails): means (per component, relative to |m|), precision (Frobenius),
+    nu, beta, alpha, and the expectations E[ln det Lambda] ("eld") and
+    E[ln pi] ("elp"), relative with a floor of 1 (they are logarithms near 0),
+    plus "dead", the number of Gaussians whose dead/alive state differs."""
+    dead_a, dead_b = is_dead(gm), is_dead(ref)
+    live = ~dead_a | ~dead_b
     out = {}
     if not live.any():
-        return dict(m=0.0, W=0.0, nu=0.0, beta=0.0, alpha=0.0)
+        return dict(m=0.0, W=0.0, nu=0.0, beta=0.0, alpha=0.0, eld=0.0, elp=0.0, dead=0)
     a, b = gm[live], ref[live]
     out["m"] = float(np.max(np.linalg.norm(a["m_k"] - b["m_k"], axis=1) /
                             np.linalg.norm(b["m_k"], axis=1)))
@@ -27,6 +32,11 @@ def record_errors(gm, ref):
     out["W"] = float(np.max(dw / np.linalg.norm(b["W_k"], axis=1)))
     for f in ("nu_k", "beta_k", "alpha_k"):
         out[f[:-2]] = rel_err(a[f], b[f], floor=1e-300)
+    out["eld"] = rel_err(a["expectedLogDetCovarianceCUDA"], b["expectedLogDetCovarianceCUDA"],
+                         floor=1.0)
+    out["elp"] = rel_err(a["expectedLogResponsivityCUDA"], b["expectedLogResponsivityCUDA"],
+                         floor=1.0)
+    out["dead"] = int(np.sum(dead_a != dead_b))
     return out
 
 
@@ -122,3 +132,50 @@ def raw_from_frame(frame, dims, seed=0, bright=3000):
     per = np.repeat(thr.astype(np.int64), np.diff(frame.sv_ptr))
     raw = (per + np.round(frame.img.astype(np.float64) * bright)).astype(np.uint16)
     return idx, raw, frame.sv_ptr.astype(np.int64), thr, d
+
+
+def scan_replay(query, ref, K, scale):
+    """A plain-Python replay of the reference's nearest-neighbour scan (the
+    EM's knnKernel, UtilsCUDA/knnCuda.cu:61-144, and knnCUDA_'s kernel,
+    temporalLogicalRules/knnCUDA/knnCuda.cu:56-140; findMaxPosition is the
+    same in both), float32 operation by operation, for small inputs. query and
+    ref are SoA (x.., y.., z..). The references are scanned in index order;
+    the x term, the x + y partial sum and the full sum are each tested against
+    the threshold (`dist > minDistThr` leaves the reference); a reference that
+    passes replaces the slot at minPos, then the first slot holding the
+    largest distance becomes minPos (`minDist < distArray[ii]`, so a NaN in
+    slot 0 keeps the threshold NaN). Returns slot-major (ind, dist) [K * Q];
+    slots never filled keep (-1, 1e32)."""
+    f32 = np.float32
+    q = np.asarray(query, f32).reshape(3, -1)
+    r = np.asarray(ref, f32).reshape(3, -1)
+    s = [f32(x) for x in scale]
+    Q, R = q.shape[1], r.shape[1]
+    ind = np.full((K, Q), -1, np.int32)
+    dist = np.full((K, Q), f32(1e32), f32)
+    with np.errstate(over="ignore", invalid="ignore"):
+        for i in range(Q):
+            md = [f32(1e32)] * K
+            mi = [-1] * K
+            thr, pos = f32(1e32), 0
+            for ii in range(R):
+                a = (q[0, i] - r[0, ii]) * s[0]
+                d = a * a
+                if d > thr:
+                    continue
+                a = (q[1, i] - r[1, ii]) * s[1]
+                d = d + a * a
+                if d > thr:
+                    continue
+                a = (q[2, i] - r[2, ii]) * s[2]
+                d = d + a * a
+                if d > thr:
+                    continue
+                md[pos], mi[pos] = d, ii
+                thr, pos = md[0], 0
+                for j in range(1, K):
+                    if thr < md[j]:
+                        thr, pos = md[j], j
+            ind[:, i] = mi
+            dist[:, i] = md
+    return ind.reshape(-1), dist.reshape(-1)

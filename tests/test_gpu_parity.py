@@ -208,3 +208,72 @@ def test_repeatable_and_order_independent(c1):
     for a, b in zip([runs[0][2], runs[0][0], runs[0][1]], perm):
         assert a["gm"].tobytes() == b["gm"].tobytes()
         assert a["ll"] == b["ll"]
+
+
+# ------------------------------------------------ reference slot layout --
+def _solve(frames, opts):
+    with em.Batch(frames, opts) as b:
+        b.solve(em.SOLVE_GRAPH)
+        return b.fetch(), b.stats()
+
+
+def _lattice(c1, step=3.0):
+    """Integer centroids and Gaussian means on a lattice: many Gaussians at
+    exactly the K-th distance of a supervoxel (ties the reference's slot rule
+    resolves by position)."""
+    f = copy.copy(c1)
+    f.centroids = np.round(c1.centroids).astype(np.float32)
+    g = c1.gm.copy()
+    g["m_k"] = np.round(g["m_k"] / step) * step
+    g["m_o"] = g["m_k"]
+    f.gm = np.concatenate([g, g.copy()]).astype(GMM_DTYPE)  # every mean twice
+    return f
+
+
+@pytest.mark.parametrize("K", [5, 8])
+def test_knn_ties_take_the_reference_scan_set(c1, K):
+    """With exact ties at the K-th distance the candidate set is the one the
+    reference's scan keeps (not the lower indices): the tied supervoxels are
+    searched again in scan order, and the slots equal the oracle's replay."""
+    f = _lattice(c1)
+    out, st = _solve([f], _opts(K=K, iters=1))
+    assert st["knn_ties_last_iteration"] > 0  # the tie path ran
+    assert st["layout_mismatches"] == 0
+    assert np.array_equal(out[0]["ind"], _brute_knn(f, f.gm, K))
+
+
+@pytest.mark.parametrize("K", [5, 8])
+def test_final_slot_layout_after_several_iterations(c1, K):
+    """After n iterations the slots hold the last search's candidates in the
+    reference's layout: the oracle's scan over the means iteration n started
+    from (the final means of an (n-1)-iteration run)."""
+    f = _dense(c1)
+    for n in (2, 5):
+        prev, _ = _solve([f], _opts(K=K, iters=n - 1))
+        cur, st = _solve([f], _opts(K=K, iters=n))
+        assert st["layout_mismatches"] == 0
+        assert np.array_equal(cur[0]["ind"],
+                              _brute_knn(f, np.ascontiguousarray(prev[0]["gm"]), K))
+
+
+@pytest.mark.parametrize("max_sv", [2, 4])
+def test_capped_supervoxel_lists_match_oracle(max_sv):
+    """Gaussians that collect more than 16 supervoxels keep the first 16 of
+    the reference's slot-major scan (GMEM.cu:4328-4350): the device's lists
+    equal the oracle's rule applied to the device's own rnk/ind, and the full
+    oracle EM's lists."""
+    f = soft_frame(max_sv)
+    out, st = _solve([f], _opts(iters=10))
+    o = out[0]
+    assert st["layout_mismatches"] == 0
+    g = f.gm.copy()
+    O.oracle().orc_assign_supervoxels(f.S, f.G, g.ctypes.data, 5, O.p(o["rnk"], O._fp),
+                                      O.p(o["ind"], O._ip), 16)
+    assert np.array_equal(o["gm"]["supervoxelNum"], g["supervoxelNum"])
+    assert np.array_equal(o["gm"]["supervoxelIdx"], g["supervoxelIdx"])
+    gm_o, rnk_o, ind_o, hist, it, ll = O.em_run(f, _oopts(iters=10))
+    capped = gm_o["supervoxelNum"] == 16
+    print("capped Gaussians: %d" % capped.sum(), flush=True)
+    assert capped.sum() > 20
+    assert np.array_equal(o["gm"]["supervoxelNum"], gm_o["supervoxelNum"])
+    assert np.array_equal(o["gm"]["supervoxelIdx"], gm_o["supervoxelIdx"])

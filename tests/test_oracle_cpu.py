@@ -8,9 +8,12 @@ Pins, per oracle function:
   orc_regularize_W    reference CPU twin (GaussianMixtureModel.h:283-368)
   orc_posterior       reference CPU twin (GaussianMixtureModel.h:252-272)
   orc_expectations    reference CPU twin (GaussianMixtureModel.h:229-243)
-  orc_knn             numpy brute force with the same FP32 operation order
-  orc_estep           invariants (rows sum to 1, symmetric ties, single
-                      candidate), dead-free padding
+  orc_knn             a plain-Python replay of the reference's scan (ties,
+                      dead Gaussians), the K smallest distances (numpy),
+                      the supervoxel scan, hand-worked slot layouts
+  orc_estep           Bishop PRML 10.46-10.49 closed form with distinct
+                      Gaussians (scipy digamma), hand-worked two-Gaussian
+                      cases, invariants
   orc_mstep           numpy weighted mean / scatter
   orc_loglik          scipy multivariate normal log-density
   orc_death, orc_converged, orc_assign_supervoxels  rule-by-rule cases
@@ -22,7 +25,7 @@ import numpy as np
 import pytest
 
 import oracle_lib as O
-from helpers import rel_err
+from helpers import rel_err, scan_replay
 from tgmm_amd import synth
 from tgmm_amd.records import GMM_DTYPE, w_full
 
@@ -185,42 +188,94 @@ def test_expectations_vs_reference_twin_golden():
 
 
 # -------------------------------------------------------------------- kNN --
-@np.errstate(over="ignore", invalid="ignore")
-def numpy_knn(cent, S, means_f32, alive, scale, K):
-    c = cent.reshape(3, S).T.astype(np.float32)
-    m = means_f32.astype(np.float32)
-    a = (c[:, None, 0] - m[None, :, 0]) * np.float32(scale[0])
-    d = a * a
-    b = (c[:, None, 1] - m[None, :, 1]) * np.float32(scale[1])
-    d = d + b * b
-    e = (c[:, None, 2] - m[None, :, 2]) * np.float32(scale[2])
-    d = d + e * e
-    d = np.where(alive[None, :] & (d <= np.float32(1e32)), d, np.inf).astype(np.float32)
-    out = np.full((K, S), -1, np.int32)
-    idx = np.arange(m.shape[0])
-    for s in range(S):
-        order = np.lexsort((idx, d[s]))
-        take = [k for k in order[:K] if np.isfinite(d[s, k])]
-        out[: len(take), s] = take
-    return out.reshape(-1)
+def _knn(cent, S, gm, K, scale=(1, 1, 5)):
+    ind = np.zeros(K * S, np.int32)
+    sc = np.asarray(scale, np.float32)
+    L.orc_knn(S, O.p(np.ascontiguousarray(cent, np.float32), fp), gm.shape[0], gm.ctypes.data,
+              O.p(sc, fp), K, O.p(ind, ip))
+    return ind
 
 
-@pytest.mark.parametrize("K,G", [(5, 200), (8, 60), (5, 3)])
-def test_knn_vs_numpy_bruteforce_with_ties(K, G):
+def _means_soa(gm):
+    """The records' means as the reference's float copy (GMEM.cu:527-539)."""
+    return np.ascontiguousarray(gm["m_k"].astype(np.float32).T.reshape(-1))
+
+
+@pytest.mark.parametrize("K,G", [(5, 200), (8, 60), (5, 3), (16, 40)])
+def test_knn_matches_the_plain_scan_replay_with_ties_and_dead(K, G):
+    """The oracle's kNN equals a plain-Python replay of knnCuda.cu's scan,
+    slot for slot, on integer coordinates (many exact ties at the threshold)
+    with dead Gaussians (m = -1e32, an infinite distance) among them."""
     rng = np.random.default_rng(K * 100 + G)
-    S = 300
+    S = 120
     cent = rng.integers(0, 40, size=3 * S).astype(np.float32)
     gm = np.zeros(G, GMM_DTYPE)
-    gm["m_k"] = rng.integers(0, 40, size=(G, 3)).astype(np.float64)  # many exact ties
+    gm["m_k"] = rng.integers(0, 40, size=(G, 3)).astype(np.float64)
     gm["m_o"] = gm["m_k"]
     dead = rng.random(G) < 0.1
     gm["m_o"][dead] = -1e32
     gm["m_k"][dead] = -1e32
-    scale = np.array([1, 1, 5], np.float32)
-    ind = np.zeros(K * S, np.int32)
-    L.orc_knn(S, O.p(cent, fp), G, gm.ctypes.data, O.p(scale, fp), K, O.p(ind, ip))
-    ref = numpy_knn(cent, S, gm["m_k"], ~dead, scale, K)
+    ind = _knn(cent, S, gm, K)
+    ref, _ = scan_replay(cent, _means_soa(gm), K, (1, 1, 5))
     assert np.array_equal(ind, ref)
+    assert not np.isin(ind, np.flatnonzero(dead)).any()
+
+
+@pytest.mark.parametrize("K", [1, 5, 8])
+def test_knn_slots_hold_the_k_smallest_distances(K):
+    """Whatever the slot layout, the filled slots hold the K smallest float
+    distances (numpy brute force, the reference's operation order)."""
+    rng = np.random.default_rng(40 + K)
+    S, G = 200, 150
+    cent = rng.uniform(0, 60, 3 * S).astype(np.float32)
+    gm = np.zeros(G, GMM_DTYPE)
+    gm["m_k"] = rng.uniform(0, 60, (G, 3))
+    ind = _knn(cent, S, gm, K).reshape(K, S)
+    c = cent.reshape(3, S).T
+    m = gm["m_k"].astype(np.float32)
+    sc = np.array([1, 1, 5], np.float32)
+    d = np.zeros((S, G), np.float32)
+    for a in range(3):
+        t = (c[:, None, a] - m[None, :, a]) * sc[a]
+        d = d + t * t
+    for s in range(S):
+        assert np.array_equal(np.sort(d[s, ind[:, s]]), np.sort(d[s])[:K])
+
+
+def test_knn_equals_the_supervoxel_scan_on_the_same_points():
+    """The EM's scan (knnCuda.cu:61-144) and knnCUDA_'s (the other knnCuda.cu)
+    are the same rule: on living means they give the same slots."""
+    rng = np.random.default_rng(44)
+    S, G = 300, 500
+    cent = np.round(rng.uniform(0, 30, 3 * S)).astype(np.float32)
+    gm = np.zeros(G, GMM_DTYPE)
+    gm["m_k"] = np.round(rng.uniform(0, 30, (G, 3)))
+    for K in (1, 5, 11):
+        a = _knn(cent, S, gm, K)
+        b, _ = O.sv_knn(cent, _means_soa(gm), K, (1, 1, 5))
+        assert np.array_equal(a, b)
+
+
+def test_knn_slot_layout_by_hand():
+    """Query at the origin, means on the x axis (scale 1):
+      distances 25, 16, 9, 4, 1 with K = 3: slots (25,0) (16,1) (9,2); 4
+      replaces the largest (slot 0), then 1 replaces 16 (slot 1): [3, 4, 2];
+      distances 4, 4, 4, 4 with K = 2: (0, 1), then each later one replaces
+      slot 0 (the first slot holding the threshold): [3, 1];
+      a dead Gaussian between them is skipped; fewer living Gaussians than
+      slots leave -1."""
+    def line(xs, K, dead=()):
+        gm = np.zeros(len(xs), GMM_DTYPE)
+        gm["m_k"][:, 0] = xs
+        for k in dead:
+            gm["m_k"][k] = -1e32
+            gm["m_o"][k] = -1e32
+        return list(_knn(np.zeros(3, np.float32), 1, gm, K, (1, 1, 1)))
+    assert line([5, 4, 3, 2, 1], 3) == [3, 4, 2]
+    assert line([2, -2, 2, -2], 2) == [3, 1]
+    assert line([5, 0, 4, 3], 2, dead=(1,)) == [3, 2]
+    assert line([3, 1], 4) == [0, 1, -1, -1]
+    assert line([3, 1, 7], 4, dead=(0, 2)) == [1, -1, -1, -1]
 
 
 # ------------------------------------------------------------- E / M / LL --

@@ -16,6 +16,7 @@ import numpy as np
 import pytest
 
 import oracle_lib as O
+from helpers import scan_replay
 
 
 def np_dist(query, ref, scale):
@@ -101,45 +102,7 @@ def test_self_search_finds_itself():
 
 
 # ------------------------------------------------------------------ NaN --
-def replay(query, ref, K, scale):
-    """A plain-Python replay of the reference's scan (knnCuda.cu:94-113 with
-    findMaxPosition :43-53), float32 operation by operation: the x term, the
-    x + y partial sum and the full sum are each tested against the threshold
-    (`dist > minDistThr` leaves the reference), then the slot at minPos is
-    replaced and the first slot holding the largest distance becomes minPos
-    (`minDist < distArray[ii]`, so a NaN in slot 0 keeps the threshold NaN)."""
-    f32 = np.float32
-    q = np.asarray(query, f32).reshape(3, -1)
-    r = np.asarray(ref, f32).reshape(3, -1)
-    s = [f32(x) for x in scale]
-    Q, R = q.shape[1], r.shape[1]
-    ind = np.full((K, Q), -1, np.int32)
-    dist = np.full((K, Q), f32(1e32), f32)
-    for i in range(Q):
-        md = [f32(1e32)] * K
-        mi = [-1] * K
-        thr, pos = f32(1e32), 0
-        for ii in range(R):
-            a = (q[0, i] - r[0, ii]) * s[0]
-            d = a * a
-            if d > thr:
-                continue
-            a = (q[1, i] - r[1, ii]) * s[1]
-            d = d + a * a
-            if d > thr:
-                continue
-            a = (q[2, i] - r[2, ii]) * s[2]
-            d = d + a * a
-            if d > thr:
-                continue
-            md[pos], mi[pos] = d, ii
-            thr, pos = md[0], 0
-            for j in range(1, K):
-                if thr < md[j]:
-                    thr, pos = md[j], j
-        ind[:, i] = mi
-        dist[:, i] = md
-    return ind.reshape(-1), dist.reshape(-1)
+replay = scan_replay
 
 
 def test_nan_partial_sums_leave_the_reference():

@@ -17,6 +17,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "../../include/tgmm_em.h"
+
 namespace tgmm {
 
 constexpr int kAlign = 256;      // supervoxel / Gaussian range alignment
@@ -127,6 +129,8 @@ struct DevPtrs {
                     // when that iteration's M-step runs)
   LLPack* llpack;   // Gcap: log-likelihood parameters (written by death)
   float4* m4;       // Gcap: f32 mean and index bits (kNN references; NaN: dead)
+  float4* mknn;     // Gcap: the means of the last kNN search, in index order
+                    // (written when they are binned; NaN: dead)
   int* gcell;
   int* gblk_frame;  // Gcap/kGB
   // M-step segments
@@ -141,6 +145,15 @@ struct DevPtrs {
   int* qcell_start; // Ccap+1: first kNN query of each cell (positions in q4)
   int* cblk_frame;  // Ccap/kCB: frame of each 64-cell block, -1 if none
   float4* cg4;      // Gcap: cell-sorted references (x, y, z, index bits)
+  // kNN: supervoxels whose K-th distance is tied with a Gaussian outside
+  // their K best (their candidate set follows the reference's scan order)
+  int* tie_list;    // Scap
+  int* knn_ctl;     // [0] ties listed this iteration, [1] supervoxels whose
+                    //     final slot layout found a different set (stays 0)
+  int2* win;        // Scap: (Gaussian, slot) of the candidate with r > 0.5, or -1
+  // results
+  GaussianMixtureModelCUDA* rec;  // Gcap: records as returned (288 B each)
+  int* svkeep;      // 17*Gcap: supervoxelIdx[16] and supervoxelNum as given
   // reductions
   double* part_g;   // Gcap/kGB
   double* part_g2;  // Gcap/kGB: sums of alpha after the posterior update (death test)
@@ -177,6 +190,21 @@ int kmax_for(int K);
 hipError_t launch_next_priors(const DevPtrs& P, long long g0, long long ng, double beta_pct,
                               double nu_pct, double alpha_pct, double alpha_total,
                               hipStream_t st);
+
+// After a solve: each supervoxel's K slots in the reference's layout (its
+// scan of the Gaussians in index order, knnCuda.cu:61-144) with the
+// responsibilities moved along, and the (Gaussian, slot) of each supervoxel's
+// candidate with r > 0.5. Frames that ran no iteration are left as they are.
+// only_frame >= 0: that frame only (its supervoxel range from P.s_lo, P.Sext).
+hipError_t launch_knn_layout(const DevPtrs& P, const KOpts& O, int only_frame, int max_ng,
+                             hipStream_t st);
+// The 288-byte records of every Gaussian (of only_frame, from P.g_lo and
+// P.Gext, when >= 0): state, fixed flag, and supervoxelIdx/supervoxelNum
+// from the r > 0.5 assignment (GMEM.cu:4328-4350; entries past the count as
+// given) or, in voxel mode, as given. writeback: the returned lists become
+// the given ones (a stream's next frame starts from its posterior records).
+hipError_t launch_finalize(const DevPtrs& P, const KOpts& O, int cap, int only_frame,
+                           int writeback, hipStream_t st);
 
 // timed_kernel ids for launch_iteration
 enum { TK_KNN = 0, TK_ESTEP = 1, TK_MSTEP = 2, TK_LOGLIK = 3 };

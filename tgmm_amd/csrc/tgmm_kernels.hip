@@ -25,6 +25,7 @@
 //                  (1913-1978), per-frame total, convergence (4145-4148)
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <climits>
 
 #include "tgmm_device.h"
@@ -407,6 +408,8 @@ __global__ __launch_bounds__(1024) void k_scan_frames(DevPtrs P, int which, int 
                                                       int K) {
   __shared__ int lds[32];
   __shared__ int carry;
+  // the grid scan opens an iteration: the kNN's tie list starts empty
+  if (which == 0 && blockIdx.x == 0 && threadIdx.x == 0) P.knn_ctl[0] = 0;
   const int f = (P.only_frame >= 0) ? P.only_frame : (int)blockIdx.x;
   if (f >= P.Fcap || (P.only_frame < 0 && f >= P.nframes)) return;
   const FrameDev* F = P.fr + f;
@@ -518,20 +521,64 @@ __global__ __launch_bounds__(kGB) void k_gauss_prep(DevPtrs P, KOpts O) {
                  gsv(P, GS_BETA, k), gsv(P, GS_ALPHA, k));
 }
 
+// Counting-sort scatter of the means into their grid cells; every mean of an
+// active frame (NaN when dead) is also kept in index order as the means of
+// this iteration's kNN search (m4 is rewritten for the next iteration before
+// the search's slot layout is replayed after the last one).
 __global__ __launch_bounds__(kGB) void k_grid_scatter(DevPtrs P) {
   const int f = P.gblk_frame[blockIdx.x];
   if (f < 0 || !P.fr[f].active) return;
   const long long k = (long long)blockIdx.x * kGB + threadIdx.x;
   if (P.g_frame[k] != f) return;
+  const float4 m = P.m4[k];
+  P.mknn[k] = m;
   const int c = P.gcell[k];
   if (c < 0) return;
   const int pos = P.cell_start[c] + atomicSub(P.cell_cnt + c, 1) - 1;
-  const long long G = P.Gcap;
-  P.cg4[pos] = P.m4[k];
+  P.cg4[pos] = m;
+}
+
+// Inserts a candidate (distance d <= 1e32, index idx) into a list of the KM
+// best kept sorted by (distance, index). `tie` records whether an element
+// left out of the K best (the candidate itself, or the K-th it displaces) has
+// the distance of the K-th after the insertion: then the reference's scan,
+// whose slot replacement rule breaks such ties by position, may hold a
+// different set than the lower-index rule, and the supervoxel is searched
+// again in the reference's order. Every Gaussian at the final K-th distance
+// is considered by the search (the termination and pruning bounds are
+// strict), so a tie is never missed; an equality with a K-th distance that
+// later falls flags a supervoxel needlessly, which changes nothing.
+template <int KM>
+__device__ inline void knn_try(float (&bd)[KM], int (&bi)[KM], float d, int idx, int K,
+                               bool& tie) {
+  if (!(d <= 1e32f)) return;
+  if (!lex_less(d, idx, bd[KM - 1], bi[KM - 1])) {
+    float kd = bd[0];
+#pragma unroll
+    for (int j = 1; j < KM; j++)
+      if (j == K - 1) kd = bd[j];
+    tie |= d == kd;
+    return;
+  }
+  float cd = d;
+  int ci = idx;
+#pragma unroll
+  for (int j = 0; j < KM; j++) {
+    if (lex_less(cd, ci, bd[j], bi[j])) {
+      const float td = bd[j];
+      const int ti = bi[j];
+      bd[j] = cd;
+      bi[j] = ci;
+      cd = td;
+      ci = ti;
+    }
+    // cd is now the best element outside the K best
+    if (j == K - 1) tie |= (cd <= 1e32f) && cd == bd[j];
+  }
 }
 
 template <int KM, int QL>
-__device__ inline void knn_merge(float (&bd)[KM], int (&bi)[KM]) {
+__device__ inline void knn_merge(float (&bd)[KM], int (&bi)[KM], int K, bool& tie) {
 #pragma unroll
   for (int m = QL / 2; m >= 1; m >>= 1) {
     float od[KM];
@@ -546,20 +593,7 @@ __device__ inline void knn_merge(float (&bd)[KM], int (&bi)[KM]) {
       bool dup = false;
 #pragma unroll
       for (int j = 0; j < KM; j++) dup |= (bi[j] == oi[t]);
-      float cd = od[t];
-      int ci = oi[t];
-      if (dup || !lex_less(cd, ci, bd[KM - 1], bi[KM - 1])) continue;
-#pragma unroll
-      for (int j = 0; j < KM; j++) {
-        if (lex_less(cd, ci, bd[j], bi[j])) {
-          const float td = bd[j];
-          const int ti = bi[j];
-          bd[j] = cd;
-          bi[j] = ci;
-          cd = td;
-          ci = ti;
-        }
-      }
+      if (!dup) knn_try<KM>(bd, bi, od[t], oi[t], K, tie);
     }
   }
 }
@@ -574,7 +608,7 @@ __device__ inline void knn_merge(float (&bd)[KM], int (&bi)[KM]) {
 template <int KM, int QL, typename Consider>
 __device__ inline void knn_shells(const DevPtrs& P, const FrameDev* F, const int* c,
                                   const double* u, int K, int sub, float (&bd)[KM],
-                                  int (&bi)[KM], Consider consider) {
+                                  int (&bi)[KM], bool& tie, Consider consider) {
   const int nx = F->dims[0], ny = F->dims[1], nz = F->dims[2];
   const int dims[3] = {nx, ny, nz};
   const double h = F->h;
@@ -643,7 +677,7 @@ __device__ inline void knn_shells(const DevPtrs& P, const FrameDev* F, const int
           }
         }
       }
-      if (QL > 1) knn_merge<KM, QL>(bd, bi);
+      if (QL > 1) knn_merge<KM, QL>(bd, bi, K, tie);
     }
     const double lb2 = lower_bound_sq(r);
     if (lb2 == INFINITY || kth() < lb2) break;  // every cell visited, or done
@@ -691,24 +725,10 @@ __global__ __launch_bounds__(256) void k_knn(DevPtrs P, KOpts O) {
     bi[j] = 0x7fffffff;
   }
   const long long c0 = F->c0;
+  bool tie = false;
   auto consider = [&](const float4 cur) {
     const float d = knn_dist(q0, q1, q2, cur.x, cur.y, cur.z, s0, s1, s2);
-    const int idx = __float_as_int(cur.w);
-    if ((d <= 1e32f) && lex_less(d, idx, bd[KM - 1], bi[KM - 1])) {
-      float cd = d;
-      int ci2 = idx;
-#pragma unroll
-      for (int j = 0; j < KM; j++) {
-        if (lex_less(cd, ci2, bd[j], bi[j])) {
-          const float td = bd[j];
-          const int ti = bi[j];
-          bd[j] = cd;
-          bi[j] = ci2;
-          cd = td;
-          ci2 = ti;
-        }
-      }
-    }
+    knn_try<KM>(bd, bi, d, __float_as_int(cur.w), K, tie);
   };
   // An upper bound of the K-th distance: the previous iteration's candidates
   // of this supervoxel, if all K are still living means of this frame (a
@@ -791,12 +811,15 @@ __global__ __launch_bounds__(256) void k_knn(DevPtrs P, KOpts O) {
         if (p0 + w * QL < tot) consider(v[w]);
     }
   }
-  if (QL > 1) knn_merge<KM, QL>(bd, bi);
-  knn_shells<KM, QL>(P, F, c, u, K, sub, bd, bi, consider);
+  if (QL > 1) knn_merge<KM, QL>(bd, bi, K, tie);
+  knn_shells<KM, QL>(P, F, c, u, K, sub, bd, bi, tie, consider);
+#pragma unroll
+  for (int m = QL / 2; m >= 1; m >>= 1) tie |= __shfl_xor((int)tie, m, QL) != 0;
   if (sub == 0) {
 #pragma unroll
     for (int j = 0; j < KM; j++)
       if (j < K) P.ind[(long long)j * S + s] = (bd[j] <= 1e32f) ? bi[j] : -1;
+    if (tie) P.tie_list[atomicAdd(P.knn_ctl, 1)] = (int)s;
   }
 }
 
@@ -810,27 +833,12 @@ __global__ __launch_bounds__(256) void k_knn(DevPtrs P, KOpts O) {
 // further shells on its own, from HBM, as in k_knn.
 constexpr int kKnnCap = 512;
 
-template <int KM>
-__device__ inline void knn_insert(float (&bd)[KM], int (&bi)[KM], float d, int idx) {
-  if (!lex_less(d, idx, bd[KM - 1], bi[KM - 1])) return;
-#pragma unroll
-  for (int j = 0; j < KM; j++) {
-    if (lex_less(d, idx, bd[j], bi[j])) {
-      const float td = bd[j];
-      const int ti = bi[j];
-      bd[j] = d;
-      bi[j] = idx;
-      d = td;
-      idx = ti;
-    }
-  }
-}
-
 template <int KM, int NW>
 __global__ __launch_bounds__(64 * NW) void k_knn_cell(DevPtrs P, KOpts O) {
   __shared__ float4 cand[kKnnCap];
   __shared__ float ld[NW > 1 ? NW - 1 : 1][KM][64];
   __shared__ int li[NW > 1 ? NW - 1 : 1][KM][64];
+  __shared__ int lt[NW > 1 ? NW - 1 : 1][64];
   const long long cc = blockIdx.x;  // global cell index
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int f = P.cblk_frame[cc / kCB];
@@ -881,9 +889,10 @@ __global__ __launch_bounds__(64 * NW) void k_knn_cell(DevPtrs P, KOpts O) {
       bd[j] = INFINITY;
       bi[j] = 0x7fffffff;
     }
+    bool tie = false;
     auto consider = [&](const float4 cur) {
       const float d = knn_dist(q0, q1, q2, cur.x, cur.y, cur.z, s0, s1, s2);
-      if (d <= 1e32f) knn_insert<KM>(bd, bi, d, __float_as_int(cur.w));
+      knn_try<KM>(bd, bi, d, __float_as_int(cur.w), K, tie);
     };
     for (int cb = 0; cb < T; cb += kKnnCap) {
       const int n = min(kKnnCap, T - cb);
@@ -918,23 +927,342 @@ __global__ __launch_bounds__(64 * NW) void k_knn_cell(DevPtrs P, KOpts O) {
           ld[w - 1][j][lane] = bd[j];
           li[w - 1][j][lane] = bi[j];
         }
+        lt[w - 1][lane] = tie ? 1 : 0;
       }
       __syncthreads();
       if (w > 0) continue;  // wave 0 finishes the cell's queries
 #pragma unroll
-      for (int v = 0; v < NW - 1; v++)
+      for (int v = 0; v < NW - 1; v++) {
 #pragma unroll
-        for (int j = 0; j < KM; j++) {
-          const float d = ld[v][j][lane];
-          if (d <= 1e32f) knn_insert<KM>(bd, bi, d, li[v][j][lane]);
-        }
+        for (int j = 0; j < KM; j++) knn_try<KM>(bd, bi, ld[v][j][lane], li[v][j][lane], K, tie);
+        tie |= lt[v][lane] != 0;
+      }
     }
     if (!act) continue;
     const double u[3] = {(double)q0 * s0, (double)q1 * s1, (double)q2 * s2};
-    knn_shells<KM, 1>(P, F, c, u, K, 0, bd, bi, consider);
+    knn_shells<KM, 1>(P, F, c, u, K, 0, bd, bi, tie, consider);
 #pragma unroll
     for (int j = 0; j < KM; j++)
       if (j < K) P.ind[(long long)j * S + s] = (bd[j] <= 1e32f) ? bi[j] : -1;
+    if (tie) P.tie_list[atomicAdd(P.knn_ctl, 1)] = (int)s;
+  }
+}
+
+// ------------------------------------------- the reference's scan order --
+// The reference's kNN (knnKernel, knnCuda.cu:61-144) scans the Gaussians in
+// index order: K slots of (distance, index) start at 1e32 with threshold
+// 1e32; it leaves a Gaussian as soon as the x term, the x + y partial sum or
+// the full sum is above the threshold (:116-124), otherwise the Gaussian
+// replaces the slot at minPos, and minPos becomes the first slot holding the
+// largest distance (findMaxPosition, :49-59). The slot layout, and under a
+// tie at the K-th distance the set itself, are the outcome of that scan.
+// A dead Gaussian (its mean -1e32, an infinite distance) is never taken; here
+// its kNN mean is NaN and it is skipped.
+template <int KM>
+struct ScanSlots {
+  float md[KM];
+  int mi[KM];
+  float thr;
+  int pos;
+  __device__ inline void init(bool live) {
+#pragma unroll
+    for (int j = 0; j < KM; j++) {
+      md[j] = 1e32f;
+      mi[j] = -1;
+    }
+    thr = live ? 1e32f : -INFINITY;  // a lane without a query takes nothing
+    pos = 0;
+  }
+  // distance terms of one Gaussian in the reference's order
+  __device__ static inline void terms(float q0, float q1, float q2, float4 r, float s0, float s1,
+                                      float s2, float& p1, float& p2, float& d) {
+#pragma clang fp contract(off)
+    float a = (q0 - r.x) * s0;
+    p1 = a * a;
+    a = (q1 - r.y) * s1;
+    p2 = p1 + a * a;
+    a = (q2 - r.z) * s2;
+    d = p2 + a * a;
+  }
+  __device__ inline bool passes(float p1, float p2, float d) const {
+    return !(p1 > thr) && !(p2 > thr) && !(d > thr);
+  }
+  __device__ inline void insert(float d, int k, int K) {
+#pragma unroll
+    for (int j = 0; j < KM; j++)
+      if (j == pos) {
+        md[j] = d;
+        mi[j] = k;
+      }
+    thr = md[0];
+    pos = 0;
+#pragma unroll
+    for (int j = 1; j < KM; j++)
+      if (j < K && thr < md[j]) {
+        thr = md[j];
+        pos = j;
+      }
+  }
+  __device__ inline void take(float q0, float q1, float q2, float4 r, int k, float s0, float s1,
+                              float s2, int K) {
+    if (r.x != r.x) return;  // dead
+    float p1, p2, d;
+    terms(q0, q1, q2, r, s0, s1, s2, p1, p2, d);
+    if (passes(p1, p2, d)) insert(d, k, K);
+  }
+};
+
+// Supervoxels whose K-th distance was tied (listed by the kNN search): the
+// reference's scan over all of the frame's Gaussians, one wave per
+// supervoxel, 64 Gaussians per step. Every lane forms one Gaussian's terms;
+// the Gaussians that pass the step's starting threshold (a superset of those
+// the scan takes, the threshold only falling) are then taken in index order
+// with the exact test, the slots held identically in every lane. The slots
+// are written in the reference's layout.
+template <int KM>
+__global__ __launch_bounds__(256) void k_knn_exact(DevPtrs P, KOpts O) {
+  const int lane = threadIdx.x & 63;
+  const int wv = blockIdx.x * 4 + (threadIdx.x >> 6), nw = gridDim.x * 4;
+  const int cnt = P.knn_ctl[0];
+  const long long S = P.Scap;
+  const int K = O.K;
+  const float s0 = O.scale[0], s1 = O.scale[1], s2 = O.scale[2];
+  for (int t = wv; t < cnt; t += nw) {
+    const long long s = P.tie_list[t];
+    const FrameDev* F = P.fr + P.sv_frame[s];
+    const long long g0 = F->g0;
+    const int ng = (int)F->ng;
+    const float q0 = P.sv_cent[s], q1 = P.sv_cent[S + s], q2 = P.sv_cent[2 * S + s];
+    ScanSlots<KM> sl;
+    sl.init(true);
+    for (int b = 0; b < ng; b += 64) {
+      const int k = b + lane;
+      float p1 = 0.f, p2 = 0.f, d = 0.f;
+      bool ok = false;
+      if (k < ng) {
+        const float4 r = P.mknn[g0 + k];
+        ok = r.x == r.x;
+        ScanSlots<KM>::terms(q0, q1, q2, r, s0, s1, s2, p1, p2, d);
+      }
+      unsigned long long mask = __ballot(ok && sl.passes(p1, p2, d));
+      const unsigned long long alive = __ballot(ok);
+      while (mask) {
+        const int u = __builtin_ctzll(mask);
+        mask &= mask - 1ull;
+        const float up1 = __shfl(p1, u, 64), up2 = __shfl(p2, u, 64), ud = __shfl(d, u, 64);
+        if (!sl.passes(up1, up2, ud)) continue;
+        sl.insert(ud, b + u, K);
+        // a NaN in slot 0 keeps the threshold NaN: the scan then takes the
+        // next Gaussian whatever its distance
+        if (sl.thr != sl.thr && u < 63) mask |= alive & (~0ull << (u + 1));
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < KM; j++)
+      if (j < K && lane == j) P.ind[(long long)j * S + s] = sl.mi[j] >= 0 ? (int)(g0 + sl.mi[j]) : -1;
+  }
+}
+
+// The slot layout of every supervoxel after a solve: the reference's scan of
+// its last kNN search, replayed exactly with the Gaussians of that search
+// (mknn, and its grid), one lane per supervoxel, 64 supervoxels of one
+// frame in grid-cell order per block.
+//  1. The first m Gaussians in index order (m ~ sqrt(K * ng)) are scanned
+//     by every lane (wave-uniform loads). A lane's threshold is then T_m,
+//     and every later Gaussian the scan takes has a distance <= T_m (the
+//     threshold only falls).
+//  2. Every later Gaussian within the largest such ball of the wave's lanes
+//     lies in the box spanned by the lanes' queries widened by
+//     sqrt(max T_m) (with a relative margin far above float rounding); the
+//     wave marks the indices of the grid cells' Gaussians in that box in an
+//     LDS bitmap and scans the marked ones in index order with the exact
+//     test. The others are farther than every lane's threshold, so the scan
+//     would not take them. When the box or the bitmap is too large (a lane
+//     with fewer than K Gaussians taken, or more Gaussians than the bitmap
+//     holds), the wave scans every later Gaussian instead.
+// The candidates found by the iterations' search (ind, rnk) are then put in
+// these slots, responsibilities moving with their Gaussian; a supervoxel
+// whose set differs counts in knn_ctl[1] (never, by construction). win[s]
+// receives the (Gaussian, slot) whose responsibility is above 0.5, if any.
+constexpr int kLayoutCells = 4096;  // most grid cells marked per wave
+constexpr int kLayoutWords = 4096;  // largest bitmap (Gaussians / 64) in LDS: 32 KiB
+
+template <int KM>
+__global__ __launch_bounds__(64) void k_knn_layout(DevPtrs P, KOpts O, int only_frame,
+                                                   int words_cap) {
+  extern __shared__ unsigned long long bits[];
+  const int lane = threadIdx.x;
+  const long long qb = P.s_lo + (long long)blockIdx.x * 64;
+  if (qb >= P.Scap) return;
+  const int f = P.sblk_frame[qb >> 8];
+  if (f < 0 || (only_frame >= 0 && f != only_frame)) return;
+  const FrameDev* F = P.fr + f;
+  if (F->iter <= 1 || F->ng <= 0 || qb >= F->s0 + F->ns) return;  // no kNN ran
+  const long long S = P.Scap;
+  const long long g0 = F->g0;
+  const int ng = (int)F->ng;
+  const int K = O.K;
+  const float s0 = O.scale[0], s1 = O.scale[1], s2 = O.scale[2];
+  const long long i = qb + lane;
+  const bool live = i < F->s0 + F->ns;
+  const float4 qq = live ? P.q4[i] : make_float4(0.f, 0.f, 0.f, 0.f);
+  const long long s = live ? __float_as_int(qq.w) : 0;
+  const float q0 = qq.x, q1 = qq.y, q2 = qq.z;
+  ScanSlots<KM> sl;
+  sl.init(live);
+  // 1. the first m Gaussians
+  int m = 64 * (int)ceil(sqrt((double)K * (double)ng) / 64.0);
+  m = min(max(m, 64), ng);
+  for (int k = 0; k < m; k++) sl.take(q0, q1, q2, P.mknn[g0 + k], k, s0, s1, s2, K);
+  // 2. the later Gaussians in the wave's box
+  if (m < ng) {
+    float R2 = live ? sl.thr : 0.0f;
+    double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+    if (live) {
+      const double u[3] = {(double)q0 * s0, (double)q1 * s1, (double)q2 * s2};
+      for (int a = 0; a < 3; a++) {
+        lo[a] = u[a];
+        hi[a] = u[a];
+      }
+    }
+    bool wide = !(R2 < 1e32f);  // fewer than K taken, or NaN
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) {
+      R2 = fmaxf(R2, __shfl_xor(R2, o, 64));
+      for (int a = 0; a < 3; a++) {
+        lo[a] = fmin(lo[a], __shfl_xor(lo[a], o, 64));
+        hi[a] = fmax(hi[a], __shfl_xor(hi[a], o, 64));
+      }
+    }
+    wide = __any(wide);
+    const int words = (ng - m + 63) / 64;
+    int cl[3] = {0, 0, 0}, ch[3] = {0, 0, 0};
+    long long ncell = 0;
+    if (!wide) {
+      const double R = sqrt((double)R2) * (1.0 + 1e-5) + 1e-30;
+      ncell = 1;
+      for (int a = 0; a < 3; a++) {
+        cl[a] = axis_cell(lo[a] - R, F->org[a], F->h, F->dims[a]);
+        ch[a] = axis_cell(hi[a] + R, F->org[a], F->h, F->dims[a]);
+        ncell *= (long long)(ch[a] - cl[a] + 1);
+      }
+    }
+    if (wide || words > words_cap || ncell > kLayoutCells) {
+      for (int k = m; k < ng; k++) sl.take(q0, q1, q2, P.mknn[g0 + k], k, s0, s1, s2, K);
+    } else {
+      for (int w = lane; w < words; w += 64) bits[w] = 0ull;
+      __syncthreads();
+      const int nx = ch[0] - cl[0] + 1, ny = ch[1] - cl[1] + 1;
+      for (long long c = lane; c < ncell; c += 64) {
+        const int cx = cl[0] + (int)(c % nx), cy = cl[1] + (int)((c / nx) % ny),
+                  cz = cl[2] + (int)(c / ((long long)nx * ny));
+        const long long cell = F->c0 + ((long long)cz * F->dims[1] + cy) * F->dims[0] + cx;
+        const int pe = P.cell_start[cell + 1];
+        for (int p = P.cell_start[cell]; p < pe; p++) {
+          const int k = (int)(__float_as_int(P.cg4[p].w) - g0) - m;
+          if (k >= 0) atomicOr(bits + (k >> 6), 1ull << (k & 63));
+        }
+      }
+      __syncthreads();
+      for (int w = 0; w < words; w++) {
+        unsigned long long word = bits[w];
+        while (word) {
+          const int k = m + 64 * w + __builtin_ctzll(word);
+          word &= word - 1ull;
+          sl.take(q0, q1, q2, P.mknn[g0 + k], k, s0, s1, s2, K);
+        }
+      }
+    }
+  }
+  if (!live) return;
+  // the search's candidates into the reference's slots
+  int oi[KM];
+  float orr[KM];
+#pragma unroll
+  for (int j = 0; j < KM; j++) {
+    oi[j] = j < K ? P.ind[(long long)j * S + s] : -1;
+    orr[j] = j < K ? P.rnk[(long long)j * S + s] : 0.0f;
+  }
+  int n_old = 0, n_new = 0;
+#pragma unroll
+  for (int j = 0; j < KM; j++) {
+    n_old += (j < K && oi[j] >= 0) ? 1 : 0;
+    n_new += (j < K && sl.mi[j] >= 0) ? 1 : 0;
+  }
+  bool same = n_old == n_new;
+  int2 w = make_int2(-1, -1);
+#pragma unroll
+  for (int j = 0; j < KM; j++) {
+    if (j >= K) continue;
+    const int k = sl.mi[j] >= 0 ? (int)(g0 + sl.mi[j]) : -1;
+    float r = 0.0f;
+    bool found = k < 0;
+#pragma unroll
+    for (int t = 0; t < KM; t++)
+      if (t < K && k >= 0 && oi[t] == k) {
+        r = orr[t];
+        found = true;
+      }
+    same &= found;
+    P.ind[(long long)j * S + s] = k;
+    P.rnk[(long long)j * S + s] = r;
+    if (k >= 0 && r > 0.5f) w = make_int2(k, j);
+  }
+  P.win[s] = w;
+  if (!same) atomicAdd(P.knn_ctl + 1, 1);
+}
+
+// The returned records (GaussianMixtureModelCUDA, 288 bytes), one lane per
+// Gaussian: the 27 state doubles, the fixed flag, and the supervoxels
+// assigned to the Gaussian: for slots j = 0, 1, ... in turn, the supervoxels
+// s in increasing order whose candidate in slot j is this Gaussian with
+// r > 0.5, at most `cap` (the reference's slot-major scan, GMEM.cu:4328-4350).
+// They are found among the Gaussian's M-step pairs (r > 1e-3), ordered by
+// supervoxel. Entries past the count are as given; voxel mode keeps the
+// given lists (no assignment, GMEM.cu:3420-3973).
+__global__ __launch_bounds__(64) void k_finalize(DevPtrs P, KOpts O, int cap, int only_frame,
+                                                 int writeback) {
+  const long long k = P.g_lo + (long long)blockIdx.x * 64 + threadIdx.x;
+  if (k >= P.Gcap) return;
+  const int f = P.g_frame[k];
+  if (f < 0 || (only_frame >= 0 && f != only_frame)) return;
+  const FrameDev* F = P.fr + f;
+  const long long G = P.Gcap;
+  GaussianMixtureModelCUDA* R = P.rec + k;
+  double* d = (double*)R;
+#pragma unroll
+  for (int i = 0; i < GS_NFIELDS; i++) d[i] = P.gs[(long long)i * G + k];
+  int* kp = P.svkeep + 17 * k;
+  int idx[TGMM_MAX_SUPERVOXELS_PER_GAUSSIAN];
+#pragma unroll
+  for (int i = 0; i < TGMM_MAX_SUPERVOXELS_PER_GAUSSIAN; i++) idx[i] = kp[i];
+  int n = kp[16];
+  if (!O.voxel) {
+    n = 0;
+    if (F->iter > 1 && cap > 0) {
+      const int st = P.seg_start[k], en = P.seg_start[k + 1];
+      for (int j = 0; j < O.K && n < cap; j++)
+        for (int e = st; e < en && n < cap; e++) {
+          const long long sv = P.pairs_sorted[e] >> kPairShift;
+          const int2 w = P.win[sv];
+          if (w.x == (int)k && w.y == j) {
+#pragma unroll
+            for (int t = 0; t < TGMM_MAX_SUPERVOXELS_PER_GAUSSIAN; t++)
+              if (t == n) idx[t] = (int)(sv - F->s0);
+            n++;
+          }
+        }
+    }
+  }
+  unsigned int* u = (unsigned int*)((char*)R + 216);  // fixed and its padding
+  *u = (unsigned int)P.fixed[k];
+#pragma unroll
+  for (int i = 0; i < TGMM_MAX_SUPERVOXELS_PER_GAUSSIAN; i++) R->supervoxelIdx[i] = idx[i];
+  R->supervoxelNum = n;
+  if (writeback && !O.voxel) {
+#pragma unroll
+    for (int i = 0; i < TGMM_MAX_SUPERVOXELS_PER_GAUSSIAN; i++) kp[i] = idx[i];
+    kp[16] = n;
   }
 }
 
@@ -1861,6 +2189,35 @@ static void launch_knn(const DevPtrs& P, const KOpts& O, hipStream_t st) {
 }
 
 template <int KM>
+static void launch_knn_layout_t(const DevPtrs& P, const KOpts& O, int only_frame, int max_ng,
+                                hipStream_t st) {
+  const int words = (int)std::min<long long>(kLayoutWords, ((long long)max_ng + 63) / 64);
+  k_knn_layout<KM><<<nb(P.Sext, 64), 64, (size_t)std::max(words, 1) * 8, st>>>(P, O, only_frame,
+                                                                               words);
+}
+
+hipError_t launch_knn_layout(const DevPtrs& P, const KOpts& O, int only_frame, int max_ng,
+                             hipStream_t st) {
+  switch (kmax_for(O.K)) {
+    case 5:
+      launch_knn_layout_t<5>(P, O, only_frame, max_ng, st);
+      break;
+    case 8:
+      launch_knn_layout_t<8>(P, O, only_frame, max_ng, st);
+      break;
+    default:
+      launch_knn_layout_t<16>(P, O, only_frame, max_ng, st);
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_finalize(const DevPtrs& P, const KOpts& O, int cap, int only_frame,
+                           int writeback, hipStream_t st) {
+  k_finalize<<<nb(P.Gext, 64), 64, 0, st>>>(P, O, cap, only_frame, writeback);
+  return hipGetLastError();
+}
+
+template <int KM>
 static hipError_t iteration_t(const DevPtrs& P, const KOpts& O, hipStream_t st, hipEvent_t e0,
                               hipEvent_t e1, int tk) {
   const unsigned gb = nb(P.Gext, kGB);
@@ -1869,6 +2226,7 @@ static hipError_t iteration_t(const DevPtrs& P, const KOpts& O, hipStream_t st, 
   k_grid_scatter<<<gb, kGB, 0, st>>>(P);
   if (tk == TK_KNN && e0) (void)hipEventRecord(e0, st);
   launch_knn<KM>(P, O, st);
+  k_knn_exact<KM><<<64, 256, 0, st>>>(P, O);
   if (tk == TK_KNN && e1) (void)hipEventRecord(e1, st);
   if (tk == TK_ESTEP && e0) (void)hipEventRecord(e0, st);
   k_estep<KM><<<nb(P.Sext, bs), bs, 0, st>>>(P, O);

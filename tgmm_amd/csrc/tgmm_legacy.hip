@@ -1,8 +1,10 @@
-// tgmm_legacy.hip -- the reference host API (GMEMupdateCUDA.h) as C-ABI
-// entry points with status codes. This is the plain per-frame form: each VI
-// call builds a one-frame workspace, runs one EM iteration per host round
-// trip with the convergence test read back on the host (as the reference's
-// loop does, GMEM.cu:4145-4264), and releases the workspace.
+// tgmm_legacy.hip -- the reference host API (GMEMupdateCUDA.h, and the device
+// helpers of UtilsCUDA/knnCuda.h that the tracker calls) as C-ABI entry
+// points with status codes. Each VI call runs its frame in a one-frame
+// workspace kept per device between calls (grown when a frame does not fit),
+// with the EM iteration replayed as a captured hipGraph and the convergence
+// flags polled every four iterations; the results come back as one copy of
+// the records formed on the device.
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
@@ -32,6 +34,74 @@ std::mutex g_mu;
 float g_scale_knn[3] = {1.f, 1.f, 1.f};
 float g_scale_reg[3] = {1.f, 1.f, 1.f};
 double g_lmin = -1.0, g_lmax = -1.0, g_emax = -1.0;
+
+// One cached one-frame workspace per device for the VI entry points (the
+// calls on a device are serialised by g_run_mu).
+std::mutex g_run_mu;
+std::vector<tgmm_batch*> g_cache;
+
+// Solve one frame in the device's cached workspace and write the records
+// into gm (host), rnk/ind into the caller's device arrays (K rows of S,
+// slot-major), and optionally the reference's debug dump.
+int run_frame(int dev, const FrameSrc& f, const tgmm_em_opts& o, const float* sreg,
+              GaussianMixtureModelCUDA* gm, float* rnkCUDA, int* indCUDA, int* indCUDAtr,
+              long long S, const char* dump, int frame) {
+  std::lock_guard<std::mutex> lk(g_run_mu);
+  if ((int)g_cache.size() <= dev) g_cache.resize(dev + 1, nullptr);
+  tgmm_batch* b = nullptr;
+  int rc = batch_build(dev, std::vector<FrameSrc>{f}, o, sreg, nullptr, &b, g_cache[dev], 1.25);
+  g_cache[dev] = rc ? nullptr : b;
+  if (rc) return rc;
+  rc = tgmm_batch_solve(b, TGMM_SOLVE_GRAPH);
+  tgmm_frame_desc d;
+  std::memset(&d, 0, sizeof(d));
+  d.gm = gm;
+  if (!rc) rc = tgmm_batch_fetch(b, &d, 1);
+  if (rc) return rc;
+  const long long Sc = b->ws.sh.Scap;
+  const size_t w = (size_t)S * sizeof(float);
+  const int K = o.K;
+  if (S > 0) {
+    hipError_t e = hipMemcpy2DAsync(rnkCUDA, w, b->ws.P.rnk, Sc * sizeof(float), w, K,
+                                    hipMemcpyDeviceToDevice, b->st);
+    if (e == hipSuccess)
+      e = hipMemcpy2DAsync(indCUDA, w, b->ws.P.ind, Sc * sizeof(int), w, K,
+                           hipMemcpyDeviceToDevice, b->st);
+    if (e == hipSuccess && indCUDAtr)
+      e = hipMemcpyAsync(indCUDAtr, indCUDA, K * w, hipMemcpyDeviceToDevice, b->st);
+    if (e == hipSuccess) e = hipStreamSynchronize(b->st);
+    if (e != hipSuccess) {
+      tgmm_set_error("copy of rnk/ind to caller buffers failed: %s", hipGetErrorString(e));
+      return TGMM_ERR_DEVICE;
+    }
+  }
+  if (getenv("TGMM_VERBOSE"))
+    printf("Frame=%d;iter=%d;Log-likelihood =%16.12f\n", frame, d.iterations + 1, d.ll);
+  if (dump && dump[0]) {
+    // {int S; int K; float rnk[S*K]; int ind[S*K]} (GMEM.cu:4364-4384)
+    std::vector<float> r2((size_t)S * K);
+    std::vector<int> i2((size_t)S * K);
+    if (S > 0 && (hipMemcpy2D(r2.data(), w, b->ws.P.rnk, Sc * 4, w, K, hipMemcpyDeviceToHost) !=
+                      hipSuccess ||
+                  hipMemcpy2D(i2.data(), w, b->ws.P.ind, Sc * 4, w, K, hipMemcpyDeviceToHost) !=
+                      hipSuccess)) {
+      tgmm_set_error("download of rnk/ind for the debug dump failed");
+      return TGMM_ERR_DEVICE;
+    }
+    FILE* fid = fopen(dump, "wb");
+    if (!fid) {
+      tgmm_set_error("opening file %s to save rnk values from variational inference", dump);
+      return TGMM_ERR_IO;
+    }
+    const int Si = (int)S, Ki = K;
+    fwrite(&Si, sizeof(int), 1, fid);
+    fwrite(&Ki, sizeof(int), 1, fid);
+    fwrite(r2.data(), sizeof(float), r2.size(), fid);
+    fwrite(i2.data(), sizeof(int), i2.size(), fid);
+    fclose(fid);
+  }
+  return TGMM_OK;
+}
 }  // namespace
 
 extern "C" int tgmm_setRegularizePrecisionMatrixConstants(double lambdaMin, double lambdaMax,
@@ -195,38 +265,8 @@ extern "C" int tgmm_GMEMvariationalInferenceCUDA(float* queryCUDA, float* imgDat
   f.query = queryCUDA;
   f.img = imgDataCUDA;
   f.gm = vecGMtemp;
-  tgmm_batch* b = nullptr;
-  int rc = batch_build(devCUDA, std::vector<FrameSrc>{f}, o, sreg, nullptr, &b);
-  if (rc) return rc;
-  rc = tgmm_batch_solve(b, TGMM_SOLVE_HOSTSYNC);
-  std::vector<double> gs;
-  std::vector<float> rnk;
-  std::vector<int> ind;
-  if (!rc) rc = batch_fetch_host(b, gs, rnk, ind);
-  double ll = 0.0;
-  int iters = 0;
-  if (!rc) {
-    frame_results(b, 0, gs, rnk, ind, vecGMtemp, nullptr, nullptr, &ll, &iters);
-    const long long Sc = b->ws.sh.Scap;
-    const size_t w = (size_t)query_nb * sizeof(float);
-    if (query_nb > 0) {
-      hipError_t e = hipMemcpy2D(rnkCUDA, w, b->ws.P.rnk, Sc * sizeof(float), w, TGMM_LEGACY_K,
-                                 hipMemcpyDeviceToDevice);
-      if (e == hipSuccess)
-        e = hipMemcpy2D(indCUDA, w, b->ws.P.ind, Sc * sizeof(int), w, TGMM_LEGACY_K,
-                        hipMemcpyDeviceToDevice);
-      if (e == hipSuccess && indCUDAtr)
-        e = hipMemcpy(indCUDAtr, indCUDA, TGMM_LEGACY_K * w, hipMemcpyDeviceToDevice);
-      if (e != hipSuccess) {
-        tgmm_set_error("copy of rnk/ind to caller buffers failed: %s", hipGetErrorString(e));
-        rc = TGMM_ERR_DEVICE;
-      }
-    }
-  }
-  if (!rc && getenv("TGMM_VERBOSE"))
-    printf("Frame=%d;iter=%d;Log-likelihood =%16.12f\n", frame, iters + 1, ll);
-  tgmm_batch_destroy(b);
-  return rc;
+  return run_frame(devCUDA, f, o, sreg, vecGMtemp, rnkCUDA, indCUDA, indCUDAtr, query_nb,
+                   nullptr, frame);
 }
 
 extern "C" int tgmm_GMEMreleaseMemoryWithSupervoxels(float** queryCUDA, float** imgDataCUDA,
@@ -313,59 +353,105 @@ extern "C" int tgmm_GMEMvariationalInferenceCUDAWithSupervoxels(
   f.cent = centroidLabelPositionCUDA;
   f.ptr = ptr.data();
   f.gm = vecGMtemp;
-  tgmm_batch* b = nullptr;
-  int rc = batch_build(devCUDA, std::vector<FrameSrc>{f}, o, sreg, nullptr, &b);
-  if (rc) return rc;
-  rc = tgmm_batch_solve(b, TGMM_SOLVE_HOSTSYNC);
-  std::vector<double> gs;
-  std::vector<float> rnk;
-  std::vector<int> ind;
-  if (!rc) rc = batch_fetch_host(b, gs, rnk, ind);
-  double ll = 0.0;
-  int iters = 0;
-  if (!rc) {
-    frame_results(b, 0, gs, rnk, ind, vecGMtemp, nullptr, nullptr, &ll, &iters);
-    const long long Sc = b->ws.sh.Scap;
-    const size_t w = (size_t)S * sizeof(float);
-    if (S > 0) {
-      hipError_t e = hipMemcpy2D(rnkCUDA, w, b->ws.P.rnk, Sc * sizeof(float), w, TGMM_LEGACY_K,
-                                 hipMemcpyDeviceToDevice);
-      if (e == hipSuccess)
-        e = hipMemcpy2D(indCUDA, w, b->ws.P.ind, Sc * sizeof(int), w, TGMM_LEGACY_K,
-                        hipMemcpyDeviceToDevice);
-      if (e == hipSuccess && indCUDAtr)
-        e = hipMemcpy(indCUDAtr, indCUDA, TGMM_LEGACY_K * w, hipMemcpyDeviceToDevice);
+  return run_frame(devCUDA, f, o, sreg, vecGMtemp, rnkCUDA, indCUDA, indCUDAtr, S, debugPath,
+                   frame);
+}
+
+// ------------------------------------------- UtilsCUDA/knnCuda.h helpers --
+// selectCUDAdeviceMaxMemory (knnCuda.h:23, knnCuda.cu:27-47): the device with
+// the most global memory, the first of equals; -1 when there is none.
+extern "C" int tgmm_selectCUDAdeviceMaxMemory(int* device) {
+  if (!device) {
+    tgmm_set_error("selectCUDAdeviceMaxMemory: device is NULL");
+    return TGMM_ERR_ARG;
+  }
+  *device = -1;
+  int count = 0;
+  HIPCHK(hipGetDeviceCount(&count));
+  size_t best = 0;
+  for (int i = 0; i < count; i++) {
+    size_t total = 0;
+    HIPCHK(hipDeviceTotalMem(&total, i));
+    if (total > best) {
+      best = total;
+      *device = i;
+    }
+  }
+  return TGMM_OK;
+}
+
+// setDeviceCUDA (knnCuda.h:20, knnCuda.cu:737-741)
+extern "C" int tgmm_setDeviceCUDA(int devCUDA) {
+  HIPCHK(hipSetDevice(devCUDA));
+  return TGMM_OK;
+}
+
+// uploadScaleCUDA (knnCuda.h:21, knnCuda.cu:827-830): the kNN anisotropy the
+// VI entry points use (the init entry points record it too).
+extern "C" int tgmm_uploadScaleCUDA(const float* scale) {
+  if (!scale) {
+    tgmm_set_error("uploadScaleCUDA: scale is NULL");
+    return TGMM_ERR_ARG;
+  }
+  std::lock_guard<std::mutex> lk(g_mu);
+  for (int a = 0; a < 3; a++) g_scale_knn[a] = scale[a];
+  return TGMM_OK;
+}
+
+// GMEMinitializeMemory (GMEMupdateCUDA.h:129-131, GMEMupdateCUDA.cu:4538-4579):
+// the voxel-mode device arrays query [3N] SoA, img [N], rnk, ind, indTr
+// [5N] on the current device, the host arrays copied, scale recorded as the
+// kNN anisotropy.
+extern "C" int tgmm_GMEMinitializeMemory(float** queryCUDA, const float* query,
+                                         float** imgDataCUDA, const float* imgData,
+                                         const float* scale, long long query_nb, float** rnkCUDA,
+                                         int** indCUDA, int** indCUDAtr) {
+  if (!queryCUDA || !imgDataCUDA || !rnkCUDA || !indCUDA || !indCUDAtr || !scale ||
+      query_nb < 0 || (query_nb > 0 && (!query || !imgData))) {
+    tgmm_set_error("GMEMinitializeMemory: NULL argument or negative size");
+    return TGMM_ERR_ARG;
+  }
+  const long long N = query_nb, K = TGMM_LEGACY_K;
+  *queryCUDA = *imgDataCUDA = *rnkCUDA = nullptr;
+  *indCUDA = *indCUDAtr = nullptr;
+  auto M = [](void** p, size_t n) { return hipMalloc(p, n > 0 ? n : 4); };
+  hipError_t e = hipSuccess;
+  if ((e = M((void**)queryCUDA, 3 * N * sizeof(float))) != hipSuccess ||
+      (e = M((void**)imgDataCUDA, N * sizeof(float))) != hipSuccess ||
+      (e = M((void**)rnkCUDA, K * N * sizeof(float))) != hipSuccess ||
+      (e = M((void**)indCUDA, K * N * sizeof(int))) != hipSuccess ||
+      (e = M((void**)indCUDAtr, K * N * sizeof(int))) != hipSuccess ||
+      (N > 0 && (e = hipMemcpy(*queryCUDA, query, 3 * N * sizeof(float),
+                               hipMemcpyHostToDevice)) != hipSuccess) ||
+      (N > 0 && (e = hipMemcpy(*imgDataCUDA, imgData, N * sizeof(float),
+                               hipMemcpyHostToDevice)) != hipSuccess)) {
+    tgmm_set_error("GMEMinitializeMemory: %s", hipGetErrorString(e));
+    tgmm_GMEMreleaseMemory(queryCUDA, imgDataCUDA, rnkCUDA, indCUDA, indCUDAtr);
+    return e == hipErrorOutOfMemory ? TGMM_ERR_OOM : TGMM_ERR_DEVICE;
+  }
+  return tgmm_uploadScaleCUDA(scale);
+}
+
+// GMEMreleaseMemory (GMEMupdateCUDA.h:138-139, GMEMupdateCUDA.cu:4637-4653):
+// frees and nulls every pointer.
+extern "C" int tgmm_GMEMreleaseMemory(float** queryCUDA, float** imgDataCUDA, float** rnkCUDA,
+                                      int** indCUDA, int** indCUDAtr) {
+  void** ps[5] = {(void**)queryCUDA, (void**)imgDataCUDA, (void**)rnkCUDA, (void**)indCUDA,
+                  (void**)indCUDAtr};
+  int rc = TGMM_OK;
+  for (void** p : ps) {
+    if (!p) {
+      rc = TGMM_ERR_ARG;
+      continue;
+    }
+    if (*p) {
+      hipError_t e = hipFree(*p);
       if (e != hipSuccess) {
-        tgmm_set_error("copy of rnk/ind to caller buffers failed: %s", hipGetErrorString(e));
+        tgmm_set_error("hipFree failed: %s", hipGetErrorString(e));
         rc = TGMM_ERR_DEVICE;
       }
     }
+    *p = nullptr;
   }
-  if (!rc && getenv("TGMM_VERBOSE"))
-    printf("Frame=%d;iter=%d;Log-likelihood =%16.12f\n", frame, iters + 1, ll);
-  if (!rc && debugPath && debugPath[0]) {
-    // {int S; int K; float rnk[S*K]; int ind[S*K]} (GMEM.cu:4364-4384)
-    FILE* fid = fopen(debugPath, "wb");
-    if (!fid) {
-      tgmm_set_error("opening file %s to save rnk values from variational inference", debugPath);
-      rc = TGMM_ERR_IO;
-    } else {
-      const int Si = (int)S, Ki = TGMM_LEGACY_K;
-      const long long Sc = b->ws.sh.Scap;
-      std::vector<float> r2((size_t)S * Ki);
-      std::vector<int> i2((size_t)S * Ki);
-      for (int j = 0; j < Ki; j++)
-        for (long long s = 0; s < S; s++) {
-          r2[j * S + s] = rnk[j * Sc + s];
-          i2[j * S + s] = ind[j * Sc + s];
-        }
-      fwrite(&Si, sizeof(int), 1, fid);
-      fwrite(&Ki, sizeof(int), 1, fid);
-      fwrite(r2.data(), sizeof(float), r2.size(), fid);
-      fwrite(i2.data(), sizeof(int), i2.size(), fid);
-      fclose(fid);
-    }
-  }
-  tgmm_batch_destroy(b);
   return rc;
 }

@@ -116,6 +116,12 @@ int ws_alloc(Workspace* w, const WsShape& sh, int K) {
   AL(P.qcell_start, C + 1);
   AL(P.cblk_frame, C / kCB + 1);
   AL(P.cg4, G);
+  AL(P.mknn, G);
+  AL(P.tie_list, S);
+  AL(P.knn_ctl, 64);
+  AL(P.win, S);
+  AL(P.rec, G);
+  AL(P.svkeep, 17 * G);
   AL(P.part_g, G / kGB);
   AL(P.part_g2, G / kGB);
   AL(P.part_s, S / kLLminB);
@@ -134,8 +140,10 @@ void ws_free(Workspace* w) {
 
 long long ws_bytes_estimate(const WsShape& sh, int K) {
   const long long S = sh.Scap, G = sh.Gcap, C = sh.Ccap, V = sh.Vcap;
-  return 16 * V + S * (16 + 8 + 4 + 12 + 80 + 80 + 12 + 16LL * K + 1) +
-         G * (4 + 8 * GS_NFIELDS * 2 + 1 + 96 + 8 + 64 + 12 + 4 + 8 + 12 + 4 + 1) + 8 * C + 64 * sh.Fcap;
+  return 16 * V + S * (16 + 8 + 4 + 12 + 80 + 80 + 12 + 16LL * K + 1 + 4 + 8) +
+         G * (4 + 8 * GS_NFIELDS * 2 + 1 + 96 + 8 + 64 + 12 + 4 + 8 + 12 + 4 + 1 + 16 + 288 +
+              68) +
+         8 * C + 64 * sh.Fcap;
 }
 
 // ------------------------------------------------------------------ batch --
@@ -203,13 +211,20 @@ bool tgmm::tiles_valid(const Tiles& t) {
          (t.knn_lanes == 0 || in(t.knn_lanes, 1, 2, 4));
 }
 
-// records (AoS, host) -> 27 x G SoA columns at [g0, g0+G) of a Gcap table
+// records (AoS, host) -> 27 x G SoA columns at [g0, g0+G) of a Gcap table,
+// the fixed flags, and (if svkeep) each record's supervoxelIdx[16] and
+// supervoxelNum as 17 ints at 17 * (g0 + k)
 void records_to_soa(const GaussianMixtureModelCUDA* gm, long long G, long long g0, long long Gcap,
-                    double* gs, unsigned char* fixed) {
+                    double* gs, unsigned char* fixed, int* svkeep) {
   for (long long k = 0; k < G; k++) {
     const double* d = (const double*)&gm[k];
     for (int fidx = 0; fidx < GS_NFIELDS; fidx++) gs[fidx * Gcap + g0 + k] = d[fidx];
     fixed[g0 + k] = gm[k].fixed ? 1 : 0;
+    if (svkeep) {
+      int* o = svkeep + 17 * (g0 + k);
+      std::memcpy(o, gm[k].supervoxelIdx, sizeof(int) * TGMM_MAX_SUPERVOXELS_PER_GAUSSIAN);
+      o[16] = gm[k].supervoxelNum;
+    }
   }
 }
 
@@ -222,7 +237,8 @@ void soa_to_records(const double* gs, long long G, long long g0, long long Gcap,
 }
 
 int batch_build(int device, const std::vector<FrameSrc>& fs_in, const tgmm_em_opts& o,
-                const float* scale_reg, hipStream_t stream, tgmm_batch** out) {
+                const float* scale_reg, hipStream_t stream, tgmm_batch** out,
+                tgmm_batch* reuse, double headroom) {
   if (int rc = validate_opts(&o)) return rc;
   HIPCHK(hipSetDevice(device));
   // voxel mode (GMEMupdateCUDA.cu:3420-3973): every voxel is its own kNN
@@ -245,23 +261,17 @@ int batch_build(int device, const std::vector<FrameSrc>& fs_in, const tgmm_em_op
     }
   }
   const std::vector<FrameSrc>& fs = o.voxel_mode ? vfs : fs_in;
-  tgmm_batch* b = new tgmm_batch();
-  b->device = device;
-  b->opts = o;
-  b->ko = make_kopts(o, scale_reg);
-  b->nframes = (int)fs.size();
   // placement
+  std::vector<FrameDev> fh(fs.size());
   WsShape sh;
   long long v = 0, s = 0, g = 0, c = 0;
-  b->fh.resize(fs.size());
   for (size_t i = 0; i < fs.size(); i++) {
     const FrameSrc& f = fs[i];
     if (f.N < 0 || f.S < 0 || f.G < 0) {
-      delete b;
       tgmm_set_error("frame %zu: negative size", i);
       return TGMM_ERR_ARG;
     }
-    FrameDev& F = b->fh[i];
+    FrameDev& F = fh[i];
     std::memset(&F, 0, sizeof(F));
     F.v0 = v;
     F.nv = f.N;
@@ -291,47 +301,72 @@ int batch_build(int device, const std::vector<FrameSrc>& fs_in, const tgmm_em_op
   sh.Gcap = std::max<long long>(g, kAlign);
   sh.Ccap = std::max<long long>(c, kCB);
   sh.Fcap = std::max<int>((int)fs.size(), 1);
+  const WsShape need = sh;  // the extents the kernels cover
   if ((sh.Scap << kPairShift) >= (1LL << 31) || (long long)o.K * sh.Scap >= (1LL << 31) ||
       sh.Gcap >= (1LL << 31) || sh.Ccap >= (1LL << 31)) {
-    delete b;
     tgmm_set_error("workspace exceeds 32-bit index limits (S=%lld, G=%lld)", sh.Scap, sh.Gcap);
     return TGMM_ERR_LIMIT;
   }
-  if (int rc = ws_alloc(&b->ws, sh, o.K)) {
-    ws_free(&b->ws);
-    delete b;
-    return rc;
+  tgmm_batch* b = nullptr;
+  if (reuse) {
+    const WsShape& r = reuse->ws.sh;
+    if (reuse->device == device && reuse->ws.K == o.K && r.Vcap >= sh.Vcap &&
+        r.Scap >= sh.Scap && r.Gcap >= sh.Gcap && r.Ccap >= sh.Ccap && r.Fcap >= sh.Fcap) {
+      b = reuse;  // the workspace fits: keep its allocations, stream and graph
+    } else {
+      tgmm_batch_destroy(reuse);
+    }
   }
+  if (!b) {
+    b = new tgmm_batch();
+    b->device = device;
+    if (headroom > 1.0) {  // grow-only caches leave room for larger frames
+      auto up = [&](long long x, long long a) {
+        return round_up((long long)std::ceil((double)x * headroom), a);
+      };
+      sh.Vcap = up(sh.Vcap, 1);
+      sh.Scap = up(sh.Scap, kAlign);
+      sh.Gcap = up(sh.Gcap, kAlign);
+      sh.Ccap = up(sh.Ccap, kCB);
+      if ((sh.Scap << kPairShift) >= (1LL << 31) || (long long)o.K * sh.Scap >= (1LL << 31) ||
+          sh.Gcap >= (1LL << 31) || sh.Ccap >= (1LL << 31))
+        sh = need;
+    }
+    if (int rc = ws_alloc(&b->ws, sh, o.K)) {
+      ws_free(&b->ws);
+      delete b;
+      return rc;
+    }
+    if (stream) {
+      b->st = stream;
+      b->own_stream = false;
+    } else {
+      if (hipStreamCreateWithFlags(&b->st, hipStreamNonBlocking) != hipSuccess) {
+        ws_free(&b->ws);
+        delete b;
+        tgmm_set_error("hipStreamCreate failed");
+        return TGMM_ERR_DEVICE;
+      }
+      b->own_stream = true;
+    }
+  }
+  b->opts = o;
+  b->ko = make_kopts(o, scale_reg);
+  b->nframes = (int)fs.size();
+  b->fh = fh;
+  b->fh.resize(b->ws.sh.Fcap);  // unused slots stay zero: no supervoxels, no Gaussians
+  for (size_t i = fs.size(); i < b->fh.size(); i++) std::memset(&b->fh[i], 0, sizeof(FrameDev));
   DevPtrs& P = b->ws.P;
   P.nframes = (int)fs.size();
   P.only_frame = -1;
-  if (stream) {
-    b->st = stream;
-    b->own_stream = false;
-  } else {
-    if (hipStreamCreateWithFlags(&b->st, hipStreamNonBlocking) != hipSuccess) {
-      ws_free(&b->ws);
-      delete b;
-      tgmm_set_error("hipStreamCreate failed");
-      return TGMM_ERR_DEVICE;
-    }
-    b->own_stream = true;
-  }
+  P.s_lo = P.g_lo = 0;
+  P.Sext = need.Scap;
+  P.Gext = need.Gcap;
+  P.Cext = need.Ccap;
   int rc = batch_upload(b, fs);
   if (rc) {
     tgmm_batch_destroy(b);
     return rc;
-  }
-  if (o.voxel_mode) {  // the records keep their supervoxel lists
-    b->sv_keep.resize(fs.size());
-    for (size_t i = 0; i < fs.size(); i++) {
-      b->sv_keep[i].resize((size_t)fs[i].G * (TGMM_MAX_SUPERVOXELS_PER_GAUSSIAN + 1));
-      for (int k = 0; k < fs[i].G; k++) {
-        int* d = b->sv_keep[i].data() + (size_t)k * (TGMM_MAX_SUPERVOXELS_PER_GAUSSIAN + 1);
-        std::memcpy(d, fs[i].gm[k].supervoxelIdx, sizeof(int) * TGMM_MAX_SUPERVOXELS_PER_GAUSSIAN);
-        d[TGMM_MAX_SUPERVOXELS_PER_GAUSSIAN] = fs[i].gm[k].supervoxelNum;
-      }
-    }
   }
   *out = b;
   return TGMM_OK;
@@ -347,6 +382,7 @@ int batch_upload(tgmm_batch* b, const std::vector<FrameSrc>& fs) {
   std::vector<int> sblk(S / kAlign, -1), gblk(G / kGB, -1), cblk(b->ws.sh.Ccap / kCB + 1, -1);
   std::vector<double> gs((size_t)GS_NFIELDS * G, 0.0);
   std::vector<unsigned char> fixed(G, 0);
+  std::vector<int> svkeep((size_t)17 * G, 0);
   for (size_t i = 0; i < fs.size(); i++) {
     const FrameSrc& f = fs[i];
     const FrameDev& F = b->fh[i];
@@ -369,7 +405,7 @@ int batch_upload(tgmm_batch* b, const std::vector<FrameSrc>& fs) {
     for (long long j = 0; j < F.nbg; j++) gblk[F.g0 / kGB + j] = (int)i;
     for (long long j = 0; j < F.ncap / kCB; j++) cblk[F.c0 / kCB + j] = (int)i;
     for (long long k = 0; k < f.G; k++) g_frame[F.g0 + k] = (int)i;
-    if (f.G > 0) records_to_soa(f.gm, f.G, F.g0, G, gs.data(), fixed.data());
+    if (f.G > 0) records_to_soa(f.gm, f.G, F.g0, G, gs.data(), fixed.data(), svkeep.data());
   }
   const hipMemcpyKind kv = hipMemcpyDefault;
   for (size_t i = 0; i < fs.size(); i++) {
@@ -395,6 +431,8 @@ int batch_upload(tgmm_batch* b, const std::vector<FrameSrc>& fs) {
   HIPCHK(hipMemcpyAsync(P.gs, gs.data(), gs.size() * 8, hipMemcpyHostToDevice, st));
   HIPCHK(hipMemcpyAsync(b->ws.gs_init, gs.data(), gs.size() * 8, hipMemcpyHostToDevice, st));
   HIPCHK(hipMemcpyAsync(P.fixed, fixed.data(), G, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(P.svkeep, svkeep.data(), svkeep.size() * 4, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemsetAsync(P.knn_ctl, 0, 64 * 4, st));
   HIPCHK(hipMemcpyAsync(P.fr, b->fh.data(), b->fh.size() * sizeof(FrameDev), hipMemcpyHostToDevice,
                         st));
   HIPCHK(hipMemsetAsync(P.ind, 0xFF, (size_t)b->ws.K * S * 4, st));
@@ -405,10 +443,12 @@ int batch_upload(tgmm_batch* b, const std::vector<FrameSrc>& fs) {
   b->grid_cell = b->ko.tiles.knn_cell;
   HIPCHK(launch_reset(P, b->ko, st));
   HIPCHK(hipStreamSynchronize(st));
-  // keep the host copies the fetch needs
-  b->fixed_host = fixed;
   b->sizes.clear();
-  for (const FrameSrc& f : fs) b->sizes.push_back({f.N, f.S, (long long)f.G});
+  b->max_ng = 0;
+  for (const FrameSrc& f : fs) {
+    b->sizes.push_back({f.N, f.S, (long long)f.G});
+    b->max_ng = std::max<long long>(b->max_ng, f.G);
+  }
   return TGMM_OK;
 }
 
@@ -467,7 +507,18 @@ static int any_active(tgmm_batch* b, bool* act) {
 }
 
 int batch_capture(tgmm_batch* b) {
+  // a captured iteration holds its kernel arguments: capture again when they
+  // changed (a cached workspace reused for frames of another size)
+  if (b->gexec && (std::memcmp(&b->cap_P, &b->ws.P, sizeof(DevPtrs)) != 0 ||
+                   std::memcmp(&b->cap_ko, &b->ko, sizeof(KOpts)) != 0)) {
+    (void)hipGraphExecDestroy(b->gexec);
+    (void)hipGraphDestroy(b->graph);
+    b->gexec = nullptr;
+    b->graph = nullptr;
+  }
   if (b->gexec) return TGMM_OK;
+  std::memcpy(&b->cap_P, &b->ws.P, sizeof(DevPtrs));  // padding included, as compared
+  std::memcpy(&b->cap_ko, &b->ko, sizeof(KOpts));
   HIPCHK(hipStreamBeginCapture(b->st, hipStreamCaptureModeThreadLocal));
   hipError_t le = launch_iteration(b->ws.P, b->ko, b->st);
   hipGraph_t g = nullptr;
@@ -503,93 +554,66 @@ extern "C" int tgmm_batch_solve(tgmm_batch* b, int mode) {
       if (!act) break;
     }
   }
+  // the last search's candidates in the reference's slot layout
+  if (iters > 0) HIPCHK(launch_knn_layout(b->ws.P, b->ko, -1, (int)b->max_ng, b->st));
   HIPCHK(hipStreamSynchronize(b->st));
   return TGMM_OK;
 }
 
-// supervoxel assignment r > 0.5 in slot-major order (GMEM.cu:4328-4350)
-static void assign_supervoxels(GaussianMixtureModelCUDA* gm, long long G, long long S, int K,
-                               const float* rnk, long long rpitch, const int* ind, long long g0,
-                               int cap) {
-  for (long long k = 0; k < G; k++) gm[k].supervoxelNum = 0;
-  for (int j = 0; j < K; j++)
-    for (long long s = 0; s < S; s++) {
-      const float r = rnk[j * rpitch + s];
-      const int k = ind[j * rpitch + s];
-      if (r > 0.5f && k >= 0) {
-        GaussianMixtureModelCUDA& g = gm[k - g0];
-        if (g.supervoxelNum < cap) g.supervoxelIdx[g.supervoxelNum++] = (int)s;
-      }
-    }
-}
-
-int batch_fetch_host(tgmm_batch* b, std::vector<double>& gs, std::vector<float>& rnk,
-                     std::vector<int>& ind) {
-  DevPtrs& P = b->ws.P;
-  const long long S = b->ws.sh.Scap, G = b->ws.sh.Gcap;
-  const int K = b->ws.K;
-  gs.resize((size_t)GS_NFIELDS * G);
-  rnk.resize((size_t)K * S);
-  ind.resize((size_t)K * S);
-  HIPCHK(hipMemcpyAsync(gs.data(), P.gs, gs.size() * 8, hipMemcpyDeviceToHost, b->st));
-  HIPCHK(hipMemcpyAsync(rnk.data(), P.rnk, rnk.size() * 4, hipMemcpyDeviceToHost, b->st));
-  HIPCHK(hipMemcpyAsync(ind.data(), P.ind, ind.size() * 4, hipMemcpyDeviceToHost, b->st));
-  HIPCHK(hipMemcpyAsync(b->fh.data(), P.fr, b->fh.size() * sizeof(FrameDev),
-                        hipMemcpyDeviceToHost, b->st));
-  HIPCHK(hipStreamSynchronize(b->st));
-  return TGMM_OK;
-}
-
-void frame_results(tgmm_batch* b, int i, const std::vector<double>& gs,
-                   const std::vector<float>& rnk, const std::vector<int>& ind,
-                   GaussianMixtureModelCUDA* gm, float* rnk_out, int* ind_out, double* ll,
-                   int* iters) {
-  const FrameDev& F = b->fh[i];
-  const long long S = b->ws.sh.Scap, G = b->ws.sh.Gcap;
-  const int K = b->ws.K;
-  if (gm && F.ng > 0) {
-    soa_to_records(gs.data(), F.ng, F.g0, G, gm);
-    for (long long k = 0; k < F.ng; k++) gm[k].fixed = b->fixed_host[F.g0 + k];  // as given
-    if (b->opts.voxel_mode) {  // voxel mode leaves the supervoxel lists as given
-      constexpr int M = TGMM_MAX_SUPERVOXELS_PER_GAUSSIAN;
-      for (long long k = 0; k < F.ng; k++) {
-        const int* d = b->sv_keep[i].data() + (size_t)k * (M + 1);
-        std::memcpy(gm[k].supervoxelIdx, d, sizeof(int) * M);
-        gm[k].supervoxelNum = d[M];
-      }
-    } else {
-      const int cap = b->opts.max_sv_per_gaussian;
-      assign_supervoxels(gm, F.ng, F.ns, K, rnk.data() + F.s0, S, ind.data() + F.s0, F.g0,
-                         cap);
-    }
-  }
-  for (int j = 0; j < K; j++)
-    for (long long s = 0; s < F.ns; s++) {
-      if (rnk_out) rnk_out[j * F.ns + s] = rnk[j * S + F.s0 + s];
-      if (ind_out) {
-        const int k = ind[j * S + F.s0 + s];
-        ind_out[j * F.ns + s] = k >= 0 ? (int)(k - F.g0) : -1;
-      }
-    }
-  if (ll) *ll = F.ll;
-  if (iters) *iters = F.iter - 1;
-}
-
+// Results of every frame: the records are formed on the device (state, fixed
+// flag, r > 0.5 supervoxel assignment) and copied into each frame's gm in one
+// transfer; rnk/ind (optional, host) are copied slot by slot, ind made
+// frame-local; ll and the iteration count come from the frame table.
 extern "C" int tgmm_batch_fetch(tgmm_batch* b, tgmm_frame_desc* frames, int nframes) {
   if (!b || !frames || nframes != b->nframes) {
     tgmm_set_error("tgmm_batch_fetch: bad arguments");
     return TGMM_ERR_ARG;
   }
   HIPCHK(hipSetDevice(b->device));
-  std::vector<double> gs;
-  std::vector<float> rnk;
-  std::vector<int> ind;
-  if (int rc = batch_fetch_host(b, gs, rnk, ind)) return rc;
+  DevPtrs& P = b->ws.P;
+  const long long S = b->ws.sh.Scap;
+  const int K = b->ws.K;
+  HIPCHK(launch_finalize(P, b->ko, b->opts.max_sv_per_gaussian, -1, 0, b->st));
   for (int i = 0; i < nframes; i++) {
+    const FrameDev& F = b->fh[i];
     tgmm_frame_desc& d = frames[i];
-    frame_results(b, i, gs, rnk, ind, d.gm, d.rnk, d.ind, &d.ll, &d.iterations);
+    if (d.gm && F.ng > 0)
+      HIPCHK(hipMemcpyAsync(d.gm, P.rec + F.g0, F.ng * sizeof(GaussianMixtureModelCUDA),
+                            hipMemcpyDeviceToHost, b->st));
+    const size_t w = (size_t)F.ns * 4;
+    if (d.rnk && F.ns > 0)
+      HIPCHK(hipMemcpy2DAsync(d.rnk, w, P.rnk + F.s0, S * 4, w, K, hipMemcpyDeviceToHost, b->st));
+    if (d.ind && F.ns > 0)
+      HIPCHK(hipMemcpy2DAsync(d.ind, w, P.ind + F.s0, S * 4, w, K, hipMemcpyDeviceToHost, b->st));
+  }
+  HIPCHK(hipMemcpyAsync(b->fh.data(), P.fr, b->fh.size() * sizeof(FrameDev),
+                        hipMemcpyDeviceToHost, b->st));
+  int ctl[2] = {0, 0};
+  HIPCHK(hipMemcpyAsync(ctl, P.knn_ctl, sizeof(ctl), hipMemcpyDeviceToHost, b->st));
+  HIPCHK(hipStreamSynchronize(b->st));
+  for (int i = 0; i < nframes; i++) {
+    const FrameDev& F = b->fh[i];
+    tgmm_frame_desc& d = frames[i];
+    if (d.ind && F.g0 > 0)
+      for (long long e = 0; e < (long long)K * F.ns; e++)
+        if (d.ind[e] >= 0) d.ind[e] -= (int)F.g0;
+    d.ll = F.ll;
+    d.iterations = F.iter - 1;
     d.status = TGMM_OK;
   }
+  b->layout_mismatch = ctl[1];
+  return TGMM_OK;
+}
+
+extern "C" int tgmm_batch_stats(tgmm_batch* b, tgmm_batch_stats_t* st) {
+  if (!b || !st) return TGMM_ERR_ARG;
+  HIPCHK(hipSetDevice(b->device));
+  int ctl[2] = {0, 0};
+  HIPCHK(hipMemcpyAsync(ctl, b->ws.P.knn_ctl, sizeof(ctl), hipMemcpyDeviceToHost, b->st));
+  HIPCHK(hipStreamSynchronize(b->st));
+  st->knn_ties_last_iteration = ctl[0];
+  st->layout_mismatches = ctl[1];
+  st->workspace_bytes = b->ws.bytes;
   return TGMM_OK;
 }
 

@@ -47,14 +47,14 @@ struct tgmm_batch {
   int nframes = 0;
   std::vector<tgmm::FrameDev> fh;  // host mirror of the frame table
   std::vector<FrameSize> sizes;
-  std::vector<unsigned char> fixed_host;
   Workspace ws;
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
+  tgmm::DevPtrs cap_P;  // launch arguments the graph was captured with
+  tgmm::KOpts cap_ko;
   int grid_cell = 0;  // knn_cell the kNN grids were last built with
-  // voxel mode: each frame's input supervoxelIdx[16] + supervoxelNum per
-  // Gaussian, returned unchanged by the fetch
-  std::vector<std::vector<int>> sv_keep;
+  long long max_ng = 0;      // most Gaussians of one frame
+  int layout_mismatch = 0;   // supervoxels whose final slot layout found another set
 };
 
 void tgmm_set_error(const char* fmt, ...);
@@ -63,16 +63,15 @@ void ws_free(Workspace* w);
 long long ws_bytes_estimate(const WsShape& sh, int K);
 tgmm::KOpts make_kopts(const tgmm_em_opts& o, const float* scale_reg);
 void records_to_soa(const GaussianMixtureModelCUDA* gm, long long G, long long g0, long long Gcap,
-                    double* gs, unsigned char* fixed);
+                    double* gs, unsigned char* fixed, int* svkeep);
 void soa_to_records(const double* gs, long long G, long long g0, long long Gcap,
                     GaussianMixtureModelCUDA* gm);
+// Places the frames in a workspace and uploads them. With `reuse`, that
+// batch's workspace, stream and graph are kept if the frames fit (else it is
+// destroyed); a new workspace is `headroom` times the size needed (>= 1).
 int batch_build(int device, const std::vector<FrameSrc>& fs, const tgmm_em_opts& o,
-                const float* scale_reg, hipStream_t stream, tgmm_batch** out);
+                const float* scale_reg, hipStream_t stream, tgmm_batch** out,
+                tgmm_batch* reuse = nullptr, double headroom = 1.0);
 int batch_upload(tgmm_batch* b, const std::vector<FrameSrc>& fs);
 int batch_capture(tgmm_batch* b);
-int batch_fetch_host(tgmm_batch* b, std::vector<double>& gs, std::vector<float>& rnk,
-                     std::vector<int>& ind);
-void frame_results(tgmm_batch* b, int i, const std::vector<double>& gs,
-                   const std::vector<float>& rnk, const std::vector<int>& ind,
-                   GaussianMixtureModelCUDA* gm, float* rnk_out, int* ind_out, double* ll,
-                   int* iters);
+

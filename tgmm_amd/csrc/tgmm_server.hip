@@ -35,6 +35,7 @@
 
 #include "../../include/tgmm_em.h"
 #include "tgmm_runtime_internal.h"
+#include "tgmm_staging.h"
 
 using namespace tgmm;
 
@@ -55,6 +56,7 @@ double now_s() {
 }
 
 long long rup(long long x, long long a) { return ((x + a - 1) / a) * a; }
+size_t up256(size_t x) { return (x + 255) & ~size_t(255); }
 
 // first-fit allocator of aligned ranges in [0, cap)
 class RangeAlloc {
@@ -153,11 +155,10 @@ struct Ticket {
   long long id = 0;
   int stream = -1;
   long long N = 0, S = 0, G = 0;
-  // inputs: the caller's arrays (borrowed) or the copies below
+  // inputs: the caller's arrays (borrowed) or a page-locked copy (in_blk)
   const float *q = nullptr, *im = nullptr, *ce = nullptr;
   const long long* pt = nullptr;
-  std::vector<float> q_own, im_own, ce_own;
-  std::vector<long long> pt_own;
+  HostBlock in_blk;
   std::vector<GaussianMixtureModelCUDA> gm_in;  // empty: priors from the stream
   int state = 0;  // 0 queued, 3 staged (ranges held, upload issued), 1 running,
                   // 4 retiring (results being copied back), 2 done
@@ -166,12 +167,9 @@ struct Ticket {
   long long v0 = 0, s0 = 0, sa = 0, g0 = 0, ga = 0, c0 = 0, ncap = 0;
   bool own_g = false;         // the Gaussian range was allocated for this ticket
   hipEvent_t up = nullptr;    // the upload has completed on the copy streams
-  std::vector<long long> beg; // staged supervoxel tables and records
-  std::vector<int> len;
-  std::vector<double> gs_in;
-  std::vector<unsigned char> fx_in;
-  // results: copied into a page-locked buffer on the compute stream at
-  // retirement (state 4 until `down` completes), turned into records at fetch
+  // results: gathered on the device at retirement and copied into a
+  // page-locked buffer (state 4 until `down` completes): the records, then
+  // rnk and frame-local ind
   ResBuf res;
   size_t res_bytes = 0;
   hipEvent_t down = nullptr;
@@ -217,6 +215,11 @@ struct tgmm_server {
   int in_flight = 0;
   long long anon_queued = 0;  // queued frames with no stream
   ResPool pool;
+  // page-locked host sources of the uploads, held until their copies are done
+  static bool host_alloc(void** p, size_t n) { return hipHostMalloc(p, n, 0) == hipSuccess; }
+  static void host_free(void* p) { (void)hipHostFree(p); }
+  HostPool<bool (*)(void**, size_t), void (*)(void*)> hpool{&host_alloc, &host_free};
+  InFlight<hipEvent_t> uploads;
   std::vector<long long> retiring;
   hipEvent_t released = nullptr;  // recorded after the last retirement's copies
   bool release_pending = false;   // uploads must wait for `released`
@@ -344,6 +347,8 @@ extern "C" int tgmm_server_create(int device, const tgmm_server_opts* opts, tgmm
       (er = hipMemsetAsync(P.rnk, 0, (size_t)e.K * S * 4, s->st)) ||
       (er = hipMemsetAsync(P.seg_cnt, 0, (G + 1) * 4, s->st)) ||
       (er = hipMemsetAsync(P.cell_cnt, 0, (sh.Ccap + 1) * 4, s->st)) ||
+      (er = hipMemsetAsync(P.knn_ctl, 0, 64 * 4, s->st)) ||
+      (er = hipMemsetAsync(P.svkeep, 0, (size_t)17 * G * 4, s->st)) ||
       (er = hipMemcpyAsync(P.fr, s->fh.data(), sh.Fcap * sizeof(FrameDev),
                            hipMemcpyHostToDevice, s->st)) ||
       (er = hipStreamSynchronize(s->st)))
@@ -366,26 +371,33 @@ extern "C" int tgmm_server_submit(tgmm_server* s, int stream_id, const tgmm_fram
   t.stream = stream_id;
   t.N = d->n_voxels;
   t.S = d->n_supervoxels;
+  for (long long j = 0; j < t.S; j++)
+    if (d->sv_ptr[j] < 0 || d->sv_ptr[j + 1] < d->sv_ptr[j] || d->sv_ptr[j + 1] > t.N) {
+      tgmm_set_error("tgmm_server_submit: supervoxel pointer array not monotone in [0, N]");
+      return TGMM_ERR_ARG;
+    }
   if (s->o.borrow_inputs) {
     t.q = d->query;
     t.im = d->img;
     t.ce = d->centroids;
     t.pt = d->sv_ptr;
-  } else {
-    t.q_own.assign(d->query, d->query + 3 * t.N);
-    t.im_own.assign(d->img, d->img + t.N);
-    t.ce_own.assign(d->centroids, d->centroids + 3 * t.S);
-    t.pt_own.assign(d->sv_ptr, d->sv_ptr + t.S + 1);
-    t.q = t.q_own.data();
-    t.im = t.im_own.data();
-    t.ce = t.ce_own.data();
-    t.pt = t.pt_own.data();
-  }
-  for (long long j = 0; j < t.S; j++)
-    if (t.pt[j] < 0 || t.pt[j + 1] < t.pt[j] || t.pt[j + 1] > t.N) {
-      tgmm_set_error("tgmm_server_submit: supervoxel pointer array not monotone in [0, N]");
-      return TGMM_ERR_ARG;
+  } else {  // a page-locked copy, uploaded from and returned after its copies
+    const size_t oq = 0, oi = up256(12 * (size_t)t.N), oc = oi + up256(4 * (size_t)t.N),
+                 op = oc + up256(12 * (size_t)t.S), total = op + 8 * (size_t)(t.S + 1);
+    if (!s->hpool.get(total, &t.in_blk)) {
+      tgmm_set_error("tgmm_server_submit: %zu bytes of page-locked memory not available", total);
+      return TGMM_ERR_OOM;
     }
+    char* b = (char*)t.in_blk.p;
+    std::memcpy(b + oq, d->query, 12 * (size_t)t.N);
+    std::memcpy(b + oi, d->img, 4 * (size_t)t.N);
+    std::memcpy(b + oc, d->centroids, 12 * (size_t)t.S);
+    std::memcpy(b + op, d->sv_ptr, 8 * (size_t)(t.S + 1));
+    t.q = (const float*)(b + oq);
+    t.im = (const float*)(b + oi);
+    t.ce = (const float*)(b + oc);
+    t.pt = (const long long*)(b + op);
+  }
   if (d->gm) {
     t.G = d->n_gaussians;
     t.gm_in.assign(d->gm, d->gm + t.G);
@@ -393,6 +405,7 @@ extern "C" int tgmm_server_submit(tgmm_server* s, int stream_id, const tgmm_fram
     auto it = s->streams.find(stream_id);
     if (stream_id < 0 || it == s->streams.end() || it->second.G_next < 0) {
       tgmm_set_error("tgmm_server_submit: no Gaussians given and no earlier frame of the stream");
+      s->hpool.put(t.in_blk);
       return TGMM_ERR_ARG;
     }
     t.G = it->second.G_next;  // priors come from the stream's previous posterior
@@ -402,6 +415,7 @@ extern "C" int tgmm_server_submit(tgmm_server* s, int stream_id, const tgmm_fram
   if (t.N > sh.Vcap || rup(std::max<long long>(t.S, 1), kAlign) > sh.Scap ||
       rup(t.G + 1, kAlign) > sh.Gcap) {
     tgmm_set_error("frame exceeds the server's capacity (N=%lld S=%lld G=%lld)", t.N, t.S, t.G);
+    s->hpool.put(t.in_blk);
     return TGMM_ERR_LIMIT;
   }
   t.t_submit = now_s();
@@ -532,29 +546,42 @@ static int stage(tgmm_server* s, Ticket& t) {
                           cs[2 % nu]));
     HIPCHK(hipMemcpyAsync(P.vimg + t.v0, t.im, t.N * 4, hipMemcpyHostToDevice, cs[3 % nu]));
   }
-  t.beg.resize(std::max<long long>(t.S, 1));
-  t.len.resize(std::max<long long>(t.S, 1));
+  // the supervoxel tables and, for given records, the Gaussian state, fixed
+  // flags and supervoxel lists, packed into one page-locked block
+  const size_t ob = 0, ol = up256(8 * (size_t)t.S), og = ol + up256(4 * (size_t)t.S);
+  const size_t nG = t.own_g ? (size_t)t.G : 0;
+  const size_t of = og + up256(8 * GS_NFIELDS * nG), ok = of + up256(nG);
+  const size_t total = ok + 68 * nG;
+  HostBlock tab;
+  if (!s->hpool.get(total, &tab)) {
+    tgmm_set_error("server: %zu bytes of page-locked memory not available", total);
+    release_ranges(s, t);
+    return TGMM_ERR_OOM;
+  }
+  char* tb = (char*)tab.p;
+  long long* beg = (long long*)(tb + ob);
+  int* len = (int*)(tb + ol);
   for (long long j = 0; j < t.S; j++) {
-    t.beg[j] = t.v0 + t.pt[j];
-    t.len[j] = (int)(t.pt[j + 1] - t.pt[j]);
+    beg[j] = t.v0 + t.pt[j];
+    len[j] = (int)(t.pt[j + 1] - t.pt[j]);
   }
   if (t.S > 0) {
     for (int a = 0; a < 3; a++)
       HIPCHK(hipMemcpyAsync(P.sv_cent + a * S + t.s0, t.ce + a * t.S, t.S * 4,
                             hipMemcpyHostToDevice, cs[0]));
-    HIPCHK(hipMemcpyAsync(P.sv_beg + t.s0, t.beg.data(), t.S * 8, hipMemcpyHostToDevice,
-                          cs[0]));
-    HIPCHK(hipMemcpyAsync(P.sv_len + t.s0, t.len.data(), t.S * 4, hipMemcpyHostToDevice,
-                          cs[0]));
+    HIPCHK(hipMemcpyAsync(P.sv_beg + t.s0, beg, t.S * 8, hipMemcpyHostToDevice, cs[0]));
+    HIPCHK(hipMemcpyAsync(P.sv_len + t.s0, len, t.S * 4, hipMemcpyHostToDevice, cs[0]));
   }
-  if (t.own_g) {
-    t.gs_in.resize((size_t)GS_NFIELDS * std::max<long long>(t.G, 1));
-    t.fx_in.resize(std::max<long long>(t.G, 1));
-    records_to_soa(t.gm_in.data(), t.G, 0, t.G, t.gs_in.data(), t.fx_in.data());
+  if (t.own_g && t.G > 0) {
+    double* gs = (double*)(tb + og);
+    unsigned char* fx = (unsigned char*)(tb + of);
+    int* kp = (int*)(tb + ok);
+    records_to_soa(t.gm_in.data(), t.G, 0, t.G, gs, fx, kp);
     for (int fi = 0; fi < GS_NFIELDS; fi++)
-      HIPCHK(hipMemcpyAsync(P.gs + fi * G + t.g0, t.gs_in.data() + fi * t.G, t.G * 8,
-                            hipMemcpyHostToDevice, cs[0]));
-    HIPCHK(hipMemcpyAsync(P.fixed + t.g0, t.fx_in.data(), t.G, hipMemcpyHostToDevice, cs[0]));
+      HIPCHK(hipMemcpyAsync(P.gs + fi * G + t.g0, gs + fi * t.G, t.G * 8, hipMemcpyHostToDevice,
+                            cs[0]));
+    HIPCHK(hipMemcpyAsync(P.fixed + t.g0, fx, t.G, hipMemcpyHostToDevice, cs[0]));
+    HIPCHK(hipMemcpyAsync(P.svkeep + 17 * t.g0, kp, 68 * t.G, hipMemcpyHostToDevice, cs[0]));
   }
   if (!t.up) HIPCHK(hipEventCreateWithFlags(&t.up, hipEventDisableTiming));
   for (int i = 1; i < nu; i++) {
@@ -562,6 +589,12 @@ static int stage(tgmm_server* s, Ticket& t) {
     HIPCHK(hipStreamWaitEvent(cs[0], s->up_join[i], 0));
   }
   HIPCHK(hipEventRecord(t.up, cs[0]));
+  // the blocks read by these copies return to the pool once t.up completes
+  s->uploads.hold(t.id, tab, t.up);
+  if (t.in_blk.p) {
+    s->uploads.hold(t.id, t.in_blk, t.up);
+    t.in_blk = HostBlock{};
+  }
   t.state = 3;
   return TGMM_OK;
 }
@@ -676,38 +709,33 @@ static int admit_staged(tgmm_server* s) {
   return TGMM_OK;
 }
 
-// A retired frame's results gathered into one contiguous block: the Gaussian
-// fields [GS_NFIELDS][G] (f64), the fixed flags [G rounded up to 8] (u8), the
-// responsibilities [K][S] (f32) and the candidate indices [K][S] (i32, made
-// frame-local; -1 stays -1).
-__global__ void __launch_bounds__(256) k_gather_results(const double* __restrict__ gs,
-    const unsigned char* __restrict__ fixed, const float* __restrict__ rnk,
-    const int* __restrict__ ind, long long Gcap, long long Scap, long long g0, long long G,
-    long long s0, long long S, int K, char* __restrict__ out) {
-  const long long gsb = (long long)GS_NFIELDS * G * 8, fxb = (G + 7) / 8 * 8, rb = K * S * 4;
+// A retired frame's results gathered into one contiguous block: its records
+// [G] (288 bytes each, formed by k_finalize), the responsibilities [K][S]
+// (f32) and the candidate indices [K][S] (i32, made frame-local; -1 stays
+// -1).
+__global__ void __launch_bounds__(256) k_gather_results(const GaussianMixtureModelCUDA* rec,
+    const float* __restrict__ rnk, const int* __restrict__ ind, long long Scap, long long g0,
+    long long G, long long s0, long long S, int K, char* __restrict__ out) {
+  const long long recb = G * (long long)sizeof(GaussianMixtureModelCUDA), rb = K * S * 4;
   const long long step = (long long)gridDim.x * blockDim.x;
   const long long i0 = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   switch (blockIdx.y) {
-    case 0:
-      for (long long i = i0; i < GS_NFIELDS * G; i += step) {
-        const long long f = i / G, k = i - f * G;
-        ((double*)out)[i] = gs[f * Gcap + g0 + k];
-      }
+    case 0: {
+      const double* src = (const double*)(rec + g0);
+      for (long long i = i0; i < recb / 8; i += step) ((double*)out)[i] = src[i];
       break;
+    }
     case 1:
-      for (long long i = i0; i < G; i += step) ((unsigned char*)(out + gsb))[i] = fixed[g0 + i];
-      break;
-    case 2:
       for (long long i = i0; i < K * S; i += step) {
         const long long j = i / S, v = i - j * S;
-        ((float*)(out + gsb + fxb))[i] = rnk[j * Scap + s0 + v];
+        ((float*)(out + recb))[i] = rnk[j * Scap + s0 + v];
       }
       break;
     default:
       for (long long i = i0; i < K * S; i += step) {
         const long long j = i / S, v = i - j * S;
         const int k = ind[j * Scap + s0 + v];
-        ((int*)(out + gsb + fxb + rb))[i] = k >= 0 ? k - (int)g0 : k;
+        ((int*)(out + recb + rb))[i] = k >= 0 ? k - (int)g0 : k;
       }
   }
 }
@@ -723,19 +751,28 @@ static int retire(tgmm_server* s, int f) {
   Ticket& t = s->tickets[id];
   DevPtrs& P = s->ws.P;
   hipStream_t st = s->st;
-  const long long S = s->ws.sh.Scap, G = s->ws.sh.Gcap;
+  const long long S = s->ws.sh.Scap;
   const int K = s->ws.K;
-  const size_t gsb = (size_t)GS_NFIELDS * t.G * 8, fxb = rup(t.G, 8),
-               rb = (size_t)K * t.S * 4;
-  const size_t bytes = gsb + fxb + 2 * rb;
+  const size_t recb = (size_t)t.G * sizeof(GaussianMixtureModelCUDA), rb = (size_t)K * t.S * 4;
+  const size_t bytes = recb + 2 * rb;
   if (!s->pool.get(bytes + 64, &t.res)) {
     tgmm_set_error("result buffers of %zu bytes not available", bytes);
     return TGMM_ERR_OOM;
   }
-  const long long mx = std::max<long long>((long long)GS_NFIELDS * t.G, (long long)K * t.S);
+  // the last search's candidates in the reference's slot layout, then the
+  // records with the r > 0.5 supervoxel assignment (a stream's next frame
+  // starts from these lists)
+  DevPtrs Q = P;
+  Q.s_lo = t.s0;
+  Q.Sext = t.sa;
+  Q.g_lo = t.g0;
+  Q.Gext = t.ga;
+  HIPCHK(launch_knn_layout(Q, s->ko, f, (int)t.G, st));
+  HIPCHK(launch_finalize(Q, s->ko, s->o.em.max_sv_per_gaussian, f, t.stream >= 0 ? 1 : 0, st));
+  const long long mx = std::max<long long>((long long)(recb / 8), (long long)K * t.S);
   const unsigned gx = (unsigned)std::max(1LL, std::min(2048LL, (mx + 255) / 256));
-  k_gather_results<<<dim3(gx, 4), 256, 0, st>>>(P.gs, P.fixed, P.rnk, P.ind, G, S, t.g0, t.G,
-                                                 t.s0, t.S, K, (char*)t.res.d);
+  k_gather_results<<<dim3(gx, 3), 256, 0, st>>>(P.rec, P.rnk, P.ind, S, t.g0, t.G, t.s0, t.S, K,
+                                                 (char*)t.res.d);
   HIPCHK(hipGetLastError());
   t.res_bytes = bytes;
   s->to_download.push_back(t.id);
@@ -787,6 +824,19 @@ static int queue_downloads(tgmm_server* s) {
   return TGMM_OK;
 }
 
+// Upload blocks whose copies have completed go back to the pool.
+static int reclaim_uploads(tgmm_server* s) {
+  const int rc = s->uploads.reclaim(s->hpool, [](hipEvent_t e) {
+    const hipError_t q = hipEventQuery(e);
+    return q == hipSuccess ? 1 : q == hipErrorNotReady ? 0 : -1;
+  });
+  if (rc < 0) {
+    tgmm_set_error("server: an upload failed");
+    return TGMM_ERR_DEVICE;
+  }
+  return TGMM_OK;
+}
+
 // Retiring tickets whose results have landed become done.
 static int collect(tgmm_server* s, bool block) {
   for (size_t i = 0; i < s->retiring.size();) {
@@ -799,18 +849,10 @@ static int collect(tgmm_server* s, bool block) {
     HIPCHK(q);
     t.state = 2;
     t.t_done = now_s();
-    std::vector<float>().swap(t.q_own);
-    std::vector<float>().swap(t.im_own);
-    std::vector<float>().swap(t.ce_own);
-    std::vector<long long>().swap(t.pt_own);
     std::vector<GaussianMixtureModelCUDA>().swap(t.gm_in);
-    std::vector<long long>().swap(t.beg);
-    std::vector<int>().swap(t.len);
-    std::vector<double>().swap(t.gs_in);
-    std::vector<unsigned char>().swap(t.fx_in);
     s->retiring.erase(s->retiring.begin() + i);
   }
-  return TGMM_OK;
+  return reclaim_uploads(s);
 }
 
 // Wait for the iteration launched by the previous step and take the frame
@@ -925,32 +967,16 @@ extern "C" int tgmm_server_fetch(tgmm_server* s, long long ticket, tgmm_frame_de
     return TGMM_ERR_BUSY;
   }
   const int K = s->ws.K;
-  const size_t gsb = (size_t)GS_NFIELDS * t.G * 8, fxb = rup(t.G, 8),
-               rb = (size_t)K * t.S * 4;
+  const size_t recb = (size_t)t.G * sizeof(GaussianMixtureModelCUDA), rb = (size_t)K * t.S * 4;
   const char* base = (const char*)t.res.h;
-  const unsigned char* fx = (const unsigned char*)(base + gsb);
-  const float* rnk = (const float*)(base + gsb + fxb);
-  const int* ind = (const int*)(base + gsb + fxb + rb);
-  std::vector<GaussianMixtureModelCUDA> gm(std::max<long long>(t.G, 1));
-  soa_to_records((const double*)base, t.G, 0, t.G, gm.data());
-  for (long long k = 0; k < t.G; k++) {
-    gm[k].fixed = fx[k];
-    gm[k].supervoxelNum = 0;
+  if (d->gm) std::memcpy(d->gm, base, recb);
+  if (d->rnk) std::memcpy(d->rnk, base + recb, rb);
+  if (d->ind) std::memcpy(d->ind, base + recb + rb, rb);
+  // the ticket's upload blocks are back in the pool before its event goes
+  if (s->uploads.held_by(ticket)) {
+    HIPCHK(hipEventSynchronize(t.up));
+    if (int rc = reclaim_uploads(s)) return rc;
   }
-  // supervoxels assigned with r > 0.5, slot-major, at most max_sv_per_gaussian
-  // (GMEM.cu:4328-4350)
-  const int cap = s->o.em.max_sv_per_gaussian;
-  for (int j = 0; j < K; j++)
-    for (long long sv = 0; sv < t.S; sv++) {
-      const int k = ind[j * t.S + sv];
-      if (rnk[j * t.S + sv] > 0.5f && k >= 0) {
-        GaussianMixtureModelCUDA& g = gm[k];
-        if (g.supervoxelNum < cap) g.supervoxelIdx[g.supervoxelNum++] = (int)sv;
-      }
-    }
-  if (d->gm) std::memcpy(d->gm, gm.data(), t.G * sizeof(GaussianMixtureModelCUDA));
-  if (d->rnk) std::memcpy(d->rnk, rnk, rb);
-  if (d->ind) std::memcpy(d->ind, ind, rb);
   s->pool.put(t.res);
   if (t.down) (void)hipEventDestroy(t.down);
   d->ll = t.ll;
@@ -1012,11 +1038,14 @@ extern "C" int tgmm_server_destroy(tgmm_server* s) {
   if (s->dst) (void)hipStreamSynchronize(s->dst);
   if (s->gexec) (void)hipGraphExecDestroy(s->gexec);
   if (s->graph) (void)hipGraphDestroy(s->graph);
+  s->uploads.drain(s->hpool, [](hipEvent_t e) { (void)hipEventSynchronize(e); });
   for (auto& kv : s->tickets) {
     if (kv.second.up) (void)hipEventDestroy(kv.second.up);
     if (kv.second.down) (void)hipEventDestroy(kv.second.down);
     s->pool.put(kv.second.res);
+    s->hpool.put(kv.second.in_blk);
   }
+  s->hpool.clear();
   if (s->released) (void)hipEventDestroy(s->released);
   for (hipEvent_t e : s->ev_it)
     if (e) (void)hipEventDestroy(e);

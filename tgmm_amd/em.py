@@ -110,6 +110,11 @@ class ServerStats(ctypes.Structure):
                 ("iterate_gpu_s", ctypes.c_double)]
 
 
+class BatchStats(ctypes.Structure):
+    _fields_ = [("knn_ties_last_iteration", ctypes.c_int), ("layout_mismatches", ctypes.c_int),
+                ("workspace_bytes", ctypes.c_longlong)]
+
+
 class Tiles(ctypes.Structure):
     """Launch shapes of the tunable kernels (tgmm_tiles). Results are bitwise
     the same for every valid choice."""
@@ -255,6 +260,13 @@ def load():
                                          ctypes.POINTER(ctypes.c_double),
                                          ctypes.POINTER(ctypes.c_double)]
     L.tgmm_batch_destroy.argtypes = [_vp]
+    L.tgmm_batch_stats.argtypes = [_vp, ctypes.POINTER(BatchStats)]
+    L.tgmm_GMEMinitializeMemory.argtypes = [_pp, _vp, _pp, _vp, _vp, ctypes.c_longlong, _pp,
+                                            _pp, _pp]
+    L.tgmm_GMEMreleaseMemory.argtypes = [_pp] * 5
+    L.tgmm_selectCUDAdeviceMaxMemory.argtypes = [ctypes.POINTER(ctypes.c_int)]
+    L.tgmm_setDeviceCUDA.argtypes = [ctypes.c_int]
+    L.tgmm_uploadScaleCUDA.argtypes = [_vp]
     L.tgmm_batch_set_tiles.argtypes = [_vp, ctypes.POINTER(Tiles)]
     L.tgmm_batch_get_tiles.argtypes = [_vp, ctypes.POINTER(Tiles)]
     L.tgmm_batch_autotune.argtypes = [_vp, ctypes.c_int, ctypes.POINTER(Tiles),
@@ -599,25 +611,39 @@ class Batch:
     def solve(self, mode=SOLVE_GRAPH):
         _check(self.L.tgmm_batch_solve(self.h, mode))
 
-    def fetch(self):
+    def fetch(self, records_only=False, into=None):
         """List of dicts: gm (records), rnk/ind (slot-major [K, S]; [K, N] in
-        voxel mode), ll, iterations."""
+        voxel mode; None with records_only), ll, iterations. `into`: a list of
+        record arrays (one per frame) to write the records into."""
         K = self.opts.K
         outs = []
         for i, f in enumerate(self.frames):
             S = (int(np.asarray(f.img).shape[0]) if self.voxel
                  else int(np.asarray(f.sv_ptr).shape[0] - 1))
-            gm = np.zeros(int(np.asarray(f.gm).shape[0]), GMM_DTYPE)
-            rnk = np.zeros(K * S, np.float32)
-            ind = np.zeros(K * S, np.int32)
+            gm = (into[i] if into is not None
+                  else np.zeros(int(np.asarray(f.gm).shape[0]), GMM_DTYPE))
             d = self.desc[i]
-            d.gm, d.rnk, d.ind = gm.ctypes.data, rnk.ctypes.data, ind.ctypes.data
+            d.gm = gm.ctypes.data
+            if records_only:
+                rnk = ind = None
+                d.rnk = d.ind = None
+            else:
+                rnk = np.zeros(K * S, np.float32)
+                ind = np.zeros(K * S, np.int32)
+                d.rnk, d.ind = rnk.ctypes.data, ind.ctypes.data
             outs.append(dict(gm=gm, rnk=rnk, ind=ind))
         _check(self.L.tgmm_batch_fetch(self.h, self.desc, len(self.frames)))
         for i, o in enumerate(outs):
             o["ll"] = self.desc[i].ll
             o["iterations"] = self.desc[i].iterations
         return outs
+
+    def stats(self):
+        """kNN ties in the last iteration, final-layout mismatches (0) and
+        workspace bytes (tgmm_batch_stats)."""
+        st = BatchStats()
+        _check(self.L.tgmm_batch_stats(self.h, ctypes.byref(st)))
+        return {k: getattr(st, k) for k, _ in BatchStats._fields_}
 
     def set_tiles(self, tiles):
         t = tiles if isinstance(tiles, Tiles) else Tiles.from_dict(tiles)
