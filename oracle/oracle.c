@@ -310,6 +310,7 @@ void orc_knn(long long S, const float* cent, int G,
     const float q0 = cent[s], q1 = cent[s + S], q2 = cent[s + 2 * S];
     for (int b0 = 0; b0 < G; b0 += 256) {
       const int n = G - b0 < 256 ? G - b0 : 256;
+      int any = 0;
       for (int u = 0; u < n; u++) {
         const float a = (q0 - mf[b0 + u]) * s0;
         const float b = (q1 - mf[(size_t)G + b0 + u]) * s1;
@@ -317,7 +318,9 @@ void orc_knn(long long S, const float* cent, int G,
         float t = a * a;
         t = t + b * b;
         d[u] = t + e * e;
+        any |= !(d[u] > thr);
       }
+      if (!any) continue; /* nothing in this block can pass */
       for (int u = 0; u < n; u++) {
         if (d[u] > thr) continue; /* finite: the partial sums are not above it either */
         const int k = b0 + u;
