@@ -2,10 +2,11 @@
 // (supervoxel and voxel mode), of the reference header
 // nmeth.3036-S2/src/UtilsCUDA/GMEMupdateCUDA.h.
 //
-// The tracker includes this file where it included GMEMupdateCUDA.h and links
+// The tracker includes this file where it included GMEMupdateCUDA.h (and
+// tgmm_utils_knn_compat.hpp where it included UtilsCUDA/knnCuda.h) and links
 // libtgmm_em.so instead of the CUDA objects. The inline functions below carry
 // the reference's names and parameter lists (GMEMupdateCUDA.h:82, 87-98,
-// 116-127, 132-144) and its error behaviour: print a message and exit(), with the
+// 116-144) and its error behaviour: print a message and exit(), with the
 // exit codes the reference uses (0 for the size limits, GMEM.cu:3992-4007;
 // 3 when the regularisation constants were never set, GMEM.cu:4217-4222;
 // EXIT_FAILURE for runtime errors, book.h:20-25).
@@ -108,6 +109,20 @@ inline void calculateLocalLikelihood(std::vector<double>& localLikelihood,
                                     indCUDA, vecGMtemp, labelListPtrCUDA, query_nb, ref_nb,
                                     numLabels),
       "calculateLocalLikelihood");
+}
+
+inline void GMEMinitializeMemory(float** queryCUDA, float* query, float** imgDataCUDA,
+                                 float* imgData, float* scale, long long int query_nb,
+                                 float** rnkCUDA, int** indCUDA, int** indCUDAtr) {
+  tgmm_compat::check(tgmm_GMEMinitializeMemory(queryCUDA, query, imgDataCUDA, imgData, scale,
+                                               query_nb, rnkCUDA, indCUDA, indCUDAtr),
+                     "GMEMinitializeMemory");
+}
+
+inline void GMEMreleaseMemory(float** queryCUDA, float** imgDataCUDA, float** rnkCUDA,
+                              int** indCUDA, int** indCUDAtr) {
+  tgmm_compat::check(tgmm_GMEMreleaseMemory(queryCUDA, imgDataCUDA, rnkCUDA, indCUDA, indCUDAtr),
+                     "GMEMreleaseMemory");
 }
 
 inline void GMEMreleaseMemoryWithSupervoxels(float** queryCUDA, float** imgDataCUDA,

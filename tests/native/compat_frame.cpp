@@ -1,7 +1,12 @@
-// compat_frame.cpp -- one frame through the reference's own C++ names, as the
-// tracker calls them (mainTrackingGaussianMixtureModel.cpp:1065-1105, 1336,
-// 1935-1937; lineageHyperTree.cpp:2568), with include/tgmm_em_compat.hpp and
-// include/tgmm_knn_compat.hpp in place of GMEMupdateCUDA.h and knnCuda.h.
+// compat_frame.cpp -- one frame through the reference's own C++ names, in
+// the tracker's order (mainTrackingGaussianMixtureModel.cpp:207 device
+// selection, :778 setDeviceCUDA, :1065-1105 EM init and solve, :1336 local
+// likelihood, :1935-1937 release; lineageHyperTree.cpp:2568 supervoxel
+// neighbours), then the voxel-mode EM (GMEMinitializeMemory,
+// GMEMvariationalInferenceCUDA, GMEMreleaseMemory, GMEMupdateCUDA.h:87-91,
+// 129-139), with include/tgmm_em_compat.hpp, include/tgmm_utils_knn_compat.hpp
+// and include/tgmm_knn_compat.hpp in place of GMEMupdateCUDA.h,
+// UtilsCUDA/knnCuda.h and knnCUDA/knnCuda.h.
 // TEST PROGRAM: tests/test_gpu_compat.py writes the input file, runs this
 // program as a child process and compares its output with the Python path.
 //
@@ -13,7 +18,9 @@
 //         int64 list_ptr[n_lists+1]; int32 list_idx[n_entries];
 //         double lambda_min, lambda_max, max_excentricity
 // output: records[G] after the EM; double local_ll[n_lists];
-//         int32 knn_ind[11 S]; float knn_dist[11 S]
+//         int32 knn_ind[11 S]; float knn_dist[11 S];
+//         records[G] after the voxel-mode EM from the same inputs;
+//         int32 device (selectCUDAdeviceMaxMemory)
 #include <cstdio>
 #include <cstdlib>
 #include <vector>
@@ -29,6 +36,7 @@ double regularizePrecisionMatrixConstants::maxExcentricity = -1.0;
 
 #include "tgmm_em_compat.hpp"
 #include "tgmm_knn_compat.hpp"
+#include "tgmm_utils_knn_compat.hpp"
 
 template <typename T>
 static void rd(FILE* f, T* p, long long n) {
@@ -66,6 +74,11 @@ int main(int argc, char** argv) {
   regularizePrecisionMatrixConstants::lambdaMax = lam[1];
   regularizePrecisionMatrixConstants::maxExcentricity = lam[2];
 
+  // main.cpp:207, 778
+  const int devCUDA = selectCUDAdeviceMaxMemory();
+  if (devCUDA < 0) return 4;
+  setDeviceCUDA(devCUDA);
+  const std::vector<GaussianMixtureModelCUDA> gm_in = gm;
   // main.cpp:1065-1070
   float *queryCUDA = nullptr, *imgDataCUDA = nullptr, *rnkCUDA = nullptr,
         *centroidLabelPositionCUDA = nullptr;
@@ -80,7 +93,8 @@ int main(int argc, char** argv) {
   GMEMvariationalInferenceCUDAWithSupervoxels(queryCUDA, imgDataCUDA, rnkCUDA, nullptr, indCUDA,
                                               indCUDAtr, centroidLabelPositionCUDA,
                                               labelListPtrCUDA, gm.data(), N, (int)G,
-                                              (unsigned short)S, max_iter, -1.0, 0, 0, true);
+                                              (unsigned short)S, max_iter, -1.0, devCUDA, 0,
+                                              true);
   // main.cpp:1336-1339
   std::vector<std::vector<int> > lists(L);
   for (long long i = 0; i < L; i++) lists[i].assign(lidx.begin() + lptr[i], lidx.begin() + lptr[i + 1]);
@@ -97,12 +111,25 @@ int main(int argc, char** argv) {
   GMEMreleaseMemoryWithSupervoxels(&queryCUDA, &imgDataCUDA, &rnkCUDA, &indCUDA, &indCUDAtr,
                                    &centroidLabelPositionCUDA, &labelListPtrCUDA);
 
+  // the voxel-mode EM on the same voxels and initial records
+  std::vector<GaussianMixtureModelCUDA> gv = gm_in;
+  GMEMinitializeMemory(&queryCUDA, query.data(), &imgDataCUDA, img.data(), scale.data(), N,
+                       &rnkCUDA, &indCUDA, &indCUDAtr);
+  uploadScaleCUDA(scale.data());
+  copyScaleToDvice(scale.data());
+  GMEMvariationalInferenceCUDA(queryCUDA, imgDataCUDA, rnkCUDA, nullptr, indCUDA, indCUDAtr,
+                               gv.data(), N, (int)G, max_iter, -1.0, devCUDA, 0, true);
+  GMEMreleaseMemory(&queryCUDA, &imgDataCUDA, &rnkCUDA, &indCUDA, &indCUDAtr);
+  if (queryCUDA || imgDataCUDA || rnkCUDA || indCUDA || indCUDAtr) return 5;
+
   FILE* out = std::fopen(argv[2], "wb");
   if (!out) return 2;
   std::fwrite(gm.data(), sizeof(GaussianMixtureModelCUDA), gm.size(), out);
   std::fwrite(local.data(), sizeof(double), local.size(), out);
   std::fwrite(kind.data(), sizeof(int), kind.size(), out);
   std::fwrite(kdist.data(), sizeof(float), kdist.size(), out);
+  std::fwrite(gv.data(), sizeof(GaussianMixtureModelCUDA), gv.size(), out);
+  std::fwrite(&devCUDA, sizeof(int), 1, out);
   std::fclose(out);
   return 0;
 }

@@ -153,45 +153,27 @@ def test_generator_is_deterministic():
 
 
 def test_compat_header_compiles_and_links(tmp_path):
-    """include/tgmm_em_compat.hpp and include/tgmm_knn_compat.hpp give the
-    reference's C++ names: a tracker
-    translation unit that calls each of them compiles and links against the
-    library. The tracker's constants.h is stood in for by a minimal struct
-    with the same static members (constants.h:19-45)."""
+    """The tracker's drop-in links: tests/native/link_tracker.cpp calls every
+    function the tracker takes from GMEMupdateCUDA.h, UtilsCUDA/knnCuda.h
+    (selectCUDAdeviceMaxMemory, setDeviceCUDA, uploadScaleCUDA) and
+    knnCUDA/knnCuda.h through include/tgmm_em_compat.hpp,
+    include/tgmm_utils_knn_compat.hpp and include/tgmm_knn_compat.hpp, and is
+    linked against libtgmm_em.so alone (an unresolved reference in the
+    program fails the link). Run without arguments it calls nothing."""
     import shutil
     import subprocess
     gxx = shutil.which("g++")
     if gxx is None:
         pytest.skip("g++ not available")
-    (tmp_path / "constants.h").write_text(
-        "struct regularizePrecisionMatrixConstants {\n"
-        "  static double lambdaMin, lambdaMax, maxExcentricity;\n};\n"
-        "double regularizePrecisionMatrixConstants::lambdaMin = -1.0;\n"
-        "double regularizePrecisionMatrixConstants::lambdaMax = -1.0;\n"
-        "double regularizePrecisionMatrixConstants::maxExcentricity = -1.0;\n")
-    (tmp_path / "tracker.cpp").write_text(
-        '#include "constants.h"\n#include "tgmm_em_compat.hpp"\n'
-        '#include "tgmm_knn_compat.hpp"\n'
-        "int main(int argc, char**) {\n"
-        "  if (argc < 5) return 0;  // compiled and linked, never run\n"
-        "  float *q = 0, *im = 0, *r = 0, *c = 0; int *i = 0, *it = 0; long long *p = 0;\n"
-        "  float sc[3] = {1, 1, 5};\n"
-        "  GaussianMixtureModelCUDA* g = 0;\n"
-        "  GMEMinitializeMemoryWithSupervoxels(&q, q, &im, im, sc, 0, &r, &i, &it, 0, &c, c,\n"
-        "                                      &p, p);\n"
-        "  copyScaleToDvice(sc);\n"
-        "  GMEMvariationalInferenceCUDAWithSupervoxels(q, im, r, 0, i, it, c, p, g, 0, 0, 0,\n"
-        "                                              10, 1e-5, 0, 0, true);\n"
-        "  GMEMvariationalInferenceCUDA(q, im, r, 0, i, it, g, 0, 0, 10, 1e-5, 0, 0, true);\n"
-        "  std::vector<double> ll; std::vector<std::vector<int> > lists(2);\n"
-        "  calculateLocalLikelihood(ll, lists, q, im, r, i, g, p, 0, 0, 0);\n"
-        "  int nn = knnCUDA_(i, r, q, c, 0, 0, maxKNN, sc, 0);\n"
-        "  (void)nn;\n"
-        "  GMEMreleaseMemoryWithSupervoxels(&q, &im, &r, &i, &it, &c, &p);\n"
-        "  return 0;\n}\n")
     lib = os.path.join(ROOT, "tgmm_amd")
-    cmd = [gxx, "-std=c++11", "-I", str(tmp_path), "-I", os.path.join(ROOT, "include"),
-           str(tmp_path / "tracker.cpp"), "-L", lib, "-ltgmm_em", "-Wl,-rpath," + lib,
-           "-Wl,--unresolved-symbols=ignore-in-shared-libs", "-o", str(tmp_path / "tracker")]
+    exe = str(tmp_path / "link_tracker")
+    cmd = [gxx, "-std=c++11", "-I", os.path.join(ROOT, "include"),
+           os.path.join(ROOT, "tests", "native", "link_tracker.cpp"), "-L", lib, "-ltgmm_em",
+           "-Wl,-rpath," + lib, "-Wl,--unresolved-symbols=ignore-in-shared-libs", "-o", exe]
     r = subprocess.run(cmd, capture_output=True, text=True)
     assert r.returncode == 0, r.stderr
+    nm = subprocess.run(["nm", "-C", "--undefined-only", exe], capture_output=True, text=True)
+    for name in ("tgmm_selectCUDAdeviceMaxMemory", "tgmm_setDeviceCUDA", "tgmm_uploadScaleCUDA",
+                 "tgmm_GMEMinitializeMemory", "tgmm_GMEMreleaseMemory",
+                 "tgmm_GMEMvariationalInferenceCUDAWithSupervoxels", "tgmm_knnCUDA_"):
+        assert name in nm.stdout, name

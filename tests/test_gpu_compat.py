@@ -1,7 +1,8 @@
 """GPU test of the C++ drop-in: tests/native/compat_frame, a program that calls
-the reference's own C++ names through include/tgmm_em_compat.hpp and
-include/tgmm_knn_compat.hpp (the EM's init, solve and release, the local
-log-likelihood and the supervoxel neighbour search, in the tracker's order),
+the reference's own C++ names through include/tgmm_em_compat.hpp,
+include/tgmm_utils_knn_compat.hpp and include/tgmm_knn_compat.hpp (device
+selection, the EM's init, solve and release, the local log-likelihood and the
+supervoxel neighbour search, in the tracker's order, then the voxel-mode EM),
 runs as a child process on one frame; its records, local log-likelihoods and
 neighbour slots equal, bitwise, those of the same calls made through the
 Python mirror of the C ABI."""
@@ -55,6 +56,10 @@ def test_cpp_compat_program_equals_python_path(tmp_path):
     ind_cpp = np.frombuffer(raw[o:o + 44 * S].tobytes(), np.int32)
     o += 44 * S
     dist_cpp = np.frombuffer(raw[o:o + 44 * S].tobytes(), np.float32)
+    o += 44 * S
+    gv_cpp = np.frombuffer(raw[o:o + G * GMM_DTYPE.itemsize].tobytes(), GMM_DTYPE)
+    o += G * GMM_DTYPE.itemsize
+    dev_cpp = int(np.frombuffer(raw[o:o + 4].tobytes(), np.int32)[0])
 
     # the same calls through the Python mirror
     em.regularizePrecisionMatrixConstants.setConstants(*LAM)
@@ -71,3 +76,10 @@ def test_cpp_compat_program_equals_python_path(tmp_path):
     assert records_equal(gm_cpp, gm)
     assert list(ll_cpp) == ll
     assert np.array_equal(ind_cpp, ind) and np.array_equal(dist_cpp, dist)
+    # the device the tracker selects: the visible one with the most memory
+    assert dev_cpp == 0
+    # voxel mode through the same names equals the batch API's voxel mode
+    with em.Batch([fr], em.make_opts(K=5, max_iter=ITERS, tol=-1.0, voxel=True)) as b:
+        b.solve(em.SOLVE_GRAPH)
+        gv = b.fetch()[0]["gm"]
+    assert records_equal(gv_cpp, gv)

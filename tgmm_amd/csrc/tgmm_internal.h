@@ -138,7 +138,6 @@ struct DevPtrs {
   int* seg_start;   // Gcap+1
   int* pairs;       // K*Scap
   int* pairs_sorted;
-  double* mstat;    // 10*Gcap: N, S1[3], S2[6] of the current M-step
   // grid
   int* cell_cnt;    // Ccap+1
   int* cell_start;  // Ccap+1

@@ -1467,94 +1467,95 @@ __global__ __launch_bounds__(256) void k_seg_sort_long(DevPtrs P) {
   for (int i = t; i < L; i += 256) P.pairs_sorted[st + i] = sh[i];
 }
 
-// Segmented FP64 sufficient statistics over each Gaussian's significant
-// pairs, accumulated about the current mean m_k:
+// M-step: segmented FP64 sufficient statistics over each Gaussian's
+// significant pairs, accumulated about the current mean m_k,
 //   N   = sum r W_s
 //   S1  = sum r W_s (mu_s - m_k)
 //   S2  = sum r [C_s + W_s (mu_s - m_k)(mu_s - m_k)^T]
 // so xbar = m_k + S1/N and sum w (x - xbar)(x - xbar)^T = S2 - S1 S1^T / N,
 // identical to the reference's per-voxel sums over r_sk * img_v
-// (GMEM.cu:1338-1453). A block covers 16 Gaussians with ML lanes each.
+// (GMEM.cu:1338-1453); then, in the same block, the posterior
+// (GMEM.cu:1601-1677) and regularisation (GMEM.cu:1693-1809) of the alive,
+// non-fixed Gaussians, and the per-64-Gaussian sums of alpha for the death
+// test. A block covers 64 Gaussians with ML lanes each; their sums go
+// through LDS to wave 0, one lane per Gaussian for the update.
 // Summation order, identical for every ML: sorted pair e of a Gaussian is
 // added, in increasing e, to class sum e mod 16, and the class sums are
 // combined as ((c + c^8) + ...^4) ...^1.
 template <int ML>
-__global__ __launch_bounds__(16 * ML) void k_mstep_stats(DevPtrs P) {
+__global__ __launch_bounds__(64 * ML) void k_mstep(DevPtrs P, KOpts O) {
   constexpr int NC = 16 / ML;
-  const long long k = (long long)blockIdx.x * 16 + threadIdx.x / ML;
-  const int lane = threadIdx.x % ML;
-  const int f = P.gblk_frame[k / kGB];
-  if (f < 0 || !P.fr[f].active) return;
-  if (P.g_frame[k] != f) return;
-  const long long G = P.Gcap, S = P.Scap;
-  const double r0 = gsv(P, GS_M, k), r1 = gsv(P, GS_M + 1, k), r2 = gsv(P, GS_M + 2, k);
-  double a[NC][10];
-#pragma unroll
-  for (int q = 0; q < NC; q++)
-#pragma unroll
-    for (int i = 0; i < 10; i++) a[q][i] = 0.0;
-  const int st = P.seg_start[k], en = P.seg_start[k + 1];
-  for (int e0 = st; e0 < en; e0 += 16) {
-#pragma unroll
-    for (int q = 0; q < NC; q++) {
-      const int e = e0 + lane + ML * q;
-      if (e >= en) continue;
-      const int code = P.pairs_sorted[e];
-      const long long s = code >> kPairShift;
-      const int j = code & ((1 << kPairShift) - 1);
-      const double r = (double)P.rnk[(long long)j * S + s];
-      const double Ws = P.wmom[s];
-      if (!(Ws != 0.0)) continue;
-      const double w = r * Ws;
-      const double d0 = P.wmom[S + s] - r0, d1 = P.wmom[2 * S + s] - r1,
-                   d2 = P.wmom[3 * S + s] - r2;
-      double* c = a[q];
-      c[0] += w;
-      c[1] += w * d0;
-      c[2] += w * d1;
-      c[3] += w * d2;
-      c[4] += r * P.wmom[4 * S + s] + w * d0 * d0;
-      c[5] += r * P.wmom[5 * S + s] + w * d0 * d1;
-      c[6] += r * P.wmom[6 * S + s] + w * d0 * d2;
-      c[7] += r * P.wmom[7 * S + s] + w * d1 * d1;
-      c[8] += r * P.wmom[8 * S + s] + w * d1 * d2;
-      c[9] += r * P.wmom[9 * S + s] + w * d2 * d2;
-    }
-  }
-#pragma unroll
-  for (int m = 8; m >= ML; m >>= 1) {
-    const int qm = m / ML;
-#pragma unroll
-    for (int q = 0; q < qm; q++)
-#pragma unroll
-      for (int i = 0; i < 10; i++) a[q][i] += a[q + qm][i];
-  }
-#pragma unroll
-  for (int i = 0; i < 10; i++) {
-#pragma unroll
-    for (int m = ML / 2; m >= 1; m >>= 1) a[0][i] += __shfl_xor(a[0][i], m, ML);
-  }
-#pragma unroll
-  for (int i = lane; i < 10; i += ML) {
-    double v = a[0][0];
-#pragma unroll
-    for (int t = 1; t < 10; t++)
-      if (i == t) v = a[0][t];
-    P.mstat[(long long)i * G + k] = v;
-  }
-}
-
-// Posterior (GMEM.cu:1601-1677) and regularisation (GMEM.cu:1693-1809) for
-// alive, non-fixed Gaussians from the M-step statistics, and the per-frame sum
-// of alpha for the death test.
-__global__ __launch_bounds__(kGB) void k_mstep_update(DevPtrs P, KOpts O) {
+  __shared__ double sums[10][64];
   __shared__ double red[1];
   const int f = P.gblk_frame[blockIdx.x];
-  if (f < 0) return;
+  if (f < 0) return;  // uniform over the block
   FrameDev* F = P.fr + f;
   if (!F->active) return;
-  const long long k = (long long)blockIdx.x * kGB + threadIdx.x;
-  const long long G = P.Gcap;
+  const long long G = P.Gcap, S = P.Scap;
+  {
+    const int g = threadIdx.x / ML;
+    const long long k = (long long)blockIdx.x * 64 + g;
+    const int lane = threadIdx.x % ML;
+    const bool mine = P.g_frame[k] == f;
+    double a[NC][10];
+#pragma unroll
+    for (int q = 0; q < NC; q++)
+#pragma unroll
+      for (int i = 0; i < 10; i++) a[q][i] = 0.0;
+    if (mine) {
+      const double r0 = gsv(P, GS_M, k), r1 = gsv(P, GS_M + 1, k), r2 = gsv(P, GS_M + 2, k);
+      const int st = P.seg_start[k], en = P.seg_start[k + 1];
+      for (int e0 = st; e0 < en; e0 += 16) {
+#pragma unroll
+        for (int q = 0; q < NC; q++) {
+          const int e = e0 + lane + ML * q;
+          if (e >= en) continue;
+          const int code = P.pairs_sorted[e];
+          const long long s = code >> kPairShift;
+          const int j = code & ((1 << kPairShift) - 1);
+          const double r = (double)P.rnk[(long long)j * S + s];
+          const double Ws = P.wmom[s];
+          if (!(Ws != 0.0)) continue;
+          const double w = r * Ws;
+          const double d0 = P.wmom[S + s] - r0, d1 = P.wmom[2 * S + s] - r1,
+                       d2 = P.wmom[3 * S + s] - r2;
+          double* c = a[q];
+          c[0] += w;
+          c[1] += w * d0;
+          c[2] += w * d1;
+          c[3] += w * d2;
+          c[4] += r * P.wmom[4 * S + s] + w * d0 * d0;
+          c[5] += r * P.wmom[5 * S + s] + w * d0 * d1;
+          c[6] += r * P.wmom[6 * S + s] + w * d0 * d2;
+          c[7] += r * P.wmom[7 * S + s] + w * d1 * d1;
+          c[8] += r * P.wmom[8 * S + s] + w * d1 * d2;
+          c[9] += r * P.wmom[9 * S + s] + w * d2 * d2;
+        }
+      }
+    }
+#pragma unroll
+    for (int m = 8; m >= ML; m >>= 1) {
+      const int qm = m / ML;
+#pragma unroll
+      for (int q = 0; q < qm; q++)
+#pragma unroll
+        for (int i = 0; i < 10; i++) a[q][i] += a[q + qm][i];
+    }
+#pragma unroll
+    for (int i = 0; i < 10; i++) {
+#pragma unroll
+      for (int m = ML / 2; m >= 1; m >>= 1) a[0][i] += __shfl_xor(a[0][i], m, ML);
+    }
+    if (lane == 0)
+#pragma unroll
+      for (int i = 0; i < 10; i++) sums[i][g] = a[0][i];
+  }
+  __syncthreads();
+  if (threadIdx.x >= 64) return;
+  const long long k = (long long)blockIdx.x * 64 + threadIdx.x;
+  double st[10];
+#pragma unroll
+  for (int i = 0; i < 10; i++) st[i] = sums[i][threadIdx.x];
   double alpha_out = 0.0;
   if (P.g_frame[k] == f) {
     const bool alive = gsv(P, GS_M_O, k) > -1e31;
@@ -1564,10 +1565,10 @@ __global__ __launch_bounds__(kGB) void k_mstep_update(DevPtrs P, KOpts O) {
     P.gs[GS_ELP * G + k] = alive ? P.epack[k].dpsi - F->psi_sum_alpha : -1e32;
     if (alive && !fixed) {
       const double r0 = gsv(P, GS_M, k), r1 = gsv(P, GS_M + 1, k), r2 = gsv(P, GS_M + 2, k);
-      const double N = P.mstat[k];
+      const double N = st[0];
       double S1[3], S2[6];
-      for (int i = 0; i < 3; i++) S1[i] = P.mstat[(1 + i) * G + k];
-      for (int i = 0; i < 6; i++) S2[i] = P.mstat[(4 + i) * G + k];
+      for (int i = 0; i < 3; i++) S1[i] = st[1 + i];
+      for (int i = 0; i < 6; i++) S2[i] = st[4 + i];
       double xk[3] = {0, 0, 0}, Ck[6] = {0, 0, 0, 0, 0, 0};
       if (N > 0.0) {
         const double m0 = S1[0] / N, m1 = S1[1] / N, m2 = S1[2] / N;
@@ -2153,14 +2154,14 @@ static void launch_loglik(const DevPtrs& P, const KOpts& O, hipStream_t st,
 #undef TGMM_LL
 }
 
-static void launch_mstep_stats(const DevPtrs& P, const KOpts& O, hipStream_t st) {
-  const unsigned g = nb(P.Gext, 16);
+static void launch_mstep(const DevPtrs& P, const KOpts& O, hipStream_t st) {
+  const unsigned g = nb(P.Gext, 64);
   if (O.tiles.ms_lanes == 4)
-    k_mstep_stats<4><<<g, 64, 0, st>>>(P);
+    k_mstep<4><<<g, 256, 0, st>>>(P, O);
   else if (O.tiles.ms_lanes == 8)
-    k_mstep_stats<8><<<g, 128, 0, st>>>(P);
+    k_mstep<8><<<g, 512, 0, st>>>(P, O);
   else
-    k_mstep_stats<16><<<g, 256, 0, st>>>(P);
+    k_mstep<16><<<g, 1024, 0, st>>>(P, O);
 }
 
 template <int KM>
@@ -2226,7 +2227,7 @@ static hipError_t iteration_t(const DevPtrs& P, const KOpts& O, hipStream_t st, 
   k_grid_scatter<<<gb, kGB, 0, st>>>(P);
   if (tk == TK_KNN && e0) (void)hipEventRecord(e0, st);
   launch_knn<KM>(P, O, st);
-  k_knn_exact<KM><<<64, 256, 0, st>>>(P, O);
+  k_knn_exact<KM><<<4, 256, 0, st>>>(P, O);
   if (tk == TK_KNN && e1) (void)hipEventRecord(e1, st);
   if (tk == TK_ESTEP && e0) (void)hipEventRecord(e0, st);
   k_estep<KM><<<nb(P.Sext, bs), bs, 0, st>>>(P, O);
@@ -2236,8 +2237,7 @@ static hipError_t iteration_t(const DevPtrs& P, const KOpts& O, hipStream_t st, 
   k_seg_sort<<<nb(P.Gext, 16), 256, 0, st>>>(P, O.voxel ? kSortShort : INT_MAX);
   if (O.voxel) k_seg_sort_long<<<(unsigned)P.Gext, 256, 0, st>>>(P);
   if (tk == TK_MSTEP && e0) (void)hipEventRecord(e0, st);
-  launch_mstep_stats(P, O, st);
-  k_mstep_update<<<gb, kGB, 0, st>>>(P, O);
+  launch_mstep(P, O, st);
   if (tk == TK_MSTEP && e1) (void)hipEventRecord(e1, st);
   k_death<<<gb, kGB, 0, st>>>(P, O);
   k_alpha_sum<<<P.nframes, 64, 0, st>>>(P, 1);

@@ -110,7 +110,6 @@ int ws_alloc(Workspace* w, const WsShape& sh, int K) {
   AL(P.seg_start, G + 1);
   AL(P.pairs, (long long)K * S);
   AL(P.pairs_sorted, (long long)K * S);
-  AL(P.mstat, 10 * G);
   AL(P.cell_cnt, C + 1);
   AL(P.cell_start, C + 1);
   AL(P.qcell_start, C + 1);
