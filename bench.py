@@ -239,6 +239,10 @@ def main():
                     help="config 4: server workspace budget")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default launch shapes (for counter runs)")
+    ap.add_argument("--tiles", type=str, default="",
+                    help="launch shapes as JSON (the keys of em.DEFAULT_TILES) or as their "
+                         "values in that order, comma-separated, instead of the autotuner's "
+                         "choice (for profiler runs)")
     ap.add_argument("--voxel", action="store_true",
                     help="per-voxel EM (GMEMvariationalInferenceCUDA) on the same frames")
     ap.add_argument("--marshal", action="store_true",
@@ -288,7 +292,14 @@ def main():
 
     # launch shapes: the autotuner times the candidates on this batch (not
     # part of the timed steps; results do not depend on the choice)
-    if args.no_autotune:
+    if args.tiles:
+        if args.tiles.lstrip().startswith("{"):
+            want = json.loads(args.tiles)
+        else:  # values in the order of em.DEFAULT_TILES' keys
+            want = dict(zip(em.DEFAULT_TILES, (int(v) for v in args.tiles.split(","))))
+        batch.set_tiles(want)
+        tiles = batch.get_tiles()
+    elif args.no_autotune:
         tiles = batch.get_tiles()
     else:
         tiles, tune_report = batch.autotune(reps=3, cache=False)
@@ -318,8 +329,9 @@ def main():
     frames_per_s = len(frames) * args.steps / elapsed
     rnk_updates = S_tot * K * iters * args.steps / elapsed
     vox_updates = N_tot * K * iters * args.steps / elapsed
-    print("# timed %d steps: %.3f ms/step" % (args.steps, ms_per_step), file=sys.stderr,
-          flush=True)
+    bstats = batch.stats()
+    print("# timed %d steps: %.3f ms/step; %s" % (args.steps, ms_per_step, json.dumps(bstats)),
+          file=sys.stderr, flush=True)
 
     # per-kernel device time with hipEvents on the library's stream
     kern = {}
@@ -399,6 +411,7 @@ def main():
         "accuracy_vs_fp64": acc,
         "kernels": kern,
         "dominant_kernel": dom,
+        "knn_ties_last_iteration": bstats["knn_ties_last_iteration"],
         "tiles": tiles,
         "roofline": roofline,
         "cpu_baseline": cpu,

@@ -63,7 +63,8 @@ struct alignas(64) LLPack {
   double lg2;   // (log alpha + log det(nu W)/2 - 1.5 log 2pi) * log2(e)
   float U[6];   // upper Cholesky factor of nu W * log2(e) / 2:
                 // u00 u01 u02 u11 u12 u22, so d^T (nu W log2(e)/2) d = |U d|^2
-  float pad[2];
+  float nf;     // Frobenius norm of U (rounded up): |U d| <= nf |d|
+  float pad;
 };
 struct alignas(32) EPack {
   double m[3];
@@ -114,6 +115,8 @@ struct DevPtrs {
   double* mom;      // 10*Scap: n, c[3], Q[6]  (unweighted, FP64)
   double* wmom;     // 10*Scap: W, mu[3], C[6] (intensity-weighted, FP64)
   float* anchor;    // 3*Scap: first voxel of the supervoxel
+  float4* sv_ball;  // Scap: ball around the supervoxel's voxels: centre
+                    // relative to the anchor, radius (rounded up)
   int* ind;         // K*Scap
   float* rnk;       // K*Scap
   int* sblk_frame;  // Scap/256
@@ -148,7 +151,9 @@ struct DevPtrs {
   // their K best (their candidate set follows the reference's scan order)
   int* tie_list;    // Scap
   int* knn_ctl;     // [0] ties listed this iteration, [1] supervoxels whose
-                    //     final slot layout found a different set (stays 0)
+                    //     final slot layout found a different set (stays 0),
+                    //     [2] Gaussians in long_list this iteration
+  int* long_list;   // Gcap: Gaussians whose pair list is long (sorted by a block each)
   int2* win;        // Scap: (Gaussian, slot) of the candidate with r > 0.5, or -1
   // results
   GaussianMixtureModelCUDA* rec;  // Gcap: records as returned (288 B each)

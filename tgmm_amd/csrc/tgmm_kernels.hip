@@ -120,6 +120,16 @@ __device__ inline double group16_sum(double x) {
   for (int m = 8; m >= 1; m >>= 1) x += __shfl_xor(x, m, 16);
   return x;
 }
+__device__ inline float group16_min(float x) {
+#pragma unroll
+  for (int m = 8; m >= 1; m >>= 1) x = fminf(x, __shfl_xor(x, m, 16));
+  return x;
+}
+__device__ inline float group16_max(float x) {
+#pragma unroll
+  for (int m = 8; m >= 1; m >>= 1) x = fmaxf(x, __shfl_xor(x, m, 16));
+  return x;
+}
 
 __global__ __launch_bounds__(256) void k_moments(DevPtrs P) {
   // every exit below is uniform over a supervoxel's 16 lanes, which is all
@@ -138,13 +148,20 @@ __global__ __launch_bounds__(256) void k_moments(DevPtrs P) {
       P.wmom[lane * S + s] = 0.0;
     }
     if (lane < 3) P.anchor[lane * S + s] = 0.0f;
+    if (lane == 0) P.sv_ball[s] = make_float4(0.f, 0.f, 0.f, 0.f);
     return;
   }
   const float a0 = P.vx[beg], a1 = P.vy[beg], a2 = P.vz[beg];
   double sx = 0, sy = 0, sz = 0, sw = 0, swx = 0, swy = 0, swz = 0;
+  float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
   for (int i = lane; i < n; i += 16) {
-    const double x = (double)P.vx[beg + i] - a0, y = (double)P.vy[beg + i] - a1,
-                 z = (double)P.vz[beg + i] - a2, w = (double)P.vimg[beg + i];
+    const float v[3] = {P.vx[beg + i], P.vy[beg + i], P.vz[beg + i]};
+    for (int a = 0; a < 3; a++) {
+      lo[a] = fminf(lo[a], v[a]);
+      hi[a] = fmaxf(hi[a], v[a]);
+    }
+    const double x = (double)v[0] - a0, y = (double)v[1] - a1, z = (double)v[2] - a2,
+                 w = (double)P.vimg[beg + i];
     sx += x;
     sy += y;
     sz += z;
@@ -202,6 +219,19 @@ __global__ __launch_bounds__(256) void k_moments(DevPtrs P) {
     P.anchor[1 * S + s] = a1;
     P.anchor[2 * S + s] = a2;
   }
+  // the ball around the voxels' box (NaN coordinates give a NaN ball, which
+  // the log-likelihood's candidate bounds ignore)
+  const float a[3] = {a0, a1, a2};
+  double bc[3], r2 = 0.0;
+  for (int i = 0; i < 3; i++) {
+    const double l = (double)group16_min(lo[i]), h = (double)group16_max(hi[i]);
+    bc[i] = 0.5 * (l + h) - (double)a[i];
+    r2 += 0.25 * (h - l) * (h - l);
+  }
+  if (lane == 0)
+    P.sv_ball[s] = make_float4((float)bc[0], (float)bc[1], (float)bc[2],
+                               (float)(sqrt(r2) * (1.0 + 1e-6) + 1e-6 * (fabs(bc[0]) +
+                                       fabs(bc[1]) + fabs(bc[2]))));
 }
 
 // Voxel mode: every supervoxel is one voxel, so its moments are the voxel
@@ -227,6 +257,7 @@ __global__ __launch_bounds__(256) void k_moments_voxel(DevPtrs P) {
     P.mom[i * S + s] = 0.0;
     P.wmom[i * S + s] = 0.0;
   }
+  P.sv_ball[s] = make_float4(0.f, 0.f, 0.f, 0.f);
 }
 
 // One block per frame: bounding box of the scaled kNN centroids and the grid
@@ -369,6 +400,9 @@ __global__ __launch_bounds__(kGB) void k_alpha_init(DevPtrs P) {
 }
 
 // ---------------------------------------------------------------- scans --
+constexpr int kSortShort = 128;  // longer pair lists are sorted by a block each
+constexpr int kSortLds = 4096;   // longest list sorted in LDS
+
 // Exclusive scan of 1024 per-thread values; returns this thread's prefix.
 __device__ inline int block_exclusive_scan1024(int v, int* lds) {
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -402,14 +436,16 @@ __device__ inline int block_exclusive_scan1024(int v, int* lds) {
 //            positions of the frame's queries in cell order).
 // Both ranges hold n+1 entries (the grid has fewer cells than its range, the
 // Gaussian range at least one padding entry), so frames never share an entry.
-// Tiles of 16384 entries, 16 consecutive per thread as four int4 loads (the
-// ranges start on multiples of 4).
+// Each thread takes a contiguous chunk (a multiple of 4 entries: int4 loads,
+// the ranges start on multiples of 4).
 __global__ __launch_bounds__(1024) void k_scan_frames(DevPtrs P, int which, int need_active,
                                                       int K) {
   __shared__ int lds[32];
-  __shared__ int carry;
   // the grid scan opens an iteration: the kNN's tie list starts empty
-  if (which == 0 && blockIdx.x == 0 && threadIdx.x == 0) P.knn_ctl[0] = 0;
+  if (which == 0 && blockIdx.x == 0 && threadIdx.x == 0) {
+    P.knn_ctl[0] = 0;
+    P.knn_ctl[2] = 0;  // and the list of long pair lists
+  }
   const int f = (P.only_frame >= 0) ? P.only_frame : (int)blockIdx.x;
   if (f >= P.Fcap || (P.only_frame < 0 && f >= P.nframes)) return;
   const FrameDev* F = P.fr + f;
@@ -429,40 +465,58 @@ __global__ __launch_bounds__(1024) void k_scan_frames(DevPtrs P, int which, int 
     n = F->ng;
     base = (int)(K * F->s0);
   }
-  if (threadIdx.x == 0) carry = base;
-  __syncthreads();
-  const long long n1 = n + 1;  // entry n carries the total
-  for (long long b0 = 0; b0 < n1; b0 += 16384) {
-    const int c = carry;
-    const long long b = b0 + (long long)threadIdx.x * 16;
-    int v[16];
-    if (b + 16 <= n) {
-      const int4* p4 = reinterpret_cast<const int4*>(in + b);
-#pragma unroll
-      for (int q = 0; q < 4; q++) {
-        const int4 t = p4[q];
-        v[4 * q] = t.x;
-        v[4 * q + 1] = t.y;
-        v[4 * q + 2] = t.z;
-        v[4 * q + 3] = t.w;
+  // each thread scans a contiguous chunk of the n + 1 entries (entry n
+  // carries the total): its chunk's sum, one block scan of the sums, then its
+  // chunk's prefixes; every load of a pass is issued at once
+  const long long n1 = n + 1;
+  const long long per = ((n1 + 1023) / 1024 + 3) & ~3LL;  // a multiple of 4 (int4 loads)
+  const long long b = (long long)threadIdx.x * per;
+  const long long e = b + per < n ? b + per : n;  // entries [b, e) are inputs
+  int tsum = 0;
+  // the pair scan lists the Gaussians whose pair list is long, for the
+  // block-wide sort (k_seg_sort_long)
+  const bool list_long = which == 1;
+  auto note = [&](long long i, int c) {
+    if (list_long && c > kSortShort) P.long_list[atomicAdd(P.knn_ctl + 2, 1)] = (int)(F->g0 + i);
+  };
+  if (b + per <= n) {
+    const int4* p4 = reinterpret_cast<const int4*>(in + b);
+    for (long long q = 0; q < per / 4; q++) {
+      const int4 t = p4[q];
+      tsum += (t.x + t.y) + (t.z + t.w);
+      if (list_long && (t.x > kSortShort || t.y > kSortShort || t.z > kSortShort ||
+                        t.w > kSortShort)) {
+        note(b + 4 * q, t.x);
+        note(b + 4 * q + 1, t.y);
+        note(b + 4 * q + 2, t.z);
+        note(b + 4 * q + 3, t.w);
       }
-    } else {
-#pragma unroll
-      for (int i = 0; i < 16; i++) v[i] = (b + i < n) ? in[b + i] : 0;
     }
-    int tsum = 0;
-#pragma unroll
-    for (int i = 0; i < 16; i++) tsum += v[i];
-    int run = c + block_exclusive_scan1024(tsum, lds);
-    const int incl_last = run + tsum;
-#pragma unroll
-    for (int i = 0; i < 16; i++) {
-      if (b + i < n1) out[b + i] = run;
-      run += v[i];
+  } else {
+    for (long long i = b; i < e; i++) {
+      tsum += in[i];
+      note(i, in[i]);
     }
-    __syncthreads();
-    if (threadIdx.x == 1023) carry = incl_last;
-    __syncthreads();
+  }
+  int run = base + block_exclusive_scan1024(tsum, lds);
+  if (b + per <= n) {  // a whole chunk of inputs: int4 loads and stores
+    const int4* p4 = reinterpret_cast<const int4*>(in + b);
+    int4* o4 = reinterpret_cast<int4*>(out + b);
+    for (long long q = 0; q < per / 4; q++) {
+      const int4 t = p4[q];
+      int4 r;
+      r.x = run;
+      r.y = r.x + t.x;
+      r.z = r.y + t.y;
+      r.w = r.z + t.z;
+      run = r.w + t.w;
+      o4[q] = r;
+    }
+  } else {  // the chunk holding entry n (the total) or past it
+    for (long long i = b; i < b + per && i < n1; i++) {
+      out[i] = run;
+      if (i < n) run += in[i];
+    }
   }
 }
 
@@ -550,13 +604,10 @@ __global__ __launch_bounds__(kGB) void k_grid_scatter(DevPtrs P) {
 // later falls flags a supervoxel needlessly, which changes nothing.
 template <int KM>
 __device__ inline void knn_try(float (&bd)[KM], int (&bi)[KM], float d, int idx, int K,
-                               bool& tie) {
+                               bool& tie, float& kd) {
+  // kd is the list's K-th distance (infinite until K are held)
   if (!(d <= 1e32f)) return;
   if (!lex_less(d, idx, bd[KM - 1], bi[KM - 1])) {
-    float kd = bd[0];
-#pragma unroll
-    for (int j = 1; j < KM; j++)
-      if (j == K - 1) kd = bd[j];
     tie |= d == kd;
     return;
   }
@@ -573,12 +624,15 @@ __device__ inline void knn_try(float (&bd)[KM], int (&bi)[KM], float d, int idx,
       ci = ti;
     }
     // cd is now the best element outside the K best
-    if (j == K - 1) tie |= (cd <= 1e32f) && cd == bd[j];
+    if (j == K - 1) {
+      tie |= (cd <= 1e32f) && cd == bd[j];
+      kd = bd[j];
+    }
   }
 }
 
 template <int KM, int QL>
-__device__ inline void knn_merge(float (&bd)[KM], int (&bi)[KM], int K, bool& tie) {
+__device__ inline void knn_merge(float (&bd)[KM], int (&bi)[KM], int K, bool& tie, float& kd) {
 #pragma unroll
   for (int m = QL / 2; m >= 1; m >>= 1) {
     float od[KM];
@@ -593,7 +647,7 @@ __device__ inline void knn_merge(float (&bd)[KM], int (&bi)[KM], int K, bool& ti
       bool dup = false;
 #pragma unroll
       for (int j = 0; j < KM; j++) dup |= (bi[j] == oi[t]);
-      if (!dup) knn_try<KM>(bd, bi, od[t], oi[t], K, tie);
+      if (!dup) knn_try<KM>(bd, bi, od[t], oi[t], K, tie, kd);
     }
   }
 }
@@ -608,7 +662,7 @@ __device__ inline void knn_merge(float (&bd)[KM], int (&bi)[KM], int K, bool& ti
 template <int KM, int QL, typename Consider>
 __device__ inline void knn_shells(const DevPtrs& P, const FrameDev* F, const int* c,
                                   const double* u, int K, int sub, float (&bd)[KM],
-                                  int (&bi)[KM], bool& tie, Consider consider) {
+                                  int (&bi)[KM], bool& tie, float& kd, Consider consider) {
   const int nx = F->dims[0], ny = F->dims[1], nz = F->dims[2];
   const int dims[3] = {nx, ny, nz};
   const double h = F->h;
@@ -677,7 +731,7 @@ __device__ inline void knn_shells(const DevPtrs& P, const FrameDev* F, const int
           }
         }
       }
-      if (QL > 1) knn_merge<KM, QL>(bd, bi, K, tie);
+      if (QL > 1) knn_merge<KM, QL>(bd, bi, K, tie, kd);
     }
     const double lb2 = lower_bound_sq(r);
     if (lb2 == INFINITY || kth() < lb2) break;  // every cell visited, or done
@@ -726,9 +780,10 @@ __global__ __launch_bounds__(256) void k_knn(DevPtrs P, KOpts O) {
   }
   const long long c0 = F->c0;
   bool tie = false;
+  float kd = INFINITY;
   auto consider = [&](const float4 cur) {
     const float d = knn_dist(q0, q1, q2, cur.x, cur.y, cur.z, s0, s1, s2);
-    knn_try<KM>(bd, bi, d, __float_as_int(cur.w), K, tie);
+    knn_try<KM>(bd, bi, d, __float_as_int(cur.w), K, tie, kd);
   };
   // An upper bound of the K-th distance: the previous iteration's candidates
   // of this supervoxel, if all K are still living means of this frame (a
@@ -811,8 +866,8 @@ __global__ __launch_bounds__(256) void k_knn(DevPtrs P, KOpts O) {
         if (p0 + w * QL < tot) consider(v[w]);
     }
   }
-  if (QL > 1) knn_merge<KM, QL>(bd, bi, K, tie);
-  knn_shells<KM, QL>(P, F, c, u, K, sub, bd, bi, tie, consider);
+  if (QL > 1) knn_merge<KM, QL>(bd, bi, K, tie, kd);
+  knn_shells<KM, QL>(P, F, c, u, K, sub, bd, bi, tie, kd, consider);
 #pragma unroll
   for (int m = QL / 2; m >= 1; m >>= 1) tie |= __shfl_xor((int)tie, m, QL) != 0;
   if (sub == 0) {
@@ -890,9 +945,10 @@ __global__ __launch_bounds__(64 * NW) void k_knn_cell(DevPtrs P, KOpts O) {
       bi[j] = 0x7fffffff;
     }
     bool tie = false;
+    float kd = INFINITY;
     auto consider = [&](const float4 cur) {
       const float d = knn_dist(q0, q1, q2, cur.x, cur.y, cur.z, s0, s1, s2);
-      knn_try<KM>(bd, bi, d, __float_as_int(cur.w), K, tie);
+      knn_try<KM>(bd, bi, d, __float_as_int(cur.w), K, tie, kd);
     };
     for (int cb = 0; cb < T; cb += kKnnCap) {
       const int n = min(kKnnCap, T - cb);
@@ -934,13 +990,14 @@ __global__ __launch_bounds__(64 * NW) void k_knn_cell(DevPtrs P, KOpts O) {
 #pragma unroll
       for (int v = 0; v < NW - 1; v++) {
 #pragma unroll
-        for (int j = 0; j < KM; j++) knn_try<KM>(bd, bi, ld[v][j][lane], li[v][j][lane], K, tie);
+        for (int j = 0; j < KM; j++)
+          knn_try<KM>(bd, bi, ld[v][j][lane], li[v][j][lane], K, tie, kd);
         tie |= lt[v][lane] != 0;
       }
     }
     if (!act) continue;
     const double u[3] = {(double)q0 * s0, (double)q1 * s1, (double)q2 * s2};
-    knn_shells<KM, 1>(P, F, c, u, K, 0, bd, bi, tie, consider);
+    knn_shells<KM, 1>(P, F, c, u, K, 0, bd, bi, tie, kd, consider);
 #pragma unroll
     for (int j = 0; j < KM; j++)
       if (j < K) P.ind[(long long)j * S + s] = (bd[j] <= 1e32f) ? bi[j] : -1;
@@ -1012,81 +1069,330 @@ struct ScanSlots {
   }
 };
 
-// Supervoxels whose K-th distance was tied (listed by the kNN search): the
-// reference's scan over all of the frame's Gaussians, one wave per
-// supervoxel, 64 Gaussians per step. Every lane forms one Gaussian's terms;
-// the Gaussians that pass the step's starting threshold (a superset of those
-// the scan takes, the threshold only falling) are then taken in index order
-// with the exact test, the slots held identically in every lane. The slots
-// are written in the reference's layout.
-template <int KM>
-__global__ __launch_bounds__(256) void k_knn_exact(DevPtrs P, KOpts O) {
+// The scan replayed exactly against the frame's Gaussians of its last kNN
+// search (mknn, in index order, and its grid), in one of two shapes:
+//   ONE = false: one query per lane (dead lanes take nothing); the Gaussians
+//     are loaded 64 at a time (one a lane), and those inside the box of the
+//     wave's balls (below) are broadcast to the lanes in index order;
+//   ONE = true: the wave's single query (every lane holds it); 64 Gaussians
+//     are tested at once (one a lane) and the sequential rule is applied to
+//     the lanes that pass the current threshold, lowest index first, the
+//     test repeated after every insertion (a Gaussian skipped at the current
+//     threshold is skipped by the scan: the threshold changes only when one
+//     is taken). The slots are the same in every lane.
+// The box of the wave's balls: the box spanned by the wave's queries
+// (scaled), widened by sqrt(max threshold) with a relative margin far above
+// float rounding. A Gaussian outside it is farther than every query's
+// threshold on one axis, so the partial sum that adds that axis is already
+// above the threshold and the scan leaves it. Thresholds only fall, so a box
+// stays valid while the scan goes on; it is formed again every round of
+// Gaussians. There is none (everything is scanned) while a query has fewer
+// than K taken or a NaN threshold.
+// The scan runs in two phases:
+//  1. The first m Gaussians in index order are scanned. m grows in steps of
+//     256 (512 for ONE) until the box holds no more Gaussians (on average)
+//     than have been scanned.
+//  2. The indices of the grid cells' Gaussians in the box are marked in an
+//     LDS bitmap, compacted in index order into an LDS list, and scanned.
+//     When the box or the bitmap is too large (no box yet, or more Gaussians
+//     than the bitmap holds), every later Gaussian is scanned.
+// LDS: words_cap bitmap words, then kListCap list entries.
+constexpr int kLayoutCells = 8192;  // most grid cells marked per wave
+constexpr int kLayoutWords = 4096;  // largest bitmap (Gaussians / 64) in LDS: 32 KiB
+constexpr int kListCap = 1024;      // marked Gaussians listed at a time
+
+struct ScanBox {
+  double qlo[3], qhi[3];  // the wave's queries (scaled)
+  double lo[3], hi[3];    // widened by the ball radius
+  bool wide;              // no box: a query without K taken, or a NaN threshold
+};
+
+// Refreshes the box from the current thresholds (wave-uniform).
+template <int KM, bool ONE>
+__device__ inline void scan_box(ScanBox& b, const ScanSlots<KM>& sl, bool live) {
+  float R2 = live ? sl.thr : 0.0f;
+  bool wide = !(R2 < 1e32f);
+  if constexpr (!ONE) {
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) R2 = fmaxf(R2, __shfl_xor(R2, o, 64));
+    wide = __any(wide);
+  }
+  b.wide = wide;
+  const double R = wide ? 0.0 : sqrt((double)R2) * (1.0 + 1e-5) + 1e-30;
+  for (int a = 0; a < 3; a++) {
+    b.lo[a] = b.qlo[a] - R;
+    b.hi[a] = b.qhi[a] + R;
+  }
+}
+
+// Up to 64 Gaussians, one a lane (index k relative to the frame, -1 past the
+// end, in increasing order over the lanes), through the scan.
+template <int KM, bool ONE>
+__device__ inline void scan_take64(ScanSlots<KM>& sl, const ScanBox& box, float q0, float q1,
+                                   float q2, float4 r, int k, float s0, float s1, float s2,
+                                   int K) {
   const int lane = threadIdx.x & 63;
-  const int wv = blockIdx.x * 4 + (threadIdx.x >> 6), nw = gridDim.x * 4;
+  const bool ok = k >= 0 && !(r.x != r.x);  // in range and alive
+  if constexpr (ONE) {
+    float p1, p2, d;
+    ScanSlots<KM>::terms(q0, q1, q2, r, s0, s1, s2, p1, p2, d);
+    unsigned long long mask = __ballot(ok && sl.passes(p1, p2, d));
+    while (mask) {
+      const int u = __builtin_ctzll(mask);
+      sl.insert(__int_as_float(__builtin_amdgcn_readlane(__float_as_int(d), u)),
+                __builtin_amdgcn_readlane(k, u), K);
+      mask = __ballot(ok && lane > u && sl.passes(p1, p2, d));
+    }
+  } else {
+    bool in = ok;
+    if (!box.wide) {  // comparisons with NaN keep the Gaussian
+      const double u0 = (double)r.x * s0, u1 = (double)r.y * s1, u2 = (double)r.z * s2;
+      in = in && !(u0 < box.lo[0] || u0 > box.hi[0] || u1 < box.lo[1] || u1 > box.hi[1] ||
+                   u2 < box.lo[2] || u2 > box.hi[2]);
+    }
+    unsigned long long mask = __ballot(in);
+    while (mask) {
+      const int u = __builtin_ctzll(mask);
+      mask &= mask - 1ull;
+      float4 ru;
+      ru.x = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(r.x), u));
+      ru.y = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(r.y), u));
+      ru.z = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(r.z), u));
+      ru.w = 0.0f;
+      sl.take(q0, q1, q2, ru, __builtin_amdgcn_readlane(k, u), s0, s1, s2, K);
+    }
+  }
+}
+
+// n Gaussians through the scan, 64 * U per round (U loads a lane issued
+// together), the box refreshed every round. idx(i) is the i-th Gaussian's
+// frame-relative index (increasing in i).
+template <int KM, bool ONE, class Idx>
+__device__ inline void scan_list(const float4* mk, int n, Idx idx, ScanSlots<KM>& sl, ScanBox& box,
+                                 bool live, float q0, float q1, float q2, float s0, float s1,
+                                 float s2, int K) {
+  constexpr int U = ONE ? 8 : 4;  // loads a lane in flight
+  const int lane = threadIdx.x & 63;
+  for (int b = 0; b < n; b += 64 * U) {
+    float4 r[U];
+    int kk[U];
+#pragma unroll
+    for (int c = 0; c < U; c++) {
+      const int i = b + 64 * c + lane;
+      kk[c] = i < n ? idx(i) : -1;
+      r[c] = kk[c] >= 0 ? mk[kk[c]] : make_float4(NAN, 0.f, 0.f, 0.f);
+    }
+#pragma unroll
+    for (int c = 0; c < U; c++)
+      if (b + 64 * c < n) scan_take64<KM, ONE>(sl, box, q0, q1, q2, r[c], kk[c], s0, s1, s2, K);
+    scan_box<KM, ONE>(box, sl, live);
+  }
+}
+
+// dynamic LDS of a kernel calling scan_replay with words_cap = words
+inline size_t scan_lds_bytes(int words) {
+  return (size_t)std::max(words, 1) * 8 + (size_t)kListCap * sizeof(int);
+}
+
+template <int KM, bool ONE>
+__device__ inline void scan_replay(const DevPtrs& P, const KOpts& O, const FrameDev* F, bool live,
+                                   float q0, float q1, float q2, int words_cap,
+                                   unsigned long long* bits, ScanSlots<KM>& sl) {
+  const int lane = threadIdx.x & 63;
+  const long long g0 = F->g0;
+  const int ng = (int)F->ng;
+  const int K = O.K;
+  const float s0 = O.scale[0], s1 = O.scale[1], s2 = O.scale[2];
+  const float4* mk = P.mknn + g0;
+  int* list = reinterpret_cast<int*>(bits + words_cap);
+  sl.init(live);
+  ScanBox box;
+  for (int a = 0; a < 3; a++) {
+    box.qlo[a] = 1e300;
+    box.qhi[a] = -1e300;
+  }
+  if (live) {
+    const double u[3] = {(double)q0 * s0, (double)q1 * s1, (double)q2 * s2};
+    for (int a = 0; a < 3; a++) {
+      box.qlo[a] = u[a];
+      box.qhi[a] = u[a];
+    }
+  }
+  if constexpr (!ONE) {
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1)
+      for (int a = 0; a < 3; a++) {
+        box.qlo[a] = fmin(box.qlo[a], __shfl_xor(box.qlo[a], o, 64));
+        box.qhi[a] = fmax(box.qhi[a], __shfl_xor(box.qhi[a], o, 64));
+      }
+  }
+  box.wide = true;
+  const double avg_per_cell =
+      (double)ng / ((double)F->dims[0] * (double)F->dims[1] * (double)F->dims[2]);
+  int m = 0;
+  int cl[3] = {0, 0, 0}, ch[3] = {0, 0, 0};
+  long long ncell = 0;
+  for (;;) {
+    const int m2 = min(ng, m + (ONE ? 512 : 256));
+    const int mb = m;
+    scan_list<KM, ONE>(mk, m2 - mb, [mb](int i) { return mb + i; }, sl, box, live, q0, q1, q2,
+                       s0, s1, s2, K);
+    m = m2;
+    if (m >= ng) break;
+    if (!box.wide) {
+      ncell = 1;
+      for (int a = 0; a < 3; a++) {
+        cl[a] = axis_cell(box.lo[a], F->org[a], F->h, F->dims[a]);
+        ch[a] = axis_cell(box.hi[a], F->org[a], F->h, F->dims[a]);
+        ncell *= (long long)(ch[a] - cl[a] + 1);
+      }
+      // stop when the box costs no more than the Gaussians scanned so far
+      if (ncell <= kLayoutCells && (double)ncell * avg_per_cell <= fmax(1024.0, (double)m)) break;
+    }
+    if (m >= 16384) break;  // the box or the rest of the scan, whichever applies
+  }
+  if (m >= ng) return;
+  const int words = (ng - m + 63) / 64;
+  if (box.wide || words > words_cap || ncell > kLayoutCells) {
+    const int mb = m;
+    scan_list<KM, ONE>(mk, ng - mb, [mb](int i) { return mb + i; }, sl, box, live, q0, q1, q2,
+                       s0, s1, s2, K);
+    return;
+  }
+  for (int w = lane; w < words; w += 64) bits[w] = 0ull;
+  // The box's rows of cells along x are contiguous runs of cg4: each lane
+  // takes a row (its run's ends), the runs are laid end to end by a prefix
+  // over the lanes (kept in the list area), and their entries are marked 64
+  // at a time, four loads a lane in flight.
+  const int ny = ch[1] - cl[1] + 1;
+  const int nrow = ny * (ch[2] - cl[2] + 1);
+  int* rpre = list;       // 64 prefixes
+  int* rbeg = list + 64;  // 64 run starts
+  for (int rb = 0; rb < nrow; rb += 64) {
+    const int r = rb + lane;
+    int b = 0, len = 0;
+    if (r < nrow) {
+      const long long row =
+          F->c0 + ((long long)(cl[2] + r / ny) * F->dims[1] + (cl[1] + r % ny)) * F->dims[0];
+      b = P.cell_start[row + cl[0]];
+      len = P.cell_start[row + ch[0] + 1] - b;
+    }
+    int x = len;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const int y = __shfl_up(x, o, 64);
+      if (lane >= o) x += y;
+    }
+    const int tot = __shfl(x, 63, 64);
+    __syncthreads();  // the previous rows' prefixes are read
+    rpre[lane] = x - len;
+    rbeg[lane] = b;
+    __syncthreads();
+    for (int j0 = 0; j0 < tot; j0 += 256) {
+      int pk[4];
+#pragma unroll
+      for (int t = 0; t < 4; t++) {
+        const int j = j0 + 64 * t + lane;
+        pk[t] = -1;
+        if (j < tot) {
+          int lo = 0;  // the last row whose run starts at or before j
+#pragma unroll
+          for (int st = 32; st >= 1; st >>= 1)
+            if (lo + st < 64 && rpre[lo + st] <= j) lo += st;
+          pk[t] = rbeg[lo] + (j - rpre[lo]);
+        }
+      }
+      int kk[4];
+#pragma unroll
+      for (int t = 0; t < 4; t++) kk[t] = pk[t] >= 0 ? __float_as_int(P.cg4[pk[t]].w) : -1;
+#pragma unroll
+      for (int t = 0; t < 4; t++) {
+        const int k = pk[t] >= 0 ? (int)(kk[t] - g0) - m : -1;
+        if (k >= 0) atomicOr(bits + (k >> 6), 1ull << (k & 63));
+      }
+    }
+  }
+  __syncthreads();
+  // the marked Gaussians in index order: lane l counts the marks of its run
+  // of words; list positions follow from the lanes' prefix; kListCap at a time
+  const int wpl = (words + 63) / 64;
+  const int w0 = min(words, lane * wpl), w1 = min(words, w0 + wpl);
+  int c = 0;
+  for (int w = w0; w < w1; w++) c += __popcll(bits[w]);
+  int x = c;  // inclusive scan of the counts over the lanes
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int y = __shfl_up(x, o, 64);
+    if (lane >= o) x += y;
+  }
+  const int tot = __shfl(x, 63, 64);
+  const int pre = x - c;
+  for (int base = 0; base < tot; base += kListCap) {
+    if (pre < base + kListCap && pre + c > base) {
+      int o = pre;
+      for (int w = w0; w < w1 && o < base + kListCap; w++) {
+        unsigned long long word = bits[w];
+        while (word) {
+          if (o >= base && o < base + kListCap)
+            list[o - base] = m + 64 * w + __builtin_ctzll(word);
+          o++;
+          word &= word - 1ull;
+        }
+      }
+    }
+    __syncthreads();
+    scan_list<KM, ONE>(mk, min(kListCap, tot - base), [list](int i) { return list[i]; }, sl, box,
+                       live, q0, q1, q2, s0, s1, s2, K);
+    __syncthreads();  // the list is rewritten next
+  }
+}
+
+// Supervoxels whose K-th distance was tied (listed by the kNN search): their
+// set as the reference's scan keeps it, one wave per listed supervoxel,
+// written in the search's (distance, index) order, as every other
+// supervoxel's: the E-step's sums then do not depend on which supervoxels
+// were searched again (the reference's slot layout is formed once, after the
+// last iteration).
+template <int KM>
+__global__ __launch_bounds__(64) void k_knn_exact(DevPtrs P, KOpts O, int words_cap) {
+  extern __shared__ unsigned long long bits[];
   const int cnt = P.knn_ctl[0];
   const long long S = P.Scap;
   const int K = O.K;
-  const float s0 = O.scale[0], s1 = O.scale[1], s2 = O.scale[2];
-  for (int t = wv; t < cnt; t += nw) {
+  for (int t = blockIdx.x; t < cnt; t += gridDim.x) {
     const long long s = P.tie_list[t];
     const FrameDev* F = P.fr + P.sv_frame[s];
-    const long long g0 = F->g0;
-    const int ng = (int)F->ng;
     const float q0 = P.sv_cent[s], q1 = P.sv_cent[S + s], q2 = P.sv_cent[2 * S + s];
     ScanSlots<KM> sl;
-    sl.init(true);
-    for (int b = 0; b < ng; b += 64) {
-      const int k = b + lane;
-      float p1 = 0.f, p2 = 0.f, d = 0.f;
-      bool ok = false;
-      if (k < ng) {
-        const float4 r = P.mknn[g0 + k];
-        ok = r.x == r.x;
-        ScanSlots<KM>::terms(q0, q1, q2, r, s0, s1, s2, p1, p2, d);
-      }
-      unsigned long long mask = __ballot(ok && sl.passes(p1, p2, d));
-      const unsigned long long alive = __ballot(ok);
-      while (mask) {
-        const int u = __builtin_ctzll(mask);
-        mask &= mask - 1ull;
-        const float up1 = __shfl(p1, u, 64), up2 = __shfl(p2, u, 64), ud = __shfl(d, u, 64);
-        if (!sl.passes(up1, up2, ud)) continue;
-        sl.insert(ud, b + u, K);
-        // a NaN in slot 0 keeps the threshold NaN: the scan then takes the
-        // next Gaussian whatever its distance
-        if (sl.thr != sl.thr && u < 63) mask |= alive & (~0ull << (u + 1));
-      }
-    }
+    scan_replay<KM, true>(P, O, F, true, q0, q1, q2, words_cap, bits, sl);
+    if (threadIdx.x == 0) {
+      float od[KM];
+      int oi[KM];
 #pragma unroll
-    for (int j = 0; j < KM; j++)
-      if (j < K && lane == j) P.ind[(long long)j * S + s] = sl.mi[j] >= 0 ? (int)(g0 + sl.mi[j]) : -1;
+      for (int j = 0; j < KM; j++) {
+        od[j] = INFINITY;
+        oi[j] = 0x7fffffff;
+      }
+      bool unused = false;
+      float kd = INFINITY;
+#pragma unroll
+      for (int j = 0; j < KM; j++)
+        if (j < K && sl.mi[j] >= 0) knn_try<KM>(od, oi, sl.md[j], sl.mi[j], K, unused, kd);
+#pragma unroll
+      for (int j = 0; j < KM; j++)
+        if (j < K) P.ind[(long long)j * S + s] = od[j] <= 1e32f ? (int)(F->g0 + oi[j]) : -1;
+    }
+    __syncthreads();  // the bitmap is free for the next listed supervoxel
   }
 }
 
 // The slot layout of every supervoxel after a solve: the reference's scan of
-// its last kNN search, replayed exactly with the Gaussians of that search
-// (mknn, and its grid), one lane per supervoxel, 64 supervoxels of one
-// frame in grid-cell order per block.
-//  1. The first m Gaussians in index order (m ~ sqrt(K * ng)) are scanned
-//     by every lane (wave-uniform loads). A lane's threshold is then T_m,
-//     and every later Gaussian the scan takes has a distance <= T_m (the
-//     threshold only falls).
-//  2. Every later Gaussian within the largest such ball of the wave's lanes
-//     lies in the box spanned by the lanes' queries widened by
-//     sqrt(max T_m) (with a relative margin far above float rounding); the
-//     wave marks the indices of the grid cells' Gaussians in that box in an
-//     LDS bitmap and scans the marked ones in index order with the exact
-//     test. The others are farther than every lane's threshold, so the scan
-//     would not take them. When the box or the bitmap is too large (a lane
-//     with fewer than K Gaussians taken, or more Gaussians than the bitmap
-//     holds), the wave scans every later Gaussian instead.
-// The candidates found by the iterations' search (ind, rnk) are then put in
-// these slots, responsibilities moving with their Gaussian; a supervoxel
-// whose set differs counts in knn_ctl[1] (never, by construction). win[s]
-// receives the (Gaussian, slot) whose responsibility is above 0.5, if any.
-constexpr int kLayoutCells = 4096;  // most grid cells marked per wave
-constexpr int kLayoutWords = 4096;  // largest bitmap (Gaussians / 64) in LDS: 32 KiB
-
+// its last kNN search replayed exactly (scan_replay), 64 supervoxels of
+// one frame in grid-cell order per wave. The candidates found by the
+// iterations' search (ind, rnk) are then put in these slots, responsibilities
+// moving with their Gaussian; a supervoxel whose set differs counts in
+// knn_ctl[1] (never, by construction). win[s] receives the (Gaussian, slot)
+// whose responsibility is above 0.5, if any.
 template <int KM>
 __global__ __launch_bounds__(64) void k_knn_layout(DevPtrs P, KOpts O, int only_frame,
                                                    int words_cap) {
@@ -1100,80 +1406,13 @@ __global__ __launch_bounds__(64) void k_knn_layout(DevPtrs P, KOpts O, int only_
   if (F->iter <= 1 || F->ng <= 0 || qb >= F->s0 + F->ns) return;  // no kNN ran
   const long long S = P.Scap;
   const long long g0 = F->g0;
-  const int ng = (int)F->ng;
   const int K = O.K;
-  const float s0 = O.scale[0], s1 = O.scale[1], s2 = O.scale[2];
   const long long i = qb + lane;
   const bool live = i < F->s0 + F->ns;
   const float4 qq = live ? P.q4[i] : make_float4(0.f, 0.f, 0.f, 0.f);
   const long long s = live ? __float_as_int(qq.w) : 0;
-  const float q0 = qq.x, q1 = qq.y, q2 = qq.z;
   ScanSlots<KM> sl;
-  sl.init(live);
-  // 1. the first m Gaussians
-  int m = 64 * (int)ceil(sqrt((double)K * (double)ng) / 64.0);
-  m = min(max(m, 64), ng);
-  for (int k = 0; k < m; k++) sl.take(q0, q1, q2, P.mknn[g0 + k], k, s0, s1, s2, K);
-  // 2. the later Gaussians in the wave's box
-  if (m < ng) {
-    float R2 = live ? sl.thr : 0.0f;
-    double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
-    if (live) {
-      const double u[3] = {(double)q0 * s0, (double)q1 * s1, (double)q2 * s2};
-      for (int a = 0; a < 3; a++) {
-        lo[a] = u[a];
-        hi[a] = u[a];
-      }
-    }
-    bool wide = !(R2 < 1e32f);  // fewer than K taken, or NaN
-#pragma unroll
-    for (int o = 32; o >= 1; o >>= 1) {
-      R2 = fmaxf(R2, __shfl_xor(R2, o, 64));
-      for (int a = 0; a < 3; a++) {
-        lo[a] = fmin(lo[a], __shfl_xor(lo[a], o, 64));
-        hi[a] = fmax(hi[a], __shfl_xor(hi[a], o, 64));
-      }
-    }
-    wide = __any(wide);
-    const int words = (ng - m + 63) / 64;
-    int cl[3] = {0, 0, 0}, ch[3] = {0, 0, 0};
-    long long ncell = 0;
-    if (!wide) {
-      const double R = sqrt((double)R2) * (1.0 + 1e-5) + 1e-30;
-      ncell = 1;
-      for (int a = 0; a < 3; a++) {
-        cl[a] = axis_cell(lo[a] - R, F->org[a], F->h, F->dims[a]);
-        ch[a] = axis_cell(hi[a] + R, F->org[a], F->h, F->dims[a]);
-        ncell *= (long long)(ch[a] - cl[a] + 1);
-      }
-    }
-    if (wide || words > words_cap || ncell > kLayoutCells) {
-      for (int k = m; k < ng; k++) sl.take(q0, q1, q2, P.mknn[g0 + k], k, s0, s1, s2, K);
-    } else {
-      for (int w = lane; w < words; w += 64) bits[w] = 0ull;
-      __syncthreads();
-      const int nx = ch[0] - cl[0] + 1, ny = ch[1] - cl[1] + 1;
-      for (long long c = lane; c < ncell; c += 64) {
-        const int cx = cl[0] + (int)(c % nx), cy = cl[1] + (int)((c / nx) % ny),
-                  cz = cl[2] + (int)(c / ((long long)nx * ny));
-        const long long cell = F->c0 + ((long long)cz * F->dims[1] + cy) * F->dims[0] + cx;
-        const int pe = P.cell_start[cell + 1];
-        for (int p = P.cell_start[cell]; p < pe; p++) {
-          const int k = (int)(__float_as_int(P.cg4[p].w) - g0) - m;
-          if (k >= 0) atomicOr(bits + (k >> 6), 1ull << (k & 63));
-        }
-      }
-      __syncthreads();
-      for (int w = 0; w < words; w++) {
-        unsigned long long word = bits[w];
-        while (word) {
-          const int k = m + 64 * w + __builtin_ctzll(word);
-          word &= word - 1ull;
-          sl.take(q0, q1, q2, P.mknn[g0 + k], k, s0, s1, s2, K);
-        }
-      }
-    }
-  }
+  scan_replay<KM, false>(P, O, F, live, qq.x, qq.y, qq.z, words_cap, bits, sl);
   if (!live) return;
   // the search's candidates into the reference's slots
   int oi[KM];
@@ -1402,40 +1641,59 @@ __global__ __launch_bounds__(256) void k_pair_scatter(DevPtrs P, KOpts O) {
   }
 }
 
-// 16 lanes per Gaussian: order its pair list by code (= by supervoxel) with a
-// rank sort, so that the FP64 accumulation order is fixed. In voxel mode
-// (hundreds of voxels per Gaussian) lists longer than kSortShort are left to
-// k_seg_sort_long; both give the same ascending order.
-constexpr int kSortShort = 128;
-constexpr int kSortLds = 4096;  // longest list sorted in LDS
-
-__global__ __launch_bounds__(256) void k_seg_sort(DevPtrs P, int short_max) {
-  const long long k = (long long)blockIdx.x * 16 + (threadIdx.x >> 4);
-  const int lane = threadIdx.x & 15;
-  const int f = P.gblk_frame[k / kGB];
-  if (f < 0 || !P.fr[f].active) return;
-  if (P.g_frame[k] != f) return;
-  const int st = P.seg_start[k], L = P.seg_start[k + 1] - st;
-  if (L > short_max) return;
-  for (int i = lane; i < L; i += 16) {
-    const int e = P.pairs[st + i];
-    int rank = 0;
-    for (int j = 0; j < L; j++) rank += (P.pairs[st + j] < e) ? 1 : 0;
-    P.pairs_sorted[st + rank] = e;
-  }
+// Orders of the LDS accesses of one wave's lanes (no block barrier).
+__device__ inline void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// One block per Gaussian with a long pair list: bitonic sort in LDS (codes
-// are distinct: a supervoxel lists a Gaussian at most once), or a block-wide
-// rank sort beyond kSortLds entries.
-__global__ __launch_bounds__(256) void k_seg_sort_long(DevPtrs P) {
-  __shared__ int sh[kSortLds];
-  const long long k = blockIdx.x;
+// 16 lanes per Gaussian: order its pair list by code (= by supervoxel), so
+// that the FP64 accumulation order is fixed: a bitonic sort in LDS over the
+// next power of two (codes are distinct: a supervoxel lists a Gaussian at
+// most once). Lists longer than kSortShort are left to k_seg_sort_long.
+__global__ __launch_bounds__(256) void k_seg_sort(DevPtrs P) {
+  __shared__ int sh[16 * kSortShort];
+  const int grp = threadIdx.x >> 4, lane = threadIdx.x & 15;
+  const long long k = (long long)blockIdx.x * 16 + grp;
   const int f = P.gblk_frame[k / kGB];
   if (f < 0 || !P.fr[f].active) return;
   if (P.g_frame[k] != f) return;
   const int st = P.seg_start[k], L = P.seg_start[k + 1] - st;
-  if (L <= kSortShort) return;
+  if (L <= 1 || L > kSortShort) {
+    if (L == 1 && lane == 0) P.pairs_sorted[st] = P.pairs[st];
+    return;
+  }
+  int* a = sh + grp * kSortShort;
+  int n = 2;
+  while (n < L) n <<= 1;
+  for (int i = lane; i < n; i += 16) a[i] = i < L ? P.pairs[st + i] : INT_MAX;
+  wave_lds_sync();
+  for (int w = 2; w <= n; w <<= 1)
+    for (int j = w >> 1; j > 0; j >>= 1) {
+      // pairs (lo, lo + j) with bit j of lo clear; ascending where bit w is
+      for (int i = lane; i < (n >> 1); i += 16) {
+        const int lo = 2 * i - (i & (j - 1)), hi = lo + j;
+        const int x = a[lo], y = a[hi];
+        if ((x > y) == ((lo & w) == 0)) {
+          a[lo] = y;
+          a[hi] = x;
+        }
+      }
+      wave_lds_sync();
+    }
+  for (int i = lane; i < L; i += 16) P.pairs_sorted[st + i] = a[i];
+}
+
+// The Gaussians with long pair lists (listed by the pair scan), one block
+// each: bitonic sort in LDS (codes are distinct: a supervoxel lists a
+// Gaussian at most once), or a block-wide rank sort beyond kSortLds entries.
+__global__ __launch_bounds__(256) void k_seg_sort_long(DevPtrs P) {
+  __shared__ int sh[kSortLds];
+  const int cnt = P.knn_ctl[2];
+  for (int li = blockIdx.x; li < cnt; li += gridDim.x) {
+  const long long k = P.long_list[li];
+  const int st = P.seg_start[k], L = P.seg_start[k + 1] - st;
   const int t = threadIdx.x;
   if (L > kSortLds) {
     for (int i = t; i < L; i += 256) {
@@ -1444,7 +1702,7 @@ __global__ __launch_bounds__(256) void k_seg_sort_long(DevPtrs P) {
       for (int j = 0; j < L; j++) rank += (P.pairs[st + j] < e) ? 1 : 0;
       P.pairs_sorted[st + rank] = e;
     }
-    return;
+    continue;
   }
   int n = 1;
   while (n < L) n <<= 1;
@@ -1465,6 +1723,85 @@ __global__ __launch_bounds__(256) void k_seg_sort_long(DevPtrs P) {
       __syncthreads();
     }
   for (int i = t; i < L; i += 256) P.pairs_sorted[st + i] = sh[i];
+  __syncthreads();  // sh is free for the next list
+  }
+}
+
+// Posterior (GMEM.cu:1601-1677) and regularisation (GMEM.cu:1693-1809) of
+// Gaussian k from its M-step sums st = (N, S1[3], S2[6]), for an alive,
+// non-fixed Gaussian; commits this iteration's E[ln det] and E[ln pi].
+// Returns alpha_k after the update. Not inlined: one compiled body for every
+// M-step launch shape, so that every shape rounds identically (contraction
+// into fused multiply-adds is decided per compiled body).
+__device__ __noinline__ double mstep_update_one(double* __restrict__ gs,
+                                                const double* __restrict__ eldn,
+                                                const EPack* __restrict__ epack,
+                                                const unsigned char* __restrict__ fixed_flags,
+                                                long long G, long long k, double psi_sum,
+                                                const double* st, int w4dof, float sr0,
+                                                float sr1, float sr2, double lmin, double lmax,
+                                                double emax) {
+#define GSV(fidx, kk) gs[(long long)(fidx) * G + (kk)]
+  const bool alive = GSV(GS_M_O, k) > -1e31;
+  const bool fixed = fixed_flags[k] != 0;
+  // this iteration's expectations, as used by its E-step
+  gs[GS_ELD * G + k] = eldn[k];
+  gs[GS_ELP * G + k] = alive ? epack[k].dpsi - psi_sum : -1e32;
+  if (alive && !fixed) {
+    const double r0 = GSV(GS_M, k), r1 = GSV(GS_M + 1, k), r2 = GSV(GS_M + 2, k);
+    const double N = st[0];
+    double S1[3], S2[6];
+    for (int i = 0; i < 3; i++) S1[i] = st[1 + i];
+    for (int i = 0; i < 6; i++) S2[i] = st[4 + i];
+    double xk[3] = {0, 0, 0}, Ck[6] = {0, 0, 0, 0, 0, 0};
+    if (N > 0.0) {
+      const double m0 = S1[0] / N, m1 = S1[1] / N, m2 = S1[2] / N;
+      xk[0] = r0 + m0;
+      xk[1] = r1 + m1;
+      xk[2] = r2 + m2;
+      Ck[0] = S2[0] - S1[0] * m0;
+      Ck[1] = S2[1] - S1[0] * m1;
+      Ck[2] = S2[2] - S1[0] * m2;
+      Ck[3] = S2[3] - S1[1] * m1;
+      Ck[4] = S2[4] - S1[1] * m2;
+      Ck[5] = S2[5] - S1[2] * m2;
+    }
+    // posterior (FP64 throughout)
+    const double beta_o = GSV(GS_BETA_O, k), nu_o = GSV(GS_NU_O, k),
+                 alpha_o = GSV(GS_ALPHA_O, k);
+    double mo[3], Wo[6];
+    for (int i = 0; i < 3; i++) mo[i] = GSV(GS_M_O + i, k);
+    for (int i = 0; i < 6; i++) Wo[i] = GSV(GS_W_O + i, k);
+    const double beta = beta_o + N, nu = nu_o + N, alpha = fmax(alpha_o + N, 0.0);
+    double m[3];
+    for (int i = 0; i < 3; i++) m[i] = (beta_o * mo[i] + N * xk[i]) / beta;
+    if (Wo[5] < 1e-8) {
+      Wo[5] = 0.5 * (Wo[0] + Wo[3]);
+      gs[(GS_W_O + 5) * G + k] = Wo[5];
+    }
+    double A[6], W[6];
+    inv3(Wo, A);
+    const double dx[3] = {xk[0] - mo[0], xk[1] - mo[1], xk[2] - mo[2]};
+    const double fct = beta_o / (beta_o + N);
+    A[0] += Ck[0] + N * fct * dx[0] * dx[0];
+    A[1] += Ck[1] + N * fct * dx[0] * dx[1];
+    A[2] += Ck[2] + N * fct * dx[0] * dx[2];
+    A[3] += Ck[3] + N * fct * dx[1] * dx[1];
+    A[4] += Ck[4] + N * fct * dx[1] * dx[2];
+    A[5] += Ck[5] + N * fct * dx[2] * dx[2];
+    inv3(A, W);
+    const float sreg[3] = {sr0, sr1, sr2};
+    regularize_W(W, nu, w4dof, sreg, lmin, lmax, emax);
+    gs[GS_BETA * G + k] = beta;
+    gs[GS_NU * G + k] = nu;
+    gs[GS_ALPHA * G + k] = alpha;
+    for (int i = 0; i < 3; i++) gs[(GS_M + i) * G + k] = m[i];
+    for (int i = 0; i < 6; i++) gs[(GS_W + i) * G + k] = W[i];
+    return alpha;
+  } else {
+    return GSV(GS_ALPHA, k);
+  }
+#undef GSV
 }
 
 // M-step: segmented FP64 sufficient statistics over each Gaussian's
@@ -1557,66 +1894,10 @@ __global__ __launch_bounds__(64 * ML) void k_mstep(DevPtrs P, KOpts O) {
 #pragma unroll
   for (int i = 0; i < 10; i++) st[i] = sums[i][threadIdx.x];
   double alpha_out = 0.0;
-  if (P.g_frame[k] == f) {
-    const bool alive = gsv(P, GS_M_O, k) > -1e31;
-    const bool fixed = P.fixed[k] != 0;
-    // this iteration's expectations, as used by its E-step
-    P.gs[GS_ELD * G + k] = P.eldn[k];
-    P.gs[GS_ELP * G + k] = alive ? P.epack[k].dpsi - F->psi_sum_alpha : -1e32;
-    if (alive && !fixed) {
-      const double r0 = gsv(P, GS_M, k), r1 = gsv(P, GS_M + 1, k), r2 = gsv(P, GS_M + 2, k);
-      const double N = st[0];
-      double S1[3], S2[6];
-      for (int i = 0; i < 3; i++) S1[i] = st[1 + i];
-      for (int i = 0; i < 6; i++) S2[i] = st[4 + i];
-      double xk[3] = {0, 0, 0}, Ck[6] = {0, 0, 0, 0, 0, 0};
-      if (N > 0.0) {
-        const double m0 = S1[0] / N, m1 = S1[1] / N, m2 = S1[2] / N;
-        xk[0] = r0 + m0;
-        xk[1] = r1 + m1;
-        xk[2] = r2 + m2;
-        Ck[0] = S2[0] - S1[0] * m0;
-        Ck[1] = S2[1] - S1[0] * m1;
-        Ck[2] = S2[2] - S1[0] * m2;
-        Ck[3] = S2[3] - S1[1] * m1;
-        Ck[4] = S2[4] - S1[1] * m2;
-        Ck[5] = S2[5] - S1[2] * m2;
-      }
-      // posterior (FP64 throughout)
-      const double beta_o = gsv(P, GS_BETA_O, k), nu_o = gsv(P, GS_NU_O, k),
-                   alpha_o = gsv(P, GS_ALPHA_O, k);
-      double mo[3], Wo[6];
-      for (int i = 0; i < 3; i++) mo[i] = gsv(P, GS_M_O + i, k);
-      for (int i = 0; i < 6; i++) Wo[i] = gsv(P, GS_W_O + i, k);
-      const double beta = beta_o + N, nu = nu_o + N, alpha = fmax(alpha_o + N, 0.0);
-      double m[3];
-      for (int i = 0; i < 3; i++) m[i] = (beta_o * mo[i] + N * xk[i]) / beta;
-      if (Wo[5] < 1e-8) {
-        Wo[5] = 0.5 * (Wo[0] + Wo[3]);
-        P.gs[(GS_W_O + 5) * G + k] = Wo[5];
-      }
-      double A[6], W[6];
-      inv3(Wo, A);
-      const double dx[3] = {xk[0] - mo[0], xk[1] - mo[1], xk[2] - mo[2]};
-      const double fct = beta_o / (beta_o + N);
-      A[0] += Ck[0] + N * fct * dx[0] * dx[0];
-      A[1] += Ck[1] + N * fct * dx[0] * dx[1];
-      A[2] += Ck[2] + N * fct * dx[0] * dx[2];
-      A[3] += Ck[3] + N * fct * dx[1] * dx[1];
-      A[4] += Ck[4] + N * fct * dx[1] * dx[2];
-      A[5] += Ck[5] + N * fct * dx[2] * dx[2];
-      inv3(A, W);
-      regularize_W(W, nu, O.w4dof, O.scale_reg, O.lmin, O.lmax, O.emax);
-      P.gs[GS_BETA * G + k] = beta;
-      P.gs[GS_NU * G + k] = nu;
-      P.gs[GS_ALPHA * G + k] = alpha;
-      for (int i = 0; i < 3; i++) P.gs[(GS_M + i) * G + k] = m[i];
-      for (int i = 0; i < 6; i++) P.gs[(GS_W + i) * G + k] = W[i];
-      alpha_out = alpha;
-    } else {
-      alpha_out = gsv(P, GS_ALPHA, k);
-    }
-  }
+  if (P.g_frame[k] == f)
+    alpha_out = mstep_update_one(P.gs, P.eldn, P.epack, P.fixed, G, k, F->psi_sum_alpha, st,
+                                 O.w4dof, O.scale_reg[0], O.scale_reg[1], O.scale_reg[2], O.lmin,
+                                 O.lmax, O.emax);
   const double t = block_sum<1>(alpha_out, red);
   if (threadIdx.x == 0) P.part_g2[blockIdx.x] = t;
 }
@@ -1684,7 +1965,12 @@ __global__ __launch_bounds__(kGB) void k_death(DevPtrs P, KOpts O) {
     lp.U[3] = pd ? (float)u11 : 0.0f;
     lp.U[4] = pd ? (float)u12 : 0.0f;
     lp.U[5] = pd ? (float)u22 : 0.0f;
-    lp.pad[0] = lp.pad[1] = 0.0f;
+    {
+      double f2 = 0.0;
+      for (int i = 0; i < 6; i++) f2 += (double)lp.U[i] * (double)lp.U[i];
+      lp.nf = (float)(sqrt(f2) * (1.0 + 1e-6));
+    }
+    lp.pad = 0.0f;
     const double dt = det3(W) * nu * nu * nu;
     lp.lg2 = (alive && alpha > 0.0 && dt > 0.0 && pd)
                  ? (log(alpha) + 0.5 * log(dt) - 1.5 * 1.8378770664093454836) * LOG2E
@@ -1732,6 +2018,19 @@ __global__ __launch_bounds__(64) void k_alpha_sum(DevPtrs P, int need_active) {
 // evaluation (its exponential or the whole term below 2^-1075) adds nothing,
 // and a voxel with no term left contributes nothing (GMEM.cu:1958-1959).
 //
+// Candidates that are negligible at every voxel of the supervoxel are left
+// out. With c the centre and r the radius of the ball around the
+// supervoxel's voxels (sv_ball) and nf >= |U|_2, every voxel x has
+// |U (x - m)| within sqrt(q_c) -+ nf r, where q_c is the form at c (FP64),
+// so t(x) = g - q(x) lies in [g - hi, g - lo]; both bounds are widened by far
+// more than the f32 evaluation's rounding. A candidate with g - hi >= -1000
+// and hi <= 1000 (never cut as underflowing) bounds every voxel's largest
+// term from below by L = max(g - hi); a candidate with g - lo < L - 40 then
+// adds less than 2^-40 of the largest term at every voxel, and is dropped.
+// The others keep their slot order. The decision depends only on the
+// supervoxel and its candidates, not on the launch shape. In voxel mode
+// every candidate is kept, as k_loglik_voxel keeps them.
+//
 // Summation order, identical for every (GW, SPG): voxel i of a supervoxel is
 // added, in increasing i, to class sum i mod 16; the 16 class sums are added
 // as ((c + c^8) + ...^4) ...^1; the sums of each 256 consecutive supervoxels
@@ -1743,7 +2042,8 @@ struct SvHead {
   long long beg;
   int n;
   float a0, a1, a2;
-  int kk;  // lane j < K: candidate j
+  float4 ball;  // centre relative to the anchor, radius
+  int kk;       // lane j < K: candidate j
 };
 
 __device__ inline SvHead ll_head(const DevPtrs& P, long long s, int f, int lane, int K) {
@@ -1755,6 +2055,7 @@ __device__ inline SvHead ll_head(const DevPtrs& P, long long s, int f, int lane,
   h.a0 = P.anchor[s];
   h.a1 = P.anchor[S + s];
   h.a2 = P.anchor[2 * S + s];
+  h.ball = P.sv_ball[s];
   h.kk = (lane < K) ? P.ind[(long long)lane * S + s] : -1;
   h.n = (sf == f) ? len : 0;
   return h;
@@ -1814,6 +2115,7 @@ __global__ __launch_bounds__(256) void k_loglik(DevPtrs P, KOpts O) {
     // candidate constants: the Cholesky factor U and e = U (a_s - m_k), FP64
     // from the FP64 mean then rounded, so y = U (x - a_s) + e = U (x - m_k)
     float pe0 = 0.0f, pe1 = 0.0f, pe2 = 0.0f, pu[6] = {0, 0, 0, 0, 0, 0}, pg = -INFINITY;
+    double tlo = -INFINITY, thi = INFINITY;  // bounds of the candidate's terms
     if (n > 0 && c.kk >= 0) {
       const LLPack lp = P.llpack[c.kk];
       const double d0 = (double)c.a0 - lp.m[0], d1 = (double)c.a1 - lp.m[1],
@@ -1824,7 +2126,37 @@ __global__ __launch_bounds__(256) void k_loglik(DevPtrs P, KOpts O) {
       pe1 = (float)((double)pu[3] * d1 + (double)pu[4] * d2);
       pe2 = (float)((double)pu[5] * d2);
       pg = (float)(lp.lg2 - lsa2);
+      {
+#pragma clang fp contract(off)
+        const double b0 = d0 + (double)c.ball.x, b1 = d1 + (double)c.ball.y,
+                     b2 = d2 + (double)c.ball.z;  // centre - mean
+        const double y0 = (double)pu[0] * b0 + (double)pu[1] * b1 + (double)pu[2] * b2;
+        const double y1 = (double)pu[3] * b1 + (double)pu[4] * b2;
+        const double y2 = (double)pu[5] * b2;
+        const double sq = sqrt(y0 * y0 + y1 * y1 + y2 * y2);
+        const double rr = (double)lp.nf * (double)c.ball.w;
+        const double err = 1e-5 * ((sq + 3.0 * rr) * (sq + 3.0 * rr) + fabs((double)pg) + 1.0);
+        const double lo = fmax(sq - rr, 0.0);
+        const double hi = (sq + rr) * (sq + rr) + err;
+        if (pg > -INFINITY) {
+          thi = (double)pg - (lo * lo - err);
+          if (hi <= 1000.0) tlo = (double)pg - hi;
+        }
+      }
     }
+    // the candidates kept (lanes j < K of the group), compacted in slot order
+    bool live;
+    {
+      double L = tlo;
+#pragma unroll
+      for (int m = 8; m >= 1; m >>= 1) L = fmax(L, __shfl_xor(L, m, 16));
+      live = lane < K && n > 0 && c.kk >= 0 && pg > -INFINITY &&
+             !(!O.voxel && L >= -1000.0 && thi < L - 40.0);
+    }
+    unsigned long long gm = __ballot(live);
+    if constexpr (GW < 64) gm = (gm >> ((threadIdx.x & 63) & ~(GW - 1))) & ((1ull << GW) - 1ull);
+    const int nlive = __popcll(gm);
+    const int slot = __popcll(gm & ((1ull << lane) - 1ull));
     // the group's candidate constants: a whole-wave group reads them into
     // scalars; smaller groups stage them in LDS (12 floats per candidate,
     // read back as broadcasts), so that only one candidate's constants are
@@ -1833,22 +2165,25 @@ __global__ __launch_bounds__(256) void k_loglik(DevPtrs P, KOpts O) {
         cg[kLds ? 1 : KM];
     if constexpr (kLds) {
       __syncthreads();  // the previous supervoxel's constants are no longer read
-      if (lane < KM) {
-        float4* d = cst + (grp * KM + lane) * 3;
+      if (live) {
+        float4* d = cst + (grp * KM + slot) * 3;
         d[0] = make_float4(pu[0], pu[1], pu[2], pe0);
         d[1] = make_float4(pu[3], pu[4], pe1, pu[5]);
-        d[2] = make_float4(pe2, lane < K ? pg : -INFINITY, 0.0f, 0.0f);
+        d[2] = make_float4(pe2, pg, 0.0f, 0.0f);
       }
       __syncthreads();
     } else {
+      unsigned long long mm = gm;
 #pragma unroll
       for (int j = 0; j < KM; j++) {
-        ce0[j] = group_bcast<GW>(pe0, j);
-        ce1[j] = group_bcast<GW>(pe1, j);
-        ce2[j] = group_bcast<GW>(pe2, j);
+        const int src = mm ? __builtin_ctzll(mm) : 0;
+        mm &= mm - 1ull;
+        ce0[j] = group_bcast<GW>(pe0, src);
+        ce1[j] = group_bcast<GW>(pe1, src);
+        ce2[j] = group_bcast<GW>(pe2, src);
 #pragma unroll
-        for (int i = 0; i < 6; i++) cu[j][i] = group_bcast<GW>(pu[i], j);
-        cg[j] = (j < K) ? group_bcast<GW>(pg, j) : -INFINITY;
+        for (int i = 0; i < 6; i++) cu[j][i] = group_bcast<GW>(pu[i], src);
+        cg[j] = group_bcast<GW>(pg, src);
       }
     }
     // candidate j's constants: U row-major (u00 u01 u02 u11 u12 u22), e, log2 weight
@@ -1891,6 +2226,11 @@ __global__ __launch_bounds__(256) void k_loglik(DevPtrs P, KOpts O) {
         }
 #pragma unroll
         for (int j = 0; j < KM; j++) {
+          if (j >= nlive) {
+#pragma unroll
+            for (int u = 0; u < U; u++) tj[u][j] = -INFINITY;
+            continue;
+          }
           float cu6[6], e0, e1, e2, g;
           cand(j, cu6, e0, e1, e2, g);
 #pragma unroll
@@ -1921,7 +2261,8 @@ __global__ __launch_bounds__(256) void k_loglik(DevPtrs P, KOpts O) {
         if (mx > -INFINITY) {
           float sum = 0.0f;
 #pragma unroll
-          for (int j = 0; j < KM; j++) sum += __builtin_amdgcn_exp2f(tj[u][j] - mx);
+          for (int j = 0; j < KM; j++)
+            if (j < nlive) sum += __builtin_amdgcn_exp2f(tj[u][j] - mx);
           t = (double)vv[u].w * (double)(mx + __builtin_amdgcn_logf(sum));
         }
         v[u] = (base + lane + GW * u < n) ? t : 0.0;
@@ -1955,10 +2296,10 @@ __global__ __launch_bounds__(256) void k_loglik(DevPtrs P, KOpts O) {
 // Log-likelihood in voxel mode, where every supervoxel is one voxel (or
 // none, as padding): one lane per supervoxel, 256 per block. A lane forms its
 // voxel's term with k_loglik's operations (the anchor is the voxel itself, so
-// y = U (x - a) + e = e), its supervoxel sum as k_loglik's class sums leave
-// it (0 + term), and the block the same pairwise tree over its 256
-// supervoxels: the partials, hence the totals and every convergence
-// decision, are bitwise those of k_loglik, without 15 idle lanes of 16.
+// y = U (x - a) + e = e) over all K candidates (bounding a one-voxel
+// supervoxel's terms costs as much as evaluating them), its supervoxel sum
+// as k_loglik's class sums leave it (0 + term), and the block the same
+// pairwise tree over its 256 supervoxels, without 15 idle lanes of 16.
 template <int KM>
 __global__ __launch_bounds__(256) void k_loglik_voxel(DevPtrs P, KOpts O) {
   __shared__ double tree[256];
@@ -2193,7 +2534,7 @@ template <int KM>
 static void launch_knn_layout_t(const DevPtrs& P, const KOpts& O, int only_frame, int max_ng,
                                 hipStream_t st) {
   const int words = (int)std::min<long long>(kLayoutWords, ((long long)max_ng + 63) / 64);
-  k_knn_layout<KM><<<nb(P.Sext, 64), 64, (size_t)std::max(words, 1) * 8, st>>>(P, O, only_frame,
+  k_knn_layout<KM><<<nb(P.Sext, 64), 64, scan_lds_bytes(words), st>>>(P, O, only_frame,
                                                                                words);
 }
 
@@ -2227,15 +2568,18 @@ static hipError_t iteration_t(const DevPtrs& P, const KOpts& O, hipStream_t st, 
   k_grid_scatter<<<gb, kGB, 0, st>>>(P);
   if (tk == TK_KNN && e0) (void)hipEventRecord(e0, st);
   launch_knn<KM>(P, O, st);
-  k_knn_exact<KM><<<4, 256, 0, st>>>(P, O);
+  {
+    const int words = (int)std::min<long long>(kLayoutWords, (P.Gext + 63) / 64);
+    k_knn_exact<KM><<<128, 64, scan_lds_bytes(words), st>>>(P, O, words);
+  }
   if (tk == TK_KNN && e1) (void)hipEventRecord(e1, st);
   if (tk == TK_ESTEP && e0) (void)hipEventRecord(e0, st);
   k_estep<KM><<<nb(P.Sext, bs), bs, 0, st>>>(P, O);
   if (tk == TK_ESTEP && e1) (void)hipEventRecord(e1, st);
   k_scan_frames<<<P.nframes, 1024, 0, st>>>(P, 1, 1, O.K);
   k_pair_scatter<<<nb(P.Sext, bs), bs, 0, st>>>(P, O);
-  k_seg_sort<<<nb(P.Gext, 16), 256, 0, st>>>(P, O.voxel ? kSortShort : INT_MAX);
-  if (O.voxel) k_seg_sort_long<<<(unsigned)P.Gext, 256, 0, st>>>(P);
+  k_seg_sort<<<nb(P.Gext, 16), 256, 0, st>>>(P);
+  k_seg_sort_long<<<64, 256, 0, st>>>(P);
   if (tk == TK_MSTEP && e0) (void)hipEventRecord(e0, st);
   launch_mstep(P, O, st);
   if (tk == TK_MSTEP && e1) (void)hipEventRecord(e1, st);

@@ -92,6 +92,7 @@ int ws_alloc(Workspace* w, const WsShape& sh, int K) {
   AL(P.mom, 10 * S);
   AL(P.wmom, 10 * S);
   AL(P.anchor, 3 * S);
+  AL(P.sv_ball, S);
   AL(P.ind, (long long)K * S);
   AL(P.rnk, (long long)K * S);
   AL(P.sblk_frame, S / kAlign);
@@ -118,6 +119,7 @@ int ws_alloc(Workspace* w, const WsShape& sh, int K) {
   AL(P.mknn, G);
   AL(P.tie_list, S);
   AL(P.knn_ctl, 64);
+  AL(P.long_list, G);
   AL(P.win, S);
   AL(P.rec, G);
   AL(P.svkeep, 17 * G);
@@ -139,9 +141,9 @@ void ws_free(Workspace* w) {
 
 long long ws_bytes_estimate(const WsShape& sh, int K) {
   const long long S = sh.Scap, G = sh.Gcap, C = sh.Ccap, V = sh.Vcap;
-  return 16 * V + S * (16 + 8 + 4 + 12 + 80 + 80 + 12 + 16LL * K + 1 + 4 + 8) +
+  return 16 * V + S * (16 + 8 + 4 + 12 + 80 + 80 + 12 + 16 + 16LL * K + 1 + 4 + 8) +
          G * (4 + 8 * GS_NFIELDS * 2 + 1 + 96 + 8 + 64 + 12 + 4 + 8 + 12 + 4 + 1 + 16 + 288 +
-              68) +
+              68 + 4) +
          8 * C + 64 * sh.Fcap;
 }
 
