@@ -422,6 +422,22 @@ typedef struct tgmm_raw_frame {
 int tgmm_marshal_frame(const tgmm_raw_frame* frame, float* queryCUDA, float* imgDataCUDA,
                        float* centroidLabelPositionCUDA, long long* labelListPtrCUDA);
 
+/* No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc). tgmm_marshal_frame with the same outputs, bitwise,
+ * that allocates nothing: its working buffers (the uploaded supervoxel
+ * pointer, the intensity range, the block partials) live in a 16 MiB
+ * device array of the library, one per device, kept until the process
+ * ends. The inputs must be device memory (device_inputs = 1). Frames whose
+ * buffers exceed the array (about 1.97 million supervoxels) return
+ * TGMM_ERR_LIMIT. Every call on a device shares that device's array and
+ * leaves its contents behind; two calls overlapping on one device would race
+ * on it, so the library serialises the calls of a process and each call
+ * returns only after its kernels have finished (each process has its own
+ * array). */
+int tgmm_marshal_frame_noalloc(const tgmm_raw_frame* frame, float* queryCUDA, float* imgDataCUDA,
+                               float* centroidLabelPositionCUDA, long long* labelListPtrCUDA);
+
 /* tgmm_GMEMinitializeMemoryWithSupervoxels (GMEMupdateCUDA.h:132-137) fed
  * from a raw frame instead of the host float arrays: allocates the same
  * seven device arrays (query_nb = frame->N, numLabels = frame->S <= 65535),

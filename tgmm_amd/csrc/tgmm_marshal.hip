@@ -205,6 +205,15 @@ struct MarshalWs {
 std::mutex g_mu;
 std::vector<MarshalWs> g_ws;
 
+// The allocation-free variant's scratch: a file-scope device array (each
+// device holds its own copy until the process ends), whose device address is
+// looked up once per device with that device current. Calls are serialised
+// by g_mu; every call overwrites what it reads (the supervoxel pointer, the
+// block partials, the range; the error flag is cleared by a kernel).
+constexpr size_t kScratchBytes = size_t(16) << 20;
+__device__ __align__(256) unsigned char scratch[kScratchBytes];
+std::vector<void*> g_scratch;
+
 int hip_fail(hipError_t e, const char* what) {
   tgmm_set_error("marshal_frame: %s failed: %s", what, hipGetErrorString(e));
   return e == hipErrorOutOfMemory ? TGMM_ERR_OOM : TGMM_ERR_DEVICE;
@@ -238,6 +247,48 @@ int check_frame(const tgmm_raw_frame* f) {
   if ((double)f->N > vol) {
     tgmm_set_error("marshal_frame: %lld voxels listed in a volume of %.0f: supervoxels must be "
                    "disjoint", f->N, vol);
+    return TGMM_ERR_ARG;
+  }
+  return TGMM_OK;
+}
+
+// The marshalling on the default stream, from device inputs (idx, raw,
+// thr), with the device buffers it needs besides the outputs: the supervoxel
+// pointer (uploaded here from f->sv_ptr), the range and error flag, and the
+// block partials of the range.
+int marshal_run(const tgmm_raw_frame* f, const unsigned long long* idx, const unsigned short* raw,
+                const unsigned short* thr, long long* ptr, unsigned* range, unsigned* part_hi,
+                unsigned* part_lo, int nsb, float* queryCUDA, float* imgDataCUDA,
+                float* centroidLabelPositionCUDA, long long* labelListPtrCUDA) {
+  const long long N = f->N, S = f->S;
+  hipError_t e;
+  if ((e = hipMemcpyAsync(ptr, f->sv_ptr, (size_t)(S + 1) * 8, hipMemcpyHostToDevice, 0)) !=
+      hipSuccess)
+    return hip_fail(e, "upload of the supervoxel pointer");
+  if (labelListPtrCUDA &&
+      (e = hipMemcpyAsync(labelListPtrCUDA, ptr, (size_t)(S + 1) * 8, hipMemcpyDeviceToDevice,
+                          0)) != hipSuccess)
+    return hip_fail(e, "copy of the supervoxel pointer");
+  k_mar_init<<<1, 64, 0, 0>>>(range);
+  if (S > 0)
+    k_mar_sv<<<nsb, kMB, 0, 0>>>(
+        idx, raw, f->volume, ptr, thr, S, N, (unsigned long long)f->dims[0],
+        (unsigned long long)f->dims[1],
+        (unsigned long long)f->dims[0] * (unsigned long long)f->dims[1] *
+            (unsigned long long)f->dims[2],
+        1.0 / (double)f->dims[0], 1.0 / (double)f->dims[1], queryCUDA, imgDataCUDA,
+        centroidLabelPositionCUDA, part_hi, part_lo, range);
+  if (N > 0) k_mar_range<<<1, 1024, 0, 0>>>(part_hi, part_lo, nsb, range);
+  const double vol = (double)f->dims[0] * (double)f->dims[1] * (double)f->dims[2];
+  if (N > 0)
+    k_mar_scale<<<nblocks(N, kMB), kMB, 0, 0>>>(imgDataCUDA, N, range, (double)N < vol ? 1 : 0);
+  if ((e = hipGetLastError()) != hipSuccess) return hip_fail(e, "kernel launch");
+  unsigned hrange[3];
+  if ((e = hipMemcpy(hrange, range, sizeof(hrange), hipMemcpyDeviceToHost)) != hipSuccess)
+    return hip_fail(e, "read-back of the range");
+  if (hrange[2]) {
+    tgmm_set_error("marshal_frame: a voxel index lies outside the %lld x %lld x %lld volume",
+                   f->dims[0], f->dims[1], f->dims[2]);
     return TGMM_ERR_ARG;
   }
   return TGMM_OK;
@@ -296,34 +347,55 @@ extern "C" int tgmm_marshal_frame(const tgmm_raw_frame* f, float* queryCUDA, flo
     raw = raw_d;
     thr = thr_d;
   }
-  if ((e = hipMemcpyAsync(ptr, f->sv_ptr, (size_t)(S + 1) * 8, hipMemcpyHostToDevice, 0)) !=
-      hipSuccess)
-    return hip_fail(e, "upload of the supervoxel pointer");
-  if (labelListPtrCUDA &&
-      (e = hipMemcpyAsync(labelListPtrCUDA, ptr, (size_t)(S + 1) * 8, hipMemcpyDeviceToDevice,
-                          0)) != hipSuccess)
-    return hip_fail(e, "copy of the supervoxel pointer");
-  k_mar_init<<<1, 64, 0, 0>>>(range);
-  if (S > 0)
-    k_mar_sv<<<nsb, kMB, 0, 0>>>(
-        idx, raw, f->volume, ptr, thr, S, N, (unsigned long long)f->dims[0],
-        (unsigned long long)f->dims[1],
-        (unsigned long long)f->dims[0] * (unsigned long long)f->dims[1] *
-            (unsigned long long)f->dims[2],
-        1.0 / (double)f->dims[0], 1.0 / (double)f->dims[1], queryCUDA, imgDataCUDA,
-        centroidLabelPositionCUDA, part_hi, part_lo, range);
-  if (N > 0) k_mar_range<<<1, 1024, 0, 0>>>(part_hi, part_lo, nsb, range);
-  const double vol = (double)f->dims[0] * (double)f->dims[1] * (double)f->dims[2];
-  if (N > 0)
-    k_mar_scale<<<nblocks(N, kMB), kMB, 0, 0>>>(imgDataCUDA, N, range, (double)N < vol ? 1 : 0);
-  if ((e = hipGetLastError()) != hipSuccess) return hip_fail(e, "kernel launch");
-  unsigned hrange[3];
-  if ((e = hipMemcpy(hrange, range, sizeof(hrange), hipMemcpyDeviceToHost)) != hipSuccess)
-    return hip_fail(e, "read-back of the range");
-  if (hrange[2]) {
-    tgmm_set_error("marshal_frame: a voxel index lies outside the %lld x %lld x %lld volume",
-                   f->dims[0], f->dims[1], f->dims[2]);
+  return marshal_run(f, idx, raw, thr, ptr, range, part_hi, part_lo, nsb, queryCUDA, imgDataCUDA,
+                     centroidLabelPositionCUDA, labelListPtrCUDA);
+}
+
+// tgmm_marshal_frame with device inputs, allocating nothing: the supervoxel
+// pointer, the range and the block partials live in the scratch array.
+extern "C" int tgmm_marshal_frame_noalloc(const tgmm_raw_frame* f, float* queryCUDA,
+                                          float* imgDataCUDA, float* centroidLabelPositionCUDA,
+                                          long long* labelListPtrCUDA) {
+  int rc = check_frame(f);
+  if (rc) return rc;
+  const long long N = f->N, S = f->S;
+  if ((N > 0 && (!queryCUDA || !imgDataCUDA)) || (S > 0 && !centroidLabelPositionCUDA)) {
+    tgmm_set_error("marshal_frame_noalloc: NULL output");
     return TGMM_ERR_ARG;
   }
-  return TGMM_OK;
+  if (!f->device_inputs) {
+    tgmm_set_error("marshal_frame_noalloc: the inputs must be device memory (device_inputs = 1); "
+                   "host inputs need staging that this entry point does not allocate");
+    return TGMM_ERR_ARG;
+  }
+  const int nsb = S > 0 ? (int)nblocks(S, kMB / kMG) : 0;
+  const size_t o_ptr = 0;
+  const size_t o_rng = o_ptr + up256((size_t)(S + 1) * 8);
+  const size_t o_part = o_rng + 256;
+  const size_t part_b = up256((size_t)nsb * sizeof(unsigned));
+  const size_t total = o_part + 2 * part_b;
+  if (total > kScratchBytes) {
+    tgmm_set_error("marshal_frame_noalloc: %lld supervoxels need %zu bytes of scratch; the "
+                   "allocation-free entry point holds %zu (about 1.97 million supervoxels)",
+                   S, total, kScratchBytes);
+    return TGMM_ERR_LIMIT;
+  }
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return hip_fail(e, "hipGetDevice");
+  std::lock_guard<std::mutex> lk(g_mu);
+  if ((int)g_scratch.size() <= dev) g_scratch.resize(dev + 1, nullptr);
+  if (!g_scratch[dev]) {
+    void* p = nullptr;
+    if ((e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch))) != hipSuccess)
+      return hip_fail(e, "hipGetSymbolAddress of the scratch array");
+    g_scratch[dev] = p;
+  }
+  char* base = (char*)g_scratch[dev];
+  auto* ptr = (long long*)(base + o_ptr);
+  auto* range = (unsigned*)(base + o_rng);
+  auto* part_hi = (unsigned*)(base + o_part);
+  auto* part_lo = (unsigned*)(base + o_part + part_b);
+  return marshal_run(f, f->pixel_idx, f->raw, f->thr, ptr, range, part_hi, part_lo, nsb,
+                     queryCUDA, imgDataCUDA, centroidLabelPositionCUDA, labelListPtrCUDA);
 }
