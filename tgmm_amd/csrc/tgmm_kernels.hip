@@ -26,6 +26,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <type_traits>
 #include <climits>
 
 #include "tgmm_device.h"
@@ -55,24 +56,27 @@ __device__ inline double block_sum(double v, double* red) {
   return t;
 }
 
-__device__ inline double load_coherent(const double* p) {
-  unsigned long long u =
-      __hip_atomic_load((const unsigned long long*)p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  return __longlong_as_double((long long)u);
-}
-
 // Sum of a frame's per-block partials by one wave, in an order fixed by the
 // partial index alone (lane l takes partials l, l+64, ... into four
 // interleaved sums, then a fixed butterfly). Every lane returns the total.
+// The partials were written by an earlier kernel, or by the calling block
+// before a barrier.
 __device__ inline double frame_sum_w64(const double* part, int nblk) {
   double a4[4] = {0.0, 0.0, 0.0, 0.0};
   const int l = threadIdx.x;
   int i = l;
+  for (; i + 960 < nblk; i += 1024) {  // 16 loads in flight, the same order
+    double v[16];
+#pragma unroll
+    for (int t = 0; t < 16; t++) v[t] = part[i + 64 * t];
+#pragma unroll
+    for (int t = 0; t < 16; t++) a4[t & 3] += v[t];
+  }
   for (; i + 192 < nblk; i += 256) {
 #pragma unroll
-    for (int q = 0; q < 4; q++) a4[q] += load_coherent(part + i + 64 * q);
+    for (int q = 0; q < 4; q++) a4[q] += part[i + 64 * q];
   }
-  for (int q = 0; i < nblk; i += 64, q++) a4[q & 3] += load_coherent(part + i);
+  for (int q = 0; i < nblk; i += 64, q++) a4[q & 3] += part[i];
   return wave_sum((a4[0] + a4[1]) + (a4[2] + a4[3]));
 }
 
@@ -401,6 +405,7 @@ __global__ __launch_bounds__(kGB) void k_alpha_init(DevPtrs P) {
 
 // ---------------------------------------------------------------- scans --
 constexpr int kSortShort = 128;  // longer pair lists are sorted by a block each
+constexpr int kScanPer = 32;     // entries per thread and pass, k_scan_frames
 constexpr int kSortLds = 4096;   // longest list sorted in LDS
 
 // Exclusive scan of 1024 per-thread values; returns this thread's prefix.
@@ -465,58 +470,69 @@ __global__ __launch_bounds__(1024) void k_scan_frames(DevPtrs P, int which, int 
     n = F->ng;
     base = (int)(K * F->s0);
   }
-  // each thread scans a contiguous chunk of the n + 1 entries (entry n
-  // carries the total): its chunk's sum, one block scan of the sums, then its
-  // chunk's prefixes; every load of a pass is issued at once
+  // 1024 * kScanPer entries per pass, each thread a contiguous run of
+  // kScanPer (its loads issued together): the runs' sums, one block scan,
+  // then each run's prefixes; entry n carries the total
   const long long n1 = n + 1;
-  const long long per = ((n1 + 1023) / 1024 + 3) & ~3LL;  // a multiple of 4 (int4 loads)
-  const long long b = (long long)threadIdx.x * per;
-  const long long e = b + per < n ? b + per : n;  // entries [b, e) are inputs
-  int tsum = 0;
   // the pair scan lists the Gaussians whose pair list is long, for the
-  // block-wide sort (k_seg_sort_long)
+  // block-wide sort (k_seg_sort)
   const bool list_long = which == 1;
-  auto note = [&](long long i, int c) {
-    if (list_long && c > kSortShort) P.long_list[atomicAdd(P.knn_ctl + 2, 1)] = (int)(F->g0 + i);
-  };
-  if (b + per <= n) {
-    const int4* p4 = reinterpret_cast<const int4*>(in + b);
-    for (long long q = 0; q < per / 4; q++) {
-      const int4 t = p4[q];
-      tsum += (t.x + t.y) + (t.z + t.w);
-      if (list_long && (t.x > kSortShort || t.y > kSortShort || t.z > kSortShort ||
-                        t.w > kSortShort)) {
-        note(b + 4 * q, t.x);
-        note(b + 4 * q + 1, t.y);
-        note(b + 4 * q + 2, t.z);
-        note(b + 4 * q + 3, t.w);
+  __shared__ int total;
+  int carry = base;
+  for (long long c0 = 0; c0 < n1; c0 += 1024LL * kScanPer) {
+    const long long b = c0 + (long long)threadIdx.x * kScanPer;
+    int v[kScanPer];
+    if (b + kScanPer <= n) {
+      const int4* p4 = reinterpret_cast<const int4*>(in + b);
+#pragma unroll
+      for (int q = 0; q < kScanPer / 4; q++) {
+        const int4 t = p4[q];
+        v[4 * q] = t.x;
+        v[4 * q + 1] = t.y;
+        v[4 * q + 2] = t.z;
+        v[4 * q + 3] = t.w;
       }
+    } else {
+#pragma unroll
+      for (int q = 0; q < kScanPer; q++) v[q] = (b + q < n) ? in[b + q] : 0;
     }
-  } else {
-    for (long long i = b; i < e; i++) {
-      tsum += in[i];
-      note(i, in[i]);
+    int tsum = 0;
+    bool any_long = false;
+#pragma unroll
+    for (int q = 0; q < kScanPer; q++) {
+      tsum += v[q];
+      any_long |= v[q] > kSortShort;
     }
-  }
-  int run = base + block_exclusive_scan1024(tsum, lds);
-  if (b + per <= n) {  // a whole chunk of inputs: int4 loads and stores
-    const int4* p4 = reinterpret_cast<const int4*>(in + b);
-    int4* o4 = reinterpret_cast<int4*>(out + b);
-    for (long long q = 0; q < per / 4; q++) {
-      const int4 t = p4[q];
-      int4 r;
-      r.x = run;
-      r.y = r.x + t.x;
-      r.z = r.y + t.y;
-      r.w = r.z + t.z;
-      run = r.w + t.w;
-      o4[q] = r;
+    if (list_long && any_long) {
+#pragma unroll
+      for (int q = 0; q < kScanPer; q++)
+        if (v[q] > kSortShort) P.long_list[atomicAdd(P.knn_ctl + 2, 1)] = (int)(F->g0 + b + q);
     }
-  } else {  // the chunk holding entry n (the total) or past it
-    for (long long i = b; i < b + per && i < n1; i++) {
-      out[i] = run;
-      if (i < n) run += in[i];
+    int run = carry + block_exclusive_scan1024(tsum, lds);
+    if (threadIdx.x == 1023) total = run + tsum;
+    if (b + kScanPer <= n1) {
+      int4* o4 = reinterpret_cast<int4*>(out + b);
+#pragma unroll
+      for (int q = 0; q < kScanPer / 4; q++) {
+        int4 r;
+        r.x = run;
+        r.y = r.x + v[4 * q];
+        r.z = r.y + v[4 * q + 1];
+        r.w = r.z + v[4 * q + 2];
+        run = r.w + v[4 * q + 3];
+        o4[q] = r;
+      }
+    } else {
+#pragma unroll
+      for (int q = 0; q < kScanPer; q++)
+        if (b + q < n1) {
+          out[b + q] = run;
+          run += v[q];
+        }
     }
+    __syncthreads();
+    carry = total;
+    __syncthreads();  // total and lds are rewritten by the next pass
   }
 }
 
@@ -1041,24 +1057,28 @@ struct ScanSlots {
     a = (q2 - r.z) * s2;
     d = p2 + a * a;
   }
+  __device__ inline float threshold() const { return thr; }
+  __device__ inline float dist(int j) const { return md[j]; }
   __device__ inline bool passes(float p1, float p2, float d) const {
     return !(p1 > thr) && !(p2 > thr) && !(d > thr);
   }
   __device__ inline void insert(float d, int k, int K) {
 #pragma unroll
-    for (int j = 0; j < KM; j++)
-      if (j == pos) {
-        md[j] = d;
-        mi[j] = k;
-      }
-    thr = md[0];
-    pos = 0;
+    for (int j = 0; j < KM; j++) {
+      const bool at = j == pos;
+      md[j] = at ? d : md[j];
+      mi[j] = at ? k : mi[j];
+    }
+    float t = md[0];
+    int p = 0;
 #pragma unroll
-    for (int j = 1; j < KM; j++)
-      if (j < K && thr < md[j]) {
-        thr = md[j];
-        pos = j;
-      }
+    for (int j = 1; j < KM; j++) {
+      const bool c = j < K && t < md[j];
+      t = c ? md[j] : t;
+      p = c ? j : p;
+    }
+    thr = t;
+    pos = p;
   }
   __device__ inline void take(float q0, float q1, float q2, float4 r, int k, float s0, float s1,
                               float s2, int K) {
@@ -1068,6 +1088,53 @@ struct ScanSlots {
     if (passes(p1, p2, d)) insert(d, k, K);
   }
 };
+
+// The same slots for a wave that scans one query (every lane holds the same
+// state). Insertion and the search for the largest slot are written as
+// selects, so that they run without branches: the slot index is wave-uniform,
+// and a branch per slot costs more than the selects themselves.
+template <int KM>
+struct ScanSlotsOne {
+  float md[KM];
+  int mi[KM];
+  float thr;
+  int pos;
+  __device__ inline void init(bool) {
+#pragma unroll
+    for (int j = 0; j < KM; j++) {
+      md[j] = 1e32f;
+      mi[j] = -1;
+    }
+    thr = 1e32f;
+    pos = 0;
+  }
+  __device__ inline float threshold() const { return thr; }
+  __device__ inline float dist(int j) const { return md[j]; }
+  __device__ inline bool passes(float p1, float p2, float d) const {
+    return !(p1 > thr) && !(p2 > thr) && !(d > thr);
+  }
+  __device__ inline void insert(float d, int k, int K) {
+#pragma unroll
+    for (int j = 0; j < KM; j++) {
+      const bool at = j == pos;
+      md[j] = at ? d : md[j];
+      mi[j] = at ? k : mi[j];
+    }
+    float t = md[0];
+    int p = 0;
+#pragma unroll
+    for (int j = 1; j < KM; j++) {
+      const bool c = j < K && t < md[j];
+      t = c ? md[j] : t;
+      p = c ? j : p;
+    }
+    thr = t;
+    pos = p;
+  }
+};
+
+template <int KM, bool ONE>
+using SlotsOf = typename std::conditional<ONE, ScanSlotsOne<KM>, ScanSlots<KM>>::type;
 
 // The scan replayed exactly against the frame's Gaussians of its last kNN
 // search (mknn, in index order, and its grid), in one of two shapes:
@@ -1100,6 +1167,7 @@ struct ScanSlots {
 constexpr int kLayoutCells = 8192;  // most grid cells marked per wave
 constexpr int kLayoutWords = 4096;  // largest bitmap (Gaussians / 64) in LDS: 32 KiB
 constexpr int kListCap = 1024;      // marked Gaussians listed at a time
+constexpr int kMarkU = 8;           // grid entries a lane marks per round
 
 struct ScanBox {
   double qlo[3], qhi[3];  // the wave's queries (scaled)
@@ -1109,8 +1177,8 @@ struct ScanBox {
 
 // Refreshes the box from the current thresholds (wave-uniform).
 template <int KM, bool ONE>
-__device__ inline void scan_box(ScanBox& b, const ScanSlots<KM>& sl, bool live) {
-  float R2 = live ? sl.thr : 0.0f;
+__device__ inline void scan_box(ScanBox& b, const SlotsOf<KM, ONE>& sl, bool live) {
+  float R2 = live ? sl.threshold() : 0.0f;
   bool wide = !(R2 < 1e32f);
   if constexpr (!ONE) {
 #pragma unroll
@@ -1128,7 +1196,7 @@ __device__ inline void scan_box(ScanBox& b, const ScanSlots<KM>& sl, bool live) 
 // Up to 64 Gaussians, one a lane (index k relative to the frame, -1 past the
 // end, in increasing order over the lanes), through the scan.
 template <int KM, bool ONE>
-__device__ inline void scan_take64(ScanSlots<KM>& sl, const ScanBox& box, float q0, float q1,
+__device__ inline void scan_take64(SlotsOf<KM, ONE>& sl, const ScanBox& box, float q0, float q1,
                                    float q2, float4 r, int k, float s0, float s1, float s2,
                                    int K) {
   const int lane = threadIdx.x & 63;
@@ -1168,7 +1236,7 @@ __device__ inline void scan_take64(ScanSlots<KM>& sl, const ScanBox& box, float 
 // together), the box refreshed every round. idx(i) is the i-th Gaussian's
 // frame-relative index (increasing in i).
 template <int KM, bool ONE, class Idx>
-__device__ inline void scan_list(const float4* mk, int n, Idx idx, ScanSlots<KM>& sl, ScanBox& box,
+__device__ inline void scan_list(const float4* mk, int n, Idx idx, SlotsOf<KM, ONE>& sl, ScanBox& box,
                                  bool live, float q0, float q1, float q2, float s0, float s1,
                                  float s2, int K) {
   constexpr int U = ONE ? 8 : 4;  // loads a lane in flight
@@ -1197,7 +1265,7 @@ inline size_t scan_lds_bytes(int words) {
 template <int KM, bool ONE>
 __device__ inline void scan_replay(const DevPtrs& P, const KOpts& O, const FrameDev* F, bool live,
                                    float q0, float q1, float q2, int words_cap,
-                                   unsigned long long* bits, ScanSlots<KM>& sl) {
+                                   unsigned long long* bits, SlotsOf<KM, ONE>& sl) {
   const int lane = threadIdx.x & 63;
   const long long g0 = F->g0;
   const int ng = (int)F->ng;
@@ -1262,8 +1330,8 @@ __device__ inline void scan_replay(const DevPtrs& P, const KOpts& O, const Frame
   for (int w = lane; w < words; w += 64) bits[w] = 0ull;
   // The box's rows of cells along x are contiguous runs of cg4: each lane
   // takes a row (its run's ends), the runs are laid end to end by a prefix
-  // over the lanes (kept in the list area), and their entries are marked 64
-  // at a time, four loads a lane in flight.
+  // over the lanes (kept in the list area), and their entries are marked
+  // 64 * kMarkU at a time, kMarkU loads a lane in flight.
   const int ny = ch[1] - cl[1] + 1;
   const int nrow = ny * (ch[2] - cl[2] + 1);
   int* rpre = list;       // 64 prefixes
@@ -1288,10 +1356,10 @@ __device__ inline void scan_replay(const DevPtrs& P, const KOpts& O, const Frame
     rpre[lane] = x - len;
     rbeg[lane] = b;
     __syncthreads();
-    for (int j0 = 0; j0 < tot; j0 += 256) {
-      int pk[4];
+    for (int j0 = 0; j0 < tot; j0 += 64 * kMarkU) {
+      int pk[kMarkU];
 #pragma unroll
-      for (int t = 0; t < 4; t++) {
+      for (int t = 0; t < kMarkU; t++) {
         const int j = j0 + 64 * t + lane;
         pk[t] = -1;
         if (j < tot) {
@@ -1302,11 +1370,11 @@ __device__ inline void scan_replay(const DevPtrs& P, const KOpts& O, const Frame
           pk[t] = rbeg[lo] + (j - rpre[lo]);
         }
       }
-      int kk[4];
+      int kk[kMarkU];
 #pragma unroll
-      for (int t = 0; t < 4; t++) kk[t] = pk[t] >= 0 ? __float_as_int(P.cg4[pk[t]].w) : -1;
+      for (int t = 0; t < kMarkU; t++) kk[t] = pk[t] >= 0 ? __float_as_int(P.cg4[pk[t]].w) : -1;
 #pragma unroll
-      for (int t = 0; t < 4; t++) {
+      for (int t = 0; t < kMarkU; t++) {
         const int k = pk[t] >= 0 ? (int)(kk[t] - g0) - m : -1;
         if (k >= 0) atomicOr(bits + (k >> 6), 1ull << (k & 63));
       }
@@ -1363,7 +1431,7 @@ __global__ __launch_bounds__(64) void k_knn_exact(DevPtrs P, KOpts O, int words_
     const long long s = P.tie_list[t];
     const FrameDev* F = P.fr + P.sv_frame[s];
     const float q0 = P.sv_cent[s], q1 = P.sv_cent[S + s], q2 = P.sv_cent[2 * S + s];
-    ScanSlots<KM> sl;
+    ScanSlotsOne<KM> sl;
     scan_replay<KM, true>(P, O, F, true, q0, q1, q2, words_cap, bits, sl);
     if (threadIdx.x == 0) {
       float od[KM];
@@ -1377,7 +1445,7 @@ __global__ __launch_bounds__(64) void k_knn_exact(DevPtrs P, KOpts O, int words_
       float kd = INFINITY;
 #pragma unroll
       for (int j = 0; j < KM; j++)
-        if (j < K && sl.mi[j] >= 0) knn_try<KM>(od, oi, sl.md[j], sl.mi[j], K, unused, kd);
+        if (j < K && sl.mi[j] >= 0) knn_try<KM>(od, oi, sl.dist(j), sl.mi[j], K, unused, kd);
 #pragma unroll
       for (int j = 0; j < KM; j++)
         if (j < K) P.ind[(long long)j * S + s] = od[j] <= 1e32f ? (int)(F->g0 + oi[j]) : -1;
@@ -1648,12 +1716,16 @@ __device__ inline void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// 16 lanes per Gaussian: order its pair list by code (= by supervoxel), so
-// that the FP64 accumulation order is fixed: a bitonic sort in LDS over the
-// next power of two (codes are distinct: a supervoxel lists a Gaussian at
-// most once). Lists longer than kSortShort are left to k_seg_sort_long.
-__global__ __launch_bounds__(256) void k_seg_sort(DevPtrs P) {
-  __shared__ int sh[16 * kSortShort];
+// Pair lists ordered by code (= by supervoxel), so that the FP64
+// accumulation order is fixed; codes are distinct (a supervoxel lists a
+// Gaussian at most once). The first nshort blocks take 16 Gaussians each, 16
+// lanes per Gaussian: a bitonic sort in LDS over the next power of two, for
+// lists up to kSortShort. The kSortLongBlocks blocks after them take the
+// Gaussians with longer lists (listed by the pair scan), one at a time: a
+// bitonic sort in LDS, or a block-wide rank sort beyond kSortLds entries.
+constexpr int kSortLongBlocks = 64;
+
+__device__ inline void seg_sort_short(const DevPtrs& P, int* sh) {
   const int grp = threadIdx.x >> 4, lane = threadIdx.x & 15;
   const long long k = (long long)blockIdx.x * 16 + grp;
   const int f = P.gblk_frame[k / kGB];
@@ -1685,46 +1757,50 @@ __global__ __launch_bounds__(256) void k_seg_sort(DevPtrs P) {
   for (int i = lane; i < L; i += 16) P.pairs_sorted[st + i] = a[i];
 }
 
-// The Gaussians with long pair lists (listed by the pair scan), one block
-// each: bitonic sort in LDS (codes are distinct: a supervoxel lists a
-// Gaussian at most once), or a block-wide rank sort beyond kSortLds entries.
-__global__ __launch_bounds__(256) void k_seg_sort_long(DevPtrs P) {
-  __shared__ int sh[kSortLds];
+__device__ inline void seg_sort_long(const DevPtrs& P, int* sh, int first) {
   const int cnt = P.knn_ctl[2];
-  for (int li = blockIdx.x; li < cnt; li += gridDim.x) {
-  const long long k = P.long_list[li];
-  const int st = P.seg_start[k], L = P.seg_start[k + 1] - st;
-  const int t = threadIdx.x;
-  if (L > kSortLds) {
-    for (int i = t; i < L; i += 256) {
-      const int e = P.pairs[st + i];
-      int rank = 0;
-      for (int j = 0; j < L; j++) rank += (P.pairs[st + j] < e) ? 1 : 0;
-      P.pairs_sorted[st + rank] = e;
+  for (int li = first; li < cnt; li += kSortLongBlocks) {
+    const long long k = P.long_list[li];
+    const int st = P.seg_start[k], L = P.seg_start[k + 1] - st;
+    const int t = threadIdx.x;
+    if (L > kSortLds) {
+      for (int i = t; i < L; i += 256) {
+        const int e = P.pairs[st + i];
+        int rank = 0;
+        for (int j = 0; j < L; j++) rank += (P.pairs[st + j] < e) ? 1 : 0;
+        P.pairs_sorted[st + rank] = e;
+      }
+      continue;
     }
-    continue;
-  }
-  int n = 1;
-  while (n < L) n <<= 1;
-  for (int i = t; i < n; i += 256) sh[i] = i < L ? P.pairs[st + i] : INT_MAX;
-  __syncthreads();
-  for (int w = 2; w <= n; w <<= 1)
-    for (int j = w >> 1; j > 0; j >>= 1) {
-      for (int i = t; i < n; i += 256) {
-        const int p = i ^ j;
-        if (p > i) {
-          const int a = sh[i], b = sh[p];
-          if ((a > b) == ((i & w) == 0)) {
-            sh[i] = b;
-            sh[p] = a;
+    int n = 1;
+    while (n < L) n <<= 1;
+    for (int i = t; i < n; i += 256) sh[i] = i < L ? P.pairs[st + i] : INT_MAX;
+    __syncthreads();
+    for (int w = 2; w <= n; w <<= 1)
+      for (int j = w >> 1; j > 0; j >>= 1) {
+        for (int i = t; i < n; i += 256) {
+          const int p = i ^ j;
+          if (p > i) {
+            const int a = sh[i], b = sh[p];
+            if ((a > b) == ((i & w) == 0)) {
+              sh[i] = b;
+              sh[p] = a;
+            }
           }
         }
+        __syncthreads();
       }
-      __syncthreads();
-    }
-  for (int i = t; i < L; i += 256) P.pairs_sorted[st + i] = sh[i];
-  __syncthreads();  // sh is free for the next list
+    for (int i = t; i < L; i += 256) P.pairs_sorted[st + i] = sh[i];
+    __syncthreads();  // sh is free for the next list
   }
+}
+
+__global__ __launch_bounds__(256) void k_seg_sort(DevPtrs P, int nshort) {
+  __shared__ int sh[kSortLds > 16 * kSortShort ? kSortLds : 16 * kSortShort];
+  if ((int)blockIdx.x < nshort)
+    seg_sort_short(P, sh);
+  else
+    seg_sort_long(P, sh, (int)blockIdx.x - nshort);
 }
 
 // Posterior (GMEM.cu:1601-1677) and regularisation (GMEM.cu:1693-1809) of
@@ -2378,19 +2454,32 @@ __global__ __launch_bounds__(256) void k_loglik_voxel(DevPtrs P, KOpts O) {
 // 256-supervoxel tree, then frame_sum_w64 over the frame's groups; the
 // convergence test (GMEM.cu:4145-4148) and the iteration count.
 template <int BPG>
-__global__ __launch_bounds__(256) void k_ll_final(DevPtrs P) {
+constexpr int ll_final_threads() { return BPG >= 32 ? 256 : 1024; }
+
+template <int BPG>
+__global__ __launch_bounds__(ll_final_threads<BPG>()) void k_ll_final(DevPtrs P) {
   const int f = blockIdx.x;
   FrameDev* F = P.fr + f;
   if (!F->active) return;
   const long long g0 = F->s0 >> 8;
   const int ng = F->nbl;
-  for (int g = threadIdx.x; g < ng; g += 256) {
-    double q[BPG];
+  // the groups' trees, kLLG groups a thread at a time (their loads together)
+  constexpr int NT = ll_final_threads<BPG>();
+  constexpr int kLLG = BPG >= 16 ? 1 : 4;
+  for (int gb = 0; gb < ng; gb += NT * kLLG) {
+    double q[kLLG][BPG];
 #pragma unroll
-    for (int i = 0; i < BPG; i++) q[i] = P.part_s[(g0 + g) * BPG + i];
-    P.part_sg[g0 + g] = pairwise_tree<BPG>(q);
+    for (int u = 0; u < kLLG; u++) {
+      const int g = gb + u * NT + threadIdx.x;
+#pragma unroll
+      for (int i = 0; i < BPG; i++) q[u][i] = g < ng ? P.part_s[(g0 + g) * BPG + i] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < kLLG; u++) {
+      const int g = gb + u * NT + threadIdx.x;
+      if (g < ng) P.part_sg[g0 + g] = pairwise_tree<BPG>(q[u]);
+    }
   }
-  __threadfence_block();
   __syncthreads();
   if (threadIdx.x >= 64) return;
   const double ll = frame_sum_w64(P.part_sg + g0, ng);
@@ -2470,7 +2559,7 @@ static void launch_loglik(const DevPtrs& P, const KOpts& O, hipStream_t st,
   if (gw == 1) {  // one lane per one-voxel supervoxel (voxel mode)
     k_loglik_voxel<KM><<<nb(P.Sext, 256), 256, 0, st>>>(P, O);
     if (ev_after) (void)hipEventRecord(ev_after, st);
-    k_ll_final<1><<<P.nframes, 256, 0, st>>>(P);
+    k_ll_final<1><<<P.nframes, ll_final_threads<1>(), 0, st>>>(P);
     return;
   }
   const long long B = (256 / gw) * spg;
@@ -2479,7 +2568,7 @@ static void launch_loglik(const DevPtrs& P, const KOpts& O, hipStream_t st,
   if (gw == GW_ && spg == SPG_) {                             \
     k_loglik<KM, GW_, SPG_><<<g, 256, 0, st>>>(P, O);         \
     if (ev_after) (void)hipEventRecord(ev_after, st);         \
-    k_ll_final<256 / ((256 / GW_) * SPG_)><<<P.nframes, 256, 0, st>>>(P); \
+    k_ll_final<256 / ((256 / GW_) * SPG_)><<<P.nframes, ll_final_threads<256 / ((256 / GW_) * SPG_)>(), 0, st>>>(P); \
     return;                                                   \
   }
   TGMM_LL(16, 1)
@@ -2578,8 +2667,7 @@ static hipError_t iteration_t(const DevPtrs& P, const KOpts& O, hipStream_t st, 
   if (tk == TK_ESTEP && e1) (void)hipEventRecord(e1, st);
   k_scan_frames<<<P.nframes, 1024, 0, st>>>(P, 1, 1, O.K);
   k_pair_scatter<<<nb(P.Sext, bs), bs, 0, st>>>(P, O);
-  k_seg_sort<<<nb(P.Gext, 16), 256, 0, st>>>(P);
-  k_seg_sort_long<<<64, 256, 0, st>>>(P);
+  k_seg_sort<<<nb(P.Gext, 16) + kSortLongBlocks, 256, 0, st>>>(P, (int)nb(P.Gext, 16));
   if (tk == TK_MSTEP && e0) (void)hipEventRecord(e0, st);
   launch_mstep(P, O, st);
   if (tk == TK_MSTEP && e1) (void)hipEventRecord(e1, st);
