@@ -281,11 +281,12 @@ int tgmm_batch_stats(tgmm_batch* b, tgmm_batch_stats_t* stats);
  * same results (the kernels fix their summation order independently of the
  * shape); only the speed changes. Defaults: {16, 4, 16, 256, 256, 8, 6, 2}. */
 typedef struct tgmm_tiles {
-  int ll_group;    /* lanes per supervoxel, log-likelihood: 16, 32 or 64;
+  int ll_group;    /* log-likelihood: 16, 32 or 64, for 2 * ll_group
+                      supervoxels per block (at most 64, and 32 for K > 8);
                       1 (with ll_unroll 1) = one lane per one-voxel
                       supervoxel, voxel mode only, its default there */
-  int ll_unroll;   /* supervoxels per lane group in turn, log-likelihood:
-                      1, 2 or 4 (8 with ll_group 16) */
+  int ll_unroll;   /* log-likelihood: voxels a thread loads at once, 1, 2
+                      or 4 (8 with ll_group 16) */
   int mstep_lanes; /* lanes per Gaussian, M-step statistics: 4, 8 or 16 */
   int sv_block;    /* threads per block, E-step and pair scatter: 64, 128, 256 */
   int knn_block;   /* threads per block, kNN search: 64, 128 or 256 */

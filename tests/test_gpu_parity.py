@@ -106,6 +106,36 @@ def test_ten_iterations_match_oracle(c1):
     assert np.array_equal(out["gm"]["supervoxelNum"], gm_o["supervoxelNum"])
 
 
+def test_fractional_coordinates_match_oracle(c1):
+    """Every other voxel half a voxel off the integer grid along x: offsets
+    from the supervoxels' anchors that are not integers, so the
+    log-likelihood reads the full coordinates instead of the packed offsets;
+    the EM still matches the oracle."""
+    f = copy.copy(c1)
+    f.query = c1.query.copy()
+    f.query[: c1.N] += (np.arange(c1.N) % 2).astype(np.float32) * np.float32(0.5)
+    out = _run_batch([f], _opts(iters=5))[0]
+    gm_o, rnk_o, ind_o, hist, it, ll = O.em_run(f, _oopts(iters=5))
+    e = record_errors(out["gm"], gm_o)
+    assert max(e.values()) < TOL, e
+    assert rel_err(out["ll"], ll) < TOL
+
+
+def test_far_voxel_offsets_match_oracle():
+    """A supervoxel whose voxels lie more than 127 voxels from its anchor
+    (its first voxel, moved 200 voxels along x): offsets beyond the packed
+    form's range, so the log-likelihood reads the full coordinates."""
+    cells = synth.sample_cells(41, 60, (512.0, 512.0, 40.0), 5.0, 3.0, 6.0)
+    f = synth.rasterize(42, cells, (512.0, 512.0, 40.0), 5.0, 12000, max_sv_size=40)
+    f.query = f.query.copy()
+    f.query[0] += np.float32(200.0)
+    out = _run_batch([f], _opts(iters=5))[0]
+    gm_o, rnk_o, ind_o, hist, it, ll = O.em_run(f, _oopts(iters=5))
+    e = record_errors(out["gm"], gm_o)
+    assert max(e.values()) < TOL, e
+    assert rel_err(out["ll"], ll) < TOL
+
+
 def soft_frame(max_sv=2, offset=1.0):
     """Tiny supervoxels and two Gaussians per cell: thousands of (s, k) pairs
     with responsibilities strictly between 1e-3 and 0.999."""

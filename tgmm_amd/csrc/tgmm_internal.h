@@ -52,7 +52,9 @@ struct FrameDev {
   int iter;           // reference numIterEM (starts at 1)
   int active;         // keeps iterating
   int status;
-  int pad;
+  int packed;         // every voxel's offset from its supervoxel's anchor is a
+                      // small integer: the log-likelihood reads vpk (set to 1
+                      // on the host, cleared by the moments kernel)
 };
 
 // Per-Gaussian parameter packs gathered by the supervoxel kernels: one
@@ -79,8 +81,9 @@ struct alignas(32) EPack {
 // results: the kernels fix their floating-point summation order
 // independently of the shape.
 struct Tiles {
-  int ll_gw;      // lanes per supervoxel, log-likelihood: 16, 32, 64
-  int ll_u;       // supervoxels per lane group, log-likelihood: 1, 2, 4 (8 with ll_gw 16)
+  int ll_gw;      // log-likelihood: 16, 32, 64 for 2*ll_gw supervoxels per block
+                  // (at most 64; 32 for K > 8); 1 = a lane per voxel (voxel mode)
+  int ll_u;       // log-likelihood: voxels a thread loads at once, 1, 2, 4 (8 with ll_gw 16)
   int ms_lanes;   // lanes per Gaussian, M-step statistics: 4, 8, 16
   int sv_block;   // threads per block, E-step and pair scatter: 64, 128, 256
   int knn_block;  // threads per block, kNN search: 64, 128, 256
@@ -107,6 +110,9 @@ struct KOpts {
 struct DevPtrs {
   // voxels
   float *vx, *vy, *vz, *vimg;
+  int2* vpk;        // Vcap: packed voxel: .x bytes 0..2 the offsets from the
+                    // supervoxel's anchor (int8), byte 3 the supervoxel's index
+                    // mod 256; .y the intensity bits
   // supervoxels
   int* sv_frame;
   long long* sv_beg;

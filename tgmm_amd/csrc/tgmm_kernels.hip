@@ -158,11 +158,26 @@ __global__ __launch_bounds__(256) void k_moments(DevPtrs P) {
   const float a0 = P.vx[beg], a1 = P.vy[beg], a2 = P.vz[beg];
   double sx = 0, sy = 0, sz = 0, sw = 0, swx = 0, swy = 0, swz = 0;
   float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+  bool fits = true;  // every offset from the anchor is an integer in [-127, 127]
   for (int i = lane; i < n; i += 16) {
     const float v[3] = {P.vx[beg + i], P.vy[beg + i], P.vz[beg + i]};
     for (int a = 0; a < 3; a++) {
       lo[a] = fminf(lo[a], v[a]);
       hi[a] = fmaxf(hi[a], v[a]);
+    }
+    {
+#pragma clang fp contract(off)
+      // the log-likelihood's f32 offsets, exactly
+      const float d0 = v[0] - a0, d1 = v[1] - a1, d2 = v[2] - a2;
+      const bool ok = fabsf(d0) <= 127.0f && fabsf(d1) <= 127.0f && fabsf(d2) <= 127.0f &&
+                      d0 == truncf(d0) && d1 == truncf(d1) && d2 == truncf(d2);
+      fits = fits && ok;
+      // byte 3: the supervoxel's position in its 256-aligned group
+      const unsigned pk = (ok ? ((unsigned)(int)d0 & 0xffu) | (((unsigned)(int)d1 & 0xffu) << 8) |
+                                    (((unsigned)(int)d2 & 0xffu) << 16)
+                              : 0u) |
+                          ((unsigned)(s & 255) << 24);
+      P.vpk[beg + i] = make_int2((int)pk, __float_as_int(P.vimg[beg + i]));
     }
     const double x = (double)v[0] - a0, y = (double)v[1] - a1, z = (double)v[2] - a2,
                  w = (double)P.vimg[beg + i];
@@ -223,6 +238,7 @@ __global__ __launch_bounds__(256) void k_moments(DevPtrs P) {
     P.anchor[1 * S + s] = a1;
     P.anchor[2 * S + s] = a2;
   }
+  if (!fits) atomicAnd(&P.fr[f].packed, 0);
   // the ball around the voxels' box (NaN coordinates give a NaN ball, which
   // the log-likelihood's candidate bounds ignore)
   const float a[3] = {a0, a1, a2};
@@ -262,6 +278,10 @@ __global__ __launch_bounds__(256) void k_moments_voxel(DevPtrs P) {
     P.wmom[i * S + s] = 0.0;
   }
   P.sv_ball[s] = make_float4(0.f, 0.f, 0.f, 0.f);
+  // the voxel is its own anchor: offsets 0, unless a coordinate is not finite
+  P.vpk[v] = make_int2((int)((unsigned)(s & 255) << 24), __float_as_int(P.vimg[v]));
+  if (!(x[0] - x[0] == 0.0f && x[1] - x[1] == 0.0f && x[2] - x[2] == 0.0f))
+    atomicAnd(&P.fr[f].packed, 0);
 }
 
 // One block per frame: bounding box of the scaled kNN centroids and the grid
@@ -863,6 +883,9 @@ __global__ __launch_bounds__(256) void k_knn(DevPtrs P, KOpts O) {
     // references rather than once per row (the K best do not depend on the
     // order references are considered in)
     const int tot = cum[9];
+    int off[9];  // reference position of flat index p in row t: off[t] + p
+#pragma unroll
+    for (int t = 0; t < 9; t++) off[t] = pb[t] - cum[t];
     for (int p0 = sub; p0 < tot; p0 += KC * QL) {
       float4 v[KC];
 #pragma unroll
@@ -870,11 +893,10 @@ __global__ __launch_bounds__(256) void k_knn(DevPtrs P, KOpts O) {
         const int p = p0 + w * QL;
         v[w] = make_float4(0.f, 0.f, 0.f, 0.f);
         if (p < tot) {
-          int a = pb[0] + p;  // row of p: the last t with cum[t] <= p
+          int o = off[0];  // row of p: the last t with cum[t] <= p
 #pragma unroll
-          for (int t = 1; t < 9; t++)
-            if (p >= cum[t]) a = pb[t] + (p - cum[t]);
-          v[w] = P.cg4[a];
+          for (int t = 1; t < 9; t++) o = p >= cum[t] ? off[t] : o;
+          v[w] = P.cg4[o + p];
         }
       }
 #pragma unroll
@@ -971,11 +993,10 @@ __global__ __launch_bounds__(64 * NW) void k_knn_cell(DevPtrs P, KOpts O) {
       __syncthreads();  // the previous chunk has been read
       for (int x = threadIdx.x; x < n; x += 64 * NW) {
         const int y = cb + x;
-        int addr = pb[0] + y;
+        int o = pb[0];  // row of y: the last t with cum[t] <= y
 #pragma unroll
-        for (int t = 1; t < 9; t++)
-          if (y >= cum[t]) addr = pb[t] + (y - cum[t]);
-        cand[x] = P.cg4[addr];
+        for (int t = 1; t < 9; t++) o = y >= cum[t] ? pb[t] - cum[t] : o;
+        cand[x] = P.cg4[o + y];
       }
       __syncthreads();
       if (act) {
@@ -2076,23 +2097,26 @@ __global__ __launch_bounds__(64) void k_alpha_sum(DevPtrs P, int need_active) {
 }
 
 
-// Log-likelihood over the voxel stream (GMEM.cu:1913-1978). A block has 256
-// threads in groups of GW lanes; each group takes SPG supervoxels in turn,
-// GW*U voxels per pass (U = 2 for 16-lane groups, else 64/GW, all loads of
-// a pass issued together), with the next supervoxel's header loaded while the current one
-// is evaluated; a block covers B = (256/GW)*SPG consecutive supervoxels of
-// one frame. Lane j < K of a group reads candidate j's parameter pack; the
-// constants are broadcast to the group (into scalars for a whole-wave group).
-// Per voxel and candidate, the scaled quadratic form is |y|^2 with
-// y = U (x - a_s) + e: U the candidate's Cholesky factor, a_s an anchor voxel
-// of the supervoxel (x - a_s is exact in f32) and e = U (a_s - m_k) formed in
-// FP64 and rounded once, so the f32 form keeps full relative precision far
-// from the origin (9 FMA-class operations per candidate). Terms are evaluated
-// in base 2 (hardware exp2/log2), log-sum-exp with the largest term factored
-// out; a voxel's contribution img * term is rounded to FP64 and accumulated
-// in FP64. A candidate whose term underflows in the reference's double
-// evaluation (its exponential or the whole term below 2^-1075) adds nothing,
-// and a voxel with no term left contributes nothing (GMEM.cu:1958-1959).
+// Log-likelihood over the voxel stream (GMEM.cu:1913-1978). A block of 256
+// threads takes B consecutive supervoxels of one frame (B divides 256):
+//  1. Candidate constants, LPS lanes per supervoxel (lane j < K: candidate
+//     j), kept in LDS: the Cholesky factor U and e = U (a_s - m_k), with a_s
+//     an anchor voxel of the supervoxel, formed in FP64 and rounded once, and
+//     the log2 weight g. Per voxel and candidate the scaled quadratic form is
+//     |y|^2 with y = U (x - a_s) + e (x - a_s is exact in f32), so the f32
+//     form keeps full relative precision far from the origin (9 FMA-class
+//     operations per candidate).
+//  2. The supervoxels' voxels, which are contiguous in the stream, in chunks
+//     of 256*UV: thread t takes voxels t, t + 256, ... (coalesced loads, UV
+//     a thread issued together), finds its supervoxel by a binary search of
+//     the block's voxel starts, and forms the voxel's value into LDS.
+//  3. The chunk's values are added to the supervoxels' class sums.
+// Terms are evaluated in base 2 (hardware exp2/log2), log-sum-exp with the
+// largest term factored out; a voxel's value img * term is rounded to FP64
+// and accumulated in FP64. A candidate whose term underflows in the
+// reference's double evaluation (its exponential or the whole term below
+// 2^-1075) adds nothing, and a voxel with no term left contributes nothing
+// (GMEM.cu:1958-1959).
 //
 // Candidates that are negligible at every voxel of the supervoxel are left
 // out. With c the centre and r the radius of the ball around the
@@ -2102,48 +2126,20 @@ __global__ __launch_bounds__(64) void k_alpha_sum(DevPtrs P, int need_active) {
 // more than the f32 evaluation's rounding. A candidate with g - hi >= -1000
 // and hi <= 1000 (never cut as underflowing) bounds every voxel's largest
 // term from below by L = max(g - hi); a candidate with g - lo < L - 40 then
-// adds less than 2^-40 of the largest term at every voxel, and is dropped.
+// adds less than 2^-39 of the largest term at every voxel (the bounds are
+// formed in f32), and is dropped.
 // The others keep their slot order. The decision depends only on the
 // supervoxel and its candidates, not on the launch shape. In voxel mode
 // every candidate is kept, as k_loglik_voxel keeps them.
 //
-// Summation order, identical for every (GW, SPG): voxel i of a supervoxel is
-// added, in increasing i, to class sum i mod 16; the 16 class sums are added
-// as ((c + c^8) + ...^4) ...^1; the sums of each 256 consecutive supervoxels
-// by one pairwise tree (a block forms the subtree over its B leaves,
-// k_ll_final the levels above); the frame's 256-supervoxel partials by
-// frame_sum_w64. No block waits for another: the frame total is formed by
+// Summation order, identical for every (B, UV): voxel i of a supervoxel is
+// added, in increasing i, to class sum i mod 16 (starting from 0); the 16
+// class sums are added as ((c + c^8) + ...^4) ...^1 (the butterfly of a
+// 16-lane group, read at lane 0); the sums of each 256 consecutive
+// supervoxels by one pairwise tree (a block forms the subtree over its B
+// leaves, k_ll_final the levels above); the frame's 256-supervoxel partials
+// by frame_sum_w64. No block waits for another: the frame total is formed by
 // k_ll_final after this kernel.
-struct SvHead {
-  long long beg;
-  int n;
-  float a0, a1, a2;
-  float4 ball;  // centre relative to the anchor, radius
-  int kk;       // lane j < K: candidate j
-};
-
-__device__ inline SvHead ll_head(const DevPtrs& P, long long s, int f, int lane, int K) {
-  const long long S = P.Scap;
-  SvHead h;
-  const int sf = P.sv_frame[s];
-  h.beg = P.sv_beg[s];
-  const int len = P.sv_len[s];
-  h.a0 = P.anchor[s];
-  h.a1 = P.anchor[S + s];
-  h.a2 = P.anchor[2 * S + s];
-  h.ball = P.sv_ball[s];
-  h.kk = (lane < K) ? P.ind[(long long)lane * S + s] : -1;
-  h.n = (sf == f) ? len : 0;
-  return h;
-}
-
-// value of lane j of the group (a whole-wave group reads it into a scalar)
-template <int GW>
-__device__ inline float group_bcast(float x, int j) {
-  if (GW == 64) return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), j));
-  return __shfl(x, j, GW);
-}
-
 template <int N>
 __device__ inline double pairwise_tree(double* v) {  // in place, v[0] = result
 #pragma unroll
@@ -2153,220 +2149,260 @@ __device__ inline double pairwise_tree(double* v) {  // in place, v[0] = result
   return v[0];
 }
 
-template <int KM, int GW, int SPG>
+// The butterfly sum of 16 class sums as lane 0 of a 16-lane group forms it
+// with tot += shfl_xor(tot, m) for m = 8, 4, 2, 1.
+__device__ inline double class_butterfly(const double* c) {
+  double a[8], b[4], d[2];
+#pragma unroll
+  for (int l = 0; l < 8; l++) a[l] = c[l] + c[l + 8];
+#pragma unroll
+  for (int l = 0; l < 4; l++) b[l] = a[l] + a[l + 4];
+#pragma unroll
+  for (int l = 0; l < 2; l++) d[l] = b[l] + b[l + 2];
+  return d[0] + d[1];
+}
+
+template <int KM, int B, int UV>
 __global__ __launch_bounds__(256) void k_loglik(DevPtrs P, KOpts O) {
-  constexpr int NGB = 256 / GW;  // groups per block
-  constexpr int B = NGB * SPG;   // supervoxels per block
-  constexpr int U = GW == 16 ? 2 : 64 / GW;  // voxels per lane per pass
-  static_assert(B <= 256 && 256 % B == 0, "block covers a divisor of 256 supervoxels");
-  static_assert(GW >= KM, "a lane per candidate");
-  constexpr bool kLds = GW < 64;  // candidate constants staged in LDS
-  __shared__ double svsum[B];
-  __shared__ float4 cst[kLds ? NGB * KM * 3 : 1];
+  constexpr int LPS = KM <= 8 ? 8 : 16;  // lanes per supervoxel, constants
+  constexpr int SPP = 256 / LPS;         // supervoxels per constants pass
+  constexpr int CH = 256 * UV;           // voxels per chunk
+  static_assert(B % SPP == 0 && 256 % B == 0, "block shape");
+  __shared__ float4 cst[B * KM * 3];
+  __shared__ float4 anc[B];   // anchor, and the number of candidates kept (int bits)
+  __shared__ int vst[B + 1];  // first voxel of each supervoxel, relative to vbase
+  __shared__ double vals[CH];
+  __shared__ double cls[B * 16];
+  __shared__ int vend_s;
   const long long sb = (long long)blockIdx.x * B;
-  const int lane = threadIdx.x % GW, grp = threadIdx.x / GW;
-  const int K = O.K;
   const int f = P.sblk_frame[sb >> 8];
   if (f < 0) return;
   const FrameDev* F = P.fr + f;
   if (!F->active) return;
+  const long long S = P.Scap;
+  const int K = O.K;
   const double lsa2 = F->log_sum_alpha * 1.4426950408889634074;
-  SvHead h = ll_head(P, sb + grp, f, lane, K);
-  for (int it = 0; it < SPG; it++) {
-    const int sv = grp + NGB * it;
-    const SvHead c = h;
-    if (it + 1 < SPG) h = ll_head(P, sb + sv + NGB, f, lane, K);
-    const int n = c.n;
-    // the first pass's voxels, then the candidates' constants
-    float4 vv[U];
+  // the block's first supervoxel is real unless the whole block is padding
+  const long long vbase = P.sv_frame[sb] == f ? P.sv_beg[sb] : 0;
+  // the voxels of a chunk, raw (packed: .x .y of vpk; else x, y, z, img
+  // bits), loaded within the frame's voxel range; the first chunk's loads
+  // are issued before the constants are formed, each next chunk's before
+  // the current one is evaluated
+  const bool packed = F->packed != 0;
+  const long long vlim = F->v0 + F->nv;
+  auto load_chunk = [&](int c0, int4 (&raw)[UV]) {
 #pragma unroll
-    for (int u = 0; u < U; u++) {
-      const int i = lane + GW * u;
-      vv[u] = make_float4(0.f, 0.f, 0.f, 0.f);
-      if (i < n) {
-        const long long g = c.beg + i;
-        vv[u] = make_float4(P.vx[g], P.vy[g], P.vz[g], P.vimg[g]);
+    for (int u = 0; u < UV; u++) {
+      const long long g = vbase + c0 + u * 256 + threadIdx.x;
+      raw[u] = make_int4(0, 0, 0, 0);
+      if (g < vlim) {
+        if (packed) {
+          const int2 pk = P.vpk[g];
+          raw[u].x = pk.x;
+          raw[u].y = pk.y;
+        } else {
+          raw[u] = make_int4(__float_as_int(P.vx[g]), __float_as_int(P.vy[g]),
+                             __float_as_int(P.vz[g]), __float_as_int(P.vimg[g]));
+        }
       }
     }
-    // candidate constants: the Cholesky factor U and e = U (a_s - m_k), FP64
-    // from the FP64 mean then rounded, so y = U (x - a_s) + e = U (x - m_k)
+  };
+  int4 cur[UV];
+  load_chunk(0, cur);
+  if (threadIdx.x == 0) vend_s = 0;
+  for (int x = threadIdx.x; x < B * 16; x += 256) cls[x] = 0.0;
+  __syncthreads();
+  // 1. constants
+  for (int p = 0; p < B; p += SPP) {
+    const int i = p + threadIdx.x / LPS;
+    const int lane = threadIdx.x % LPS;
+    const long long s = sb + i;
+    const bool real = P.sv_frame[s] == f;
+    const int n = real ? P.sv_len[s] : 0;
+    const float a0 = P.anchor[s], a1 = P.anchor[S + s], a2 = P.anchor[2 * S + s];
+    const float4 ball = P.sv_ball[s];
+    const int kk = (n > 0 && lane < K) ? P.ind[(long long)lane * S + s] : -1;
     float pe0 = 0.0f, pe1 = 0.0f, pe2 = 0.0f, pu[6] = {0, 0, 0, 0, 0, 0}, pg = -INFINITY;
     double tlo = -INFINITY, thi = INFINITY;  // bounds of the candidate's terms
-    if (n > 0 && c.kk >= 0) {
-      const LLPack lp = P.llpack[c.kk];
-      const double d0 = (double)c.a0 - lp.m[0], d1 = (double)c.a1 - lp.m[1],
-                   d2 = (double)c.a2 - lp.m[2];
+    if (kk >= 0) {
+      const LLPack lp = P.llpack[kk];
+      const double d0 = (double)a0 - lp.m[0], d1 = (double)a1 - lp.m[1], d2 = (double)a2 - lp.m[2];
 #pragma unroll
-      for (int i = 0; i < 6; i++) pu[i] = lp.U[i];
+      for (int q = 0; q < 6; q++) pu[q] = lp.U[q];
       pe0 = (float)((double)pu[0] * d0 + (double)pu[1] * d1 + (double)pu[2] * d2);
       pe1 = (float)((double)pu[3] * d1 + (double)pu[4] * d2);
       pe2 = (float)((double)pu[5] * d2);
       pg = (float)(lp.lg2 - lsa2);
       {
+        // in f32: the rounding of y = U (c - m) is below 1e-6 nf |c - m|,
+        // added to the ball's reach, and a relative error of sq below 1e-6
+        // is far inside err's margin
 #pragma clang fp contract(off)
-        const double b0 = d0 + (double)c.ball.x, b1 = d1 + (double)c.ball.y,
-                     b2 = d2 + (double)c.ball.z;  // centre - mean
-        const double y0 = (double)pu[0] * b0 + (double)pu[1] * b1 + (double)pu[2] * b2;
-        const double y1 = (double)pu[3] * b1 + (double)pu[4] * b2;
-        const double y2 = (double)pu[5] * b2;
-        const double sq = sqrt(y0 * y0 + y1 * y1 + y2 * y2);
-        const double rr = (double)lp.nf * (double)c.ball.w;
-        const double err = 1e-5 * ((sq + 3.0 * rr) * (sq + 3.0 * rr) + fabs((double)pg) + 1.0);
-        const double lo = fmax(sq - rr, 0.0);
-        const double hi = (sq + rr) * (sq + rr) + err;
+        const float b0 = (float)(d0 + (double)ball.x), b1 = (float)(d1 + (double)ball.y),
+                    b2 = (float)(d2 + (double)ball.z);  // centre - mean
+        const float y0 = pu[0] * b0 + pu[1] * b1 + pu[2] * b2;
+        const float y1 = pu[3] * b1 + pu[4] * b2;
+        const float y2 = pu[5] * b2;
+        const float sq = sqrtf(y0 * y0 + y1 * y1 + y2 * y2);
+        const float bn = sqrtf(b0 * b0 + b1 * b1 + b2 * b2);
+        const float rr = lp.nf * (ball.w + 1e-6f * bn);
+        const float w = sq + 3.0f * rr;
+        const float err = 1e-5f * (w * w + fabsf(pg) + 1.0f);
+        const float lo = fmaxf(sq - rr, 0.0f);
+        const float hi = (sq + rr) * (sq + rr) + err;
         if (pg > -INFINITY) {
-          thi = (double)pg - (lo * lo - err);
-          if (hi <= 1000.0) tlo = (double)pg - hi;
+          thi = (double)(pg - (lo * lo - err));
+          if (hi <= 1000.0f) tlo = (double)(pg - hi);
         }
       }
     }
-    // the candidates kept (lanes j < K of the group), compacted in slot order
-    bool live;
-    {
-      double L = tlo;
+    // the candidates kept, compacted in slot order
+    double L = tlo;
 #pragma unroll
-      for (int m = 8; m >= 1; m >>= 1) L = fmax(L, __shfl_xor(L, m, 16));
-      live = lane < K && n > 0 && c.kk >= 0 && pg > -INFINITY &&
-             !(!O.voxel && L >= -1000.0 && thi < L - 40.0);
-    }
-    unsigned long long gm = __ballot(live);
-    if constexpr (GW < 64) gm = (gm >> ((threadIdx.x & 63) & ~(GW - 1))) & ((1ull << GW) - 1ull);
-    const int nlive = __popcll(gm);
+    for (int m = LPS / 2; m >= 1; m >>= 1) L = fmax(L, __shfl_xor(L, m, LPS));
+    const bool live =
+        kk >= 0 && pg > -INFINITY && !(!O.voxel && L >= -1000.0 && thi < L - 40.0);
+    const unsigned long long gm =
+        (__ballot(live) >> ((threadIdx.x & 63) & ~(LPS - 1))) & ((1ull << LPS) - 1ull);
     const int slot = __popcll(gm & ((1ull << lane) - 1ull));
-    // the group's candidate constants: a whole-wave group reads them into
-    // scalars; smaller groups stage them in LDS (12 floats per candidate,
-    // read back as broadcasts), so that only one candidate's constants are
-    // live in registers at a time
-    float ce0[kLds ? 1 : KM], ce1[kLds ? 1 : KM], ce2[kLds ? 1 : KM], cu[kLds ? 1 : KM][6],
-        cg[kLds ? 1 : KM];
-    if constexpr (kLds) {
-      __syncthreads();  // the previous supervoxel's constants are no longer read
-      if (live) {
-        float4* d = cst + (grp * KM + slot) * 3;
-        d[0] = make_float4(pu[0], pu[1], pu[2], pe0);
-        d[1] = make_float4(pu[3], pu[4], pe1, pu[5]);
-        d[2] = make_float4(pe2, pg, 0.0f, 0.0f);
-      }
-      __syncthreads();
-    } else {
-      unsigned long long mm = gm;
+    if (live) {
+      float4* d = cst + (i * KM + slot) * 3;
+      d[0] = make_float4(pu[0], pu[1], pu[2], pe0);
+      d[1] = make_float4(pu[3], pu[4], pe1, pu[5]);
+      d[2] = make_float4(pe2, pg, 0.0f, 0.0f);
+    }
+    if (lane == 0) {
+      anc[i] = make_float4(a0, a1, a2, __int_as_float(__popcll(gm)));
+      vst[i] = real ? (int)(P.sv_beg[s] - vbase) : -1;
+      if (real) atomicMax(&vend_s, (int)(P.sv_beg[s] + n - vbase));
+    }
+  }
+  __syncthreads();
+  const int vend = vend_s;
+  // padding supervoxels (after the frame's last) start where the real ones end
+  for (int i = threadIdx.x; i <= B; i += 256)
+    if (i == B || vst[i] < 0) vst[i] = vend;
+  __syncthreads();
+  // 2. and 3. the voxels, chunk by chunk
+  for (int c0 = 0; c0 < vend; c0 += CH) {
+    int4 nxt[UV];
 #pragma unroll
-      for (int j = 0; j < KM; j++) {
-        const int src = mm ? __builtin_ctzll(mm) : 0;
-        mm &= mm - 1ull;
-        ce0[j] = group_bcast<GW>(pe0, src);
-        ce1[j] = group_bcast<GW>(pe1, src);
-        ce2[j] = group_bcast<GW>(pe2, src);
+    for (int u = 0; u < UV; u++) nxt[u] = make_int4(0, 0, 0, 0);
+    if (c0 + CH < vend) load_chunk(c0 + CH, nxt);
+    // coordinates (or offsets from the anchor, packed) and intensity
+    float4 vv[UV];
+    int svi[UV];  // the voxels' supervoxels in the block (packed form)
 #pragma unroll
-        for (int i = 0; i < 6; i++) cu[j][i] = group_bcast<GW>(pu[i], src);
-        cg[j] = group_bcast<GW>(pg, src);
+    for (int u = 0; u < UV; u++) {
+      if (packed) {
+        vv[u] = make_float4((float)(signed char)(cur[u].x & 0xff),
+                            (float)(signed char)((cur[u].x >> 8) & 0xff),
+                            (float)(signed char)((cur[u].x >> 16) & 0xff), __int_as_float(cur[u].y));
+        svi[u] = (int)(((unsigned)cur[u].x >> 24) - (unsigned)(sb & 255));
+      } else {
+        vv[u] = make_float4(__int_as_float(cur[u].x), __int_as_float(cur[u].y),
+                            __int_as_float(cur[u].z), __int_as_float(cur[u].w));
+        svi[u] = 0;
       }
     }
-    // candidate j's constants: U row-major (u00 u01 u02 u11 u12 u22), e, log2 weight
-    auto cand = [&](int j, float (&u)[6], float& e0, float& e1, float& e2, float& g) {
-      if constexpr (kLds) {
-        const float4* c4 = cst + (grp * KM + j) * 3;
-        const float4 A = c4[0], B = c4[1], C = c4[2];
-        u[0] = A.x; u[1] = A.y; u[2] = A.z; e0 = A.w;
-        u[3] = B.x; u[4] = B.y; e1 = B.z; u[5] = B.w;
-        e2 = C.x; g = C.y;
+#pragma unroll
+    for (int u = 0; u < UV; u++) {
+      const int v = c0 + u * 256 + threadIdx.x;
+      if (v >= vend) continue;
+      int i = 0;  // the last supervoxel starting at or before v
+      if (packed) {
+        i = svi[u];
       } else {
 #pragma unroll
-        for (int i = 0; i < 6; i++) u[i] = cu[j][i];
-        e0 = ce0[j]; e1 = ce1[j]; e2 = ce2[j]; g = cg[j];
+        for (int st = B / 2; st >= 1; st >>= 1)
+          if (vst[i + st] <= v) i += st;
       }
-    };
-    double cls = 0.0;  // class sum (lanes 0..15 of the group)
-    for (int base = 0; base < n; base += GW * U) {
-      if (base > 0) {
-#pragma unroll
-        for (int u = 0; u < U; u++) {
-          const int i = base + lane + GW * u;
-          vv[u] = make_float4(0.f, 0.f, 0.f, 0.f);
-          if (i < n) {
-            const long long g = c.beg + i;
-            vv[u] = make_float4(P.vx[g], P.vy[g], P.vz[g], P.vimg[g]);
-          }
-        }
-      }
-      // per voxel and candidate t = g - |U (x - a) + e|^2, candidates outer
-      float tj[U][KM];
+      const float4 an = anc[i];
+      const int nlive = __float_as_int(an.w);
+      float tj[KM];
+      int nm = 0;  // candidate slots any lane of the wave uses
       {
 #pragma clang fp contract(off)
-        float xd[U][3];
-#pragma unroll
-        for (int u = 0; u < U; u++) {
-          xd[u][0] = vv[u].x - c.a0;
-          xd[u][1] = vv[u].y - c.a1;
-          xd[u][2] = vv[u].z - c.a2;
-        }
+        const float x0 = packed ? vv[u].x : vv[u].x - an.x;
+        const float x1 = packed ? vv[u].y : vv[u].y - an.y;
+        const float x2 = packed ? vv[u].z : vv[u].z - an.z;
 #pragma unroll
         for (int j = 0; j < KM; j++) {
-          if (j >= nlive) {
-#pragma unroll
-            for (int u = 0; u < U; u++) tj[u][j] = -INFINITY;
-            continue;
-          }
-          float cu6[6], e0, e1, e2, g;
-          cand(j, cu6, e0, e1, e2, g);
-#pragma unroll
-          for (int u = 0; u < U; u++) {
-            const float x0 = xd[u][0], x1 = xd[u][1], x2 = xd[u][2];
-            const float y0 = __builtin_fmaf(cu6[0], x0,
-                                            __builtin_fmaf(cu6[1], x1,
-                                                           __builtin_fmaf(cu6[2], x2, e0)));
-            const float y1 = __builtin_fmaf(cu6[3], x1, __builtin_fmaf(cu6[4], x2, e1));
-            const float y2 = __builtin_fmaf(cu6[5], x2, e2);
+          tj[j] = -INFINITY;
+          if (!__any(j < nlive)) break;
+          nm = j + 1;
+          if (j < nlive) {
+            const float4* c4 = cst + (i * KM + j) * 3;
+            const float4 A = c4[0], Bq = c4[1], C = c4[2];
+            const float y0 = __builtin_fmaf(A.x, x0, __builtin_fmaf(A.y, x1,
+                                                                    __builtin_fmaf(A.z, x2, A.w)));
+            const float y1 = __builtin_fmaf(Bq.x, x1, __builtin_fmaf(Bq.y, x2, Bq.z));
+            const float y2 = __builtin_fmaf(Bq.w, x2, C.x);
             const float q = __builtin_fmaf(y0, y0, __builtin_fmaf(y1, y1, y2 * y2));
             // the reference's double term underflows to 0 when its exponential
             // (2^-q) or the whole term (2^t) is below 2^-1075
-            const float t = g - q;
-            tj[u][j] = (q <= 1075.0f && t >= -1075.0f) ? t : -INFINITY;
+            const float t = C.y - q;
+            tj[j] = (q <= 1075.0f && t >= -1075.0f) ? t : -INFINITY;
           }
         }
       }
-      // img * log2(sum_j 2^t_j) of each voxel, rounded to FP64
-      double v[U];
-#pragma unroll
-      for (int u = 0; u < U; u++) {
+      double val = 0.0;
+      {
 #pragma clang fp contract(off)
         float mx = -INFINITY;
 #pragma unroll
-        for (int j = 0; j < KM; j++) mx = fmaxf(mx, tj[u][j]);
-        double t = 0.0;
-        if (mx > -INFINITY) {
+        for (int j = 0; j < KM; j++)
+          if (j < nm) mx = fmaxf(mx, tj[j]);
+        if (nlive == 1) {
+          // a single term t: the value is img * t (k_loglik_voxel does the same)
+          if (mx > -INFINITY) val = (double)vv[u].w * (double)mx;
+        } else if (mx > -INFINITY) {
           float sum = 0.0f;
 #pragma unroll
           for (int j = 0; j < KM; j++)
-            if (j < nlive) sum += __builtin_amdgcn_exp2f(tj[u][j] - mx);
-          t = (double)vv[u].w * (double)(mx + __builtin_amdgcn_logf(sum));
+            if (j < nm && j < nlive) sum += __builtin_amdgcn_exp2f(tj[j] - mx);
+          val = (double)vv[u].w * (double)(mx + __builtin_amdgcn_logf(sum));
         }
-        v[u] = (base + lane + GW * u < n) ? t : 0.0;
       }
-      // the pass's voxels of class (lane mod 16), in increasing index
+      vals[u * 256 + threadIdx.x] = val;
+    }
+    __syncthreads();
+    // class (i, c): the chunk's voxels of supervoxel i at positions = c mod 16,
+    // for the supervoxels the chunk overlaps
+    int ia = 0, ib = 0;
+    {
+      const int vl = min(c0 + CH, vend) - 1;
 #pragma unroll
-      for (int u = 0; u < U; u++) {
-#pragma clang fp contract(off)
-        cls += v[u];
-        if (GW >= 32) cls += __shfl_down(v[u], 16, GW);
-        if (GW >= 64) {
-          cls += __shfl_down(v[u], 32, GW);
-          cls += __shfl_down(v[u], 48, GW);
-        }
+      for (int st = B / 2; st >= 1; st >>= 1) {
+        if (vst[ia + st] <= c0) ia += st;
+        if (vst[ib + st] <= vl) ib += st;
       }
     }
-    double tot = cls;
+    for (int task = ia * 16 + threadIdx.x; task < (ib + 1) * 16; task += 256) {
+      const int i = task >> 4, c = task & 15;
+      const int s0 = vst[i], lo = max(s0, c0), hi = min(vst[i + 1], c0 + CH);
+      if (lo >= hi) continue;
+      int v = s0 + c;
+      if (v < lo) v += ((lo - v + 15) >> 4) << 4;
+      double a = cls[task];
+      for (; v < hi; v += 16) a += vals[v - c0];
+      cls[task] = a;
+    }
+    __syncthreads();
 #pragma unroll
-    for (int m = 8; m >= 1; m >>= 1) tot += __shfl_xor(tot, m, 16);
-    if (lane == 0) svsum[sv] = tot * 0.69314718055994530942;
+    for (int u = 0; u < UV; u++) cur[u] = nxt[u];
   }
+  // the supervoxel sums, then pairwise_tree<B>'s levels in LDS
+  if (threadIdx.x < B) vals[threadIdx.x] = class_butterfly(cls + 16 * threadIdx.x) * 0.69314718055994530942;
   __syncthreads();
-  if (threadIdx.x == 0) {
-    double tr[B];
-#pragma unroll
-    for (int i = 0; i < B; i++) tr[i] = svsum[i];
-    P.part_s[blockIdx.x] = pairwise_tree<B>(tr);
+  for (int w = B / 2; w >= 1; w >>= 1) {
+    double a = 0.0;
+    if ((int)threadIdx.x < w) a = vals[2 * threadIdx.x] + vals[2 * threadIdx.x + 1];
+    __syncthreads();
+    if ((int)threadIdx.x < w) vals[threadIdx.x] = a;
+    __syncthreads();
   }
+  if (threadIdx.x == 0) P.part_s[blockIdx.x] = vals[0];
 }
 
 // Log-likelihood in voxel mode, where every supervoxel is one voxel (or
@@ -2394,6 +2430,7 @@ __global__ __launch_bounds__(256) void k_loglik_voxel(DevPtrs P, KOpts O) {
     const float4 vv = make_float4(P.vx[g], P.vy[g], P.vz[g], P.vimg[g]);
     const float ca0 = P.anchor[s], ca1 = P.anchor[S + s], ca2 = P.anchor[2 * S + s];
     float tj[KM];
+    int nterm = 0;  // candidates with a density (kept by k_loglik's rule in voxel mode)
 #pragma unroll
     for (int j = 0; j < KM; j++) {
       const int kk = (j < K) ? P.ind[(long long)j * S + s] : -1;
@@ -2409,6 +2446,7 @@ __global__ __launch_bounds__(256) void k_loglik_voxel(DevPtrs P, KOpts O) {
         pe2 = (float)((double)pu[5] * d2);
         pg = (float)(lp.lg2 - lsa2);
       }
+      nterm += (kk >= 0 && pg > -INFINITY) ? 1 : 0;
       {
 #pragma clang fp contract(off)
         const float x0 = vv.x - ca0, x1 = vv.y - ca1, x2 = vv.z - ca2;
@@ -2426,7 +2464,9 @@ __global__ __launch_bounds__(256) void k_loglik_voxel(DevPtrs P, KOpts O) {
       float mx = -INFINITY;
 #pragma unroll
       for (int j = 0; j < KM; j++) mx = fmaxf(mx, tj[j]);
-      if (mx > -INFINITY) {
+      if (nterm == 1) {
+        if (mx > -INFINITY) t = (double)vv.w * (double)mx;
+      } else if (mx > -INFINITY) {
         float sum = 0.0f;
 #pragma unroll
         for (int j = 0; j < KM; j++) sum += __builtin_amdgcn_exp2f(tj[j] - mx);
@@ -2562,25 +2602,27 @@ static void launch_loglik(const DevPtrs& P, const KOpts& O, hipStream_t st,
     k_ll_final<1><<<P.nframes, ll_final_threads<1>(), 0, st>>>(P);
     return;
   }
-  const long long B = (256 / gw) * spg;
+  // B supervoxels per block: 2 * ll_group, at most 64 (32 for K > 8, whose
+  // constants take twice the LDS); ll_unroll voxels per thread in flight
+  const int B = std::min(2 * gw, KM > 8 ? 32 : 64);
   const unsigned g = nb(P.Sext, B);
-#define TGMM_LL(GW_, SPG_)                                    \
-  if (gw == GW_ && spg == SPG_) {                             \
-    k_loglik<KM, GW_, SPG_><<<g, 256, 0, st>>>(P, O);         \
-    if (ev_after) (void)hipEventRecord(ev_after, st);         \
-    k_ll_final<256 / ((256 / GW_) * SPG_)><<<P.nframes, ll_final_threads<256 / ((256 / GW_) * SPG_)>(), 0, st>>>(P); \
-    return;                                                   \
+#define TGMM_LL(B_, UV_)                                                        \
+  if (B == B_ && spg == UV_) {                                                  \
+    k_loglik<KM, B_, UV_><<<g, 256, 0, st>>>(P, O);                             \
+    if (ev_after) (void)hipEventRecord(ev_after, st);                           \
+    k_ll_final<256 / B_><<<P.nframes, ll_final_threads<256 / B_>(), 0, st>>>(P); \
+    return;                                                                     \
   }
-  TGMM_LL(16, 1)
-  TGMM_LL(16, 2)
-  TGMM_LL(16, 4)
-  TGMM_LL(16, 8)
   TGMM_LL(32, 1)
   TGMM_LL(32, 2)
   TGMM_LL(32, 4)
-  TGMM_LL(64, 1)
-  TGMM_LL(64, 2)
-  TGMM_LL(64, 4)
+  TGMM_LL(32, 8)
+  if constexpr (KM <= 8) {
+    TGMM_LL(64, 1)
+    TGMM_LL(64, 2)
+    TGMM_LL(64, 4)
+    TGMM_LL(64, 8)
+  }
 #undef TGMM_LL
 }
 

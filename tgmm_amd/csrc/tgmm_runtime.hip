@@ -85,6 +85,7 @@ int ws_alloc(Workspace* w, const WsShape& sh, int K) {
   AL(P.vy, V);
   AL(P.vz, V);
   AL(P.vimg, V);
+  AL(P.vpk, V);
   AL(P.sv_frame, S);
   AL(P.sv_beg, S);
   AL(P.sv_len, S);
@@ -141,7 +142,7 @@ void ws_free(Workspace* w) {
 
 long long ws_bytes_estimate(const WsShape& sh, int K) {
   const long long S = sh.Scap, G = sh.Gcap, C = sh.Ccap, V = sh.Vcap;
-  return 16 * V + S * (16 + 8 + 4 + 12 + 80 + 80 + 12 + 16 + 16LL * K + 1 + 4 + 8) +
+  return 24 * V + S * (16 + 8 + 4 + 12 + 80 + 80 + 12 + 16 + 16LL * K + 1 + 4 + 8) +
          G * (4 + 8 * GS_NFIELDS * 2 + 1 + 96 + 8 + 64 + 12 + 4 + 8 + 12 + 4 + 1 + 16 + 288 +
               68 + 4) +
          8 * C + 64 * sh.Fcap;
@@ -292,6 +293,7 @@ int batch_build(int device, const std::vector<FrameSrc>& fs_in, const tgmm_em_op
     F.nbl = (int)(round_up(std::max<long long>(f.S, 1), kAlign) / 256);
     F.max_iter = o.max_iter;
     F.tol = o.tol;
+    F.packed = 1;
     v += f.N;
     s += round_up(std::max<long long>(f.S, 1), kAlign);
     g += round_up(f.G + 1, kAlign);

@@ -632,6 +632,7 @@ static int activate(tgmm_server* s, Ticket& t, int f) {
   F.nbl = (int)(t.sa / 256);
   F.max_iter = s->o.em.max_iter;
   F.tol = s->o.em.tol;
+  F.packed = 1;
   k_set_frame<<<1, 64, 0, st>>>(P.fr + f, F);
   HIPCHK(hipGetLastError());
   if (int rc = map_ranges(s, t, f)) return rc;
