@@ -281,8 +281,9 @@ int tgmm_batch_stats(tgmm_batch* b, tgmm_batch_stats_t* stats);
  * same results (the kernels fix their summation order independently of the
  * shape); only the speed changes. Defaults: {16, 4, 16, 256, 256, 8, 6, 2}. */
 typedef struct tgmm_tiles {
-  int ll_group;    /* log-likelihood: 16, 32 or 64, for 2 * ll_group
-                      supervoxels per block (at most 64, and 32 for K > 8);
+  int ll_group;    /* log-likelihood: 16 or 32 for 2 * ll_group supervoxels
+                      per block of 256 threads (32 for K > 8), 64 for 16
+                      per block of 128;
                       1 (with ll_unroll 1) = one lane per one-voxel
                       supervoxel, voxel mode only, its default there */
   int ll_unroll;   /* log-likelihood: voxels a thread loads at once, 1, 2

@@ -81,8 +81,9 @@ struct alignas(32) EPack {
 // results: the kernels fix their floating-point summation order
 // independently of the shape.
 struct Tiles {
-  int ll_gw;      // log-likelihood: 16, 32, 64 for 2*ll_gw supervoxels per block
-                  // (at most 64; 32 for K > 8); 1 = a lane per voxel (voxel mode)
+  int ll_gw;      // log-likelihood: 16, 32 for 2*ll_gw supervoxels per 256-thread block
+                  // (32 for K > 8), 64 for 16 per 128-thread block; 1 = a lane per
+                  // voxel (voxel mode)
   int ll_u;       // log-likelihood: voxels a thread loads at once, 1, 2, 4 (8 with ll_gw 16)
   int ms_lanes;   // lanes per Gaussian, M-step statistics: 4, 8, 16
   int sv_block;   // threads per block, E-step and pair scatter: 64, 128, 256

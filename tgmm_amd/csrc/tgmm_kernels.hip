@@ -2162,12 +2162,12 @@ __device__ inline double class_butterfly(const double* c) {
   return d[0] + d[1];
 }
 
-template <int KM, int B, int UV>
-__global__ __launch_bounds__(256) void k_loglik(DevPtrs P, KOpts O) {
+template <int KM, int B, int UV, int NT>
+__global__ __launch_bounds__(NT) void k_loglik(DevPtrs P, KOpts O) {
   constexpr int LPS = KM <= 8 ? 8 : 16;  // lanes per supervoxel, constants
-  constexpr int SPP = 256 / LPS;         // supervoxels per constants pass
-  constexpr int CH = 256 * UV;           // voxels per chunk
-  static_assert(B % SPP == 0 && 256 % B == 0, "block shape");
+  constexpr int SPP = NT / LPS;          // supervoxels per constants pass
+  constexpr int CH = NT * UV;            // voxels per chunk
+  static_assert(B % SPP == 0 && 256 % B == 0 && B >= 4, "block shape");
   __shared__ float4 cst[B * KM * 3];
   __shared__ float4 anc[B];   // anchor, and the number of candidates kept (int bits)
   __shared__ int vst[B + 1];  // first voxel of each supervoxel, relative to vbase
@@ -2193,7 +2193,7 @@ __global__ __launch_bounds__(256) void k_loglik(DevPtrs P, KOpts O) {
   auto load_chunk = [&](int c0, int4 (&raw)[UV]) {
 #pragma unroll
     for (int u = 0; u < UV; u++) {
-      const long long g = vbase + c0 + u * 256 + threadIdx.x;
+      const long long g = vbase + c0 + u * NT + threadIdx.x;
       raw[u] = make_int4(0, 0, 0, 0);
       if (g < vlim) {
         if (packed) {
@@ -2210,7 +2210,7 @@ __global__ __launch_bounds__(256) void k_loglik(DevPtrs P, KOpts O) {
   int4 cur[UV];
   load_chunk(0, cur);
   if (threadIdx.x == 0) vend_s = 0;
-  for (int x = threadIdx.x; x < B * 16; x += 256) cls[x] = 0.0;
+  for (int x = threadIdx.x; x < B * 16; x += NT) cls[x] = 0.0;
   __syncthreads();
   // 1. constants
   for (int p = 0; p < B; p += SPP) {
@@ -2280,7 +2280,7 @@ __global__ __launch_bounds__(256) void k_loglik(DevPtrs P, KOpts O) {
   __syncthreads();
   const int vend = vend_s;
   // padding supervoxels (after the frame's last) start where the real ones end
-  for (int i = threadIdx.x; i <= B; i += 256)
+  for (int i = threadIdx.x; i <= B; i += NT)
     if (i == B || vst[i] < 0) vst[i] = vend;
   __syncthreads();
   // 2. and 3. the voxels, chunk by chunk
@@ -2307,7 +2307,7 @@ __global__ __launch_bounds__(256) void k_loglik(DevPtrs P, KOpts O) {
     }
 #pragma unroll
     for (int u = 0; u < UV; u++) {
-      const int v = c0 + u * 256 + threadIdx.x;
+      const int v = c0 + u * NT + threadIdx.x;
       if (v >= vend) continue;
       int i = 0;  // the last supervoxel starting at or before v
       if (packed) {
@@ -2364,7 +2364,7 @@ __global__ __launch_bounds__(256) void k_loglik(DevPtrs P, KOpts O) {
           val = (double)vv[u].w * (double)(mx + __builtin_amdgcn_logf(sum));
         }
       }
-      vals[u * 256 + threadIdx.x] = val;
+      vals[u * NT + threadIdx.x] = val;
     }
     __syncthreads();
     // class (i, c): the chunk's voxels of supervoxel i at positions = c mod 16,
@@ -2378,7 +2378,7 @@ __global__ __launch_bounds__(256) void k_loglik(DevPtrs P, KOpts O) {
         if (vst[ib + st] <= vl) ib += st;
       }
     }
-    for (int task = ia * 16 + threadIdx.x; task < (ib + 1) * 16; task += 256) {
+    for (int task = ia * 16 + threadIdx.x; task < (ib + 1) * 16; task += NT) {
       const int i = task >> 4, c = task & 15;
       const int s0 = vst[i], lo = max(s0, c0), hi = min(vst[i + 1], c0 + CH);
       if (lo >= hi) continue;
@@ -2602,26 +2602,34 @@ static void launch_loglik(const DevPtrs& P, const KOpts& O, hipStream_t st,
     k_ll_final<1><<<P.nframes, ll_final_threads<1>(), 0, st>>>(P);
     return;
   }
-  // B supervoxels per block: 2 * ll_group, at most 64 (32 for K > 8, whose
-  // constants take twice the LDS); ll_unroll voxels per thread in flight
-  const int B = std::min(2 * gw, KM > 8 ? 32 : 64);
+  // ll_group 16 and 32: 2 * ll_group supervoxels per block of 256 threads
+  // (32 for K > 8, whose constants take twice the LDS); 64: 16 supervoxels
+  // per block of 128 threads (more blocks in flight). ll_unroll voxels per
+  // thread in flight.
+  const int B = gw == 64 ? 16 : std::min(2 * gw, KM > 8 ? 32 : 64);
   const unsigned g = nb(P.Sext, B);
-#define TGMM_LL(B_, UV_)                                                        \
-  if (B == B_ && spg == UV_) {                                                  \
-    k_loglik<KM, B_, UV_><<<g, 256, 0, st>>>(P, O);                             \
+#define TGMM_LL(B_, UV_, NT_)                                                   \
+  if (B == B_ && spg == UV_ && (NT_ == 128) == (gw == 64)) {                    \
+    k_loglik<KM, B_, UV_, NT_><<<g, NT_, 0, st>>>(P, O);                        \
     if (ev_after) (void)hipEventRecord(ev_after, st);                           \
     k_ll_final<256 / B_><<<P.nframes, ll_final_threads<256 / B_>(), 0, st>>>(P); \
     return;                                                                     \
   }
-  TGMM_LL(32, 1)
-  TGMM_LL(32, 2)
-  TGMM_LL(32, 4)
-  TGMM_LL(32, 8)
+  TGMM_LL(32, 1, 256)
+  TGMM_LL(32, 2, 256)
+  TGMM_LL(32, 4, 256)
+  TGMM_LL(32, 8, 256)
   if constexpr (KM <= 8) {
-    TGMM_LL(64, 1)
-    TGMM_LL(64, 2)
-    TGMM_LL(64, 4)
-    TGMM_LL(64, 8)
+    TGMM_LL(64, 1, 256)
+    TGMM_LL(64, 2, 256)
+    TGMM_LL(64, 4, 256)
+    TGMM_LL(16, 1, 128)
+    TGMM_LL(16, 2, 128)
+    TGMM_LL(16, 4, 128)
+  } else {
+    TGMM_LL(16, 1, 128)
+    TGMM_LL(16, 2, 128)
+    TGMM_LL(16, 4, 128)
   }
 #undef TGMM_LL
 }
