@@ -309,8 +309,9 @@ def main():
     # a step: the initial Gaussians restored on the device, the solve (graph
     # replays, the last search's slot layout), and the results on the host:
     # the records with the r > 0.5 supervoxel assignment formed on the device
-    # and copied back (rnk and ind stay in HBM, as the reference leaves them)
-    recs = [np.zeros(f.G, GMM_DTYPE) for f in frames]
+    # and copied back into page-locked host arrays kept across frames (rnk
+    # and ind stay in HBM, as the reference leaves them)
+    recs = [em.pinned(np.zeros(f.G, GMM_DTYPE)) for f in frames]
 
     def step():
         batch.reset()
@@ -403,7 +404,7 @@ def main():
                    "voxels": N_tot, "supervoxels": S_tot,
                    "gaussians": sum(f.G for f in frames), "K": K, "iterations": iters,
                    "step": "reset + graph-replayed solve + slot layout + records with the "
-                           "supervoxel assignment copied to the host",
+                           "supervoxel assignment copied to page-locked host memory",
                    "parallelism": "1 GPU"},
         "rnk_updates_per_s": rnk_updates,
         "voxel_candidate_updates_per_s": vox_updates,

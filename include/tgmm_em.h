@@ -313,8 +313,11 @@ int tgmm_batch_autotune(tgmm_batch* b, int reps, tgmm_tiles* chosen, char* repor
                         int report_cap);
 
 /* Average device time (ms) of one launch of the named kernel over `reps`
- * eager iterations, measured with hipEvents on the batch stream. Names:
- * "loglik", "estep", "knn", "mstep", "moments". Leaves the state advanced. */
+ * eager iterations, measured with hipEvents on the batch stream. In each
+ * iteration the events bracket 8 back-to-back launches of "loglik" (it
+ * writes only its block partials) and one launch of "estep", "knn" and
+ * "mstep" (they update state that a repeated launch would read). Leaves the
+ * state advanced. */
 int tgmm_batch_time_kernel(tgmm_batch* b, const char* name, int reps,
                            double* avg_ms, double* bytes_per_launch);
 int tgmm_batch_destroy(tgmm_batch* b);

@@ -157,6 +157,7 @@ struct DevPtrs {
   // kNN: supervoxels whose K-th distance is tied with a Gaussian outside
   // their K best (their candidate set follows the reference's scan order)
   int* tie_list;    // Scap
+  unsigned char* tie_mark;  // Scap: 1 if the supervoxel is in this iteration's tie list
   int* knn_ctl;     // [0] ties listed this iteration, [1] supervoxels whose
                     //     final slot layout found a different set (stays 0),
                     //     [2] Gaussians in long_list this iteration
@@ -188,10 +189,19 @@ hipError_t launch_prep(const DevPtrs& P, const KOpts& O, hipStream_t st);
 // iteration state, initial sum of alpha and first E-step constants per frame;
 // the frames' kNN grid counts must be zero
 hipError_t launch_reset(const DevPtrs& P, const KOpts& O, hipStream_t st);
-// one EM iteration of every active frame; optional events around one kernel
+// A second stream, and the events that fork it from and join it back to
+// the iteration's stream (recorded into a captured graph as two branches).
+struct SideStream {
+  hipStream_t st = nullptr;
+  hipEvent_t fork = nullptr, join = nullptr;
+};
+// one EM iteration of every active frame; optional events around one kernel.
+// With a side stream (and no timed kernel), the replays of the supervoxels
+// whose kNN search listed a tie run on it beside the other supervoxels'
+// E-step; the results are the same either way.
 hipError_t launch_iteration(const DevPtrs& P, const KOpts& O, hipStream_t st,
                             hipEvent_t ev_before = nullptr, hipEvent_t ev_after = nullptr,
-                            int timed_kernel = -1);
+                            int timed_kernel = -1, const SideStream* side = nullptr);
 int kmax_for(int K);
 
 // Priors of a stream's next frame from its previous posterior, in place on
@@ -219,5 +229,12 @@ hipError_t launch_finalize(const DevPtrs& P, const KOpts& O, int cap, int only_f
 
 // timed_kernel ids for launch_iteration
 enum { TK_KNN = 0, TK_ESTEP = 1, TK_MSTEP = 2, TK_LOGLIK = 3 };
+// launches between the timing events: the log-likelihood, which writes only
+// its block partials, runs kTimedLaunches times back to back, so the event
+// pair's own dispatch gap is spread over them; every other kernel updates
+// state that its next launch would read (the E-step adds to the pair
+// counts) and runs once
+constexpr int kTimedLaunches = 8;
+inline int timed_launches(int tk) { return tk == TK_LOGLIK ? kTimedLaunches : 1; }
 
 }  // namespace tgmm

@@ -785,8 +785,13 @@ __device__ inline void knn_shells(const DevPtrs& P, const FrameDev* F, const int
 // lanes of a merged list is kept once), so every lane of the group holds the
 // K nearest of all references seen.
 
+// Candidates a lane queues before they are inserted, k_knn (dynamic LDS:
+// kKnnQueue entries per thread).
+constexpr int kKnnQueue = 20;
+
 template <int KM, int KC, int QL>
 __global__ __launch_bounds__(256) void k_knn(DevPtrs P, KOpts O) {
+  extern __shared__ float2 kq[];  // entry e of thread t at e * blockDim.x + t
   const int sub = threadIdx.x % QL;
   const long long qpb = blockDim.x / QL;  // queries per block
   const long long i = (long long)blockIdx.x * qpb + threadIdx.x / QL;
@@ -840,6 +845,29 @@ __global__ __launch_bounds__(256) void k_knn(DevPtrs P, KOpts O) {
     }
     if (all) kbound = mx;
   }
+  // The 3x3x3 block's references are first filtered, each lane queueing in
+  // LDS those within kbound and the list's K-th distance (a reference beyond
+  // either cannot enter the K best nor tie the final K-th, which is at most
+  // both), then inserted together: the wave then runs the insertion once
+  // per queued entry of its longest queue rather than once per reference.
+  float qthr = kbound;
+  int nq = 0;
+  auto enqueue = [&](const float4 cur) {
+    const float d = knn_dist(q0, q1, q2, cur.x, cur.y, cur.z, s0, s1, s2);
+    if (d <= qthr) {
+      kq[nq * blockDim.x + threadIdx.x] = make_float2(d, cur.w);
+      nq++;
+    }
+  };
+  auto flush = [&]() {  // every lane of the wave takes part
+    for (int e = 0; __any(e < nq); e++)
+      if (e < nq) {
+        const float2 cq = kq[e * blockDim.x + threadIdx.x];
+        knn_try<KM>(bd, bi, cq.x, __float_as_int(cq.y), K, tie, kd);
+      }
+    nq = 0;
+    qthr = fminf(kbound, kd);
+  };
   // squared distance from the query to the box of cell index ci along axis a
   // (FP64 from the f32 grid values), shrunk by the same relative margin as
   // the shell bound; 0 inside. A mean binned into a cell (an edge cell holds
@@ -886,7 +914,7 @@ __global__ __launch_bounds__(256) void k_knn(DevPtrs P, KOpts O) {
     int off[9];  // reference position of flat index p in row t: off[t] + p
 #pragma unroll
     for (int t = 0; t < 9; t++) off[t] = pb[t] - cum[t];
-    for (int p0 = sub; p0 < tot; p0 += KC * QL) {
+    for (int p0 = sub; __any(p0 < tot); p0 += KC * QL) {
       float4 v[KC];
 #pragma unroll
       for (int w = 0; w < KC; w++) {
@@ -901,8 +929,10 @@ __global__ __launch_bounds__(256) void k_knn(DevPtrs P, KOpts O) {
       }
 #pragma unroll
       for (int w = 0; w < KC; w++)
-        if (p0 + w * QL < tot) consider(v[w]);
+        if (p0 + w * QL < tot) enqueue(v[w]);
+      if (__any(nq > kKnnQueue - KC)) flush();
     }
+    flush();
   }
   if (QL > 1) knn_merge<KM, QL>(bd, bi, K, tie, kd);
   knn_shells<KM, QL>(P, F, c, u, K, sub, bd, bi, tie, kd, consider);
@@ -912,6 +942,7 @@ __global__ __launch_bounds__(256) void k_knn(DevPtrs P, KOpts O) {
 #pragma unroll
     for (int j = 0; j < KM; j++)
       if (j < K) P.ind[(long long)j * S + s] = (bd[j] <= 1e32f) ? bi[j] : -1;
+    P.tie_mark[s] = tie ? 1 : 0;
     if (tie) P.tie_list[atomicAdd(P.knn_ctl, 1)] = (int)s;
   }
 }
@@ -1038,6 +1069,7 @@ __global__ __launch_bounds__(64 * NW) void k_knn_cell(DevPtrs P, KOpts O) {
 #pragma unroll
     for (int j = 0; j < KM; j++)
       if (j < K) P.ind[(long long)j * S + s] = (bd[j] <= 1e32f) ? bi[j] : -1;
+    P.tie_mark[s] = tie ? 1 : 0;
     if (tie) P.tie_list[atomicAdd(P.knn_ctl, 1)] = (int)s;
   }
 }
@@ -1058,13 +1090,15 @@ struct ScanSlots {
   int mi[KM];
   float thr;
   int pos;
+  // a lane without a query takes nothing and keeps its slots (the lanes of a
+  // wave may be replayed in turns)
   __device__ inline void init(bool live) {
 #pragma unroll
     for (int j = 0; j < KM; j++) {
-      md[j] = 1e32f;
-      mi[j] = -1;
+      md[j] = live ? 1e32f : md[j];
+      mi[j] = live ? -1 : mi[j];
     }
-    thr = live ? 1e32f : -INFINITY;  // a lane without a query takes nothing
+    thr = live ? 1e32f : -INFINITY;
     pos = 0;
   }
   // distance terms of one Gaussian in the reference's order
@@ -1477,11 +1511,14 @@ __global__ __launch_bounds__(64) void k_knn_exact(DevPtrs P, KOpts O, int words_
 
 // The slot layout of every supervoxel after a solve: the reference's scan of
 // its last kNN search replayed exactly (scan_replay), 64 supervoxels of
-// one frame in grid-cell order per wave. The candidates found by the
-// iterations' search (ind, rnk) are then put in these slots, responsibilities
-// moving with their Gaussian; a supervoxel whose set differs counts in
-// knn_ctl[1] (never, by construction). win[s] receives the (Gaussian, slot)
-// whose responsibility is above 0.5, if any.
+// one frame in grid-cell order per wave. The wave's supervoxels are replayed
+// one row of grid cells (z, y) at a time, so that the box of the queries
+// replayed together stays compact where the wave's cells wrap from the end
+// of one row to the start of the next. The candidates found by the
+// iterations' search (ind, rnk) are then put in these slots,
+// responsibilities moving with their Gaussian; a supervoxel whose set
+// differs counts in knn_ctl[1] (never, by construction). win[s] receives the
+// (Gaussian, slot) whose responsibility is above 0.5, if any.
 template <int KM>
 __global__ __launch_bounds__(64) void k_knn_layout(DevPtrs P, KOpts O, int only_frame,
                                                    int words_cap) {
@@ -1497,47 +1534,64 @@ __global__ __launch_bounds__(64) void k_knn_layout(DevPtrs P, KOpts O, int only_
   const long long g0 = F->g0;
   const int K = O.K;
   const long long i = qb + lane;
-  const bool live = i < F->s0 + F->ns;
-  const float4 qq = live ? P.q4[i] : make_float4(0.f, 0.f, 0.f, 0.f);
-  const long long s = live ? __float_as_int(qq.w) : 0;
+  const bool in = i < F->s0 + F->ns;
+  const float4 qq = in ? P.q4[i] : make_float4(0.f, 0.f, 0.f, 0.f);
+  const long long s = in ? __float_as_int(qq.w) : 0;
+  // the row of grid cells of the lane's query
+  const int row = in ? (int)((P.qcell[s] - F->c0) / F->dims[0]) : 0x7fffffff;
   ScanSlots<KM> sl;
-  scan_replay<KM, false>(P, O, F, live, qq.x, qq.y, qq.z, words_cap, bits, sl);
-  if (!live) return;
-  // the search's candidates into the reference's slots
-  int oi[KM];
-  float orr[KM];
 #pragma unroll
   for (int j = 0; j < KM; j++) {
-    oi[j] = j < K ? P.ind[(long long)j * S + s] : -1;
-    orr[j] = j < K ? P.rnk[(long long)j * S + s] : 0.0f;
+    sl.md[j] = 1e32f;
+    sl.mi[j] = -1;
   }
-  int n_old = 0, n_new = 0;
+  bool todo = in;
+  while (__any(todo)) {
+    int r0 = todo ? row : 0x7fffffff;  // the lowest row left
 #pragma unroll
-  for (int j = 0; j < KM; j++) {
-    n_old += (j < K && oi[j] >= 0) ? 1 : 0;
-    n_new += (j < K && sl.mi[j] >= 0) ? 1 : 0;
+    for (int o = 32; o >= 1; o >>= 1) r0 = min(r0, __shfl_xor(r0, o, 64));
+    const bool live = todo && row == r0;
+    scan_replay<KM, false>(P, O, F, live, qq.x, qq.y, qq.z, words_cap, bits, sl);
+    __syncthreads();  // the bitmap and list are free for the next row
+    todo = todo && !live;
   }
-  bool same = n_old == n_new;
-  int2 w = make_int2(-1, -1);
+  if (in) {
+    // the search's candidates into the reference's slots
+    int oi[KM];
+    float orr[KM];
 #pragma unroll
-  for (int j = 0; j < KM; j++) {
-    if (j >= K) continue;
-    const int k = sl.mi[j] >= 0 ? (int)(g0 + sl.mi[j]) : -1;
-    float r = 0.0f;
-    bool found = k < 0;
+    for (int j = 0; j < KM; j++) {
+      oi[j] = j < K ? P.ind[(long long)j * S + s] : -1;
+      orr[j] = j < K ? P.rnk[(long long)j * S + s] : 0.0f;
+    }
+    int n_old = 0, n_new = 0;
 #pragma unroll
-    for (int t = 0; t < KM; t++)
-      if (t < K && k >= 0 && oi[t] == k) {
-        r = orr[t];
-        found = true;
-      }
-    same &= found;
-    P.ind[(long long)j * S + s] = k;
-    P.rnk[(long long)j * S + s] = r;
-    if (k >= 0 && r > 0.5f) w = make_int2(k, j);
+    for (int j = 0; j < KM; j++) {
+      n_old += (j < K && oi[j] >= 0) ? 1 : 0;
+      n_new += (j < K && sl.mi[j] >= 0) ? 1 : 0;
+    }
+    bool same = n_old == n_new;
+    int2 w = make_int2(-1, -1);
+#pragma unroll
+    for (int j = 0; j < KM; j++) {
+      if (j >= K) continue;
+      const int k = sl.mi[j] >= 0 ? (int)(g0 + sl.mi[j]) : -1;
+      float r = 0.0f;
+      bool found = k < 0;
+#pragma unroll
+      for (int t = 0; t < KM; t++)
+        if (t < K && k >= 0 && oi[t] == k) {
+          r = orr[t];
+          found = true;
+        }
+      same &= found;
+      P.ind[(long long)j * S + s] = k;
+      P.rnk[(long long)j * S + s] = r;
+      if (k >= 0 && r > 0.5f) w = make_int2(k, j);
+    }
+    P.win[s] = w;
+    if (!same) atomicAdd(P.knn_ctl + 1, 1);
   }
-  P.win[s] = w;
-  if (!same) atomicAdd(P.knn_ctl + 1, 1);
 }
 
 // The returned records (GaussianMixtureModelCUDA, 288 bytes), one lane per
@@ -1641,12 +1695,13 @@ __device__ inline int wave_take(int* cnt, int key) {
   return key >= 0 ? old - 1 - (lane - r.head) : -1;
 }
 
+// The E-step of supervoxel s (ok: a real supervoxel of an active frame):
+// its responsibilities from the candidates' log scores, and the pair counts
+// of its significant candidates. Every lane of the wave calls it together
+// (the pair counting is wave-wide).
 template <int KM>
-__global__ __launch_bounds__(256) void k_estep(DevPtrs P, KOpts O) {
-  const long long s = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  const int f = P.sblk_frame[s >> 8];
-  // no early exit: every lane takes part in the wave-wide pair counting
-  const bool ok = f >= 0 && P.fr[f].active && P.sv_frame[s] == f;
+__device__ inline void estep_one(const DevPtrs& P, const KOpts& O, long long s, bool ok) {
+  const int f = ok ? P.sv_frame[s] : 0;
   const long long S = P.Scap;
   const int K = O.K;
   const double psi_sum = ok ? P.fr[f].psi_sum_alpha : 0.0;
@@ -1713,6 +1768,31 @@ __global__ __launch_bounds__(256) void k_estep(DevPtrs P, KOpts O) {
       if (ok) P.rnk[(long long)j * S + s] = r;
       wave_count(P.seg_cnt, (ok && kk[j] >= 0 && pair_significant(r)) ? kk[j] : -1);
     }
+  }
+}
+
+// The E-step of every supervoxel of the active frames; skip_ties: not of
+// those whose kNN search listed a tie this iteration (their candidates are
+// being replayed meanwhile, and k_estep_ties takes them).
+template <int KM>
+__global__ __launch_bounds__(256) void k_estep(DevPtrs P, KOpts O, int skip_ties) {
+  const long long s = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int f = P.sblk_frame[s >> 8];
+  // no early exit: every lane takes part in the wave-wide pair counting
+  const bool ok = f >= 0 && P.fr[f].active && P.sv_frame[s] == f &&
+                  !(skip_ties && P.tie_mark[s] != 0);
+  estep_one<KM>(P, O, s, ok);
+}
+
+// The E-step of the supervoxels listed with a tie this iteration, after
+// their replay (k_knn_exact).
+template <int KM>
+__global__ __launch_bounds__(256) void k_estep_ties(DevPtrs P, KOpts O) {
+  const int cnt = P.knn_ctl[0];
+  for (int tb = blockIdx.x * blockDim.x; tb < cnt; tb += gridDim.x * blockDim.x) {
+    const int t = tb + (int)threadIdx.x;
+    const bool ok = t < cnt;
+    estep_one<KM>(P, O, ok ? P.tie_list[t] : 0, ok);
   }
 }
 
@@ -2195,14 +2275,17 @@ __global__ __launch_bounds__(NT) void k_loglik(DevPtrs P, KOpts O) {
     for (int u = 0; u < UV; u++) {
       const long long g = vbase + c0 + u * NT + threadIdx.x;
       raw[u] = make_int4(0, 0, 0, 0);
-      if (g < vlim) {
+      if (g < vlim) {  // streamed once: non-temporal loads
         if (packed) {
-          const int2 pk = P.vpk[g];
-          raw[u].x = pk.x;
-          raw[u].y = pk.y;
+          const unsigned long long pk =
+              __builtin_nontemporal_load(reinterpret_cast<const unsigned long long*>(P.vpk) + g);
+          raw[u].x = (int)(unsigned)pk;
+          raw[u].y = (int)(unsigned)(pk >> 32);
         } else {
-          raw[u] = make_int4(__float_as_int(P.vx[g]), __float_as_int(P.vy[g]),
-                             __float_as_int(P.vz[g]), __float_as_int(P.vimg[g]));
+          raw[u] = make_int4(__float_as_int(__builtin_nontemporal_load(P.vx + g)),
+                             __float_as_int(__builtin_nontemporal_load(P.vy + g)),
+                             __float_as_int(__builtin_nontemporal_load(P.vz + g)),
+                             __float_as_int(__builtin_nontemporal_load(P.vimg + g)));
         }
       }
     }
@@ -2489,40 +2572,31 @@ __global__ __launch_bounds__(256) void k_loglik_voxel(DevPtrs P, KOpts O) {
   if (threadIdx.x == 0) P.part_s[blockIdx.x] = tree[0];
 }
 
-// Frame totals of the log-likelihood from the block partials of k_loglik (B
-// supervoxels per block), one block per frame: the upper levels of each
-// 256-supervoxel tree, then frame_sum_w64 over the frame's groups; the
+// The upper levels of each 256-supervoxel group's tree from the block
+// partials of k_loglik (BPG = 256 / B blocks per group), one thread per
+// group of an active frame, all frames together.
+template <int BPG>
+__global__ __launch_bounds__(256) void k_ll_groups(DevPtrs P) {
+  const long long g = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (g >= (P.Sext + 255) / 256) return;
+  const int f = P.sblk_frame[g];
+  if (f < 0 || !P.fr[f].active) return;
+  double q[BPG];
+#pragma unroll
+  for (int i = 0; i < BPG; i++) q[i] = P.part_s[g * BPG + i];
+  P.part_sg[g] = pairwise_tree<BPG>(q);
+}
+
+// Frame totals of the log-likelihood, one wave per frame: frame_sum_w64 over
+// the frame's group sums (the block partials themselves when BPG = 1); the
 // convergence test (GMEM.cu:4145-4148) and the iteration count.
 template <int BPG>
-constexpr int ll_final_threads() { return BPG >= 32 ? 256 : 1024; }
-
-template <int BPG>
-__global__ __launch_bounds__(ll_final_threads<BPG>()) void k_ll_final(DevPtrs P) {
+__global__ __launch_bounds__(64) void k_ll_final(DevPtrs P) {
   const int f = blockIdx.x;
   FrameDev* F = P.fr + f;
   if (!F->active) return;
   const long long g0 = F->s0 >> 8;
-  const int ng = F->nbl;
-  // the groups' trees, kLLG groups a thread at a time (their loads together)
-  constexpr int NT = ll_final_threads<BPG>();
-  constexpr int kLLG = BPG >= 16 ? 1 : 4;
-  for (int gb = 0; gb < ng; gb += NT * kLLG) {
-    double q[kLLG][BPG];
-#pragma unroll
-    for (int u = 0; u < kLLG; u++) {
-      const int g = gb + u * NT + threadIdx.x;
-#pragma unroll
-      for (int i = 0; i < BPG; i++) q[u][i] = g < ng ? P.part_s[(g0 + g) * BPG + i] : 0.0;
-    }
-#pragma unroll
-    for (int u = 0; u < kLLG; u++) {
-      const int g = gb + u * NT + threadIdx.x;
-      if (g < ng) P.part_sg[g0 + g] = pairwise_tree<BPG>(q[u]);
-    }
-  }
-  __syncthreads();
-  if (threadIdx.x >= 64) return;
-  const double ll = frame_sum_w64(P.part_sg + g0, ng);
+  const double ll = frame_sum_w64((BPG == 1 ? P.part_s : P.part_sg) + g0, F->nbl);
   if (threadIdx.x == 0) {
     F->ll3 = F->ll2;
     F->ll2 = F->ll1;
@@ -2596,10 +2670,13 @@ template <int KM>
 static void launch_loglik(const DevPtrs& P, const KOpts& O, hipStream_t st,
                           hipEvent_t ev_after) {
   const int gw = O.tiles.ll_gw, spg = O.tiles.ll_u;
+  // timed runs: kTimedLaunches back-to-back launches (the kernel only writes
+  // its block partials, so every launch gives the same values)
+  const int reps = ev_after ? kTimedLaunches : 1;
   if (gw == 1) {  // one lane per one-voxel supervoxel (voxel mode)
-    k_loglik_voxel<KM><<<nb(P.Sext, 256), 256, 0, st>>>(P, O);
+    for (int r = 0; r < reps; r++) k_loglik_voxel<KM><<<nb(P.Sext, 256), 256, 0, st>>>(P, O);
     if (ev_after) (void)hipEventRecord(ev_after, st);
-    k_ll_final<1><<<P.nframes, ll_final_threads<1>(), 0, st>>>(P);
+    k_ll_final<1><<<P.nframes, 64, 0, st>>>(P);
     return;
   }
   // ll_group 16 and 32: 2 * ll_group supervoxels per block of 256 threads
@@ -2610,9 +2687,10 @@ static void launch_loglik(const DevPtrs& P, const KOpts& O, hipStream_t st,
   const unsigned g = nb(P.Sext, B);
 #define TGMM_LL(B_, UV_, NT_)                                                   \
   if (B == B_ && spg == UV_ && (NT_ == 128) == (gw == 64)) {                    \
-    k_loglik<KM, B_, UV_, NT_><<<g, NT_, 0, st>>>(P, O);                        \
+    for (int r = 0; r < reps; r++) k_loglik<KM, B_, UV_, NT_><<<g, NT_, 0, st>>>(P, O); \
     if (ev_after) (void)hipEventRecord(ev_after, st);                           \
-    k_ll_final<256 / B_><<<P.nframes, ll_final_threads<256 / B_>(), 0, st>>>(P); \
+    k_ll_groups<256 / B_><<<nb(nb(P.Sext, 256), 256), 256, 0, st>>>(P);        \
+    k_ll_final<256 / B_><<<P.nframes, 64, 0, st>>>(P);                          \
     return;                                                                     \
   }
   TGMM_LL(32, 1, 256)
@@ -2658,14 +2736,15 @@ static void launch_knn(const DevPtrs& P, const KOpts& O, hipStream_t st) {
   }
   const int bk = O.tiles.knn_block, ql = O.tiles.knn_lanes;
   const unsigned g = nb(P.Sext, bk / ql);
+  const size_t lds = (size_t)bk * kKnnQueue * sizeof(float2);
   if (O.tiles.knn_chunk == 4) {
-    if (ql == 1) k_knn<KM, 4, 1><<<g, bk, 0, st>>>(P, O);
-    else if (ql == 2) k_knn<KM, 4, 2><<<g, bk, 0, st>>>(P, O);
-    else k_knn<KM, 4, 4><<<g, bk, 0, st>>>(P, O);
+    if (ql == 1) k_knn<KM, 4, 1><<<g, bk, lds, st>>>(P, O);
+    else if (ql == 2) k_knn<KM, 4, 2><<<g, bk, lds, st>>>(P, O);
+    else k_knn<KM, 4, 4><<<g, bk, lds, st>>>(P, O);
   } else {
-    if (ql == 1) k_knn<KM, 8, 1><<<g, bk, 0, st>>>(P, O);
-    else if (ql == 2) k_knn<KM, 8, 2><<<g, bk, 0, st>>>(P, O);
-    else k_knn<KM, 8, 4><<<g, bk, 0, st>>>(P, O);
+    if (ql == 1) k_knn<KM, 8, 1><<<g, bk, lds, st>>>(P, O);
+    else if (ql == 2) k_knn<KM, 8, 2><<<g, bk, lds, st>>>(P, O);
+    else k_knn<KM, 8, 4><<<g, bk, lds, st>>>(P, O);
   }
 }
 
@@ -2700,21 +2779,31 @@ hipError_t launch_finalize(const DevPtrs& P, const KOpts& O, int cap, int only_f
 
 template <int KM>
 static hipError_t iteration_t(const DevPtrs& P, const KOpts& O, hipStream_t st, hipEvent_t e0,
-                              hipEvent_t e1, int tk) {
+                              hipEvent_t e1, int tk, const SideStream* side) {
   const unsigned gb = nb(P.Gext, kGB);
   const int bs = O.tiles.sv_block;
   k_scan_frames<<<P.nframes, 1024, 0, st>>>(P, 0, 1, O.K);
   k_grid_scatter<<<gb, kGB, 0, st>>>(P);
   if (tk == TK_KNN && e0) (void)hipEventRecord(e0, st);
   launch_knn<KM>(P, O, st);
-  {
-    const int words = (int)std::min<long long>(kLayoutWords, (P.Gext + 63) / 64);
-    k_knn_exact<KM><<<128, 64, scan_lds_bytes(words), st>>>(P, O, words);
-  }
   if (tk == TK_KNN && e1) (void)hipEventRecord(e1, st);
-  if (tk == TK_ESTEP && e0) (void)hipEventRecord(e0, st);
-  k_estep<KM><<<nb(P.Sext, bs), bs, 0, st>>>(P, O);
-  if (tk == TK_ESTEP && e1) (void)hipEventRecord(e1, st);
+  const int words = (int)std::min<long long>(kLayoutWords, (P.Gext + 63) / 64);
+  if (side && side->st && !e0) {
+    // the replays of the tied supervoxels and their E-step on the side
+    // stream, beside the E-step of every other supervoxel
+    (void)hipEventRecord(side->fork, st);
+    (void)hipStreamWaitEvent(side->st, side->fork, 0);
+    k_knn_exact<KM><<<128, 64, scan_lds_bytes(words), side->st>>>(P, O, words);
+    k_estep_ties<KM><<<4, 256, 0, side->st>>>(P, O);
+    (void)hipEventRecord(side->join, side->st);
+    k_estep<KM><<<nb(P.Sext, bs), bs, 0, st>>>(P, O, 1);
+    (void)hipStreamWaitEvent(st, side->join, 0);
+  } else {
+    k_knn_exact<KM><<<128, 64, scan_lds_bytes(words), st>>>(P, O, words);
+    if (tk == TK_ESTEP && e0) (void)hipEventRecord(e0, st);
+    k_estep<KM><<<nb(P.Sext, bs), bs, 0, st>>>(P, O, 0);
+    if (tk == TK_ESTEP && e1) (void)hipEventRecord(e1, st);
+  }
   k_scan_frames<<<P.nframes, 1024, 0, st>>>(P, 1, 1, O.K);
   k_pair_scatter<<<nb(P.Sext, bs), bs, 0, st>>>(P, O);
   k_seg_sort<<<nb(P.Gext, 16) + kSortLongBlocks, 256, 0, st>>>(P, (int)nb(P.Gext, 16));
@@ -2729,15 +2818,15 @@ static hipError_t iteration_t(const DevPtrs& P, const KOpts& O, hipStream_t st, 
 }
 
 hipError_t launch_iteration(const DevPtrs& P, const KOpts& O, hipStream_t st, hipEvent_t e0,
-                            hipEvent_t e1, int tk) {
+                            hipEvent_t e1, int tk, const SideStream* side) {
   if (!tiles_valid(O.tiles) || (O.tiles.ll_gw == 1 && !O.voxel)) return hipErrorInvalidValue;
   switch (kmax_for(O.K)) {
     case 5:
-      return iteration_t<5>(P, O, st, e0, e1, tk);
+      return iteration_t<5>(P, O, st, e0, e1, tk, side);
     case 8:
-      return iteration_t<8>(P, O, st, e0, e1, tk);
+      return iteration_t<8>(P, O, st, e0, e1, tk, side);
     default:
-      return iteration_t<16>(P, O, st, e0, e1, tk);
+      return iteration_t<16>(P, O, st, e0, e1, tk, side);
   }
 }
 

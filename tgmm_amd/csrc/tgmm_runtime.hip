@@ -119,6 +119,7 @@ int ws_alloc(Workspace* w, const WsShape& sh, int K) {
   AL(P.cg4, G);
   AL(P.mknn, G);
   AL(P.tie_list, S);
+  AL(P.tie_mark, S);
   AL(P.knn_ctl, 64);
   AL(P.long_list, G);
   AL(P.win, S);
@@ -142,7 +143,7 @@ void ws_free(Workspace* w) {
 
 long long ws_bytes_estimate(const WsShape& sh, int K) {
   const long long S = sh.Scap, G = sh.Gcap, C = sh.Ccap, V = sh.Vcap;
-  return 24 * V + S * (16 + 8 + 4 + 12 + 80 + 80 + 12 + 16 + 16LL * K + 1 + 4 + 8) +
+  return 24 * V + S * (16 + 8 + 4 + 12 + 80 + 80 + 12 + 16 + 16LL * K + 1 + 4 + 8 + 1) +
          G * (4 + 8 * GS_NFIELDS * 2 + 1 + 96 + 8 + 64 + 12 + 4 + 8 + 12 + 4 + 1 + 16 + 288 +
               68 + 4) +
          8 * C + 64 * sh.Fcap;
@@ -522,8 +523,9 @@ int batch_capture(tgmm_batch* b) {
   if (b->gexec) return TGMM_OK;
   std::memcpy(&b->cap_P, &b->ws.P, sizeof(DevPtrs));  // padding included, as compared
   std::memcpy(&b->cap_ko, &b->ko, sizeof(KOpts));
+  if (int rc = side_stream_create(&b->side)) return rc;
   HIPCHK(hipStreamBeginCapture(b->st, hipStreamCaptureModeThreadLocal));
-  hipError_t le = launch_iteration(b->ws.P, b->ko, b->st);
+  hipError_t le = launch_iteration(b->ws.P, b->ko, b->st, nullptr, nullptr, -1, &b->side);
   hipGraph_t g = nullptr;
   hipError_t ee = hipStreamEndCapture(b->st, &g);
   if (le != hipSuccess || ee != hipSuccess) {
@@ -534,6 +536,28 @@ int batch_capture(tgmm_batch* b) {
   b->graph = g;
   HIPCHK(hipGraphInstantiate(&b->gexec, g, nullptr, nullptr, 0));
   return TGMM_OK;
+}
+
+int side_stream_create(SideStream* side) {
+  if (side->st) return TGMM_OK;
+  if (hipStreamCreateWithFlags(&side->st, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&side->fork, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&side->join, hipEventDisableTiming) != hipSuccess) {
+    side_stream_destroy(side);
+    tgmm_set_error("side stream creation failed");
+    return TGMM_ERR_DEVICE;
+  }
+  return TGMM_OK;
+}
+
+void side_stream_destroy(SideStream* side) {
+  if (side->st) {
+    (void)hipStreamSynchronize(side->st);
+    (void)hipStreamDestroy(side->st);
+  }
+  if (side->fork) (void)hipEventDestroy(side->fork);
+  if (side->join) (void)hipEventDestroy(side->join);
+  *side = SideStream();
 }
 
 extern "C" int tgmm_batch_solve(tgmm_batch* b, int mode) {
@@ -548,7 +572,8 @@ extern "C" int tgmm_batch_solve(tgmm_batch* b, int mode) {
     if (mode == TGMM_SOLVE_GRAPH) {
       HIPCHK(hipGraphLaunch(b->gexec, b->st));
     } else {
-      HIPCHK(launch_iteration(b->ws.P, b->ko, b->st));
+      if (int rc = side_stream_create(&b->side)) return rc;
+      HIPCHK(launch_iteration(b->ws.P, b->ko, b->st, nullptr, nullptr, -1, &b->side));
     }
     const bool check = (mode == TGMM_SOLVE_HOSTSYNC) || (poll && (it % 4 == 3));
     if (check && it + 1 < iters) {
@@ -625,11 +650,22 @@ extern "C" int tgmm_batch_stats(tgmm_batch* b, tgmm_batch_stats_t* st) {
 static double kernel_bytes(tgmm_batch* b, int tk) {
   const int K = b->ws.K;
   double t = 0.0;
-  for (const auto& z : b->sizes) {
+  // whether each frame's voxels are read packed (decided on the device)
+  std::vector<FrameDev> fd(b->fh.size());
+  if (!fd.empty() &&
+      hipMemcpy(fd.data(), b->ws.P.fr, fd.size() * sizeof(FrameDev), hipMemcpyDeviceToHost) !=
+          hipSuccess)
+    fd.clear();
+  for (size_t i = 0; i < b->sizes.size(); i++) {
+    const auto& z = b->sizes[i];
     const double N = (double)z.N, S = (double)z.S, G = (double)z.G;
+    const bool packed = i < fd.size() && fd[i].packed != 0 && !b->ko.voxel;
     switch (tk) {
       case TK_LOGLIK:
-        t += 16.0 * N + S * (4 + 8 + 4 + 12 + 4.0 * K) + G * (24 + 24 + 8);
+        // voxels (8 B packed, else 16 B); per supervoxel its frame, length,
+        // first voxel, anchor, ball and K candidates; a 64-byte pack per
+        // Gaussian
+        t += (packed ? 8.0 : 16.0) * N + S * (4 + 4 + 8 + 12 + 16 + 4.0 * K) + G * 64;
         break;
       case TK_ESTEP:
         t += S * (4 + 80 + 4.0 * K + 4.0 * K) + G * (8 * 11);
@@ -679,7 +715,7 @@ extern "C" int tgmm_batch_time_kernel(tgmm_batch* b, const char* name, int reps,
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
-  *avg_ms = tot / reps;
+  *avg_ms = tot / ((double)reps * timed_launches(tk));
   if (bytes_per_launch) *bytes_per_launch = kernel_bytes(b, tk);
   return TGMM_OK;
 }
@@ -741,8 +777,8 @@ extern "C" int tgmm_batch_get_tiles(tgmm_batch* b, tgmm_tiles* t) {
   return TGMM_OK;
 }
 
-// Device time of kernel `tk` summed over `iters` eager iterations from the
-// initial state, with launch shapes `t`.
+// Device time of one launch of kernel `tk`, averaged over `iters` eager
+// iterations from the initial state, with launch shapes `t`.
 static int time_shape(tgmm_batch* b, const Tiles& t, int tk, int iters, hipEvent_t e0,
                       hipEvent_t e1, double* ms_out) {
   KOpts ko = b->ko;
@@ -758,7 +794,7 @@ static int time_shape(tgmm_batch* b, const Tiles& t, int tk, int iters, hipEvent
     HIPCHK(hipEventElapsedTime(&ms, e0, e1));
     tot += ms;
   }
-  *ms_out = tot / iters;
+  *ms_out = tot / ((double)iters * timed_launches(tk));
   return TGMM_OK;
 }
 
@@ -869,6 +905,7 @@ extern "C" int tgmm_batch_destroy(tgmm_batch* b) {
   if (b->graph) (void)hipGraphDestroy(b->graph);
   if (b->st) (void)hipStreamSynchronize(b->st);
   ws_free(&b->ws);
+  side_stream_destroy(&b->side);
   if (b->own_stream && b->st) (void)hipStreamDestroy(b->st);
   delete b;
   return TGMM_OK;

@@ -190,6 +190,7 @@ struct Stream {
 struct tgmm_server {
   int device = 0;
   hipStream_t st = nullptr;   // compute
+  SideStream side;            // the iteration's side stream
   // uploads: a frame's voxel arrays go out on several copy streams at once
   // (each stream's copies run on one DMA engine, which alone does not fill
   // the host link); the first stream joins the others before the frame's
@@ -235,8 +236,9 @@ struct tgmm_server {
 
 static int server_capture(tgmm_server* s) {
   if (s->gexec) return TGMM_OK;
+  if (int rc = side_stream_create(&s->side)) return rc;
   HIPCHK(hipStreamBeginCapture(s->st, hipStreamCaptureModeThreadLocal));
-  hipError_t le = launch_iteration(s->ws.P, s->ko, s->st);
+  hipError_t le = launch_iteration(s->ws.P, s->ko, s->st, nullptr, nullptr, -1, &s->side);
   hipGraph_t g = nullptr;
   hipError_t ee = hipStreamEndCapture(s->st, &g);
   if (le != hipSuccess || ee != hipSuccess) {
@@ -1053,6 +1055,7 @@ extern "C" int tgmm_server_destroy(tgmm_server* s) {
   if (s->ev_fh) (void)hipEventDestroy(s->ev_fh);
   if (s->fh_pin) (void)hipHostFree(s->fh_pin);
   ws_free(&s->ws);
+  side_stream_destroy(&s->side);
   if (s->st) (void)hipStreamDestroy(s->st);
   for (int i = 0; i < tgmm_server::kUp; i++) {
     if (s->cst[i]) (void)hipStreamDestroy(s->cst[i]);
