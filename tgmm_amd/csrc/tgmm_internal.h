@@ -157,7 +157,9 @@ struct DevPtrs {
   // kNN: supervoxels whose K-th distance is tied with a Gaussian outside
   // their K best (their candidate set follows the reference's scan order)
   int* tie_list;    // Scap
-  unsigned char* tie_mark;  // Scap: 1 if the supervoxel is in this iteration's tie list
+  unsigned char* tie_mark;  // Scap: 1 for the supervoxels in the tie list (set by the kNN
+                            // search, cleared when the next iteration empties the list;
+                            // zero whenever knn_ctl[0] is)
   int* knn_ctl;     // [0] ties listed this iteration, [1] supervoxels whose
                     //     final slot layout found a different set (stays 0),
                     //     [2] Gaussians in long_list this iteration

@@ -466,10 +466,16 @@ __device__ inline int block_exclusive_scan1024(int v, int* lds) {
 __global__ __launch_bounds__(1024) void k_scan_frames(DevPtrs P, int which, int need_active,
                                                       int K) {
   __shared__ int lds[32];
-  // the grid scan opens an iteration: the kNN's tie list starts empty
-  if (which == 0 && blockIdx.x == 0 && threadIdx.x == 0) {
-    P.knn_ctl[0] = 0;
-    P.knn_ctl[2] = 0;  // and the list of long pair lists
+  // the grid scan opens an iteration: the kNN's tie list starts empty, the
+  // marks of the last iteration's list cleared
+  if (which == 0 && blockIdx.x == 0) {
+    const int cnt = P.knn_ctl[0];
+    for (int t = threadIdx.x; t < cnt; t += blockDim.x) P.tie_mark[P.tie_list[t]] = 0;
+    __syncthreads();  // every thread has read the count
+    if (threadIdx.x == 0) {
+      P.knn_ctl[0] = 0;
+      P.knn_ctl[2] = 0;  // and the list of long pair lists
+    }
   }
   const int f = (P.only_frame >= 0) ? P.only_frame : (int)blockIdx.x;
   if (f >= P.Fcap || (P.only_frame < 0 && f >= P.nframes)) return;
@@ -790,7 +796,7 @@ __device__ inline void knn_shells(const DevPtrs& P, const FrameDev* F, const int
 constexpr int kKnnQueue = 20;
 
 template <int KM, int KC, int QL>
-__global__ __launch_bounds__(256) void k_knn(DevPtrs P, KOpts O) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(KM <= 8 ? 4 : 1))) void k_knn(DevPtrs P, KOpts O) {
   extern __shared__ float2 kq[];  // entry e of thread t at e * blockDim.x + t
   const int sub = threadIdx.x % QL;
   const long long qpb = blockDim.x / QL;  // queries per block
@@ -942,8 +948,10 @@ __global__ __launch_bounds__(256) void k_knn(DevPtrs P, KOpts O) {
 #pragma unroll
     for (int j = 0; j < KM; j++)
       if (j < K) P.ind[(long long)j * S + s] = (bd[j] <= 1e32f) ? bi[j] : -1;
-    P.tie_mark[s] = tie ? 1 : 0;
-    if (tie) P.tie_list[atomicAdd(P.knn_ctl, 1)] = (int)s;
+    if (tie) {
+      P.tie_mark[s] = 1;
+      P.tie_list[atomicAdd(P.knn_ctl, 1)] = (int)s;
+    }
   }
 }
 
@@ -1069,8 +1077,10 @@ __global__ __launch_bounds__(64 * NW) void k_knn_cell(DevPtrs P, KOpts O) {
 #pragma unroll
     for (int j = 0; j < KM; j++)
       if (j < K) P.ind[(long long)j * S + s] = (bd[j] <= 1e32f) ? bi[j] : -1;
-    P.tie_mark[s] = tie ? 1 : 0;
-    if (tie) P.tie_list[atomicAdd(P.knn_ctl, 1)] = (int)s;
+    if (tie) {
+      P.tie_mark[s] = 1;
+      P.tie_list[atomicAdd(P.knn_ctl, 1)] = (int)s;
+    }
   }
 }
 
@@ -2403,47 +2413,50 @@ __global__ __launch_bounds__(NT) void k_loglik(DevPtrs P, KOpts O) {
       const float4 an = anc[i];
       const int nlive = __float_as_int(an.w);
       float tj[KM];
-      int nm = 0;  // candidate slots any lane of the wave uses
+      int nm = 0;          // candidate slots any lane of the wave uses
+      float mx = -INFINITY;  // the largest term
       {
 #pragma clang fp contract(off)
         const float x0 = packed ? vv[u].x : vv[u].x - an.x;
         const float x1 = packed ? vv[u].y : vv[u].y - an.y;
         const float x2 = packed ? vv[u].z : vv[u].z - an.z;
+        // the supervoxel's constants; slots at or past nlive are evaluated
+        // too (within the block's constants) and their terms dropped, so
+        // that the wave runs the slots without branching
+        const float4* c4 = cst + i * (KM * 3);
 #pragma unroll
         for (int j = 0; j < KM; j++) {
           tj[j] = -INFINITY;
           if (!__any(j < nlive)) break;
           nm = j + 1;
-          if (j < nlive) {
-            const float4* c4 = cst + (i * KM + j) * 3;
-            const float4 A = c4[0], Bq = c4[1], C = c4[2];
-            const float y0 = __builtin_fmaf(A.x, x0, __builtin_fmaf(A.y, x1,
-                                                                    __builtin_fmaf(A.z, x2, A.w)));
-            const float y1 = __builtin_fmaf(Bq.x, x1, __builtin_fmaf(Bq.y, x2, Bq.z));
-            const float y2 = __builtin_fmaf(Bq.w, x2, C.x);
-            const float q = __builtin_fmaf(y0, y0, __builtin_fmaf(y1, y1, y2 * y2));
-            // the reference's double term underflows to 0 when its exponential
-            // (2^-q) or the whole term (2^t) is below 2^-1075
-            const float t = C.y - q;
-            tj[j] = (q <= 1075.0f && t >= -1075.0f) ? t : -INFINITY;
-          }
+          const float4 A = c4[3 * j], Bq = c4[3 * j + 1], C = c4[3 * j + 2];
+          const float y0 = __builtin_fmaf(A.x, x0, __builtin_fmaf(A.y, x1,
+                                                                  __builtin_fmaf(A.z, x2, A.w)));
+          const float y1 = __builtin_fmaf(Bq.x, x1, __builtin_fmaf(Bq.y, x2, Bq.z));
+          const float y2 = __builtin_fmaf(Bq.w, x2, C.x);
+          const float q = __builtin_fmaf(y0, y0, __builtin_fmaf(y1, y1, y2 * y2));
+          // the reference's double term underflows to 0 when its exponential
+          // (2^-q) or the whole term (2^t) is below 2^-1075
+          const float t = C.y - q;
+          tj[j] = (j < nlive && q <= 1075.0f && t >= -1075.0f) ? t : -INFINITY;
+          mx = tj[j] > mx ? tj[j] : mx;  // terms are never NaN
         }
       }
+      nm = __builtin_amdgcn_readfirstlane(nm);  // the same in every lane
       double val = 0.0;
       {
 #pragma clang fp contract(off)
-        float mx = -INFINITY;
-#pragma unroll
-        for (int j = 0; j < KM; j++)
-          if (j < nm) mx = fmaxf(mx, tj[j]);
         if (nlive == 1) {
           // a single term t: the value is img * t (k_loglik_voxel does the same)
           if (mx > -INFINITY) val = (double)vv[u].w * (double)mx;
         } else if (mx > -INFINITY) {
+          // slots at or past nlive hold -inf and add exp2(-inf) = +0
           float sum = 0.0f;
 #pragma unroll
-          for (int j = 0; j < KM; j++)
-            if (j < nm && j < nlive) sum += __builtin_amdgcn_exp2f(tj[j] - mx);
+          for (int j = 0; j < KM; j++) {
+            if (j >= nm) break;
+            sum += __builtin_amdgcn_exp2f(tj[j] - mx);
+          }
           val = (double)vv[u].w * (double)(mx + __builtin_amdgcn_logf(sum));
         }
       }

@@ -437,6 +437,7 @@ int batch_upload(tgmm_batch* b, const std::vector<FrameSrc>& fs) {
   HIPCHK(hipMemcpyAsync(P.fixed, fixed.data(), G, hipMemcpyHostToDevice, st));
   HIPCHK(hipMemcpyAsync(P.svkeep, svkeep.data(), svkeep.size() * 4, hipMemcpyHostToDevice, st));
   HIPCHK(hipMemsetAsync(P.knn_ctl, 0, 64 * 4, st));
+  HIPCHK(hipMemsetAsync(P.tie_mark, 0, S, st));
   HIPCHK(hipMemcpyAsync(P.fr, b->fh.data(), b->fh.size() * sizeof(FrameDev), hipMemcpyHostToDevice,
                         st));
   HIPCHK(hipMemsetAsync(P.ind, 0xFF, (size_t)b->ws.K * S * 4, st));

@@ -350,6 +350,7 @@ extern "C" int tgmm_server_create(int device, const tgmm_server_opts* opts, tgmm
       (er = hipMemsetAsync(P.seg_cnt, 0, (G + 1) * 4, s->st)) ||
       (er = hipMemsetAsync(P.cell_cnt, 0, (sh.Ccap + 1) * 4, s->st)) ||
       (er = hipMemsetAsync(P.knn_ctl, 0, 64 * 4, s->st)) ||
+      (er = hipMemsetAsync(P.tie_mark, 0, S, s->st)) ||
       (er = hipMemsetAsync(P.svkeep, 0, (size_t)17 * G * 4, s->st)) ||
       (er = hipMemcpyAsync(P.fr, s->fh.data(), sh.Fcap * sizeof(FrameDev),
                            hipMemcpyHostToDevice, s->st)) ||
