@@ -2265,15 +2265,44 @@ __global__ __launch_bounds__(NT) void k_loglik(DevPtrs P, KOpts O) {
   __shared__ double cls[B * 16];
   __shared__ int vend_s;
   const long long sb = (long long)blockIdx.x * B;
+  const long long S = P.Scap;
+  const int K = O.K;
+  // A lane's loads for its (supervoxel, slot) of a constants pass: they
+  // depend on the block's supervoxels alone, so the first pass's are issued
+  // before the frame's, together, and the candidate's parameters right after
+  // its index (in range whenever it is not -1).
+  struct Fetch {
+    int frame, len, kraw;
+    long long beg;
+    float a0, a1, a2;
+    float4 ball;
+    LLPack lp;
+  };
+  auto fetch = [&](int p) {
+    Fetch r;
+    const long long s = sb + p + threadIdx.x / LPS;
+    const int lane = threadIdx.x % LPS;
+    r.frame = P.sv_frame[s];
+    r.len = P.sv_len[s];
+    r.beg = P.sv_beg[s];
+    r.a0 = P.anchor[s];
+    r.a1 = P.anchor[S + s];
+    r.a2 = P.anchor[2 * S + s];
+    r.ball = P.sv_ball[s];
+    r.kraw = lane < K ? P.ind[(long long)lane * S + s] : -1;
+    if (r.kraw >= 0) r.lp = P.llpack[r.kraw];
+    return r;
+  };
+  const int fr0 = P.sv_frame[sb];
+  const long long beg0 = P.sv_beg[sb];
+  const Fetch first = fetch(0);
   const int f = P.sblk_frame[sb >> 8];
   if (f < 0) return;
   const FrameDev* F = P.fr + f;
   if (!F->active) return;
-  const long long S = P.Scap;
-  const int K = O.K;
   const double lsa2 = F->log_sum_alpha * 1.4426950408889634074;
   // the block's first supervoxel is real unless the whole block is padding
-  const long long vbase = P.sv_frame[sb] == f ? P.sv_beg[sb] : 0;
+  const long long vbase = fr0 == f ? beg0 : 0;
   // the voxels of a chunk, raw (packed: .x .y of vpk; else x, y, z, img
   // bits), loaded within the frame's voxel range; the first chunk's loads
   // are issued before the constants are formed, each next chunk's before
@@ -2306,19 +2335,20 @@ __global__ __launch_bounds__(NT) void k_loglik(DevPtrs P, KOpts O) {
   for (int x = threadIdx.x; x < B * 16; x += NT) cls[x] = 0.0;
   __syncthreads();
   // 1. constants
+#pragma unroll
   for (int p = 0; p < B; p += SPP) {
     const int i = p + threadIdx.x / LPS;
     const int lane = threadIdx.x % LPS;
-    const long long s = sb + i;
-    const bool real = P.sv_frame[s] == f;
-    const int n = real ? P.sv_len[s] : 0;
-    const float a0 = P.anchor[s], a1 = P.anchor[S + s], a2 = P.anchor[2 * S + s];
-    const float4 ball = P.sv_ball[s];
-    const int kk = (n > 0 && lane < K) ? P.ind[(long long)lane * S + s] : -1;
+    const Fetch r = p == 0 ? first : fetch(p);
+    const bool real = r.frame == f;
+    const int n = real ? r.len : 0;
+    const float a0 = r.a0, a1 = r.a1, a2 = r.a2;
+    const float4 ball = r.ball;
+    const int kk = n > 0 ? r.kraw : -1;
     float pe0 = 0.0f, pe1 = 0.0f, pe2 = 0.0f, pu[6] = {0, 0, 0, 0, 0, 0}, pg = -INFINITY;
     double tlo = -INFINITY, thi = INFINITY;  // bounds of the candidate's terms
     if (kk >= 0) {
-      const LLPack lp = P.llpack[kk];
+      const LLPack& lp = r.lp;
       const double d0 = (double)a0 - lp.m[0], d1 = (double)a1 - lp.m[1], d2 = (double)a2 - lp.m[2];
 #pragma unroll
       for (int q = 0; q < 6; q++) pu[q] = lp.U[q];
@@ -2366,8 +2396,8 @@ __global__ __launch_bounds__(NT) void k_loglik(DevPtrs P, KOpts O) {
     }
     if (lane == 0) {
       anc[i] = make_float4(a0, a1, a2, __int_as_float(__popcll(gm)));
-      vst[i] = real ? (int)(P.sv_beg[s] - vbase) : -1;
-      if (real) atomicMax(&vend_s, (int)(P.sv_beg[s] + n - vbase));
+      vst[i] = real ? (int)(r.beg - vbase) : -1;
+      if (real) atomicMax(&vend_s, (int)(r.beg + n - vbase));
     }
   }
   __syncthreads();
