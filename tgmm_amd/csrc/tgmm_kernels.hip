@@ -1711,13 +1711,26 @@ __device__ inline int wave_take(int* cnt, int key) {
 // (the pair counting is wave-wide).
 template <int KM>
 __device__ inline void estep_one(const DevPtrs& P, const KOpts& O, long long s, bool ok) {
-  const int f = ok ? P.sv_frame[s] : 0;
   const long long S = P.Scap;
   const int K = O.K;
+  // The candidates' indices and parameters and the moments depend on s
+  // alone (s < Scap; an index is -1 or in range): they are loaded whether or
+  // not the supervoxel is live, without waiting for the frame table, and the
+  // results of a supervoxel that is not are dropped.
+  int kraw[KM];
+#pragma unroll
+  for (int j = 0; j < KM; j++) kraw[j] = j < K ? P.ind[(long long)j * S + s] : -1;
+  const int f = ok ? P.sv_frame[s] : 0;
   const double psi_sum = ok ? P.fr[f].psi_sum_alpha : 0.0;
   double n = 0.0, c0 = 0.0, c1 = 0.0, c2 = 0.0, Q[6] = {0, 0, 0, 0, 0, 0};
-  if (!ok) {
-  } else if (O.voxel) {  // the voxel itself: count 1, no scatter (the same values as mom)
+  if (!O.voxel) {
+    n = P.mom[s];
+    c0 = P.mom[S + s];
+    c1 = P.mom[2 * S + s];
+    c2 = P.mom[3 * S + s];
+#pragma unroll
+    for (int i = 0; i < 6; i++) Q[i] = P.mom[(4 + i) * S + s];
+  } else if (ok) {  // the voxel itself: count 1, no scatter (the same values as mom)
     const long long v = P.sv_beg[s];
     n = 1.0;
     c0 = (double)P.vx[v];
@@ -1725,13 +1738,6 @@ __device__ inline void estep_one(const DevPtrs& P, const KOpts& O, long long s, 
     c2 = (double)P.vz[v];
 #pragma unroll
     for (int i = 0; i < 6; i++) Q[i] = 0.0;
-  } else {
-    n = P.mom[s];
-    c0 = P.mom[S + s];
-    c1 = P.mom[2 * S + s];
-    c2 = P.mom[3 * S + s];
-#pragma unroll
-    for (int i = 0; i < 6; i++) Q[i] = P.mom[(4 + i) * S + s];
   }
   double L[KM];
   int kk[KM];
@@ -1740,9 +1746,9 @@ __device__ inline void estep_one(const DevPtrs& P, const KOpts& O, long long s, 
   for (int j = 0; j < KM; j++) {
     L[j] = -INFINITY;
     kk[j] = -1;
-    if (ok && j < K) {
-      const int k = P.ind[(long long)j * S + s];
-      kk[j] = k;
+    if (j < K) {
+      const int k = kraw[j];
+      kk[j] = ok ? k : -1;
       if (k >= 0) {
         const EPack ep = P.epack[k];
         const double d0 = c0 - ep.m[0], d1 = c1 - ep.m[1], d2 = c2 - ep.m[2];
