@@ -120,7 +120,9 @@ struct DevPtrs {
   int* sv_len;
   float* sv_cent;   // 3*Scap SoA
   double* mom;      // 10*Scap: n, c[3], Q[6]  (unweighted, FP64)
-  double* wmom;     // 10*Scap: W, mu[3], C[6] (intensity-weighted, FP64)
+  double* wmom;     // 10*Scap: W, mu[3], C[6] (intensity-weighted, FP64), the 10 values
+                    // of a supervoxel contiguous (wmom[10 s + i]: one M-step pair reads
+                    // 80 bytes in a row)
   float* anchor;    // 3*Scap: first voxel of the supervoxel
   float4* sv_ball;  // Scap: ball around the supervoxel's voxels: centre
                     // relative to the anchor, radius (rounded up)
@@ -157,9 +159,6 @@ struct DevPtrs {
   // kNN: supervoxels whose K-th distance is tied with a Gaussian outside
   // their K best (their candidate set follows the reference's scan order)
   int* tie_list;    // Scap
-  unsigned char* tie_mark;  // Scap: 1 for the supervoxels in the tie list (set by the kNN
-                            // search, cleared when the next iteration empties the list;
-                            // zero whenever knn_ctl[0] is)
   int* knn_ctl;     // [0] ties listed this iteration, [1] supervoxels whose
                     //     final slot layout found a different set (stays 0),
                     //     [2] Gaussians in long_list this iteration
@@ -191,19 +190,10 @@ hipError_t launch_prep(const DevPtrs& P, const KOpts& O, hipStream_t st);
 // iteration state, initial sum of alpha and first E-step constants per frame;
 // the frames' kNN grid counts must be zero
 hipError_t launch_reset(const DevPtrs& P, const KOpts& O, hipStream_t st);
-// A second stream, and the events that fork it from and join it back to
-// the iteration's stream (recorded into a captured graph as two branches).
-struct SideStream {
-  hipStream_t st = nullptr;
-  hipEvent_t fork = nullptr, join = nullptr;
-};
-// one EM iteration of every active frame; optional events around one kernel.
-// With a side stream (and no timed kernel), the replays of the supervoxels
-// whose kNN search listed a tie run on it beside the other supervoxels'
-// E-step; the results are the same either way.
+// one EM iteration of every active frame; optional events around one kernel
 hipError_t launch_iteration(const DevPtrs& P, const KOpts& O, hipStream_t st,
                             hipEvent_t ev_before = nullptr, hipEvent_t ev_after = nullptr,
-                            int timed_kernel = -1, const SideStream* side = nullptr);
+                            int timed_kernel = -1);
 int kmax_for(int K);
 
 // Priors of a stream's next frame from its previous posterior, in place on

@@ -149,7 +149,7 @@ __global__ __launch_bounds__(256) void k_moments(DevPtrs P) {
   if (n <= 0) {
     if (lane < 10) {
       P.mom[lane * S + s] = 0.0;
-      P.wmom[lane * S + s] = 0.0;
+      P.wmom[s * 10 + lane] = 0.0;
     }
     if (lane < 3) P.anchor[lane * S + s] = 0.0f;
     if (lane == 0) P.sv_ball[s] = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -229,11 +229,12 @@ __global__ __launch_bounds__(256) void k_moments(DevPtrs P) {
     P.mom[2 * S + s] = (double)a1 + cy;
     P.mom[3 * S + s] = (double)a2 + cz;
     for (int i = 0; i < 6; i++) P.mom[(4 + i) * S + s] = q[i];
-    P.wmom[0 * S + s] = sw;
-    P.wmom[1 * S + s] = (double)a0 + ux;
-    P.wmom[2 * S + s] = (double)a1 + uy;
-    P.wmom[3 * S + s] = (double)a2 + uz;
-    for (int i = 0; i < 6; i++) P.wmom[(4 + i) * S + s] = c[i];
+    double* wm = P.wmom + s * 10;
+    wm[0] = sw;
+    wm[1] = (double)a0 + ux;
+    wm[2] = (double)a1 + uy;
+    wm[3] = (double)a2 + uz;
+    for (int i = 0; i < 6; i++) wm[4 + i] = c[i];
     P.anchor[0 * S + s] = a0;
     P.anchor[1 * S + s] = a1;
     P.anchor[2 * S + s] = a2;
@@ -265,17 +266,17 @@ __global__ __launch_bounds__(256) void k_moments_voxel(DevPtrs P) {
   const long long S = P.Scap, v = P.sv_beg[s];
   const float x[3] = {P.vx[v], P.vy[v], P.vz[v]};
   P.mom[s] = 1.0;
-  P.wmom[s] = 1.0;
+  P.wmom[s * 10] = 1.0;
 #pragma unroll
   for (int a = 0; a < 3; a++) {
     P.mom[(1 + a) * S + s] = (double)x[a];
-    P.wmom[(1 + a) * S + s] = (double)x[a];
+    P.wmom[s * 10 + 1 + a] = (double)x[a];
     P.anchor[a * S + s] = x[a];
   }
 #pragma unroll
   for (int i = 4; i < 10; i++) {
     P.mom[i * S + s] = 0.0;
-    P.wmom[i * S + s] = 0.0;
+    P.wmom[s * 10 + i] = 0.0;
   }
   P.sv_ball[s] = make_float4(0.f, 0.f, 0.f, 0.f);
   // the voxel is its own anchor: offsets 0, unless a coordinate is not finite
@@ -466,16 +467,10 @@ __device__ inline int block_exclusive_scan1024(int v, int* lds) {
 __global__ __launch_bounds__(1024) void k_scan_frames(DevPtrs P, int which, int need_active,
                                                       int K) {
   __shared__ int lds[32];
-  // the grid scan opens an iteration: the kNN's tie list starts empty, the
-  // marks of the last iteration's list cleared
-  if (which == 0 && blockIdx.x == 0) {
-    const int cnt = P.knn_ctl[0];
-    for (int t = threadIdx.x; t < cnt; t += blockDim.x) P.tie_mark[P.tie_list[t]] = 0;
-    __syncthreads();  // every thread has read the count
-    if (threadIdx.x == 0) {
-      P.knn_ctl[0] = 0;
-      P.knn_ctl[2] = 0;  // and the list of long pair lists
-    }
+  // the grid scan opens an iteration: the kNN's tie list starts empty
+  if (which == 0 && blockIdx.x == 0 && threadIdx.x == 0) {
+    P.knn_ctl[0] = 0;
+    P.knn_ctl[2] = 0;  // and the list of long pair lists
   }
   const int f = (P.only_frame >= 0) ? P.only_frame : (int)blockIdx.x;
   if (f >= P.Fcap || (P.only_frame < 0 && f >= P.nframes)) return;
@@ -812,9 +807,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(KM <= 8 ? 4
   const float s0 = O.scale[0], s1 = O.scale[1], s2 = O.scale[2];
   const float q0 = qq.x, q1 = qq.y, q2 = qq.z;
   const int nx = F->dims[0], ny = F->dims[1], nz = F->dims[2];
-  const double h = F->h;
   const double u[3] = {(double)q0 * s0, (double)q1 * s1, (double)q2 * s2};
-  const double org[3] = {F->org[0], F->org[1], F->org[2]};
   const int dims[3] = {nx, ny, nz};
   int c[3];
   for (int a = 0; a < 3; a++) c[a] = axis_cell(u[a], F->org[a], F->h, dims[a]);
@@ -879,7 +872,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(KM <= 8 ? 4
   // the shell bound; 0 inside. A mean binned into a cell (an edge cell holds
   // the means beyond the grid on its outer side) is at least that far.
   auto box_d2 = [&](int a, int ci) -> double {
-    const double lo = org[a] + (double)ci * h, hi = org[a] + (double)(ci + 1) * h;
+    const double h = F->h, o = F->org[a];  // the frame's grid (scalar loads)
+    const double lo = o + (double)ci * h, hi = o + (double)(ci + 1) * h;
     const double d = fmax(fmax(lo - u[a], u[a] - hi), 0.0) * (1.0 - 1e-5);
     return d * d;
   };
@@ -948,10 +942,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(KM <= 8 ? 4
 #pragma unroll
     for (int j = 0; j < KM; j++)
       if (j < K) P.ind[(long long)j * S + s] = (bd[j] <= 1e32f) ? bi[j] : -1;
-    if (tie) {
-      P.tie_mark[s] = 1;
-      P.tie_list[atomicAdd(P.knn_ctl, 1)] = (int)s;
-    }
+    if (tie) P.tie_list[atomicAdd(P.knn_ctl, 1)] = (int)s;
   }
 }
 
@@ -1077,10 +1068,7 @@ __global__ __launch_bounds__(64 * NW) void k_knn_cell(DevPtrs P, KOpts O) {
 #pragma unroll
     for (int j = 0; j < KM; j++)
       if (j < K) P.ind[(long long)j * S + s] = (bd[j] <= 1e32f) ? bi[j] : -1;
-    if (tie) {
-      P.tie_mark[s] = 1;
-      P.tie_list[atomicAdd(P.knn_ctl, 1)] = (int)s;
-    }
+    if (tie) P.tie_list[atomicAdd(P.knn_ctl, 1)] = (int)s;
   }
 }
 
@@ -1787,29 +1775,14 @@ __device__ inline void estep_one(const DevPtrs& P, const KOpts& O, long long s, 
   }
 }
 
-// The E-step of every supervoxel of the active frames; skip_ties: not of
-// those whose kNN search listed a tie this iteration (their candidates are
-// being replayed meanwhile, and k_estep_ties takes them).
+// The E-step of every supervoxel of the active frames.
 template <int KM>
-__global__ __launch_bounds__(256) void k_estep(DevPtrs P, KOpts O, int skip_ties) {
+__global__ __launch_bounds__(256) void k_estep(DevPtrs P, KOpts O) {
   const long long s = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   const int f = P.sblk_frame[s >> 8];
   // no early exit: every lane takes part in the wave-wide pair counting
-  const bool ok = f >= 0 && P.fr[f].active && P.sv_frame[s] == f &&
-                  !(skip_ties && P.tie_mark[s] != 0);
+  const bool ok = f >= 0 && P.fr[f].active && P.sv_frame[s] == f;
   estep_one<KM>(P, O, s, ok);
-}
-
-// The E-step of the supervoxels listed with a tie this iteration, after
-// their replay (k_knn_exact).
-template <int KM>
-__global__ __launch_bounds__(256) void k_estep_ties(DevPtrs P, KOpts O) {
-  const int cnt = P.knn_ctl[0];
-  for (int tb = blockIdx.x * blockDim.x; tb < cnt; tb += gridDim.x * blockDim.x) {
-    const int t = tb + (int)threadIdx.x;
-    const bool ok = t < cnt;
-    estep_one<KM>(P, O, ok ? P.tie_list[t] : 0, ok);
-  }
 }
 
 __global__ __launch_bounds__(256) void k_pair_scatter(DevPtrs P, KOpts O) {
@@ -2044,22 +2017,24 @@ __global__ __launch_bounds__(64 * ML) void k_mstep(DevPtrs P, KOpts O) {
           const long long s = code >> kPairShift;
           const int j = code & ((1 << kPairShift) - 1);
           const double r = (double)P.rnk[(long long)j * S + s];
-          const double Ws = P.wmom[s];
+          // the supervoxel's 10 weighted moments, contiguous (80 bytes)
+          const double2* wm = reinterpret_cast<const double2*>(P.wmom + s * 10);
+          const double2 m01 = wm[0], m23 = wm[1], m45 = wm[2], m67 = wm[3], m89 = wm[4];
+          const double Ws = m01.x;
           if (!(Ws != 0.0)) continue;
           const double w = r * Ws;
-          const double d0 = P.wmom[S + s] - r0, d1 = P.wmom[2 * S + s] - r1,
-                       d2 = P.wmom[3 * S + s] - r2;
+          const double d0 = m01.y - r0, d1 = m23.x - r1, d2 = m23.y - r2;
           double* c = a[q];
           c[0] += w;
           c[1] += w * d0;
           c[2] += w * d1;
           c[3] += w * d2;
-          c[4] += r * P.wmom[4 * S + s] + w * d0 * d0;
-          c[5] += r * P.wmom[5 * S + s] + w * d0 * d1;
-          c[6] += r * P.wmom[6 * S + s] + w * d0 * d2;
-          c[7] += r * P.wmom[7 * S + s] + w * d1 * d1;
-          c[8] += r * P.wmom[8 * S + s] + w * d1 * d2;
-          c[9] += r * P.wmom[9 * S + s] + w * d2 * d2;
+          c[4] += r * m45.x + w * d0 * d0;
+          c[5] += r * m45.y + w * d0 * d1;
+          c[6] += r * m67.x + w * d0 * d2;
+          c[7] += r * m67.y + w * d1 * d1;
+          c[8] += r * m89.x + w * d1 * d2;
+          c[9] += r * m89.y + w * d2 * d2;
         }
       }
     }
@@ -2828,7 +2803,7 @@ hipError_t launch_finalize(const DevPtrs& P, const KOpts& O, int cap, int only_f
 
 template <int KM>
 static hipError_t iteration_t(const DevPtrs& P, const KOpts& O, hipStream_t st, hipEvent_t e0,
-                              hipEvent_t e1, int tk, const SideStream* side) {
+                              hipEvent_t e1, int tk) {
   const unsigned gb = nb(P.Gext, kGB);
   const int bs = O.tiles.sv_block;
   k_scan_frames<<<P.nframes, 1024, 0, st>>>(P, 0, 1, O.K);
@@ -2836,23 +2811,13 @@ static hipError_t iteration_t(const DevPtrs& P, const KOpts& O, hipStream_t st, 
   if (tk == TK_KNN && e0) (void)hipEventRecord(e0, st);
   launch_knn<KM>(P, O, st);
   if (tk == TK_KNN && e1) (void)hipEventRecord(e1, st);
-  const int words = (int)std::min<long long>(kLayoutWords, (P.Gext + 63) / 64);
-  if (side && side->st && !e0) {
-    // the replays of the tied supervoxels and their E-step on the side
-    // stream, beside the E-step of every other supervoxel
-    (void)hipEventRecord(side->fork, st);
-    (void)hipStreamWaitEvent(side->st, side->fork, 0);
-    k_knn_exact<KM><<<128, 64, scan_lds_bytes(words), side->st>>>(P, O, words);
-    k_estep_ties<KM><<<4, 256, 0, side->st>>>(P, O);
-    (void)hipEventRecord(side->join, side->st);
-    k_estep<KM><<<nb(P.Sext, bs), bs, 0, st>>>(P, O, 1);
-    (void)hipStreamWaitEvent(st, side->join, 0);
-  } else {
+  {
+    const int words = (int)std::min<long long>(kLayoutWords, (P.Gext + 63) / 64);
     k_knn_exact<KM><<<128, 64, scan_lds_bytes(words), st>>>(P, O, words);
-    if (tk == TK_ESTEP && e0) (void)hipEventRecord(e0, st);
-    k_estep<KM><<<nb(P.Sext, bs), bs, 0, st>>>(P, O, 0);
-    if (tk == TK_ESTEP && e1) (void)hipEventRecord(e1, st);
   }
+  if (tk == TK_ESTEP && e0) (void)hipEventRecord(e0, st);
+  k_estep<KM><<<nb(P.Sext, bs), bs, 0, st>>>(P, O);
+  if (tk == TK_ESTEP && e1) (void)hipEventRecord(e1, st);
   k_scan_frames<<<P.nframes, 1024, 0, st>>>(P, 1, 1, O.K);
   k_pair_scatter<<<nb(P.Sext, bs), bs, 0, st>>>(P, O);
   k_seg_sort<<<nb(P.Gext, 16) + kSortLongBlocks, 256, 0, st>>>(P, (int)nb(P.Gext, 16));
@@ -2867,15 +2832,15 @@ static hipError_t iteration_t(const DevPtrs& P, const KOpts& O, hipStream_t st, 
 }
 
 hipError_t launch_iteration(const DevPtrs& P, const KOpts& O, hipStream_t st, hipEvent_t e0,
-                            hipEvent_t e1, int tk, const SideStream* side) {
+                            hipEvent_t e1, int tk) {
   if (!tiles_valid(O.tiles) || (O.tiles.ll_gw == 1 && !O.voxel)) return hipErrorInvalidValue;
   switch (kmax_for(O.K)) {
     case 5:
-      return iteration_t<5>(P, O, st, e0, e1, tk, side);
+      return iteration_t<5>(P, O, st, e0, e1, tk);
     case 8:
-      return iteration_t<8>(P, O, st, e0, e1, tk, side);
+      return iteration_t<8>(P, O, st, e0, e1, tk);
     default:
-      return iteration_t<16>(P, O, st, e0, e1, tk, side);
+      return iteration_t<16>(P, O, st, e0, e1, tk);
   }
 }
 

@@ -119,7 +119,6 @@ int ws_alloc(Workspace* w, const WsShape& sh, int K) {
   AL(P.cg4, G);
   AL(P.mknn, G);
   AL(P.tie_list, S);
-  AL(P.tie_mark, S);
   AL(P.knn_ctl, 64);
   AL(P.long_list, G);
   AL(P.win, S);
@@ -143,7 +142,7 @@ void ws_free(Workspace* w) {
 
 long long ws_bytes_estimate(const WsShape& sh, int K) {
   const long long S = sh.Scap, G = sh.Gcap, C = sh.Ccap, V = sh.Vcap;
-  return 24 * V + S * (16 + 8 + 4 + 12 + 80 + 80 + 12 + 16 + 16LL * K + 1 + 4 + 8 + 1) +
+  return 24 * V + S * (16 + 8 + 4 + 12 + 80 + 80 + 12 + 16 + 16LL * K + 1 + 4 + 8) +
          G * (4 + 8 * GS_NFIELDS * 2 + 1 + 96 + 8 + 64 + 12 + 4 + 8 + 12 + 4 + 1 + 16 + 288 +
               68 + 4) +
          8 * C + 64 * sh.Fcap;
@@ -437,7 +436,6 @@ int batch_upload(tgmm_batch* b, const std::vector<FrameSrc>& fs) {
   HIPCHK(hipMemcpyAsync(P.fixed, fixed.data(), G, hipMemcpyHostToDevice, st));
   HIPCHK(hipMemcpyAsync(P.svkeep, svkeep.data(), svkeep.size() * 4, hipMemcpyHostToDevice, st));
   HIPCHK(hipMemsetAsync(P.knn_ctl, 0, 64 * 4, st));
-  HIPCHK(hipMemsetAsync(P.tie_mark, 0, S, st));
   HIPCHK(hipMemcpyAsync(P.fr, b->fh.data(), b->fh.size() * sizeof(FrameDev), hipMemcpyHostToDevice,
                         st));
   HIPCHK(hipMemsetAsync(P.ind, 0xFF, (size_t)b->ws.K * S * 4, st));
@@ -524,9 +522,8 @@ int batch_capture(tgmm_batch* b) {
   if (b->gexec) return TGMM_OK;
   std::memcpy(&b->cap_P, &b->ws.P, sizeof(DevPtrs));  // padding included, as compared
   std::memcpy(&b->cap_ko, &b->ko, sizeof(KOpts));
-  if (int rc = side_stream_create(&b->side)) return rc;
   HIPCHK(hipStreamBeginCapture(b->st, hipStreamCaptureModeThreadLocal));
-  hipError_t le = launch_iteration(b->ws.P, b->ko, b->st, nullptr, nullptr, -1, &b->side);
+  hipError_t le = launch_iteration(b->ws.P, b->ko, b->st);
   hipGraph_t g = nullptr;
   hipError_t ee = hipStreamEndCapture(b->st, &g);
   if (le != hipSuccess || ee != hipSuccess) {
@@ -537,28 +534,6 @@ int batch_capture(tgmm_batch* b) {
   b->graph = g;
   HIPCHK(hipGraphInstantiate(&b->gexec, g, nullptr, nullptr, 0));
   return TGMM_OK;
-}
-
-int side_stream_create(SideStream* side) {
-  if (side->st) return TGMM_OK;
-  if (hipStreamCreateWithFlags(&side->st, hipStreamNonBlocking) != hipSuccess ||
-      hipEventCreateWithFlags(&side->fork, hipEventDisableTiming) != hipSuccess ||
-      hipEventCreateWithFlags(&side->join, hipEventDisableTiming) != hipSuccess) {
-    side_stream_destroy(side);
-    tgmm_set_error("side stream creation failed");
-    return TGMM_ERR_DEVICE;
-  }
-  return TGMM_OK;
-}
-
-void side_stream_destroy(SideStream* side) {
-  if (side->st) {
-    (void)hipStreamSynchronize(side->st);
-    (void)hipStreamDestroy(side->st);
-  }
-  if (side->fork) (void)hipEventDestroy(side->fork);
-  if (side->join) (void)hipEventDestroy(side->join);
-  *side = SideStream();
 }
 
 extern "C" int tgmm_batch_solve(tgmm_batch* b, int mode) {
@@ -573,8 +548,7 @@ extern "C" int tgmm_batch_solve(tgmm_batch* b, int mode) {
     if (mode == TGMM_SOLVE_GRAPH) {
       HIPCHK(hipGraphLaunch(b->gexec, b->st));
     } else {
-      if (int rc = side_stream_create(&b->side)) return rc;
-      HIPCHK(launch_iteration(b->ws.P, b->ko, b->st, nullptr, nullptr, -1, &b->side));
+      HIPCHK(launch_iteration(b->ws.P, b->ko, b->st));
     }
     const bool check = (mode == TGMM_SOLVE_HOSTSYNC) || (poll && (it % 4 == 3));
     if (check && it + 1 < iters) {
@@ -906,7 +880,6 @@ extern "C" int tgmm_batch_destroy(tgmm_batch* b) {
   if (b->graph) (void)hipGraphDestroy(b->graph);
   if (b->st) (void)hipStreamSynchronize(b->st);
   ws_free(&b->ws);
-  side_stream_destroy(&b->side);
   if (b->own_stream && b->st) (void)hipStreamDestroy(b->st);
   delete b;
   return TGMM_OK;

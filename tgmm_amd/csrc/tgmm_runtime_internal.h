@@ -42,7 +42,6 @@ struct tgmm_batch {
   int device = 0;
   hipStream_t st = nullptr;
   bool own_stream = false;
-  tgmm::SideStream side;  // created with the first solve
   tgmm_em_opts opts;
   tgmm::KOpts ko;
   int nframes = 0;
@@ -75,7 +74,4 @@ int batch_build(int device, const std::vector<FrameSrc>& fs, const tgmm_em_opts&
                 tgmm_batch* reuse = nullptr, double headroom = 1.0);
 int batch_upload(tgmm_batch* b, const std::vector<FrameSrc>& fs);
 int batch_capture(tgmm_batch* b);
-// The side stream of the iteration (created once per batch).
-int side_stream_create(tgmm::SideStream* side);
-void side_stream_destroy(tgmm::SideStream* side);
 

@@ -190,7 +190,6 @@ struct Stream {
 struct tgmm_server {
   int device = 0;
   hipStream_t st = nullptr;   // compute
-  SideStream side;            // the iteration's side stream
   // uploads: a frame's voxel arrays go out on several copy streams at once
   // (each stream's copies run on one DMA engine, which alone does not fill
   // the host link); the first stream joins the others before the frame's
@@ -236,9 +235,8 @@ struct tgmm_server {
 
 static int server_capture(tgmm_server* s) {
   if (s->gexec) return TGMM_OK;
-  if (int rc = side_stream_create(&s->side)) return rc;
   HIPCHK(hipStreamBeginCapture(s->st, hipStreamCaptureModeThreadLocal));
-  hipError_t le = launch_iteration(s->ws.P, s->ko, s->st, nullptr, nullptr, -1, &s->side);
+  hipError_t le = launch_iteration(s->ws.P, s->ko, s->st);
   hipGraph_t g = nullptr;
   hipError_t ee = hipStreamEndCapture(s->st, &g);
   if (le != hipSuccess || ee != hipSuccess) {
@@ -350,7 +348,6 @@ extern "C" int tgmm_server_create(int device, const tgmm_server_opts* opts, tgmm
       (er = hipMemsetAsync(P.seg_cnt, 0, (G + 1) * 4, s->st)) ||
       (er = hipMemsetAsync(P.cell_cnt, 0, (sh.Ccap + 1) * 4, s->st)) ||
       (er = hipMemsetAsync(P.knn_ctl, 0, 64 * 4, s->st)) ||
-      (er = hipMemsetAsync(P.tie_mark, 0, S, s->st)) ||
       (er = hipMemsetAsync(P.svkeep, 0, (size_t)17 * G * 4, s->st)) ||
       (er = hipMemcpyAsync(P.fr, s->fh.data(), sh.Fcap * sizeof(FrameDev),
                            hipMemcpyHostToDevice, s->st)) ||
@@ -1056,7 +1053,6 @@ extern "C" int tgmm_server_destroy(tgmm_server* s) {
   if (s->ev_fh) (void)hipEventDestroy(s->ev_fh);
   if (s->fh_pin) (void)hipHostFree(s->fh_pin);
   ws_free(&s->ws);
-  side_stream_destroy(&s->side);
   if (s->st) (void)hipStreamDestroy(s->st);
   for (int i = 0; i < tgmm_server::kUp; i++) {
     if (s->cst[i]) (void)hipStreamDestroy(s->cst[i]);
