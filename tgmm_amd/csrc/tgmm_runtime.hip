@@ -646,7 +646,10 @@ static double kernel_bytes(tgmm_batch* b, int tk) {
         t += S * (4 + 80 + 4.0 * K + 4.0 * K) + G * (8 * 11);
         break;
       case TK_KNN:
-        t += S * (4 + 12 + 4.0 * K) + G * 16;
+        // per query its centroid and index (16 B), the previous candidates
+        // (the bound) and the new ones (4 B each); per Gaussian its binned
+        // reference and its mean for the bound (16 B each)
+        t += S * (16 + 8.0 * K) + G * 32;
         break;
       case TK_MSTEP:
         t += G * (27 * 8 * 2 + 8) + S * (4 + 4 + 80);
