@@ -691,15 +691,19 @@ __device__ inline void knn_merge(float (&bd)[KM], int (&bi)[KM], int K, bool& ti
 
 // Shells r >= 2 of the exact search around query cell c (scaled query u),
 // until the K-th best is below the open-face bound of the visited block.
+// consider(ref) may defer a reference; flush() (called by every lane of the
+// wave after each shell) completes the deferred ones before the K-th best is
+// read.
 // Cells are pruned by the current K-th best: a row whose y/z boxes, or a cell
 // whose box, lies farther than it (with the bound's relative margin) holds no
 // reference that can enter the K best. QL lanes share a query (lane `sub`
 // takes every QL-th reference of a range); their lists are merged after each
 // shell.
-template <int KM, int QL, typename Consider>
+template <int KM, int QL, typename Consider, typename Flush>
 __device__ inline void knn_shells(const DevPtrs& P, const FrameDev* F, const int* c,
                                   const double* u, int K, int sub, float (&bd)[KM],
-                                  int (&bi)[KM], bool& tie, float& kd, Consider consider) {
+                                  int (&bi)[KM], bool& tie, float& kd, Consider consider,
+                                  Flush flush) {
   const int nx = F->dims[0], ny = F->dims[1], nz = F->dims[2];
   const int dims[3] = {nx, ny, nz};
   const double h = F->h;
@@ -768,6 +772,7 @@ __device__ inline void knn_shells(const DevPtrs& P, const FrameDev* F, const int
           }
         }
       }
+      flush();  // every lane of the wave takes part
       if (QL > 1) knn_merge<KM, QL>(bd, bi, K, tie, kd);
     }
     const double lb2 = lower_bound_sq(r);
@@ -821,10 +826,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(KM <= 8 ? 4
   const long long c0 = F->c0;
   bool tie = false;
   float kd = INFINITY;
-  auto consider = [&](const float4 cur) {
-    const float d = knn_dist(q0, q1, q2, cur.x, cur.y, cur.z, s0, s1, s2);
-    knn_try<KM>(bd, bi, d, __float_as_int(cur.w), K, tie, kd);
-  };
   // An upper bound of the K-th distance: the previous iteration's candidates
   // of this supervoxel, if all K are still living means of this frame (a
   // mean's distance is a real candidate distance; dead means are NaN).
@@ -906,36 +907,48 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(KM <= 8 ? 4
       pb[t] = b0;
       cum[t + 1] = cum[t] + (e0 - b0);
     }
-    // the 9 trimmed rows as one range of cum[9] references: KC per lane in
-    // flight across row boundaries, so a lane waits for memory once per KC
-    // references rather than once per row (the K best do not depend on the
-    // order references are considered in)
-    const int tot = cum[9];
-    int off[9];  // reference position of flat index p in row t: off[t] + p
+    // each z slab's 3 trimmed rows as one range of references: KC per lane
+    // in flight across row boundaries, so a lane waits for memory once per
+    // KC references rather than once per row, and finds a reference's row
+    // among 3 (the K best do not depend on the order references are
+    // considered in)
 #pragma unroll
-    for (int t = 0; t < 9; t++) off[t] = pb[t] - cum[t];
-    for (int p0 = sub; __any(p0 < tot); p0 += KC * QL) {
-      float4 v[KC];
+    for (int zs = 0; zs < 3; zs++) {
+      const int t0 = 3 * zs;
+      const int c1 = cum[t0 + 1] - cum[t0], c2 = cum[t0 + 2] - cum[t0];
+      const int tot = cum[t0 + 3] - cum[t0];
+      const int o0 = pb[t0], o1 = pb[t0 + 1] - c1, o2 = pb[t0 + 2] - c2;
+      for (int p0 = sub; __any(p0 < tot); p0 += KC * QL) {
+        float4 v[KC];
 #pragma unroll
-      for (int w = 0; w < KC; w++) {
-        const int p = p0 + w * QL;
-        v[w] = make_float4(0.f, 0.f, 0.f, 0.f);
-        if (p < tot) {
-          int o = off[0];  // row of p: the last t with cum[t] <= p
-#pragma unroll
-          for (int t = 1; t < 9; t++) o = p >= cum[t] ? off[t] : o;
-          v[w] = P.cg4[o + p];
+        for (int w = 0; w < KC; w++) {
+          const int p = p0 + w * QL;
+          v[w] = make_float4(0.f, 0.f, 0.f, 0.f);
+          if (p < tot) v[w] = P.cg4[(p >= c2 ? o2 : (p >= c1 ? o1 : o0)) + p];
         }
-      }
 #pragma unroll
-      for (int w = 0; w < KC; w++)
-        if (p0 + w * QL < tot) enqueue(v[w]);
-      if (__any(nq > kKnnQueue - KC)) flush();
+        for (int w = 0; w < KC; w++)
+          if (p0 + w * QL < tot) enqueue(v[w]);
+        if (__any(nq > kKnnQueue - KC)) flush();
+      }
     }
     flush();
   }
   if (QL > 1) knn_merge<KM, QL>(bd, bi, K, tie, kd);
-  knn_shells<KM, QL>(P, F, c, u, K, sub, bd, bi, tie, kd, consider);
+  // the further shells queue as well; a full queue inserts directly (the
+  // shells' loops differ between lanes, so the wave flushes only after each)
+  auto defer = [&](const float4 cur) {
+    const float d = knn_dist(q0, q1, q2, cur.x, cur.y, cur.z, s0, s1, s2);
+    if (d <= qthr) {
+      if (nq < kKnnQueue) {
+        kq[nq * blockDim.x + threadIdx.x] = make_float2(d, cur.w);
+        nq++;
+      } else {
+        knn_try<KM>(bd, bi, d, __float_as_int(cur.w), K, tie, kd);
+      }
+    }
+  };
+  knn_shells<KM, QL>(P, F, c, u, K, sub, bd, bi, tie, kd, defer, flush);
 #pragma unroll
   for (int m = QL / 2; m >= 1; m >>= 1) tie |= __shfl_xor((int)tie, m, QL) != 0;
   if (sub == 0) {
@@ -1064,7 +1077,7 @@ __global__ __launch_bounds__(64 * NW) void k_knn_cell(DevPtrs P, KOpts O) {
     }
     if (!act) continue;
     const double u[3] = {(double)q0 * s0, (double)q1 * s1, (double)q2 * s2};
-    knn_shells<KM, 1>(P, F, c, u, K, 0, bd, bi, tie, kd, consider);
+    knn_shells<KM, 1>(P, F, c, u, K, 0, bd, bi, tie, kd, consider, [] {});
 #pragma unroll
     for (int j = 0; j < KM; j++)
       if (j < K) P.ind[(long long)j * S + s] = (bd[j] <= 1e32f) ? bi[j] : -1;
