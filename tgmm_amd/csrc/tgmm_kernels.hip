@@ -2383,10 +2383,12 @@ __global__ __launch_bounds__(NT) void k_loglik(DevPtrs P, KOpts O) {
         (__ballot(live) >> ((threadIdx.x & 63) & ~(LPS - 1))) & ((1ull << LPS) - 1ull);
     const int slot = __popcll(gm & ((1ull << lane) - 1ull));
     if (live) {
+      // .z of the third: the largest q whose term the reference does not
+      // flush to zero (2^-q and 2^(pg - q) both at least 2^-1075)
       float4* d = cst + (i * KM + slot) * 3;
       d[0] = make_float4(pu[0], pu[1], pu[2], pe0);
       d[1] = make_float4(pu[3], pu[4], pe1, pu[5]);
-      d[2] = make_float4(pe2, pg, 0.0f, 0.0f);
+      d[2] = make_float4(pe2, pg, fminf(1075.0f, pg + 1075.0f), 0.0f);
     }
     if (lane == 0) {
       anc[i] = make_float4(a0, a1, a2, __int_as_float(__popcll(gm)));
@@ -2460,10 +2462,10 @@ __global__ __launch_bounds__(NT) void k_loglik(DevPtrs P, KOpts O) {
           const float y2 = __builtin_fmaf(Bq.w, x2, C.x);
           const float q = __builtin_fmaf(y0, y0, __builtin_fmaf(y1, y1, y2 * y2));
           // the reference's double term underflows to 0 when its exponential
-          // (2^-q) or the whole term (2^t) is below 2^-1075
+          // (2^-q) or the whole term (2^(pg - q)) is below 2^-1075
           const float t = C.y - q;
-          tj[j] = (j < nlive && q <= 1075.0f && t >= -1075.0f) ? t : -INFINITY;
-          mx = tj[j] > mx ? tj[j] : mx;  // terms are never NaN
+          tj[j] = ((j < nlive) & (q <= C.z)) ? t : -INFINITY;
+          mx = __builtin_amdgcn_fmed3f(tj[j], mx, INFINITY);  // max; terms are never NaN
         }
       }
       nm = __builtin_amdgcn_readfirstlane(nm);  // the same in every lane
