@@ -2626,16 +2626,11 @@ __global__ __launch_bounds__(256) void k_ll_groups(DevPtrs P) {
   P.part_sg[g] = pairwise_tree<BPG>(q);
 }
 
-// Frame totals of the log-likelihood, one wave per frame: frame_sum_w64 over
-// the frame's group sums (the block partials themselves when BPG = 1); the
-// convergence test (GMEM.cu:4145-4148) and the iteration count.
-template <int BPG>
-__global__ __launch_bounds__(64) void k_ll_final(DevPtrs P) {
-  const int f = blockIdx.x;
-  FrameDev* F = P.fr + f;
-  if (!F->active) return;
-  const long long g0 = F->s0 >> 8;
-  const double ll = frame_sum_w64((BPG == 1 ? P.part_s : P.part_sg) + g0, F->nbl);
+// The frame's total (frame_sum_w64 over its group sums, in lanes 0..63 of
+// the calling block), the convergence test (GMEM.cu:4145-4148) and the
+// iteration count.
+__device__ inline void ll_frame_total(FrameDev* F, const double* groups) {
+  const double ll = frame_sum_w64(groups + (F->s0 >> 8), F->nbl);
   if (threadIdx.x == 0) {
     F->ll3 = F->ll2;
     F->ll2 = F->ll1;
@@ -2644,6 +2639,34 @@ __global__ __launch_bounds__(64) void k_ll_final(DevPtrs P) {
     F->iter += 1;
     F->active = (keep_going(F->ll, F->ll1, F->ll2, F->ll3, F->tol) && F->iter <= F->max_iter) ? 1 : 0;
   }
+}
+
+// Frame totals of the log-likelihood, one wave per frame, from the group
+// sums (the block partials themselves when BPG = 1).
+template <int BPG>
+__global__ __launch_bounds__(64) void k_ll_final(DevPtrs P) {
+  FrameDev* F = P.fr + blockIdx.x;
+  if (!F->active) return;
+  ll_frame_total(F, BPG == 1 ? P.part_s : P.part_sg);
+}
+
+// Both steps in one block per frame, for batches with few groups (the
+// groups' trees a thread at a time, then the total): one launch fewer.
+constexpr int kLLFrameGroups = 2048;  // most groups of all frames for this form
+template <int BPG>
+__global__ __launch_bounds__(1024) void k_ll_frame(DevPtrs P) {
+  FrameDev* F = P.fr + blockIdx.x;
+  if (!F->active) return;
+  const long long g0 = F->s0 >> 8;
+  for (int g = threadIdx.x; g < F->nbl; g += 1024) {
+    double q[BPG];
+#pragma unroll
+    for (int i = 0; i < BPG; i++) q[i] = P.part_s[(g0 + g) * BPG + i];
+    P.part_sg[g0 + g] = pairwise_tree<BPG>(q);
+  }
+  __syncthreads();
+  if (threadIdx.x >= 64) return;
+  ll_frame_total(F, P.part_sg);
 }
 
 // ---------------------------------------------------------------- launchers --
@@ -2728,8 +2751,12 @@ static void launch_loglik(const DevPtrs& P, const KOpts& O, hipStream_t st,
   if (B == B_ && spg == UV_ && (NT_ == 128) == (gw == 64)) {                    \
     for (int r = 0; r < reps; r++) k_loglik<KM, B_, UV_, NT_><<<g, NT_, 0, st>>>(P, O); \
     if (ev_after) (void)hipEventRecord(ev_after, st);                           \
-    k_ll_groups<256 / B_><<<nb(nb(P.Sext, 256), 256), 256, 0, st>>>(P);        \
-    k_ll_final<256 / B_><<<P.nframes, 64, 0, st>>>(P);                          \
+    if (nb(P.Sext, 256) <= (unsigned)kLLFrameGroups) {                         \
+      k_ll_frame<256 / B_><<<P.nframes, 1024, 0, st>>>(P);                      \
+    } else {                                                                    \
+      k_ll_groups<256 / B_><<<nb(nb(P.Sext, 256), 256), 256, 0, st>>>(P);      \
+      k_ll_final<256 / B_><<<P.nframes, 64, 0, st>>>(P);                        \
+    }                                                                           \
     return;                                                                     \
   }
   TGMM_LL(32, 1, 256)
