@@ -111,8 +111,8 @@ int tgmm_setRegularizePrecisionMatrixConstants(double lambdaMin,
  * reference); indCUDAtr receives a copy of indCUDA. debugPath NULL or ""
  * disables the {int S; int K; float rnk[]; int ind[]} dump. The frame runs in
  * a one-frame workspace kept per device between calls (grown when a frame
- * does not fit, held until the process ends), its iteration replayed as a
- * captured hipGraph, convergence polled every four iterations. */
+ * does not fit, held until the process ends), its iterations replayed as
+ * captured hipGraphs, convergence polled every four iterations. */
 int tgmm_GMEMvariationalInferenceCUDAWithSupervoxels(
     float* queryCUDA, float* imgDataCUDA, float* rnkCUDA, float* rnkCUDAtr,
     int* indCUDA, int* indCUDAtr, float* centroidLabelPositionCUDA,
@@ -246,7 +246,7 @@ typedef struct tgmm_batch tgmm_batch;
 
 /* Solve modes */
 #define TGMM_SOLVE_EAGER 0     /* kernels launched per iteration, no host sync */
-#define TGMM_SOLVE_GRAPH 1     /* one iteration captured in a hipGraph, replayed */
+#define TGMM_SOLVE_GRAPH 1     /* iterations captured in hipGraphs, replayed */
 #define TGMM_SOLVE_HOSTSYNC 2  /* plain form: host reads ll/convergence each iteration */
 
 /* Uploads all frames into one device workspace (one allocation set), builds

@@ -509,21 +509,23 @@ static int any_active(tgmm_batch* b, bool* act) {
   return TGMM_OK;
 }
 
-int batch_capture(tgmm_batch* b) {
-  // a captured iteration holds its kernel arguments: capture again when they
-  // changed (a cached workspace reused for frames of another size)
-  if (b->gexec && (std::memcmp(&b->cap_P, &b->ws.P, sizeof(DevPtrs)) != 0 ||
-                   std::memcmp(&b->cap_ko, &b->ko, sizeof(KOpts)) != 0)) {
-    (void)hipGraphExecDestroy(b->gexec);
-    (void)hipGraphDestroy(b->graph);
-    b->gexec = nullptr;
-    b->graph = nullptr;
-  }
-  if (b->gexec) return TGMM_OK;
-  std::memcpy(&b->cap_P, &b->ws.P, sizeof(DevPtrs));  // padding included, as compared
-  std::memcpy(&b->cap_ko, &b->ko, sizeof(KOpts));
+static void destroy_graphs(tgmm_batch* b) {
+  if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  if (b->gexec_n) (void)hipGraphExecDestroy(b->gexec_n);
+  if (b->graph_n) (void)hipGraphDestroy(b->graph_n);
+  b->gexec = nullptr;
+  b->graph = nullptr;
+  b->gexec_n = nullptr;
+  b->graph_n = nullptr;
+  b->cap_n = 0;
+}
+
+// One capture of n iterations on the batch's stream.
+static int capture_iterations(tgmm_batch* b, int n, hipGraph_t* graph, hipGraphExec_t* exec) {
   HIPCHK(hipStreamBeginCapture(b->st, hipStreamCaptureModeThreadLocal));
-  hipError_t le = launch_iteration(b->ws.P, b->ko, b->st);
+  hipError_t le = hipSuccess;
+  for (int i = 0; i < n && le == hipSuccess; i++) le = launch_iteration(b->ws.P, b->ko, b->st);
   hipGraph_t g = nullptr;
   hipError_t ee = hipStreamEndCapture(b->st, &g);
   if (le != hipSuccess || ee != hipSuccess) {
@@ -531,8 +533,35 @@ int batch_capture(tgmm_batch* b) {
     tgmm_set_error("graph capture failed: %s / %s", hipGetErrorString(le), hipGetErrorString(ee));
     return TGMM_ERR_DEVICE;
   }
-  b->graph = g;
-  HIPCHK(hipGraphInstantiate(&b->gexec, g, nullptr, nullptr, 0));
+  *graph = g;
+  HIPCHK(hipGraphInstantiate(exec, g, nullptr, nullptr, 0));
+  return TGMM_OK;
+}
+
+// The solve's graphs: one iteration, and n iterations back to back (n >= 2),
+// so that a solve launches few graphs (each launch boundary idles the GPU
+// for several microseconds).
+int batch_capture(tgmm_batch* b, int n) {
+  // a captured iteration holds its kernel arguments: capture again when they
+  // changed (a cached workspace reused for frames of another size)
+  if (b->gexec && (std::memcmp(&b->cap_P, &b->ws.P, sizeof(DevPtrs)) != 0 ||
+                   std::memcmp(&b->cap_ko, &b->ko, sizeof(KOpts)) != 0 ||
+                   b->cap_n != (n >= 2 ? n : 0)))
+    destroy_graphs(b);
+  if (b->gexec) return TGMM_OK;
+  std::memcpy(&b->cap_P, &b->ws.P, sizeof(DevPtrs));  // padding included, as compared
+  std::memcpy(&b->cap_ko, &b->ko, sizeof(KOpts));
+  if (int rc = capture_iterations(b, 1, &b->graph, &b->gexec)) {
+    destroy_graphs(b);
+    return rc;
+  }
+  if (n >= 2) {
+    if (int rc = capture_iterations(b, n, &b->graph_n, &b->gexec_n)) {
+      destroy_graphs(b);
+      return rc;
+    }
+    b->cap_n = n;
+  }
   return TGMM_OK;
 }
 
@@ -541,17 +570,27 @@ extern "C" int tgmm_batch_solve(tgmm_batch* b, int mode) {
   HIPCHK(hipSetDevice(b->device));
   const int iters = b->opts.max_iter;
   const bool poll = b->opts.tol >= 0.0;
+  // with a tolerance, the host looks at the frames' states every kPollEvery
+  // iterations (frames that converged have stopped on the device already)
   if (mode == TGMM_SOLVE_GRAPH) {
-    if (int rc = batch_capture(b)) return rc;
+    if (int rc = batch_capture(b, poll ? kPollEvery : std::min(iters, kGraphIters))) return rc;
   }
-  for (int it = 0; it < iters; it++) {
+  int it = 0;  // iterations launched
+  while (it < iters) {
     if (mode == TGMM_SOLVE_GRAPH) {
-      HIPCHK(hipGraphLaunch(b->gexec, b->st));
+      if (b->gexec_n && iters - it >= b->cap_n) {
+        HIPCHK(hipGraphLaunch(b->gexec_n, b->st));
+        it += b->cap_n;
+      } else {
+        HIPCHK(hipGraphLaunch(b->gexec, b->st));
+        it++;
+      }
     } else {
       HIPCHK(launch_iteration(b->ws.P, b->ko, b->st));
+      it++;
     }
-    const bool check = (mode == TGMM_SOLVE_HOSTSYNC) || (poll && (it % 4 == 3));
-    if (check && it + 1 < iters) {
+    const bool check = (mode == TGMM_SOLVE_HOSTSYNC) || (poll && it % kPollEvery == 0);
+    if (check && it < iters) {
       bool act = true;
       if (int rc = any_active(b, &act)) return rc;
       if (!act) break;
@@ -719,13 +758,6 @@ static tgmm_tiles to_public(const Tiles& t) {
                     t.knn_block, t.knn_chunk, t.knn_cell, t.knn_lanes};
 }
 
-static void drop_graph(tgmm_batch* b) {
-  if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
-  if (b->graph) (void)hipGraphDestroy(b->graph);
-  b->gexec = nullptr;
-  b->graph = nullptr;
-}
-
 extern "C" int tgmm_batch_set_tiles(tgmm_batch* b, const tgmm_tiles* t) {
   if (!b || !t) return TGMM_ERR_ARG;
   const Tiles x = from_public(*t);
@@ -741,7 +773,7 @@ extern "C" int tgmm_batch_set_tiles(tgmm_batch* b, const tgmm_tiles* t) {
   HIPCHK(hipSetDevice(b->device));
   HIPCHK(hipStreamSynchronize(b->st));
   b->ko.tiles = x;
-  drop_graph(b);  // recaptured by the next graph solve
+  destroy_graphs(b);  // recaptured by the next graph solve
   if (x.knn_cell != b->grid_cell) {  // the kNN grids are sized by knn_cell
     if (int rc = regrid(b, b->ko)) return rc;
     return tgmm_batch_reset(b);
@@ -863,7 +895,7 @@ extern "C" int tgmm_batch_autotune(tgmm_batch* b, int reps, tgmm_tiles* chosen, 
   (void)hipEventDestroy(e1);
   if (rc != TGMM_OK) return rc;
   b->ko.tiles = best;
-  drop_graph(b);
+  destroy_graphs(b);
   if (best.knn_cell != b->grid_cell)
     if (int r2 = regrid(b, b->ko)) return r2;
   if (int r2 = tgmm_batch_reset(b)) return r2;
@@ -879,8 +911,7 @@ extern "C" int tgmm_batch_autotune(tgmm_batch* b, int reps, tgmm_tiles* chosen, 
 extern "C" int tgmm_batch_destroy(tgmm_batch* b) {
   if (!b) return TGMM_ERR_ARG;
   (void)hipSetDevice(b->device);
-  if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
-  if (b->graph) (void)hipGraphDestroy(b->graph);
+  destroy_graphs(b);
   if (b->st) (void)hipStreamSynchronize(b->st);
   ws_free(&b->ws);
   if (b->own_stream && b->st) (void)hipStreamDestroy(b->st);

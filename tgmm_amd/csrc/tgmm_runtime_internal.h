@@ -48,9 +48,12 @@ struct tgmm_batch {
   std::vector<tgmm::FrameDev> fh;  // host mirror of the frame table
   std::vector<FrameSize> sizes;
   Workspace ws;
-  hipGraph_t graph = nullptr;
+  hipGraph_t graph = nullptr;  // one iteration
   hipGraphExec_t gexec = nullptr;
-  tgmm::DevPtrs cap_P;  // launch arguments the graph was captured with
+  hipGraph_t graph_n = nullptr;  // cap_n iterations back to back (none when cap_n < 2)
+  hipGraphExec_t gexec_n = nullptr;
+  int cap_n = 0;
+  tgmm::DevPtrs cap_P;  // launch arguments the graphs were captured with
   tgmm::KOpts cap_ko;
   int grid_cell = 0;  // knn_cell the kNN grids were last built with
   long long max_ng = 0;      // most Gaussians of one frame
@@ -73,5 +76,9 @@ int batch_build(int device, const std::vector<FrameSrc>& fs, const tgmm_em_opts&
                 const float* scale_reg, hipStream_t stream, tgmm_batch** out,
                 tgmm_batch* reuse = nullptr, double headroom = 1.0);
 int batch_upload(tgmm_batch* b, const std::vector<FrameSrc>& fs);
-int batch_capture(tgmm_batch* b);
+// Iterations between the host's looks at the frames' states when the solve
+// has a tolerance, and the most iterations captured into one graph without one.
+constexpr int kPollEvery = 4;
+constexpr int kGraphIters = 32;
+int batch_capture(tgmm_batch* b, int n);
 
