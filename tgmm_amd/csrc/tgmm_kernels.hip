@@ -426,31 +426,8 @@ __global__ __launch_bounds__(kGB) void k_alpha_init(DevPtrs P) {
 
 // ---------------------------------------------------------------- scans --
 constexpr int kSortShort = 128;  // longer pair lists are sorted by a block each
-constexpr int kScanPer = 32;     // entries per thread and pass, k_scan_frames
+constexpr int kScanQ = 8;        // int4 loads per thread and pass, k_scan_frames
 constexpr int kSortLds = 4096;   // longest list sorted in LDS
-
-// Exclusive scan of 1024 per-thread values; returns this thread's prefix.
-__device__ inline int block_exclusive_scan1024(int v, int* lds) {
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  int x = v;
-#pragma unroll
-  for (int o = 1; o < 64; o <<= 1) {
-    const int y = __shfl_up(x, o, 64);
-    if (lane >= o) x += y;
-  }
-  if (lane == 63) lds[w] = x;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    int run = 0;
-    for (int i = 0; i < 16; i++) {
-      const int t = lds[i];
-      lds[16 + i] = run;
-      run += t;
-    }
-  }
-  __syncthreads();
-  return lds[16 + w] + x - v;
-}
 
 // Exclusive scan of each frame's counts, one 1024-thread block per frame:
 // out[off + i] = base + sum(in[off .. off + i)) for i = 0..n, with
@@ -462,11 +439,18 @@ __device__ inline int block_exclusive_scan1024(int v, int* lds) {
 //            positions of the frame's queries in cell order).
 // Both ranges hold n+1 entries (the grid has fewer cells than its range, the
 // Gaussian range at least one padding entry), so frames never share an entry.
-// Each thread takes a contiguous chunk (a multiple of 4 entries: int4 loads,
-// the ranges start on multiples of 4).
+// A pass covers kScanQ segments of 4096 entries; in segment q thread t loads
+// and stores entries 4t .. 4t+3 as one int4 (consecutive threads, consecutive
+// 16 bytes: a wave's load touches 8 cache lines, where a contiguous run per
+// thread touched 64), and the ranges start on multiples of 4. The prefix of
+// an int4 is the carry, the sums of the pass's earlier segments, and the sums
+// of the segment's earlier threads: one wave scan per segment, the
+// (segment, wave) totals scanned by wave 0.
 __global__ __launch_bounds__(1024) void k_scan_frames(DevPtrs P, int which, int need_active,
                                                       int K) {
-  __shared__ int lds[32];
+  constexpr int NW = 16;  // waves per block
+  __shared__ int wsum[kScanQ * NW];
+  __shared__ int total;
   // the grid scan opens an iteration: the kNN's tie list starts empty
   if (which == 0 && blockIdx.x == 0 && threadIdx.x == 0) {
     P.knn_ctl[0] = 0;
@@ -491,69 +475,86 @@ __global__ __launch_bounds__(1024) void k_scan_frames(DevPtrs P, int which, int 
     n = F->ng;
     base = (int)(K * F->s0);
   }
-  // 1024 * kScanPer entries per pass, each thread a contiguous run of
-  // kScanPer (its loads issued together): the runs' sums, one block scan,
-  // then each run's prefixes; entry n carries the total
-  const long long n1 = n + 1;
+  const long long n1 = n + 1;  // entry n carries the total
   // the pair scan lists the Gaussians whose pair list is long, for the
   // block-wide sort (k_seg_sort)
   const bool list_long = which == 1;
-  __shared__ int total;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   int carry = base;
-  for (long long c0 = 0; c0 < n1; c0 += 1024LL * kScanPer) {
-    const long long b = c0 + (long long)threadIdx.x * kScanPer;
-    int v[kScanPer];
-    if (b + kScanPer <= n) {
-      const int4* p4 = reinterpret_cast<const int4*>(in + b);
+  for (long long c0 = 0; c0 < n1; c0 += 4096LL * kScanQ) {
+    int4 v[kScanQ];
+    int x[kScanQ], sq[kScanQ];
 #pragma unroll
-      for (int q = 0; q < kScanPer / 4; q++) {
-        const int4 t = p4[q];
-        v[4 * q] = t.x;
-        v[4 * q + 1] = t.y;
-        v[4 * q + 2] = t.z;
-        v[4 * q + 3] = t.w;
+    for (int q = 0; q < kScanQ; q++) {
+      const long long e = c0 + 4096LL * q + 4LL * threadIdx.x;
+      if (e + 4 <= n) {
+        v[q] = *reinterpret_cast<const int4*>(in + e);
+      } else {
+        v[q].x = e < n ? in[e] : 0;
+        v[q].y = e + 1 < n ? in[e + 1] : 0;
+        v[q].z = e + 2 < n ? in[e + 2] : 0;
+        v[q].w = e + 3 < n ? in[e + 3] : 0;
       }
-    } else {
-#pragma unroll
-      for (int q = 0; q < kScanPer; q++) v[q] = (b + q < n) ? in[b + q] : 0;
     }
-    int tsum = 0;
-    bool any_long = false;
 #pragma unroll
-    for (int q = 0; q < kScanPer; q++) {
-      tsum += v[q];
-      any_long |= v[q] > kSortShort;
-    }
-    if (list_long && any_long) {
+    for (int q = 0; q < kScanQ; q++) {
+      sq[q] = v[q].x + v[q].y + v[q].z + v[q].w;
+      x[q] = sq[q];
+      if (list_long && (v[q].x > kSortShort || v[q].y > kSortShort || v[q].z > kSortShort ||
+                        v[q].w > kSortShort)) {
+        const long long e = c0 + 4096LL * q + 4LL * threadIdx.x;
+        const int c[4] = {v[q].x, v[q].y, v[q].z, v[q].w};
 #pragma unroll
-      for (int q = 0; q < kScanPer; q++)
-        if (v[q] > kSortShort) P.long_list[atomicAdd(P.knn_ctl + 2, 1)] = (int)(F->g0 + b + q);
-    }
-    int run = carry + block_exclusive_scan1024(tsum, lds);
-    if (threadIdx.x == 1023) total = run + tsum;
-    if (b + kScanPer <= n1) {
-      int4* o4 = reinterpret_cast<int4*>(out + b);
-#pragma unroll
-      for (int q = 0; q < kScanPer / 4; q++) {
-        int4 r;
-        r.x = run;
-        r.y = r.x + v[4 * q];
-        r.z = r.y + v[4 * q + 1];
-        r.w = r.z + v[4 * q + 2];
-        run = r.w + v[4 * q + 3];
-        o4[q] = r;
+        for (int i = 0; i < 4; i++)
+          if (c[i] > kSortShort) P.long_list[atomicAdd(P.knn_ctl + 2, 1)] = (int)(F->g0 + e + i);
       }
-    } else {
+    }
+    // inclusive wave scans, the segments side by side
 #pragma unroll
-      for (int q = 0; q < kScanPer; q++)
-        if (b + q < n1) {
-          out[b + q] = run;
-          run += v[q];
-        }
+    for (int o = 1; o < 64; o <<= 1) {
+#pragma unroll
+      for (int q = 0; q < kScanQ; q++) {
+        const int y = __shfl_up(x[q], o, 64);
+        if (lane >= o) x[q] += y;
+      }
+    }
+    if (lane == 63)
+#pragma unroll
+      for (int q = 0; q < kScanQ; q++) wsum[q * NW + w] = x[q];
+    __syncthreads();
+    if (w == 0) {
+      // the kScanQ * NW totals in (segment, wave) order, two per lane
+      const int a0 = wsum[2 * lane], a1 = wsum[2 * lane + 1];
+      int y = a0 + a1;
+#pragma unroll
+      for (int o = 1; o < 64; o <<= 1) {
+        const int t = __shfl_up(y, o, 64);
+        if (lane >= o) y += t;
+      }
+      const int ex = y - a0 - a1;
+      wsum[2 * lane] = ex;
+      wsum[2 * lane + 1] = ex + a0;
+      if (lane == 63) total = y;
     }
     __syncthreads();
-    carry = total;
-    __syncthreads();  // total and lds are rewritten by the next pass
+#pragma unroll
+    for (int q = 0; q < kScanQ; q++) {
+      const long long e = c0 + 4096LL * q + 4LL * threadIdx.x;
+      int4 r;
+      r.x = carry + wsum[q * NW + w] + x[q] - sq[q];
+      r.y = r.x + v[q].x;
+      r.z = r.y + v[q].y;
+      r.w = r.z + v[q].z;
+      if (e + 4 <= n1) {
+        *reinterpret_cast<int4*>(out + e) = r;
+      } else {
+        if (e < n1) out[e] = r.x;
+        if (e + 1 < n1) out[e + 1] = r.y;
+        if (e + 2 < n1) out[e + 2] = r.z;
+      }
+    }
+    carry += total;
+    __syncthreads();  // wsum and total are rewritten by the next pass
   }
 }
 
