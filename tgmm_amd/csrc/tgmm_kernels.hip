@@ -426,7 +426,7 @@ __global__ __launch_bounds__(kGB) void k_alpha_init(DevPtrs P) {
 
 // ---------------------------------------------------------------- scans --
 constexpr int kSortShort = 128;  // longer pair lists are sorted by a block each
-constexpr int kScanQ = 8;        // int4 loads per thread and pass, k_scan_frames
+constexpr int kScanQ = 8;        // int4 loads per thread and pass, k_scan_frames (4 | kScanQ)
 constexpr int kSortLds = 4096;   // longest list sorted in LDS
 
 // Exclusive scan of each frame's counts, one 1024-thread block per frame:
@@ -449,6 +449,7 @@ constexpr int kSortLds = 4096;   // longest list sorted in LDS
 __global__ __launch_bounds__(1024) void k_scan_frames(DevPtrs P, int which, int need_active,
                                                       int K) {
   constexpr int NW = 16;  // waves per block
+  static_assert(kScanQ % 4 == 0, "whole values per lane in the totals scan");
   __shared__ int wsum[kScanQ * NW];
   __shared__ int total;
   // the grid scan opens an iteration: the kNN's tie list starts empty
@@ -523,17 +524,26 @@ __global__ __launch_bounds__(1024) void k_scan_frames(DevPtrs P, int which, int 
       for (int q = 0; q < kScanQ; q++) wsum[q * NW + w] = x[q];
     __syncthreads();
     if (w == 0) {
-      // the kScanQ * NW totals in (segment, wave) order, two per lane
-      const int a0 = wsum[2 * lane], a1 = wsum[2 * lane + 1];
-      int y = a0 + a1;
+      // the kScanQ * NW totals in (segment, wave) order, PL per lane
+      constexpr int PL = kScanQ * NW / 64;
+      int a[PL], y = 0;
+#pragma unroll
+      for (int i = 0; i < PL; i++) {
+        a[i] = wsum[PL * lane + i];
+        y += a[i];
+      }
+      const int own = y;
 #pragma unroll
       for (int o = 1; o < 64; o <<= 1) {
         const int t = __shfl_up(y, o, 64);
         if (lane >= o) y += t;
       }
-      const int ex = y - a0 - a1;
-      wsum[2 * lane] = ex;
-      wsum[2 * lane + 1] = ex + a0;
+      int ex = y - own;
+#pragma unroll
+      for (int i = 0; i < PL; i++) {
+        wsum[PL * lane + i] = ex;
+        ex += a[i];
+      }
       if (lane == 63) total = y;
     }
     __syncthreads();
