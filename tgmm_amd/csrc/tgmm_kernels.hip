@@ -1706,17 +1706,6 @@ __device__ inline void wave_count(int* cnt, int key) {
   if (key >= 0 && r.head == __lane_id()) atomicAdd(cnt + key, r.len);
 }
 
-// Takes one slot per lane with key >= 0 from the counter cnt[key] (counting
-// down); returns the slot (distinct per lane), or -1.
-__device__ inline int wave_take(int* cnt, int key) {
-  const WaveRun r = wave_run(key);
-  const int lane = __lane_id();
-  int old = 0;
-  if (key >= 0 && r.head == lane) old = atomicSub(cnt + key, r.len);
-  old = __shfl(old, r.head, 64);
-  return key >= 0 ? old - 1 - (lane - r.head) : -1;
-}
-
 // The E-step of supervoxel s (ok: a real supervoxel of an active frame):
 // its responsibilities from the candidates' log scores, and the pair counts
 // of its significant candidates. Every lane of the wave calls it together
@@ -1809,17 +1798,47 @@ __global__ __launch_bounds__(256) void k_estep(DevPtrs P, KOpts O) {
   estep_one<KM>(P, O, s, ok);
 }
 
+// Each significant pair's code at its Gaussian's list, in a slot taken from
+// the count (the lists are ordered by k_seg_sort). The candidates' indices
+// and responsibilities depend on s alone (s < Scap) and are loaded before the
+// frame checks; the slot reservations of all candidates are issued back to
+// back and their results used after, so that a wave waits for one round of
+// atomics, not K. Branch-free over the candidates (slots j >= K read row 0
+// and hold no pair), so that no wait is forced at branch joins.
+template <int KM>
 __global__ __launch_bounds__(256) void k_pair_scatter(DevPtrs P, KOpts O) {
   const long long s = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long S = P.Scap;
+  const int K = O.K;
+  const int lane = __lane_id();
+  int kr[KM];
+  float rr[KM];
+#pragma unroll
+  for (int j = 0; j < KM; j++) {
+    const long long row = j < K ? j : 0;
+    kr[j] = P.ind[row * S + s];
+    rr[j] = P.rnk[row * S + s];
+  }
   const int f = P.sblk_frame[s >> 8];
   // no early exit: every lane takes part in the wave-wide slot reservation
   const bool ok = f >= 0 && P.fr[f].active && P.sv_frame[s] == f;
-  const long long S = P.Scap;
-  for (int j = 0; j < O.K; j++) {
-    const int k = ok ? P.ind[(long long)j * S + s] : -1;
-    const bool take = k >= 0 && pair_significant(P.rnk[(long long)j * S + s]);
-    const int slot = wave_take(P.seg_cnt, take ? k : -1);
-    if (take) P.pairs[P.seg_start[k] + slot] = (int)((s << kPairShift) | j);
+  int key[KM], head[KM], old[KM], st[KM];
+#pragma unroll
+  for (int j = 0; j < KM; j++) {
+    key[j] = (j < K && ok && kr[j] >= 0 && pair_significant(rr[j])) ? kr[j] : -1;
+    const WaveRun r = wave_run(key[j]);
+    head[j] = r.head;
+    int o = 0;
+    if (key[j] >= 0 && r.head == lane) o = atomicSub(P.seg_cnt + key[j], r.len);
+    old[j] = o;
+    st[j] = P.seg_start[key[j] >= 0 ? key[j] : 0];
+  }
+#pragma unroll
+  for (int j = 0; j < KM; j++) {
+    // a run of lanes with one key takes the slots below the value its head
+    // took, counting down in lane order
+    const int o = __shfl(old[j], head[j], 64);
+    if (key[j] >= 0) P.pairs[st[j] + o - 1 - (lane - head[j])] = (int)((s << kPairShift) | j);
   }
 }
 
@@ -1842,10 +1861,14 @@ constexpr int kSortLongBlocks = 64;
 __device__ inline void seg_sort_short(const DevPtrs& P, int* sh) {
   const int grp = threadIdx.x >> 4, lane = threadIdx.x & 15;
   const long long k = (long long)blockIdx.x * 16 + grp;
+  // the list's bounds and the Gaussian's frame are in range for every k <
+  // Gext; they are loaded before the frame checks, used only after them
+  const int gf = P.g_frame[k];
+  const int st = P.seg_start[k], en = P.seg_start[k + 1];
   const int f = P.gblk_frame[k / kGB];
   if (f < 0 || !P.fr[f].active) return;
-  if (P.g_frame[k] != f) return;
-  const int st = P.seg_start[k], L = P.seg_start[k + 1] - st;
+  if (gf != f) return;
+  const int L = en - st;
   if (L <= 1 || L > kSortShort) {
     if (L == 1 && lane == 0) P.pairs_sorted[st] = P.pairs[st];
     return;
@@ -2872,7 +2895,7 @@ static hipError_t iteration_t(const DevPtrs& P, const KOpts& O, hipStream_t st, 
   k_estep<KM><<<nb(P.Sext, bs), bs, 0, st>>>(P, O);
   if (tk == TK_ESTEP && e1) (void)hipEventRecord(e1, st);
   k_scan_frames<<<P.nframes, 1024, 0, st>>>(P, 1, 1, O.K);
-  k_pair_scatter<<<nb(P.Sext, bs), bs, 0, st>>>(P, O);
+  k_pair_scatter<KM><<<nb(P.Sext, bs), bs, 0, st>>>(P, O);
   k_seg_sort<<<nb(P.Gext, 16) + kSortLongBlocks, 256, 0, st>>>(P, (int)nb(P.Gext, 16));
   if (tk == TK_MSTEP && e0) (void)hipEventRecord(e0, st);
   launch_mstep(P, O, st);
