@@ -2017,6 +2017,31 @@ __device__ __noinline__ double mstep_update_one(double* __restrict__ gs,
 #undef GSV
 }
 
+// One pair's terms added to the M-step sums c (N, S1[3], S2[6]) about the
+// mean (r0, r1, r2): r its responsibility, m01..m89 its supervoxel's weighted
+// moments (W_s first). The fused multiply-adds are written out and nothing
+// else is contracted, so that every launch shape rounds the sums alike; an
+// entry with use false adds exact zeros (w = r = 0, finite moments).
+__device__ inline void mstep_add(double* c, bool use, double rr, double2 m01, double2 m23,
+                                 double2 m45, double2 m67, double2 m89, double r0, double r1,
+                                 double r2) {
+#pragma clang fp contract(off)
+  const double r = use ? rr : 0.0;
+  const double w = use ? rr * m01.x : 0.0;
+  const double d0 = m01.y - r0, d1 = m23.x - r1, d2 = m23.y - r2;
+  const double wd0 = w * d0, wd1 = w * d1, wd2 = w * d2;
+  c[0] = c[0] + w;
+  c[1] = c[1] + wd0;
+  c[2] = c[2] + wd1;
+  c[3] = c[3] + wd2;
+  c[4] = c[4] + fma(r, m45.x, wd0 * d0);
+  c[5] = c[5] + fma(r, m45.y, wd0 * d1);
+  c[6] = c[6] + fma(r, m67.x, wd0 * d2);
+  c[7] = c[7] + fma(r, m67.y, wd1 * d1);
+  c[8] = c[8] + fma(r, m89.x, wd1 * d2);
+  c[9] = c[9] + fma(r, m89.y, wd2 * d2);
+}
+
 // M-step: segmented FP64 sufficient statistics over each Gaussian's
 // significant pairs, accumulated about the current mean m_k,
 //   N   = sum r W_s
@@ -2056,59 +2081,55 @@ __global__ __launch_bounds__(64 * ML) void k_mstep(DevPtrs P, KOpts O) {
       const double r0 = gsv(P, GS_M, k), r1 = gsv(P, GS_M + 1, k), r2 = gsv(P, GS_M + 2, k);
       const int st = P.seg_start[k], en = P.seg_start[k + 1];
       for (int e0 = st; e0 < en; e0 += 16) {
-        // the entries QB at a time (two where the accumulators leave room):
-        // the entries' loads first (an entry past the list reads the list's
-        // last pair, in range since e0 < en), then their sums, where an entry
-        // past the list or of zero weight adds exact zeros (its w and r are 0
-        // and its moments finite): no branch, so no load waits for the
-        // previous entry's
-        constexpr int QB = NC == 2 ? 2 : 1;
+        if constexpr (NC == 4) {
+          // four entries per lane: one at a time, an entry past the list or
+          // of zero weight skipped (its loads are not issued)
 #pragma unroll
-        for (int q0 = 0; q0 < NC; q0 += QB) {
-          int code[QB];
-#pragma unroll
-          for (int u = 0; u < QB; u++) {
-            const int e = e0 + lane + ML * (q0 + u);
-            code[u] = P.pairs_sorted[e < en ? e : en - 1];
+          for (int q = 0; q < NC; q++) {
+            const int e = e0 + lane + ML * q;
+            if (e >= en) continue;
+            const int code = P.pairs_sorted[e];
+            const long long s = code >> kPairShift;
+            const int j = code & ((1 << kPairShift) - 1);
+            const double r = (double)P.rnk[(long long)j * S + s];
+            const double2* wm = reinterpret_cast<const double2*>(P.wmom + s * 10);
+            const double2 m01 = wm[0], m23 = wm[1], m45 = wm[2], m67 = wm[3], m89 = wm[4];
+            if (!(m01.x != 0.0)) continue;
+            mstep_add(a[q], true, r, m01, m23, m45, m67, m89, r0, r1, r2);
           }
-          float rq[QB];
-          double2 m01[QB], m23[QB], m45[QB], m67[QB], m89[QB];
+        } else {
+          // one or two entries per lane: the entries' loads first (an entry
+          // past the list reads the list's last pair, in range since
+          // e0 < en), then their sums, where an entry past the list or of
+          // zero weight adds exact zeros (its w and r are 0 and its moments
+          // finite): no branch, so no load waits for the previous entry's
+          int code[NC];
 #pragma unroll
-          for (int u = 0; u < QB; u++) {
-            const long long s = code[u] >> kPairShift;
-            const int j = code[u] & ((1 << kPairShift) - 1);
-            rq[u] = P.rnk[(long long)j * S + s];
+          for (int q = 0; q < NC; q++) {
+            const int e = e0 + lane + ML * q;
+            code[q] = P.pairs_sorted[e < en ? e : en - 1];
+          }
+          float rq[NC];
+          double2 m01[NC], m23[NC], m45[NC], m67[NC], m89[NC];
+#pragma unroll
+          for (int q = 0; q < NC; q++) {
+            const long long s = code[q] >> kPairShift;
+            const int j = code[q] & ((1 << kPairShift) - 1);
+            rq[q] = P.rnk[(long long)j * S + s];
             // the supervoxel's 10 weighted moments, contiguous (80 bytes)
             const double2* wm = reinterpret_cast<const double2*>(P.wmom + s * 10);
-            m01[u] = wm[0];
-            m23[u] = wm[1];
-            m45[u] = wm[2];
-            m67[u] = wm[3];
-            m89[u] = wm[4];
+            m01[q] = wm[0];
+            m23[q] = wm[1];
+            m45[q] = wm[2];
+            m67[q] = wm[3];
+            m89[q] = wm[4];
           }
 #pragma unroll
-          for (int u = 0; u < QB; u++) {
-            // fused multiply-adds written out and no other contraction, so
-            // that every launch shape (ML) rounds the sums identically
-#pragma clang fp contract(off)
-            const int e = e0 + lane + ML * (q0 + u);
-            const double Ws = m01[u].x;
-            const bool use = e < en && Ws != 0.0;
-            const double r = use ? (double)rq[u] : 0.0;
-            const double w = use ? r * Ws : 0.0;
-            const double d0 = m01[u].y - r0, d1 = m23[u].x - r1, d2 = m23[u].y - r2;
-            const double wd0 = w * d0, wd1 = w * d1, wd2 = w * d2;
-            double* c = a[q0 + u];
-            c[0] = c[0] + w;
-            c[1] = c[1] + wd0;
-            c[2] = c[2] + wd1;
-            c[3] = c[3] + wd2;
-            c[4] = c[4] + fma(r, m45[u].x, wd0 * d0);
-            c[5] = c[5] + fma(r, m45[u].y, wd0 * d1);
-            c[6] = c[6] + fma(r, m67[u].x, wd0 * d2);
-            c[7] = c[7] + fma(r, m67[u].y, wd1 * d1);
-            c[8] = c[8] + fma(r, m89[u].x, wd1 * d2);
-            c[9] = c[9] + fma(r, m89[u].y, wd2 * d2);
+          for (int q = 0; q < NC; q++) {
+            const int e = e0 + lane + ML * q;
+            const bool use = e < en && m01[q].x != 0.0;
+            mstep_add(a[q], use, (double)rq[q], m01[q], m23[q], m45[q], m67[q], m89[q], r0, r1,
+                      r2);
           }
         }
       }
