@@ -842,17 +842,28 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(KM <= 8 ? 4
   // mean's distance is a real candidate distance; dead means are NaN).
   float kbound = INFINITY;
   {
+    // the K indices, then the K means, each one round of loads (a slot past
+    // K reads row 0, an index outside the frame reads mean 0; neither is
+    // used), and the bound formed branch-free, so that no load waits for
+    // the previous candidate's
+    int kk[KM];
+#pragma unroll
+    for (int j = 0; j < KM; j++) kk[j] = P.ind[(long long)(j < K ? j : 0) * S + s];
+    float4 mm[KM];
+#pragma unroll
+    for (int j = 0; j < KM; j++) {
+      const bool in = kk[j] >= F->g0 && kk[j] < F->g0 + F->ng;
+      mm[j] = P.m4[in ? kk[j] : 0];
+    }
     float mx = -INFINITY;
     bool all = true;
 #pragma unroll
     for (int j = 0; j < KM; j++) {
-      if (j >= K) continue;
-      const int k = P.ind[(long long)j * S + s];
-      const bool in = k >= F->g0 && k < F->g0 + F->ng;
-      const float4 m = in ? P.m4[k] : make_float4(NAN, NAN, NAN, 0.f);
+      const bool in = kk[j] >= F->g0 && kk[j] < F->g0 + F->ng;
+      const float4 m = in ? mm[j] : make_float4(NAN, NAN, NAN, 0.f);
       const float d = knn_dist(q0, q1, q2, m.x, m.y, m.z, s0, s1, s2);
-      all &= (d <= 1e32f);
-      mx = fmaxf(mx, d);
+      all &= (j >= K) | (d <= 1e32f);
+      mx = j < K ? fmaxf(mx, d) : mx;
     }
     if (all) kbound = mx;
   }
