@@ -1592,10 +1592,15 @@ __global__ __launch_bounds__(64) void k_knn_layout(DevPtrs P, KOpts O, int only_
     // the search's candidates into the reference's slots
     int oi[KM];
     float orr[KM];
+    // one round of loads (a slot past K reads row 0, not used): no load
+    // waits for the previous slot's
 #pragma unroll
     for (int j = 0; j < KM; j++) {
-      oi[j] = j < K ? P.ind[(long long)j * S + s] : -1;
-      orr[j] = j < K ? P.rnk[(long long)j * S + s] : 0.0f;
+      const long long row = j < K ? j : 0;
+      const int ij = P.ind[row * S + s];
+      const float rj = P.rnk[row * S + s];
+      oi[j] = j < K ? ij : -1;
+      orr[j] = j < K ? rj : 0.0f;
     }
     int n_old = 0, n_new = 0;
 #pragma unroll
